@@ -1,0 +1,201 @@
+/* partia_hip.h -- C ABI of the MI355X (gfx950) SPH interpolation library.
+ *
+ * This is the drop-in boundary for Partia.jl's interpolation hot path.  A host
+ * binding (Julia `ccall`, Python ctypes) calls these functions in place of the
+ * reference's CPU/CUDA methods; every entry point below names the reference
+ * interface it replaces (paths relative to the reference root; KI/ =
+ * src/julia/KernelInterpolation/, LBVH/ = src/julia/LinearBoundingVolumeHierarchy/).
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  `const double*` arguments named *_host are
+ *    host memory read during the call; `*_dev` pointers are device memory of the
+ *    context's GPU (allocate with ph_device_alloc or any HIP allocator).
+ *  - Every function returns an int status: PH_OK (0) or a negative PH_ERR_* code;
+ *    ph_last_error(ctx) returns the message of the last failure on that context.
+ *  - One context = one GPU + one HIP stream.  Calls on one context are
+ *    serialised and not thread-safe; separate contexts are independent.
+ *  - Tree arrays use the reference's unified node IDs (int32): internal nodes
+ *    1..n-1, leaf k (1-based) <-> n-1+k, 0 = null, root = 1 when n >= 2
+ *    (LBVH/BinaryRadixTree/BinaryRadixTree.jl:37-44).
+ */
+#ifndef PARTIA_HIP_H
+#define PARTIA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the host binding maps them to the reference's exceptions) ---- */
+#define PH_OK 0
+#define PH_ERR_ARGUMENT (-1)      /* ArgumentError                                   */
+#define PH_ERR_DIMENSION (-2)     /* DimensionMismatch (InterpolationInput.jl:85-96)  */
+#define PH_ERR_STRATEGY (-3)      /* ArgumentError: unsupported strategy (LineSamples_driver.jl:40-44) */
+#define PH_ERR_LEAF_ORDER (-4)    /* ArgumentError: leaf order mismatch (PointSamples_driver.jl:100-103) */
+#define PH_ERR_CAPACITY (-5)      /* ArgumentError: empty tree / > 2^30 leaves (BinaryRadixTree.jl:86-91) */
+#define PH_ERR_OUT_OF_MEMORY (-6) /* device allocation failed                        */
+#define PH_ERR_HIP (-7)           /* HIP runtime error                               */
+#define PH_ERR_INTERNAL (-8)      /* AssertionError: refit visit count (LinearBVH.jl:89-91) */
+#define PH_ERR_NO_DEVICE (-9)     /* no usable GPU                                   */
+
+/* ---- enumerations mirroring the reference's type tags ---- */
+/* kernel_function/kernels/<name>.jl */
+#define PH_KERNEL_M4 0
+#define PH_KERNEL_M5 1
+#define PH_KERNEL_M6 2
+#define PH_KERNEL_C2 3
+#define PH_KERNEL_C4 4
+#define PH_KERNEL_C6 5
+/* interpolation_setup/InterpolationStrategy.jl:1-5 */
+#define PH_GATHER 0
+#define PH_SCATTER 1
+#define PH_SYMMETRIC 2
+/* working precision of an interpolation call */
+#define PH_PREC_FP64 64  /* Float64 semantics of the reference                 */
+#define PH_PREC_FP32 32  /* Float32 semantics of the reference (Float32 input)  */
+#define PH_PREC_FP32C 33 /* Float64 input, FP32-compensated evaluation          */
+
+/* ph_lbvh_get selectors */
+#define PH_GET_ORDER 0      /* uint64[n]: original 1-based index (MortonEncoding.order)   */
+#define PH_GET_CODES 1      /* uint64[n]: Morton codes after the duplicate rewrite        */
+#define PH_GET_LEFT 2       /* int32[2n-1]                                                */
+#define PH_GET_RIGHT 3      /* int32[2n-1]                                                */
+#define PH_GET_ESCAPE 4     /* int32[2n-1]                                                */
+#define PH_GET_PARENT 5     /* int32[2n-1]                                                */
+#define PH_GET_LEAF_COOR 6  /* double[3][n]                                               */
+#define PH_GET_LEAF_SCALE 7 /* double[n]                                                  */
+#define PH_GET_NODE_MIN 8   /* double[3][n-1]                                             */
+#define PH_GET_NODE_MAX 9   /* double[3][n-1]                                             */
+#define PH_GET_NODE_SCALE 10 /* double[n-1]                                               */
+#define PH_GET_COLUMNS 11   /* double[6+ncol][n]: x,y,z,m,h,rho,quant... in Morton order  */
+
+typedef struct ph_context ph_context;
+typedef struct ph_lbvh ph_lbvh;
+
+/* Concise catalog: mirrors InterpolationCatalogConcise (KI/interpolation_setup/
+ * InterpolationCatalog.jl:26-32).  Slots are the reference's 1-based column
+ * indices; a gradient slot of 0 selects the density gradient.  Output order is
+ * scalars, gradients (x,y,z each), divergences, curls (x,y,z each) -- L values. */
+typedef struct ph_catalog {
+    int32_t n_scalar;
+    const int32_t* scalar_slots;
+    const uint8_t* scalar_shepard; /* 1 = Shepard-normalise (all names except :rho / :ρ) */
+    int32_t n_grad;
+    const int32_t* grad_slots;
+    int32_t n_div;
+    const int32_t* div_slots; /* 3 per divergence */
+    int32_t n_curl;
+    const int32_t* curl_slots; /* 3 per curl */
+} ph_catalog;
+
+/* Per-call statistics and device-side timings (hipEvent on the context stream). */
+typedef struct ph_stats {
+    double build_ms;        /* last LBVH build                                       */
+    double interp_ms;       /* last interpolation call, whole device-side work        */
+    double kernel_ms;       /* last call's dominant (traversal + accumulation) kernel */
+    int64_t samples;        /* samples (points or rays) of the last call             */
+    int64_t accepted;       /* accepted neighbours of the last call (if counted)      */
+    int64_t rechecks;       /* FP64 guard-band rechecks of the last FP32C call        */
+} ph_stats;
+
+/* ---- context ---- */
+int ph_ctx_create(int device, ph_context** out);
+int ph_ctx_destroy(ph_context* ctx);
+const char* ph_last_error(const ph_context* ctx);
+int ph_get_stats(const ph_context* ctx, ph_stats* out);
+/* Count accepted neighbours during interpolation calls (adds one atomic per sample). */
+int ph_set_count_neighbours(ph_context* ctx, int enable);
+
+/* ---- device memory helpers (for callers without their own HIP allocator) ---- */
+int ph_device_alloc(ph_context* ctx, int64_t bytes, void** dptr);
+int ph_device_free(ph_context* ctx, void* dptr);
+int ph_memcpy_htod(ph_context* ctx, void* dst_dev, const void* src_host, int64_t bytes);
+int ph_memcpy_dtoh(ph_context* ctx, void* dst_host, const void* src_dev, int64_t bytes);
+int ph_synchronize(ph_context* ctx);
+
+/* ---- LBVH build: replaces LinearBVH!(input; CodeType) (KI/interpolation_setup/
+ * InterpolationInput.jl:201-221), i.e. MortonEncoding (LBVH/MortonEncoding/
+ * MortonEncoding.jl:36-43), sort_by_morton! (toolbox.jl:20-33), BinaryRadixTree
+ * (BinaryRadixTree.jl:82-133) and LinearBVH with BoxScale(h, true)
+ * (LinearBVH.jl:56-94), all on the GPU.  code_bits is 63 (CodeType=UInt64, the
+ * reference default) or 30 (CodeType=UInt32).  precision PH_PREC_FP64 keeps the
+ * columns as float64; PH_PREC_FP32 converts them to float32 first and runs the
+ * whole build in float32 (a Float32 InterpolationInput).  The particle columns
+ * are permuted into Morton order on the device; read the permutation back with
+ * PH_GET_ORDER to permute host arrays as LinearBVH! does. */
+int ph_lbvh_build(ph_context* ctx, int64_t n, const double* x_host, const double* y_host, const double* z_host,
+                  const double* m_host, const double* h_host, const double* rho_host, int32_t ncol,
+                  const double* const* cols_host, int32_t kernel, int32_t code_bits, int32_t precision,
+                  ph_lbvh** out);
+/* Same, with the columns already resident on the device (float64 arrays). */
+int ph_lbvh_build_device(ph_context* ctx, int64_t n, const double* x_dev, const double* y_dev, const double* z_dev,
+                         const double* m_dev, const double* h_dev, const double* rho_dev, int32_t ncol,
+                         const double* const* cols_dev, int32_t kernel, int32_t code_bits, int32_t precision,
+                         ph_lbvh** out);
+int ph_lbvh_destroy(ph_lbvh* bvh);
+/* Tree/encoding accessors for the host mirror of LinearBVH (LBVH/LinearBVH/LinearBVH.jl:8-14). */
+int ph_lbvh_info(const ph_lbvh* bvh, int64_t* n, int32_t* root, int32_t* ncol, int32_t* code_bits,
+                 int32_t* precision);
+int ph_lbvh_get(ph_lbvh* bvh, int32_t which, void* dst_host);
+
+/* ---- PointSamples_interpolation: replaces the per-sample loop of
+ * KI/grid_interpolation/drivers/PointSamples_driver.jl:34-48,98-118 and its CUDA
+ * counterpart ext/CUDAExt/KernelInterpolation/grid_interpolation/drivers/
+ * PointSamples_driver.jl:12-35,44-66: nearest-h (LBVH_find_nearest_h) then
+ * _general_quantity_interpolate_kernel for every sample.  out_host is L*S
+ * doubles, output k of sample i at out_host[k*S + i]. */
+int ph_point_samples(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                     int64_t S, const double* px_host, const double* py_host, const double* pz_host,
+                     double* out_host);
+/* Device-resident variant: sample coordinates (float64) and output on the device.
+ * Output element type is double for PH_PREC_FP64 and float for FP32 / FP32C. */
+int ph_point_samples_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy,
+                            int32_t precision, int64_t S, const double* px_dev, const double* py_dev,
+                            const double* pz_dev, void* out_dev);
+
+/* ---- StructuredGrid_interpolation (KI/grid_interpolation/drivers/
+ * StructuredGrid_driver.jl:46-64) for Cartesian grids: the flattened sample
+ * coordinates (Grids/grids/StructuredGrid.jl:140-155, column-major, x fastest)
+ * are generated on the device from the three axes, so no host coordinate array
+ * exists.  Output layout as for ph_point_samples with S = nx*ny*nz. */
+int ph_structured_grid(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                       const double* ax_host, int64_t nx, const double* ay_host, int64_t ny, const double* az_host,
+                       int64_t nz, double* out_host);
+int ph_structured_grid_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy,
+                              int32_t precision, const double* ax_dev, int64_t nx, const double* ay_dev, int64_t ny,
+                              const double* az_dev, int64_t nz, void* out_dev);
+
+/* ---- LineSamples_interpolation: replaces KI/grid_interpolation/drivers/
+ * LineSamples_driver.jl:39-57,116-139 (and the CUDA twin LineSamples_driver.jl:
+ * 11-34,49-77): line-integrated scalars along infinite lines (origin, unit
+ * direction).  Only PH_SCATTER is accepted (PH_ERR_STRATEGY otherwise), and the
+ * catalog may hold scalars only. */
+int ph_line_samples(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                    int64_t S, const double* ox_host, const double* oy_host, const double* oz_host,
+                    const double* dx_host, const double* dy_host, const double* dz_host, double* out_host);
+int ph_line_samples_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                           int64_t S, const double* ox_dev, const double* oy_dev, const double* oz_dev,
+                           const double* dx_dev, const double* dy_dev, const double* dz_dev, void* out_dev);
+
+/* Parallel-ray projection on a regular image plane (the column-density map of
+ * a line-integrated StructuredGrid): ray (i, j) has origin o + u*ax[i] + v*ay[j]
+ * and direction dir; the rays are generated on the device.  Output k of ray
+ * (i, j) at out[k*nx*ny + i + nx*j]. */
+int ph_line_grid_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                        const double* origin3, const double* u3, const double* v3, const double* dir3,
+                        const double* ax_dev, int64_t nx, const double* ay_dev, int64_t ny, void* out_dev);
+
+/* ---- building blocks exposed for parity tests ---- */
+/* LBVH_find_nearest_h (LBVH/LinearBVH/LinearBVHQueries.jl:118-140) per point. */
+int ph_nearest_h(ph_context* ctx, ph_lbvh* bvh, int64_t S, const double* px_host, const double* py_host,
+                 const double* pz_host, double* ha_host);
+/* Line-integrated kernel table (KI/kernel_function/line_integrated_kernel.jl:13-19,
+ * tables of KI/table/line_integrated_kernel_tables.jl): copies the library's
+ * 4096-entry float64 Inorm table of a kernel. */
+int ph_line_table(int32_t kernel, double* inorm_host /* 4096 */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PARTIA_HIP_H */

@@ -1,0 +1,840 @@
+// partia_oracle.hpp -- CPU restatement of Partia.jl's SPH interpolation hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: tests/, the smoke
+// check in __graft_entry__.py and the cpu_baseline leg of bench.py are the only
+// code allowed to build, load or call it, and only as the checker.  The product
+// (the HIP library under partia.jl_amd/) never links or calls it.
+//
+// Every function restates the reference rule by rule; the citation above each
+// names the file:line it follows (paths relative to the reference root,
+// src/julia/ abbreviated as LBVH/ = LinearBoundingVolumeHierarchy/ and
+// KI/ = KernelInterpolation/).  Index conventions are kept 1-based where the
+// reference exposes them: `order` holds original 1-based indices, tree arrays
+// hold unified node IDs (internal 1..n-1, leaf k <-> n-1+k, 0 = null).
+//
+// Deliberate, documented deviations:
+//   * integer powers >= 4 in the kernel shapes are repeated multiplication
+//     (Julia Base's Float64 `^` is a compensated power, within 1 ulp of this);
+//     `^2`/`^3` match Julia's literal_pow exactly (x*x, x*x*x);
+//   * a negative line distance^2 (rounding, |d2| ~ ulp) is clamped to 0 before
+//     sqrt, where Julia's sqrt would throw DomainError;
+//   * the line-integrated tables come from this project's own quadrature
+//     (partia.jl_amd/csrc/gen_line_lut.py), not from the reference's literal
+//     table; the two agree to <= 1.6e-10 of I(0).
+// Everything else is evaluated in the same order as the reference so that, for a
+// given float type, results are reproducible to the last bit on this CPU.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+#include <vector>
+
+namespace oracle {
+
+using i32 = int32_t;
+using i64 = int64_t;
+using u32 = uint32_t;
+using u64 = uint64_t;
+
+enum KernelId { M4 = 0, M5 = 1, M6 = 2, C2 = 3, C4 = 4, C6 = 5 };
+enum Strategy { GATHER = 0, SCATTER = 1, SYMMETRIC = 2 };
+
+// ---------------------------------------------------------------------------
+// Kernel shapes: KI/kernel_function/kernels/{M4,M5,M6}_spline.jl:13-37 and
+// {C2,C4,C6}_Wendland.jl:13-36.  f(q<0) = NaN, f = 0 beyond the support.
+// ---------------------------------------------------------------------------
+template <class T> inline T p2(T x) { return x * x; }          // literal_pow ^2
+template <class T> inline T p3(T x) { return x * x * x; }      // literal_pow ^3
+template <class T> inline T p4(T x) { T x2 = x * x; return x2 * x2; }
+template <class T> inline T p5(T x) { return p4(x) * x; }
+template <class T> inline T p6(T x) { T x2 = x * x; return (x2 * x2) * x2; }
+template <class T> inline T p7(T x) { return p6(x) * x; }
+template <class T> inline T p8(T x) { T x4 = p4(x); return x4 * x4; }
+
+template <class T> inline T kernel_f(int K, T q) {
+    const T nan = std::numeric_limits<T>::quiet_NaN();
+    if (q < T(0)) return nan;
+    switch (K) {
+    case M4:
+        if (q < T(1)) return T(0.25) * p3(T(2) - q) - p3(T(1) - q);
+        if (q < T(2)) return T(0.25) * p3(T(2) - q);
+        return T(0);
+    case M5:
+        if (q < T(0.5)) return p4(T(2.5) - q) - T(5) * p4(T(1.5) - q) + T(10) * p4(T(0.5) - q);
+        if (q < T(1.5)) return p4(T(2.5) - q) - T(5) * p4(T(1.5) - q);
+        if (q < T(2.5)) return p4(T(2.5) - q);
+        return T(0);
+    case M6:
+        if (q < T(1)) return p5(T(3) - q) - T(6) * p5(T(2) - q) + T(15) * p5(T(1) - q);
+        if (q < T(2)) return p5(T(3) - q) - T(6) * p5(T(2) - q);
+        if (q < T(3)) return p5(T(3) - q);
+        return T(0);
+    case C2:
+        if (q < T(2)) return p4(T(1) - T(0.5) * q) * (T(2) * q + T(1));
+        return T(0);
+    case C4:
+        if (q < T(2)) return p6(T(1) - T(0.5) * q) * ((T(35) / T(12)) * p2(q) + T(3) * q + T(1));
+        return T(0);
+    case C6:
+        if (q < T(2))
+            return p8(T(1) - T(0.5) * q) * (T(4) * p3(q) + T(6.25) * p2(q) + T(4) * q + T(1));
+        return T(0);
+    }
+    return nan;
+}
+
+// Derivatives: _dM4_spline ... _dC6_Wendland (same files, lines 26-37 / 24-36).
+template <class T> inline T kernel_df(int K, T q) {
+    const T nan = std::numeric_limits<T>::quiet_NaN();
+    if (q < T(0)) return nan;
+    switch (K) {
+    case M4:
+        if (q < T(1)) return T(-0.75) * p2(T(2) - q) + T(3) * p2(T(1) - q);
+        if (q < T(2)) return T(-0.75) * p2(T(2) - q);
+        return T(0);
+    case M5:
+        if (q < T(0.5))
+            return T(-4) * p3(T(2.5) - q) + T(20) * p3(T(1.5) - q) - T(40) * p3(T(0.5) - q);
+        if (q < T(1.5)) return T(-4) * p3(T(2.5) - q) + T(20) * p3(T(1.5) - q);
+        if (q < T(2.5)) return T(-4) * p3(T(2.5) - q);
+        return T(0);
+    case M6:
+        if (q < T(1))
+            return T(-5) * p4(T(3) - q) + T(30) * p4(T(2) - q) - T(75) * p4(T(1) - q);
+        if (q < T(2)) return T(-5) * p4(T(3) - q) + T(30) * p4(T(2) - q);
+        if (q < T(3)) return T(-5) * p4(T(3) - q);
+        return T(0);
+    case C2:
+        if (q < T(2))
+            return p4(T(1) - T(0.5) * q) * T(2) - p3(T(1) - T(0.5) * q) * (T(4) * q + T(2));
+        return T(0);
+    case C4:
+        if (q < T(2))
+            return p6(T(1) - T(0.5) * q) * ((T(35) / T(6)) * q + T(3)) -
+                   p5(T(1) - T(0.5) * q) * ((T(35) / T(4)) * p2(q) + T(9) * q + T(3));
+        return T(0);
+    case C6:
+        if (q < T(2))
+            return p8(T(1) - T(0.5) * q) * (T(12) * p2(q) + T(12.5) * q + T(4)) -
+                   p7(T(1) - T(0.5) * q) * (T(16) * p3(q) + T(25) * p2(q) + T(16) * q + T(4));
+        return T(0);
+    }
+    return nan;
+}
+
+// KernelFunctionValid (M4_spline.jl:66 and siblings).
+template <class T> inline T kernel_support(int K) {
+    switch (K) {
+    case M5: return T(2.5);
+    case M6: return T(3.0);
+    default: return T(2.0);
+    }
+}
+
+// KernelFunctionnorm(K, Val(3), T) (M4_spline.jl:101 and siblings); T(pi) is
+// the correctly rounded pi of T.
+template <class T> inline T kernel_sigma3(int K) {
+    const T pi = T(3.14159265358979323846264338327950288L);
+    switch (K) {
+    case M4: return T(1) / pi;
+    case M5: return T(0.05) / pi;
+    case M6: return T(1) / (T(120) * pi);
+    case C2: return T(21) / (T(16) * pi);
+    case C4: return T(495) / (T(256) * pi);
+    case C6: return T(1365) / (T(512) * pi);
+    }
+    return T(0);
+}
+
+// Smoothed_kernel_function(K, r, h, Val(3)) -- KI/kernel_function/kernel.jl:141-146,86-88.
+template <class T> inline T kernel_W(int K, T r, T h) {
+    T invh = T(1) / h;
+    T inv_hD = invh * invh * invh;
+    T q = r * invh;
+    return inv_hD * (kernel_sigma3<T>(K) * kernel_f<T>(K, q));
+}
+
+// Smoothed_gradient_kernel_function(K, dx, dy, dz, h) -- kernel.jl:273-285,360-365.
+template <class T> inline void kernel_gradW(int K, T dx, T dy, T dz, T h, T g[3]) {
+    T invh = T(1) / h;
+    T inv_hDp1 = invh * invh * invh * invh;
+    T r2 = dx * dx + dy * dy + dz * dz;
+    T gx = T(0), gy = T(0), gz = T(0);
+    if (!(r2 == T(0))) {
+        T r = std::sqrt(r2);
+        T invr = T(1) / r;
+        T q = r / h;
+        T coeff = kernel_df<T>(K, q) * kernel_sigma3<T>(K) * invr;
+        gx = dx * coeff;
+        gy = dy * coeff;
+        gz = dz * coeff;
+    }
+    g[0] = inv_hDp1 * gx;
+    g[1] = inv_hDp1 * gy;
+    g[2] = inv_hDp1 * gz;
+}
+
+// ---------------------------------------------------------------------------
+// Line-integrated kernel lookup: KI/kernel_function/line_integrated_kernel.jl:13-19
+// (_lin_lut), 21-51 (Float32 tables are the Float64 tables rounded), 90-94
+// (cutoff q >= K -> 0), 132-137 (1/h prefactor).
+// ---------------------------------------------------------------------------
+struct LineTables {
+    static constexpr int N = 4096;
+    const double (*inorm)[4096];  // [6][4096] float64 Inorm tables
+};
+
+template <class T> struct LineLUT {
+    std::vector<T> inorm[6];
+    T dq[6];
+    explicit LineLUT(const double (*inorm64)[4096]) {
+        for (int k = 0; k < 6; ++k) {
+            inorm[k].resize(4096);
+            for (int i = 0; i < 4096; ++i) inorm[k][i] = T(inorm64[k][i]);
+            // Q_1 = 0, Q_2 = (1*K)/4095 evaluated in float64, then converted to T.
+            double q2 = (1.0 * kernel_support<double>(k)) / 4095.0;
+            dq[k] = T(q2) - T(0.0);
+        }
+    }
+    T lookup(int K, T q) const {  // _lin_lut
+        T idxf = q / dq[K] + T(1);
+        i32 i = (i32)idxf;  // unsafe_trunc(Int32, idxf)
+        i = std::min(std::max(i, (i32)1), (i32)(4096 - 1));
+        T t = idxf - T(i);
+        return inorm[K][i - 1] * (T(1) - t) + inorm[K][i] * t;
+    }
+    T dimensionless(int K, T q) const {
+        if (q >= kernel_support<T>(K)) return T(0);
+        return lookup(K, q);
+    }
+    T W(int K, T r, T h) const {  // line_integrated_kernel_function(K, r, h)
+        T invh = T(1) / h;
+        T q = r * invh;
+        return invh * dimensionless(K, q);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Morton encoding: LBVH/MortonEncoding/toolbox.jl
+// ---------------------------------------------------------------------------
+inline u32 expand_bits3(u32 x) {  // toolbox.jl:38-45
+    x = (x | (x << 16)) & 0x30000ffu;
+    x = (x | (x << 8)) & 0x300f00fu;
+    x = (x | (x << 4)) & 0x30c30c3u;
+    x = (x | (x << 2)) & 0x9249249u;
+    return x;
+}
+inline u64 expand_bits3(u64 x) {  // toolbox.jl:47-55
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+template <class TI> inline TI encode3(TI ix, TI iy, TI iz) {  // toolbox.jl:113-118
+    return (expand_bits3(ix) << 2) | (expand_bits3(iy) << 1) | expand_bits3(iz);
+}
+template <class TI> constexpr int axis_bits() { return (int(sizeof(TI) * 8) - int(sizeof(TI) * 8) % 3) / 3; }
+
+// _normalize_vector + _quantize_coords (toolbox.jl:281-294, 311-336).
+template <class T, class TI> void quantize_axis(const std::vector<T>& v, std::vector<TI>& out) {
+    size_t n = v.size();
+    out.resize(n);
+    T scale = T(std::exp2(T(axis_bits<TI>()))) - T(1);
+    T vmin = v[0], vmax = v[0];
+    for (size_t i = 1; i < n; ++i) {  // minimum / maximum
+        vmin = std::min(vmin, v[i]);
+        vmax = std::max(vmax, v[i]);
+    }
+    T dv = vmax - vmin;
+    if (dv == T(0)) {
+        for (size_t i = 0; i < n; ++i) out[i] = (TI)std::floor(scale * T(0.5));
+        return;
+    }
+    T inv = T(1) / dv;
+    for (size_t i = 0; i < n; ++i) {
+        T f = (v[i] - vmin) * inv;
+        out[i] = (TI)std::floor(scale * f);
+    }
+}
+
+// leading_zeros for the code width / UInt64 (toolbox.jl:214-225).
+template <class TI> inline int clz_t(TI v) {
+    if (v == 0) return int(sizeof(TI) * 8);
+    if (sizeof(TI) == 4) return __builtin_clz((u32)v);
+    return __builtin_clzll((u64)v);
+}
+template <class TI> inline i64 lcp(const std::vector<TI>& c, i64 i, i64 j) {  // 1-based i, j
+    TI a = c[i - 1], b = c[j - 1];
+    if (a == b) return clz_t<u64>((u64)i ^ (u64)j);
+    return clz_t<TI>(a ^ b);
+}
+
+template <class T, class TI> struct Encoding {
+    std::vector<TI> order;  // 1-based original index, in code type (toolbox.jl:127-134)
+    std::vector<TI> codes;
+    std::vector<T> coord[3];
+};
+
+// MortonEncoding(x, y, z; CodeType) -- MortonEncoding.jl:36-43 + sort_by_morton!
+// (toolbox.jl:20-33): stable sort by code (sortperm's Perm ordering breaks ties
+// by index), permute codes/order/coords, then rewrite duplicates upward.
+template <class T, class TI>
+Encoding<T, TI> morton_encoding(const std::vector<T>& x, const std::vector<T>& y, const std::vector<T>& z) {
+    size_t n = x.size();
+    std::vector<TI> ix, iy, iz;
+    quantize_axis<T, TI>(x, ix);
+    quantize_axis<T, TI>(y, iy);
+    quantize_axis<T, TI>(z, iz);
+    std::vector<TI> code(n), order(n);
+    for (size_t i = 0; i < n; ++i) {
+        code[i] = encode3<TI>(ix[i], iy[i], iz[i]);
+        order[i] = TI(i + 1);
+    }
+    std::vector<size_t> p(n);
+    std::iota(p.begin(), p.end(), size_t(0));
+    std::stable_sort(p.begin(), p.end(), [&](size_t a, size_t b) { return code[a] < code[b]; });
+    Encoding<T, TI> e;
+    e.codes.resize(n);
+    e.order.resize(n);
+    for (int d = 0; d < 3; ++d) e.coord[d].resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        e.codes[i] = code[p[i]];
+        e.order[i] = order[p[i]];
+        e.coord[0][i] = x[p[i]];
+        e.coord[1][i] = y[p[i]];
+        e.coord[2][i] = z[p[i]];
+    }
+    for (size_t i = 1; i < n; ++i)
+        if (e.codes[i] <= e.codes[i - 1]) e.codes[i] = e.codes[i - 1] + TI(1);
+    return e;
+}
+
+// ---------------------------------------------------------------------------
+// Binary radix tree: LBVH/BinaryRadixTree/BinaryRadixTree.jl:82-133, toolbox.jl
+// ---------------------------------------------------------------------------
+struct BRT {
+    i32 root = 0;
+    i64 nleaf = 0;
+    std::vector<i32> left, right, escape, parent;  // index = unified id - 1
+    std::vector<i32> range_lo, range_hi;           // per internal node (1-based leaf idx)
+};
+
+template <class TI> inline int range_direction(const std::vector<TI>& c, i64 i) {  // toolbox.jl:14-19
+    i64 n = (i64)c.size();
+    i64 dl = (i > 1) ? lcp(c, i, i - 1) : -1;
+    i64 dr = (i < n) ? lcp(c, i, i + 1) : -1;
+    i64 s = dr - dl;
+    int d = (s > 0) - (s < 0);
+    return d == 0 ? 1 : d;
+}
+
+template <class TI> inline void find_range(const std::vector<TI>& c, i64 i, i64& lo, i64& hi) {  // :22-53
+    i64 n = (i64)c.size();
+    int d = range_direction(c, i);
+    i64 dmin = (1 <= i - d && i - d <= n) ? lcp(c, i, i - d) : -1;
+    i64 lmax = 2;
+    while (true) {
+        i64 j = i + lmax * d;
+        if (j < 1 || j > n) break;
+        if (lcp(c, i, j) <= dmin) break;
+        lmax *= 2;
+    }
+    i64 l = 0;
+    i64 t = lmax >> 1;
+    while (t > 0) {
+        i64 j = i + (l + t) * d;
+        if (j >= 1 && j <= n && lcp(c, i, j) > dmin) l += t;
+        t >>= 1;
+    }
+    i64 j = i + l * d;
+    lo = std::min(i, j);
+    hi = std::max(i, j);
+}
+
+template <class TI> inline i64 split_position(const std::vector<TI>& c, i64 first, i64 last) {  // :55-76
+    if (c[first - 1] == c[last - 1]) return std::min(std::max((first + last) >> 1, first), last - 1);
+    i64 pfl = lcp(c, first, last);
+    i64 split = first;
+    i64 step = last - first;
+    while (step > 1) {
+        step = (step + 1) >> 1;
+        i64 ns = split + step;
+        if (ns < last) {
+            if (lcp(c, first, ns) > pfl) split = ns;
+        }
+    }
+    return split;
+}
+
+template <class TI> BRT build_brt(const std::vector<TI>& codes) {
+    i64 n = (i64)codes.size();
+    if (n < 1) throw std::invalid_argument("BinaryRadixTree: enc.codes must be non-empty (got n=0).");
+    const i64 nmax = (i64)(std::numeric_limits<i32>::max() / 2) + 1;
+    if (n > nmax) throw std::invalid_argument("BinaryRadixTree: n exceeds the Int32 node-id capacity");
+    BRT t;
+    t.nleaf = n;
+    t.root = n >= 2 ? 1 : 0;
+    i64 tot = 2 * n - 1;
+    t.left.assign(tot, 0);
+    t.right.assign(tot, 0);
+    t.escape.assign(tot, 0);
+    t.parent.assign(tot, 0);
+    if (n < 2) return t;
+    i64 nint = n - 1, off = n - 1;
+    t.range_lo.assign(nint, 0);
+    t.range_hi.assign(nint, 0);
+    for (i64 i = 1; i <= nint; ++i) {  // _build_child! (toolbox.jl:83-117)
+        i64 lo, hi;
+        find_range(codes, i, lo, hi);
+        t.range_lo[i - 1] = (i32)lo;
+        t.range_hi[i - 1] = (i32)hi;
+        i64 s = split_position(codes, lo, hi);
+        i64 idx = (s == lo) ? lo + off : s;
+        t.left[i - 1] = (i32)idx;
+        t.parent[idx - 1] = (i32)i;
+        idx = (s + 1 == hi) ? hi + off : s + 1;
+        t.right[i - 1] = (i32)idx;
+        t.parent[idx - 1] = (i32)i;
+    }
+    std::vector<i32> adj(n);  // _build_adjacent! + sentinel (BinaryRadixTree.jl:112-116)
+    for (i64 i = 1; i <= n - 1; ++i) adj[i - 1] = (i32)lcp(codes, i, i + 1);
+    adj[n - 1] = std::numeric_limits<i32>::max();
+    for (i64 i = 1; i <= nint; ++i) {  // internal escapes (toolbox.jl:119-134)
+        i64 hi = t.range_hi[i - 1];
+        if (hi == n) t.escape[i - 1] = 0;
+        else if (hi == n - 1) t.escape[i - 1] = (i32)(off + n);
+        else {
+            i64 nx = hi + 1;
+            t.escape[i - 1] = (adj[nx - 1] < adj[nx - 2]) ? (i32)(off + nx) : (i32)nx;
+        }
+    }
+    for (i64 id = n; id <= tot; ++id) {  // leaf escapes (toolbox.jl:136-156)
+        i32 p = t.parent[id - 1];
+        if (p == 0) { t.escape[id - 1] = 0; continue; }
+        if (t.left[p - 1] == (i32)id) t.escape[id - 1] = t.right[p - 1];
+        else t.escape[id - 1] = t.escape[p - 1];
+    }
+    return t;
+}
+
+inline bool is_leaf_id(i32 node, i64 n) { return node != 0 && node >= (i32)n; }
+inline i64 leaf_index(i32 node, i64 n) { return (i64)node - (n - 1); }  // 1..n
+
+// ---------------------------------------------------------------------------
+// LinearBVH: LBVH/LinearBVH/LinearBVH.jl:56-94, toolbox.jl:9-94
+// ---------------------------------------------------------------------------
+template <class T> struct LBVH {
+    BRT brt;
+    std::vector<T> leaf[3], leaf_scale;
+    std::vector<T> nmin[3], nmax[3], nscale;  // per internal node (index = id - 1)
+};
+
+template <class T, class TI>
+LBVH<T> build_lbvh(const Encoding<T, TI>& e, BRT brt, const std::vector<T>& h_sorted) {
+    LBVH<T> b;
+    i64 n = brt.nleaf, nint = n - 1;
+    for (int d = 0; d < 3; ++d) {
+        b.leaf[d] = e.coord[d];
+        b.nmin[d].assign(std::max<i64>(nint, 0), T(0));
+        b.nmax[d].assign(std::max<i64>(nint, 0), T(0));
+    }
+    b.leaf_scale.assign(h_sorted.begin(), h_sorted.begin() + n);  // BoxScale(h, true)
+    b.nscale.assign(std::max<i64>(nint, 0), T(0));
+    std::vector<u32> visited(std::max<i64>(nint, 0), 0);
+    for (i64 start = n; start <= 2 * n - 1; ++start) {  // _build_internal_aabb!
+        i32 p = brt.parent[start - 1];
+        while (p != 0) {
+            if (++visited[p - 1] == 2) {
+                i32 l = brt.left[p - 1], r = brt.right[p - 1];
+                bool ll = is_leaf_id(l, n), rl = is_leaf_id(r, n);
+                T hl = ll ? b.leaf_scale[leaf_index(l, n) - 1] : b.nscale[l - 1];
+                T hr = rl ? b.leaf_scale[leaf_index(r, n) - 1] : b.nscale[r - 1];
+                b.nscale[p - 1] = (hl > hr) ? hl : hr;
+                for (int d = 0; d < 3; ++d) {
+                    T lmin = ll ? b.leaf[d][leaf_index(l, n) - 1] : b.nmin[d][l - 1];
+                    T rmin = rl ? b.leaf[d][leaf_index(r, n) - 1] : b.nmin[d][r - 1];
+                    T lmax = ll ? b.leaf[d][leaf_index(l, n) - 1] : b.nmax[d][l - 1];
+                    T rmax = rl ? b.leaf[d][leaf_index(r, n) - 1] : b.nmax[d][r - 1];
+                    b.nmin[d][p - 1] = (lmin < rmin) ? lmin : rmin;
+                    b.nmax[d][p - 1] = (lmax > rmax) ? lmax : rmax;
+                }
+                p = brt.parent[p - 1];
+            } else {
+                break;
+            }
+        }
+    }
+    for (i64 i = 0; i < nint; ++i)  // LinearBVH.jl:89-91
+        if (visited[i] < 2) throw std::logic_error("LinearBVH refit: internal node not reached twice");
+    b.brt = std::move(brt);
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// Geometry helpers: LBVH/LinearBVH/LinearBVHTraversalMacros.jl:800-953
+// ---------------------------------------------------------------------------
+template <class T> inline T d2_point_leaf(const T p[3], const LBVH<T>& b, i64 k /*1-based*/) {
+    T s = T(0);
+    for (int d = 0; d < 3; ++d) {
+        T dd = p[d] - b.leaf[d][k - 1];
+        s += dd * dd;
+    }
+    return s;
+}
+template <class T> inline T d2_point_aabb(const T p[3], const LBVH<T>& b, i64 idx /*1-based*/) {
+    T s = T(0);
+    for (int d = 0; d < 3; ++d) {
+        T a = b.nmin[d][idx - 1], c = b.nmax[d][idx - 1], x = p[d];
+        T t = x < a ? (a - x) : (x > c ? (x - c) : T(0));
+        s += t * t;
+    }
+    return s;
+}
+template <class T> inline T d2_point_line(const T p[3], const T o[3], const T m[3]) {  // :837-857
+    T D2 = T(0), Dm = T(0);
+    for (int d = 0; d < 3; ++d) {
+        T D = p[d] - o[d];
+        D2 += D * D;
+        Dm += D * m[d];
+    }
+    return D2 - Dm * Dm;
+}
+template <class T> inline T d2_line_leaf(const T o[3], const T m[3], const LBVH<T>& b, i64 k) {  // :859-876
+    T D2 = T(0), Dm = T(0);
+    for (int d = 0; d < 3; ++d) {
+        T D = b.leaf[d][k - 1] - o[d];
+        D2 += D * D;
+        Dm += D * m[d];
+    }
+    return D2 - Dm * Dm;
+}
+template <class T> inline bool line_hits_aabb(const T o[3], const T m[3], const LBVH<T>& b, i64 idx) {  // :878-916
+    T tmin = -std::numeric_limits<T>::infinity(), tmax = std::numeric_limits<T>::infinity();
+    for (int d = 0; d < 3; ++d) {
+        T a = b.nmin[d][idx - 1], c = b.nmax[d][idx - 1];
+        if (m[d] == T(0)) {
+            if (o[d] < a || o[d] > c) return false;
+        } else {
+            T t0 = (a - o[d]) / m[d], t1 = (c - o[d]) / m[d];
+            if (t0 > t1) std::swap(t0, t1);
+            tmin = std::max(tmin, t0);
+            tmax = std::min(tmax, t1);
+            if (tmin > tmax) return false;
+        }
+    }
+    return true;
+}
+template <class T> inline T d2_line_aabb_lb(const T o[3], const T m[3], const LBVH<T>& b, i64 idx) {  // :918-953
+    const T half = T(1) / T(2);
+    if (line_hits_aabb(o, m, b, idx)) return T(0);
+    T c[3];
+    T r2 = T(0);
+    for (int d = 0; d < 3; ++d) c[d] = (b.nmin[d][idx - 1] + b.nmax[d][idx - 1]) * half;
+    for (int d = 0; d < 3; ++d) {
+        T hh = (b.nmax[d][idx - 1] - b.nmin[d][idx - 1]) * half;
+        r2 += hh * hh;
+    }
+    T dc2 = d2_point_line(c, o, m);
+    if (dc2 <= r2) return T(0);
+    T dl = std::sqrt(dc2) - std::sqrt(r2);
+    return dl * dl;
+}
+
+// ---------------------------------------------------------------------------
+// Stackless traversals (LinearBVHTraversalMacros.jl:26-108 gather point,
+// 135-236 scatter point, 268-371 symmetric point, 408-494 / 539-644 / 691-796
+// the line variants).  `hit(k, d2, hb)` is called on each accepted leaf in DFS
+// preorder (= increasing leaf index).
+// ---------------------------------------------------------------------------
+template <class T, class LeafD2, class NodeD2, class LeafR2, class NodeR2, class Hit>
+inline void traverse(const LBVH<T>& b, LeafD2 leafd2, NodeD2 noded2, LeafR2 leafr2, NodeR2 noder2, Hit hit) {
+    const BRT& t = b.brt;
+    i64 n = t.nleaf;
+    if (t.root == 0) {
+        for (i64 k = 1; k <= n; ++k) {
+            T d2 = leafd2(k);
+            if (d2 <= leafr2(k)) hit(k, d2, b.leaf_scale[k - 1]);
+        }
+        return;
+    }
+    i32 node = t.root;
+    while (node != 0) {
+        if (is_leaf_id(node, n)) {
+            i64 k = leaf_index(node, n);
+            T d2 = leafd2(k);
+            if (d2 <= leafr2(k)) hit(k, d2, b.leaf_scale[k - 1]);
+            node = t.escape[node - 1];
+            continue;
+        }
+        i64 idx = node;
+        T d2n = noded2(idx);
+        if (d2n > noder2(idx)) {
+            node = t.escape[node - 1];
+            continue;
+        }
+        node = t.left[idx - 1];
+    }
+}
+
+// Radius rules (A13): gather r2 = (K ha)^2; scatter (K h)^2; symmetric max of both.
+template <class T> struct Radius {
+    int strategy;
+    T Kvalid, radius2;  // radius2 = (K*ha)^2 for gather / symmetric
+    const LBVH<T>* b;
+    T leaf(i64 k) const {
+        if (strategy == GATHER) return radius2;
+        T r = Kvalid * b->leaf_scale[k - 1];
+        T rr = r * r;
+        return strategy == SCATTER ? rr : std::max(radius2, rr);
+    }
+    T node(i64 idx) const {
+        if (strategy == GATHER) return radius2;
+        T r = Kvalid * b->nscale[idx - 1];
+        T rr = r * r;
+        return strategy == SCATTER ? rr : std::max(radius2, rr);
+    }
+};
+
+template <class T, class Hit>
+inline void point_traversal(const LBVH<T>& b, const T p[3], int strategy, T Kvalid, T ha, Hit hit) {
+    T r = Kvalid * ha;
+    Radius<T> R{strategy, Kvalid, r * r, &b};
+    traverse<T>(
+        b, [&](i64 k) { return d2_point_leaf(p, b, k); }, [&](i64 i) { return d2_point_aabb(p, b, i); },
+        [&](i64 k) { return R.leaf(k); }, [&](i64 i) { return R.node(i); }, hit);
+}
+
+template <class T, class Hit>
+inline void line_traversal(const LBVH<T>& b, const T o[3], const T m[3], int strategy, T Kvalid, T ha, Hit hit) {
+    T r = Kvalid * ha;
+    Radius<T> R{strategy, Kvalid, r * r, &b};
+    traverse<T>(
+        b, [&](i64 k) { return d2_line_leaf(o, m, b, k); }, [&](i64 i) { return d2_line_aabb_lb(o, m, b, i); },
+        [&](i64 k) { return R.leaf(k); }, [&](i64 i) { return R.node(i); }, hit);
+}
+
+// LBVH_find_nearest_h (LBVH/LinearBVH/LinearBVHQueries.jl:118-140): the bound
+// shrinks as the traversal proceeds; strict `<` keeps the first leaf in DFS order.
+template <class T> inline T nearest_h(const LBVH<T>& b, const T p[3], i64* best_out = nullptr) {
+    const BRT& t = b.brt;
+    i64 n = t.nleaf;
+    i64 best = 0;
+    T bd = std::numeric_limits<T>::infinity();  // typemax(Float) = Inf
+    auto leaf_step = [&](i64 k) {
+        T d2 = d2_point_leaf(p, b, k);
+        if (d2 <= bd) {
+            if (d2 < bd) { bd = d2; best = k; }
+        }
+    };
+    if (t.root == 0) {
+        for (i64 k = 1; k <= n; ++k) leaf_step(k);
+    } else {
+        i32 node = t.root;
+        while (node != 0) {
+            if (is_leaf_id(node, n)) {
+                leaf_step(leaf_index(node, n));
+                node = t.escape[node - 1];
+                continue;
+            }
+            if (d2_point_aabb(p, b, node) > bd) { node = t.escape[node - 1]; continue; }
+            node = t.left[node - 1];
+        }
+    }
+    if (best_out) *best_out = best;
+    if (best == 0) return std::numeric_limits<T>::quiet_NaN();
+    return b.leaf_scale[best - 1];
+}
+
+// ---------------------------------------------------------------------------
+// Interpolation input after LinearBVH! (KI/interpolation_setup/InterpolationInput.jl:201-221)
+// ---------------------------------------------------------------------------
+template <class T> struct Input {
+    int kernel = M4;
+    std::vector<T> x, y, z, m, h, rho;  // Morton order after LinearBVH!
+    std::vector<std::vector<T>> quant;
+    std::vector<u64> order, codes;
+    LBVH<T> bvh;
+};
+
+template <class T, class TI> void linear_bvh_inplace(Input<T>& in) {
+    Encoding<T, TI> e = morton_encoding<T, TI>(in.x, in.y, in.z);
+    size_t n = in.x.size();
+    auto perm = [&](std::vector<T>& v) {
+        std::vector<T> w(n);
+        for (size_t i = 0; i < n; ++i) w[i] = v[(size_t)e.order[i] - 1];
+        v.swap(w);
+    };
+    perm(in.x); perm(in.y); perm(in.z); perm(in.m); perm(in.h); perm(in.rho);
+    for (auto& c : in.quant) perm(c);
+    in.order.assign(e.order.begin(), e.order.end());
+    in.codes.assign(e.codes.begin(), e.codes.end());
+    BRT brt = build_brt<TI>(e.codes);
+    in.bvh = build_lbvh<T, TI>(e, std::move(brt), in.h);
+}
+
+// Concise catalog (KI/interpolation_setup/InterpolationCatalog.jl:26-32): 1-based
+// column slots; a gradient slot of 0 means the density gradient branch.
+struct Catalog {
+    std::vector<int> sslots;
+    std::vector<unsigned char> sflags;
+    std::vector<int> gslots;
+    std::vector<int> dslots;  // 3 per divergence
+    std::vector<int> cslots;  // 3 per curl
+    int L() const { return int(sslots.size() + 3 * gslots.size() + dslots.size() / 3 + cslots.size()); }
+};
+
+// _general_quantity_interpolate_kernel (KI/single_point_interpolation/kernels/
+// general_interpolation.jl:1-223 gather, 225-444 scatter, 446-669 symmetric)
+// with the per-neighbour terms of accumulations/{scalar,gradient,divergence,
+// curl}_accumulation.jl.  Writes L values in catalog order (A17) to out[k*stride].
+template <class T>
+void general_interpolate(const Input<T>& in, const Catalog& cat, int strategy, const T p[3], T ha, T* out,
+                         i64 stride) {
+    const int K = in.kernel;
+    const T Kvalid = kernel_support<T>(K);
+    const size_t N = cat.sslots.size(), G = cat.gslots.size(), D = cat.dslots.size() / 3,
+                 C = cat.cslots.size() / 3;
+    T S1 = T(0);
+    std::vector<T> sc(N, T(0)), gf(3 * G, T(0)), gb(3 * G, T(0)), gs(G, T(0));
+    std::vector<T> df(D, T(0)), db(3 * D, T(0)), ds(3 * D, T(0));
+    std::vector<T> cf(3 * C, T(0)), cb(3 * C, T(0)), cs(3 * C, T(0));
+    const bool sym = strategy == SYMMETRIC;
+
+    auto Wsel = [&](T dr, T hb) -> T {
+        if (sym) return T(0.5) * (kernel_W<T>(K, dr, ha) + kernel_W<T>(K, dr, hb));
+        return kernel_W<T>(K, dr, strategy == GATHER ? ha : hb);
+    };
+    auto Gsel = [&](T dx, T dy, T dz, T hb, T g[3]) {
+        if (sym) {
+            T ga[3], gb_[3];
+            kernel_gradW<T>(K, dx, dy, dz, ha, ga);
+            kernel_gradW<T>(K, dx, dy, dz, hb, gb_);
+            for (int d = 0; d < 3; ++d) g[d] = T(0.5) * (ga[d] + gb_[d]);
+        } else {
+            kernel_gradW<T>(K, dx, dy, dz, strategy == GATHER ? ha : hb, g);
+        }
+    };
+
+    point_traversal<T>(in.bvh, p, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
+        const size_t i = (size_t)k - 1;
+        T dx = p[0] - in.x[i], dy = p[1] - in.y[i], dz = p[2] - in.z[i];
+        T dr = std::sqrt(d2);
+        T mb = in.m[i], rb = in.rho[i];
+        T W = Wsel(dr, hb);
+        S1 += mb * W / rb;
+        for (size_t j = 0; j < N; ++j) sc[j] += in.quant[cat.sslots[j] - 1][i] * (mb * W / rb);
+        if (G + D + C == 0) return;
+        T g[3];
+        Gsel(dx, dy, dz, hb, g);
+        T invr = T(1) / rb;
+        for (size_t j = 0; j < G; ++j) {
+            int slot = cat.gslots[j];
+            if (slot == 0) {  // gradient of density (gradient_accumulation.jl:7-35)
+                for (int d = 0; d < 3; ++d) {
+                    T mg = mb * g[d];
+                    gf[3 * j + d] += mg;
+                    gb[3 * j + d] += mg * invr;
+                }
+                gs[j] += mb * W;
+            } else {
+                T A = in.quant[slot - 1][i];
+                for (int d = 0; d < 3; ++d) {
+                    T w = mb * invr * g[d];
+                    gf[3 * j + d] += w * A;
+                    gb[3 * j + d] += w;
+                }
+                gs[j] += A * (mb * W / rb);
+            }
+        }
+        for (size_t j = 0; j < D; ++j) {
+            T A[3], w[3];
+            for (int d = 0; d < 3; ++d) {
+                A[d] = in.quant[cat.dslots[3 * j + d] - 1][i];
+                w[d] = mb * invr * g[d];
+            }
+            df[j] += w[0] * A[0] + w[1] * A[1] + w[2] * A[2];
+            for (int d = 0; d < 3; ++d) {
+                db[3 * j + d] += w[d];
+                ds[3 * j + d] += A[d] * (mb * W / rb);
+            }
+        }
+        for (size_t j = 0; j < C; ++j) {
+            T A[3], w[3];
+            for (int d = 0; d < 3; ++d) {
+                A[d] = in.quant[cat.cslots[3 * j + d] - 1][i];
+                w[d] = mb * invr * g[d];
+            }
+            cf[3 * j + 0] += A[1] * w[2] - A[2] * w[1];
+            cf[3 * j + 1] += A[2] * w[0] - A[0] * w[2];
+            cf[3 * j + 2] += A[0] * w[1] - A[1] * w[0];
+            for (int d = 0; d < 3; ++d) {
+                cb[3 * j + d] += w[d];
+                cs[3 * j + d] += A[d] * (mb * W / rb);
+            }
+        }
+    });
+
+    i64 o = 0;
+    if (S1 == T(0)) {  // general_interpolation.jl:128-131
+        const T nan = std::numeric_limits<T>::quiet_NaN();
+        for (int k = 0; k < cat.L(); ++k) out[(o++) * stride] = nan;
+        return;
+    }
+    T invS1 = T(1) / S1;
+    for (size_t j = 0; j < N; ++j) out[(o++) * stride] = cat.sflags[j] ? sc[j] * invS1 : sc[j];
+    for (size_t j = 0; j < G; ++j) {
+        T A = gs[j] * invS1;
+        for (int d = 0; d < 3; ++d) out[(o++) * stride] = gf[3 * j + d] - gb[3 * j + d] * A;
+    }
+    for (size_t j = 0; j < D; ++j) {
+        T Ax = ds[3 * j] * invS1, Ay = ds[3 * j + 1] * invS1, Az = ds[3 * j + 2] * invS1;
+        T b = Ax * db[3 * j] + Ay * db[3 * j + 1] + Az * db[3 * j + 2];
+        out[(o++) * stride] = df[j] - b;
+    }
+    for (size_t j = 0; j < C; ++j) {
+        T Ax = cs[3 * j] * invS1, Ay = cs[3 * j + 1] * invS1, Az = cs[3 * j + 2] * invS1;
+        T bx = cb[3 * j], by = cb[3 * j + 1], bz = cb[3 * j + 2];
+        T cx = Ay * bz - Az * by, cy = Az * bx - Ax * bz, cz = Ax * by - Ay * bx;
+        out[(o++) * stride] = -(cf[3 * j] - cx);
+        out[(o++) * stride] = -(cf[3 * j + 1] - cy);
+        out[(o++) * stride] = -(cf[3 * j + 2] - cz);
+    }
+}
+
+// _line_integrated_quantities_interpolate_kernel (KI/line_integrated_interpolation/
+// kernels/line_integrated_scalar_interpolation.jl:76-214) with the accumulations
+// of accumulations/line_integrated_scalar_accumulation.jl:6-46.
+template <class T>
+void line_interpolate(const Input<T>& in, const LineLUT<T>& lut, const Catalog& cat, int strategy, const T o[3],
+                      const T m[3], T ha, T* out, i64 stride) {
+    const int K = in.kernel;
+    const T Kvalid = kernel_support<T>(K);
+    const size_t N = cat.sslots.size();
+    std::vector<T> acc(N, T(0));
+    T S1 = T(0);
+    const bool sym = strategy == SYMMETRIC;
+    line_traversal<T>(in.bvh, o, m, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
+        const size_t i = (size_t)k - 1;
+        T dr = std::sqrt(d2 < T(0) ? T(0) : d2);
+        T mb = in.m[i], rb = in.rho[i];
+        T W;
+        if (sym) W = T(0.5) * (lut.W(K, dr, ha) + lut.W(K, dr, hb));
+        else W = lut.W(K, dr, strategy == GATHER ? ha : hb);
+        S1 += mb * W / rb;
+        for (size_t j = 0; j < N; ++j) acc[j] += in.quant[cat.sslots[j] - 1][i] * (mb * W / rb);
+    });
+    if (S1 == T(0)) {
+        for (size_t j = 0; j < N; ++j) out[j * stride] = std::numeric_limits<T>::quiet_NaN();
+        return;
+    }
+    T invS1 = T(1) / S1;
+    for (size_t j = 0; j < N; ++j) out[j * stride] = cat.sflags[j] ? acc[j] * invS1 : acc[j];
+}
+
+}  // namespace oracle

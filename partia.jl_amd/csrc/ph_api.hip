@@ -1,0 +1,552 @@
+// ph_api.hip -- extern "C" entry points of libpartia_hip (declared in include/partia_hip.h).
+#include <cstring>
+#include <memory>
+
+#include "ph_build.h"
+#include "ph_interp.h"
+#include "line_lut_tables.inc"
+
+using namespace ph;
+
+namespace {
+
+thread_local std::string g_noctx_error;
+
+int fail(ph_context* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->last_error = msg;
+    else g_noctx_error = msg;
+    return code;
+}
+
+template <class F> int guarded(ph_context* ctx, F&& f) {
+    try {
+        if (ctx) {
+            PH_HIP(hipSetDevice(ctx->device));
+            ctx->last_error.clear();
+        }
+        return f();
+    } catch (const Error& e) {
+        return fail(ctx, e.code, e.msg);
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, PH_ERR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(ctx, PH_ERR_INTERNAL, e.what());
+    }
+}
+
+void need(bool ok, int code, const std::string& msg) {
+    if (!ok) throw Error{code, msg};
+}
+
+float elapsed(ph_context* ctx, int a, int b) {
+    float ms = 0.f;
+    PH_HIP(hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]));
+    return ms;
+}
+
+void check_catalog(const ph_lbvh* B, const ph_catalog* cat) {
+    need(cat != nullptr, PH_ERR_ARGUMENT, "catalog is null");
+    need(cat->n_scalar >= 0 && cat->n_grad >= 0 && cat->n_div >= 0 && cat->n_curl >= 0, PH_ERR_ARGUMENT,
+         "negative catalog size");
+    auto slot_ok = [&](int s, bool allow0) { return (s >= 1 && s <= B->ncol) || (allow0 && s == 0); };
+    for (int j = 0; j < cat->n_scalar; ++j)
+        need(slot_ok(cat->scalar_slots[j], false), PH_ERR_ARGUMENT, "scalar slot out of range");
+    for (int j = 0; j < cat->n_grad; ++j)
+        need(slot_ok(cat->grad_slots[j], true), PH_ERR_ARGUMENT, "gradient slot out of range");
+    for (int j = 0; j < 3 * cat->n_div; ++j)
+        need(slot_ok(cat->div_slots[j], false), PH_ERR_ARGUMENT, "divergence slot out of range");
+    for (int j = 0; j < 3 * cat->n_curl; ++j)
+        need(slot_ok(cat->curl_slots[j], false), PH_ERR_ARGUMENT, "curl slot out of range");
+}
+
+void check_precision(const ph_lbvh* B, int precision) {
+    need(precision == PH_PREC_FP64 || precision == PH_PREC_FP32 || precision == PH_PREC_FP32C, PH_ERR_ARGUMENT,
+         "unknown precision");
+    if (precision == PH_PREC_FP32)
+        need(B->precision == PH_PREC_FP32, PH_ERR_ARGUMENT, "PH_PREC_FP32 needs a tree built from Float32 columns");
+    else
+        need(B->precision == PH_PREC_FP64, PH_ERR_ARGUMENT, "float64 evaluation needs a tree built from Float64 columns");
+    need(precision != PH_PREC_FP32C, PH_ERR_ARGUMENT, "PH_PREC_FP32C is not available in this build");
+}
+
+size_t out_elem(int precision) { return precision == PH_PREC_FP64 ? sizeof(double) : sizeof(float); }
+
+struct Counters {
+    u64* dev;
+};
+
+// Runs the interpolation of all catalog chunks; device-side timing into stats.
+template <class Launch> void run_chunks(ph_context* ctx, const ph_catalog* cat, i64 S, Launch&& launch) {
+    std::vector<CatChunk> chunks;
+    const int L = split_catalog(cat, chunks);
+    u64* counter = nullptr;
+    if (ctx->count_neighbours) {
+        ctx->counters.alloc(64);
+        counter = ctx->counters.as<u64>();
+        PH_HIP(hipMemsetAsync(counter, 0, 16, ctx->stream));
+    }
+    PH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    for (const CatChunk& c : chunks) launch(c, L, counter);
+    PH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    PH_HIP(hipEventSynchronize(ctx->ev[1]));
+    ctx->stats.kernel_ms = elapsed(ctx, 0, 1);
+    ctx->stats.interp_ms = ctx->stats.kernel_ms;
+    ctx->stats.samples = S;
+    ctx->stats.accepted = -1;
+    if (counter) {
+        u64 h = 0;
+        PH_HIP(hipMemcpy(&h, counter, sizeof(u64), hipMemcpyDeviceToHost));
+        ctx->stats.accepted = (int64_t)(h / (chunks.empty() ? 1 : chunks.size()));
+    }
+}
+
+// Copies a device L x S block of T to a host double array.
+void download(ph_context* ctx, const void* dev, int precision, int64_t count, double* host) {
+    if (precision == PH_PREC_FP64) {
+        PH_HIP(hipMemcpy(host, dev, sizeof(double) * count, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(count);
+        PH_HIP(hipMemcpy(tmp.data(), dev, sizeof(float) * count, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < count; ++i) host[i] = tmp[i];
+    }
+}
+
+int catalog_L(const ph_catalog* c) { return c->n_scalar + 3 * c->n_grad + c->n_div + 3 * c->n_curl; }
+
+}  // namespace
+
+extern "C" {
+
+int ph_ctx_create(int device, ph_context** out) {
+    if (!out) return fail(nullptr, PH_ERR_ARGUMENT, "out is null");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return fail(nullptr, PH_ERR_NO_DEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= count) return fail(nullptr, PH_ERR_ARGUMENT, "device index out of range");
+    auto ctx = std::make_unique<ph_context>();
+    ctx->device = device;
+    int rc = guarded(ctx.get(), [&] {
+        PH_HIP(hipSetDevice(device));
+        PH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        for (auto& e : ctx->ev) PH_HIP(hipEventCreate(&e));
+        hipDeviceProp_t prop;
+        PH_HIP(hipGetDeviceProperties(&prop, device));
+        ctx->num_cu = prop.multiProcessorCount;
+        // line-integrated tables, float64 and float32 copies
+        const size_t nt = 6 * (size_t)PH_LUT_N;
+        ctx->lut64.alloc(sizeof(double) * nt);
+        ctx->lut32.alloc(sizeof(float) * nt);
+        std::vector<float> t32(nt);
+        for (size_t i = 0; i < nt; ++i) t32[i] = (float)(&ph_lut_inorm[0][0])[i];
+        PH_HIP(hipMemcpy(ctx->lut64.p, &ph_lut_inorm[0][0], sizeof(double) * nt, hipMemcpyHostToDevice));
+        PH_HIP(hipMemcpy(ctx->lut32.p, t32.data(), sizeof(float) * nt, hipMemcpyHostToDevice));
+        return PH_OK;
+    });
+    if (rc != PH_OK) {
+        g_noctx_error = ctx->last_error;
+        return rc;
+    }
+    *out = ctx.release();
+    return PH_OK;
+}
+
+int ph_ctx_destroy(ph_context* ctx) {
+    if (!ctx) return PH_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->lut64.release();
+    ctx->lut32.release();
+    ctx->counters.release();
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PH_OK;
+}
+
+const char* ph_last_error(const ph_context* ctx) { return ctx ? ctx->last_error.c_str() : g_noctx_error.c_str(); }
+
+int ph_get_stats(const ph_context* ctx, ph_stats* out) {
+    if (!ctx || !out) return PH_ERR_ARGUMENT;
+    *out = ctx->stats;
+    return PH_OK;
+}
+
+int ph_set_count_neighbours(ph_context* ctx, int enable) {
+    if (!ctx) return PH_ERR_ARGUMENT;
+    ctx->count_neighbours = enable ? 1 : 0;
+    return PH_OK;
+}
+
+int ph_device_alloc(ph_context* ctx, int64_t bytes, void** dptr) {
+    return guarded(ctx, [&] {
+        need(ctx && dptr && bytes >= 0, PH_ERR_ARGUMENT, "bad arguments");
+        *dptr = nullptr;
+        hipError_t e = hipMalloc(dptr, bytes > 0 ? (size_t)bytes : 16);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            *dptr = nullptr;
+            throw Error{PH_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+        }
+        return PH_OK;
+    });
+}
+
+int ph_device_free(ph_context* ctx, void* dptr) {
+    return guarded(ctx, [&] {
+        if (dptr) PH_HIP(hipFree(dptr));
+        return PH_OK;
+    });
+}
+
+int ph_memcpy_htod(ph_context* ctx, void* dst, const void* src, int64_t bytes) {
+    return guarded(ctx, [&] {
+        need(bytes >= 0 && (bytes == 0 || (dst && src)), PH_ERR_ARGUMENT, "bad copy arguments");
+        PH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        PH_HIP(hipStreamSynchronize(ctx->stream));
+        return PH_OK;
+    });
+}
+
+int ph_memcpy_dtoh(ph_context* ctx, void* dst, const void* src, int64_t bytes) {
+    return guarded(ctx, [&] {
+        need(bytes >= 0 && (bytes == 0 || (dst && src)), PH_ERR_ARGUMENT, "bad copy arguments");
+        PH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        PH_HIP(hipStreamSynchronize(ctx->stream));
+        return PH_OK;
+    });
+}
+
+int ph_synchronize(ph_context* ctx) {
+    return guarded(ctx, [&] {
+        PH_HIP(hipStreamSynchronize(ctx->stream));
+        return PH_OK;
+    });
+}
+
+static int build_common(ph_context* ctx, int64_t n, const double* const* src, bool host_src, int32_t ncol,
+                        int32_t kernel, int32_t code_bits, int32_t precision, ph_lbvh** out) {
+    return guarded(ctx, [&] {
+        need(ctx != nullptr && out != nullptr, PH_ERR_ARGUMENT, "null context or output");
+        *out = nullptr;
+        need(n >= 1, PH_ERR_CAPACITY, "BinaryRadixTree: enc.codes must be non-empty (got n=0).");
+        need(n <= (int64_t)1073741824, PH_ERR_CAPACITY,
+             "BinaryRadixTree: n=" + std::to_string(n) + " exceeds the Int32 node-id capacity (n <= 1073741824).");
+        need(ncol >= 0, PH_ERR_ARGUMENT, "negative column count");
+        need(kernel >= 0 && kernel <= 5, PH_ERR_ARGUMENT, "unknown kernel");
+        need(code_bits == 30 || code_bits == 63, PH_ERR_ARGUMENT, "code_bits must be 30 (UInt32) or 63 (UInt64)");
+        need(precision == PH_PREC_FP64 || precision == PH_PREC_FP32, PH_ERR_ARGUMENT,
+             "tree precision must be PH_PREC_FP64 or PH_PREC_FP32");
+        for (int c = 0; c < 6 + ncol; ++c) need(src[c] != nullptr, PH_ERR_ARGUMENT, "null column pointer");
+        auto B = std::make_unique<ph_lbvh>();
+        B->ctx = ctx;
+        B->n = n;
+        B->ncol = ncol;
+        B->kernel = kernel;
+        B->code_bits = code_bits;
+        B->precision = precision;
+        BuildScratch W;
+        PH_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+        build_lbvh(B.get(), src, host_src, ctx->stream, W);
+        PH_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
+        PH_HIP(hipEventSynchronize(ctx->ev[3]));
+        ctx->stats.build_ms = elapsed(ctx, 2, 3);
+        *out = B.release();
+        return PH_OK;
+    });
+}
+
+int ph_lbvh_build(ph_context* ctx, int64_t n, const double* x, const double* y, const double* z, const double* m,
+                  const double* h, const double* rho, int32_t ncol, const double* const* cols, int32_t kernel,
+                  int32_t code_bits, int32_t precision, ph_lbvh** out) {
+    std::vector<const double*> src = {x, y, z, m, h, rho};
+    for (int c = 0; c < ncol; ++c) src.push_back(cols ? cols[c] : nullptr);
+    return build_common(ctx, n, src.data(), true, ncol, kernel, code_bits, precision, out);
+}
+
+int ph_lbvh_build_device(ph_context* ctx, int64_t n, const double* x, const double* y, const double* z,
+                         const double* m, const double* h, const double* rho, int32_t ncol, const double* const* cols,
+                         int32_t kernel, int32_t code_bits, int32_t precision, ph_lbvh** out) {
+    std::vector<const double*> src = {x, y, z, m, h, rho};
+    for (int c = 0; c < ncol; ++c) src.push_back(cols ? cols[c] : nullptr);
+    return build_common(ctx, n, src.data(), false, ncol, kernel, code_bits, precision, out);
+}
+
+int ph_lbvh_destroy(ph_lbvh* B) {
+    if (!B) return PH_OK;
+    (void)hipSetDevice(B->ctx->device);
+    (void)hipStreamSynchronize(B->ctx->stream);
+    delete B;
+    return PH_OK;
+}
+
+int ph_lbvh_info(const ph_lbvh* B, int64_t* n, int32_t* root, int32_t* ncol, int32_t* code_bits, int32_t* precision) {
+    if (!B) return PH_ERR_ARGUMENT;
+    if (n) *n = B->n;
+    if (root) *root = B->root;
+    if (ncol) *ncol = B->ncol;
+    if (code_bits) *code_bits = B->code_bits;
+    if (precision) *precision = B->precision;
+    return PH_OK;
+}
+
+int ph_lbvh_get(ph_lbvh* B, int32_t which, void* dst) {
+    if (!B) return PH_ERR_ARGUMENT;
+    ph_context* ctx = B->ctx;
+    return guarded(ctx, [&] {
+        need(dst != nullptr, PH_ERR_ARGUMENT, "destination is null");
+        const i64 n = B->n, ni = n - 1, tot = 2 * n - 1;
+        PH_HIP(hipStreamSynchronize(ctx->stream));
+        auto tree = [&](const DevBuf& b) { PH_HIP(hipMemcpy(dst, b.p, sizeof(i32) * tot, hipMemcpyDeviceToHost)); };
+        const bool f64 = B->precision == PH_PREC_FP64;
+        auto col = [&](int c, double* o) {
+            if (f64) {
+                PH_HIP(hipMemcpy(o, B->cols.as<double>() + (size_t)c * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+            } else {
+                std::vector<float> t(n);
+                PH_HIP(hipMemcpy(t.data(), B->cols.as<float>() + (size_t)c * n, sizeof(float) * n,
+                                 hipMemcpyDeviceToHost));
+                for (i64 i = 0; i < n; ++i) o[i] = t[i];
+            }
+        };
+        auto nodes = [&](auto tag, int field, double* o) {
+            using T = decltype(tag);
+            std::vector<NodeRec<T>> v((size_t)(ni > 0 ? ni : 0));
+            if (ni > 0)
+                PH_HIP(hipMemcpy(v.data(), B->nodes.p, sizeof(NodeRec<T>) * ni, hipMemcpyDeviceToHost));
+            for (i64 i = 0; i < ni; ++i) {
+                if (field == 0)
+                    for (int d = 0; d < 3; ++d) o[d * ni + i] = v[i].lo[d];
+                else if (field == 1)
+                    for (int d = 0; d < 3; ++d) o[d * ni + i] = v[i].hi[d];
+                else
+                    o[i] = v[i].scale;
+            }
+        };
+        switch (which) {
+        case PH_GET_ORDER: {
+            std::vector<u32> t(n);
+            PH_HIP(hipMemcpy(t.data(), B->order.p, sizeof(u32) * n, hipMemcpyDeviceToHost));
+            u64* o = static_cast<u64*>(dst);
+            for (i64 i = 0; i < n; ++i) o[i] = (u64)t[i] + 1;
+            break;
+        }
+        case PH_GET_CODES: PH_HIP(hipMemcpy(dst, B->codes.p, sizeof(u64) * n, hipMemcpyDeviceToHost)); break;
+        case PH_GET_LEFT: tree(B->left); break;
+        case PH_GET_RIGHT: tree(B->right); break;
+        case PH_GET_ESCAPE: tree(B->escape); break;
+        case PH_GET_PARENT: tree(B->parent); break;
+        case PH_GET_LEAF_COOR:
+            for (int d = 0; d < 3; ++d) col(d, static_cast<double*>(dst) + (size_t)d * n);
+            break;
+        case PH_GET_LEAF_SCALE: col(4, static_cast<double*>(dst)); break;
+        case PH_GET_NODE_MIN:
+        case PH_GET_NODE_MAX:
+        case PH_GET_NODE_SCALE: {
+            const int field = which - PH_GET_NODE_MIN;
+            if (f64) nodes(double(0), field, static_cast<double*>(dst));
+            else nodes(float(0), field, static_cast<double*>(dst));
+            break;
+        }
+        case PH_GET_COLUMNS:
+            for (int c = 0; c < 6 + B->ncol; ++c) col(c, static_cast<double*>(dst) + (size_t)c * n);
+            break;
+        default: throw Error{PH_ERR_ARGUMENT, "unknown selector"};
+        }
+        return PH_OK;
+    });
+}
+
+int ph_point_samples_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                            int64_t S, const double* px, const double* py, const double* pz, void* out) {
+    return guarded(ctx, [&] {
+        need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
+        need(strategy >= PH_GATHER && strategy <= PH_SYMMETRIC, PH_ERR_ARGUMENT, "unknown strategy");
+        need(S >= 0, PH_ERR_ARGUMENT, "negative sample count");
+        check_precision(B, precision);
+        check_catalog(B, cat);
+        need(S == 0 || (px && py && pz && out), PH_ERR_ARGUMENT, "null sample or output pointer");
+        PointList pl{px, py, pz};
+        run_chunks(ctx, cat, S, [&](const CatChunk& c, int L, u64* counter) {
+            launch_points(B, c, strategy, precision, &pl, nullptr, S, L, out, counter, ctx->stream);
+        });
+        return PH_OK;
+    });
+}
+
+int ph_point_samples(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                     int64_t S, const double* px, const double* py, const double* pz, double* out_host) {
+    return guarded(ctx, [&] {
+        need(S >= 0, PH_ERR_ARGUMENT, "negative sample count");
+        need(cat != nullptr, PH_ERR_ARGUMENT, "catalog is null");
+        need(S == 0 || (px && py && pz && out_host), PH_ERR_ARGUMENT, "null sample or output pointer");
+        const int L = catalog_L(cat);
+        DevBuf dp, dout;
+        dp.alloc(sizeof(double) * 3 * (size_t)(S > 0 ? S : 1));
+        dout.alloc(out_elem(precision) * (size_t)L * (size_t)(S > 0 ? S : 1));
+        double* d = dp.as<double>();
+        if (S > 0) {
+            PH_HIP(hipMemcpy(d, px, sizeof(double) * S, hipMemcpyHostToDevice));
+            PH_HIP(hipMemcpy(d + S, py, sizeof(double) * S, hipMemcpyHostToDevice));
+            PH_HIP(hipMemcpy(d + 2 * S, pz, sizeof(double) * S, hipMemcpyHostToDevice));
+        }
+        int rc = ph_point_samples_device(ctx, B, cat, strategy, precision, S, d, d + S, d + 2 * S, dout.p);
+        if (rc != PH_OK) return rc;
+        if (S > 0) download(ctx, dout.p, precision, (int64_t)L * S, out_host);
+        return PH_OK;
+    });
+}
+
+int ph_structured_grid_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                              const double* ax, int64_t nx, const double* ay, int64_t ny, const double* az, int64_t nz,
+                              void* out) {
+    return guarded(ctx, [&] {
+        need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
+        need(strategy >= PH_GATHER && strategy <= PH_SYMMETRIC, PH_ERR_ARGUMENT, "unknown strategy");
+        need(nx >= 0 && ny >= 0 && nz >= 0, PH_ERR_ARGUMENT, "negative grid size");
+        check_precision(B, precision);
+        check_catalog(B, cat);
+        const i64 S = nx * ny * nz;
+        need(S == 0 || (ax && ay && az && out), PH_ERR_ARGUMENT, "null axis or output pointer");
+        GridAxes ga{ax, ay, az, nx, ny, nz};
+        run_chunks(ctx, cat, S, [&](const CatChunk& c, int L, u64* counter) {
+            launch_points(B, c, strategy, precision, nullptr, &ga, S, L, out, counter, ctx->stream);
+        });
+        return PH_OK;
+    });
+}
+
+int ph_structured_grid(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                       const double* ax, int64_t nx, const double* ay, int64_t ny, const double* az, int64_t nz,
+                       double* out_host) {
+    return guarded(ctx, [&] {
+        need(cat != nullptr, PH_ERR_ARGUMENT, "catalog is null");
+        need(nx >= 0 && ny >= 0 && nz >= 0, PH_ERR_ARGUMENT, "negative grid size");
+        const i64 S = nx * ny * nz;
+        need(S == 0 || (ax && ay && az && out_host), PH_ERR_ARGUMENT, "null axis or output pointer");
+        const int L = catalog_L(cat);
+        DevBuf da, dout;
+        da.alloc(sizeof(double) * (size_t)(nx + ny + nz + 1));
+        dout.alloc(out_elem(precision) * (size_t)L * (size_t)(S > 0 ? S : 1));
+        double* d = da.as<double>();
+        if (nx) PH_HIP(hipMemcpy(d, ax, sizeof(double) * nx, hipMemcpyHostToDevice));
+        if (ny) PH_HIP(hipMemcpy(d + nx, ay, sizeof(double) * ny, hipMemcpyHostToDevice));
+        if (nz) PH_HIP(hipMemcpy(d + nx + ny, az, sizeof(double) * nz, hipMemcpyHostToDevice));
+        int rc = ph_structured_grid_device(ctx, B, cat, strategy, precision, d, nx, d + nx, ny, d + nx + ny, nz, dout.p);
+        if (rc != PH_OK) return rc;
+        if (S > 0) download(ctx, dout.p, precision, (int64_t)L * S, out_host);
+        return PH_OK;
+    });
+}
+
+static void check_line_catalog(const ph_lbvh* B, const ph_catalog* cat, int32_t strategy) {
+    need(strategy == PH_SCATTER, PH_ERR_STRATEGY,
+         "LineSamples_interpolation only supports itpScatter. Line-integrated samples do not have a well-defined "
+         "query smoothing length ha, so itpGather and itpSymmetric are not supported.");
+    check_catalog(B, cat);
+    need(cat->n_grad == 0 && cat->n_div == 0 && cat->n_curl == 0, PH_ERR_ARGUMENT,
+         "line-integrated interpolation supports scalar quantities only");
+}
+
+int ph_line_samples_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                           int64_t S, const double* ox, const double* oy, const double* oz, const double* dx,
+                           const double* dy, const double* dz, void* out) {
+    return guarded(ctx, [&] {
+        need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
+        need(S >= 0, PH_ERR_ARGUMENT, "negative sample count");
+        check_line_catalog(B, cat, strategy);
+        check_precision(B, precision);
+        need(S == 0 || (ox && oy && oz && dx && dy && dz && out), PH_ERR_ARGUMENT, "null ray or output pointer");
+        LineList ll{ox, oy, oz, dx, dy, dz};
+        const void* lut = precision == PH_PREC_FP64 ? ctx->lut64.p : ctx->lut32.p;
+        run_chunks(ctx, cat, S, [&](const CatChunk& c, int L, u64* counter) {
+            launch_lines(B, c, strategy, precision, &ll, nullptr, S, L, out, counter, lut, ctx->stream);
+        });
+        return PH_OK;
+    });
+}
+
+int ph_line_samples(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision, int64_t S,
+                    const double* ox, const double* oy, const double* oz, const double* dx, const double* dy,
+                    const double* dz, double* out_host) {
+    return guarded(ctx, [&] {
+        need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
+        need(S >= 0, PH_ERR_ARGUMENT, "negative sample count");
+        check_line_catalog(B, cat, strategy);
+        need(S == 0 || (ox && oy && oz && dx && dy && dz && out_host), PH_ERR_ARGUMENT, "null ray or output pointer");
+        const int L = catalog_L(cat);
+        DevBuf dr, dout;
+        dr.alloc(sizeof(double) * 6 * (size_t)(S > 0 ? S : 1));
+        dout.alloc(out_elem(precision) * (size_t)L * (size_t)(S > 0 ? S : 1));
+        double* d = dr.as<double>();
+        const double* srcs[6] = {ox, oy, oz, dx, dy, dz};
+        for (int k = 0; k < 6 && S > 0; ++k)
+            PH_HIP(hipMemcpy(d + (size_t)k * S, srcs[k], sizeof(double) * S, hipMemcpyHostToDevice));
+        int rc = ph_line_samples_device(ctx, B, cat, strategy, precision, S, d, d + S, d + 2 * S, d + 3 * S, d + 4 * S,
+                                        d + 5 * S, dout.p);
+        if (rc != PH_OK) return rc;
+        if (S > 0) download(ctx, dout.p, precision, (int64_t)L * S, out_host);
+        return PH_OK;
+    });
+}
+
+int ph_line_grid_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                        const double* origin3, const double* u3, const double* v3, const double* dir3,
+                        const double* ax, int64_t nx, const double* ay, int64_t ny, void* out) {
+    return guarded(ctx, [&] {
+        need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
+        need(nx >= 0 && ny >= 0, PH_ERR_ARGUMENT, "negative image size");
+        check_line_catalog(B, cat, strategy);
+        check_precision(B, precision);
+        need(origin3 && u3 && v3 && dir3, PH_ERR_ARGUMENT, "null frame vector");
+        const i64 S = nx * ny;
+        need(S == 0 || (ax && ay && out), PH_ERR_ARGUMENT, "null axis or output pointer");
+        LineGrid lg{};
+        for (int d = 0; d < 3; ++d) {
+            lg.o[d] = origin3[d];
+            lg.u[d] = u3[d];
+            lg.v[d] = v3[d];
+            lg.d[d] = dir3[d];
+        }
+        lg.ax = ax;
+        lg.ay = ay;
+        lg.nx = nx;
+        lg.ny = ny;
+        const void* lut = precision == PH_PREC_FP64 ? ctx->lut64.p : ctx->lut32.p;
+        run_chunks(ctx, cat, S, [&](const CatChunk& c, int L, u64* counter) {
+            launch_lines(B, c, strategy, precision, nullptr, &lg, S, L, out, counter, lut, ctx->stream);
+        });
+        return PH_OK;
+    });
+}
+
+int ph_nearest_h(ph_context* ctx, ph_lbvh* B, int64_t S, const double* px, const double* py, const double* pz,
+                 double* ha) {
+    return guarded(ctx, [&] {
+        need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
+        need(S >= 0 && (S == 0 || (px && py && pz && ha)), PH_ERR_ARGUMENT, "bad arguments");
+        if (S == 0) return PH_OK;
+        DevBuf d;
+        d.alloc(sizeof(double) * 4 * (size_t)S);
+        double* p = d.as<double>();
+        PH_HIP(hipMemcpy(p, px, sizeof(double) * S, hipMemcpyHostToDevice));
+        PH_HIP(hipMemcpy(p + S, py, sizeof(double) * S, hipMemcpyHostToDevice));
+        PH_HIP(hipMemcpy(p + 2 * S, pz, sizeof(double) * S, hipMemcpyHostToDevice));
+        PointList pl{p, p + S, p + 2 * S};
+        launch_nearest_h(B, pl, S, p + 3 * S, ctx->stream);
+        PH_HIP(hipStreamSynchronize(ctx->stream));
+        PH_HIP(hipMemcpy(ha, p + 3 * S, sizeof(double) * S, hipMemcpyDeviceToHost));
+        return PH_OK;
+    });
+}
+
+int ph_line_table(int32_t kernel, double* inorm) {
+    if (kernel < 0 || kernel > 5 || !inorm) return fail(nullptr, PH_ERR_ARGUMENT, "bad arguments");
+    std::memcpy(inorm, ph_lut_inorm[kernel], sizeof(double) * PH_LUT_N);
+    return PH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,382 @@
+// ph_build.hip -- LinearBVH! on the GPU.
+//
+// Pipeline (one HIP stream, all device-resident):
+//   1. per-axis min/max                    LBVH/MortonEncoding/toolbox.jl:281-294
+//   2. quantise + 3D interleave            toolbox.jl:311-336, 38-55, 113-118
+//   3. stable LSD radix sort (code, index) toolbox.jl:20-26 (sortperm, ties by index)
+//   4. permute every column                KI/interpolation_setup/InterpolationInput.jl:209-217
+//   5. duplicate rewrite as a prefix max   toolbox.jl:27-31:  c'[i] = i + max_{j<=i}(c[j] - j)
+//   6. Karras range/split + parents        LBVH/BinaryRadixTree/toolbox.jl:14-117
+//   7. adjacent LCP, internal escapes      BinaryRadixTree.jl:111-121, toolbox.jl:119-134
+//   8. leaf escapes                        BinaryRadixTree.jl:124-126, toolbox.jl:136-156
+//   9. bottom-up AABB / max-h refit with   LBVH/LinearBVH/LinearBVH.jl:77-91, toolbox.jl:130-188
+//      agent-scope acq_rel "second arrival" counters, then a visit-count check.
+// Tree arrays keep the reference's unified node IDs so they compare with memcmp.
+#include "ph_build.h"
+#include "ph_primitives.h"
+
+namespace ph {
+namespace {
+
+template <class TI> struct CodeTraits;
+template <> struct CodeTraits<u32> {
+    static constexpr int axis_bits = 10;
+    static constexpr int sort_bits = 32;
+    __device__ static u32 expand(u32 x) {
+        x = (x | (x << 16)) & 0x30000ffu;
+        x = (x | (x << 8)) & 0x300f00fu;
+        x = (x | (x << 4)) & 0x30c30c3u;
+        x = (x | (x << 2)) & 0x9249249u;
+        return x;
+    }
+    __device__ static int clz(u32 v) { return v ? __clz((int)v) : 32; }
+};
+template <> struct CodeTraits<u64> {
+    static constexpr int axis_bits = 21;
+    static constexpr int sort_bits = 64;
+    __device__ static u64 expand(u64 x) {
+        x = (x | (x << 32)) & 0x1f00000000ffffull;
+        x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+        x = (x | (x << 8)) & 0x100f00f00f00f00full;
+        x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+        x = (x | (x << 2)) & 0x1249249249249249ull;
+        return x;
+    }
+    __device__ static int clz(u64 v) { return v ? __clzll((long long)v) : 64; }
+};
+
+__device__ inline int clz64(u64 v) { return v ? __clzll((long long)v) : 64; }
+
+// Quantise one coordinate: floor(scale * ((v - vmin) * inv(vmax - vmin))), 0.5 when flat.
+template <class T, class TI> __device__ inline TI quantise(T v, T vmin, T vmax) {
+    const T scale = T((1ull << CodeTraits<TI>::axis_bits) - 1);  // exp2(b) - 1, exact in T
+    T dv = vmax - vmin;
+    T f;
+    if (dv == T(0)) {
+        f = T(0.5);
+    } else {
+        T inv = T(1) / dv;
+        f = (v - vmin) * inv;
+    }
+    return (TI)floor(scale * f);
+}
+
+template <class T, class TI>
+__global__ void __launch_bounds__(kBlock) k_encode(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z, i64 n,
+                                           const T* __restrict__ mm6, TI* __restrict__ codes, u32* __restrict__ idx) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    TI ix = quantise<T, TI>(x[i], mm6[0], mm6[3]);
+    TI iy = quantise<T, TI>(y[i], mm6[1], mm6[4]);
+    TI iz = quantise<T, TI>(z[i], mm6[2], mm6[5]);
+    codes[i] = (CodeTraits<TI>::expand(ix) << 2) | (CodeTraits<TI>::expand(iy) << 1) | CodeTraits<TI>::expand(iz);
+    idx[i] = (u32)i;
+}
+
+// Gather every column into Morton order: dst[c][i] = src[c][idx[i]].
+template <class TS, class TD>
+__global__ void __launch_bounds__(kBlock) k_permute(const TS* const* __restrict__ src, TD* __restrict__ dst, int ncols, i64 n,
+                                            const u32* __restrict__ idx) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const u32 s = idx[i];
+    for (int c = 0; c < ncols; ++c) dst[(i64)c * n + i] = (TD)src[c][s];
+}
+
+// Dedup as a prefix max: key_i = c_i + 2^31 - i (never negative for i < 2^31).
+template <class TI> __global__ void k_dedup_keys(const TI* codes, u64* keys, i64 n) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = (u64)codes[i] + (1ull << 31) - (u64)i;
+}
+template <class TI> __global__ void k_dedup_apply(TI* codes, const u64* keys, u64* codes64, i64 n) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const u64 c = keys[i] - (1ull << 31) + (u64)i;
+    codes[i] = (TI)c;
+    codes64[i] = c;
+}
+
+// delta(i, j) on 1-based indices; -1 outside [1, n].
+template <class TI> __device__ inline i64 lcp(const TI* __restrict__ c, i64 n, i64 i, i64 j) {
+    if (j < 1 || j > n) return -1;
+    const TI a = c[i - 1], b = c[j - 1];
+    if (a == b) return clz64((u64)i ^ (u64)j);
+    return CodeTraits<TI>::clz(a ^ b);
+}
+
+// _build_child! for every internal node i (toolbox.jl:83-117, with _find_range
+// :22-53 and _split_position :55-76).
+template <class TI>
+__global__ void __launch_bounds__(kBlock) k_karras(const TI* __restrict__ c, i64 n, i32* __restrict__ left, i32* __restrict__ right,
+                                           i32* __restrict__ parent, i32* __restrict__ range) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
+    if (i > n - 1) return;
+    // _range_direction
+    const i64 dl = lcp(c, n, i, i - 1), dr = lcp(c, n, i, i + 1);
+    const i64 sd = dr - dl;
+    const i64 d = sd >= 0 ? 1 : -1;  // sign, with 0 treated as +1
+    const i64 dmin = lcp(c, n, i, i - d);
+    i64 lmax = 2;
+    while (true) {
+        const i64 j = i + lmax * d;
+        if (j < 1 || j > n) break;
+        if (lcp(c, n, i, j) <= dmin) break;
+        lmax *= 2;
+    }
+    i64 l = 0;
+    for (i64 t = lmax >> 1; t > 0; t >>= 1) {
+        const i64 j = i + (l + t) * d;
+        if (j >= 1 && j <= n && lcp(c, n, i, j) > dmin) l += t;
+    }
+    const i64 jj = i + l * d;
+    const i64 lo = i < jj ? i : jj, hi = i < jj ? jj : i;
+    i64 s;
+    if (c[lo - 1] == c[hi - 1]) {
+        s = (lo + hi) >> 1;
+        s = s < lo ? lo : (s > hi - 1 ? hi - 1 : s);
+    } else {
+        const i64 pfl = lcp(c, n, lo, hi);
+        s = lo;
+        i64 step = hi - lo;
+        while (step > 1) {
+            step = (step + 1) >> 1;
+            const i64 ns = s + step;
+            if (ns < hi && lcp(c, n, lo, ns) > pfl) s = ns;
+        }
+    }
+    const i64 off = n - 1;
+    range[2 * (i - 1)] = (i32)lo;
+    range[2 * (i - 1) + 1] = (i32)hi;
+    i64 idx = (s == lo) ? lo + off : s;
+    left[i - 1] = (i32)idx;
+    parent[idx - 1] = (i32)i;
+    idx = (s + 1 == hi) ? hi + off : s + 1;
+    right[i - 1] = (i32)idx;
+    parent[idx - 1] = (i32)i;
+}
+
+// adj[i] = delta(i, i+1) for i < n, adj[n] = typemax(Int32).
+template <class TI> __global__ void k_adjacent(const TI* c, i64 n, i32* adj) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
+    if (i > n) return;
+    adj[i - 1] = (i < n) ? (i32)lcp(c, n, i, i + 1) : 0x7fffffff;
+}
+
+__global__ void k_internal_escape(const i32* __restrict__ range, const i32* __restrict__ adj, i64 n, i32* __restrict__ escape) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
+    if (i > n - 1) return;
+    const i64 off = n - 1, hi = range[2 * (i - 1) + 1];
+    i32 e;
+    if (hi == n) e = 0;
+    else if (hi == n - 1) e = (i32)(off + n);
+    else {
+        const i64 nx = hi + 1;
+        e = (adj[nx - 1] < adj[nx - 2]) ? (i32)(off + nx) : (i32)nx;
+    }
+    escape[i - 1] = e;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_leaf_escape(const i32* __restrict__ parent, const i32* __restrict__ left, const i32* __restrict__ right,
+                                                i64 n, i32* __restrict__ escape, i32* __restrict__ leaf_escape,
+                                                NodeRec<T>* __restrict__ nodes) {
+    const i64 k = (i64)blockIdx.x * kBlock + threadIdx.x + 1;  // leaf 1..n, also internal node k < n
+    if (k > n) return;
+    const i64 id = n - 1 + k;
+    const i32 p = parent[id - 1];
+    i32 e;
+    if (p == 0) e = 0;
+    else if (left[p - 1] == (i32)id) e = right[p - 1];
+    else e = escape[p - 1];
+    escape[id - 1] = e;
+    leaf_escape[k - 1] = e;
+    if (k < n) {
+        nodes[k - 1].left = left[k - 1];
+        nodes[k - 1].escape = escape[k - 1];
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_leaves(const T* __restrict__ cols, i64 n, LeafRec<T>* __restrict__ leaves) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    LeafRec<T> L;
+    L.x = cols[i];
+    L.y = cols[n + i];
+    L.z = cols[2 * n + i];
+    L.h = cols[4 * n + i];
+    leaves[i] = L;
+}
+
+__global__ void k_fill_u32(u32* p, i64 n, u32 v) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Karras bottom-up refit: the second thread to reach a node combines its children.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_refit(const LeafRec<T>* __restrict__ leaves, const i32* __restrict__ parent,
+                                          const i32* __restrict__ left, const i32* __restrict__ right, i64 n,
+                                          NodeRec<T>* nodes, u32* visited) {
+    const i64 k = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
+    if (k > n) return;
+    i32 p = parent[n - 1 + k - 1];
+    for (i64 depth = n; p != 0 && depth > 0; --depth) {  // a tree is at most n-1 levels deep
+        const u32 old = __hip_atomic_fetch_add(&visited[p - 1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old != 1u) break;  // first arrival: the sibling subtree is not ready yet
+        const i32 l = left[p - 1], r = right[p - 1];
+        T lmin[3], lmax[3], rmin[3], rmax[3], hl, hr;
+        if (l >= (i32)n) {
+            const LeafRec<T> L = leaves[l - n];
+            lmin[0] = lmax[0] = L.x; lmin[1] = lmax[1] = L.y; lmin[2] = lmax[2] = L.z; hl = L.h;
+        } else {
+            const NodeRec<T>* N = &nodes[l - 1];
+            for (int d = 0; d < 3; ++d) { lmin[d] = N->lo[d]; lmax[d] = N->hi[d]; }
+            hl = N->scale;
+        }
+        if (r >= (i32)n) {
+            const LeafRec<T> R = leaves[r - n];
+            rmin[0] = rmax[0] = R.x; rmin[1] = rmax[1] = R.y; rmin[2] = rmax[2] = R.z; hr = R.h;
+        } else {
+            const NodeRec<T>* N = &nodes[r - 1];
+            for (int d = 0; d < 3; ++d) { rmin[d] = N->lo[d]; rmax[d] = N->hi[d]; }
+            hr = N->scale;
+        }
+        NodeRec<T>* P = &nodes[p - 1];
+        for (int d = 0; d < 3; ++d) {
+            P->lo[d] = (lmin[d] < rmin[d]) ? lmin[d] : rmin[d];
+            P->hi[d] = (lmax[d] > rmax[d]) ? lmax[d] : rmax[d];
+        }
+        P->scale = (hl > hr) ? hl : hr;
+        p = parent[p - 1];
+    }
+}
+
+__global__ void k_check_visited(const u32* visited, i64 m, u32* bad) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
+    if (i < m && visited[i] < 2u) atomicAdd(bad, 1u);
+}
+
+__global__ void k_d2f(const double* __restrict__ s, float* __restrict__ d, i64 m) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
+    if (i < m) d[i] = (float)s[i];
+}
+
+template <class T, class TI>
+void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipStream_t s, BuildScratch& W) {
+    const i64 n = B->n;
+    const int ncols = 6 + B->ncol;
+    const int g = grid_for(n);
+    auto align = [](size_t b) { return (b + 255) & ~size_t(255); };
+
+    // column pointer table on the device
+    W.ptrs.alloc(sizeof(void*) * ncols);
+    PH_HIP(hipMemcpyAsync(W.ptrs.p, src_cols, sizeof(void*) * ncols, hipMemcpyHostToDevice, s));
+
+    // 1-2: min/max, encode
+    W.mm.alloc(sizeof(T) * 8);
+    minmax3<T>(src_cols[0], src_cols[1], src_cols[2], n, W.mm.as<T>(), W.tmp, s);
+    W.keys.alloc(align(sizeof(TI) * n));
+    B->order.alloc(sizeof(u32) * n);
+    k_encode<T, TI><<<g, kBlock, 0, s>>>(src_cols[0], src_cols[1], src_cols[2], n, W.mm.as<T>(), W.keys.as<TI>(),
+                                         B->order.as<u32>());
+    PH_HIP(hipGetLastError());
+    // 3: stable sort by code
+    radix_sort_pairs(W.keys.as<TI>(), B->order.as<u32>(), n, CodeTraits<TI>::sort_bits, W.tmp, s);
+    // 4: permute the columns
+    B->cols.alloc(sizeof(T) * (size_t)ncols * n);
+    k_permute<T, T><<<g, kBlock, 0, s>>>(W.ptrs.as<const T*>(), B->cols.as<T>(), ncols, n, B->order.as<u32>());
+    PH_HIP(hipGetLastError());
+    // 5: duplicate rewrite
+    W.dk.alloc(sizeof(u64) * n);
+    B->codes.alloc(sizeof(u64) * n);
+    k_dedup_keys<TI><<<g, kBlock, 0, s>>>(W.keys.as<TI>(), W.dk.as<u64>(), n);
+    PH_HIP(hipGetLastError());
+    inclusive_max_u64(W.dk.as<u64>(), n, W.tmp, s);
+    k_dedup_apply<TI><<<g, kBlock, 0, s>>>(W.keys.as<TI>(), W.dk.as<u64>(), B->codes.as<u64>(), n);
+    PH_HIP(hipGetLastError());
+
+    // 6-8: topology
+    const i64 tot = 2 * n - 1;
+    for (DevBuf* b : {&B->left, &B->right, &B->escape, &B->parent}) {
+        b->alloc(sizeof(i32) * tot);
+        PH_HIP(hipMemsetAsync(b->p, 0, sizeof(i32) * tot, s));
+    }
+    B->leaf_escape.alloc(sizeof(i32) * n);
+    B->leaves.alloc(sizeof(LeafRec<T>) * n);
+    B->nodes.alloc(sizeof(NodeRec<T>) * (size_t)(n > 1 ? n - 1 : 1));
+    B->range.alloc(sizeof(i32) * 2 * (size_t)(n > 1 ? n - 1 : 1));
+    B->root = n >= 2 ? 1 : 0;
+    k_leaves<T><<<g, kBlock, 0, s>>>(B->cols.as<T>(), n, B->leaves.as<LeafRec<T>>());
+    PH_HIP(hipGetLastError());
+    if (n >= 2) {
+        const int gi = grid_for(n - 1);
+        k_karras<TI><<<gi, kBlock, 0, s>>>(W.keys.as<TI>(), n, B->left.as<i32>(), B->right.as<i32>(),
+                                           B->parent.as<i32>(), B->range.as<i32>());
+        PH_HIP(hipGetLastError());
+        W.adj.alloc(sizeof(i32) * n);
+        k_adjacent<TI><<<g, kBlock, 0, s>>>(W.keys.as<TI>(), n, W.adj.as<i32>());
+        PH_HIP(hipGetLastError());
+        k_internal_escape<<<gi, kBlock, 0, s>>>(B->range.as<i32>(), W.adj.as<i32>(), n, B->escape.as<i32>());
+        PH_HIP(hipGetLastError());
+    }
+    k_leaf_escape<T><<<g, kBlock, 0, s>>>(B->parent.as<i32>(), B->left.as<i32>(), B->right.as<i32>(), n,
+                                          B->escape.as<i32>(), B->leaf_escape.as<i32>(), B->nodes.as<NodeRec<T>>());
+    PH_HIP(hipGetLastError());
+    // 9: refit
+    if (n >= 2) {
+        W.visited.alloc(sizeof(u32) * (n - 1) + sizeof(u32) * 64);
+        u32* visited = W.visited.as<u32>();
+        u32* bad = visited + (n - 1);
+        k_fill_u32<<<grid_for(n - 1 + 1), kBlock, 0, s>>>(visited, n - 1 + 1, 0u);
+        PH_HIP(hipGetLastError());
+        k_refit<T><<<g, kBlock, 0, s>>>(B->leaves.as<LeafRec<T>>(), B->parent.as<i32>(), B->left.as<i32>(),
+                                        B->right.as<i32>(), n, B->nodes.as<NodeRec<T>>(), visited);
+        PH_HIP(hipGetLastError());
+        k_check_visited<<<grid_for(n - 1), kBlock, 0, s>>>(visited, n - 1, bad);
+        PH_HIP(hipGetLastError());
+        u32 hbad = 0;
+        PH_HIP(hipMemcpyAsync(&hbad, bad, sizeof(u32), hipMemcpyDeviceToHost, s));
+        PH_HIP(hipStreamSynchronize(s));
+        if (hbad != 0)
+            throw Error{PH_ERR_INTERNAL,
+                        "LinearBVH refit: " + std::to_string(hbad) + " internal nodes were not reached twice"};
+    }
+}
+
+}  // namespace
+
+void build_lbvh(ph_lbvh* B, const double* const* src64, bool host_src, hipStream_t s, BuildScratch& W) {
+    const i64 n = B->n;
+    const int ncols = 6 + B->ncol;
+    std::vector<const double*> dptr(ncols);
+    if (host_src) {
+        // stage host columns on the device (float64, original order)
+        W.stage.alloc(sizeof(double) * (size_t)ncols * n);
+        for (int c = 0; c < ncols; ++c) {
+            double* d = W.stage.as<double>() + (size_t)c * n;
+            PH_HIP(hipMemcpyAsync(d, src64[c], sizeof(double) * n, hipMemcpyHostToDevice, s));
+            dptr[c] = d;
+        }
+    } else {
+        for (int c = 0; c < ncols; ++c) dptr[c] = src64[c];
+    }
+    if (B->precision == PH_PREC_FP32) {
+        W.stage32.alloc(sizeof(float) * (size_t)ncols * n);
+        std::vector<const float*> fptr(ncols);
+        for (int c = 0; c < ncols; ++c) {
+            float* d = W.stage32.as<float>() + (size_t)c * n;
+            k_d2f<<<grid_for(n), kBlock, 0, s>>>(dptr[c], d, n);
+            PH_HIP(hipGetLastError());
+            fptr[c] = d;
+        }
+        if (B->code_bits == 30) build_typed<float, u32>(B, fptr.data(), s, W);
+        else build_typed<float, u64>(B, fptr.data(), s, W);
+    } else {
+        if (B->code_bits == 30) build_typed<double, u32>(B, dptr.data(), s, W);
+        else build_typed<double, u64>(B, dptr.data(), s, W);
+    }
+}
+
+}  // namespace ph

@@ -1,0 +1,120 @@
+// ph_common.h -- internal definitions shared by the HIP sources of libpartia_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/partia_hip.h"
+
+#define PH_LUT_N_ENTRIES 4096  // entries of each line-integrated kernel table
+
+namespace ph {
+
+using i32 = int32_t;
+using i64 = int64_t;
+using u32 = uint32_t;
+using u64 = uint64_t;
+
+constexpr int kWave = 64;        // CDNA wavefront width
+constexpr int kBlock = 256;      // 4 waves per workgroup
+constexpr int kNumXcd = 8;       // MI355X: 8 accelerator complex dies, 32 CUs each
+
+// Status with message, propagated to ph_last_error.
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define PH_HIP(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t _e = (expr);                                                                        \
+        if (_e != hipSuccess) {                                                                        \
+            if (_e == hipErrorOutOfMemory || _e == hipErrorMemoryAllocation)                           \
+                throw ::ph::Error{PH_ERR_OUT_OF_MEMORY, std::string("out of device memory: ") + #expr}; \
+            throw ::ph::Error{PH_ERR_HIP, std::string(hipGetErrorString(_e)) + " at " + #expr};          \
+        }                                                                                              \
+    } while (0)
+
+// Device buffer owned by the library.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b) {
+        if (b <= bytes && p) return;
+        release();
+        if (b == 0) b = 16;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            throw Error{PH_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(b) + " bytes failed"};
+        }
+        bytes = b;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Packed internal-node record for traversal: bounding box, subtree max scale,
+// left child and escape (unified IDs).  64 bytes for double, 48 for float.
+template <class T> struct alignas(16) NodeRec {
+    T lo[3];
+    T hi[3];
+    T scale;
+    i32 left;
+    i32 escape;
+};
+// Leaf record: position and smoothing length of the particle in Morton order.
+template <class T> struct alignas(16) LeafRec {
+    T x, y, z, h;
+};
+
+}  // namespace ph
+
+// Context: one device, one stream.
+struct ph_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    ph_stats stats{};
+    int count_neighbours = 0;
+    ph::DevBuf lut64, lut32;   // line-integrated tables [6][4096]
+    ph::DevBuf counters;       // small device counters (accepted neighbours, rechecks, flags)
+    int num_cu = 256;
+};
+
+// Device-resident InterpolationInput after LinearBVH!.
+struct ph_lbvh {
+    ph_context* ctx = nullptr;
+    int64_t n = 0;
+    int32_t ncol = 0;
+    int32_t kernel = 0;
+    int32_t code_bits = 63;
+    int32_t precision = PH_PREC_FP64;  // PH_PREC_FP64 or PH_PREC_FP32 columns
+    int32_t root = 0;
+    // Morton order columns: x,y,z,m,h,rho then ncol quantity columns, (6+ncol) x n
+    ph::DevBuf cols;
+    ph::DevBuf codes;   // u64 [n]
+    ph::DevBuf order;   // u32 [n], 0-based source index
+    ph::DevBuf left, right, escape, parent;  // i32 [2n-1]
+    ph::DevBuf range;   // i32 [2*(n-1)]: lo, hi (1-based leaf indices) per internal node
+    ph::DevBuf nodes;   // NodeRec<T> [n-1]
+    ph::DevBuf leaves;  // LeafRec<T> [n]
+    ph::DevBuf leaf_escape;  // i32 [n]
+    // FP32-compensated view, built on first use from the float64 columns.
+    bool c32_ready = false;
+    ph::DevBuf c32_nodes, c32_leaves, c32_attr;
+};

@@ -1,0 +1,57 @@
+// ph_interp.h -- launchers of the interpolation kernels.
+#pragma once
+
+#include "ph_common.h"
+
+namespace ph {
+
+// One catalog chunk as the kernels see it (capacities fixed at compile time).
+constexpr int kMaxS = 4, kMaxG = 1, kMaxD = 1, kMaxC = 1;
+struct CatChunk {
+    i32 ns, ng, nd, nc;
+    i32 sslot[kMaxS];
+    i32 sflag[kMaxS];
+    i32 sout[kMaxS];  // output index of each entry
+    i32 gslot[kMaxG];
+    i32 gout[kMaxG];
+    i32 dslot[3 * kMaxD];
+    i32 dout[kMaxD];
+    i32 cslot[3 * kMaxC];
+    i32 cout[kMaxC];
+};
+
+// Splits a ph_catalog into chunks; returns L.
+int split_catalog(const ph_catalog* cat, std::vector<CatChunk>& chunks);
+
+// Sample sources.
+struct PointList {
+    const double* x;
+    const double* y;
+    const double* z;
+};
+struct GridAxes {
+    const double* ax;
+    const double* ay;
+    const double* az;
+    i64 nx, ny, nz;
+};
+struct LineList {
+    const double *ox, *oy, *oz, *dx, *dy, *dz;
+};
+struct LineGrid {
+    double o[3], u[3], v[3], d[3];
+    const double* ax;
+    const double* ay;
+    i64 nx, ny;
+};
+
+// Point samples (list or grid); out is L x S of T (double for FP64, float for FP32 / FP32C).
+void launch_points(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl,
+                   const GridAxes* ga, i64 S, int L, void* out, u64* counters, hipStream_t s);
+void launch_lines(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const LineList* ll, const LineGrid* lg,
+                  i64 S, int L, void* out, u64* counters, const void* lut, hipStream_t s);
+void launch_nearest_h(ph_lbvh* B, const PointList& pl, i64 S, double* ha, hipStream_t s);
+// FP32-compensated view of B (nodes rounded outward, leaf offsets, attribute records).
+void prepare_c32(ph_lbvh* B, hipStream_t s);
+
+}  // namespace ph

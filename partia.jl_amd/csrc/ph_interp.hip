@@ -1,0 +1,660 @@
+// ph_interp.hip -- point / grid / line interpolation in the reference's own
+// precision (Float64, or Float32 for a Float32 input).
+//
+// One lane per sample.  Each lane walks the tree with the stackless escape-rope
+// DFS of LBVH/LinearBVH/LinearBVHTraversalMacros.jl (26-108 gather, 135-236
+// scatter, 268-371 symmetric; 408-796 the line variants), so accepted leaves
+// arrive in DFS preorder (increasing Morton index), exactly the order in which
+// the reference accumulates; the per-neighbour terms and the epilogue follow
+// KI/single_point_interpolation/kernels/general_interpolation.jl and
+// KI/line_integrated_interpolation/kernels/line_integrated_scalar_interpolation.jl
+// operation by operation (no FMA contraction), so the float64 results are those
+// of the reference algorithm, not an approximation of them.
+//
+// Grid samples are generated on the device (Grids/grids/StructuredGrid.jl:140-155
+// column-major flattening) and each wavefront owns a 4x4x4 block of grid points,
+// so the 64 lanes of a wave walk nearly the same nodes; workgroups are remapped
+// so that each of the 8 XCDs (separate L2 caches) owns a contiguous run of tiles.
+#include "ph_interp.h"
+#include "ph_sph.h"
+
+namespace ph {
+
+int split_catalog(const ph_catalog* cat, std::vector<CatChunk>& chunks) {
+    chunks.clear();
+    const int N = cat->n_scalar, G = cat->n_grad, D = cat->n_div, C = cat->n_curl;
+    const int L = N + 3 * G + D + 3 * C;
+    int si = 0, gi = 0, di = 0, ci = 0;
+    while (si < N || gi < G || di < D || ci < C || chunks.empty()) {
+        CatChunk c{};
+        for (; c.ns < kMaxS && si < N; ++si, ++c.ns) {
+            c.sslot[c.ns] = cat->scalar_slots[si];
+            c.sflag[c.ns] = cat->scalar_shepard[si] ? 1 : 0;
+            c.sout[c.ns] = si;
+        }
+        for (; c.ng < kMaxG && gi < G; ++gi, ++c.ng) {
+            c.gslot[c.ng] = cat->grad_slots[gi];
+            c.gout[c.ng] = N + 3 * gi;
+        }
+        for (; c.nd < kMaxD && di < D; ++di, ++c.nd) {
+            for (int d = 0; d < 3; ++d) c.dslot[3 * c.nd + d] = cat->div_slots[3 * di + d];
+            c.dout[c.nd] = N + 3 * G + di;
+        }
+        for (; c.nc < kMaxC && ci < C; ++ci, ++c.nc) {
+            for (int d = 0; d < 3; ++d) c.cslot[3 * c.nc + d] = cat->curl_slots[3 * ci + d];
+            c.cout[c.nc] = N + 3 * G + D + 3 * ci;
+        }
+        chunks.push_back(c);
+        if (L == 0) break;
+    }
+    return L;
+}
+
+namespace {
+
+template <class T> struct TreeView {
+    i64 n;
+    i32 root;
+    const NodeRec<T>* __restrict__ nodes;
+    const LeafRec<T>* __restrict__ leaves;
+    const i32* __restrict__ leaf_escape;
+};
+
+template <class T> TreeView<T> view_of(const ph_lbvh* B) {
+    return TreeView<T>{B->n, B->root, B->nodes.as<NodeRec<T>>(), B->leaves.as<LeafRec<T>>(),
+                       B->leaf_escape.as<i32>()};
+}
+
+// XCD-aware workgroup index: hardware dispatches workgroup b to XCD b % 8; give
+// each XCD a contiguous range of logical blocks so neighbouring tiles share an L2.
+__device__ inline i64 xcd_block() {
+    const i64 nb = gridDim.x, b = blockIdx.x;
+    const i64 q = nb / kNumXcd, r = nb % kNumXcd;
+    const i64 x = b % kNumXcd, pos = b / kNumXcd;
+    return x * q + (x < r ? x : r) + pos;
+}
+
+// ---- geometry (LinearBVHTraversalMacros.jl:800-953) ----
+template <class T> __device__ inline T d2_point(T px, T py, T pz, const LeafRec<T>& L) {
+    T dx = px - L.x, dy = py - L.y, dz = pz - L.z;
+    return dx * dx + dy * dy + dz * dz;
+}
+template <class T> __device__ inline T d2_aabb(T px, T py, T pz, const NodeRec<T>& N) {
+    const T p[3] = {px, py, pz};
+    T s = T(0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const T a = N.lo[d], b = N.hi[d];
+        const T t = p[d] < a ? (a - p[d]) : (p[d] > b ? (p[d] - b) : T(0));
+        s += t * t;
+    }
+    return s;
+}
+template <class T> struct Line {
+    T o[3], m[3];
+};
+template <class T> __device__ inline T d2_line_leaf(const Line<T>& l, const LeafRec<T>& L) {
+    const T c[3] = {L.x, L.y, L.z};
+    T D2 = T(0), Dm = T(0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const T D = c[d] - l.o[d];
+        D2 += D * D;
+        Dm += D * l.m[d];
+    }
+    return D2 - Dm * Dm;
+}
+template <class T> __device__ inline T d2_line_aabb_lb(const Line<T>& l, const NodeRec<T>& N) {
+    // _line_intersects_aabb (:878-916)
+    T tmin = -T(INFINITY), tmax = T(INFINITY);
+    bool hit = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const T a = N.lo[d], b = N.hi[d], o = l.o[d], m = l.m[d];
+        if (m == T(0)) {
+            if (o < a || o > b) hit = false;
+        } else {
+            T t0 = (a - o) / m, t1 = (b - o) / m;
+            if (t0 > t1) {
+                const T tt = t0;
+                t0 = t1;
+                t1 = tt;
+            }
+            tmin = fmax(tmin, t0);
+            tmax = fmin(tmax, t1);
+            if (tmin > tmax) hit = false;
+        }
+    }
+    if (hit) return T(0);
+    // bounding-sphere lower bound (:918-953)
+    const T half = T(1) / T(2);
+    T c[3], r2 = T(0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) c[d] = (N.lo[d] + N.hi[d]) * half;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const T hh = (N.hi[d] - N.lo[d]) * half;
+        r2 += hh * hh;
+    }
+    T D2 = T(0), Dm = T(0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const T D = c[d] - l.o[d];
+        D2 += D * D;
+        Dm += D * l.m[d];
+    }
+    const T dc2 = D2 - Dm * Dm;
+    if (dc2 <= r2) return T(0);
+    const T dl = sqrt(dc2) - sqrt(r2);
+    return dl * dl;
+}
+
+// Radius rule of the strategy (A13): gather (K ha)^2, scatter (K h)^2, symmetric max.
+template <class T, int STRAT> __device__ inline T radius2(T Kvalid, T r2a, T h) {
+    if constexpr (STRAT == PH_GATHER) {
+        return r2a;
+    } else {
+        const T r = Kvalid * h;
+        const T rr = r * r;
+        if constexpr (STRAT == PH_SCATTER) return rr;
+        else return (rr > r2a) ? rr : r2a;
+    }
+}
+
+// Stackless DFS: hit(k0, d2, leaf) on each accepted leaf (k0 = 0-based leaf index).
+template <class T, int STRAT, class Dist, class NodeDist, class Hit>
+__device__ inline void dfs(const TreeView<T>& tv, T Kvalid, T r2a, Dist leaf_d2, NodeDist node_d2, Hit hit) {
+    const i64 n = tv.n;
+    if (tv.root == 0) {
+        for (i64 k = 0; k < n; ++k) {
+            const LeafRec<T> L = tv.leaves[k];
+            const T d2 = leaf_d2(L);
+            if (d2 <= radius2<T, STRAT>(Kvalid, r2a, L.h)) hit(k, d2, L);
+        }
+        return;
+    }
+    i32 node = tv.root;
+    // A valid tree is walked in at most 2n-1 steps; the cap only guards against
+    // a corrupted tree turning into an endless loop on the device.
+    for (i64 steps = 2 * n + 8; node != 0 && steps > 0; --steps) {
+        if (node >= (i32)n) {
+            const i64 k = (i64)node - n;
+            const LeafRec<T> L = tv.leaves[k];
+            const T d2 = leaf_d2(L);
+            if (d2 <= radius2<T, STRAT>(Kvalid, r2a, L.h)) hit(k, d2, L);
+            node = tv.leaf_escape[k];
+        } else {
+            const NodeRec<T> N = tv.nodes[node - 1];
+            const T d2n = node_d2(N);
+            node = (d2n > radius2<T, STRAT>(Kvalid, r2a, N.scale)) ? N.escape : N.left;
+        }
+    }
+}
+
+// LBVH_find_nearest_h (LinearBVHQueries.jl:118-140): shrinking bound, strict `<`.
+template <class T> __device__ inline T nearest_h(const TreeView<T>& tv, T px, T py, T pz) {
+    const i64 n = tv.n;
+    T best = T(INFINITY), hbest = sph_nan<T>();
+    if (tv.root == 0) {
+        for (i64 k = 0; k < n; ++k) {
+            const LeafRec<T> L = tv.leaves[k];
+            const T d2 = d2_point(px, py, pz, L);
+            if (d2 < best) {
+                best = d2;
+                hbest = L.h;
+            }
+        }
+        return hbest;
+    }
+    i32 node = tv.root;
+    for (i64 steps = 2 * n + 8; node != 0 && steps > 0; --steps) {
+        if (node >= (i32)n) {
+            const i64 k = (i64)node - n;
+            const LeafRec<T> L = tv.leaves[k];
+            const T d2 = d2_point(px, py, pz, L);
+            if (d2 < best) {
+                best = d2;
+                hbest = L.h;
+            }
+            node = tv.leaf_escape[k];
+        } else {
+            const NodeRec<T> N = tv.nodes[node - 1];
+            node = (d2_aabb(px, py, pz, N) > best) ? N.escape : N.left;
+        }
+    }
+    return hbest;
+}
+
+// ---- accumulation (general_interpolation.jl + accumulations/*.jl) ----
+template <class T, int K, int STRAT, int MS, int MG, int MD, int MC> struct Acc {
+    T S1;
+    T sc[MS > 0 ? MS : 1];
+    T gf[MG > 0 ? MG : 1][3], gb[MG > 0 ? MG : 1][3], gs[MG > 0 ? MG : 1];
+    T df[MD > 0 ? MD : 1], db[MD > 0 ? MD : 1][3], ds[MD > 0 ? MD : 1][3];
+    T cf[MC > 0 ? MC : 1][3], cb[MC > 0 ? MC : 1][3], cs[MC > 0 ? MC : 1][3];
+
+    __device__ void init() {
+        S1 = T(0);
+#pragma unroll
+        for (int j = 0; j < (MS > 0 ? MS : 1); ++j) sc[j] = T(0);
+#pragma unroll
+        for (int j = 0; j < (MG > 0 ? MG : 1); ++j) {
+            gs[j] = T(0);
+            for (int d = 0; d < 3; ++d) gf[j][d] = gb[j][d] = T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < (MD > 0 ? MD : 1); ++j) {
+            df[j] = T(0);
+            for (int d = 0; d < 3; ++d) db[j][d] = ds[j][d] = T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < (MC > 0 ? MC : 1); ++j)
+            for (int d = 0; d < 3; ++d) cf[j][d] = cb[j][d] = cs[j][d] = T(0);
+    }
+
+    __device__ inline T Wsel(T dr, T ha, T hb) const {
+        if constexpr (STRAT == PH_SYMMETRIC) return T(0.5) * (kernel_W<T, K>(dr, ha) + kernel_W<T, K>(dr, hb));
+        else if constexpr (STRAT == PH_GATHER) return kernel_W<T, K>(dr, ha);
+        else return kernel_W<T, K>(dr, hb);
+    }
+    __device__ inline void Gsel(T dx, T dy, T dz, T ha, T hb, T g[3]) const {
+        if constexpr (STRAT == PH_SYMMETRIC) {
+            T ax, ay, az, bx, by, bz;
+            kernel_gradW<T, K>(dx, dy, dz, ha, ax, ay, az);
+            kernel_gradW<T, K>(dx, dy, dz, hb, bx, by, bz);
+            g[0] = T(0.5) * (ax + bx);
+            g[1] = T(0.5) * (ay + by);
+            g[2] = T(0.5) * (az + bz);
+        } else {
+            kernel_gradW<T, K>(dx, dy, dz, STRAT == PH_GATHER ? ha : hb, g[0], g[1], g[2]);
+        }
+    }
+
+    // One accepted neighbour i (0-based Morton index).
+    __device__ inline void add(const CatChunk& c, const T* __restrict__ cols, i64 n, i64 i, T dx, T dy, T dz, T d2,
+                               T ha, T hb) {
+        const T dr = sqrt(d2);
+        const T mb = cols[3 * n + i], rb = cols[5 * n + i];
+        const T W = Wsel(dr, ha, hb);
+        S1 += mb * W / rb;
+#pragma unroll
+        for (int j = 0; j < MS; ++j)
+            if (j < c.ns) sc[j] += cols[(5 + c.sslot[j]) * n + i] * (mb * W / rb);
+        if constexpr (MG + MD + MC > 0) {
+            if (c.ng + c.nd + c.nc == 0) return;
+            T g[3];
+            Gsel(dx, dy, dz, ha, hb, g);
+            const T invr = T(1) / rb;
+#pragma unroll
+            for (int j = 0; j < MG; ++j) {
+                if (j >= c.ng) break;
+                const int slot = c.gslot[j];
+                if (slot == 0) {  // density gradient (gradient_accumulation.jl:7-35)
+                    for (int d = 0; d < 3; ++d) {
+                        const T mg = mb * g[d];
+                        gf[j][d] += mg;
+                        gb[j][d] += mg * invr;
+                    }
+                    gs[j] += mb * W;
+                } else {
+                    const T A = cols[(5 + slot) * n + i];
+                    for (int d = 0; d < 3; ++d) {
+                        const T w = mb * invr * g[d];
+                        gf[j][d] += w * A;
+                        gb[j][d] += w;
+                    }
+                    gs[j] += A * (mb * W / rb);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < MD; ++j) {
+                if (j >= c.nd) break;
+                T A[3], w[3];
+                for (int d = 0; d < 3; ++d) {
+                    A[d] = cols[(5 + c.dslot[3 * j + d]) * n + i];
+                    w[d] = mb * invr * g[d];
+                }
+                df[j] += w[0] * A[0] + w[1] * A[1] + w[2] * A[2];
+                for (int d = 0; d < 3; ++d) {
+                    db[j][d] += w[d];
+                    ds[j][d] += A[d] * (mb * W / rb);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < MC; ++j) {
+                if (j >= c.nc) break;
+                T A[3], w[3];
+                for (int d = 0; d < 3; ++d) {
+                    A[d] = cols[(5 + c.cslot[3 * j + d]) * n + i];
+                    w[d] = mb * invr * g[d];
+                }
+                cf[j][0] += A[1] * w[2] - A[2] * w[1];
+                cf[j][1] += A[2] * w[0] - A[0] * w[2];
+                cf[j][2] += A[0] * w[1] - A[1] * w[0];
+                for (int d = 0; d < 3; ++d) {
+                    cb[j][d] += w[d];
+                    cs[j][d] += A[d] * (mb * W / rb);
+                }
+            }
+        }
+    }
+
+    // Epilogue (general_interpolation.jl:128-214); out[k*S + s].
+    __device__ inline void finish(const CatChunk& c, T* __restrict__ out, i64 S, i64 s) const {
+        if (S1 == T(0)) {
+            const T nan = sph_nan<T>();
+#pragma unroll
+            for (int j = 0; j < MS; ++j)
+                if (j < c.ns) out[c.sout[j] * S + s] = nan;
+#pragma unroll
+            for (int j = 0; j < MG; ++j)
+                if (j < c.ng)
+                    for (int d = 0; d < 3; ++d) out[(c.gout[j] + d) * S + s] = nan;
+#pragma unroll
+            for (int j = 0; j < MD; ++j)
+                if (j < c.nd) out[c.dout[j] * S + s] = nan;
+#pragma unroll
+            for (int j = 0; j < MC; ++j)
+                if (j < c.nc)
+                    for (int d = 0; d < 3; ++d) out[(c.cout[j] + d) * S + s] = nan;
+            return;
+        }
+        const T invS1 = T(1) / S1;
+#pragma unroll
+        for (int j = 0; j < MS; ++j)
+            if (j < c.ns) out[c.sout[j] * S + s] = c.sflag[j] ? sc[j] * invS1 : sc[j];
+#pragma unroll
+        for (int j = 0; j < MG; ++j)
+            if (j < c.ng) {
+                const T A = gs[j] * invS1;
+                for (int d = 0; d < 3; ++d) out[(c.gout[j] + d) * S + s] = gf[j][d] - gb[j][d] * A;
+            }
+#pragma unroll
+        for (int j = 0; j < MD; ++j)
+            if (j < c.nd) {
+                const T Ax = ds[j][0] * invS1, Ay = ds[j][1] * invS1, Az = ds[j][2] * invS1;
+                const T b = Ax * db[j][0] + Ay * db[j][1] + Az * db[j][2];
+                out[c.dout[j] * S + s] = df[j] - b;
+            }
+#pragma unroll
+        for (int j = 0; j < MC; ++j)
+            if (j < c.nc) {
+                const T Ax = cs[j][0] * invS1, Ay = cs[j][1] * invS1, Az = cs[j][2] * invS1;
+                const T bx = cb[j][0], by = cb[j][1], bz = cb[j][2];
+                const T qx = Ay * bz - Az * by, qy = Az * bx - Ax * bz, qz = Ax * by - Ay * bx;
+                out[(c.cout[j] + 0) * S + s] = -(cf[j][0] - qx);
+                out[(c.cout[j] + 1) * S + s] = -(cf[j][1] - qy);
+                out[(c.cout[j] + 2) * S + s] = -(cf[j][2] - qz);
+            }
+    }
+};
+
+// Sample generation.  SRC 0: list of points; SRC 1: Cartesian grid with 4x4x4 wave tiles.
+template <class T, int SRC> struct Sampler;
+template <class T> struct Sampler<T, 0> {
+    PointList pl;
+    i64 S;
+    __device__ inline bool get(i64& s, T& x, T& y, T& z) const {
+        s = xcd_block() * kBlock + threadIdx.x;
+        if (s >= S) return false;
+        x = T(pl.x[s]);
+        y = T(pl.y[s]);
+        z = T(pl.z[s]);
+        return true;
+    }
+};
+template <class T> struct Sampler<T, 1> {
+    GridAxes g;
+    i64 S;
+    __device__ inline bool get(i64& s, T& x, T& y, T& z) const {
+        const i64 tile = xcd_block() * (kBlock / kWave) + threadIdx.x / kWave;
+        const int lane = threadIdx.x & (kWave - 1);
+        const i64 tx = (g.nx + 3) / 4, ty = (g.ny + 3) / 4;
+        const i64 bx = tile % tx, by = (tile / tx) % ty, bz = tile / (tx * ty);
+        const i64 ix = bx * 4 + (lane & 3), iy = by * 4 + ((lane >> 2) & 3), iz = bz * 4 + (lane >> 4);
+        if (ix >= g.nx || iy >= g.ny || iz >= g.nz) return false;
+        s = ix + g.nx * (iy + g.ny * iz);
+        x = T(g.ax[ix]);
+        y = T(g.ay[iy]);
+        z = T(g.az[iz]);
+        return true;
+    }
+};
+
+__device__ inline void count_add(u64* counter, u64 v) {
+    if (!counter) return;
+    // wave reduction, one atomic per wave
+    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && v) atomicAdd((unsigned long long*)counter, (unsigned long long)v);
+}
+
+template <class T, int K, int STRAT, int MS, int MG, int MD, int MC, int SRC>
+__global__ void __launch_bounds__(kBlock) k_points(TreeView<T> tv, const T* __restrict__ cols, CatChunk c,
+                                                   Sampler<T, SRC> smp, T* __restrict__ out, u64* counter) {
+    i64 s;
+    T px, py, pz;
+    const bool valid = smp.get(s, px, py, pz);
+    u64 accepted = 0;
+    if (valid) {
+        const T Kvalid = T(support_of<K>());
+        T ha = T(0);
+        if constexpr (STRAT != PH_SCATTER) ha = nearest_h<T>(tv, px, py, pz);
+        const T ra = Kvalid * ha;
+        const T r2a = ra * ra;
+        Acc<T, K, STRAT, MS, MG, MD, MC> acc;
+        acc.init();
+        const i64 n = tv.n;
+        dfs<T, STRAT>(
+            tv, Kvalid, r2a, [&](const LeafRec<T>& L) { return d2_point(px, py, pz, L); },
+            [&](const NodeRec<T>& N) { return d2_aabb(px, py, pz, N); },
+            [&](i64 k, T d2, const LeafRec<T>& L) {
+                ++accepted;
+                acc.add(c, cols, n, k, px - L.x, py - L.y, pz - L.z, d2, ha, L.h);
+            });
+        acc.finish(c, out, smp.S, s);
+    }
+    count_add(counter, accepted);
+}
+
+template <class T, int K, int STRAT, int SRC> struct LineSampler;
+template <class T, int K, int STRAT> struct LineSampler<T, K, STRAT, 0> {
+    LineList ll;
+    i64 S;
+    __device__ inline bool get(i64& s, Line<T>& l) const {
+        s = xcd_block() * kBlock + threadIdx.x;
+        if (s >= S) return false;
+        l.o[0] = T(ll.ox[s]); l.o[1] = T(ll.oy[s]); l.o[2] = T(ll.oz[s]);
+        l.m[0] = T(ll.dx[s]); l.m[1] = T(ll.dy[s]); l.m[2] = T(ll.dz[s]);
+        return true;
+    }
+};
+template <class T, int K, int STRAT> struct LineSampler<T, K, STRAT, 1> {
+    LineGrid lg;
+    i64 S;
+    __device__ inline bool get(i64& s, Line<T>& l) const {
+        const i64 tile = xcd_block() * (kBlock / kWave) + threadIdx.x / kWave;
+        const int lane = threadIdx.x & (kWave - 1);
+        const i64 tx = (lg.nx + 7) / 8;
+        const i64 i = (tile % tx) * 8 + (lane & 7), j = (tile / tx) * 8 + (lane >> 3);
+        if (i >= lg.nx || j >= lg.ny) return false;
+        s = i + lg.nx * j;
+        const double a = lg.ax[i], b = lg.ay[j];
+        for (int d = 0; d < 3; ++d) {
+            l.o[d] = T((lg.o[d] + lg.u[d] * a) + lg.v[d] * b);
+            l.m[d] = T(lg.d[d]);
+        }
+        return true;
+    }
+};
+
+// _line_integrated_quantities_interpolate_kernel, scatter (:123-166).
+template <class T, int K, int STRAT, int SRC>
+__global__ void __launch_bounds__(kBlock) k_lines(TreeView<T> tv, const T* __restrict__ cols, CatChunk c,
+                                                  LineSampler<T, K, STRAT, SRC> smp, const T* __restrict__ inorm, T dq,
+                                                  T* __restrict__ out, u64* counter) {
+    i64 s;
+    Line<T> l;
+    const bool valid = smp.get(s, l);
+    u64 accepted = 0;
+    if (valid) {
+        const T Kvalid = T(support_of<K>());
+        T S1 = T(0);
+        T acc[kMaxS];
+#pragma unroll
+        for (int j = 0; j < kMaxS; ++j) acc[j] = T(0);
+        const i64 n = tv.n;
+        dfs<T, STRAT>(
+            tv, Kvalid, T(0), [&](const LeafRec<T>& L) { return d2_line_leaf(l, L); },
+            [&](const NodeRec<T>& N) { return d2_line_aabb_lb(l, N); },
+            [&](i64 k, T d2, const LeafRec<T>& L) {
+                ++accepted;
+                const T dr = sqrt(d2 < T(0) ? T(0) : d2);
+                const T mb = cols[3 * n + k], rb = cols[5 * n + k];
+                const T W = line_W<T, K>(dr, L.h, inorm, dq);
+                S1 += mb * W / rb;
+#pragma unroll
+                for (int j = 0; j < kMaxS; ++j)
+                    if (j < c.ns) acc[j] += cols[(5 + c.sslot[j]) * n + k] * (mb * W / rb);
+            });
+        if (S1 == T(0)) {
+#pragma unroll
+            for (int j = 0; j < kMaxS; ++j)
+                if (j < c.ns) out[c.sout[j] * smp.S + s] = sph_nan<T>();
+        } else {
+            const T invS1 = T(1) / S1;
+#pragma unroll
+            for (int j = 0; j < kMaxS; ++j)
+                if (j < c.ns) out[c.sout[j] * smp.S + s] = c.sflag[j] ? acc[j] * invS1 : acc[j];
+        }
+    }
+    count_add(counter, accepted);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_nearest(TreeView<T> tv, PointList pl, i64 S, double* ha) {
+    const i64 s = (i64)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= S) return;
+    ha[s] = (double)nearest_h<T>(tv, T(pl.x[s]), T(pl.y[s]), T(pl.z[s]));
+}
+
+inline unsigned blocks_for_points(int src, i64 S, const GridAxes* ga) {
+    if (src == 1) {
+        const i64 tiles = ((ga->nx + 3) / 4) * ((ga->ny + 3) / 4) * ((ga->nz + 3) / 4);
+        return (unsigned)((tiles + 3) / 4);
+    }
+    return (unsigned)((S + kBlock - 1) / kBlock);
+}
+
+template <class T, int K, int STRAT, int MS, int MG, int MD, int MC>
+void launch_points_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const GridAxes* ga, i64 S, void* out,
+                     u64* counters, hipStream_t s) {
+    const TreeView<T> tv = view_of<T>(B);
+    const T* cols = B->cols.as<T>();
+    if (ga) {
+        Sampler<T, 1> smp{*ga, S};
+        k_points<T, K, STRAT, MS, MG, MD, MC, 1><<<blocks_for_points(1, S, ga), kBlock, 0, s>>>(
+            tv, cols, c, smp, static_cast<T*>(out), counters);
+    } else {
+        Sampler<T, 0> smp{*pl, S};
+        k_points<T, K, STRAT, MS, MG, MD, MC, 0><<<blocks_for_points(0, S, nullptr), kBlock, 0, s>>>(
+            tv, cols, c, smp, static_cast<T*>(out), counters);
+    }
+    PH_HIP(hipGetLastError());
+}
+
+template <class T, int K, int STRAT>
+void launch_points_k(ph_lbvh* B, const CatChunk& c, const PointList* pl, const GridAxes* ga, i64 S, void* out,
+                     u64* counters, hipStream_t s) {
+    if (c.ng + c.nd + c.nc == 0 && c.ns <= 2)
+        launch_points_t<T, K, STRAT, 2, 0, 0, 0>(B, c, pl, ga, S, out, counters, s);
+    else
+        launch_points_t<T, K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC>(B, c, pl, ga, S, out, counters, s);
+}
+
+template <class T, int K>
+void launch_points_s(ph_lbvh* B, const CatChunk& c, int strategy, const PointList* pl, const GridAxes* ga, i64 S,
+                     void* out, u64* counters, hipStream_t s) {
+    switch (strategy) {
+    case PH_GATHER: launch_points_k<T, K, PH_GATHER>(B, c, pl, ga, S, out, counters, s); break;
+    case PH_SCATTER: launch_points_k<T, K, PH_SCATTER>(B, c, pl, ga, S, out, counters, s); break;
+    default: launch_points_k<T, K, PH_SYMMETRIC>(B, c, pl, ga, S, out, counters, s); break;
+    }
+}
+
+template <class T>
+void launch_points_p(ph_lbvh* B, const CatChunk& c, int strategy, const PointList* pl, const GridAxes* ga, i64 S,
+                     void* out, u64* counters, hipStream_t s) {
+    switch (B->kernel) {
+    case PH_KERNEL_M4: launch_points_s<T, PH_KERNEL_M4>(B, c, strategy, pl, ga, S, out, counters, s); break;
+    case PH_KERNEL_M5: launch_points_s<T, PH_KERNEL_M5>(B, c, strategy, pl, ga, S, out, counters, s); break;
+    case PH_KERNEL_M6: launch_points_s<T, PH_KERNEL_M6>(B, c, strategy, pl, ga, S, out, counters, s); break;
+    case PH_KERNEL_C2: launch_points_s<T, PH_KERNEL_C2>(B, c, strategy, pl, ga, S, out, counters, s); break;
+    case PH_KERNEL_C4: launch_points_s<T, PH_KERNEL_C4>(B, c, strategy, pl, ga, S, out, counters, s); break;
+    default: launch_points_s<T, PH_KERNEL_C6>(B, c, strategy, pl, ga, S, out, counters, s); break;
+    }
+}
+
+template <class T, int K>
+void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const LineGrid* lg, i64 S, void* out,
+                    u64* counters, const T* inorm, hipStream_t s) {
+    const TreeView<T> tv = view_of<T>(B);
+    const T* cols = B->cols.as<T>();
+    // Q_2 - Q_1 of the table in T (line_integrated_kernel.jl:14)
+    const T dq = T((1.0 * support_of<K>()) / 4095.0) - T(0.0);
+    const T* tab = inorm + (size_t)K * PH_LUT_N_ENTRIES;
+    if (lg) {
+        LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S};
+        const i64 tiles = ((lg->nx + 7) / 8) * ((lg->ny + 7) / 8);
+        k_lines<T, K, PH_SCATTER, 1><<<(unsigned)((tiles + 3) / 4), kBlock, 0, s>>>(tv, cols, c, smp, tab, dq,
+                                                                                    static_cast<T*>(out), counters);
+    } else {
+        LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S};
+        k_lines<T, K, PH_SCATTER, 0><<<(unsigned)((S + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+            tv, cols, c, smp, tab, dq, static_cast<T*>(out), counters);
+    }
+    PH_HIP(hipGetLastError());
+}
+
+template <class T>
+void launch_lines_p(ph_lbvh* B, const CatChunk& c, const LineList* ll, const LineGrid* lg, i64 S, void* out,
+                    u64* counters, const T* inorm, hipStream_t s) {
+    switch (B->kernel) {
+    case PH_KERNEL_M4: launch_lines_k<T, PH_KERNEL_M4>(B, c, ll, lg, S, out, counters, inorm, s); break;
+    case PH_KERNEL_M5: launch_lines_k<T, PH_KERNEL_M5>(B, c, ll, lg, S, out, counters, inorm, s); break;
+    case PH_KERNEL_M6: launch_lines_k<T, PH_KERNEL_M6>(B, c, ll, lg, S, out, counters, inorm, s); break;
+    case PH_KERNEL_C2: launch_lines_k<T, PH_KERNEL_C2>(B, c, ll, lg, S, out, counters, inorm, s); break;
+    case PH_KERNEL_C4: launch_lines_k<T, PH_KERNEL_C4>(B, c, ll, lg, S, out, counters, inorm, s); break;
+    default: launch_lines_k<T, PH_KERNEL_C6>(B, c, ll, lg, S, out, counters, inorm, s); break;
+    }
+}
+
+}  // namespace
+
+void launch_points(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl,
+                   const GridAxes* ga, i64 S, int L, void* out, u64* counters, hipStream_t s) {
+    (void)L;
+    if (S <= 0) return;
+    if (precision == PH_PREC_FP64) launch_points_p<double>(B, c, strategy, pl, ga, S, out, counters, s);
+    else launch_points_p<float>(B, c, strategy, pl, ga, S, out, counters, s);
+}
+
+void launch_lines(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const LineList* ll, const LineGrid* lg,
+                  i64 S, int L, void* out, u64* counters, const void* lut, hipStream_t s) {
+    (void)L;
+    (void)strategy;
+    if (S <= 0) return;
+    if (precision == PH_PREC_FP64)
+        launch_lines_p<double>(B, c, ll, lg, S, out, counters, static_cast<const double*>(lut), s);
+    else
+        launch_lines_p<float>(B, c, ll, lg, S, out, counters, static_cast<const float*>(lut), s);
+}
+
+void launch_nearest_h(ph_lbvh* B, const PointList& pl, i64 S, double* ha, hipStream_t s) {
+    if (S <= 0) return;
+    const unsigned g = (unsigned)((S + kBlock - 1) / kBlock);
+    if (B->precision == PH_PREC_FP64) k_nearest<double><<<g, kBlock, 0, s>>>(view_of<double>(B), pl, S, ha);
+    else k_nearest<float><<<g, kBlock, 0, s>>>(view_of<float>(B), pl, S, ha);
+    PH_HIP(hipGetLastError());
+}
+
+}  // namespace ph
