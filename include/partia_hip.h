@@ -134,6 +134,14 @@ int ph_lbvh_build_device(ph_context* ctx, int64_t n, const double* x_dev, const 
                          const double* const* cols_dev, int32_t kernel, int32_t code_bits, int32_t precision,
                          ph_lbvh** out);
 int ph_lbvh_destroy(ph_lbvh* bvh);
+/* Replace the m, rho and quantity columns of a built tree (host arrays already
+ * in Morton order, i.e. permuted as LinearBVH! left them).  The external-LBVH
+ * driver overloads (KI/grid_interpolation/drivers/PointSamples_driver.jl:98-118,
+ * LineSamples_driver.jl:116-139) read these columns from the current input, so
+ * the host binding refreshes them before each such call; ncol may differ from
+ * the build's. */
+int ph_lbvh_set_columns(ph_lbvh* bvh, const double* m_host, const double* rho_host, int32_t ncol,
+                        const double* const* cols_host);
 /* Tree/encoding accessors for the host mirror of LinearBVH (LBVH/LinearBVH/LinearBVH.jl:8-14). */
 int ph_lbvh_info(const ph_lbvh* bvh, int64_t* n, int32_t* root, int32_t* ncol, int32_t* code_bits,
                  int32_t* precision);

@@ -287,6 +287,8 @@ template <class T, class TI> struct Encoding {
 template <class T, class TI>
 Encoding<T, TI> morton_encoding(const std::vector<T>& x, const std::vector<T>& y, const std::vector<T>& z) {
     size_t n = x.size();
+    // minimum/maximum of an empty column throw in the reference (and the tree needs n >= 1)
+    if (n == 0) throw std::invalid_argument("BinaryRadixTree: enc.codes must be non-empty (got n=0).");
     std::vector<TI> ix, iy, iz;
     quantize_axis<T, TI>(x, ix);
     quantize_axis<T, TI>(y, iy);

@@ -283,6 +283,42 @@ int ph_lbvh_destroy(ph_lbvh* B) {
     return PH_OK;
 }
 
+int ph_lbvh_set_columns(ph_lbvh* B, const double* m, const double* rho, int32_t ncol, const double* const* cols) {
+    if (!B) return PH_ERR_ARGUMENT;
+    ph_context* ctx = B->ctx;
+    return guarded(ctx, [&] {
+        need(m && rho && ncol >= 0 && (ncol == 0 || cols), PH_ERR_ARGUMENT, "null column pointer");
+        for (int c = 0; c < ncol; ++c) need(cols[c] != nullptr, PH_ERR_ARGUMENT, "null column pointer");
+        const i64 n = B->n;
+        const bool f64 = B->precision == PH_PREC_FP64;
+        const size_t es = f64 ? sizeof(double) : sizeof(float);
+        PH_HIP(hipStreamSynchronize(ctx->stream));
+        if (ncol != B->ncol) {
+            // keep x, y, z, (m), h, (rho); reallocate for the new column count
+            DevBuf nc;
+            nc.alloc(es * (size_t)(6 + ncol) * n);
+            PH_HIP(hipMemcpy(nc.p, B->cols.p, es * 6 * (size_t)n, hipMemcpyDeviceToDevice));
+            std::swap(nc.p, B->cols.p);
+            std::swap(nc.bytes, B->cols.bytes);
+            B->ncol = ncol;
+        }
+        auto put = [&](int c, const double* src) {
+            if (f64) {
+                PH_HIP(hipMemcpy(B->cols.as<double>() + (size_t)c * n, src, sizeof(double) * n, hipMemcpyHostToDevice));
+            } else {
+                std::vector<float> t(src, src + n);
+                PH_HIP(hipMemcpy(B->cols.as<float>() + (size_t)c * n, t.data(), sizeof(float) * n,
+                                 hipMemcpyHostToDevice));
+            }
+        };
+        put(3, m);
+        put(5, rho);
+        for (int c = 0; c < ncol; ++c) put(6 + c, cols[c]);
+        B->c32_ready = false;
+        return PH_OK;
+    });
+}
+
 int ph_lbvh_info(const ph_lbvh* B, int64_t* n, int32_t* root, int32_t* ncol, int32_t* code_bits, int32_t* precision) {
     if (!B) return PH_ERR_ARGUMENT;
     if (n) *n = B->n;

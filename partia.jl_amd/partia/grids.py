@@ -1,0 +1,216 @@
+"""Sample containers: PointSamples, LineSamples, StructuredGrid, GridBundle.
+
+Mirrors Grids/grids/PointSamples.jl:25-65, LineSamples.jl:31-75,
+StructuredGrid.jl:19-23,74-76,140-222,285-291 and transform.jl:49-112, and
+Grids/griddataset/GridBundle.jl:1-4.  Arrays follow Julia's column-major layout:
+a StructuredGrid's values are an order='F' numpy array and flattening is x
+fastest.  Non-Cartesian coordinate systems are converted on the host.
+"""
+import numpy as np
+
+from .errors import ArgumentError
+from .synthetic import linrange
+
+
+class AbstractCoordinateSystem:
+    pass
+
+
+class Cartesian(AbstractCoordinateSystem):
+    pass
+
+
+class Polar(AbstractCoordinateSystem):
+    pass
+
+
+class Cylindrical(AbstractCoordinateSystem):
+    pass
+
+
+class Spherical(AbstractCoordinateSystem):
+    pass
+
+
+class AbstractGrid:
+    pass
+
+
+class AbstractSamples(AbstractGrid):
+    pass
+
+
+class PointSamples(AbstractSamples):
+    def __init__(self, grid, coor):
+        coor = tuple(coor)
+        n = len(grid)
+        for d, c in enumerate(coor):
+            if len(c) != n:
+                raise ArgumentError("coor[%d] length mismatch" % (d + 1))
+        self.grid = grid
+        self.coor = coor
+
+    def similar(self, dtype=None):
+        """similar(grid) shares the coordinates; similar(grid, T) copies them as T."""
+        if dtype is None or np.dtype(dtype) == self.grid.dtype:
+            return PointSamples(np.empty_like(self.grid), self.coor)
+        return PointSamples(np.empty(len(self.grid), dtype=dtype), tuple(np.asarray(c, dtype=dtype) for c in self.coor))
+
+    @property
+    def dtype(self):
+        return self.grid.dtype
+
+    def __len__(self):
+        return len(self.grid)
+
+
+class LineSamples(AbstractSamples):
+    def __init__(self, grid, origin, direction):
+        origin, direction = tuple(origin), tuple(direction)
+        n = len(grid)
+        for d in range(len(origin)):
+            if len(origin[d]) != n:
+                raise ArgumentError("origin[%d] length mismatch" % (d + 1))
+            if len(direction[d]) != n:
+                raise ArgumentError("direction[%d] length mismatch" % (d + 1))
+        self.grid = grid
+        self.origin = origin
+        self.direction = direction
+
+    def similar(self, dtype=None):
+        if dtype is None or np.dtype(dtype) == self.grid.dtype:
+            return LineSamples(np.empty_like(self.grid), self.origin, self.direction)
+        return LineSamples(np.empty(len(self.grid), dtype=dtype), tuple(np.asarray(c, dtype=dtype) for c in self.origin),
+                           tuple(np.asarray(c, dtype=dtype) for c in self.direction))
+
+    @property
+    def dtype(self):
+        return self.grid.dtype
+
+    def __len__(self):
+        return len(self.grid)
+
+
+class StructuredGrid(AbstractGrid):
+    """StructuredGrid{D, TF}: values on the tensor product of D axes."""
+
+    def __init__(self, *args, dtype=None):
+        if len(args) >= 1 and isinstance(args[0], type) and issubclass(args[0], AbstractCoordinateSystem):
+            coord, params = args[0], args[1:]
+            self._from_params(coord, params, dtype)
+        else:
+            grid, axes, size = args
+            self.grid = grid
+            self.axes = tuple(axes)
+            self.size = tuple(size)
+
+    def _from_params(self, coord, params, dtype):
+        if dtype is None:
+            dtype = np.float64
+            for p in params:
+                if isinstance(p, tuple) and isinstance(p[0], np.float32):
+                    dtype = np.float32
+        T = np.dtype(dtype).type
+        if coord is Cartesian:
+            axes = tuple(linrange(T(p[0]), T(p[1]), int(p[2]), dtype) for p in params)
+        elif coord in (Polar, Cylindrical):
+            (smin, smax, ns), (pmin, pmax, npp) = params[0], params[1]
+            if not (ns >= 1 and npp >= 1):
+                raise ArgumentError("ns and nϕ must be ≥ 1")
+            if not (smin >= 0 and smax > smin):
+                raise ArgumentError("radial range must satisfy 0 ≤ smin < smax")
+            if not (pmin >= 0 and (pmax <= 2 * np.pi or np.isclose(pmax, 2 * np.pi)) and pmax > pmin):
+                raise ArgumentError("angular range must satisfy 0 ≤ ϕmin < ϕmax ≤ 2π")
+            saxes = linrange(T(smin), T(smax), ns, dtype)
+            dphi = (T(pmax) - T(pmin)) / T(npp)
+            paxes = (T(pmin) + np.arange(npp, dtype=dtype) * dphi).astype(dtype)
+            axes = (saxes, paxes)
+            if coord is Cylindrical:
+                zp = params[2]
+                if isinstance(zp, tuple):
+                    zmin, zmax, nz = zp
+                    if not zmax > zmin:
+                        raise ArgumentError("axial range must satisfy zmin < zmax")
+                    axes = axes + (linrange(T(zmin), T(zmax), nz, dtype),)
+                else:
+                    axes = axes + (np.array([T(zp)], dtype=dtype),)
+        elif coord is Spherical:
+            (rmin, rmax, nr), (pmin, pmax, npp), (tmin, tmax, nt) = params
+            axes = (linrange(T(rmin), T(rmax), nr, dtype), linrange(T(pmin), T(pmax), npp, dtype),
+                    linrange(T(tmin), T(tmax), nt, dtype))
+        else:
+            raise ArgumentError("unknown coordinate system")
+        self.axes = axes
+        self.size = tuple(len(a) for a in axes)
+        self.grid = np.zeros(self.size, dtype=dtype, order="F")
+
+    @property
+    def dtype(self):
+        return self.grid.dtype
+
+    def similar(self, dtype=None):
+        if dtype is None or np.dtype(dtype) == self.grid.dtype:
+            return StructuredGrid(np.zeros(self.size, dtype=self.grid.dtype, order="F"), self.axes, self.size)
+        return StructuredGrid(np.zeros(self.size, dtype=dtype, order="F"),
+                              tuple(np.asarray(a, dtype=dtype) for a in self.axes), self.size)
+
+    def __len__(self):
+        return int(np.prod(self.size))
+
+
+def _mesh(axes):
+    return np.meshgrid(*axes, indexing="ij")
+
+
+def coordinate_grid(coord, grid):
+    """Flattened (column-major) Cartesian coordinates of every grid point
+    (StructuredGrid.jl:140-222)."""
+    m = [a.ravel(order="F") for a in _mesh(grid.axes)]
+    if coord is Cartesian:
+        return tuple(m)
+    if coord is Polar:
+        s, phi = m
+        return (s * np.cos(phi), s * np.sin(phi))
+    if coord is Cylindrical:
+        s, phi, z = m
+        return (s * np.cos(phi), s * np.sin(phi), z)
+    if coord is Spherical:
+        r, phi, theta = m
+        return (r * np.sin(theta) * np.cos(phi), r * np.sin(theta) * np.sin(phi), r * np.cos(theta))
+    raise ArgumentError("unknown coordinate system")
+
+
+def flatten(coord, grid):
+    """flatten(COORD, grid) -> PointSamples sharing the grid values (transform.jl:49-67)."""
+    return PointSamples(grid.grid.ravel(order="F"), coordinate_grid(coord, grid))
+
+
+def restore_struct(coord, points, axes, atol=1e-8, rtol=1e-8):
+    """restore_struct(COORD, points, axes) (transform.jl:103-112): checks the
+    coordinates against the axes, then reshapes the values."""
+    size = tuple(len(a) for a in axes)
+    expected = coordinate_grid(coord, StructuredGrid(np.zeros(size, order="F"), axes, size))
+    for got, exp in zip(points.coor, expected):
+        if got.shape != exp.shape or not np.allclose(got, exp, atol=atol, rtol=rtol):
+            raise ArgumentError("PointSamples coordinates do not match the provided axes under %s dispatch"
+                                % coord.__name__)
+    return StructuredGrid(np.reshape(points.grid, size, order="F"), axes, size)
+
+
+class GridBundle:
+    """GridBundle{L, G}: the grids returned by every driver, with their names."""
+
+    def __init__(self, grids, names):
+        self.grids = tuple(grids)
+        self.names = tuple(names)
+
+    def __len__(self):
+        return len(self.grids)
+
+    def __getitem__(self, key):
+        if isinstance(key, str):
+            return self.grids[self.names.index(key)]
+        return self.grids[key]
+
+    def datatype(self):
+        return self.grids[0].dtype if self.grids else np.dtype(np.float64)

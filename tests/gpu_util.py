@@ -1,0 +1,121 @@
+"""Helpers for the GPU parity tests: build a device tree through the C ABI and
+read its arrays, side by side with the oracle."""
+import ctypes as C
+
+import numpy as np
+
+from partia import _native as N
+
+import pyoracle as O
+
+
+class DeviceSet:
+    """One ph_lbvh built from host float64 columns (x, y, z, m, h, rho, cols...)."""
+
+    def __init__(self, ctx, x, y, z, m, h, rho, cols=(), kernel=0, code_bits=63, precision=64):
+        self.ctx = ctx
+        self.n = len(x)
+        self.ncol = len(cols)
+        arrs = [N.f64(a) for a in (x, y, z, m, h, rho)]
+        cl = [N.f64(c) for c in cols]
+        cptr = (C.c_void_p * max(1, len(cl)))(*[c.ctypes.data for c in cl])
+        self.h = C.c_void_p()
+        ctx.check(N.lib().ph_lbvh_build(ctx.handle, self.n, *[N.ptr(a) for a in arrs], len(cl), cptr, kernel,
+                                        code_bits, precision, C.byref(self.h)))
+
+    def get(self, which, dtype, size):
+        out = np.zeros(size, dtype)
+        self.ctx.check(N.lib().ph_lbvh_get(self.h, N.GET[which], N.ptr(out)))
+        return out
+
+    def tree(self):
+        n = self.n
+        t = {k: self.get(k, np.int32, 2 * n - 1) for k in ("left", "right", "escape", "parent")}
+        t["order"] = self.get("order", np.uint64, n)
+        t["codes"] = self.get("codes", np.uint64, n)
+        root = C.c_int32()
+        N.lib().ph_lbvh_info(self.h, None, C.byref(root), None, None, None)
+        t["root"] = root.value
+        return t
+
+    def bvh(self):
+        n, ni = self.n, max(self.n - 1, 0)
+        return dict(leaf_coor=self.get("leaf_coor", np.float64, 3 * n).reshape(3, n),
+                    leaf_scale=self.get("leaf_scale", np.float64, n),
+                    node_min=self.get("node_min", np.float64, 3 * ni).reshape(3, ni),
+                    node_max=self.get("node_max", np.float64, 3 * ni).reshape(3, ni),
+                    node_scale=self.get("node_scale", np.float64, ni))
+
+    def columns(self):
+        return self.get("columns", np.float64, (6 + self.ncol) * self.n).reshape(6 + self.ncol, self.n)
+
+    def points(self, cat, strategy, px, py, pz, precision=64):
+        ca = N.CatalogArgs(**cat)
+        S = len(px)
+        out = np.zeros(max(ca.L, 1) * S)
+        x, y, z = N.f64(px), N.f64(py), N.f64(pz)
+        self.ctx.check(N.lib().ph_point_samples(self.ctx.handle, self.h, ca.ref(), strategy, precision, S, N.ptr(x),
+                                                N.ptr(y), N.ptr(z), N.ptr(out)))
+        return out[:ca.L * S].reshape(ca.L, S)
+
+    def grid(self, cat, strategy, ax, ay, az, precision=64):
+        ca = N.CatalogArgs(**cat)
+        S = len(ax) * len(ay) * len(az)
+        out = np.zeros(max(ca.L, 1) * S)
+        a, b, c = N.f64(ax), N.f64(ay), N.f64(az)
+        self.ctx.check(N.lib().ph_structured_grid(self.ctx.handle, self.h, ca.ref(), strategy, precision, N.ptr(a),
+                                                  len(a), N.ptr(b), len(b), N.ptr(c), len(c), N.ptr(out)))
+        return out[:ca.L * S].reshape(ca.L, S)
+
+    def lines(self, cat, strategy, origin, direction, precision=64):
+        ca = N.CatalogArgs(**cat)
+        S = len(origin[0])
+        out = np.zeros(max(ca.L, 1) * S)
+        arrs = [N.f64(a) for a in list(origin) + list(direction)]
+        self.ctx.check(N.lib().ph_line_samples(self.ctx.handle, self.h, ca.ref(), strategy, precision, S,
+                                               *[N.ptr(a) for a in arrs], N.ptr(out)))
+        return out[:ca.L * S].reshape(ca.L, S)
+
+    def nearest_h(self, px, py, pz):
+        x, y, z = N.f64(px), N.f64(py), N.f64(pz)
+        ha = np.zeros(len(x))
+        self.ctx.check(N.lib().ph_nearest_h(self.ctx.handle, self.h, len(x), N.ptr(x), N.ptr(y), N.ptr(z), N.ptr(ha)))
+        return ha
+
+    def close(self):
+        if self.h:
+            N.lib().ph_lbvh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def grid_points(ax, ay, az):
+    g = np.meshgrid(ax, ay, az, indexing="ij")
+    return [a.ravel(order="F") for a in g]
+
+
+def assert_same(got, ref, bitwise=True, rtol=1e-12):
+    """NaN masks identical; values bit-identical (float64 path) or within rtol."""
+    assert got.shape == ref.shape
+    assert np.array_equal(np.isnan(got), np.isnan(ref)), "NaN masks differ"
+    fin = ~np.isnan(ref)
+    if bitwise:
+        bad = np.nonzero(got[fin] != ref[fin])[0]
+        assert len(bad) == 0, "%d values differ, max rel %.3e" % (
+            len(bad), np.max(np.abs(got[fin] - ref[fin]) / np.maximum(np.abs(ref[fin]), 1e-300)))
+    else:
+        # normwise per output row: max |got - ref| / max |ref|
+        for k in range(ref.shape[0]):
+            f = fin[k]
+            if f.any():
+                scale = max(np.max(np.abs(ref[k][f])), 1e-300)
+                assert np.max(np.abs(got[k][f] - ref[k][f])) <= rtol * scale, "output %d" % k
+
+
+def oracle_of(x, y, z, m, h, rho, cols=(), kernel=0, code_bits=63, precision=64):
+    return O.OracleSet(x, y, z, m, h, rho, cols, kernel=kernel, code_bits=code_bits, precision=precision)
