@@ -9,8 +9,10 @@
 //   6. Karras range/split + parents        LBVH/BinaryRadixTree/toolbox.jl:14-117
 //   7. adjacent LCP, internal escapes      BinaryRadixTree.jl:111-121, toolbox.jl:119-134
 //   8. leaf escapes                        BinaryRadixTree.jl:124-126, toolbox.jl:136-156
-//   9. bottom-up AABB / max-h refit with   LBVH/LinearBVH/LinearBVH.jl:77-91, toolbox.jl:130-188
-//      agent-scope acq_rel "second arrival" counters, then a visit-count check.
+//   9. bottom-up AABB / max-h refit        LBVH/LinearBVH/LinearBVH.jl:77-91, toolbox.jl:130-188
+//      ("second arrival" combines): nodes inside a workgroup's 2048-leaf chunk
+//      count arrivals in LDS; the few nodes spanning chunks are finished by a
+//      second pass with agent-scope acq_rel counters; then a visit-count check.
 // Tree arrays keep the reference's unified node IDs so they compare with memcmp.
 #include "ph_build.h"
 #include "ph_primitives.h"
@@ -214,42 +216,87 @@ __global__ void k_fill_u32(u32* p, i64 n, u32 v) {
     if (i < n) p[i] = v;
 }
 
-// Karras bottom-up refit: the second thread to reach a node combines its children.
+// Combines the two children of internal node p into nodes[p-1].
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_refit(const LeafRec<T>* __restrict__ leaves, const i32* __restrict__ parent,
-                                          const i32* __restrict__ left, const i32* __restrict__ right, i64 n,
-                                          NodeRec<T>* nodes, u32* visited) {
-    const i64 k = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
-    if (k > n) return;
-    i32 p = parent[n - 1 + k - 1];
-    for (i64 depth = n; p != 0 && depth > 0; --depth) {  // a tree is at most n-1 levels deep
-        const u32 old = __hip_atomic_fetch_add(&visited[p - 1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old != 1u) break;  // first arrival: the sibling subtree is not ready yet
-        const i32 l = left[p - 1], r = right[p - 1];
-        T lmin[3], lmax[3], rmin[3], rmax[3], hl, hr;
-        if (l >= (i32)n) {
-            const LeafRec<T> L = leaves[l - n];
-            lmin[0] = lmax[0] = L.x; lmin[1] = lmax[1] = L.y; lmin[2] = lmax[2] = L.z; hl = L.h;
-        } else {
-            const NodeRec<T>* N = &nodes[l - 1];
-            for (int d = 0; d < 3; ++d) { lmin[d] = N->lo[d]; lmax[d] = N->hi[d]; }
-            hl = N->scale;
+__device__ inline void combine_children(const LeafRec<T>* __restrict__ leaves, const i32* __restrict__ left,
+                                        const i32* __restrict__ right, i64 n, NodeRec<T>* nodes, i32 p) {
+    const i32 l = left[p - 1], r = right[p - 1];
+    T lmin[3], lmax[3], rmin[3], rmax[3], hl, hr;
+    if (l >= (i32)n) {
+        const LeafRec<T> L = leaves[l - n];
+        lmin[0] = lmax[0] = L.x; lmin[1] = lmax[1] = L.y; lmin[2] = lmax[2] = L.z; hl = L.h;
+    } else {
+        const NodeRec<T>* N = &nodes[l - 1];
+        for (int d = 0; d < 3; ++d) { lmin[d] = N->lo[d]; lmax[d] = N->hi[d]; }
+        hl = N->scale;
+    }
+    if (r >= (i32)n) {
+        const LeafRec<T> R = leaves[r - n];
+        rmin[0] = rmax[0] = R.x; rmin[1] = rmax[1] = R.y; rmin[2] = rmax[2] = R.z; hr = R.h;
+    } else {
+        const NodeRec<T>* N = &nodes[r - 1];
+        for (int d = 0; d < 3; ++d) { rmin[d] = N->lo[d]; rmax[d] = N->hi[d]; }
+        hr = N->scale;
+    }
+    NodeRec<T>* P = &nodes[p - 1];
+    for (int d = 0; d < 3; ++d) {
+        P->lo[d] = (lmin[d] < rmin[d]) ? lmin[d] : rmin[d];
+        P->hi[d] = (lmax[d] > rmax[d]) ? lmax[d] : rmax[d];
+    }
+    P->scale = (hl > hr) ? hl : hr;
+}
+
+constexpr int kRefitChunk = kBlock * 8;  // leaves per workgroup in the local refit
+
+// Refit, phase 1: workgroup b owns leaves [c0, c1] (1-based) and refits every
+// internal node whose leaf range lies inside it, counting arrivals in LDS
+// (workgroup scope, no cross-XCD cache maintenance).  A node whose parent
+// spans another chunk is appended to `starts` for phase 2.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_refit_local(const LeafRec<T>* __restrict__ leaves, const i32* __restrict__ parent,
+                                                const i32* __restrict__ left, const i32* __restrict__ right,
+                                                const i32* __restrict__ range, i64 n, NodeRec<T>* nodes, u32* visited,
+                                                i32* starts, u32* nstarts) {
+    __shared__ u32 cnt[kRefitChunk];
+    const i64 c0 = (i64)blockIdx.x * kRefitChunk + 1;
+    const i64 c1 = c0 + kRefitChunk - 1 < n ? c0 + kRefitChunk - 1 : n;
+    for (int j = threadIdx.x; j < kRefitChunk; j += kBlock) cnt[j] = 0u;
+    __syncthreads();
+    for (i64 k = c0 + threadIdx.x; k <= c1; k += kBlock) {
+        i32 child = (i32)(n - 1 + k);
+        i32 p = parent[child - 1];
+        for (i64 depth = n; p != 0 && depth > 0; --depth) {
+            const i64 lo = range[2 * (p - 1)], hi = range[2 * (p - 1) + 1];
+            if (lo < c0 || hi > c1) {  // p spans chunks: phase 2 climbs from `child`
+                const u32 slot = atomicAdd(nstarts, 1u);
+                starts[slot] = child;
+                break;
+            }
+            const u32 old = __hip_atomic_fetch_add(&cnt[p - c0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old != 1u) break;  // first arrival
+            combine_children<T>(leaves, left, right, n, nodes, p);
+            visited[p - 1] = 2u;
+            child = p;
+            p = parent[p - 1];
         }
-        if (r >= (i32)n) {
-            const LeafRec<T> R = leaves[r - n];
-            rmin[0] = rmax[0] = R.x; rmin[1] = rmax[1] = R.y; rmin[2] = rmax[2] = R.z; hr = R.h;
-        } else {
-            const NodeRec<T>* N = &nodes[r - 1];
-            for (int d = 0; d < 3; ++d) { rmin[d] = N->lo[d]; rmax[d] = N->hi[d]; }
-            hr = N->scale;
+    }
+}
+
+// Refit, phase 2: climbs from the recorded subtree roots through the nodes that
+// span chunks, with agent-scope acq_rel counters (visited[] zero for them).
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_refit_top(const LeafRec<T>* __restrict__ leaves, const i32* __restrict__ parent,
+                                              const i32* __restrict__ left, const i32* __restrict__ right, i64 n,
+                                              NodeRec<T>* nodes, u32* visited, const i32* starts, const u32* nstarts) {
+    const u32 total = *nstarts;
+    for (u32 t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
+        i32 p = parent[starts[t] - 1];
+        for (i64 depth = n; p != 0 && depth > 0; --depth) {
+            const u32 old = __hip_atomic_fetch_add(&visited[p - 1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (old != 1u) break;
+            combine_children<T>(leaves, left, right, n, nodes, p);
+            p = parent[p - 1];
         }
-        NodeRec<T>* P = &nodes[p - 1];
-        for (int d = 0; d < 3; ++d) {
-            P->lo[d] = (lmin[d] < rmin[d]) ? lmin[d] : rmin[d];
-            P->hi[d] = (lmax[d] > rmax[d]) ? lmax[d] : rmax[d];
-        }
-        P->scale = (hl > hr) ? hl : hr;
-        p = parent[p - 1];
     }
 }
 
@@ -324,15 +371,23 @@ void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipS
     k_leaf_escape<T><<<g, kBlock, 0, s>>>(B->parent.as<i32>(), B->left.as<i32>(), B->right.as<i32>(), n,
                                           B->escape.as<i32>(), B->leaf_escape.as<i32>(), B->nodes.as<NodeRec<T>>());
     PH_HIP(hipGetLastError());
-    // 9: refit
+    // 9: refit (phase 1 per chunk in LDS, phase 2 across chunks)
     if (n >= 2) {
-        W.visited.alloc(sizeof(u32) * (n - 1) + sizeof(u32) * 64);
+        W.visited.alloc(sizeof(u32) * (n - 1) + sizeof(u32) * 64 + sizeof(i32) * n);
         u32* visited = W.visited.as<u32>();
         u32* bad = visited + (n - 1);
-        k_fill_u32<<<grid_for(n - 1 + 1), kBlock, 0, s>>>(visited, n - 1 + 1, 0u);
+        u32* nstarts = bad + 1;
+        i32* starts = reinterpret_cast<i32*>(visited + (n - 1) + 64);
+        k_fill_u32<<<grid_for(n - 1 + 2), kBlock, 0, s>>>(visited, n - 1 + 2, 0u);
         PH_HIP(hipGetLastError());
-        k_refit<T><<<g, kBlock, 0, s>>>(B->leaves.as<LeafRec<T>>(), B->parent.as<i32>(), B->left.as<i32>(),
-                                        B->right.as<i32>(), n, B->nodes.as<NodeRec<T>>(), visited);
+        const int nchunks = (int)((n + kRefitChunk - 1) / kRefitChunk);
+        k_refit_local<T><<<nchunks, kBlock, 0, s>>>(B->leaves.as<LeafRec<T>>(), B->parent.as<i32>(), B->left.as<i32>(),
+                                                    B->right.as<i32>(), B->range.as<i32>(), n,
+                                                    B->nodes.as<NodeRec<T>>(), visited, starts, nstarts);
+        PH_HIP(hipGetLastError());
+        k_refit_top<T><<<grid_for(n, kBlock, 1024), kBlock, 0, s>>>(B->leaves.as<LeafRec<T>>(), B->parent.as<i32>(),
+                                                                    B->left.as<i32>(), B->right.as<i32>(), n,
+                                                                    B->nodes.as<NodeRec<T>>(), visited, starts, nstarts);
         PH_HIP(hipGetLastError());
         k_check_visited<<<grid_for(n - 1), kBlock, 0, s>>>(visited, n - 1, bad);
         PH_HIP(hipGetLastError());

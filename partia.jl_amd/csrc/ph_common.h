@@ -88,7 +88,7 @@ struct ph_context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string last_error;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // 0-1 call, 2-3 build, 4-5 kernel
     ph_stats stats{};
     int count_neighbours = 0;
     ph::DevBuf lut64, lut32;   // line-integrated tables [6][4096]
@@ -114,6 +114,8 @@ struct ph_lbvh {
     ph::DevBuf nodes;   // NodeRec<T> [n-1]
     ph::DevBuf leaves;  // LeafRec<T> [n]
     ph::DevBuf leaf_escape;  // i32 [n]
+    ph::DevBuf rec;          // packed per-particle records of the last interpolation call
+    ph::DevBuf ha;           // per-sample smoothing length of the nearest particle
     // FP32-compensated view, built on first use from the float64 columns.
     bool c32_ready = false;
     ph::DevBuf c32_nodes, c32_leaves, c32_attr;

@@ -6,18 +6,24 @@
 namespace ph {
 
 // One catalog chunk as the kernels see it (capacities fixed at compile time).
+// The kernels read a packed per-particle record of width rw holding m, rho and
+// the distinct columns the chunk uses; *pos fields index into that record.
 constexpr int kMaxS = 4, kMaxG = 1, kMaxD = 1, kMaxC = 1;
+constexpr int kMaxRec = 2 + kMaxS + kMaxG + 3 * kMaxD + 3 * kMaxC + 1;  // 14 (even)
 struct CatChunk {
     i32 ns, ng, nd, nc;
-    i32 sslot[kMaxS];
     i32 sflag[kMaxS];
     i32 sout[kMaxS];  // output index of each entry
-    i32 gslot[kMaxG];
+    i32 spos[kMaxS];  // record positions
     i32 gout[kMaxG];
-    i32 dslot[3 * kMaxD];
+    i32 gpos[kMaxG];  // -1: density gradient (reference grad slot 0)
     i32 dout[kMaxD];
-    i32 cslot[3 * kMaxC];
+    i32 dpos[3 * kMaxD];
     i32 cout[kMaxC];
+    i32 cpos[3 * kMaxC];
+    i32 rw;              // record width (even, >= 2)
+    i32 nsrc;            // columns packed into the record
+    i32 rsrc[kMaxRec];   // source column (0-based index into x,y,z,m,h,rho,quant...) of each record slot
 };
 
 // Splits a ph_catalog into chunks; returns L.
@@ -46,10 +52,12 @@ struct LineGrid {
 };
 
 // Point samples (list or grid); out is L x S of T (double for FP64, float for FP32 / FP32C).
+// ev (optional, 2 events) brackets the dominant traversal + accumulation kernel.
 void launch_points(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl,
-                   const GridAxes* ga, i64 S, int L, void* out, u64* counters, hipStream_t s);
+                   const GridAxes* ga, i64 S, int L, void* out, u64* counters, hipStream_t s,
+                   hipEvent_t* ev = nullptr);
 void launch_lines(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const LineList* ll, const LineGrid* lg,
-                  i64 S, int L, void* out, u64* counters, const void* lut, hipStream_t s);
+                  i64 S, int L, void* out, u64* counters, const void* lut, hipStream_t s, hipEvent_t* ev = nullptr);
 void launch_nearest_h(ph_lbvh* B, const PointList& pl, i64 S, double* ha, hipStream_t s);
 // FP32-compensated view of B (nodes rounded outward, leaf offsets, attribute records).
 void prepare_c32(ph_lbvh* B, hipStream_t s);

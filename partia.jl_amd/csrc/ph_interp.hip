@@ -16,6 +16,7 @@
 // so the 64 lanes of a wave walk nearly the same nodes; workgroups are remapped
 // so that each of the 8 XCDs (separate L2 caches) owns a contiguous run of tiles.
 #include "ph_interp.h"
+#include "ph_primitives.h"
 #include "ph_sph.h"
 
 namespace ph {
@@ -27,23 +28,34 @@ int split_catalog(const ph_catalog* cat, std::vector<CatChunk>& chunks) {
     int si = 0, gi = 0, di = 0, ci = 0;
     while (si < N || gi < G || di < D || ci < C || chunks.empty()) {
         CatChunk c{};
+        c.rsrc[0] = 3;  // m
+        c.rsrc[1] = 5;  // rho
+        c.nsrc = 2;
+        auto pos_of = [&](int slot) {  // record position of a 1-based quantity column
+            const int src = 5 + slot;
+            for (int j = 2; j < c.nsrc; ++j)
+                if (c.rsrc[j] == src) return j;
+            c.rsrc[c.nsrc] = src;
+            return c.nsrc++;
+        };
         for (; c.ns < kMaxS && si < N; ++si, ++c.ns) {
-            c.sslot[c.ns] = cat->scalar_slots[si];
+            c.spos[c.ns] = pos_of(cat->scalar_slots[si]);
             c.sflag[c.ns] = cat->scalar_shepard[si] ? 1 : 0;
             c.sout[c.ns] = si;
         }
         for (; c.ng < kMaxG && gi < G; ++gi, ++c.ng) {
-            c.gslot[c.ng] = cat->grad_slots[gi];
+            c.gpos[c.ng] = cat->grad_slots[gi] == 0 ? -1 : pos_of(cat->grad_slots[gi]);
             c.gout[c.ng] = N + 3 * gi;
         }
         for (; c.nd < kMaxD && di < D; ++di, ++c.nd) {
-            for (int d = 0; d < 3; ++d) c.dslot[3 * c.nd + d] = cat->div_slots[3 * di + d];
+            for (int d = 0; d < 3; ++d) c.dpos[3 * c.nd + d] = pos_of(cat->div_slots[3 * di + d]);
             c.dout[c.nd] = N + 3 * G + di;
         }
         for (; c.nc < kMaxC && ci < C; ++ci, ++c.nc) {
-            for (int d = 0; d < 3; ++d) c.cslot[3 * c.nc + d] = cat->curl_slots[3 * ci + d];
+            for (int d = 0; d < 3; ++d) c.cpos[3 * c.nc + d] = pos_of(cat->curl_slots[3 * ci + d]);
             c.cout[c.nc] = N + 3 * G + D + 3 * ci;
         }
+        c.rw = (c.nsrc + 1) & ~1;
         chunks.push_back(c);
         if (L == 0) break;
     }
@@ -191,30 +203,38 @@ __device__ inline void dfs(const TreeView<T>& tv, T Kvalid, T r2a, Dist leaf_d2,
     }
 }
 
-// LBVH_find_nearest_h (LinearBVHQueries.jl:118-140): shrinking bound, strict `<`.
+// LBVH_find_nearest_h (LinearBVHQueries.jl:118-140).  The reference walks the
+// DFS from an infinite bound and keeps the first leaf (in DFS = Morton order) of
+// minimal d2, i.e. the lexicographic minimum of (d2, leaf index).  Here a greedy
+// descent (nearer child first) supplies a starting leaf and bound, then the
+// escape-rope DFS prunes with `d2_node > best` and replaces the candidate when
+// d2 < best or (d2 == best and the index is smaller): the same leaf, found after
+// visiting far fewer nodes.
 template <class T> __device__ inline T nearest_h(const TreeView<T>& tv, T px, T py, T pz) {
     const i64 n = tv.n;
-    T best = T(INFINITY), hbest = sph_nan<T>();
-    if (tv.root == 0) {
-        for (i64 k = 0; k < n; ++k) {
-            const LeafRec<T> L = tv.leaves[k];
-            const T d2 = d2_point(px, py, pz, L);
-            if (d2 < best) {
-                best = d2;
-                hbest = L.h;
-            }
-        }
-        return hbest;
-    }
+    if (tv.root == 0) return tv.leaves[0].h;  // n == 1
+    // greedy descent to a nearby leaf
     i32 node = tv.root;
+    for (i64 steps = n; node < (i32)n && steps > 0; --steps) {
+        const NodeRec<T> N = tv.nodes[node - 1];
+        const i32 l = N.left;
+        // the right child is the node after the left subtree: left's escape for a
+        // leaf, or the escape of the left internal node
+        const i32 r = (l >= (i32)n) ? tv.leaf_escape[l - n] : tv.nodes[l - 1].escape;
+        const T dl = (l >= (i32)n) ? d2_point(px, py, pz, tv.leaves[l - n]) : d2_aabb(px, py, pz, tv.nodes[l - 1]);
+        const T dr = (r >= (i32)n) ? d2_point(px, py, pz, tv.leaves[r - n]) : d2_aabb(px, py, pz, tv.nodes[r - 1]);
+        node = (dr < dl) ? r : l;
+    }
+    i64 bk = (i64)node - n;
+    T best = d2_point(px, py, pz, tv.leaves[bk]);
+    node = tv.root;
     for (i64 steps = 2 * n + 8; node != 0 && steps > 0; --steps) {
         if (node >= (i32)n) {
             const i64 k = (i64)node - n;
-            const LeafRec<T> L = tv.leaves[k];
-            const T d2 = d2_point(px, py, pz, L);
-            if (d2 < best) {
+            const T d2 = d2_point(px, py, pz, tv.leaves[k]);
+            if (d2 < best || (d2 == best && k < bk)) {
                 best = d2;
-                hbest = L.h;
+                bk = k;
             }
             node = tv.leaf_escape[k];
         } else {
@@ -222,7 +242,7 @@ template <class T> __device__ inline T nearest_h(const TreeView<T>& tv, T px, T 
             node = (d2_aabb(px, py, pz, N) > best) ? N.escape : N.left;
         }
     }
-    return hbest;
+    return tv.leaves[bk].h;
 }
 
 // ---- accumulation (general_interpolation.jl + accumulations/*.jl) ----
@@ -270,16 +290,15 @@ template <class T, int K, int STRAT, int MS, int MG, int MD, int MC> struct Acc 
         }
     }
 
-    // One accepted neighbour i (0-based Morton index).
-    __device__ inline void add(const CatChunk& c, const T* __restrict__ cols, i64 n, i64 i, T dx, T dy, T dz, T d2,
-                               T ha, T hb) {
+    // One accepted neighbour with packed record r = {m, rho, columns...}.
+    __device__ inline void add(const CatChunk& c, const T* __restrict__ r, T dx, T dy, T dz, T d2, T ha, T hb) {
         const T dr = sqrt(d2);
-        const T mb = cols[3 * n + i], rb = cols[5 * n + i];
+        const T mb = r[0], rb = r[1];
         const T W = Wsel(dr, ha, hb);
         S1 += mb * W / rb;
 #pragma unroll
         for (int j = 0; j < MS; ++j)
-            if (j < c.ns) sc[j] += cols[(5 + c.sslot[j]) * n + i] * (mb * W / rb);
+            if (j < c.ns) sc[j] += r[c.spos[j]] * (mb * W / rb);
         if constexpr (MG + MD + MC > 0) {
             if (c.ng + c.nd + c.nc == 0) return;
             T g[3];
@@ -288,8 +307,8 @@ template <class T, int K, int STRAT, int MS, int MG, int MD, int MC> struct Acc 
 #pragma unroll
             for (int j = 0; j < MG; ++j) {
                 if (j >= c.ng) break;
-                const int slot = c.gslot[j];
-                if (slot == 0) {  // density gradient (gradient_accumulation.jl:7-35)
+                const int pos = c.gpos[j];
+                if (pos < 0) {  // density gradient (gradient_accumulation.jl:7-35)
                     for (int d = 0; d < 3; ++d) {
                         const T mg = mb * g[d];
                         gf[j][d] += mg;
@@ -297,7 +316,7 @@ template <class T, int K, int STRAT, int MS, int MG, int MD, int MC> struct Acc 
                     }
                     gs[j] += mb * W;
                 } else {
-                    const T A = cols[(5 + slot) * n + i];
+                    const T A = r[pos];
                     for (int d = 0; d < 3; ++d) {
                         const T w = mb * invr * g[d];
                         gf[j][d] += w * A;
@@ -311,7 +330,7 @@ template <class T, int K, int STRAT, int MS, int MG, int MD, int MC> struct Acc 
                 if (j >= c.nd) break;
                 T A[3], w[3];
                 for (int d = 0; d < 3; ++d) {
-                    A[d] = cols[(5 + c.dslot[3 * j + d]) * n + i];
+                    A[d] = r[c.dpos[3 * j + d]];
                     w[d] = mb * invr * g[d];
                 }
                 df[j] += w[0] * A[0] + w[1] * A[1] + w[2] * A[2];
@@ -325,7 +344,7 @@ template <class T, int K, int STRAT, int MS, int MG, int MD, int MC> struct Acc 
                 if (j >= c.nc) break;
                 T A[3], w[3];
                 for (int d = 0; d < 3; ++d) {
-                    A[d] = cols[(5 + c.cslot[3 * j + d]) * n + i];
+                    A[d] = r[c.cpos[3 * j + d]];
                     w[d] = mb * invr * g[d];
                 }
                 cf[j][0] += A[1] * w[2] - A[2] * w[1];
@@ -428,31 +447,76 @@ __device__ inline void count_add(u64* counter, u64 v) {
     if ((threadIdx.x & (kWave - 1)) == 0 && v) atomicAdd((unsigned long long*)counter, (unsigned long long)v);
 }
 
-template <class T, int K, int STRAT, int MS, int MG, int MD, int MC, int SRC>
-__global__ void __launch_bounds__(kBlock) k_points(TreeView<T> tv, const T* __restrict__ cols, CatChunk c,
-                                                   Sampler<T, SRC> smp, T* __restrict__ out, u64* counter) {
+// Packs {m, rho, columns...} of every particle into records of width c.rw.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pack_records(const T* __restrict__ cols, i64 n, CatChunk c, T* __restrict__ rec) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    T* r = rec + i * c.rw;
+    for (int j = 0; j < c.rw; ++j) r[j] = j < c.nsrc ? cols[(i64)c.rsrc[j] * n + i] : T(0);
+}
+
+constexpr int kBatch = 16;  // accepted leaves buffered per lane between accumulation rounds
+
+// Point samples.  Each lane walks its own escape-rope DFS, buffering accepted
+// leaves (in DFS order) in LDS until it holds kBatch of them or is done; then
+// the whole wave accumulates its buffers in lockstep.  The expensive per-neighbour
+// arithmetic thus runs without the traversal's divergence, and each sample still
+// accumulates its neighbours in the reference's order.
+// Smoothing length of the nearest particle of every sample (gather / symmetric).
+template <class T, int SRC>
+__global__ void __launch_bounds__(kBlock) k_nearest_samples(TreeView<T> tv, Sampler<T, SRC> smp, T* __restrict__ ha) {
     i64 s;
     T px, py, pz;
+    if (smp.get(s, px, py, pz)) ha[s] = nearest_h<T>(tv, px, py, pz);
+}
+
+template <class T, int K, int STRAT, int MS, int MG, int MD, int MC, int SRC>
+__global__ void __launch_bounds__(kBlock) k_points(TreeView<T> tv, const T* __restrict__ rec, CatChunk c,
+                                                   Sampler<T, SRC> smp, const T* __restrict__ ha_s, T* __restrict__ out,
+                                                   u64* counter) {
+    __shared__ i32 buf[kBatch][kBlock];
+    const int tid = threadIdx.x;
+    i64 s = 0;
+    T px = T(0), py = T(0), pz = T(0);
     const bool valid = smp.get(s, px, py, pz);
-    u64 accepted = 0;
-    if (valid) {
-        const T Kvalid = T(support_of<K>());
-        T ha = T(0);
-        if constexpr (STRAT != PH_SCATTER) ha = nearest_h<T>(tv, px, py, pz);
-        const T ra = Kvalid * ha;
-        const T r2a = ra * ra;
-        Acc<T, K, STRAT, MS, MG, MD, MC> acc;
-        acc.init();
-        const i64 n = tv.n;
-        dfs<T, STRAT>(
-            tv, Kvalid, r2a, [&](const LeafRec<T>& L) { return d2_point(px, py, pz, L); },
-            [&](const NodeRec<T>& N) { return d2_aabb(px, py, pz, N); },
-            [&](i64 k, T d2, const LeafRec<T>& L) {
-                ++accepted;
-                acc.add(c, cols, n, k, px - L.x, py - L.y, pz - L.z, d2, ha, L.h);
-            });
-        acc.finish(c, out, smp.S, s);
+    const i64 n = tv.n;
+    const T Kvalid = T(support_of<K>());
+    T ha = T(0);
+    if constexpr (STRAT != PH_SCATTER) {
+        if (valid) ha = ha_s[s];
     }
+    const T ra = Kvalid * ha;
+    const T r2a = ra * ra;
+    Acc<T, K, STRAT, MS, MG, MD, MC> acc;
+    acc.init();
+    // n == 1 has no internal node (root 0): start at the single leaf, whose escape is 0.
+    i32 node = valid ? (tv.root != 0 ? tv.root : (i32)n) : 0;
+    i64 steps = 2 * n + 8;  // a valid tree is walked in at most 2n-1 steps
+    u64 accepted = 0;
+    while (true) {
+        int cnt = 0;
+        while (node != 0 && steps > 0 && cnt < kBatch) {
+            --steps;
+            if (node >= (i32)n) {
+                const i64 k = (i64)node - n;
+                const LeafRec<T> L = tv.leaves[k];
+                if (d2_point(px, py, pz, L) <= radius2<T, STRAT>(Kvalid, r2a, L.h)) buf[cnt++][tid] = (i32)k;
+                node = tv.leaf_escape[k];
+            } else {
+                const NodeRec<T> N = tv.nodes[node - 1];
+                node = (d2_aabb(px, py, pz, N) > radius2<T, STRAT>(Kvalid, r2a, N.scale)) ? N.escape : N.left;
+            }
+        }
+        accepted += cnt;
+        for (int j = 0; j < cnt; ++j) {
+            const i64 k = buf[j][tid];
+            const LeafRec<T> L = tv.leaves[k];
+            acc.add(c, rec + k * c.rw, px - L.x, py - L.y, pz - L.z, d2_point(px, py, pz, L), ha, L.h);
+        }
+        if (!__any(node != 0 && steps > 0)) break;
+    }
+    if (valid) acc.finish(c, out, smp.S, s);
     count_add(counter, accepted);
 }
 
@@ -514,7 +578,7 @@ __global__ void __launch_bounds__(kBlock) k_lines(TreeView<T> tv, const T* __res
                 S1 += mb * W / rb;
 #pragma unroll
                 for (int j = 0; j < kMaxS; ++j)
-                    if (j < c.ns) acc[j] += cols[(5 + c.sslot[j]) * n + k] * (mb * W / rb);
+                    if (j < c.ns) acc[j] += cols[c.rsrc[c.spos[j]] * n + k] * (mb * W / rb);
             });
         if (S1 == T(0)) {
 #pragma unroll
@@ -547,61 +611,81 @@ inline unsigned blocks_for_points(int src, i64 S, const GridAxes* ga) {
 
 template <class T, int K, int STRAT, int MS, int MG, int MD, int MC>
 void launch_points_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const GridAxes* ga, i64 S, void* out,
-                     u64* counters, hipStream_t s) {
+                     u64* counters, hipStream_t s, hipEvent_t* ev) {
     const TreeView<T> tv = view_of<T>(B);
-    const T* cols = B->cols.as<T>();
+    B->rec.alloc(sizeof(T) * (size_t)c.rw * (size_t)B->n);
+    const T* cols = B->rec.as<T>();
+    k_pack_records<T><<<grid_for(B->n), kBlock, 0, s>>>(B->cols.as<T>(), B->n, c, B->rec.as<T>());
+    PH_HIP(hipGetLastError());
+    B->ha.alloc(sizeof(T) * (size_t)(S > 0 ? S : 1));
+    T* ha = B->ha.as<T>();
     if (ga) {
         Sampler<T, 1> smp{*ga, S};
-        k_points<T, K, STRAT, MS, MG, MD, MC, 1><<<blocks_for_points(1, S, ga), kBlock, 0, s>>>(
-            tv, cols, c, smp, static_cast<T*>(out), counters);
+        const unsigned g = blocks_for_points(1, S, ga);
+        if (STRAT != PH_SCATTER) {
+            k_nearest_samples<T, 1><<<g, kBlock, 0, s>>>(tv, smp, ha);
+            PH_HIP(hipGetLastError());
+        }
+        if (ev) PH_HIP(hipEventRecord(ev[0], s));
+        k_points<T, K, STRAT, MS, MG, MD, MC, 1><<<g, kBlock, 0, s>>>(tv, cols, c, smp, ha, static_cast<T*>(out),
+                                                                     counters);
+        if (ev) PH_HIP(hipEventRecord(ev[1], s));
     } else {
         Sampler<T, 0> smp{*pl, S};
-        k_points<T, K, STRAT, MS, MG, MD, MC, 0><<<blocks_for_points(0, S, nullptr), kBlock, 0, s>>>(
-            tv, cols, c, smp, static_cast<T*>(out), counters);
+        const unsigned g = blocks_for_points(0, S, nullptr);
+        if (STRAT != PH_SCATTER) {
+            k_nearest_samples<T, 0><<<g, kBlock, 0, s>>>(tv, smp, ha);
+            PH_HIP(hipGetLastError());
+        }
+        if (ev) PH_HIP(hipEventRecord(ev[0], s));
+        k_points<T, K, STRAT, MS, MG, MD, MC, 0><<<g, kBlock, 0, s>>>(tv, cols, c, smp, ha, static_cast<T*>(out),
+                                                                     counters);
+        if (ev) PH_HIP(hipEventRecord(ev[1], s));
     }
     PH_HIP(hipGetLastError());
 }
 
 template <class T, int K, int STRAT>
 void launch_points_k(ph_lbvh* B, const CatChunk& c, const PointList* pl, const GridAxes* ga, i64 S, void* out,
-                     u64* counters, hipStream_t s) {
+                     u64* counters, hipStream_t s, hipEvent_t* ev) {
     if (c.ng + c.nd + c.nc == 0 && c.ns <= 2)
-        launch_points_t<T, K, STRAT, 2, 0, 0, 0>(B, c, pl, ga, S, out, counters, s);
+        launch_points_t<T, K, STRAT, 2, 0, 0, 0>(B, c, pl, ga, S, out, counters, s, ev);
     else
-        launch_points_t<T, K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC>(B, c, pl, ga, S, out, counters, s);
+        launch_points_t<T, K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC>(B, c, pl, ga, S, out, counters, s, ev);
 }
 
 template <class T, int K>
 void launch_points_s(ph_lbvh* B, const CatChunk& c, int strategy, const PointList* pl, const GridAxes* ga, i64 S,
-                     void* out, u64* counters, hipStream_t s) {
+                     void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
     switch (strategy) {
-    case PH_GATHER: launch_points_k<T, K, PH_GATHER>(B, c, pl, ga, S, out, counters, s); break;
-    case PH_SCATTER: launch_points_k<T, K, PH_SCATTER>(B, c, pl, ga, S, out, counters, s); break;
-    default: launch_points_k<T, K, PH_SYMMETRIC>(B, c, pl, ga, S, out, counters, s); break;
+    case PH_GATHER: launch_points_k<T, K, PH_GATHER>(B, c, pl, ga, S, out, counters, s, ev); break;
+    case PH_SCATTER: launch_points_k<T, K, PH_SCATTER>(B, c, pl, ga, S, out, counters, s, ev); break;
+    default: launch_points_k<T, K, PH_SYMMETRIC>(B, c, pl, ga, S, out, counters, s, ev); break;
     }
 }
 
 template <class T>
 void launch_points_p(ph_lbvh* B, const CatChunk& c, int strategy, const PointList* pl, const GridAxes* ga, i64 S,
-                     void* out, u64* counters, hipStream_t s) {
+                     void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
     switch (B->kernel) {
-    case PH_KERNEL_M4: launch_points_s<T, PH_KERNEL_M4>(B, c, strategy, pl, ga, S, out, counters, s); break;
-    case PH_KERNEL_M5: launch_points_s<T, PH_KERNEL_M5>(B, c, strategy, pl, ga, S, out, counters, s); break;
-    case PH_KERNEL_M6: launch_points_s<T, PH_KERNEL_M6>(B, c, strategy, pl, ga, S, out, counters, s); break;
-    case PH_KERNEL_C2: launch_points_s<T, PH_KERNEL_C2>(B, c, strategy, pl, ga, S, out, counters, s); break;
-    case PH_KERNEL_C4: launch_points_s<T, PH_KERNEL_C4>(B, c, strategy, pl, ga, S, out, counters, s); break;
-    default: launch_points_s<T, PH_KERNEL_C6>(B, c, strategy, pl, ga, S, out, counters, s); break;
+    case PH_KERNEL_M4: launch_points_s<T, PH_KERNEL_M4>(B, c, strategy, pl, ga, S, out, counters, s, ev); break;
+    case PH_KERNEL_M5: launch_points_s<T, PH_KERNEL_M5>(B, c, strategy, pl, ga, S, out, counters, s, ev); break;
+    case PH_KERNEL_M6: launch_points_s<T, PH_KERNEL_M6>(B, c, strategy, pl, ga, S, out, counters, s, ev); break;
+    case PH_KERNEL_C2: launch_points_s<T, PH_KERNEL_C2>(B, c, strategy, pl, ga, S, out, counters, s, ev); break;
+    case PH_KERNEL_C4: launch_points_s<T, PH_KERNEL_C4>(B, c, strategy, pl, ga, S, out, counters, s, ev); break;
+    default: launch_points_s<T, PH_KERNEL_C6>(B, c, strategy, pl, ga, S, out, counters, s, ev); break;
     }
 }
 
 template <class T, int K>
 void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const LineGrid* lg, i64 S, void* out,
-                    u64* counters, const T* inorm, hipStream_t s) {
+                    u64* counters, const T* inorm, hipStream_t s, hipEvent_t* ev) {
     const TreeView<T> tv = view_of<T>(B);
     const T* cols = B->cols.as<T>();
     // Q_2 - Q_1 of the table in T (line_integrated_kernel.jl:14)
     const T dq = T((1.0 * support_of<K>()) / 4095.0) - T(0.0);
     const T* tab = inorm + (size_t)K * PH_LUT_N_ENTRIES;
+    if (ev) PH_HIP(hipEventRecord(ev[0], s));
     if (lg) {
         LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S};
         const i64 tiles = ((lg->nx + 7) / 8) * ((lg->ny + 7) / 8);
@@ -613,40 +697,41 @@ void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const Lin
             tv, cols, c, smp, tab, dq, static_cast<T*>(out), counters);
     }
     PH_HIP(hipGetLastError());
+    if (ev) PH_HIP(hipEventRecord(ev[1], s));
 }
 
 template <class T>
 void launch_lines_p(ph_lbvh* B, const CatChunk& c, const LineList* ll, const LineGrid* lg, i64 S, void* out,
-                    u64* counters, const T* inorm, hipStream_t s) {
+                    u64* counters, const T* inorm, hipStream_t s, hipEvent_t* ev) {
     switch (B->kernel) {
-    case PH_KERNEL_M4: launch_lines_k<T, PH_KERNEL_M4>(B, c, ll, lg, S, out, counters, inorm, s); break;
-    case PH_KERNEL_M5: launch_lines_k<T, PH_KERNEL_M5>(B, c, ll, lg, S, out, counters, inorm, s); break;
-    case PH_KERNEL_M6: launch_lines_k<T, PH_KERNEL_M6>(B, c, ll, lg, S, out, counters, inorm, s); break;
-    case PH_KERNEL_C2: launch_lines_k<T, PH_KERNEL_C2>(B, c, ll, lg, S, out, counters, inorm, s); break;
-    case PH_KERNEL_C4: launch_lines_k<T, PH_KERNEL_C4>(B, c, ll, lg, S, out, counters, inorm, s); break;
-    default: launch_lines_k<T, PH_KERNEL_C6>(B, c, ll, lg, S, out, counters, inorm, s); break;
+    case PH_KERNEL_M4: launch_lines_k<T, PH_KERNEL_M4>(B, c, ll, lg, S, out, counters, inorm, s, ev); break;
+    case PH_KERNEL_M5: launch_lines_k<T, PH_KERNEL_M5>(B, c, ll, lg, S, out, counters, inorm, s, ev); break;
+    case PH_KERNEL_M6: launch_lines_k<T, PH_KERNEL_M6>(B, c, ll, lg, S, out, counters, inorm, s, ev); break;
+    case PH_KERNEL_C2: launch_lines_k<T, PH_KERNEL_C2>(B, c, ll, lg, S, out, counters, inorm, s, ev); break;
+    case PH_KERNEL_C4: launch_lines_k<T, PH_KERNEL_C4>(B, c, ll, lg, S, out, counters, inorm, s, ev); break;
+    default: launch_lines_k<T, PH_KERNEL_C6>(B, c, ll, lg, S, out, counters, inorm, s, ev); break;
     }
 }
 
 }  // namespace
 
 void launch_points(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl,
-                   const GridAxes* ga, i64 S, int L, void* out, u64* counters, hipStream_t s) {
+                   const GridAxes* ga, i64 S, int L, void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
     (void)L;
     if (S <= 0) return;
-    if (precision == PH_PREC_FP64) launch_points_p<double>(B, c, strategy, pl, ga, S, out, counters, s);
-    else launch_points_p<float>(B, c, strategy, pl, ga, S, out, counters, s);
+    if (precision == PH_PREC_FP64) launch_points_p<double>(B, c, strategy, pl, ga, S, out, counters, s, ev);
+    else launch_points_p<float>(B, c, strategy, pl, ga, S, out, counters, s, ev);
 }
 
 void launch_lines(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const LineList* ll, const LineGrid* lg,
-                  i64 S, int L, void* out, u64* counters, const void* lut, hipStream_t s) {
+                  i64 S, int L, void* out, u64* counters, const void* lut, hipStream_t s, hipEvent_t* ev) {
     (void)L;
     (void)strategy;
     if (S <= 0) return;
     if (precision == PH_PREC_FP64)
-        launch_lines_p<double>(B, c, ll, lg, S, out, counters, static_cast<const double*>(lut), s);
+        launch_lines_p<double>(B, c, ll, lg, S, out, counters, static_cast<const double*>(lut), s, ev);
     else
-        launch_lines_p<float>(B, c, ll, lg, S, out, counters, static_cast<const float*>(lut), s);
+        launch_lines_p<float>(B, c, ll, lg, S, out, counters, static_cast<const float*>(lut), s, ev);
 }
 
 void launch_nearest_h(ph_lbvh* B, const PointList& pl, i64 S, double* ha, hipStream_t s) {
