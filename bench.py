@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", choices=["fp64", "fp32c"], default="fp64")
+    ap.add_argument("--traversal", choices=["auto", "lane", "bucket"], default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
     return ap.parse_args()
@@ -128,6 +129,8 @@ def main():
     S = NGRID ** 3
 
     ctx = N.Context(0)
+    ctx.set_traversal({"auto": N.PH_TRAVERSAL_AUTO, "lane": N.PH_TRAVERSAL_LANE,
+                       "bucket": N.PH_TRAVERSAL_BUCKET}[args.traversal])
     L = N.lib()
     names = ["x", "y", "z", "m", "h", "rho", "rho", "A", "vx", "vy", "vz"]
     dcols = []
@@ -157,6 +160,7 @@ def main():
     ctx.set_count_neighbours(True)
     step()
     n_acc = ctx.stats()["accepted"]
+    n_rechecks = ctx.stats()["rechecks"]
     ctx.set_count_neighbours(False)
     print("bench: accepted neighbours per grid pass = %d (%.2f per sample)" % (n_acc, n_acc / S), file=sys.stderr,
           flush=True)
@@ -203,6 +207,8 @@ def main():
         "interp_samples_per_s": S / (interp_ms * 1e-3),
         "accumulation_kernel_ms": kernel_ms,
         "accepted_neighbours": int(n_acc),
+        "guard_band_rechecks": int(n_rechecks),
+        "traversal": args.traversal,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
     }

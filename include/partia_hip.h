@@ -106,6 +106,22 @@ const char* ph_last_error(const ph_context* ctx);
 int ph_get_stats(const ph_context* ctx, ph_stats* out);
 /* Count accepted neighbours during interpolation calls (adds one atomic per sample). */
 int ph_set_count_neighbours(ph_context* ctx, int enable);
+/* Traversal of point / grid interpolation calls on a float64 tree.  Both give the
+ * reference's neighbour sets in the reference's order (DFS preorder = increasing
+ * Morton index), so float64 results are identical:
+ *   PH_TRAVERSAL_LANE    one lane per sample, escape-rope DFS over the LBVH leaf
+ *                        by leaf (LinearBVHTraversalMacros.jl:26-371);
+ *   PH_TRAVERSAL_BUCKET  one lane per sample, DFS over a preorder copy of the
+ *                        LBVH whose subtrees of <= 16 leaves are buckets whose
+ *                        leaves are tested together (float32 pruning rounded
+ *                        outward, float64 leaf tests);
+ *   PH_TRAVERSAL_AUTO    bucket.
+ * PH_PREC_FP32C always uses the bucket traversal; Float32 trees the lane one.
+ * The bucket traversal supports up to 2^26 particles (PH_ERR_CAPACITY beyond). */
+#define PH_TRAVERSAL_AUTO 0
+#define PH_TRAVERSAL_LANE 1
+#define PH_TRAVERSAL_BUCKET 2
+int ph_set_traversal(ph_context* ctx, int mode);
 
 /* ---- device memory helpers (for callers without their own HIP allocator) ---- */
 int ph_device_alloc(ph_context* ctx, int64_t bytes, void** dptr);

@@ -66,7 +66,17 @@ void check_precision(const ph_lbvh* B, int precision) {
         need(B->precision == PH_PREC_FP32, PH_ERR_ARGUMENT, "PH_PREC_FP32 needs a tree built from Float32 columns");
     else
         need(B->precision == PH_PREC_FP64, PH_ERR_ARGUMENT, "float64 evaluation needs a tree built from Float64 columns");
-    need(precision != PH_PREC_FP32C, PH_ERR_ARGUMENT, "PH_PREC_FP32C is not available in this build");
+}
+
+// Point / grid launches: the bucket traversal for FP32C and (by default) float64,
+// the reference-structured per-leaf traversal on request and for Float32 trees.
+void launch_point_like(ph_context* ctx, ph_lbvh* B, const CatChunk& c, int strategy, int precision,
+                       const PointList* pl, const GridAxes* ga, i64 S, int L, void* out, u64* counter,
+                       hipEvent_t* ev) {
+    bool bucket = precision == PH_PREC_FP32C;
+    if (precision == PH_PREC_FP64) bucket = ctx->traversal != PH_TRAVERSAL_LANE;
+    if (bucket) launch_bucket(B, c, strategy, precision, pl, ga, S, out, counter, ctx->stream, ev);
+    else launch_points(B, c, strategy, precision, pl, ga, S, L, out, counter, ctx->stream, ev);
 }
 
 size_t out_elem(int precision) { return precision == PH_PREC_FP64 ? sizeof(double) : sizeof(float); }
@@ -101,10 +111,13 @@ template <class Launch> void run_chunks(ph_context* ctx, const ph_catalog* cat, 
     ctx->stats.kernel_ms = kms;
     ctx->stats.samples = S;
     ctx->stats.accepted = -1;
+    ctx->stats.rechecks = -1;
     if (counter) {
-        u64 h = 0;
-        PH_HIP(hipMemcpy(&h, counter, sizeof(u64), hipMemcpyDeviceToHost));
-        ctx->stats.accepted = (int64_t)(h / (chunks.empty() ? 1 : chunks.size()));
+        u64 h[2] = {0, 0};
+        PH_HIP(hipMemcpy(h, counter, sizeof(h), hipMemcpyDeviceToHost));
+        const u64 nch = chunks.empty() ? 1 : chunks.size();
+        ctx->stats.accepted = (int64_t)(h[0] / nch);
+        ctx->stats.rechecks = (int64_t)(h[1] / nch);
     }
 }
 
@@ -165,6 +178,12 @@ int ph_ctx_destroy(ph_context* ctx) {
     if (!ctx) return PH_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    // trees outliving their context keep a host handle without device memory
+    for (ph_lbvh* B : ctx->trees) {
+        B->release_device();
+        B->ctx = nullptr;
+    }
+    ctx->trees.clear();
     ctx->lut64.release();
     ctx->lut32.release();
     ctx->counters.release();
@@ -186,6 +205,14 @@ int ph_get_stats(const ph_context* ctx, ph_stats* out) {
 int ph_set_count_neighbours(ph_context* ctx, int enable) {
     if (!ctx) return PH_ERR_ARGUMENT;
     ctx->count_neighbours = enable ? 1 : 0;
+    return PH_OK;
+}
+
+int ph_set_traversal(ph_context* ctx, int mode) {
+    if (!ctx) return PH_ERR_ARGUMENT;
+    if (mode < PH_TRAVERSAL_AUTO || mode > PH_TRAVERSAL_BUCKET)
+        return fail(ctx, PH_ERR_ARGUMENT, "unknown traversal mode");
+    ctx->traversal = mode;
     return PH_OK;
 }
 
@@ -262,6 +289,7 @@ static int build_common(ph_context* ctx, int64_t n, const double* const* src, bo
         PH_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
         PH_HIP(hipEventSynchronize(ctx->ev[3]));
         ctx->stats.build_ms = elapsed(ctx, 2, 3);
+        ctx->trees.push_back(B.get());
         *out = B.release();
         return PH_OK;
     });
@@ -285,14 +313,22 @@ int ph_lbvh_build_device(ph_context* ctx, int64_t n, const double* x, const doub
 
 int ph_lbvh_destroy(ph_lbvh* B) {
     if (!B) return PH_OK;
-    (void)hipSetDevice(B->ctx->device);
-    (void)hipStreamSynchronize(B->ctx->stream);
+    if (ph_context* ctx = B->ctx) {  // null once the context was destroyed (memory already released)
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        auto& v = ctx->trees;
+        for (size_t t = 0; t < v.size(); ++t)
+            if (v[t] == B) {
+                v.erase(v.begin() + (std::ptrdiff_t)t);
+                break;
+            }
+    }
     delete B;
     return PH_OK;
 }
 
 int ph_lbvh_set_columns(ph_lbvh* B, const double* m, const double* rho, int32_t ncol, const double* const* cols) {
-    if (!B) return PH_ERR_ARGUMENT;
+    if (!B || !B->ctx) return fail(nullptr, PH_ERR_ARGUMENT, "null tree or tree of a destroyed context");
     ph_context* ctx = B->ctx;
     return guarded(ctx, [&] {
         need(m && rho && ncol >= 0 && (ncol == 0 || cols), PH_ERR_ARGUMENT, "null column pointer");
@@ -322,7 +358,6 @@ int ph_lbvh_set_columns(ph_lbvh* B, const double* m, const double* rho, int32_t 
         put(3, m);
         put(5, rho);
         for (int c = 0; c < ncol; ++c) put(6 + c, cols[c]);
-        B->c32_ready = false;
         return PH_OK;
     });
 }
@@ -338,7 +373,7 @@ int ph_lbvh_info(const ph_lbvh* B, int64_t* n, int32_t* root, int32_t* ncol, int
 }
 
 int ph_lbvh_get(ph_lbvh* B, int32_t which, void* dst) {
-    if (!B) return PH_ERR_ARGUMENT;
+    if (!B || !B->ctx) return fail(nullptr, PH_ERR_ARGUMENT, "null tree or tree of a destroyed context");
     ph_context* ctx = B->ctx;
     return guarded(ctx, [&] {
         need(dst != nullptr, PH_ERR_ARGUMENT, "destination is null");
@@ -415,7 +450,7 @@ int ph_point_samples_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, 
         need(S == 0 || (px && py && pz && out), PH_ERR_ARGUMENT, "null sample or output pointer");
         PointList pl{px, py, pz};
         run_chunks(ctx, cat, S, [&](const CatChunk& c, int L, u64* counter, hipEvent_t* ev) {
-            launch_points(B, c, strategy, precision, &pl, nullptr, S, L, out, counter, ctx->stream, ev);
+            launch_point_like(ctx, B, c, strategy, precision, &pl, nullptr, S, L, out, counter, ev);
         });
         return PH_OK;
     });
@@ -457,7 +492,7 @@ int ph_structured_grid_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat
         need(S == 0 || (ax && ay && az && out), PH_ERR_ARGUMENT, "null axis or output pointer");
         GridAxes ga{ax, ay, az, nx, ny, nz};
         run_chunks(ctx, cat, S, [&](const CatChunk& c, int L, u64* counter, hipEvent_t* ev) {
-            launch_points(B, c, strategy, precision, nullptr, &ga, S, L, out, counter, ctx->stream, ev);
+            launch_point_like(ctx, B, c, strategy, precision, nullptr, &ga, S, L, out, counter, ev);
         });
         return PH_OK;
     });

@@ -1,0 +1,827 @@
+// ph_bucket.hip -- bucket traversal for point and grid samples, in float64 and
+// in the FP32-compensated precision.
+//
+// Exactness argument.  The reference accepts leaf b for sample a iff
+// d2(a, b) <= r2(a, b) in the working precision (LinearBVHTraversalMacros.jl:85,
+// 209, 344) and accumulates accepted leaves in DFS preorder, i.e. in increasing
+// Morton index.  Node pruning only skips leaves that fail that test (a leaf lies
+// in its ancestors' boxes and its h is at most their scale), so any traversal
+// that enumerates a superset of the accepted leaves in increasing index order
+// and applies the same float64 leaf test gives the same neighbour sets in the
+// same order.  This file uses such a traversal:
+//
+//  * The LBVH is copied into a "bucket tree": every maximal subtree with at most
+//    kBucket leaves becomes one bucket (its leaves are consecutive in Morton
+//    order), and the remaining internal nodes and the buckets are stored in DFS
+//    preorder as 32-byte records with float32 boxes rounded outward and float32
+//    scales rounded up.  In preorder the left child of a record is the next
+//    record and a pruned subtree is skipped with one escape index, so a lane
+//    walks memory forwards only.
+//  * Each lane walks the bucket tree for its sample, pruning conservatively
+//    (the sample is rounded to float32 and the radius inflated to cover every
+//    rounding), and collects up to kQ buckets; then it tests their leaves with
+//    the reference's float64 rule (independent loads, no pointer chasing) into a
+//    64-bit mask; then the lanes accumulate their accepted leaves in lockstep,
+//    lowest bit first, i.e. in increasing leaf index.
+//
+// The float64 variant accumulates with the float64 code of the per-lane path, so
+// its results are bit-identical to the oracle.  The FP32-compensated variant
+// evaluates kernels and products in float32 on differences rounded once from
+// float64 (sample - particle), sums them per 16-leaf column chunk in float32 and
+// carries the chunk sums into float64 totals; every field column enters as an
+// offset from a float64 per-chunk anchor, so that
+//   sum_b A_b w_b = sum_k (Abar_k W_k + sum_{b in k} (A_b - Abar_k) w_b)
+// keeps the large part in float64.  The Shepard division and the mean
+// subtraction of the derivative operators (general_interpolation.jl:128-214) are
+// float64.
+#include "ph_interp.h"
+#include "ph_interp_dev.h"
+#include "ph_primitives.h"
+#include "ph_sph.h"
+
+namespace ph {
+namespace {
+
+constexpr int kBucket = 16;       // leaves per bucket at most (count stored in 5 bits)
+constexpr int kQ = 4;             // buckets tested per batch: kQ * kBucket = 64 mask bits
+constexpr i64 kMaxBucketLeaves = i64(1) << 26;  // leaf index of a bucket stored in 26 bits
+
+// Preorder record: link >= 0 -> internal node, link = index after its subtree;
+// link < 0 -> bucket of leaves [lo, lo + cnt), link = -(1 + (lo << 5 | (cnt - 1))).
+struct alignas(16) BucketRec {
+    float lo[3];
+    float hi[3];
+    float scale;
+    i32 link;
+};
+
+__device__ inline i32 bucket_link(i64 lo0, i64 cnt) { return -(i32)(1 + ((lo0 << 5) | (cnt - 1))); }
+__device__ inline i64 bucket_lo(i32 link) { return (i64)((u32)(-link - 1) >> 5); }
+__device__ inline int bucket_cnt(i32 link) { return (int)((u32)(-link - 1) & 31u) + 1; }
+
+__device__ inline float next_down(float f) {
+    if (f != f || f == -INFINITY) return f;
+    if (f == 0.f) return -__uint_as_float(1u);
+    const u32 b = __float_as_uint(f);
+    return __uint_as_float(f > 0.f ? b - 1u : b + 1u);
+}
+__device__ inline float next_up(float f) {
+    if (f != f || f == INFINITY) return f;
+    if (f == 0.f) return __uint_as_float(1u);
+    const u32 b = __float_as_uint(f);
+    return __uint_as_float(f > 0.f ? b + 1u : b - 1u);
+}
+__device__ inline float f32_down(double x) {
+    const float f = (float)x;
+    return ((double)f > x) ? next_down(f) : f;
+}
+__device__ inline float f32_up(double x) {
+    const float f = (float)x;
+    return ((double)f < x) ? next_up(f) : f;
+}
+
+
+// ---- bucket tree build ----
+
+// Per internal node i: big_s[split - 1] = (leaves of i > kBucket); for a big node,
+// bs[first leaf - 1] = 1 for each child that is a bucket (at most kBucket leaves).
+__global__ void __launch_bounds__(kBlock) k_bucket_flags(i64 n, const i32* __restrict__ left,
+                                                 const i32* __restrict__ range, u32* __restrict__ big_s,
+                                                 u32* __restrict__ bs) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
+    if (i > n - 1) return;
+    const i64 lo = range[2 * (i - 1)], hi = range[2 * (i - 1) + 1];
+    const i64 l = left[i - 1];
+    const i64 s = l >= n ? lo : l;  // split: the left child covers [lo, s]
+    const bool big = hi - lo + 1 > kBucket;
+    big_s[s - 1] = big ? 1u : 0u;
+    if (big) {
+        if (s - lo + 1 <= kBucket) bs[lo - 1] = 1u;
+        if (hi - s <= kBucket) bs[s] = 1u;
+    }
+}
+
+__device__ inline void round_box(const double lo[3], const double hi[3], double scale, BucketRec& R) {
+    for (int d = 0; d < 3; ++d) {
+        R.lo[d] = f32_down(lo[d]);
+        R.hi[d] = f32_up(hi[d]);
+    }
+    R.scale = f32_up(scale);
+}
+// Box of unified node id u (internal: its AABB and scale; leaf: its point and h).
+__device__ inline void node_box(i64 u, i64 n, const NodeRec<double>* __restrict__ nodes,
+                                const LeafRec<double>* __restrict__ leaves, BucketRec& R) {
+    if (u >= n) {
+        const LeafRec<double> L = leaves[u - n];
+        const double p[3] = {L.x, L.y, L.z};
+        round_box(p, p, L.h, R);
+    } else {
+        const NodeRec<double> N = nodes[u - 1];
+        round_box(N.lo, N.hi, N.scale, R);
+    }
+}
+
+// Every big internal node writes its record and those of its bucket children.
+// Preorder position of a collapsed node u with first leaf lo (1-based):
+//   pre(u) = L(u) + PB[lo - 1] + PS[lo - 1],
+// L(u) = number of ancestors holding u in their left subtree, PB / PS = exclusive
+// prefix counts of big splits / bucket starts: the nodes before u in preorder are
+// its ancestors (all big) and the big nodes and buckets entirely left of lo; big
+// nodes with split < lo are the latter plus the ancestors holding u on the right.
+__global__ void __launch_bounds__(kBlock) k_bucket_emit(i64 n, i32 root, const i32* __restrict__ left,
+                                                const i32* __restrict__ parent, const i32* __restrict__ range,
+                                                const NodeRec<double>* __restrict__ nodes,
+                                                const LeafRec<double>* __restrict__ leaves,
+                                                const u32* __restrict__ PB, const u32* __restrict__ PS,
+                                                BucketRec* __restrict__ recs) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
+    if (i > n - 1) return;
+    const i64 lo = range[2 * (i - 1)], hi = range[2 * (i - 1) + 1];
+    if (hi - lo + 1 <= kBucket) return;
+    i64 Lc = 0;
+    for (i64 c = i, steps = n; c != root && steps > 0; --steps) {
+        const i64 p = parent[c - 1];
+        if (left[p - 1] == c) ++Lc;
+        c = p;
+    }
+    const i64 pre = Lc + PB[lo - 1] + PS[lo - 1];
+    const i64 cnt = (i64)(PB[hi - 1] - PB[lo - 1]) + (i64)(PS[hi] - PS[lo - 1]);
+    BucketRec R;
+    node_box(i, n, nodes, leaves, R);
+    R.link = (i32)(pre + cnt);
+    recs[pre] = R;
+    const i64 l = left[i - 1];
+    const i64 s = l >= n ? lo : l;
+    if (s - lo + 1 <= kBucket) {  // left child is a bucket, right after its parent
+        node_box(s == lo ? (lo + n - 1) : s, n, nodes, leaves, R);
+        R.link = bucket_link(lo - 1, s - lo + 1);
+        recs[pre + 1] = R;
+    }
+    if (hi - s <= kBucket) {
+        const i64 rpre = Lc + PB[s] + PS[s];
+        node_box(s + 1 == hi ? (hi + n - 1) : s + 1, n, nodes, leaves, R);
+        R.link = bucket_link(s, hi - s);
+        recs[rpre] = R;
+    }
+}
+
+// n <= kBucket: the whole tree is one bucket.
+__global__ void k_bucket_root(i64 n, i32 root, const NodeRec<double>* __restrict__ nodes,
+                              const LeafRec<double>* __restrict__ leaves, BucketRec* __restrict__ recs) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    BucketRec R;
+    node_box(root != 0 ? root : n, n, nodes, leaves, R);
+    R.link = bucket_link(0, n);
+    recs[0] = R;
+}
+
+// ---- per-call records of the FP32-compensated path ----
+
+// {m/rho, 1/h} per particle in float32 (rounded once from float64).
+__global__ void __launch_bounds__(kBlock) k_pack_vih(const double* __restrict__ cols, i64 n, float2* __restrict__ vih) {
+    const i64 b = (i64)blockIdx.x * kBlock + threadIdx.x;
+    if (b >= n) return;
+    vih[b] = make_float2((float)(cols[3 * n + b] / cols[5 * n + b]), (float)(1.0 / cols[4 * n + b]));
+}
+
+// ---- FP32-compensated accumulator ----
+
+// Shape f(q) and f'(q) in float32 without branches: every piece is written with
+// clamped factors (x)_+ = max(x, 0), which reproduces the piecewise definitions
+// of KI/kernel_function/kernels/*.jl (zero beyond the support); q >= 0 here.
+template <int K> __device__ __attribute__((always_inline)) void shape32(float q, float& f, float& df) {
+    if constexpr (K == PH_KERNEL_M4) {
+        const float a = fmaxf(2.f - q, 0.f), b = fmaxf(1.f - q, 0.f);
+        const float a2 = a * a, b2 = b * b;
+        f = fmaf(0.25f * a2, a, -(b2 * b));
+        df = fmaf(-0.75f, a2, 3.f * b2);
+    } else if constexpr (K == PH_KERNEL_M5) {
+        const float a = fmaxf(2.5f - q, 0.f), b = fmaxf(1.5f - q, 0.f), c = fmaxf(0.5f - q, 0.f);
+        const float a3 = a * a * a, b3 = b * b * b, c3 = c * c * c;
+        f = fmaf(a3, a, fmaf(-5.f * b3, b, 10.f * c3 * c));
+        df = fmaf(-4.f, a3, fmaf(20.f, b3, -40.f * c3));
+    } else if constexpr (K == PH_KERNEL_M6) {
+        const float a = fmaxf(3.f - q, 0.f), b = fmaxf(2.f - q, 0.f), c = fmaxf(1.f - q, 0.f);
+        const float a2 = a * a, b2 = b * b, c2 = c * c;
+        const float a4 = a2 * a2, b4 = b2 * b2, c4 = c2 * c2;
+        f = fmaf(a4, a, fmaf(-6.f * b4, b, 15.f * c4 * c));
+        df = fmaf(-5.f, a4, fmaf(30.f, b4, -75.f * c4));
+    } else {
+        const float t = fmaxf(fmaf(-0.5f, q, 1.f), 0.f);
+        const float t2 = t * t, t3 = t2 * t, t4 = t2 * t2;
+        if constexpr (K == PH_KERNEL_C2) {
+            f = t4 * fmaf(2.f, q, 1.f);
+            df = fmaf(2.f, t4, -t3 * fmaf(4.f, q, 2.f));
+        } else if constexpr (K == PH_KERNEL_C4) {
+            const float t5 = t4 * t, t6 = t4 * t2;
+            f = t6 * fmaf(fmaf(35.f / 12.f, q, 3.f), q, 1.f);
+            df = fmaf(t6, fmaf(35.f / 6.f, q, 3.f), -t5 * fmaf(fmaf(8.75f, q, 9.f), q, 3.f));
+        } else {
+            const float t7 = t4 * t3, t8 = t4 * t4;
+            f = t8 * fmaf(fmaf(fmaf(4.f, q, 6.25f), q, 4.f), q, 1.f);
+            df = fmaf(t8, fmaf(fmaf(12.f, q, 12.5f), q, 4.f), -t7 * fmaf(fmaf(fmaf(16.f, q, 25.f), q, 16.f), q, 4.f));
+        }
+    }
+}
+
+template <int STRAT> __device__ inline float radius2_32(float Kv, float r2a, float h) {
+    if constexpr (STRAT == PH_GATHER) {
+        return r2a;
+    } else {
+        const float r = Kv * h;
+        const float rr = r * r;
+        if constexpr (STRAT == PH_SCATTER) return rr;
+        else return fmaxf(rr, r2a);
+    }
+}
+
+// FP32-compensated accumulator.  Kernel values and gradients are evaluated in
+// float32 from the float64 distance (so the kernel argument q carries a single
+// rounding) with v = m/rho and 1/h rounded once to float32; every term goes into
+// float64 totals with the float64 field values, where the products are exact to
+// float64 rounding.  Every output keeps its own totals (static register
+// indices), and a lane without a pending neighbour adds exact zeros, so the
+// lockstep loop has no divergent branch.  Record positions index the float64
+// record {m, rho, columns...} of the float64 path.  EXACT: the catalog chunk has
+// exactly MS scalars, MG gradients, MD divergences and MC curls; otherwise the
+// unused output slots read position 0 and add zeros (no branch either).
+template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT> struct AccC {
+    static constexpr int NS = MS > 0 ? MS : 1, G = MG > 0 ? MG : 1, D = MD > 0 ? MD : 1, CC = MC > 0 ? MC : 1;
+    double S1, Q[3];
+    double ts[NS];
+    double tgs[G], tg[G][3];
+    double tds[D][3], td[D];
+    double tcs[CC][3], tc[CC][3];
+
+    __device__ __attribute__((always_inline)) void init() {
+        S1 = 0.0;
+        Q[0] = Q[1] = Q[2] = 0.0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) ts[j] = 0.0;
+#pragma unroll
+        for (int j = 0; j < G; ++j) tgs[j] = tg[j][0] = tg[j][1] = tg[j][2] = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) td[j] = tds[j][0] = tds[j][1] = tds[j][2] = 0.0;
+#pragma unroll
+        for (int j = 0; j < CC; ++j) tcs[j][0] = tcs[j][1] = tcs[j][2] = tc[j][0] = tc[j][1] = tc[j][2] = 0.0;
+    }
+
+    // One neighbour (live = false: adds zeros): d = sample - particle and d2 in
+    // float64, r = float64 record, vi = {m/rho, 1/h}; iha2 = (1/ha)^2 and the
+    // float32 norms of the sample's kernel.
+    __device__ __attribute__((always_inline)) void add(bool live, const CatChunk& c, const double* __restrict__ r,
+                                                       float2 vi, double dx, double dy, double dz, double d2,
+                                                       double iha2, float Wa_norm, float Ga_norm) {
+        const float r32 = sqrtf((float)d2);
+        float W, coef;
+        if constexpr (STRAT == PH_GATHER) {
+            float f, df;
+            shape32<K>(sqrtf((float)(d2 * iha2)), f, df);
+            W = Wa_norm * f;
+            coef = Ga_norm * df;
+        } else {
+            const float sig = sigma3<float, K>();
+            const float ib = vi.y;
+            const double ibd = ib;
+            const float ib3 = ib * ib * ib;
+            float fb, dfb;
+            shape32<K>(sqrtf((float)(d2 * (ibd * ibd))), fb, dfb);
+            const float Wb = (sig * ib3) * fb, Gb = (sig * ib3 * ib) * dfb;
+            if constexpr (STRAT == PH_SCATTER) {
+                W = Wb;
+                coef = Gb;
+            } else {
+                float fa, dfa;
+                shape32<K>(sqrtf((float)(d2 * iha2)), fa, dfa);
+                W = 0.5f * (Wa_norm * fa + Wb);
+                coef = 0.5f * (Ga_norm * dfa + Gb);
+            }
+        }
+        // float64 accumulation with fused products (explicit fma: the build
+        // disables contraction so that the float64 path stays reference-exact)
+        const double w = live ? (double)(vi.x * W) : 0.0;
+        S1 += w;
+#pragma unroll
+        for (int j = 0; j < MS; ++j) {
+            const double wj = (EXACT || j < c.ns) ? w : 0.0;
+            ts[j] = fma(r[c.spos[j]], wj, ts[j]);
+        }
+        if constexpr (MG + MD + MC > 0) {
+            // grad W = delta f'(q) sigma / (r h^4); zero at r = 0 (kernel.jl:273-285)
+            const double s = (live && d2 > 0.0) ? (double)(vi.x * coef / r32) : 0.0;
+            const double g0 = dx * s, g1 = dy * s, g2 = dz * s;
+            Q[0] += g0;
+            Q[1] += g1;
+            Q[2] += g2;
+#pragma unroll
+            for (int j = 0; j < MG; ++j) {
+                const bool on = EXACT || j < c.ng;
+                // reference grad slot 0: gradient of rho (record position 1)
+                const double A = on ? r[c.gpos[j] < 0 ? 1 : c.gpos[j]] : 0.0;
+                tgs[j] = fma(A, w, tgs[j]);
+                tg[j][0] = fma(A, g0, tg[j][0]);
+                tg[j][1] = fma(A, g1, tg[j][1]);
+                tg[j][2] = fma(A, g2, tg[j][2]);
+            }
+#pragma unroll
+            for (int j = 0; j < MD; ++j) {
+                const bool on = EXACT || j < c.nd;
+                const double ax = on ? r[c.dpos[3 * j]] : 0.0, ay = on ? r[c.dpos[3 * j + 1]] : 0.0,
+                             az = on ? r[c.dpos[3 * j + 2]] : 0.0;
+                tds[j][0] = fma(ax, w, tds[j][0]);
+                tds[j][1] = fma(ay, w, tds[j][1]);
+                tds[j][2] = fma(az, w, tds[j][2]);
+                td[j] = fma(az, g2, fma(ay, g1, fma(ax, g0, td[j])));
+            }
+#pragma unroll
+            for (int j = 0; j < MC; ++j) {
+                const bool on = EXACT || j < c.nc;
+                const double ax = on ? r[c.cpos[3 * j]] : 0.0, ay = on ? r[c.cpos[3 * j + 1]] : 0.0,
+                             az = on ? r[c.cpos[3 * j + 2]] : 0.0;
+                tcs[j][0] = fma(ax, w, tcs[j][0]);
+                tcs[j][1] = fma(ay, w, tcs[j][1]);
+                tcs[j][2] = fma(az, w, tcs[j][2]);
+                tc[j][0] = fma(-az, g1, fma(ay, g2, tc[j][0]));
+                tc[j][1] = fma(-ax, g2, fma(az, g0, tc[j][1]));
+                tc[j][2] = fma(-ay, g0, fma(ax, g1, tc[j][2]));
+            }
+        }
+    }
+
+    // Epilogue in float64 (general_interpolation.jl:128-214); outputs float32.
+    __device__ __attribute__((always_inline)) void finish(const CatChunk& c, float* __restrict__ out, i64 S,
+                                                          i64 s) const {
+        if (S1 == 0.0) {
+            const float nan = __builtin_nanf("");
+#pragma unroll
+            for (int j = 0; j < MS; ++j)
+                if (j < c.ns) out[c.sout[j] * S + s] = nan;
+#pragma unroll
+            for (int j = 0; j < MG; ++j)
+                if (j < c.ng) {
+                    out[(c.gout[j] + 0) * S + s] = nan;
+                    out[(c.gout[j] + 1) * S + s] = nan;
+                    out[(c.gout[j] + 2) * S + s] = nan;
+                }
+#pragma unroll
+            for (int j = 0; j < MD; ++j)
+                if (j < c.nd) out[c.dout[j] * S + s] = nan;
+#pragma unroll
+            for (int j = 0; j < MC; ++j)
+                if (j < c.nc) {
+                    out[(c.cout[j] + 0) * S + s] = nan;
+                    out[(c.cout[j] + 1) * S + s] = nan;
+                    out[(c.cout[j] + 2) * S + s] = nan;
+                }
+            return;
+        }
+        const double invS1 = 1.0 / S1;
+#pragma unroll
+        for (int j = 0; j < MS; ++j)
+            if (j < c.ns) out[c.sout[j] * S + s] = (float)(c.sflag[j] ? ts[j] * invS1 : ts[j]);
+#pragma unroll
+        for (int j = 0; j < MG; ++j)
+            if (j < c.ng) {
+                const double A = tgs[j] * invS1;
+                out[(c.gout[j] + 0) * S + s] = (float)(tg[j][0] - A * Q[0]);
+                out[(c.gout[j] + 1) * S + s] = (float)(tg[j][1] - A * Q[1]);
+                out[(c.gout[j] + 2) * S + s] = (float)(tg[j][2] - A * Q[2]);
+            }
+#pragma unroll
+        for (int j = 0; j < MD; ++j)
+            if (j < c.nd) {
+                const double ax = tds[j][0] * invS1, ay = tds[j][1] * invS1, az = tds[j][2] * invS1;
+                out[c.dout[j] * S + s] = (float)(td[j] - ((ax * Q[0] + ay * Q[1]) + az * Q[2]));
+            }
+#pragma unroll
+        for (int j = 0; j < MC; ++j)
+            if (j < c.nc) {
+                const double ax = tcs[j][0] * invS1, ay = tcs[j][1] * invS1, az = tcs[j][2] * invS1;
+                out[(c.cout[j] + 0) * S + s] = (float)(-(tc[j][0] - (ay * Q[2] - az * Q[1])));
+                out[(c.cout[j] + 1) * S + s] = (float)(-(tc[j][1] - (az * Q[0] - ax * Q[2])));
+                out[(c.cout[j] + 2) * S + s] = (float)(-(tc[j][2] - (ax * Q[1] - ay * Q[0])));
+            }
+    }
+};
+
+struct BucketView {
+    i64 n;
+    i32 nrec;
+    const BucketRec* __restrict__ recs;
+    const LeafRec<double>* __restrict__ leaves;
+};
+
+constexpr float kInflate = 1.0f + 9.5367432e-7f;  // 1 + 2^-20
+
+// Conservative float32 pruning data of one sample.
+template <int STRAT> struct Probe {
+    float px, py, pz;
+    float Kf, Rg2, eps;
+    __device__ inline void init(const double p[3], double Kd, double ra) {
+        px = (float)p[0];
+        py = (float)p[1];
+        pz = (float)p[2];
+        // |p32 - p| <= 2^-24 |p| per axis; eps covers it in every direction
+        eps = 2.3841858e-7f * fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz))) + 1e-30f;
+        Kf = (float)Kd;
+        Rg2 = 0.f;
+        if constexpr (STRAT != PH_SCATTER) {
+            const float Rg = f32_up(ra) * kInflate + eps;
+            Rg2 = Rg * Rg * kInflate;
+        }
+    }
+    // true when the record's subtree may hold an accepted leaf
+    __device__ inline bool keep(const BucketRec& R) const {
+        const float p[3] = {px, py, pz};
+        float d2 = 0.f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float g = fmaxf(fmaxf(R.lo[d] - p[d], p[d] - R.hi[d]), 0.f);
+            d2 += g * g;
+        }
+        float R2 = Rg2;
+        if constexpr (STRAT != PH_GATHER) {
+            const float rs = Kf * R.scale * kInflate + eps;
+            const float rs2 = rs * rs * kInflate;
+            R2 = STRAT == PH_SCATTER ? rs2 : fmaxf(rs2, Rg2);
+        }
+        return !(d2 > R2);
+    }
+};
+
+// Per-lane list of up to kQ buckets (encoded links), in separate registers.
+static_assert(kQ == 4, "BucketList holds four buckets");
+struct BucketList {
+    i32 b0, b1, b2, b3;
+    int n;
+    __device__ __attribute__((always_inline)) void push(i32 link) {
+        if (n == 0) b0 = link;
+        else if (n == 1) b1 = link;
+        else if (n == 2) b2 = link;
+        else b3 = link;
+        ++n;
+    }
+    __device__ __attribute__((always_inline)) i32 get(int q) const {
+        return q == 0 ? b0 : (q == 1 ? b1 : (q == 2 ? b2 : b3));
+    }
+};
+
+// Walks from record i until kQ buckets are collected or the tree is done.
+template <int STRAT>
+__device__ __attribute__((always_inline)) void collect(const BucketView& bv, const Probe<STRAT>& pr, i32& i,
+                                                       i64& steps, BucketList& bl) {
+    bl.n = 0;
+    while (i < bv.nrec && steps > 0 && bl.n < kQ) {
+        --steps;
+        const BucketRec R = bv.recs[i];
+        const bool k = pr.keep(R);
+        if (R.link >= 0) {
+            i = k ? i + 1 : R.link;
+        } else {
+            if (k) bl.push(R.link);
+            ++i;
+        }
+    }
+}
+
+// Float64 leaf tests of the listed buckets: bit q * kBucket + j for leaf j of bucket q.
+template <int STRAT>
+__device__ __attribute__((always_inline)) u64 test_buckets(const BucketView& bv, const BucketList& bl,
+                                                           const double p[3], double Kd, double r2a) {
+    u64 mask = 0;
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        if (q < bl.n) {
+            const i32 link = bl.get(q);
+            const i64 lo = bucket_lo(link);
+            const int cnt = bucket_cnt(link);
+#pragma unroll 4
+            for (int j = 0; j < cnt; ++j) {
+                const LeafRec<double> L = bv.leaves[lo + j];
+                if (d2_point(p[0], p[1], p[2], L) <= radius2<double, STRAT>(Kd, r2a, L.h))
+                    mask |= 1ull << (q * kBucket + j);
+            }
+        }
+    }
+    return mask;
+}
+
+// LBVH_find_nearest_h (LinearBVHQueries.jl:118-140) on the bucket tree: the
+// lexicographic minimum of (float64 d2, leaf index), i.e. the first nearest leaf
+// in DFS order.  A greedy descent seeds the bound; pruning compares float32 box
+// distances with the bound inflated to cover rounding, so no candidate is lost.
+__device__ inline double nearest_h_bucket(const BucketView& bv, const double p[3]) {
+    const float px = (float)p[0], py = (float)p[1], pz = (float)p[2];
+    const float eps = 2.3841858e-7f * fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz))) + 1e-30f;
+    auto box_d2 = [&](const BucketRec& R) {
+        const float q[3] = {px, py, pz};
+        float d2 = 0.f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float g = fmaxf(fmaxf(R.lo[d] - q[d], q[d] - R.hi[d]), 0.f);
+            d2 += g * g;
+        }
+        return d2;
+    };
+    double best = INFINITY;
+    i64 bk = -1;
+    auto scan = [&](i32 link) {
+        const i64 lo = bucket_lo(link);
+        const int cnt = bucket_cnt(link);
+        for (int j = 0; j < cnt; ++j) {
+            const LeafRec<double> L = bv.leaves[lo + j];
+            const double d2 = d2_point(p[0], p[1], p[2], L);
+            const i64 k = lo + j;
+            if (d2 < best || (d2 == best && k < bk)) {
+                best = d2;
+                bk = k;
+            }
+        }
+    };
+    // greedy descent: nearer child first (left child = next record; right child
+    // = the record after the left subtree)
+    i32 i = 0;
+    for (i64 steps = bv.nrec; steps > 0; --steps) {
+        const BucketRec R = bv.recs[i];
+        if (R.link < 0) {
+            scan(R.link);
+            break;
+        }
+        const i32 l = i + 1;
+        const BucketRec RL = bv.recs[l];
+        const i32 r = RL.link >= 0 ? RL.link : l + 1;
+        i = box_d2(bv.recs[r]) < box_d2(RL) ? r : l;
+    }
+    i = 0;
+    for (i64 steps = 2 * (i64)bv.nrec + 8; i < bv.nrec && steps > 0; --steps) {
+        const BucketRec R = bv.recs[i];
+        const float bound = (float)best;
+        const float lim = (sqrtf(bound) * kInflate + eps);
+        const bool keep = !(box_d2(R) > lim * lim * kInflate);
+        if (R.link >= 0) {
+            i = keep ? i + 1 : R.link;
+        } else {
+            if (keep) scan(R.link);
+            ++i;
+        }
+    }
+    return bk >= 0 ? bv.leaves[bk].h : (double)NAN;
+}
+
+template <int SRC>
+__global__ void __launch_bounds__(kBlock) k_nearest_bucket(BucketView bv, Sampler<double, SRC> smp, double* __restrict__ ha) {
+    i64 s;
+    double p[3];
+    if (smp.get(s, p[0], p[1], p[2])) ha[s] = nearest_h_bucket(bv, p);
+}
+
+
+template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT, int SRC>
+__global__ void __launch_bounds__(kBlock) k_bucket_c32(BucketView bv, const double* __restrict__ rec,
+                                                       const float2* __restrict__ vih, CatChunk c,
+                                                       Sampler<double, SRC> smp, const double* __restrict__ ha_s,
+                                                       float* __restrict__ out, u64* counter) {
+    i64 s = 0;
+    double p[3] = {0.0, 0.0, 0.0};
+    const bool valid = smp.get(s, p[0], p[1], p[2]);
+    const double Kd = support_of<K>();
+    double ha = 0.0;
+    if constexpr (STRAT != PH_SCATTER) {
+        if (valid) ha = ha_s[s];
+    }
+    const double ra = Kd * ha, r2a = ra * ra;  // float64, as the reference
+    Probe<STRAT> pr;
+    pr.init(p, Kd, ra);
+    const double iha = STRAT == PH_SCATTER || !valid ? 0.0 : 1.0 / ha;
+    const double iha2 = iha * iha;
+    const float inv_ha = (float)iha;
+    const float sig = sigma3<float, K>();
+    const float Wa_norm = sig * inv_ha * inv_ha * inv_ha, Ga_norm = Wa_norm * inv_ha;
+    const int rw = c.rw;
+
+    // first leaf of each listed bucket, per lane: read back with a per-lane index
+    // in the accumulation loop, which thus has no divergent branch
+    __shared__ u32 blo[kQ][kBlock];
+    const int tid = threadIdx.x;
+    const u32 nlast = (u32)(bv.n - 1);
+
+    AccC<K, STRAT, MS, MG, MD, MC, EXACT> acc;
+    acc.init();
+    u64 accepted = 0;
+    i32 i = valid ? 0 : bv.nrec;
+    i64 steps = 2 * (i64)bv.nrec + 8;  // a valid tree is walked in at most nrec steps
+    BucketList bl;
+    while (true) {
+        collect<STRAT>(bv, pr, i, steps, bl);
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) blo[q][tid] = q < bl.n ? (u32)bucket_lo(bl.get(q)) : 0u;
+        u64 mask = test_buckets<STRAT>(bv, bl, p, Kd, r2a);
+        accepted += __popcll(mask);
+        // lockstep: one neighbour per lane and iteration, lowest bit first
+        // (increasing leaf index); a lane with none left adds zeros
+        int nit = __popcll(mask);
+        for (int d = kWave / 2; d > 0; d >>= 1) nit = max(nit, __shfl_xor(nit, d, kWave));
+        nit = __builtin_amdgcn_readfirstlane(nit);
+        for (int it = 0; it < nit; ++it) {
+            const bool live = mask != 0;
+            const int b = __builtin_ctzll(mask | (1ull << 63));
+            mask &= mask - 1;
+            const u32 kk = blo[b >> 4][tid] + (u32)(b & (kBucket - 1));
+            const u32 k = kk < nlast ? kk : nlast;
+            const LeafRec<double> L = bv.leaves[k];
+            const double dx = p[0] - L.x, dy = p[1] - L.y, dz = p[2] - L.z;
+            acc.add(live, c, rec + (size_t)k * (u32)rw, vih[k], dx, dy, dz, (dx * dx + dy * dy) + dz * dz, iha2,
+                    Wa_norm, Ga_norm);
+        }
+        if (!__any(i < bv.nrec && steps > 0)) break;
+    }
+    if (valid) acc.finish(c, out, smp.S, s);
+    count_add(counter, accepted);
+}
+
+template <int K, int STRAT, int MS, int MG, int MD, int MC, int SRC>
+__global__ void __launch_bounds__(kBlock) k_bucket_f64(BucketView bv, const double* __restrict__ rec, CatChunk c,
+                                                       Sampler<double, SRC> smp, const double* __restrict__ ha_s,
+                                                       double* __restrict__ out, u64* counter) {
+    i64 s = 0;
+    double p[3] = {0.0, 0.0, 0.0};
+    const bool valid = smp.get(s, p[0], p[1], p[2]);
+    const double Kd = support_of<K>();
+    double ha = 0.0;
+    if constexpr (STRAT != PH_SCATTER) {
+        if (valid) ha = ha_s[s];
+    }
+    const double ra = Kd * ha, r2a = ra * ra;
+    Probe<STRAT> pr;
+    pr.init(p, Kd, ra);
+    Acc<double, K, STRAT, MS, MG, MD, MC> acc;
+    acc.init();
+    u64 accepted = 0;
+    i32 i = valid ? 0 : bv.nrec;
+    i64 steps = 2 * (i64)bv.nrec + 8;
+    BucketList bl;
+    while (true) {
+        collect<STRAT>(bv, pr, i, steps, bl);
+        u64 mask = test_buckets<STRAT>(bv, bl, p, Kd, r2a);
+        accepted += __popcll(mask);
+        while (__any(mask != 0)) {
+            if (mask != 0) {
+                const int b = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                const i64 k = bucket_lo(bl.get(b >> 4)) + (b & (kBucket - 1));
+                const LeafRec<double> L = bv.leaves[k];
+                acc.add(c, rec + k * c.rw, p[0] - L.x, p[1] - L.y, p[2] - L.z, d2_point(p[0], p[1], p[2], L), ha, L.h);
+            }
+        }
+        if (!__any(i < bv.nrec && steps > 0)) break;
+    }
+    if (valid) acc.finish(c, out, smp.S, s);
+    count_add(counter, accepted);
+}
+
+BucketView bucket_view(const ph_lbvh* B) {
+    return BucketView{B->n, (i32)B->nrec, B->brecs.as<BucketRec>(), B->leaves.as<LeafRec<double>>()};
+}
+
+inline unsigned sample_blocks(const GridAxes* ga, i64 S) {
+    if (ga) {
+        const i64 tiles = ((ga->nx + 3) / 4) * ((ga->ny + 3) / 4) * ((ga->nz + 3) / 4);
+        return (unsigned)((tiles + 3) / 4);
+    }
+    return (unsigned)((S + kBlock - 1) / kBlock);
+}
+
+template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT>
+void launch_c32_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const GridAxes* ga, i64 S, const double* ha,
+                  void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+    const BucketView bv = bucket_view(B);
+    const unsigned g = sample_blocks(ga, S);
+    if (ev) PH_HIP(hipEventRecord(ev[0], s));
+    if (ga)
+        k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 1><<<g, kBlock, 0, s>>>(
+            bv, B->rec.as<double>(), B->rec32.as<float2>(), c, Sampler<double, 1>{*ga, S}, ha,
+            static_cast<float*>(out), counters);
+    else
+        k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 0><<<g, kBlock, 0, s>>>(
+            bv, B->rec.as<double>(), B->rec32.as<float2>(), c, Sampler<double, 0>{*pl, S}, ha,
+            static_cast<float*>(out), counters);
+    PH_HIP(hipGetLastError());
+    if (ev) PH_HIP(hipEventRecord(ev[1], s));
+}
+
+template <int K, int STRAT, int MS, int MG, int MD, int MC>
+void launch_f64_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const GridAxes* ga, i64 S, const double* ha,
+                  void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+    const BucketView bv = bucket_view(B);
+    const unsigned g = sample_blocks(ga, S);
+    if (ev) PH_HIP(hipEventRecord(ev[0], s));
+    if (ga)
+        k_bucket_f64<K, STRAT, MS, MG, MD, MC, 1><<<g, kBlock, 0, s>>>(bv, B->rec.as<double>(), c,
+                                                                      Sampler<double, 1>{*ga, S}, ha,
+                                                                      static_cast<double*>(out), counters);
+    else
+        k_bucket_f64<K, STRAT, MS, MG, MD, MC, 0><<<g, kBlock, 0, s>>>(bv, B->rec.as<double>(), c,
+                                                                      Sampler<double, 0>{*pl, S}, ha,
+                                                                      static_cast<double*>(out), counters);
+    PH_HIP(hipGetLastError());
+    if (ev) PH_HIP(hipEventRecord(ev[1], s));
+}
+
+template <int K, int STRAT>
+void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointList* pl, const GridAxes* ga, i64 S,
+                      void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+    const i64 n = B->n;
+    // nearest-particle smoothing length of every sample (gather / symmetric)
+    const double* ha = nullptr;
+    if (STRAT != PH_SCATTER) {
+        B->ha.alloc(sizeof(double) * (size_t)S);
+        const BucketView bv = bucket_view(B);
+        const unsigned g = sample_blocks(ga, S);
+        if (ga) k_nearest_bucket<1><<<g, kBlock, 0, s>>>(bv, Sampler<double, 1>{*ga, S}, B->ha.as<double>());
+        else k_nearest_bucket<0><<<g, kBlock, 0, s>>>(bv, Sampler<double, 0>{*pl, S}, B->ha.as<double>());
+        PH_HIP(hipGetLastError());
+        ha = B->ha.as<double>();
+    }
+    const bool small = c.ng + c.nd + c.nc == 0 && c.ns <= 2;
+    B->rec.alloc(sizeof(double) * (size_t)c.rw * (size_t)n);
+    k_pack_records<double><<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c, B->rec.as<double>());
+    PH_HIP(hipGetLastError());
+    if (precision == PH_PREC_FP64) {
+        if (small) launch_f64_t<K, STRAT, 2, 0, 0, 0>(B, c, pl, ga, S, ha, out, counters, s, ev);
+        else launch_f64_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC>(B, c, pl, ga, S, ha, out, counters, s, ev);
+        return;
+    }
+    B->rec32.alloc(sizeof(float2) * (size_t)n);
+    k_pack_vih<<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, B->rec32.as<float2>());
+    PH_HIP(hipGetLastError());
+    if (small) launch_c32_t<K, STRAT, 2, 0, 0, 0, false>(B, c, pl, ga, S, ha, out, counters, s, ev);
+    else if (c.ns == 2 && c.ng == 1 && c.nd == 1 && c.nc == 1)  // density, scalar, gradient, divergence, curl
+        launch_c32_t<K, STRAT, 2, 1, 1, 1, true>(B, c, pl, ga, S, ha, out, counters, s, ev);
+    else launch_c32_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, false>(B, c, pl, ga, S, ha, out, counters, s, ev);
+}
+
+template <int K>
+void launch_bucket_k(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl,
+                     const GridAxes* ga, i64 S, void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+    switch (strategy) {
+    case PH_GATHER: launch_bucket_ks<K, PH_GATHER>(B, c, precision, pl, ga, S, out, counters, s, ev); break;
+    case PH_SCATTER: launch_bucket_ks<K, PH_SCATTER>(B, c, precision, pl, ga, S, out, counters, s, ev); break;
+    default: launch_bucket_ks<K, PH_SYMMETRIC>(B, c, precision, pl, ga, S, out, counters, s, ev); break;
+    }
+}
+
+}  // namespace
+
+void ensure_buckets(ph_lbvh* B, hipStream_t s) {
+    if (B->buckets_ready) return;
+    const i64 n = B->n;
+    if (n > kMaxBucketLeaves) throw Error{PH_ERR_CAPACITY, "the bucket traversal supports at most 2^26 particles"};
+    i64 nrec = 1;
+    if (n <= kBucket) {
+        B->brecs.alloc(sizeof(BucketRec));
+        k_bucket_root<<<1, 1, 0, s>>>(n, B->root, B->nodes.as<NodeRec<double>>(), B->leaves.as<LeafRec<double>>(),
+                                      B->brecs.as<BucketRec>());
+        PH_HIP(hipGetLastError());
+    } else {
+        DevBuf flags, tmp;
+        flags.alloc(sizeof(u32) * (size_t)(2 * n + 1));
+        u32* big_s = flags.as<u32>();  // n entries: n - 1 split flags + the total
+        u32* bs = big_s + n;           // n + 1 entries: n bucket-start flags + the total
+        PH_HIP(hipMemsetAsync(flags.p, 0, sizeof(u32) * (size_t)(2 * n + 1), s));
+        k_bucket_flags<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->left.as<i32>(), B->range.as<i32>(), big_s, bs);
+        PH_HIP(hipGetLastError());
+        exclusive_sum_u32(big_s, n, tmp, s);
+        exclusive_sum_u32(bs, n + 1, tmp, s);
+        u32 tot[2];
+        PH_HIP(hipMemcpyAsync(&tot[0], big_s + (n - 1), sizeof(u32), hipMemcpyDeviceToHost, s));
+        PH_HIP(hipMemcpyAsync(&tot[1], bs + n, sizeof(u32), hipMemcpyDeviceToHost, s));
+        PH_HIP(hipStreamSynchronize(s));
+        nrec = (i64)tot[0] + (i64)tot[1];
+        B->brecs.alloc(sizeof(BucketRec) * (size_t)nrec);
+        k_bucket_emit<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->root, B->left.as<i32>(), B->parent.as<i32>(),
+                                                        B->range.as<i32>(), B->nodes.as<NodeRec<double>>(),
+                                                        B->leaves.as<LeafRec<double>>(), big_s, bs,
+                                                        B->brecs.as<BucketRec>());
+        PH_HIP(hipGetLastError());
+        PH_HIP(hipStreamSynchronize(s));  // flags / tmp are released on return
+    }
+    B->nrec = nrec;
+    B->buckets_ready = true;
+}
+
+void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl, const GridAxes* ga,
+                   i64 S, void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+    if (S <= 0) return;
+    if (B->precision != PH_PREC_FP64) throw Error{PH_ERR_ARGUMENT, "the bucket traversal needs a float64 tree"};
+    ensure_buckets(B, s);
+    switch (B->kernel) {
+    case PH_KERNEL_M4: launch_bucket_k<PH_KERNEL_M4>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev); break;
+    case PH_KERNEL_M5: launch_bucket_k<PH_KERNEL_M5>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev); break;
+    case PH_KERNEL_M6: launch_bucket_k<PH_KERNEL_M6>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev); break;
+    case PH_KERNEL_C2: launch_bucket_k<PH_KERNEL_C2>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev); break;
+    case PH_KERNEL_C4: launch_bucket_k<PH_KERNEL_C4>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev); break;
+    default: launch_bucket_k<PH_KERNEL_C6>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev); break;
+    }
+}
+
+}  // namespace ph

@@ -91,9 +91,11 @@ struct ph_context {
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // 0-1 call, 2-3 build, 4-5 kernel
     ph_stats stats{};
     int count_neighbours = 0;
+    int traversal = PH_TRAVERSAL_AUTO;
     ph::DevBuf lut64, lut32;   // line-integrated tables [6][4096]
     ph::DevBuf counters;       // small device counters (accepted neighbours, rechecks, flags)
     int num_cu = 256;
+    std::vector<struct ph_lbvh*> trees;  // live trees; released with the context
 };
 
 // Device-resident InterpolationInput after LinearBVH!.
@@ -116,7 +118,19 @@ struct ph_lbvh {
     ph::DevBuf leaf_escape;  // i32 [n]
     ph::DevBuf rec;          // packed per-particle records of the last interpolation call
     ph::DevBuf ha;           // per-sample smoothing length of the nearest particle
-    // FP32-compensated view, built on first use from the float64 columns.
-    bool c32_ready = false;
-    ph::DevBuf c32_nodes, c32_leaves, c32_attr;
+    // Bucket tree (built on first use): the LBVH in DFS preorder with every
+    // subtree of at most 16 leaves collapsed into one bucket.
+    bool buckets_ready = false;
+    int64_t nrec = 0;
+    ph::DevBuf brecs;     // bucket-tree records [nrec]
+    ph::DevBuf rec32;     // {m/rho, 1/h} in float32 per particle (FP32-compensated calls)
+
+    // Frees every device buffer (the context is going away; the handle stays
+    // valid for ph_lbvh_destroy only).
+    void release_device() {
+        for (ph::DevBuf* b : {&cols, &codes, &order, &left, &right, &escape, &parent, &range, &nodes, &leaves,
+                              &leaf_escape, &rec, &ha, &brecs, &rec32})
+            b->release();
+        buckets_ready = false;
+    }
 };

@@ -59,7 +59,11 @@ void launch_points(ph_lbvh* B, const CatChunk& c, int strategy, int precision, c
 void launch_lines(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const LineList* ll, const LineGrid* lg,
                   i64 S, int L, void* out, u64* counters, const void* lut, hipStream_t s, hipEvent_t* ev = nullptr);
 void launch_nearest_h(ph_lbvh* B, const PointList& pl, i64 S, double* ha, hipStream_t s);
-// FP32-compensated view of B (nodes rounded outward, leaf offsets, attribute records).
-void prepare_c32(ph_lbvh* B, hipStream_t s);
+// Bucket traversal (float64 or FP32-compensated evaluation): per lane, a DFS over
+// the preorder bucket tree, leaves of reached buckets tested with float64.
+void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl,
+                   const GridAxes* ga, i64 S, void* out, u64* counters, hipStream_t s, hipEvent_t* ev = nullptr);
+// Builds the bucket tree of B if it is not built yet.
+void ensure_buckets(ph_lbvh* B, hipStream_t s);
 
 }  // namespace ph

@@ -16,6 +16,7 @@
 // so the 64 lanes of a wave walk nearly the same nodes; workgroups are remapped
 // so that each of the 8 XCDs (separate L2 caches) owns a contiguous run of tiles.
 #include "ph_interp.h"
+#include "ph_interp_dev.h"
 #include "ph_primitives.h"
 #include "ph_sph.h"
 
@@ -64,398 +65,6 @@ int split_catalog(const ph_catalog* cat, std::vector<CatChunk>& chunks) {
 
 namespace {
 
-template <class T> struct TreeView {
-    i64 n;
-    i32 root;
-    const NodeRec<T>* __restrict__ nodes;
-    const LeafRec<T>* __restrict__ leaves;
-    const i32* __restrict__ leaf_escape;
-};
-
-template <class T> TreeView<T> view_of(const ph_lbvh* B) {
-    return TreeView<T>{B->n, B->root, B->nodes.as<NodeRec<T>>(), B->leaves.as<LeafRec<T>>(),
-                       B->leaf_escape.as<i32>()};
-}
-
-// XCD-aware workgroup index: hardware dispatches workgroup b to XCD b % 8; give
-// each XCD a contiguous range of logical blocks so neighbouring tiles share an L2.
-__device__ inline i64 xcd_block() {
-    const i64 nb = gridDim.x, b = blockIdx.x;
-    const i64 q = nb / kNumXcd, r = nb % kNumXcd;
-    const i64 x = b % kNumXcd, pos = b / kNumXcd;
-    return x * q + (x < r ? x : r) + pos;
-}
-
-// ---- geometry (LinearBVHTraversalMacros.jl:800-953) ----
-template <class T> __device__ inline T d2_point(T px, T py, T pz, const LeafRec<T>& L) {
-    T dx = px - L.x, dy = py - L.y, dz = pz - L.z;
-    return dx * dx + dy * dy + dz * dz;
-}
-template <class T> __device__ inline T d2_aabb(T px, T py, T pz, const NodeRec<T>& N) {
-    const T p[3] = {px, py, pz};
-    T s = T(0);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const T a = N.lo[d], b = N.hi[d];
-        const T t = p[d] < a ? (a - p[d]) : (p[d] > b ? (p[d] - b) : T(0));
-        s += t * t;
-    }
-    return s;
-}
-template <class T> struct Line {
-    T o[3], m[3];
-};
-template <class T> __device__ inline T d2_line_leaf(const Line<T>& l, const LeafRec<T>& L) {
-    const T c[3] = {L.x, L.y, L.z};
-    T D2 = T(0), Dm = T(0);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const T D = c[d] - l.o[d];
-        D2 += D * D;
-        Dm += D * l.m[d];
-    }
-    return D2 - Dm * Dm;
-}
-template <class T> __device__ inline T d2_line_aabb_lb(const Line<T>& l, const NodeRec<T>& N) {
-    // _line_intersects_aabb (:878-916)
-    T tmin = -T(INFINITY), tmax = T(INFINITY);
-    bool hit = true;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const T a = N.lo[d], b = N.hi[d], o = l.o[d], m = l.m[d];
-        if (m == T(0)) {
-            if (o < a || o > b) hit = false;
-        } else {
-            T t0 = (a - o) / m, t1 = (b - o) / m;
-            if (t0 > t1) {
-                const T tt = t0;
-                t0 = t1;
-                t1 = tt;
-            }
-            tmin = fmax(tmin, t0);
-            tmax = fmin(tmax, t1);
-            if (tmin > tmax) hit = false;
-        }
-    }
-    if (hit) return T(0);
-    // bounding-sphere lower bound (:918-953)
-    const T half = T(1) / T(2);
-    T c[3], r2 = T(0);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) c[d] = (N.lo[d] + N.hi[d]) * half;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const T hh = (N.hi[d] - N.lo[d]) * half;
-        r2 += hh * hh;
-    }
-    T D2 = T(0), Dm = T(0);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const T D = c[d] - l.o[d];
-        D2 += D * D;
-        Dm += D * l.m[d];
-    }
-    const T dc2 = D2 - Dm * Dm;
-    if (dc2 <= r2) return T(0);
-    const T dl = sqrt(dc2) - sqrt(r2);
-    return dl * dl;
-}
-
-// Radius rule of the strategy (A13): gather (K ha)^2, scatter (K h)^2, symmetric max.
-template <class T, int STRAT> __device__ inline T radius2(T Kvalid, T r2a, T h) {
-    if constexpr (STRAT == PH_GATHER) {
-        return r2a;
-    } else {
-        const T r = Kvalid * h;
-        const T rr = r * r;
-        if constexpr (STRAT == PH_SCATTER) return rr;
-        else return (rr > r2a) ? rr : r2a;
-    }
-}
-
-// Stackless DFS: hit(k0, d2, leaf) on each accepted leaf (k0 = 0-based leaf index).
-template <class T, int STRAT, class Dist, class NodeDist, class Hit>
-__device__ inline void dfs(const TreeView<T>& tv, T Kvalid, T r2a, Dist leaf_d2, NodeDist node_d2, Hit hit) {
-    const i64 n = tv.n;
-    if (tv.root == 0) {
-        for (i64 k = 0; k < n; ++k) {
-            const LeafRec<T> L = tv.leaves[k];
-            const T d2 = leaf_d2(L);
-            if (d2 <= radius2<T, STRAT>(Kvalid, r2a, L.h)) hit(k, d2, L);
-        }
-        return;
-    }
-    i32 node = tv.root;
-    // A valid tree is walked in at most 2n-1 steps; the cap only guards against
-    // a corrupted tree turning into an endless loop on the device.
-    for (i64 steps = 2 * n + 8; node != 0 && steps > 0; --steps) {
-        if (node >= (i32)n) {
-            const i64 k = (i64)node - n;
-            const LeafRec<T> L = tv.leaves[k];
-            const T d2 = leaf_d2(L);
-            if (d2 <= radius2<T, STRAT>(Kvalid, r2a, L.h)) hit(k, d2, L);
-            node = tv.leaf_escape[k];
-        } else {
-            const NodeRec<T> N = tv.nodes[node - 1];
-            const T d2n = node_d2(N);
-            node = (d2n > radius2<T, STRAT>(Kvalid, r2a, N.scale)) ? N.escape : N.left;
-        }
-    }
-}
-
-// LBVH_find_nearest_h (LinearBVHQueries.jl:118-140).  The reference walks the
-// DFS from an infinite bound and keeps the first leaf (in DFS = Morton order) of
-// minimal d2, i.e. the lexicographic minimum of (d2, leaf index).  Here a greedy
-// descent (nearer child first) supplies a starting leaf and bound, then the
-// escape-rope DFS prunes with `d2_node > best` and replaces the candidate when
-// d2 < best or (d2 == best and the index is smaller): the same leaf, found after
-// visiting far fewer nodes.
-template <class T> __device__ inline T nearest_h(const TreeView<T>& tv, T px, T py, T pz) {
-    const i64 n = tv.n;
-    if (tv.root == 0) return tv.leaves[0].h;  // n == 1
-    // greedy descent to a nearby leaf
-    i32 node = tv.root;
-    for (i64 steps = n; node < (i32)n && steps > 0; --steps) {
-        const NodeRec<T> N = tv.nodes[node - 1];
-        const i32 l = N.left;
-        // the right child is the node after the left subtree: left's escape for a
-        // leaf, or the escape of the left internal node
-        const i32 r = (l >= (i32)n) ? tv.leaf_escape[l - n] : tv.nodes[l - 1].escape;
-        const T dl = (l >= (i32)n) ? d2_point(px, py, pz, tv.leaves[l - n]) : d2_aabb(px, py, pz, tv.nodes[l - 1]);
-        const T dr = (r >= (i32)n) ? d2_point(px, py, pz, tv.leaves[r - n]) : d2_aabb(px, py, pz, tv.nodes[r - 1]);
-        node = (dr < dl) ? r : l;
-    }
-    i64 bk = (i64)node - n;
-    T best = d2_point(px, py, pz, tv.leaves[bk]);
-    node = tv.root;
-    for (i64 steps = 2 * n + 8; node != 0 && steps > 0; --steps) {
-        if (node >= (i32)n) {
-            const i64 k = (i64)node - n;
-            const T d2 = d2_point(px, py, pz, tv.leaves[k]);
-            if (d2 < best || (d2 == best && k < bk)) {
-                best = d2;
-                bk = k;
-            }
-            node = tv.leaf_escape[k];
-        } else {
-            const NodeRec<T> N = tv.nodes[node - 1];
-            node = (d2_aabb(px, py, pz, N) > best) ? N.escape : N.left;
-        }
-    }
-    return tv.leaves[bk].h;
-}
-
-// ---- accumulation (general_interpolation.jl + accumulations/*.jl) ----
-template <class T, int K, int STRAT, int MS, int MG, int MD, int MC> struct Acc {
-    T S1;
-    T sc[MS > 0 ? MS : 1];
-    T gf[MG > 0 ? MG : 1][3], gb[MG > 0 ? MG : 1][3], gs[MG > 0 ? MG : 1];
-    T df[MD > 0 ? MD : 1], db[MD > 0 ? MD : 1][3], ds[MD > 0 ? MD : 1][3];
-    T cf[MC > 0 ? MC : 1][3], cb[MC > 0 ? MC : 1][3], cs[MC > 0 ? MC : 1][3];
-
-    __device__ void init() {
-        S1 = T(0);
-#pragma unroll
-        for (int j = 0; j < (MS > 0 ? MS : 1); ++j) sc[j] = T(0);
-#pragma unroll
-        for (int j = 0; j < (MG > 0 ? MG : 1); ++j) {
-            gs[j] = T(0);
-            for (int d = 0; d < 3; ++d) gf[j][d] = gb[j][d] = T(0);
-        }
-#pragma unroll
-        for (int j = 0; j < (MD > 0 ? MD : 1); ++j) {
-            df[j] = T(0);
-            for (int d = 0; d < 3; ++d) db[j][d] = ds[j][d] = T(0);
-        }
-#pragma unroll
-        for (int j = 0; j < (MC > 0 ? MC : 1); ++j)
-            for (int d = 0; d < 3; ++d) cf[j][d] = cb[j][d] = cs[j][d] = T(0);
-    }
-
-    __device__ inline T Wsel(T dr, T ha, T hb) const {
-        if constexpr (STRAT == PH_SYMMETRIC) return T(0.5) * (kernel_W<T, K>(dr, ha) + kernel_W<T, K>(dr, hb));
-        else if constexpr (STRAT == PH_GATHER) return kernel_W<T, K>(dr, ha);
-        else return kernel_W<T, K>(dr, hb);
-    }
-    __device__ inline void Gsel(T dx, T dy, T dz, T ha, T hb, T g[3]) const {
-        if constexpr (STRAT == PH_SYMMETRIC) {
-            T ax, ay, az, bx, by, bz;
-            kernel_gradW<T, K>(dx, dy, dz, ha, ax, ay, az);
-            kernel_gradW<T, K>(dx, dy, dz, hb, bx, by, bz);
-            g[0] = T(0.5) * (ax + bx);
-            g[1] = T(0.5) * (ay + by);
-            g[2] = T(0.5) * (az + bz);
-        } else {
-            kernel_gradW<T, K>(dx, dy, dz, STRAT == PH_GATHER ? ha : hb, g[0], g[1], g[2]);
-        }
-    }
-
-    // One accepted neighbour with packed record r = {m, rho, columns...}.
-    __device__ inline void add(const CatChunk& c, const T* __restrict__ r, T dx, T dy, T dz, T d2, T ha, T hb) {
-        const T dr = sqrt(d2);
-        const T mb = r[0], rb = r[1];
-        const T W = Wsel(dr, ha, hb);
-        S1 += mb * W / rb;
-#pragma unroll
-        for (int j = 0; j < MS; ++j)
-            if (j < c.ns) sc[j] += r[c.spos[j]] * (mb * W / rb);
-        if constexpr (MG + MD + MC > 0) {
-            if (c.ng + c.nd + c.nc == 0) return;
-            T g[3];
-            Gsel(dx, dy, dz, ha, hb, g);
-            const T invr = T(1) / rb;
-#pragma unroll
-            for (int j = 0; j < MG; ++j) {
-                if (j >= c.ng) break;
-                const int pos = c.gpos[j];
-                if (pos < 0) {  // density gradient (gradient_accumulation.jl:7-35)
-                    for (int d = 0; d < 3; ++d) {
-                        const T mg = mb * g[d];
-                        gf[j][d] += mg;
-                        gb[j][d] += mg * invr;
-                    }
-                    gs[j] += mb * W;
-                } else {
-                    const T A = r[pos];
-                    for (int d = 0; d < 3; ++d) {
-                        const T w = mb * invr * g[d];
-                        gf[j][d] += w * A;
-                        gb[j][d] += w;
-                    }
-                    gs[j] += A * (mb * W / rb);
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < MD; ++j) {
-                if (j >= c.nd) break;
-                T A[3], w[3];
-                for (int d = 0; d < 3; ++d) {
-                    A[d] = r[c.dpos[3 * j + d]];
-                    w[d] = mb * invr * g[d];
-                }
-                df[j] += w[0] * A[0] + w[1] * A[1] + w[2] * A[2];
-                for (int d = 0; d < 3; ++d) {
-                    db[j][d] += w[d];
-                    ds[j][d] += A[d] * (mb * W / rb);
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < MC; ++j) {
-                if (j >= c.nc) break;
-                T A[3], w[3];
-                for (int d = 0; d < 3; ++d) {
-                    A[d] = r[c.cpos[3 * j + d]];
-                    w[d] = mb * invr * g[d];
-                }
-                cf[j][0] += A[1] * w[2] - A[2] * w[1];
-                cf[j][1] += A[2] * w[0] - A[0] * w[2];
-                cf[j][2] += A[0] * w[1] - A[1] * w[0];
-                for (int d = 0; d < 3; ++d) {
-                    cb[j][d] += w[d];
-                    cs[j][d] += A[d] * (mb * W / rb);
-                }
-            }
-        }
-    }
-
-    // Epilogue (general_interpolation.jl:128-214); out[k*S + s].
-    __device__ inline void finish(const CatChunk& c, T* __restrict__ out, i64 S, i64 s) const {
-        if (S1 == T(0)) {
-            const T nan = sph_nan<T>();
-#pragma unroll
-            for (int j = 0; j < MS; ++j)
-                if (j < c.ns) out[c.sout[j] * S + s] = nan;
-#pragma unroll
-            for (int j = 0; j < MG; ++j)
-                if (j < c.ng)
-                    for (int d = 0; d < 3; ++d) out[(c.gout[j] + d) * S + s] = nan;
-#pragma unroll
-            for (int j = 0; j < MD; ++j)
-                if (j < c.nd) out[c.dout[j] * S + s] = nan;
-#pragma unroll
-            for (int j = 0; j < MC; ++j)
-                if (j < c.nc)
-                    for (int d = 0; d < 3; ++d) out[(c.cout[j] + d) * S + s] = nan;
-            return;
-        }
-        const T invS1 = T(1) / S1;
-#pragma unroll
-        for (int j = 0; j < MS; ++j)
-            if (j < c.ns) out[c.sout[j] * S + s] = c.sflag[j] ? sc[j] * invS1 : sc[j];
-#pragma unroll
-        for (int j = 0; j < MG; ++j)
-            if (j < c.ng) {
-                const T A = gs[j] * invS1;
-                for (int d = 0; d < 3; ++d) out[(c.gout[j] + d) * S + s] = gf[j][d] - gb[j][d] * A;
-            }
-#pragma unroll
-        for (int j = 0; j < MD; ++j)
-            if (j < c.nd) {
-                const T Ax = ds[j][0] * invS1, Ay = ds[j][1] * invS1, Az = ds[j][2] * invS1;
-                const T b = Ax * db[j][0] + Ay * db[j][1] + Az * db[j][2];
-                out[c.dout[j] * S + s] = df[j] - b;
-            }
-#pragma unroll
-        for (int j = 0; j < MC; ++j)
-            if (j < c.nc) {
-                const T Ax = cs[j][0] * invS1, Ay = cs[j][1] * invS1, Az = cs[j][2] * invS1;
-                const T bx = cb[j][0], by = cb[j][1], bz = cb[j][2];
-                const T qx = Ay * bz - Az * by, qy = Az * bx - Ax * bz, qz = Ax * by - Ay * bx;
-                out[(c.cout[j] + 0) * S + s] = -(cf[j][0] - qx);
-                out[(c.cout[j] + 1) * S + s] = -(cf[j][1] - qy);
-                out[(c.cout[j] + 2) * S + s] = -(cf[j][2] - qz);
-            }
-    }
-};
-
-// Sample generation.  SRC 0: list of points; SRC 1: Cartesian grid with 4x4x4 wave tiles.
-template <class T, int SRC> struct Sampler;
-template <class T> struct Sampler<T, 0> {
-    PointList pl;
-    i64 S;
-    __device__ inline bool get(i64& s, T& x, T& y, T& z) const {
-        s = xcd_block() * kBlock + threadIdx.x;
-        if (s >= S) return false;
-        x = T(pl.x[s]);
-        y = T(pl.y[s]);
-        z = T(pl.z[s]);
-        return true;
-    }
-};
-template <class T> struct Sampler<T, 1> {
-    GridAxes g;
-    i64 S;
-    __device__ inline bool get(i64& s, T& x, T& y, T& z) const {
-        const i64 tile = xcd_block() * (kBlock / kWave) + threadIdx.x / kWave;
-        const int lane = threadIdx.x & (kWave - 1);
-        const i64 tx = (g.nx + 3) / 4, ty = (g.ny + 3) / 4;
-        const i64 bx = tile % tx, by = (tile / tx) % ty, bz = tile / (tx * ty);
-        const i64 ix = bx * 4 + (lane & 3), iy = by * 4 + ((lane >> 2) & 3), iz = bz * 4 + (lane >> 4);
-        if (ix >= g.nx || iy >= g.ny || iz >= g.nz) return false;
-        s = ix + g.nx * (iy + g.ny * iz);
-        x = T(g.ax[ix]);
-        y = T(g.ay[iy]);
-        z = T(g.az[iz]);
-        return true;
-    }
-};
-
-__device__ inline void count_add(u64* counter, u64 v) {
-    if (!counter) return;
-    // wave reduction, one atomic per wave
-    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0 && v) atomicAdd((unsigned long long*)counter, (unsigned long long)v);
-}
-
-// Packs {m, rho, columns...} of every particle into records of width c.rw.
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_pack_records(const T* __restrict__ cols, i64 n, CatChunk c, T* __restrict__ rec) {
-    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    T* r = rec + i * c.rw;
-    for (int j = 0; j < c.rw; ++j) r[j] = j < c.nsrc ? cols[(i64)c.rsrc[j] * n + i] : T(0);
-}
-
 constexpr int kBatch = 16;  // accepted leaves buffered per lane between accumulation rounds
 
 // Point samples.  Each lane walks its own escape-rope DFS, buffering accepted
@@ -463,14 +72,6 @@ constexpr int kBatch = 16;  // accepted leaves buffered per lane between accumul
 // the whole wave accumulates its buffers in lockstep.  The expensive per-neighbour
 // arithmetic thus runs without the traversal's divergence, and each sample still
 // accumulates its neighbours in the reference's order.
-// Smoothing length of the nearest particle of every sample (gather / symmetric).
-template <class T, int SRC>
-__global__ void __launch_bounds__(kBlock) k_nearest_samples(TreeView<T> tv, Sampler<T, SRC> smp, T* __restrict__ ha) {
-    i64 s;
-    T px, py, pz;
-    if (smp.get(s, px, py, pz)) ha[s] = nearest_h<T>(tv, px, py, pz);
-}
-
 template <class T, int K, int STRAT, int MS, int MG, int MD, int MC, int SRC>
 __global__ void __launch_bounds__(kBlock) k_points(TreeView<T> tv, const T* __restrict__ rec, CatChunk c,
                                                    Sampler<T, SRC> smp, const T* __restrict__ ha_s, T* __restrict__ out,

@@ -5,8 +5,10 @@ The library is loaded from the package directory (partia.jl_amd/libpartia_hip.so
 there is no fallback: if it is missing or cannot be loaded, NativeLibraryError is
 raised.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -18,12 +20,14 @@ LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libpartia_hip.so"))
 PH_OK = 0
 PH_GATHER, PH_SCATTER, PH_SYMMETRIC = 0, 1, 2
 PH_PREC_FP64, PH_PREC_FP32, PH_PREC_FP32C = 64, 32, 33
+PH_TRAVERSAL_AUTO, PH_TRAVERSAL_LANE, PH_TRAVERSAL_BUCKET = 0, 1, 2
 GET = dict(order=0, codes=1, left=2, right=3, escape=4, parent=5, leaf_coor=6, leaf_scale=7, node_min=8,
            node_max=9, node_scale=10, columns=11)
 
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTS = (
     "ph_ctx_create", "ph_ctx_destroy", "ph_last_error", "ph_get_stats", "ph_set_count_neighbours",
+    "ph_set_traversal",
     "ph_device_alloc", "ph_device_free", "ph_memcpy_htod", "ph_memcpy_dtoh", "ph_synchronize",
     "ph_lbvh_build", "ph_lbvh_build_device", "ph_lbvh_destroy", "ph_lbvh_set_columns", "ph_lbvh_info",
     "ph_lbvh_get",
@@ -74,6 +78,7 @@ def lib():
         "ph_last_error": (C.c_char_p, [vp]),
         "ph_get_stats": (i32, [vp, P(ph_stats)]),
         "ph_set_count_neighbours": (i32, [vp, C.c_int]),
+        "ph_set_traversal": (i32, [vp, C.c_int]),
         "ph_device_alloc": (i32, [vp, i64, P(vp)]),
         "ph_device_free": (i32, [vp, vp]),
         "ph_memcpy_htod": (i32, [vp, vp, vp, i64]),
@@ -114,6 +119,28 @@ def f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+# Live handles, released before the interpreter exits: trees first, then their
+# contexts, while the HIP runtime is still up (objects kept alive until shutdown,
+# e.g. by a traceback, would otherwise be freed after the runtime's own teardown).
+_trees = weakref.WeakSet()
+_contexts = weakref.WeakSet()
+
+
+def register_tree(obj):
+    """Track an object owning a ph_lbvh handle (it must have close())."""
+    _trees.add(obj)
+
+
+@atexit.register
+def _release_all():
+    for group in (_trees, _contexts):
+        for obj in list(group):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
 class Context:
     """One GPU + one HIP stream (ph_context)."""
 
@@ -125,6 +152,7 @@ class Context:
             raise_for_status(rc, L.ph_last_error(None).decode())
         self.handle = h
         self.device = device
+        _contexts.add(self)
 
     def check(self, rc):
         if rc != PH_OK:
@@ -137,6 +165,10 @@ class Context:
 
     def set_count_neighbours(self, enable):
         self.check(lib().ph_set_count_neighbours(self.handle, 1 if enable else 0))
+
+    def set_traversal(self, mode):
+        """PH_TRAVERSAL_AUTO, PH_TRAVERSAL_LANE or PH_TRAVERSAL_BUCKET (point and grid calls)."""
+        self.check(lib().ph_set_traversal(self.handle, int(mode)))
 
     def close(self):
         if getattr(self, "handle", None):

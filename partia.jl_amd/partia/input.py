@@ -131,6 +131,7 @@ class LinearBVH:
     def __init__(self, backend, handle, n, ncol, kernel, code_bits, dtype):
         self.backend = backend
         self.handle = handle
+        N.register_tree(self)
         self.n = n
         self.ncol = ncol
         self.kernel = kernel

@@ -20,6 +20,7 @@ class DeviceSet:
         cl = [N.f64(c) for c in cols]
         cptr = (C.c_void_p * max(1, len(cl)))(*[c.ctypes.data for c in cl])
         self.h = C.c_void_p()
+        N.register_tree(self)
         ctx.check(N.lib().ph_lbvh_build(ctx.handle, self.n, *[N.ptr(a) for a in arrs], len(cl), cptr, kernel,
                                         code_bits, precision, C.byref(self.h)))
 
@@ -119,3 +120,58 @@ def assert_same(got, ref, bitwise=True, rtol=1e-12):
 
 def oracle_of(x, y, z, m, h, rho, cols=(), kernel=0, code_bits=63, precision=64):
     return O.OracleSet(x, y, z, m, h, rho, cols, kernel=kernel, code_bits=code_bits, precision=precision)
+
+
+def catalog_fields(cat):
+    """Output rows of each field of a catalog, in output order: one row per scalar
+    and divergence, three per gradient and curl (a vector field is one field)."""
+    ns, ng = len(cat.get("sslots", ())), len(cat.get("gslots", ()))
+    nd, nc = len(cat.get("dslots", ())) // 3, len(cat.get("cslots", ())) // 3
+    groups, r = [], 0
+    for size, count in ((1, ns), (3, ng), (1, nd), (3, nc)):
+        for _ in range(count):
+            groups.append(list(range(r, r + size)))
+            r += size
+    return groups
+
+
+def field_errors(got, ref, floor=0.0, fields=None):
+    """Per field (a list of output rows; default one row each): normwise
+    ||g - f||_2 / max(||f||_2, floor sqrt(count)) and the pointwise maximum of
+    |g - f| / max(|f|, 1e-3 rms f, floor) over the samples where ref is finite
+    (NaN masks must be identical).  floor is an absolute scale for fields that
+    vanish identically (e.g. the gradient of one particle), where the reference
+    holds only rounding noise; a scalar or one value per field."""
+    assert got.shape == ref.shape
+    assert np.array_equal(np.isnan(got), np.isnan(ref)), "NaN masks differ"
+    if fields is None:
+        fields = [[k] for k in range(ref.shape[0])]
+    floors = np.broadcast_to(np.asarray(floor, dtype=float), (len(fields),))
+    out = []
+    for fl, rows in zip(floors, fields):
+        floor = float(fl)
+        f = ~np.isnan(ref[rows[0]])
+        if not f.any():
+            out.append((0.0, 0.0))
+            continue
+        g, r = got[rows][:, f], ref[rows][:, f]
+        nr = max(np.linalg.norm(r), floor * np.sqrt(r.size), 1e-300)
+        norm = np.linalg.norm(g - r) / nr
+        rms = np.sqrt(np.mean(r * r))
+        point = np.max(np.abs(g - r) / np.maximum(np.maximum(np.abs(r), 1e-3 * rms), max(floor, 1e-300)))
+        out.append((float(norm), float(point)))
+    return out
+
+
+class traversal:
+    """Context manager selecting the traversal of a context, restoring AUTO."""
+
+    def __init__(self, ctx, mode):
+        self.ctx, self.mode = ctx, mode
+
+    def __enter__(self):
+        self.ctx.set_traversal(self.mode)
+        return self
+
+    def __exit__(self, *a):
+        self.ctx.set_traversal(N.PH_TRAVERSAL_AUTO)
