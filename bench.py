@@ -10,10 +10,13 @@ rho, A (Shepard), grad A, div v, curl v (L = 9) on a 256^3 Cartesian grid over
 refit; the particle columns are already in HBM) + the StructuredGrid
 interpolation (grid coordinates generated on the device, outputs left in HBM).
 
-Prints one JSON line.  `value` is samples/s of whole steps (build included);
-the interpolation-only rate, the build time, the dominant kernel's roofline and
-a CPU baseline (the C++ oracle on this host's cores, on a bounded subset of the
-samples) are reported beside it.
+Prints one JSON line.  `value` is samples/s of whole steps (build included) in
+the FP32-compensated precision (float64 input, float32 kernel evaluation,
+float64 accumulation; <= 1e-6 normwise error against float64, checked by the
+GPU tests); the same steps in float64 are timed beside it for the speedup.  The
+interpolation-only rate, the build time, the dominant kernel's roofline and a
+CPU baseline (the C++ oracle on this host's cores, on a bounded subset of the
+samples) are reported too.
 """
 import argparse
 import ctypes as C
@@ -49,7 +52,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", choices=["fp64", "fp32c"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32c"], default="fp32c")
+    ap.add_argument("--no-fp64-compare", action="store_true", help="skip timing the float64 path beside FP32C")
     ap.add_argument("--traversal", choices=["auto", "lane", "bucket"], default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
@@ -72,6 +76,20 @@ class Dev:
         if self.p:
             N.lib().ph_device_free(self.ctx.handle, self.p)
             self.p = C.c_void_p()
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-probe
+    calibration); None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", "%s_plummer2e6_c4_symmetric_grid256_hbm.md" % kernel)
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def cpu_baseline(data, axes, target_seconds):
@@ -140,12 +158,12 @@ def main():
         dcols.append(d)
     dax = Dev(ctx, 8 * NGRID)
     dax.upload(axes)
-    out_bytes = (8 if prec == N.PH_PREC_FP64 else 4) * L_OUT * S
+    out_bytes = 8 * L_OUT * S  # room for float64 outputs (the float64 comparison)
     dout = Dev(ctx, out_bytes)
     cat = N.CatalogArgs(**CATALOG)
     qcols = (C.c_void_p * 5)(*[d.p.value for d in dcols[6:]])
 
-    def step():
+    def step(prec=prec):
         h = C.c_void_p()
         ctx.check(L.ph_lbvh_build_device(ctx.handle, NPART, *[d.p for d in dcols[:6]], 5, qcols, KERNEL_C4, 63,
                                          N.PH_PREC_FP64, C.byref(h)))
@@ -183,9 +201,22 @@ def main():
     interp_ms = float(np.mean(interps))
     build_ms = float(np.mean(builds))
 
+    fp64 = None
+    if prec != N.PH_PREC_FP64 and not args.no_fp64_compare:
+        step(N.PH_PREC_FP64)  # warm-up
+        ctx.check(L.ph_synchronize(ctx.handle))
+        t1 = time.perf_counter()
+        k64 = [step(N.PH_PREC_FP64)[2] for _ in range(args.steps)]
+        ctx.check(L.ph_synchronize(ctx.handle))
+        ms64 = 1e3 * (time.perf_counter() - t1) / args.steps
+        fp64 = {"ms_per_step": ms64, "samples_per_s": S / (ms64 * 1e-3), "accumulation_kernel_ms": float(np.mean(k64))}
+        print("bench: float64 %.2f ms per step (accumulation kernel %.2f ms)" % (ms64, fp64["accumulation_kernel_ms"]),
+              file=sys.stderr, flush=True)
+
     fbytes = 8 if prec == N.PH_PREC_FP64 else 4
     alg_bytes = n_acc * FIELDS_PER_NEIGHBOUR * fbytes + S * (3 + L_OUT) * fbytes
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic("k_bucket_f64" if prec == N.PH_PREC_FP64 else "k_bucket_c32")
     result = {
         "metric": METRIC,
         "value": S / (ms_per_step * 1e-3),
@@ -209,8 +240,11 @@ def main():
         "accepted_neighbours": int(n_acc),
         "guard_band_rechecks": int(n_rechecks),
         "traversal": args.traversal,
+        "fp64": fp64,
+        "speedup_vs_fp64": (fp64["ms_per_step"] / ms_per_step) if fp64 else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes": int(alg_bytes)},
     }
     for d in dcols + [dax, dout]:
         d.free()

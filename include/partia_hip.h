@@ -96,7 +96,7 @@ typedef struct ph_stats {
     double kernel_ms;       /* last call's dominant (traversal + accumulation) kernel */
     int64_t samples;        /* samples (points or rays) of the last call             */
     int64_t accepted;       /* accepted neighbours of the last call (if counted)      */
-    int64_t rechecks;       /* FP64 guard-band rechecks of the last FP32C call        */
+    int64_t rechecks;       /* reserved (0): leaves are always tested in float64      */
 } ph_stats;
 
 /* ---- context ---- */
@@ -112,7 +112,7 @@ int ph_set_count_neighbours(ph_context* ctx, int enable);
  *   PH_TRAVERSAL_LANE    one lane per sample, escape-rope DFS over the LBVH leaf
  *                        by leaf (LinearBVHTraversalMacros.jl:26-371);
  *   PH_TRAVERSAL_BUCKET  one lane per sample, DFS over a preorder copy of the
- *                        LBVH whose subtrees of <= 16 leaves are buckets whose
+ *                        LBVH whose subtrees of <= 4 leaves are buckets whose
  *                        leaves are tested together (float32 pruning rounded
  *                        outward, float64 leaf tests);
  *   PH_TRAVERSAL_AUTO    bucket.

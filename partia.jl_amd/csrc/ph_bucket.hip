@@ -26,14 +26,14 @@
 //
 // The float64 variant accumulates with the float64 code of the per-lane path, so
 // its results are bit-identical to the oracle.  The FP32-compensated variant
-// evaluates kernels and products in float32 on differences rounded once from
-// float64 (sample - particle), sums them per 16-leaf column chunk in float32 and
-// carries the chunk sums into float64 totals; every field column enters as an
-// offset from a float64 per-chunk anchor, so that
-//   sum_b A_b w_b = sum_k (Abar_k W_k + sum_{b in k} (A_b - Abar_k) w_b)
-// keeps the large part in float64.  The Shepard division and the mean
-// subtraction of the derivative operators (general_interpolation.jl:128-214) are
-// float64.
+// evaluates kernel values and gradients in float32 from the float64 distance
+// (the kernel argument q = r/h carries a single rounding; v = m/rho and 1/h are
+// rounded once), and accumulates every term into float64 totals with the
+// float64 field values: on MI355X float64 FMA issues at the float32 rate, so
+// float64 accumulators cost less than float32 two-sum accumulators and carry the
+// sums exactly to float64 rounding.  The Shepard division and the mean
+// subtraction of the derivative operators (general_interpolation.jl:128-214)
+// are float64; outputs are float32.
 #include "ph_interp.h"
 #include "ph_interp_dev.h"
 #include "ph_primitives.h"
@@ -42,8 +42,21 @@
 namespace ph {
 namespace {
 
-constexpr int kBucket = 16;       // leaves per bucket at most (count stored in 5 bits)
-constexpr int kQ = 4;             // buckets tested per batch: kQ * kBucket = 64 mask bits
+// Bucket geometry (compile-time; kQ * kBucket mask bits per lane and batch).
+#ifndef PH_BUCKET_LEAVES
+#define PH_BUCKET_LEAVES 4
+#endif
+#ifndef PH_BUCKET_BATCH
+#define PH_BUCKET_BATCH 16
+#endif
+#ifndef PH_ACC_UNROLL
+#define PH_ACC_UNROLL 1
+#endif
+constexpr int kBucket = PH_BUCKET_LEAVES;  // leaves per bucket at most (count stored in 5 bits)
+constexpr int kLog2Bucket = kBucket == 4 ? 2 : (kBucket == 8 ? 3 : (kBucket == 16 ? 4 : 5));
+constexpr int kQ = PH_BUCKET_BATCH;        // buckets tested per batch
+static_assert(kBucket == 4 || kBucket == 8 || kBucket == 16 || kBucket == 32, "bucket size 4, 8, 16 or 32");
+static_assert(kQ * kBucket <= 64, "one 64-bit mask per lane and batch");
 constexpr i64 kMaxBucketLeaves = i64(1) << 26;  // leaf index of a bucket stored in 26 bits
 
 // Preorder record: link >= 0 -> internal node, link = index after its subtree;
@@ -53,6 +66,26 @@ struct alignas(16) BucketRec {
     float hi[3];
     float scale;
     i32 link;
+};
+
+// Raw buffer view (gfx9 descriptor, word 3 = 0x00020000): loads take a per-lane
+// byte offset in a VGPR and a uniform one in an SGPR, so a field at a run-time
+// (uniform) position costs no vector address arithmetic.  Offsets and the size
+// are unsigned 32-bit (buffers below 4 GiB).
+struct Buf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ static Buf of(const void* p, i64 bytes) {
+        return Buf{__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)(u32)bytes, 0x00020000)};
+    }
+    __device__ double f64(u32 voff, u32 soff) const {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+    }
+    __device__ LeafRec<double> leaf(u32 voff) const {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, 0, 0);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff + 16, 0, 0);
+        const double2 xy = __builtin_bit_cast(double2, a), zh = __builtin_bit_cast(double2, b);
+        return LeafRec<double>{xy.x, xy.y, zh.x, zh.y};
+    }
 };
 
 __device__ inline i32 bucket_link(i64 lo0, i64 cnt) { return -(i32)(1 + ((lo0 << 5) | (cnt - 1))); }
@@ -177,11 +210,18 @@ __global__ void k_bucket_root(i64 n, i32 root, const NodeRec<double>* __restrict
 
 // ---- per-call records of the FP32-compensated path ----
 
-// {m/rho, 1/h} per particle in float32 (rounded once from float64).
-__global__ void __launch_bounds__(kBlock) k_pack_vih(const double* __restrict__ cols, i64 n, float2* __restrict__ vih) {
+// FP32-compensated record per particle, rw doubles: element 0 holds {m/rho, 1/h}
+// as two float32 (rounded once from float64), elements 1..ncol the float64
+// values of the distinct columns the catalog chunk uses (source c32.rsrc[1 + j]).
+__global__ void __launch_bounds__(kBlock) k_pack_c32(const double* __restrict__ cols, i64 n, CatChunk c32, int ncol,
+                                             int rw, double* __restrict__ rec) {
     const i64 b = (i64)blockIdx.x * kBlock + threadIdx.x;
     if (b >= n) return;
-    vih[b] = make_float2((float)(cols[3 * n + b] / cols[5 * n + b]), (float)(1.0 / cols[4 * n + b]));
+    double* r = rec + b * rw;
+    const float2 vi = make_float2((float)(cols[3 * n + b] / cols[5 * n + b]), (float)(1.0 / cols[4 * n + b]));
+    r[0] = __builtin_bit_cast(double, vi);
+    for (int j = 0; j < ncol; ++j) r[1 + j] = cols[(i64)c32.rsrc[1 + j] * n + b];
+    for (int j = 1 + ncol; j < rw; ++j) r[j] = 0.0;
 }
 
 // ---- FP32-compensated accumulator ----
@@ -241,8 +281,8 @@ template <int STRAT> __device__ inline float radius2_32(float Kv, float r2a, flo
 // float64 totals with the float64 field values, where the products are exact to
 // float64 rounding.  Every output keeps its own totals (static register
 // indices), and a lane without a pending neighbour adds exact zeros, so the
-// lockstep loop has no divergent branch.  Record positions index the float64
-// record {m, rho, columns...} of the float64 path.  EXACT: the catalog chunk has
+// lockstep loop has no divergent branch.  Record positions index the
+// FP32-compensated record (position 0: {m/rho, 1/h}; >= 1: float64 columns).  EXACT: the catalog chunk has
 // exactly MS scalars, MG gradients, MD divergences and MC curls; otherwise the
 // unused output slots read position 0 and add zeros (no branch either).
 template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT> struct AccC {
@@ -267,16 +307,18 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT> struct A
     }
 
     // One neighbour (live = false: adds zeros): d = sample - particle and d2 in
-    // float64, r = float64 record, vi = {m/rho, 1/h}; iha2 = (1/ha)^2 and the
-    // float32 norms of the sample's kernel.
-    __device__ __attribute__((always_inline)) void add(bool live, const CatChunk& c, const double* __restrict__ r,
-                                                       float2 vi, double dx, double dy, double dz, double d2,
-                                                       double iha2, float Wa_norm, float Ga_norm) {
-        const float r32 = sqrtf((float)d2);
+    // float64, r = record; iha2 = (1/ha)^2 and the float32 norms of the sample's
+    // kernel.
+    __device__ __attribute__((always_inline)) void add(bool live, const CatChunk& c, const Buf& rb, u32 voff,
+                                                       double dx, double dy, double dz, double d2, double iha2,
+                                                       float Wa_norm, float Ga_norm) {
+        auto r = [&](int pos) { return rb.f64(voff, (u32)pos * 8u); };
+        const float2 vi = __builtin_bit_cast(float2, r(0));
+        const float d2f = (float)d2;
         float W, coef;
         if constexpr (STRAT == PH_GATHER) {
             float f, df;
-            shape32<K>(sqrtf((float)(d2 * iha2)), f, df);
+            shape32<K>(__builtin_amdgcn_sqrtf((float)(d2 * iha2)), f, df);
             W = Wa_norm * f;
             coef = Ga_norm * df;
         } else {
@@ -285,14 +327,14 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT> struct A
             const double ibd = ib;
             const float ib3 = ib * ib * ib;
             float fb, dfb;
-            shape32<K>(sqrtf((float)(d2 * (ibd * ibd))), fb, dfb);
+            shape32<K>(__builtin_amdgcn_sqrtf((float)(d2 * (ibd * ibd))), fb, dfb);
             const float Wb = (sig * ib3) * fb, Gb = (sig * ib3 * ib) * dfb;
             if constexpr (STRAT == PH_SCATTER) {
                 W = Wb;
                 coef = Gb;
             } else {
                 float fa, dfa;
-                shape32<K>(sqrtf((float)(d2 * iha2)), fa, dfa);
+                shape32<K>(__builtin_amdgcn_sqrtf((float)(d2 * iha2)), fa, dfa);
                 W = 0.5f * (Wa_norm * fa + Wb);
                 coef = 0.5f * (Ga_norm * dfa + Gb);
             }
@@ -304,11 +346,11 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT> struct A
 #pragma unroll
         for (int j = 0; j < MS; ++j) {
             const double wj = (EXACT || j < c.ns) ? w : 0.0;
-            ts[j] = fma(r[c.spos[j]], wj, ts[j]);
+            ts[j] = fma(r(c.spos[j]), wj, ts[j]);
         }
         if constexpr (MG + MD + MC > 0) {
             // grad W = delta f'(q) sigma / (r h^4); zero at r = 0 (kernel.jl:273-285)
-            const double s = (live && d2 > 0.0) ? (double)(vi.x * coef / r32) : 0.0;
+            const double s = (live && d2 > 0.0) ? (double)(vi.x * coef * __builtin_amdgcn_rsqf(d2f)) : 0.0;
             const double g0 = dx * s, g1 = dy * s, g2 = dz * s;
             Q[0] += g0;
             Q[1] += g1;
@@ -316,8 +358,7 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT> struct A
 #pragma unroll
             for (int j = 0; j < MG; ++j) {
                 const bool on = EXACT || j < c.ng;
-                // reference grad slot 0: gradient of rho (record position 1)
-                const double A = on ? r[c.gpos[j] < 0 ? 1 : c.gpos[j]] : 0.0;
+                const double A = on ? r(c.gpos[j]) : 0.0;
                 tgs[j] = fma(A, w, tgs[j]);
                 tg[j][0] = fma(A, g0, tg[j][0]);
                 tg[j][1] = fma(A, g1, tg[j][1]);
@@ -326,8 +367,8 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT> struct A
 #pragma unroll
             for (int j = 0; j < MD; ++j) {
                 const bool on = EXACT || j < c.nd;
-                const double ax = on ? r[c.dpos[3 * j]] : 0.0, ay = on ? r[c.dpos[3 * j + 1]] : 0.0,
-                             az = on ? r[c.dpos[3 * j + 2]] : 0.0;
+                const double ax = on ? r(c.dpos[3 * j]) : 0.0, ay = on ? r(c.dpos[3 * j + 1]) : 0.0,
+                             az = on ? r(c.dpos[3 * j + 2]) : 0.0;
                 tds[j][0] = fma(ax, w, tds[j][0]);
                 tds[j][1] = fma(ay, w, tds[j][1]);
                 tds[j][2] = fma(az, w, tds[j][2]);
@@ -336,8 +377,8 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT> struct A
 #pragma unroll
             for (int j = 0; j < MC; ++j) {
                 const bool on = EXACT || j < c.nc;
-                const double ax = on ? r[c.cpos[3 * j]] : 0.0, ay = on ? r[c.cpos[3 * j + 1]] : 0.0,
-                             az = on ? r[c.cpos[3 * j + 2]] : 0.0;
+                const double ax = on ? r(c.cpos[3 * j]) : 0.0, ay = on ? r(c.cpos[3 * j + 1]) : 0.0,
+                             az = on ? r(c.cpos[3 * j + 2]) : 0.0;
                 tcs[j][0] = fma(ax, w, tcs[j][0]);
                 tcs[j][1] = fma(ay, w, tcs[j][1]);
                 tcs[j][2] = fma(az, w, tcs[j][2]);
@@ -449,21 +490,14 @@ template <int STRAT> struct Probe {
     }
 };
 
-// Per-lane list of up to kQ buckets (encoded links), in separate registers.
-static_assert(kQ == 4, "BucketList holds four buckets");
+// Per-lane list of up to kQ buckets in LDS (stride kBlock): the bucket code
+// -link - 1 = first leaf << 5 | (count - 1).
 struct BucketList {
-    i32 b0, b1, b2, b3;
+    u32* slot;
     int n;
-    __device__ __attribute__((always_inline)) void push(i32 link) {
-        if (n == 0) b0 = link;
-        else if (n == 1) b1 = link;
-        else if (n == 2) b2 = link;
-        else b3 = link;
-        ++n;
-    }
-    __device__ __attribute__((always_inline)) i32 get(int q) const {
-        return q == 0 ? b0 : (q == 1 ? b1 : (q == 2 ? b2 : b3));
-    }
+    __device__ __attribute__((always_inline)) void push(i32 link) { slot[n++ * kBlock] = (u32)(-link - 1); }
+    __device__ __attribute__((always_inline)) u32 code(int q) const { return slot[q * kBlock]; }
+    __device__ __attribute__((always_inline)) u32 lo(int q) const { return code(q) >> 5; }
 };
 
 // Walks from record i until kQ buckets are collected or the tree is done.
@@ -492,9 +526,9 @@ __device__ __attribute__((always_inline)) u64 test_buckets(const BucketView& bv,
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
         if (q < bl.n) {
-            const i32 link = bl.get(q);
-            const i64 lo = bucket_lo(link);
-            const int cnt = bucket_cnt(link);
+            const u32 code = bl.code(q);
+            const i64 lo = code >> 5;
+            const int cnt = (int)(code & 31u) + 1;
 #pragma unroll 4
             for (int j = 0; j < cnt; ++j) {
                 const LeafRec<double> L = bv.leaves[lo + j];
@@ -577,8 +611,7 @@ __global__ void __launch_bounds__(kBlock) k_nearest_bucket(BucketView bv, Sample
 
 
 template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT, int SRC>
-__global__ void __launch_bounds__(kBlock) k_bucket_c32(BucketView bv, const double* __restrict__ rec,
-                                                       const float2* __restrict__ vih, CatChunk c,
+__global__ void __launch_bounds__(kBlock) k_bucket_c32(BucketView bv, const double* __restrict__ rec, CatChunk c,
                                                        Sampler<double, SRC> smp, const double* __restrict__ ha_s,
                                                        float* __restrict__ out, u64* counter) {
     i64 s = 0;
@@ -599,39 +632,48 @@ __global__ void __launch_bounds__(kBlock) k_bucket_c32(BucketView bv, const doub
     const float Wa_norm = sig * inv_ha * inv_ha * inv_ha, Ga_norm = Wa_norm * inv_ha;
     const int rw = c.rw;
 
-    // first leaf of each listed bucket, per lane: read back with a per-lane index
-    // in the accumulation loop, which thus has no divergent branch
-    __shared__ u32 blo[kQ][kBlock];
+    // per-lane bucket list in LDS: read back with a per-lane index in the
+    // accumulation loop, which thus has no divergent branch
+    __shared__ u32 blist[kQ][kBlock];
     const int tid = threadIdx.x;
     const u32 nlast = (u32)(bv.n - 1);
+    const Buf rb = Buf::of(rec, (i64)bv.n * rw * 8);
+    const Buf lb = Buf::of(bv.leaves, (i64)bv.n * (i64)sizeof(LeafRec<double>));
 
     AccC<K, STRAT, MS, MG, MD, MC, EXACT> acc;
     acc.init();
     u64 accepted = 0;
     i32 i = valid ? 0 : bv.nrec;
     i64 steps = 2 * (i64)bv.nrec + 8;  // a valid tree is walked in at most nrec steps
-    BucketList bl;
+    BucketList bl{&blist[0][tid], 0};
     while (true) {
         collect<STRAT>(bv, pr, i, steps, bl);
-#pragma unroll
-        for (int q = 0; q < kQ; ++q) blo[q][tid] = q < bl.n ? (u32)bucket_lo(bl.get(q)) : 0u;
         u64 mask = test_buckets<STRAT>(bv, bl, p, Kd, r2a);
         accepted += __popcll(mask);
         // lockstep: one neighbour per lane and iteration, lowest bit first
         // (increasing leaf index); a lane with none left adds zeros
         int nit = __popcll(mask);
         for (int d = kWave / 2; d > 0; d >>= 1) nit = max(nit, __shfl_xor(nit, d, kWave));
-        nit = __builtin_amdgcn_readfirstlane(nit);
+        nit = __builtin_amdgcn_readfirstlane((nit + PH_ACC_UNROLL - 1) / PH_ACC_UNROLL);
         for (int it = 0; it < nit; ++it) {
-            const bool live = mask != 0;
-            const int b = __builtin_ctzll(mask | (1ull << 63));
-            mask &= mask - 1;
-            const u32 kk = blo[b >> 4][tid] + (u32)(b & (kBucket - 1));
-            const u32 k = kk < nlast ? kk : nlast;
-            const LeafRec<double> L = bv.leaves[k];
-            const double dx = p[0] - L.x, dy = p[1] - L.y, dz = p[2] - L.z;
-            acc.add(live, c, rec + (size_t)k * (u32)rw, vih[k], dx, dy, dz, (dx * dx + dy * dy) + dz * dz, iha2,
-                    Wa_norm, Ga_norm);
+            bool live[PH_ACC_UNROLL];
+            u32 k[PH_ACC_UNROLL];
+            LeafRec<double> L[PH_ACC_UNROLL];
+#pragma unroll
+            for (int u = 0; u < PH_ACC_UNROLL; ++u) {  // loads of all unrolled neighbours first
+                live[u] = mask != 0;
+                const int b = __builtin_ctzll(mask | (1ull << 63));
+                mask &= mask - 1;
+                const u32 kk = bl.lo(b >> kLog2Bucket) + (u32)(b & (kBucket - 1));
+                k[u] = kk < nlast ? kk : nlast;
+                L[u] = lb.leaf(k[u] * (u32)sizeof(LeafRec<double>));
+            }
+#pragma unroll
+            for (int u = 0; u < PH_ACC_UNROLL; ++u) {
+                const double dx = p[0] - L[u].x, dy = p[1] - L[u].y, dz = p[2] - L[u].z;
+                acc.add(live[u], c, rb, k[u] * (u32)rw * 8u, dx, dy, dz, (dx * dx + dy * dy) + dz * dz, iha2, Wa_norm,
+                        Ga_norm);
+            }
         }
         if (!__any(i < bv.nrec && steps > 0)) break;
     }
@@ -659,7 +701,8 @@ __global__ void __launch_bounds__(kBlock) k_bucket_f64(BucketView bv, const doub
     u64 accepted = 0;
     i32 i = valid ? 0 : bv.nrec;
     i64 steps = 2 * (i64)bv.nrec + 8;
-    BucketList bl;
+    __shared__ u32 blist[kQ][kBlock];
+    BucketList bl{&blist[0][threadIdx.x], 0};
     while (true) {
         collect<STRAT>(bv, pr, i, steps, bl);
         u64 mask = test_buckets<STRAT>(bv, bl, p, Kd, r2a);
@@ -668,7 +711,7 @@ __global__ void __launch_bounds__(kBlock) k_bucket_f64(BucketView bv, const doub
             if (mask != 0) {
                 const int b = __builtin_ctzll(mask);
                 mask &= mask - 1;
-                const i64 k = bucket_lo(bl.get(b >> 4)) + (b & (kBucket - 1));
+                const i64 k = (i64)bl.lo(b >> kLog2Bucket) + (b & (kBucket - 1));
                 const LeafRec<double> L = bv.leaves[k];
                 acc.add(c, rec + k * c.rw, p[0] - L.x, p[1] - L.y, p[2] - L.z, d2_point(p[0], p[1], p[2], L), ha, L.h);
             }
@@ -681,6 +724,28 @@ __global__ void __launch_bounds__(kBlock) k_bucket_f64(BucketView bv, const doub
 
 BucketView bucket_view(const ph_lbvh* B) {
     return BucketView{B->n, (i32)B->nrec, B->brecs.as<BucketRec>(), B->leaves.as<LeafRec<double>>()};
+}
+
+// Compact FP32-compensated record layout of a catalog chunk: positions 1..ncol
+// are the distinct source columns the chunk reads (rsrc[1 + j]); the density
+// gradient (reference grad slot 0) reads the rho column.
+CatChunk compact_c32(const CatChunk& c, int& ncol) {
+    CatChunk o = c;
+    ncol = 0;
+    auto map = [&](int src) {
+        for (int j = 0; j < ncol; ++j)
+            if (o.rsrc[1 + j] == src) return 1 + j;
+        o.rsrc[1 + ncol] = src;
+        return 1 + ncol++;
+    };
+    for (int j = 0; j < c.ns; ++j) o.spos[j] = map(c.rsrc[c.spos[j]]);
+    for (int j = 0; j < c.ng; ++j) o.gpos[j] = map(c.gpos[j] < 0 ? 5 : c.rsrc[c.gpos[j]]);
+    for (int j = 0; j < 3 * c.nd; ++j) o.dpos[j] = map(c.rsrc[c.dpos[j]]);
+    for (int j = 0; j < 3 * c.nc; ++j) o.cpos[j] = map(c.rsrc[c.cpos[j]]);
+    for (int j = c.ns; j < kMaxS; ++j) o.spos[j] = 0;
+    o.nsrc = 1 + ncol;
+    o.rw = (1 + ncol + 1) & ~1;
+    return o;
 }
 
 inline unsigned sample_blocks(const GridAxes* ga, i64 S) {
@@ -699,11 +764,11 @@ void launch_c32_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const Grid
     if (ev) PH_HIP(hipEventRecord(ev[0], s));
     if (ga)
         k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 1><<<g, kBlock, 0, s>>>(
-            bv, B->rec.as<double>(), B->rec32.as<float2>(), c, Sampler<double, 1>{*ga, S}, ha,
+            bv, B->rec32.as<double>(), c, Sampler<double, 1>{*ga, S}, ha,
             static_cast<float*>(out), counters);
     else
         k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 0><<<g, kBlock, 0, s>>>(
-            bv, B->rec.as<double>(), B->rec32.as<float2>(), c, Sampler<double, 0>{*pl, S}, ha,
+            bv, B->rec32.as<double>(), c, Sampler<double, 0>{*pl, S}, ha,
             static_cast<float*>(out), counters);
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[1], s));
@@ -743,21 +808,25 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
         ha = B->ha.as<double>();
     }
     const bool small = c.ng + c.nd + c.nc == 0 && c.ns <= 2;
-    B->rec.alloc(sizeof(double) * (size_t)c.rw * (size_t)n);
-    k_pack_records<double><<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c, B->rec.as<double>());
-    PH_HIP(hipGetLastError());
     if (precision == PH_PREC_FP64) {
+        B->rec.alloc(sizeof(double) * (size_t)c.rw * (size_t)n);
+        k_pack_records<double><<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c, B->rec.as<double>());
+        PH_HIP(hipGetLastError());
         if (small) launch_f64_t<K, STRAT, 2, 0, 0, 0>(B, c, pl, ga, S, ha, out, counters, s, ev);
         else launch_f64_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC>(B, c, pl, ga, S, ha, out, counters, s, ev);
         return;
     }
-    B->rec32.alloc(sizeof(float2) * (size_t)n);
-    k_pack_vih<<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, B->rec32.as<float2>());
+    int ncol = 0;
+    const CatChunk c32 = compact_c32(c, ncol);
+    if ((i64)c32.rw * 8 * n >= (i64(1) << 32))
+        throw Error{PH_ERR_CAPACITY, "FP32C records exceed 4 GiB; split the catalog or use PH_PREC_FP64"};
+    B->rec32.alloc(sizeof(double) * (size_t)c32.rw * (size_t)n);
+    k_pack_c32<<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c32, ncol, c32.rw, B->rec32.as<double>());
     PH_HIP(hipGetLastError());
-    if (small) launch_c32_t<K, STRAT, 2, 0, 0, 0, false>(B, c, pl, ga, S, ha, out, counters, s, ev);
+    if (small) launch_c32_t<K, STRAT, 2, 0, 0, 0, false>(B, c32, pl, ga, S, ha, out, counters, s, ev);
     else if (c.ns == 2 && c.ng == 1 && c.nd == 1 && c.nc == 1)  // density, scalar, gradient, divergence, curl
-        launch_c32_t<K, STRAT, 2, 1, 1, 1, true>(B, c, pl, ga, S, ha, out, counters, s, ev);
-    else launch_c32_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, false>(B, c, pl, ga, S, ha, out, counters, s, ev);
+        launch_c32_t<K, STRAT, 2, 1, 1, 1, true>(B, c32, pl, ga, S, ha, out, counters, s, ev);
+    else launch_c32_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, false>(B, c32, pl, ga, S, ha, out, counters, s, ev);
 }
 
 template <int K>

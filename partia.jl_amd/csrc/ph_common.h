@@ -119,11 +119,11 @@ struct ph_lbvh {
     ph::DevBuf rec;          // packed per-particle records of the last interpolation call
     ph::DevBuf ha;           // per-sample smoothing length of the nearest particle
     // Bucket tree (built on first use): the LBVH in DFS preorder with every
-    // subtree of at most 16 leaves collapsed into one bucket.
+    // subtree of at most 4 leaves collapsed into one bucket.
     bool buckets_ready = false;
     int64_t nrec = 0;
     ph::DevBuf brecs;     // bucket-tree records [nrec]
-    ph::DevBuf rec32;     // {m/rho, 1/h} in float32 per particle (FP32-compensated calls)
+    ph::DevBuf rec32;     // FP32-compensated records of the last call ({m/rho, 1/h}, columns)
 
     // Frees every device buffer (the context is going away; the handle stays
     // valid for ph_lbvh_destroy only).

@@ -55,9 +55,9 @@ def test_bucket_and_lane_grids_bitwise(backend, strategy):
             assert_same(d.grid(FULL, strategy, *axes), ref)
 
 
-@pytest.mark.parametrize("n", [1, 2, 15, 16, 17, 33, 257])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 15, 16, 17, 33, 257])
 def test_bucket_boundaries(backend, n):
-    """Particle counts around the 16-leaf bucket size and tiny trees (n = 1)."""
+    """Particle counts around the bucket size (4 leaves) and tiny trees (n = 1)."""
     ctx = backend.context
     c6, cols = cloud(n, 50 + n)
     c6 = (c6[0], c6[1], c6[2], c6[3], c6[4] * 0 + 0.3, c6[5])
