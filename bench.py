@@ -163,15 +163,23 @@ def main():
     cat = N.CatalogArgs(**CATALOG)
     qcols = (C.c_void_p * 5)(*[d.p.value for d in dcols[6:]])
 
+    wall = {"build": [], "interp": [], "destroy": []}
+
     def step(prec=prec):
         h = C.c_void_p()
+        t0 = time.perf_counter()
         ctx.check(L.ph_lbvh_build_device(ctx.handle, NPART, *[d.p for d in dcols[:6]], 5, qcols, KERNEL_C4, 63,
                                          N.PH_PREC_FP64, C.byref(h)))
+        t1 = time.perf_counter()
         build_ms = ctx.stats()["build_ms"]
         ctx.check(L.ph_structured_grid_device(ctx.handle, h, cat.ref(), N.PH_SYMMETRIC, prec, dax.p, NGRID, dax.p,
                                               NGRID, dax.p, NGRID, dout.p))
+        t2 = time.perf_counter()
         st = ctx.stats()
         L.ph_lbvh_destroy(h)
+        t3 = time.perf_counter()
+        for k, v in (("build", t1 - t0), ("interp", t2 - t1), ("destroy", t3 - t2)):
+            wall[k].append(1e3 * v)
         return build_ms, st["interp_ms"], st["kernel_ms"]
 
     # one counting pass for the algorithmic bytes (untimed)
@@ -195,8 +203,10 @@ def main():
         print("bench: step %d build %.2f ms interp %.2f ms (accumulation kernel %.2f ms)" % (i, b, it, k),
               file=sys.stderr, flush=True)
     ctx.check(L.ph_synchronize(ctx.handle))
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / args.steps
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    print("bench: host wall ms per call (median of all steps): " +
+          ", ".join("%s %.2f" % (k, float(np.median(v))) for k, v in wall.items()), file=sys.stderr, flush=True)
     kernel_ms = float(np.mean(kernels))
     interp_ms = float(np.mean(interps))
     build_ms = float(np.mean(builds))

@@ -18,8 +18,30 @@ int fail(ph_context* ctx, int code, const std::string& msg) {
     return code;
 }
 
+// Routes the call's device allocations through the context's block cache.
+struct CacheScope {
+    BlockCache* saved;
+    explicit CacheScope(ph_context* ctx) : saved(tl_cache) { tl_cache = ctx ? &ctx->cache : nullptr; }
+    ~CacheScope() { tl_cache = saved; }
+};
+
+// Blocks released by the call return to the cache once the stream has drained.
+void settle_cache(ph_context* ctx) {
+    if (!ctx || ctx->cache.pending.empty()) return;
+    if (ctx->stream && hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        return;  // left pending: freed with the context
+    }
+    ctx->cache.settle();
+}
+
 template <class F> int guarded(ph_context* ctx, F&& f) {
+    struct Settle {
+        ph_context* c;
+        ~Settle() { settle_cache(c); }
+    } settle{ctx};
     try {
+        CacheScope scope(ctx);
         if (ctx) {
             PH_HIP(hipSetDevice(ctx->device));
             ctx->last_error.clear();
@@ -187,6 +209,7 @@ int ph_ctx_destroy(ph_context* ctx) {
     ctx->lut64.release();
     ctx->lut32.release();
     ctx->counters.release();
+    ctx->cache.purge();
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -221,6 +244,11 @@ int ph_device_alloc(ph_context* ctx, int64_t bytes, void** dptr) {
         need(ctx && dptr && bytes >= 0, PH_ERR_ARGUMENT, "bad arguments");
         *dptr = nullptr;
         hipError_t e = hipMalloc(dptr, bytes > 0 ? (size_t)bytes : 16);
+        if (e != hipSuccess && ctx->cache.parked_bytes) {  // give parked blocks back and retry
+            (void)hipGetLastError();
+            ctx->cache.purge_parked();
+            e = hipMalloc(dptr, bytes > 0 ? (size_t)bytes : 16);
+        }
         if (e != hipSuccess) {
             (void)hipGetLastError();
             *dptr = nullptr;
@@ -322,6 +350,9 @@ int ph_lbvh_destroy(ph_lbvh* B) {
                 v.erase(v.begin() + (std::ptrdiff_t)t);
                 break;
             }
+        delete B;  // device blocks return to the context's cache; the stream is idle
+        ctx->cache.settle();
+        return PH_OK;
     }
     delete B;
     return PH_OK;
@@ -341,9 +372,10 @@ int ph_lbvh_set_columns(ph_lbvh* B, const double* m, const double* rho, int32_t 
             // keep x, y, z, (m), h, (rho); reallocate for the new column count
             DevBuf nc;
             nc.alloc(es * (size_t)(6 + ncol) * n);
-            PH_HIP(hipMemcpy(nc.p, B->cols.p, es * 6 * (size_t)n, hipMemcpyDeviceToDevice));
+                PH_HIP(hipMemcpy(nc.p, B->cols.p, es * 6 * (size_t)n, hipMemcpyDeviceToDevice));
             std::swap(nc.p, B->cols.p);
             std::swap(nc.bytes, B->cols.bytes);
+            std::swap(nc.cache, B->cols.cache);
             B->ncol = ncol;
         }
         auto put = [&](int c, const double* src) {
@@ -468,7 +500,7 @@ int ph_point_samples(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t
         dout.alloc(out_elem(precision) * (size_t)L * (size_t)(S > 0 ? S : 1));
         double* d = dp.as<double>();
         if (S > 0) {
-            PH_HIP(hipMemcpy(d, px, sizeof(double) * S, hipMemcpyHostToDevice));
+                PH_HIP(hipMemcpy(d, px, sizeof(double) * S, hipMemcpyHostToDevice));
             PH_HIP(hipMemcpy(d + S, py, sizeof(double) * S, hipMemcpyHostToDevice));
             PH_HIP(hipMemcpy(d + 2 * S, pz, sizeof(double) * S, hipMemcpyHostToDevice));
         }

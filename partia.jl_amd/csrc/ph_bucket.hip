@@ -796,10 +796,22 @@ template <int K, int STRAT>
 void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointList* pl, const GridAxes* ga, i64 S,
                       void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
     const i64 n = B->n;
+    const bool small = c.ng + c.nd + c.nc == 0 && c.ns <= 2;
+    // buffers first (the pool's allocations wait for the stream), then kernels
+    int ncol = 0;
+    CatChunk c32{};
+    if (precision == PH_PREC_FP64) {
+        B->rec.alloc(sizeof(double) * (size_t)c.rw * (size_t)n);
+    } else {
+        c32 = compact_c32(c, ncol);
+        if ((i64)c32.rw * 8 * n >= (i64(1) << 32))
+            throw Error{PH_ERR_CAPACITY, "FP32C records exceed 4 GiB; split the catalog or use PH_PREC_FP64"};
+        B->rec32.alloc(sizeof(double) * (size_t)c32.rw * (size_t)n);
+    }
+    if (STRAT != PH_SCATTER) B->ha.alloc(sizeof(double) * (size_t)S);
     // nearest-particle smoothing length of every sample (gather / symmetric)
     const double* ha = nullptr;
     if (STRAT != PH_SCATTER) {
-        B->ha.alloc(sizeof(double) * (size_t)S);
         const BucketView bv = bucket_view(B);
         const unsigned g = sample_blocks(ga, S);
         if (ga) k_nearest_bucket<1><<<g, kBlock, 0, s>>>(bv, Sampler<double, 1>{*ga, S}, B->ha.as<double>());
@@ -807,20 +819,13 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
         PH_HIP(hipGetLastError());
         ha = B->ha.as<double>();
     }
-    const bool small = c.ng + c.nd + c.nc == 0 && c.ns <= 2;
     if (precision == PH_PREC_FP64) {
-        B->rec.alloc(sizeof(double) * (size_t)c.rw * (size_t)n);
         k_pack_records<double><<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c, B->rec.as<double>());
         PH_HIP(hipGetLastError());
         if (small) launch_f64_t<K, STRAT, 2, 0, 0, 0>(B, c, pl, ga, S, ha, out, counters, s, ev);
         else launch_f64_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC>(B, c, pl, ga, S, ha, out, counters, s, ev);
         return;
     }
-    int ncol = 0;
-    const CatChunk c32 = compact_c32(c, ncol);
-    if ((i64)c32.rw * 8 * n >= (i64(1) << 32))
-        throw Error{PH_ERR_CAPACITY, "FP32C records exceed 4 GiB; split the catalog or use PH_PREC_FP64"};
-    B->rec32.alloc(sizeof(double) * (size_t)c32.rw * (size_t)n);
     k_pack_c32<<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c32, ncol, c32.rw, B->rec32.as<double>());
     PH_HIP(hipGetLastError());
     if (small) launch_c32_t<K, STRAT, 2, 0, 0, 0, false>(B, c32, pl, ga, S, ha, out, counters, s, ev);

@@ -317,27 +317,39 @@ void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipS
     const int g = grid_for(n);
     auto align = [](size_t b) { return (b + 255) & ~size_t(255); };
 
-    // column pointer table on the device
+    // every buffer first, then the kernels: allocations come from the stream-
+    // ordered pool and wait for the stream, so none sits between two launches
+    const i64 tot = 2 * n - 1;
     W.ptrs.alloc(sizeof(void*) * ncols);
+    W.mm.alloc(sizeof(T) * 8);
+    W.keys.alloc(align(sizeof(TI) * n));
+    W.dk.alloc(sizeof(u64) * n);
+    W.adj.alloc(sizeof(i32) * n);
+    W.visited.alloc(sizeof(u32) * (n - 1) + sizeof(u32) * 64 + sizeof(i32) * n);
+    W.tmp.alloc(align(16 * (size_t)n) + (size_t(1) << 20));  // radix sort, scans, min/max
+    B->order.alloc(sizeof(u32) * n);
+    B->cols.alloc(sizeof(T) * (size_t)ncols * n);
+    B->codes.alloc(sizeof(u64) * n);
+    for (DevBuf* b : {&B->left, &B->right, &B->escape, &B->parent}) b->alloc(sizeof(i32) * tot);
+    B->leaf_escape.alloc(sizeof(i32) * n);
+    B->leaves.alloc(sizeof(LeafRec<T>) * n);
+    B->nodes.alloc(sizeof(NodeRec<T>) * (size_t)(n > 1 ? n - 1 : 1));
+    B->range.alloc(sizeof(i32) * 2 * (size_t)(n > 1 ? n - 1 : 1));
+
+    // column pointer table on the device
     PH_HIP(hipMemcpyAsync(W.ptrs.p, src_cols, sizeof(void*) * ncols, hipMemcpyHostToDevice, s));
 
     // 1-2: min/max, encode
-    W.mm.alloc(sizeof(T) * 8);
     minmax3<T>(src_cols[0], src_cols[1], src_cols[2], n, W.mm.as<T>(), W.tmp, s);
-    W.keys.alloc(align(sizeof(TI) * n));
-    B->order.alloc(sizeof(u32) * n);
     k_encode<T, TI><<<g, kBlock, 0, s>>>(src_cols[0], src_cols[1], src_cols[2], n, W.mm.as<T>(), W.keys.as<TI>(),
                                          B->order.as<u32>());
     PH_HIP(hipGetLastError());
     // 3: stable sort by code
     radix_sort_pairs(W.keys.as<TI>(), B->order.as<u32>(), n, CodeTraits<TI>::sort_bits, W.tmp, s);
     // 4: permute the columns
-    B->cols.alloc(sizeof(T) * (size_t)ncols * n);
     k_permute<T, T><<<g, kBlock, 0, s>>>(W.ptrs.as<const T*>(), B->cols.as<T>(), ncols, n, B->order.as<u32>());
     PH_HIP(hipGetLastError());
     // 5: duplicate rewrite
-    W.dk.alloc(sizeof(u64) * n);
-    B->codes.alloc(sizeof(u64) * n);
     k_dedup_keys<TI><<<g, kBlock, 0, s>>>(W.keys.as<TI>(), W.dk.as<u64>(), n);
     PH_HIP(hipGetLastError());
     inclusive_max_u64(W.dk.as<u64>(), n, W.tmp, s);
@@ -345,15 +357,8 @@ void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipS
     PH_HIP(hipGetLastError());
 
     // 6-8: topology
-    const i64 tot = 2 * n - 1;
-    for (DevBuf* b : {&B->left, &B->right, &B->escape, &B->parent}) {
-        b->alloc(sizeof(i32) * tot);
+    for (DevBuf* b : {&B->left, &B->right, &B->escape, &B->parent})
         PH_HIP(hipMemsetAsync(b->p, 0, sizeof(i32) * tot, s));
-    }
-    B->leaf_escape.alloc(sizeof(i32) * n);
-    B->leaves.alloc(sizeof(LeafRec<T>) * n);
-    B->nodes.alloc(sizeof(NodeRec<T>) * (size_t)(n > 1 ? n - 1 : 1));
-    B->range.alloc(sizeof(i32) * 2 * (size_t)(n > 1 ? n - 1 : 1));
     B->root = n >= 2 ? 1 : 0;
     k_leaves<T><<<g, kBlock, 0, s>>>(B->cols.as<T>(), n, B->leaves.as<LeafRec<T>>());
     PH_HIP(hipGetLastError());
@@ -362,7 +367,6 @@ void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipS
         k_karras<TI><<<gi, kBlock, 0, s>>>(W.keys.as<TI>(), n, B->left.as<i32>(), B->right.as<i32>(),
                                            B->parent.as<i32>(), B->range.as<i32>());
         PH_HIP(hipGetLastError());
-        W.adj.alloc(sizeof(i32) * n);
         k_adjacent<TI><<<g, kBlock, 0, s>>>(W.keys.as<TI>(), n, W.adj.as<i32>());
         PH_HIP(hipGetLastError());
         k_internal_escape<<<gi, kBlock, 0, s>>>(B->range.as<i32>(), W.adj.as<i32>(), n, B->escape.as<i32>());
@@ -373,7 +377,6 @@ void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipS
     PH_HIP(hipGetLastError());
     // 9: refit (phase 1 per chunk in LDS, phase 2 across chunks)
     if (n >= 2) {
-        W.visited.alloc(sizeof(u32) * (n - 1) + sizeof(u32) * 64 + sizeof(i32) * n);
         u32* visited = W.visited.as<u32>();
         u32* bad = visited + (n - 1);
         u32* nstarts = bad + 1;

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -39,30 +40,102 @@ struct Error {
         }                                                                                              \
     } while (0)
 
+// Per-context cache of device blocks: a freed block is parked here and handed
+// out again to a request of at most its size (and at least half of it), so
+// steady-state calls (build, interpolate, destroy, repeated) make no driver
+// allocation.  A block released during a call may still be read by work queued
+// on the context's stream, and the host-side copies of a call are not ordered
+// with that stream, so released blocks wait in `pending` and become available
+// only after the stream has drained (settle).  The cache is emptied when the
+// context is destroyed, and its parked blocks before retrying a failed
+// allocation.
+struct BlockCache {
+    std::multimap<size_t, void*> parked;  // block size -> pointer, free to hand out
+    std::vector<std::pair<size_t, void*>> pending;  // released, stream not yet drained
+    size_t parked_bytes = 0;
+    BlockCache() = default;
+    BlockCache(const BlockCache&) = delete;
+    BlockCache& operator=(const BlockCache&) = delete;
+    ~BlockCache() { purge(); }
+    void* take(size_t b, size_t& got) {
+        auto it = parked.lower_bound(b);
+        if (it == parked.end() || it->first > 2 * b) return nullptr;
+        void* p = it->second;
+        got = it->first;
+        parked_bytes -= got;
+        parked.erase(it);
+        return p;
+    }
+    void put(void* p, size_t b) { pending.emplace_back(b, p); }
+    // The stream that may still use the pending blocks has drained.
+    void settle() {
+        for (auto& e : pending) {
+            parked.emplace(e.first, e.second);
+            parked_bytes += e.first;
+        }
+        pending.clear();
+    }
+    void purge_parked() {
+        for (auto& kv : parked) (void)hipFree(kv.second);
+        parked.clear();
+        parked_bytes = 0;
+    }
+    void purge() {  // hipFree waits for the device, so pending blocks are idle by then
+        purge_parked();
+        for (auto& e : pending) (void)hipFree(e.second);
+        pending.clear();
+    }
+};
+
+// Cache of the context whose entry point runs on this thread (set by the entry
+// points), or null: then buffers use hipMalloc / hipFree directly.
+inline thread_local BlockCache* tl_cache = nullptr;
+
 // Device buffer owned by the library.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    BlockCache* cache = nullptr;  // where the block returns on release, or null (hipFree)
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (cache) cache->put(p, bytes);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
+        cache = nullptr;
     }
     void alloc(size_t b) {
         if (b <= bytes && p) return;
         release();
         if (b == 0) b = 16;
+        b = (b + 255) & ~size_t(255);
+        if (tl_cache) {
+            size_t got = 0;
+            if (void* q = tl_cache->take(b, got)) {
+                p = q;
+                bytes = got;
+                cache = tl_cache;
+                return;
+            }
+        }
         hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess && tl_cache && tl_cache->parked_bytes) {  // give parked blocks back and retry
+            (void)hipGetLastError();
+            tl_cache->purge_parked();
+            e = hipMalloc(&p, b);
+        }
         if (e != hipSuccess) {
             (void)hipGetLastError();
             p = nullptr;
             throw Error{PH_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(b) + " bytes failed"};
         }
         bytes = b;
+        cache = tl_cache;
     }
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
@@ -87,6 +160,7 @@ template <class T> struct alignas(16) LeafRec {
 struct ph_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    ph::BlockCache cache;      // parked device blocks of this context's buffers (destroyed last)
     std::string last_error;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // 0-1 call, 2-3 build, 4-5 kernel
     ph_stats stats{};
