@@ -17,6 +17,15 @@ GPU tests); the same steps in float64 are timed beside it for the speedup.  The
 interpolation-only rate, the build time, the dominant kernel's roofline and a
 CPU baseline (the C++ oracle on this host's cores, on a bounded subset of the
 samples) are reported too.
+
+Other workloads (`--workload`), each printing one line of the same form:
+  column2048  8e6 particles in a log-normal density field (Gaussian random field
+              with P(k) ~ k^-11/3 on 128^3, sigma_ln = 1.5, seed 3), tabulated
+              line-integrated M4 kernel, Float32; column density (:rho,) on a
+              2048^2 image of parallel rays along z over [0,1]^2.  One step =
+              Float32 LinearBVH! + the projection; value = rays/s.
+  grid64      1e5 uniform particles (seed 1), M4, gather, Float64; density and
+              A (Shepard) on a 64^3 grid over [0,1]^3.
 """
 import argparse
 import ctypes as C
@@ -57,6 +66,7 @@ def parse():
     ap.add_argument("--traversal", choices=["auto", "lane", "bucket"], default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
+    ap.add_argument("--workload", choices=["grid256", "column2048", "grid64"], default="grid256")
     return ap.parse_args()
 
 
@@ -78,11 +88,11 @@ class Dev:
             self.p = C.c_void_p()
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` on this workload, from the committed
+def measured_traffic(kernel, workload="plummer2e6_c4_symmetric_grid256"):
+    """HBM bytes per launch of `kernel` on a workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-probe
     calibration); None when no summary exists."""
-    path = os.path.join(ROOT, "profiles", "%s_plummer2e6_c4_symmetric_grid256_hbm.md" % kernel)
+    path = os.path.join(ROOT, "profiles", "%s_%s_hbm.md" % (kernel, workload))
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
@@ -102,8 +112,7 @@ def cpu_baseline(data, axes, target_seconds):
     o = O.OracleSet(data["x"], data["y"], data["z"], data["m"], data["h"], data["rho"], cols,
                     kernel="C4_Wendland", code_bits=63)
     t_build = time.perf_counter() - t0
-    nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    nthreads = max(1, min(int(nthreads), 16))
+    nthreads = threads()
     S = NGRID ** 3
     ax = axes
     # strided subset of the flattened grid: spread over the whole volume
@@ -134,12 +143,217 @@ def cpu_baseline(data, axes, target_seconds):
     }
 
 
+def threads():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(int(n), 16))
+
+
+def timed_steps(ctx, step, steps, warmup, label):
+    """Untimed warm-up, then `steps` timed steps between two synchronisations;
+    returns (ms per step, list of per-step results)."""
+    L = N.lib()
+    for _ in range(warmup):
+        step()
+    ctx.check(L.ph_synchronize(ctx.handle))
+    res = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        r = step()
+        res.append(r)
+        print("bench: %s step %d %s" % (label, i, " ".join("%s %.2f ms" % kv for kv in r.items())),
+              file=sys.stderr, flush=True)
+    ctx.check(L.ph_synchronize(ctx.handle))
+    return 1e3 * (time.perf_counter() - t0) / steps, res
+
+
+def upload_columns(ctx, data, names):
+    out = []
+    for nm in names:
+        d = Dev(ctx, 8 * len(data[nm]))
+        d.upload(np.asarray(data[nm], dtype=np.float64))
+        out.append(d)
+    return out
+
+
+def column2048(args):
+    """8e6 log-normal particles, M4 line-integrated column density on 2048^2 rays, Float32."""
+    npart, nimg = 8_000_000, 2048
+    kernel_m4 = 0
+    print("bench: generating %d log-normal particles" % npart, file=sys.stderr, flush=True)
+    data = synthetic.lognormal_field(npart, seed=3)
+    ax = synthetic.linrange(0.0, 1.0, nimg)
+    S = nimg * nimg
+    ctx = N.Context(0)
+    if args.traversal == "lane":
+        ctx.set_traversal(N.PH_TRAVERSAL_LANE)  # one DFS per ray instead of one per 8 x 8 tile
+    L = N.lib()
+    dcols = upload_columns(ctx, data, ["x", "y", "z", "m", "h", "rho", "rho"])
+    dax = Dev(ctx, 8 * nimg)
+    dax.upload(ax)
+    dout = Dev(ctx, 4 * S)
+    cat = N.CatalogArgs(sslots=[1], sflags=[0])
+    qcols = (C.c_void_p * 1)(dcols[6].p.value)
+    frame = [np.ascontiguousarray(np.array(v, dtype=np.float64)) for v in
+             ([0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0])]
+
+    def step():
+        h = C.c_void_p()
+        ctx.check(L.ph_lbvh_build_device(ctx.handle, npart, *[d.p for d in dcols[:6]], 1, qcols, kernel_m4, 63,
+                                         N.PH_PREC_FP32, C.byref(h)))
+        b = ctx.stats()["build_ms"]
+        ctx.check(L.ph_line_grid_device(ctx.handle, h, cat.ref(), N.PH_SCATTER, N.PH_PREC_FP32,
+                                        *[N.ptr(f) for f in frame], dax.p, nimg, dax.p, nimg, dout.p))
+        st = ctx.stats()
+        L.ph_lbvh_destroy(h)
+        return {"build": b, "interp": st["interp_ms"], "kernel": st["kernel_ms"]}
+
+    ctx.set_count_neighbours(True)
+    step()
+    n_acc = ctx.stats()["accepted"]
+    ctx.set_count_neighbours(False)
+    print("bench: accepted neighbours per image = %d (%.1f per ray)" % (n_acc, n_acc / S), file=sys.stderr,
+          flush=True)
+    ms, res = timed_steps(ctx, step, args.steps, args.warmup, "column2048")
+    kernel_ms = float(np.mean([r["kernel"] for r in res]))
+    # per accepted neighbour x, y, z, h, m, rho and the quantity column (7 float32);
+    # per ray 3 origin + 3 direction floats in, one output out
+    alg_bytes = n_acc * 7 * 4 + S * 7 * 4
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic("k_lines_wave", "lognormal8e6_m4_column2048")
+    result = {
+        "metric": "GPU rays/s, line-integrated column density (Float32); LBVH build ms",
+        "value": S / (ms * 1e-3), "unit": "rays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded log-normal Gaussian random field, SplitMix64)",
+        "config": {"workload": "lognormal_8e6_M4_column_density_2048", "particles": npart, "image": [nimg, nimg],
+                   "kernel": "M4_spline", "strategy": "scatter", "outputs": 1, "code_bits": 63,
+                   "precision": "fp32"},
+        "lbvh_build_ms": float(np.mean([r["build"] for r in res])),
+        "interp_ms": float(np.mean([r["interp"] for r in res])),
+        "accumulation_kernel_ms": kernel_ms,
+        "accepted_neighbours": int(n_acc),
+        "traversal": "per-ray" if args.traversal == "lane" else "wave per 8x8 tile",
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes": int(alg_bytes)},
+    }
+    for d in dcols + [dax, dout]:
+        d.free()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        print("bench: CPU baseline (oracle)", file=sys.stderr, flush=True)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        t0 = time.perf_counter()
+        o = O.OracleSet(data["x"], data["y"], data["z"], data["m"], data["h"], data["rho"], [data["rho"]],
+                        kernel="M4_spline", code_bits=63, precision=32)
+        t_build = time.perf_counter() - t0
+        nt = threads()
+        stride = 1 << 14
+        while True:
+            idx = np.arange(0, S, stride)
+            i, j = idx % nimg, idx // nimg
+            z = np.zeros(len(idx))
+            t0 = time.perf_counter()
+            o.line_samples({"sslots": [1], "sflags": [0]}, O.SCATTER,
+                           [ax[i].astype(np.float32).astype(np.float64), ax[j].astype(np.float32).astype(np.float64),
+                            z], [z, z, z + 1.0], nthreads=nt)
+            dt = time.perf_counter() - t0
+            if dt * 4 > args.cpu_seconds or stride <= 16:
+                break
+            stride //= 4
+        rate = len(idx) / dt
+        result["cpu_baseline"] = {
+            "value": S / (t_build + S / rate), "unit": "rays/s", "cores": nt, "kind": "port",
+            "sample": ("C++ oracle (reference algorithm restated, Float32) on %d host threads: LinearBVH! on all %d "
+                       "particles (%.2f s) + %d of the %d rays (every %d-th, %.2f s); whole-image time extrapolated"
+                       % (nt, npart, t_build, len(idx), S, stride, dt)),
+            "build_s": t_build, "interp_rays_per_s": rate}
+    print(json.dumps(result))
+
+
+def grid64(args):
+    """1e5 uniform particles, M4 gather, density + A on 64^3, Float64."""
+    npart, ng = 100_000, 64
+    data = synthetic.uniform_box(npart, seed=1)
+    ax = synthetic.linrange(0.0, 1.0, ng)
+    S = ng ** 3
+    ctx = N.Context(0)
+    L = N.lib()
+    dcols = upload_columns(ctx, data, ["x", "y", "z", "m", "h", "rho", "rho", "A"])
+    dax = Dev(ctx, 8 * ng)
+    dax.upload(ax)
+    dout = Dev(ctx, 8 * 2 * S)
+    cat = N.CatalogArgs(sslots=[1, 2], sflags=[0, 1])
+    qcols = (C.c_void_p * 2)(dcols[6].p.value, dcols[7].p.value)
+
+    def step():
+        h = C.c_void_p()
+        ctx.check(L.ph_lbvh_build_device(ctx.handle, npart, *[d.p for d in dcols[:6]], 2, qcols, 0, 63,
+                                         N.PH_PREC_FP64, C.byref(h)))
+        b = ctx.stats()["build_ms"]
+        ctx.check(L.ph_structured_grid_device(ctx.handle, h, cat.ref(), N.PH_GATHER, N.PH_PREC_FP64, dax.p, ng,
+                                              dax.p, ng, dax.p, ng, dout.p))
+        st = ctx.stats()
+        L.ph_lbvh_destroy(h)
+        return {"build": b, "interp": st["interp_ms"], "kernel": st["kernel_ms"]}
+
+    ctx.set_count_neighbours(True)
+    step()
+    n_acc = ctx.stats()["accepted"]
+    ctx.set_count_neighbours(False)
+    ms, res = timed_steps(ctx, step, args.steps, args.warmup, "grid64")
+    kernel_ms = float(np.mean([r["kernel"] for r in res]))
+    alg_bytes = n_acc * 8 * 8 + S * 5 * 8  # x,y,z,h,m,rho,rho,A per neighbour; 3 in + 2 out per sample
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    result = {
+        "metric": "GPU samples/s (Float64); LBVH build ms",
+        "value": S / (ms * 1e-3), "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded uniform box, SplitMix64)",
+        "config": {"workload": "uniform_1e5_M4_gather_grid64", "particles": npart, "grid": [ng] * 3,
+                   "kernel": "M4_spline", "strategy": "gather", "outputs": 2, "code_bits": 63,
+                   "precision": "fp64"},
+        "lbvh_build_ms": float(np.mean([r["build"] for r in res])),
+        "interp_ms": float(np.mean([r["interp"] for r in res])),
+        "accumulation_kernel_ms": kernel_ms,
+        "accepted_neighbours": int(n_acc),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": int(alg_bytes)},
+    }
+    for d in dcols + [dax, dout]:
+        d.free()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        t0 = time.perf_counter()
+        o = O.OracleSet(data["x"], data["y"], data["z"], data["m"], data["h"], data["rho"], [data["rho"], data["A"]],
+                        kernel="M4_spline", code_bits=63)
+        t_build = time.perf_counter() - t0
+        nt = threads()
+        g = np.meshgrid(ax, ax, ax, indexing="ij")
+        P = [a.ravel(order="F") for a in g]
+        t0 = time.perf_counter()
+        o.point_samples({"sslots": [1, 2], "sflags": [0, 1]}, O.GATHER, *P, nthreads=nt)
+        dt = time.perf_counter() - t0
+        result["cpu_baseline"] = {
+            "value": S / (t_build + dt), "unit": "samples/s", "cores": nt, "kind": "port",
+            "sample": "C++ oracle on %d host threads: LinearBVH! (%.2f s) + the whole 64^3 grid (%.2f s)"
+                      % (nt, t_build, dt), "build_s": t_build}
+    print(json.dumps(result))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (got --gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "column2048":
+        return column2048(args)
+    if args.workload == "grid64":
+        return grid64(args)
     prec = N.PH_PREC_FP64 if args.precision == "fp64" else N.PH_PREC_FP32C
     print("bench: generating %d Plummer particles" % NPART, file=sys.stderr, flush=True)
     data = synthetic.plummer_sphere(NPART, seed=2)

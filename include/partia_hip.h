@@ -117,10 +117,17 @@ int ph_set_count_neighbours(ph_context* ctx, int enable);
  *                        outward, float64 leaf tests);
  *   PH_TRAVERSAL_AUTO    bucket.
  * PH_PREC_FP32C always uses the bucket traversal; Float32 trees the lane one.
- * The bucket traversal supports up to 2^26 particles (PH_ERR_CAPACITY beyond). */
+ * The bucket traversal supports up to 2^26 particles (PH_ERR_CAPACITY beyond).
+ * Line calls (LinearBVHTraversalMacros.jl:539-644) give identical results in
+ * every mode: PH_TRAVERSAL_LANE walks one DFS per ray; otherwise the rays of an
+ * image plane (ph_line_grid_device) are walked 64 at a time by one wave over
+ * the union of their DFS, each ray testing exactly its own nodes, and explicit
+ * ray lists use the per-ray walk unless PH_TRAVERSAL_WAVE asks for the wave
+ * walk for them too (worth it when consecutive rays are close together). */
 #define PH_TRAVERSAL_AUTO 0
 #define PH_TRAVERSAL_LANE 1
 #define PH_TRAVERSAL_BUCKET 2
+#define PH_TRAVERSAL_WAVE 3
 int ph_set_traversal(ph_context* ctx, int mode);
 
 /* ---- device memory helpers (for callers without their own HIP allocator) ---- */

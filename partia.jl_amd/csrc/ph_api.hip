@@ -233,7 +233,7 @@ int ph_set_count_neighbours(ph_context* ctx, int enable) {
 
 int ph_set_traversal(ph_context* ctx, int mode) {
     if (!ctx) return PH_ERR_ARGUMENT;
-    if (mode < PH_TRAVERSAL_AUTO || mode > PH_TRAVERSAL_BUCKET)
+    if (mode < PH_TRAVERSAL_AUTO || mode > PH_TRAVERSAL_WAVE)
         return fail(ctx, PH_ERR_ARGUMENT, "unknown traversal mode");
     ctx->traversal = mode;
     return PH_OK;
@@ -574,7 +574,8 @@ int ph_line_samples_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, i
         LineList ll{ox, oy, oz, dx, dy, dz};
         const void* lut = precision == PH_PREC_FP64 ? ctx->lut64.p : ctx->lut32.p;
         run_chunks(ctx, cat, S, [&](const CatChunk& c, int L, u64* counter, hipEvent_t* ev) {
-            launch_lines(B, c, strategy, precision, &ll, nullptr, S, L, out, counter, lut, ctx->stream, ev);
+            launch_lines(B, c, strategy, precision, &ll, nullptr, S, L, out, counter, lut,
+                         ctx->traversal == PH_TRAVERSAL_WAVE, ctx->stream, ev);
         });
         return PH_OK;
     });
@@ -628,7 +629,8 @@ int ph_line_grid_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int3
         lg.ny = ny;
         const void* lut = precision == PH_PREC_FP64 ? ctx->lut64.p : ctx->lut32.p;
         run_chunks(ctx, cat, S, [&](const CatChunk& c, int L, u64* counter, hipEvent_t* ev) {
-            launch_lines(B, c, strategy, precision, nullptr, &lg, S, L, out, counter, lut, ctx->stream, ev);
+            launch_lines(B, c, strategy, precision, nullptr, &lg, S, L, out, counter, lut,
+                         ctx->traversal != PH_TRAVERSAL_LANE, ctx->stream, ev);
         });
         return PH_OK;
     });

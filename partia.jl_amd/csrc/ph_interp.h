@@ -57,7 +57,8 @@ void launch_points(ph_lbvh* B, const CatChunk& c, int strategy, int precision, c
                    const GridAxes* ga, i64 S, int L, void* out, u64* counters, hipStream_t s,
                    hipEvent_t* ev = nullptr);
 void launch_lines(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const LineList* ll, const LineGrid* lg,
-                  i64 S, int L, void* out, u64* counters, const void* lut, hipStream_t s, hipEvent_t* ev = nullptr);
+                  i64 S, int L, void* out, u64* counters, const void* lut, bool wave, hipStream_t s,
+                  hipEvent_t* ev = nullptr);
 void launch_nearest_h(ph_lbvh* B, const PointList& pl, i64 S, double* ha, hipStream_t s);
 // Bucket traversal (float64 or FP32-compensated evaluation): per lane, a DFS over
 // the preorder bucket tree, leaves of reached buckets tested with float64.

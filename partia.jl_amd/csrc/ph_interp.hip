@@ -152,7 +152,42 @@ template <class T, int K, int STRAT> struct LineSampler<T, K, STRAT, 1> {
     }
 };
 
-// _line_integrated_quantities_interpolate_kernel, scatter (:123-166).
+// Per-ray sums of _line_integrated_quantities_interpolate_kernel, scatter
+// (line_integrated_scalar_interpolation.jl:123-166): S1 = sum m_b W / rho_b and
+// the catalog scalars sum A_b m_b W / rho_b, W = I(dr / h_b) / h_b from the table.
+template <class T, int K> struct LineAcc {
+    T S1 = T(0);
+    T acc[kMaxS];
+    __device__ inline LineAcc() {
+#pragma unroll
+        for (int j = 0; j < kMaxS; ++j) acc[j] = T(0);
+    }
+    __device__ inline void add(const T* __restrict__ cols, const CatChunk& c, i64 n, i64 k, T d2, T h,
+                               const T* __restrict__ inorm, T dq) {
+        const T dr = sqrt(d2 < T(0) ? T(0) : d2);
+        const T mb = cols[3 * n + k], rb = cols[5 * n + k];
+        const T W = line_W<T, K>(dr, h, inorm, dq);
+        S1 += mb * W / rb;
+#pragma unroll
+        for (int j = 0; j < kMaxS; ++j)
+            if (j < c.ns) acc[j] += cols[c.rsrc[c.spos[j]] * n + k] * (mb * W / rb);
+    }
+    // NaN outputs when no particle contributes (:154-156), Shepard scalars * 1/S1.
+    __device__ inline void store(const CatChunk& c, T* __restrict__ out, i64 S, i64 s) const {
+        if (S1 == T(0)) {
+#pragma unroll
+            for (int j = 0; j < kMaxS; ++j)
+                if (j < c.ns) out[c.sout[j] * S + s] = sph_nan<T>();
+        } else {
+            const T invS1 = T(1) / S1;
+#pragma unroll
+            for (int j = 0; j < kMaxS; ++j)
+                if (j < c.ns) out[c.sout[j] * S + s] = c.sflag[j] ? acc[j] * invS1 : acc[j];
+        }
+    }
+};
+
+// One lane per ray, the reference's escape-rope DFS with the line geometry.
 template <class T, int K, int STRAT, int SRC>
 __global__ void __launch_bounds__(kBlock) k_lines(TreeView<T> tv, const T* __restrict__ cols, CatChunk c,
                                                   LineSampler<T, K, STRAT, SRC> smp, const T* __restrict__ inorm, T dq,
@@ -163,36 +198,86 @@ __global__ void __launch_bounds__(kBlock) k_lines(TreeView<T> tv, const T* __res
     u64 accepted = 0;
     if (valid) {
         const T Kvalid = T(support_of<K>());
-        T S1 = T(0);
-        T acc[kMaxS];
-#pragma unroll
-        for (int j = 0; j < kMaxS; ++j) acc[j] = T(0);
+        LineAcc<T, K> a;
         const i64 n = tv.n;
         dfs<T, STRAT>(
             tv, Kvalid, T(0), [&](const LeafRec<T>& L) { return d2_line_leaf(l, L); },
             [&](const NodeRec<T>& N) { return d2_line_aabb_lb(l, N); },
             [&](i64 k, T d2, const LeafRec<T>& L) {
                 ++accepted;
-                const T dr = sqrt(d2 < T(0) ? T(0) : d2);
-                const T mb = cols[3 * n + k], rb = cols[5 * n + k];
-                const T W = line_W<T, K>(dr, L.h, inorm, dq);
-                S1 += mb * W / rb;
-#pragma unroll
-                for (int j = 0; j < kMaxS; ++j)
-                    if (j < c.ns) acc[j] += cols[c.rsrc[c.spos[j]] * n + k] * (mb * W / rb);
+                a.add(cols, c, n, k, d2, L.h, inorm, dq);
             });
-        if (S1 == T(0)) {
-#pragma unroll
-            for (int j = 0; j < kMaxS; ++j)
-                if (j < c.ns) out[c.sout[j] * smp.S + s] = sph_nan<T>();
-        } else {
-            const T invS1 = T(1) / S1;
-#pragma unroll
-            for (int j = 0; j < kMaxS; ++j)
-                if (j < c.ns) out[c.sout[j] * smp.S + s] = c.sflag[j] ? acc[j] * invS1 : acc[j];
-        }
+        a.store(c, out, smp.S, s);
     }
     count_add(counter, accepted);
+}
+
+// One wave per 64 rays (an 8 x 8 tile of an image plane): the wave walks the
+// union of its lanes' traversals once, with every node and leaf record fetched
+// once per wave at a wave-uniform address instead of once per ray.  Each lane
+// replays its own traversal inside the union: a lane that prunes node u goes
+// idle until the walk reaches escape(u), which the union walk always reaches
+// right after u's subtree, so every lane tests exactly the nodes and leaves of
+// its own DFS, in the same order, with the same arithmetic as k_lines.  The
+// results are therefore identical to k_lines, bit for bit.
+template <class T, int K, int SRC>
+__global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const T* __restrict__ cols, CatChunk c,
+                                                       LineSampler<T, K, PH_SCATTER, SRC> smp,
+                                                       const T* __restrict__ inorm, T dq, T* __restrict__ out,
+                                                       u64* counter) {
+    i64 s;
+    Line<T> l;
+    const bool valid = smp.get(s, l);
+    if (__ballot(valid) == 0) return;  // wave-uniform
+    const T Kvalid = T(support_of<K>());
+    const i64 n = tv.n;
+    LineAcc<T, K> a;
+    u64 accepted = 0;
+    if (tv.root == 0) {  // n == 1: the reference's brute-force branch
+        for (i64 k = 0; k < n; ++k) {
+            const LeafRec<T> L = tv.leaves[k];
+            const T d2 = d2_line_leaf(l, L);
+            if (valid && d2 <= radius2<T, PH_SCATTER>(Kvalid, T(0), L.h)) {
+                ++accepted;
+                a.add(cols, c, n, k, d2, L.h, inorm, dq);
+            }
+        }
+    } else {
+        bool active = valid;
+        i32 resume = -1;  // node at which an idle lane rejoins (0: never)
+        i32 node = tv.root;
+        for (i64 steps = 2 * n + 8; node != 0 && steps > 0; --steps) {
+            node = __builtin_amdgcn_readfirstlane(node);
+            if (!active && node == resume) active = true;
+            if (node >= (i32)n) {
+                const i64 k = (i64)node - n;
+                const LeafRec<T> L = tv.leaves[k];
+                if (active) {
+                    const T d2 = d2_line_leaf(l, L);
+                    if (d2 <= radius2<T, PH_SCATTER>(Kvalid, T(0), L.h)) {
+                        ++accepted;
+                        a.add(cols, c, n, k, d2, L.h, inorm, dq);
+                    }
+                }
+                node = tv.leaf_escape[k];
+            } else {
+                const NodeRec<T> N = tv.nodes[node - 1];
+                bool descend = false;
+                if (active) {
+                    const T d2n = d2_line_aabb_lb(l, N);
+                    if (d2n > radius2<T, PH_SCATTER>(Kvalid, T(0), N.scale)) {
+                        active = false;
+                        resume = N.escape;
+                    } else {
+                        descend = true;
+                    }
+                }
+                node = __ballot(descend) ? N.left : N.escape;
+            }
+        }
+    }
+    if (valid) a.store(c, out, smp.S, s);
+    count_add(counter, valid ? accepted : 0);
 }
 
 template <class T>
@@ -280,7 +365,7 @@ void launch_points_p(ph_lbvh* B, const CatChunk& c, int strategy, const PointLis
 
 template <class T, int K>
 void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const LineGrid* lg, i64 S, void* out,
-                    u64* counters, const T* inorm, hipStream_t s, hipEvent_t* ev) {
+                    u64* counters, const T* inorm, bool wave, hipStream_t s, hipEvent_t* ev) {
     const TreeView<T> tv = view_of<T>(B);
     const T* cols = B->cols.as<T>();
     // Q_2 - Q_1 of the table in T (line_integrated_kernel.jl:14)
@@ -290,12 +375,20 @@ void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const Lin
     if (lg) {
         LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S};
         const i64 tiles = ((lg->nx + 7) / 8) * ((lg->ny + 7) / 8);
-        k_lines<T, K, PH_SCATTER, 1><<<(unsigned)((tiles + 3) / 4), kBlock, 0, s>>>(tv, cols, c, smp, tab, dq,
-                                                                                    static_cast<T*>(out), counters);
+        const unsigned g = (unsigned)((tiles + 3) / 4);
+        if (wave)
+            k_lines_wave<T, K, 1><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out), counters);
+        else
+            k_lines<T, K, PH_SCATTER, 1><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out),
+                                                              counters);
     } else {
         LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S};
-        k_lines<T, K, PH_SCATTER, 0><<<(unsigned)((S + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-            tv, cols, c, smp, tab, dq, static_cast<T*>(out), counters);
+        const unsigned g = (unsigned)((S + kBlock - 1) / kBlock);
+        if (wave)
+            k_lines_wave<T, K, 0><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out), counters);
+        else
+            k_lines<T, K, PH_SCATTER, 0><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out),
+                                                              counters);
     }
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[1], s));
@@ -303,14 +396,14 @@ void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const Lin
 
 template <class T>
 void launch_lines_p(ph_lbvh* B, const CatChunk& c, const LineList* ll, const LineGrid* lg, i64 S, void* out,
-                    u64* counters, const T* inorm, hipStream_t s, hipEvent_t* ev) {
+                    u64* counters, const T* inorm, bool wave, hipStream_t s, hipEvent_t* ev) {
     switch (B->kernel) {
-    case PH_KERNEL_M4: launch_lines_k<T, PH_KERNEL_M4>(B, c, ll, lg, S, out, counters, inorm, s, ev); break;
-    case PH_KERNEL_M5: launch_lines_k<T, PH_KERNEL_M5>(B, c, ll, lg, S, out, counters, inorm, s, ev); break;
-    case PH_KERNEL_M6: launch_lines_k<T, PH_KERNEL_M6>(B, c, ll, lg, S, out, counters, inorm, s, ev); break;
-    case PH_KERNEL_C2: launch_lines_k<T, PH_KERNEL_C2>(B, c, ll, lg, S, out, counters, inorm, s, ev); break;
-    case PH_KERNEL_C4: launch_lines_k<T, PH_KERNEL_C4>(B, c, ll, lg, S, out, counters, inorm, s, ev); break;
-    default: launch_lines_k<T, PH_KERNEL_C6>(B, c, ll, lg, S, out, counters, inorm, s, ev); break;
+    case PH_KERNEL_M4: launch_lines_k<T, PH_KERNEL_M4>(B, c, ll, lg, S, out, counters, inorm, wave, s, ev); break;
+    case PH_KERNEL_M5: launch_lines_k<T, PH_KERNEL_M5>(B, c, ll, lg, S, out, counters, inorm, wave, s, ev); break;
+    case PH_KERNEL_M6: launch_lines_k<T, PH_KERNEL_M6>(B, c, ll, lg, S, out, counters, inorm, wave, s, ev); break;
+    case PH_KERNEL_C2: launch_lines_k<T, PH_KERNEL_C2>(B, c, ll, lg, S, out, counters, inorm, wave, s, ev); break;
+    case PH_KERNEL_C4: launch_lines_k<T, PH_KERNEL_C4>(B, c, ll, lg, S, out, counters, inorm, wave, s, ev); break;
+    default: launch_lines_k<T, PH_KERNEL_C6>(B, c, ll, lg, S, out, counters, inorm, wave, s, ev); break;
     }
 }
 
@@ -325,14 +418,14 @@ void launch_points(ph_lbvh* B, const CatChunk& c, int strategy, int precision, c
 }
 
 void launch_lines(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const LineList* ll, const LineGrid* lg,
-                  i64 S, int L, void* out, u64* counters, const void* lut, hipStream_t s, hipEvent_t* ev) {
+                  i64 S, int L, void* out, u64* counters, const void* lut, bool wave, hipStream_t s, hipEvent_t* ev) {
     (void)L;
     (void)strategy;
     if (S <= 0) return;
     if (precision == PH_PREC_FP64)
-        launch_lines_p<double>(B, c, ll, lg, S, out, counters, static_cast<const double*>(lut), s, ev);
+        launch_lines_p<double>(B, c, ll, lg, S, out, counters, static_cast<const double*>(lut), wave, s, ev);
     else
-        launch_lines_p<float>(B, c, ll, lg, S, out, counters, static_cast<const float*>(lut), s, ev);
+        launch_lines_p<float>(B, c, ll, lg, S, out, counters, static_cast<const float*>(lut), wave, s, ev);
 }
 
 void launch_nearest_h(ph_lbvh* B, const PointList& pl, i64 S, double* ha, hipStream_t s) {

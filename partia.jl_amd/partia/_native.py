@@ -20,7 +20,7 @@ LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libpartia_hip.so"))
 PH_OK = 0
 PH_GATHER, PH_SCATTER, PH_SYMMETRIC = 0, 1, 2
 PH_PREC_FP64, PH_PREC_FP32, PH_PREC_FP32C = 64, 32, 33
-PH_TRAVERSAL_AUTO, PH_TRAVERSAL_LANE, PH_TRAVERSAL_BUCKET = 0, 1, 2
+PH_TRAVERSAL_AUTO, PH_TRAVERSAL_LANE, PH_TRAVERSAL_BUCKET, PH_TRAVERSAL_WAVE = 0, 1, 2, 3
 GET = dict(order=0, codes=1, left=2, right=3, escape=4, parent=5, leaf_coor=6, leaf_scale=7, node_min=8,
            node_max=9, node_scale=10, columns=11)
 
