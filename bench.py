@@ -26,6 +26,13 @@ Other workloads (`--workload`), each printing one line of the same form:
               Float32 LinearBVH! + the projection; value = rays/s.
   grid64      1e5 uniform particles (seed 1), M4, gather, Float64; density and
               A (Shepard) on a 64^3 grid over [0,1]^3.
+  batch512    4e6 particles (half uniform, half a Plummer clump, seed 4) and 512
+              sampling requests of uneven size (point lists of 1e3-1e6 random
+              points, oriented planes 256^2-1024^2, oriented boxes 32^3-128^3)
+              served by one graph launch of the persistent work-queue kernel
+              (C4, symmetric, FP32-compensated, the 9 outputs above); value =
+              samples/s of the whole batch, with per-request p50/p99 latency and
+              the same requests issued as 512 separate calls beside it.
 """
 import argparse
 import ctypes as C
@@ -66,7 +73,7 @@ def parse():
     ap.add_argument("--traversal", choices=["auto", "lane", "bucket"], default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
-    ap.add_argument("--workload", choices=["grid256", "column2048", "grid64"], default="grid256")
+    ap.add_argument("--workload", choices=["grid256", "column2048", "grid64", "batch512"], default="grid256")
     return ap.parse_args()
 
 
@@ -344,6 +351,107 @@ def grid64(args):
     print(json.dumps(result))
 
 
+def batch512(args):
+    """512 uneven requests over a 4e6-particle uniform-plus-clump set in one graph launch."""
+    from partia import batch as PB
+    from partia import grids
+    npart, nreq = 4_000_000, 512
+    print("bench: generating %d particles and %d requests" % (npart, nreq), file=sys.stderr, flush=True)
+    data = synthetic.uniform_plus_clump(npart, seed=4)
+    mix = synthetic.request_mix(nreq, seed=4)
+    ctx = N.Context(0)
+    L = N.lib()
+    names = ["x", "y", "z", "m", "h", "rho", "rho", "A", "vx", "vy", "vz"]
+    dcols = upload_columns(ctx, data, names)
+    qcols = (C.c_void_p * 5)(*[d.p.value for d in dcols[6:]])
+    h = C.c_void_p()
+    ctx.check(L.ph_lbvh_build_device(ctx.handle, npart, *[d.p for d in dcols[:6]], 5, qcols, KERNEL_C4, 63,
+                                     N.PH_PREC_FP64, C.byref(h)))
+    build_ms = ctx.stats()["build_ms"]
+    cat = N.CatalogArgs(**CATALOG)
+    reqs, sizes, dev, host_pts = [], [], [], []
+    for q in mix:
+        if q["kind"] == "points":
+            st = synthetic.Stream(q["seed"])
+            P = [st.uniform(q["count"]) for _ in range(3)]
+        elif q["kind"] == "plane":
+            P = grids.cartesian_plane_coordinates(q["position"], q["right"], q["up"], q["axis"], q["axis"])
+        else:
+            P = grids.cartesian_box_coordinates(q["position"], q["right"], q["up"], q["forward"], q["axis"],
+                                                q["axis"], q["axis"])
+        S = len(P[0])
+        sizes.append(S)
+        host_pts.append(P)
+        out = Dev(ctx, 4 * L_OUT * max(S, 1))
+        dev.append(out)
+        if q["kind"] == "points":
+            dp = [Dev(ctx, 8 * S) for _ in range(3)]
+            for dd, a in zip(dp, P):
+                dd.upload(a)
+            dev += dp
+            reqs.append(PB.points(dp[0].p, dp[1].p, dp[2].p, S, out.p))
+        elif q["kind"] == "plane":
+            reqs.append(PB.plane(q["position"], q["right"], q["up"], q["axis"], q["axis"], out.p))
+        else:
+            reqs.append(PB.box(q["position"], q["right"], q["up"], q["forward"], q["axis"], q["axis"], q["axis"],
+                               out.p))
+    S_tot = int(sum(sizes))
+    print("bench: %d samples in %d requests" % (S_tot, nreq), file=sys.stderr, flush=True)
+    b = PB.Batch(ctx, h, cat, N.PH_SYMMETRIC, N.PH_PREC_FP32C, reqs)
+
+    def step():
+        b.run()
+        return {"batch": ctx.stats()["interp_ms"]}
+
+    ms, res = timed_steps(ctx, step, args.steps, args.warmup, "batch512")
+    t_first, t_last = b.times()
+    lat = t_last / 1e3  # completion after the launch began, ms
+    span = (t_last - t_first) / 1e3
+    # the same requests as separate calls (plane / box coordinates uploaded beforehand)
+    sep_ms = None
+    if not args.no_fp64_compare:
+        pre = []
+        for q, P, S, r in zip(mix, host_pts, sizes, reqs):
+            if q["kind"] == "points":
+                pre.append((r.px, r.py, r.pz))
+            else:
+                dp = [Dev(ctx, 8 * S) for _ in range(3)]
+                for dd, a in zip(dp, P):
+                    dd.upload(a)
+                dev += dp
+                pre.append(tuple(dd.p.value for dd in dp))
+        ctx.check(L.ph_synchronize(ctx.handle))
+        t0 = time.perf_counter()
+        for (px, py, pz), S, r in zip(pre, sizes, reqs):
+            ctx.check(L.ph_point_samples_device(ctx.handle, h, cat.ref(), N.PH_SYMMETRIC, N.PH_PREC_FP32C, S,
+                                                C.c_void_p(px), C.c_void_p(py), C.c_void_p(pz), C.c_void_p(r.out)))
+        ctx.check(L.ph_synchronize(ctx.handle))
+        sep_ms = 1e3 * (time.perf_counter() - t0)
+        print("bench: the same requests as %d separate calls: %.1f ms" % (nreq, sep_ms), file=sys.stderr, flush=True)
+    result = {
+        "metric": "GPU samples/s over 512 uneven requests in one graph launch; per-request p50/p99 latency",
+        "value": S_tot / (ms * 1e-3), "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32c",
+        "data": "synthetic (seeded uniform + Plummer clump, seeded request mix, SplitMix64)",
+        "config": {"workload": "uniform_plus_clump_4e6_512_requests", "particles": npart, "requests": nreq,
+                   "samples": S_tot, "kernel": "C4_Wendland", "strategy": "symmetric", "outputs": L_OUT,
+                   "precision": "fp32c"},
+        "lbvh_build_ms": build_ms,
+        "batch_ms": float(np.mean([r["batch"] for r in res])),
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "max": float(np.max(lat))},
+        "service_span_ms": {"p50": float(np.percentile(span, 50)), "p99": float(np.percentile(span, 99))},
+        "separate_calls_ms": sep_ms,
+        "speedup_vs_separate_calls": (sep_ms / ms) if sep_ms else None,
+    }
+    b.close()
+    L.ph_lbvh_destroy(h)
+    for d in dcols + dev:
+        d.free()
+    ctx.close()
+    print(json.dumps(result))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -354,6 +462,8 @@ def main():
         return column2048(args)
     if args.workload == "grid64":
         return grid64(args)
+    if args.workload == "batch512":
+        return batch512(args)
     prec = N.PH_PREC_FP64 if args.precision == "fp64" else N.PH_PREC_FP32C
     print("bench: generating %d Plummer particles" % NPART, file=sys.stderr, flush=True)
     data = synthetic.plummer_sphere(NPART, seed=2)

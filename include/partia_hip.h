@@ -226,6 +226,52 @@ int ph_nearest_h(ph_context* ctx, ph_lbvh* bvh, int64_t S, const double* px_host
  * 4096-entry float64 Inorm table of a kernel. */
 int ph_line_table(int32_t kernel, double* inorm_host /* 4096 */);
 
+/* ---- batched sampling: many PointSamples requests of uneven size over one tree
+ * in one launch.  Each request is what one PointSamples_interpolation call
+ * (KI/grid_interpolation/drivers/PointSamples_driver.jl:98-118, external-LBVH
+ * overload) computes, and its outputs are identical to that call's
+ * (ph_point_samples_device on the same coordinates), bit for bit.  The samples
+ * of a request are either an explicit device list or the frame-based Cartesian
+ * plane / box of PointSamples(Cartesian, frame, xparams, yparams[, zparams])
+ * (Grids/grids/PointSamples.jl:328-356, coordinates by
+ * Grids/grids/coordinate.jl:44-143), generated on the device with the
+ * reference's arithmetic.  The frame vectors are frame_position / frame_right /
+ * frame_up / frame_forward (Frames/struct/Frame.jl:145-186) evaluated by the
+ * host. ---- */
+#define PH_REQ_POINTS 0 /* count points at px, py, pz (device float64)                  */
+#define PH_REQ_PLANE 1  /* n[0] x n[1] points: position + xi right + eta up              */
+#define PH_REQ_BOX 2    /* n[0] x n[1] x n[2] points: ... + zeta forward                 */
+typedef struct ph_request {
+    int32_t kind;
+    int32_t reserved;
+    int64_t count;                      /* PH_REQ_POINTS                                  */
+    const double* px;                   /* PH_REQ_POINTS: device coordinates              */
+    const double* py;
+    const double* pz;
+    double position[3], right[3], up[3], forward[3]; /* PLANE / BOX frame              */
+    double amin[3], amax[3];            /* (xmin, ymin, zmin), (xmax, ymax, zmax)         */
+    int64_t n[3];                       /* (nx, ny, nz), each >= 2; nz unused by PLANE    */
+    void* out;                          /* device, L x S: double (FP64) or float (FP32C)  */
+} ph_request;
+typedef struct ph_batch ph_batch;
+/* Validates the requests (the reference's ArgumentError rules of coordinate.jl:
+ * max > min and n >= 2 per axis), copies them and records the batch as one HIP
+ * graph: per catalog chunk the particle records are packed and a persistent
+ * work-queue kernel serves fixed-size work units of every request (nearest-h
+ * and accumulation fused) until the queue is empty.  The float64 tree must
+ * stay alive; device pointers in the requests must stay valid while the batch
+ * is used (their contents may change between runs). */
+int ph_batch_create(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                    int32_t nreq, const ph_request* reqs, ph_batch** out);
+/* One launch of the recorded graph; returns when every request is done.
+ * ph_get_stats: interp_ms = kernel_ms = the launch, samples = all requests. */
+int ph_batch_run(ph_batch* batch);
+/* Per request, from device timestamps of the last run: first_us = start of its
+ * first work unit, last_us = end of its last, both in microseconds after the
+ * launch began (NaN for requests without samples). */
+int ph_batch_times(ph_batch* batch, double* first_us_host, double* last_us_host);
+int ph_batch_destroy(ph_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -200,7 +200,14 @@ int ph_ctx_destroy(ph_context* ctx) {
     if (!ctx) return PH_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    // trees outliving their context keep a host handle without device memory
+    // batches and trees outliving their context keep a host handle without device state
+    for (ph_batch* b : ctx->batches) {
+        batch_free(b->st);
+        b->st = nullptr;
+        b->ctx = nullptr;
+        b->B = nullptr;
+    }
+    ctx->batches.clear();
     for (ph_lbvh* B : ctx->trees) {
         B->release_device();
         B->ctx = nullptr;
@@ -350,6 +357,8 @@ int ph_lbvh_destroy(ph_lbvh* B) {
                 v.erase(v.begin() + (std::ptrdiff_t)t);
                 break;
             }
+        for (ph_batch* b : ctx->batches)  // batches over this tree can no longer run
+            if (b->B == B) b->B = nullptr;
         delete B;  // device blocks return to the context's cache; the stream is idle
         ctx->cache.settle();
         return PH_OK;
@@ -659,6 +668,66 @@ int ph_nearest_h(ph_context* ctx, ph_lbvh* B, int64_t S, const double* px, const
 int ph_line_table(int32_t kernel, double* inorm) {
     if (kernel < 0 || kernel > 5 || !inorm) return fail(nullptr, PH_ERR_ARGUMENT, "bad arguments");
     std::memcpy(inorm, ph_lut_inorm[kernel], sizeof(double) * PH_LUT_N);
+    return PH_OK;
+}
+
+int ph_batch_create(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                    int32_t nreq, const ph_request* reqs, ph_batch** out) {
+    if (out) *out = nullptr;
+    return guarded(ctx, [&] {
+        need(ctx != nullptr && out != nullptr, PH_ERR_ARGUMENT, "null context or output handle");
+        need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
+        need(strategy >= PH_GATHER && strategy <= PH_SYMMETRIC, PH_ERR_ARGUMENT, "unknown strategy");
+        need(nreq >= 0 && (nreq == 0 || reqs), PH_ERR_ARGUMENT, "bad request list");
+        need(precision == PH_PREC_FP64 || precision == PH_PREC_FP32C, PH_ERR_ARGUMENT,
+             "batches run in PH_PREC_FP64 or PH_PREC_FP32C");
+        check_precision(B, precision);
+        check_catalog(B, cat);
+        need(B->n <= (int64_t(1) << 26), PH_ERR_CAPACITY, "batches support up to 2^26 particles");
+        auto b = std::make_unique<ph_batch>();
+        b->ctx = ctx;
+        b->B = B;
+        b->st = batch_build(ctx, B, cat, strategy, precision, nreq, reqs);
+        ctx->batches.push_back(b.get());
+        *out = b.release();
+        return PH_OK;
+    });
+}
+
+int ph_batch_run(ph_batch* b) {
+    if (!b || !b->ctx) return fail(nullptr, PH_ERR_ARGUMENT, "null batch or batch of a destroyed context");
+    ph_context* ctx = b->ctx;
+    return guarded(ctx, [&] {
+        need(b->B != nullptr, PH_ERR_ARGUMENT, "the batch's tree was destroyed");
+        batch_run(b->st, ctx, b->B);
+        return PH_OK;
+    });
+}
+
+int ph_batch_times(ph_batch* b, double* first_us, double* last_us) {
+    if (!b || !b->ctx) return fail(nullptr, PH_ERR_ARGUMENT, "null batch or batch of a destroyed context");
+    ph_context* ctx = b->ctx;
+    return guarded(ctx, [&] {
+        batch_times(b->st, ctx, first_us, last_us);
+        return PH_OK;
+    });
+}
+
+int ph_batch_destroy(ph_batch* b) {
+    if (!b) return PH_OK;
+    if (ph_context* ctx = b->ctx) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        auto& v = ctx->batches;
+        for (size_t t = 0; t < v.size(); ++t)
+            if (v[t] == b) {
+                v.erase(v.begin() + (std::ptrdiff_t)t);
+                break;
+            }
+        batch_free(b->st);  // device blocks return to the context's cache; the stream is idle
+        ctx->cache.settle();
+    }
+    delete b;
     return PH_OK;
 }
 

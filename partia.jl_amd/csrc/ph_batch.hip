@@ -1,0 +1,373 @@
+// ph_batch.hip -- batched sampling: many PointSamples requests of uneven size
+// over one tree, served by a persistent work-queue kernel recorded in a HIP graph.
+//
+// Work is cut into units of kBlock consecutive samples of one request.  Every
+// workgroup of the persistent kernel takes the next unit from a global counter,
+// finds its request by binary search over the requests' first units, generates
+// the sample coordinates (explicit list, or the frame-based Cartesian plane /
+// box of Grids/grids/coordinate.jl:44-143 with the reference's arithmetic),
+// computes the nearest-particle smoothing length (gather / symmetric) and runs
+// the same per-sample body as the per-call bucket kernels.  Every sample is
+// computed with exactly the per-call arithmetic, so each request's outputs equal
+// those of a ph_point_samples_device call on the same coordinates, bit for bit.
+// Device timestamps (s_memrealtime) give each request's first-unit start and
+// last-unit end.
+#include <algorithm>
+#include <memory>
+
+#include "ph_bucket_dev.h"
+
+namespace ph {
+
+// Device descriptor of one request.
+struct BatchReq {
+    i64 S;      // samples
+    i64 unit0;  // first work unit
+    i32 kind;
+    i32 pad;
+    const double* px;
+    const double* py;
+    const double* pz;
+    double o[3], r[3], u[3], f[3];
+    double amin[3], d[3];  // axis minimum and step (max - min) / (n - 1)
+    i64 n[3];
+    void* out;
+};
+
+struct BatchState {
+    int strategy = 0, precision = 0, nreq = 0;
+    i64 units = 0, samples = 0;
+    std::vector<BatchReq> host;
+    std::vector<CatChunk> chunks;  // as launched (FP32C: compacted)
+    std::vector<int> ncols;        // FP32C: distinct columns per chunk
+    int rw_max = 2;
+    DevBuf reqs, rec, ctl, times;  // ctl: per-chunk counters + launch timestamp
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void* cols_at_capture = nullptr;
+    double clock_khz = 100000.0;
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+namespace {
+
+// _cartesian_plane_coordinates / _cartesian_box_coordinates
+// (Grids/grids/coordinate.jl:44-143): xi = xmin + (i - 1) dx with
+// dx = (xmax - xmin) / (nx - 1), x = x0 + xi rx + eta ux [+ zeta fx], evaluated
+// left to right without contraction; sample k = i + (j - 1) nx + (k3 - 1) nx ny.
+__device__ inline void batch_point(const BatchReq& R, i64 s, double p[3]) {
+    if (R.kind == PH_REQ_POINTS) {
+        p[0] = R.px[s];
+        p[1] = R.py[s];
+        p[2] = R.pz[s];
+        return;
+    }
+    const i64 nx = R.n[0], ny = R.n[1];
+    const i64 i = s % nx, j = (s / nx) % ny, k3 = s / (nx * ny);
+    const double xi = R.amin[0] + (double)i * R.d[0];
+    const double eta = R.amin[1] + (double)j * R.d[1];
+    if (R.kind == PH_REQ_PLANE) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) p[a] = (R.o[a] + xi * R.r[a]) + eta * R.u[a];
+    } else {
+        const double zeta = R.amin[2] + (double)k3 * R.d[2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) p[a] = ((R.o[a] + xi * R.r[a]) + eta * R.u[a]) + zeta * R.f[a];
+    }
+}
+
+// Clears a chunk's work counter; for the first chunk also stamps the launch
+// time and clears the per-request times (a kernel, not a memset: the graph is
+// replayed).
+__global__ void k_batch_reset(unsigned long long* __restrict__ ctr, unsigned long long* __restrict__ stamp,
+                              unsigned long long* __restrict__ times, i32 nreq) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        *ctr = 0ull;
+        if (stamp) *stamp = (unsigned long long)wall_clock64();
+    }
+    if (stamp)
+        for (i64 j = i; j < nreq; j += (i64)gridDim.x * blockDim.x) {
+            times[j] = ~0ull;
+            times[nreq + j] = 0ull;
+        }
+}
+
+__device__ inline i64 uniform_i64(i64 v) {
+    const u32 lo = (u32)__builtin_amdgcn_readfirstlane((int)(u32)v);
+    const u32 hi = (u32)__builtin_amdgcn_readfirstlane((int)(u32)((u64)v >> 32));
+    return (i64)(((u64)hi << 32) | lo);
+}
+
+// Persistent loop.  Thread 0 is the only thread with work of its own, in one
+// region per iteration between barriers: it records the times of the unit the
+// workgroup just finished and takes the next unit.  The unit is then read back
+// into scalar registers (wave-uniform), so the loop exit is a uniform branch:
+// every wave leaves in the same iteration and executes the same barriers.
+template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT, bool C32>
+__global__ void __launch_bounds__(kBlock) k_batch(BucketView bv, const double* __restrict__ rec, CatChunk c,
+                                                  const BatchReq* __restrict__ reqs, i32 nreq, i64 units,
+                                                  unsigned long long* __restrict__ ctr,
+                                                  unsigned long long* __restrict__ times) {
+    __shared__ i64 s_unit;
+    __shared__ unsigned long long s_t0;
+    i32 done_req = -1;              // thread 0: request of the unit just finished
+    unsigned long long done_t0 = 0;  // thread 0: its start time
+    for (;;) {
+        if (threadIdx.x == 0) {
+            if (done_req >= 0) {
+                atomicMin(&times[done_req], done_t0);
+                atomicMax(&times[nreq + done_req], (unsigned long long)wall_clock64());
+            }
+            s_unit = (i64)atomicAdd(ctr, 1ull);
+            s_t0 = (unsigned long long)wall_clock64();
+        }
+        __syncthreads();
+        const i64 u = uniform_i64(s_unit);
+        const unsigned long long t0 = s_t0;
+        __syncthreads();  // every thread has read the unit before thread 0 takes the next
+        if (u >= units) break;
+        // the last request whose first unit is <= u (requests without samples
+        // share their first unit with the next one and are never chosen)
+        i32 lo = 0, hi = nreq - 1;
+        while (lo < hi) {
+            const i32 mid = (lo + hi + 1) >> 1;
+            if (reqs[mid].unit0 <= u) lo = mid;
+            else hi = mid - 1;
+        }
+        lo = __builtin_amdgcn_readfirstlane(lo);
+        const BatchReq& R = reqs[lo];
+        const i64 s = (u - R.unit0) * kBlock + threadIdx.x;
+        const bool valid = s < R.S;
+        double p[3] = {0.0, 0.0, 0.0};
+        if (valid) batch_point(R, s, p);
+        double ha = 0.0;
+        if constexpr (STRAT != PH_SCATTER) {
+            if (valid) ha = nearest_h_bucket(bv, p);
+        }
+        u64 accepted = 0;
+        if constexpr (C32)
+            bucket_c32_sample<K, STRAT, MS, MG, MD, MC, EXACT>(bv, rec, c, valid, p, ha, static_cast<float*>(R.out),
+                                                               R.S, s, accepted);
+        else
+            bucket_f64_sample<K, STRAT, MS, MG, MD, MC>(bv, rec, c, valid, p, ha, static_cast<double*>(R.out), R.S, s,
+                                                        accepted);
+        __syncthreads();  // the unit is complete
+        done_req = lo;
+        done_t0 = t0;
+    }
+}
+
+template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT, bool C32>
+void launch_batch_t(BatchState& st, const BucketView& bv, const CatChunk& c, unsigned long long* ctr, int num_cu,
+                    hipStream_t s) {
+    auto kern = k_batch<K, STRAT, MS, MG, MD, MC, EXACT, C32>;
+    int per_cu = 0;
+    PH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
+    const i64 want = (i64)(per_cu > 0 ? per_cu : 1) * num_cu;
+    const unsigned g = (unsigned)(want < st.units ? want : (st.units > 0 ? st.units : 1));
+    kern<<<g, kBlock, 0, s>>>(bv, st.rec.as<double>(), c, st.reqs.as<BatchReq>(), st.nreq, st.units, ctr,
+                              st.times.as<unsigned long long>());
+    PH_HIP(hipGetLastError());
+}
+
+template <int K, int STRAT>
+void launch_batch_ks(BatchState& st, const BucketView& bv, const CatChunk& c, unsigned long long* ctr, int num_cu,
+                     hipStream_t s) {
+    const bool small = c.ng + c.nd + c.nc == 0 && c.ns <= 2;
+    if (st.precision == PH_PREC_FP64) {
+        if (small) launch_batch_t<K, STRAT, 2, 0, 0, 0, false, false>(st, bv, c, ctr, num_cu, s);
+        else launch_batch_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, false, false>(st, bv, c, ctr, num_cu, s);
+    } else {
+        if (small) launch_batch_t<K, STRAT, 2, 0, 0, 0, false, true>(st, bv, c, ctr, num_cu, s);
+        else if (c.ns == 2 && c.ng == 1 && c.nd == 1 && c.nc == 1)
+            launch_batch_t<K, STRAT, 2, 1, 1, 1, true, true>(st, bv, c, ctr, num_cu, s);
+        else launch_batch_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, false, true>(st, bv, c, ctr, num_cu, s);
+    }
+}
+
+template <int K>
+void launch_batch_k(BatchState& st, const BucketView& bv, const CatChunk& c, unsigned long long* ctr, int num_cu,
+                    hipStream_t s) {
+    switch (st.strategy) {
+    case PH_GATHER: launch_batch_ks<K, PH_GATHER>(st, bv, c, ctr, num_cu, s); break;
+    case PH_SCATTER: launch_batch_ks<K, PH_SCATTER>(st, bv, c, ctr, num_cu, s); break;
+    default: launch_batch_ks<K, PH_SYMMETRIC>(st, bv, c, ctr, num_cu, s); break;
+    }
+}
+
+// Records the launch sequence of one run on stream s.  ctl[0] = launch
+// timestamp, ctl[1 + q] = work counter of catalog chunk q.
+void record(BatchState& st, ph_lbvh* B, int num_cu, hipStream_t s) {
+    const BucketView bv = bucket_view(B);
+    unsigned long long* ctl = st.ctl.as<unsigned long long>();
+    unsigned long long* times = st.times.as<unsigned long long>();
+    const i64 n = B->n;
+    for (size_t q = 0; q < st.chunks.size(); ++q) {
+        const CatChunk& c = st.chunks[q];
+        unsigned long long* ctr = ctl + 1 + q;
+        k_batch_reset<<<q == 0 ? grid_for(st.nreq) : 1, kBlock, 0, s>>>(ctr, q == 0 ? ctl : nullptr, times, st.nreq);
+        PH_HIP(hipGetLastError());
+        if (st.precision == PH_PREC_FP64)
+            k_pack_records<double><<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c, st.rec.as<double>());
+        else
+            k_pack_c32<<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c, st.ncols[q], c.rw,
+                                                      st.rec.as<double>());
+        PH_HIP(hipGetLastError());
+        if (st.units == 0) continue;
+        switch (B->kernel) {
+        case PH_KERNEL_M4: launch_batch_k<PH_KERNEL_M4>(st, bv, c, ctr, num_cu, s); break;
+        case PH_KERNEL_M5: launch_batch_k<PH_KERNEL_M5>(st, bv, c, ctr, num_cu, s); break;
+        case PH_KERNEL_M6: launch_batch_k<PH_KERNEL_M6>(st, bv, c, ctr, num_cu, s); break;
+        case PH_KERNEL_C2: launch_batch_k<PH_KERNEL_C2>(st, bv, c, ctr, num_cu, s); break;
+        case PH_KERNEL_C4: launch_batch_k<PH_KERNEL_C4>(st, bv, c, ctr, num_cu, s); break;
+        default: launch_batch_k<PH_KERNEL_C6>(st, bv, c, ctr, num_cu, s); break;
+        }
+    }
+}
+
+void capture(BatchState& st, ph_context* ctx, ph_lbvh* B) {
+    st.drop_graph();
+    hipStream_t s = ctx->stream;
+    PH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        record(st, B, ctx->num_cu, s);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        throw;
+    }
+    PH_HIP(hipStreamEndCapture(s, &st.graph));
+    PH_HIP(hipGraphInstantiate(&st.exec, st.graph, nullptr, nullptr, 0));
+    st.cols_at_capture = B->cols.p;
+}
+
+}  // namespace
+
+BatchState* batch_build(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int strategy, int precision, int nreq,
+                        const ph_request* reqs) {
+    auto st = std::make_unique<BatchState>();
+    st->strategy = strategy;
+    st->precision = precision;
+    st->nreq = nreq;
+    st->host.resize((size_t)nreq);
+    i64 unit = 0;
+    for (int r = 0; r < nreq; ++r) {
+        const ph_request& q = reqs[r];
+        BatchReq& d = st->host[(size_t)r];
+        d = BatchReq{};
+        d.kind = q.kind;
+        if (q.kind == PH_REQ_POINTS) {
+            if (q.count < 0) throw Error{PH_ERR_ARGUMENT, "request " + std::to_string(r) + ": negative count"};
+            if (q.count > 0 && !(q.px && q.py && q.pz))
+                throw Error{PH_ERR_ARGUMENT, "request " + std::to_string(r) + ": null coordinate pointer"};
+            d.S = q.count;
+            d.px = q.px;
+            d.py = q.py;
+            d.pz = q.pz;
+        } else if (q.kind == PH_REQ_PLANE || q.kind == PH_REQ_BOX) {
+            const int axes = q.kind == PH_REQ_PLANE ? 2 : 3;
+            static const char* nm[3] = {"x", "y", "z"};
+            d.S = 1;
+            for (int a = 0; a < axes; ++a) {
+                // coordinate.jl:48-61, 94-113
+                if (!(q.amax[a] > q.amin[a]))
+                    throw Error{PH_ERR_ARGUMENT, std::string(nm[a]) + "max must be greater than " + nm[a] + "min."};
+                if (q.n[a] < 2) throw Error{PH_ERR_ARGUMENT, std::string("n") + nm[a] + " must be at least 2."};
+                d.amin[a] = q.amin[a];
+                d.d[a] = (q.amax[a] - q.amin[a]) / (double)(q.n[a] - 1);
+                d.n[a] = q.n[a];
+                if (d.S > (i64(1) << 40) / q.n[a]) throw Error{PH_ERR_ARGUMENT, "request too large"};
+                d.S *= q.n[a];
+            }
+            if (axes == 2) d.n[2] = 1;
+            for (int a = 0; a < 3; ++a) {
+                d.o[a] = q.position[a];
+                d.r[a] = q.right[a];
+                d.u[a] = q.up[a];
+                d.f[a] = q.forward[a];
+            }
+        } else {
+            throw Error{PH_ERR_ARGUMENT, "request " + std::to_string(r) + ": unknown kind"};
+        }
+        if (d.S > 0 && !q.out) throw Error{PH_ERR_ARGUMENT, "request " + std::to_string(r) + ": null output pointer"};
+        d.out = q.out;
+        d.unit0 = unit;
+        unit += (d.S + kBlock - 1) / kBlock;
+        st->samples += d.S;
+    }
+    st->units = unit;
+    split_catalog(cat, st->chunks);
+    if (st->chunks.empty()) st->chunks.push_back(CatChunk{});
+    st->ncols.assign(st->chunks.size(), 0);
+    for (size_t q = 0; q < st->chunks.size(); ++q) {
+        if (precision != PH_PREC_FP64) st->chunks[q] = compact_c32(st->chunks[q], st->ncols[q]);
+        st->rw_max = std::max(st->rw_max, st->chunks[q].rw);
+    }
+    const i64 n = B->n;
+    if (precision != PH_PREC_FP64 && (i64)st->rw_max * 8 * n >= (i64(1) << 32))
+        throw Error{PH_ERR_CAPACITY, "FP32C records exceed 4 GiB; split the catalog or use PH_PREC_FP64"};
+    ensure_buckets(B, ctx->stream);
+    st->rec.alloc(sizeof(double) * (size_t)st->rw_max * (size_t)n);
+    st->ctl.alloc(sizeof(unsigned long long) * (1 + st->chunks.size()));
+    st->times.alloc(sizeof(unsigned long long) * 2 * (size_t)std::max(nreq, 1));
+    st->reqs.alloc(sizeof(BatchReq) * (size_t)std::max(nreq, 1));
+    if (nreq > 0)
+        PH_HIP(hipMemcpyAsync(st->reqs.p, st->host.data(), sizeof(BatchReq) * (size_t)nreq, hipMemcpyHostToDevice,
+                              ctx->stream));
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) == hipSuccess && khz > 0)
+        st->clock_khz = khz;
+    (void)hipGetLastError();
+    PH_HIP(hipStreamSynchronize(ctx->stream));
+    capture(*st, ctx, B);
+    return st.release();
+}
+
+void batch_run(BatchState* st, ph_context* ctx, ph_lbvh* B) {
+    if (B->cols.p != st->cols_at_capture) capture(*st, ctx, B);  // the columns were reallocated
+    PH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    PH_HIP(hipGraphLaunch(st->exec, ctx->stream));
+    PH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    PH_HIP(hipEventSynchronize(ctx->ev[1]));
+    float ms = 0.f;
+    PH_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    ctx->stats.interp_ms = ms;
+    ctx->stats.kernel_ms = ms;
+    ctx->stats.samples = st->samples;
+    ctx->stats.accepted = -1;
+    ctx->stats.rechecks = -1;
+}
+
+void batch_times(BatchState* st, ph_context* ctx, double* first_us, double* last_us) {
+    const int nreq = st->nreq;
+    std::vector<unsigned long long> t(2 * (size_t)nreq + 1);
+    PH_HIP(hipStreamSynchronize(ctx->stream));
+    PH_HIP(hipMemcpy(t.data(), st->ctl.p, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    if (nreq > 0)
+        PH_HIP(hipMemcpy(t.data() + 1, st->times.p, sizeof(unsigned long long) * 2 * (size_t)nreq,
+                         hipMemcpyDeviceToHost));
+    const double us_per_tick = 1000.0 / st->clock_khz;
+    const unsigned long long t0 = t[0];
+    for (int r = 0; r < nreq; ++r) {
+        const unsigned long long a = t[1 + (size_t)r], b = t[1 + (size_t)nreq + r];
+        const bool none = st->host[(size_t)r].S == 0 || a == ~0ull;
+        if (first_us) first_us[r] = none ? NAN : (double)(long long)(a - t0) * us_per_tick;
+        if (last_us) last_us[r] = none ? NAN : (double)(long long)(b - t0) * us_per_tick;
+    }
+}
+
+void batch_free(BatchState* st) {
+    if (!st) return;
+    st->drop_graph();
+    delete st;
+}
+
+}  // namespace ph

@@ -170,6 +170,7 @@ struct ph_context {
     ph::DevBuf counters;       // small device counters (accepted neighbours, rechecks, flags)
     int num_cu = 256;
     std::vector<struct ph_lbvh*> trees;  // live trees; released with the context
+    std::vector<struct ph_batch*> batches;  // live batches; their device state is released with the context
 };
 
 // Device-resident InterpolationInput after LinearBVH!.
@@ -207,4 +208,14 @@ struct ph_lbvh {
             b->release();
         buckets_ready = false;
     }
+};
+
+namespace ph {
+struct BatchState;
+}
+// Batched sampling requests recorded over one tree (ph_batch_create).
+struct ph_batch {
+    ph_context* ctx = nullptr;  // null once the context was destroyed
+    ph_lbvh* B = nullptr;       // null once the tree was destroyed
+    ph::BatchState* st = nullptr;
 };

@@ -67,4 +67,12 @@ void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, c
 // Builds the bucket tree of B if it is not built yet.
 void ensure_buckets(ph_lbvh* B, hipStream_t s);
 
+// Batched sampling (ph_batch.hip).  batch_build validates and copies the
+// requests, packs nothing yet and records the launch graph; throws Error.
+BatchState* batch_build(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int strategy, int precision, int nreq,
+                        const ph_request* reqs);
+void batch_run(BatchState* st, ph_context* ctx, ph_lbvh* B);
+void batch_times(BatchState* st, ph_context* ctx, double* first_us, double* last_us);
+void batch_free(BatchState* st);
+
 }  // namespace ph

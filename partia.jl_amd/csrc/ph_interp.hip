@@ -172,6 +172,19 @@ template <class T, int K> struct LineAcc {
         for (int j = 0; j < kMaxS; ++j)
             if (j < c.ns) acc[j] += cols[c.rsrc[c.spos[j]] * n + k] * (mb * W / rb);
     }
+    // Same terms for a wave in lockstep: every lane evaluates them, and a lane
+    // with take == false adds exact zeros (S1 and the sums start at +0 and never
+    // become -0, so x + 0 == x).  mb, rb and the quantities qv are the leaf's.
+    __device__ inline void add_sel(bool take, T mb, T rb, const T (&qv)[kMaxS], const CatChunk& c, T d2, T h,
+                                   const T* __restrict__ inorm, T dq) {
+        const T dr = sqrt(d2 < T(0) ? T(0) : d2);
+        const T W = line_W_nb<T, K>(dr, h, inorm, dq);
+        const T t = mb * W / rb;
+        S1 += take ? t : T(0);
+#pragma unroll
+        for (int j = 0; j < kMaxS; ++j)
+            if (j < c.ns) acc[j] += take ? qv[j] * t : T(0);
+    }
     // NaN outputs when no particle contributes (:154-156), Shepard scalars * 1/S1.
     __device__ inline void store(const CatChunk& c, T* __restrict__ out, i64 S, i64 s) const {
         if (S1 == T(0)) {
@@ -219,59 +232,60 @@ __global__ void __launch_bounds__(kBlock) k_lines(TreeView<T> tv, const T* __res
 // idle until the walk reaches escape(u), which the union walk always reaches
 // right after u's subtree, so every lane tests exactly the nodes and leaves of
 // its own DFS, in the same order, with the same arithmetic as k_lines.  The
-// results are therefore identical to k_lines, bit for bit.
+// results are therefore identical to k_lines, bit for bit.  The steps are
+// branch-free within the wave (idle lanes compute and discard), and the
+// line-integrated table is staged in LDS.
 template <class T, int K, int SRC>
 __global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const T* __restrict__ cols, CatChunk c,
                                                        LineSampler<T, K, PH_SCATTER, SRC> smp,
                                                        const T* __restrict__ inorm, T dq, T* __restrict__ out,
                                                        u64* counter) {
+    __shared__ T tab[PH_LUT_N_ENTRIES];
+    for (int i = threadIdx.x; i < PH_LUT_N_ENTRIES; i += kBlock) tab[i] = inorm[i];
+    __syncthreads();
     i64 s;
     Line<T> l;
     const bool valid = smp.get(s, l);
-    if (__ballot(valid) == 0) return;  // wave-uniform
+    if (__ballot(valid) == 0) return;  // wave-uniform; no barrier follows
     const T Kvalid = T(support_of<K>());
     const i64 n = tv.n;
+    const T* __restrict__ mcol = cols + 3 * n;
+    const T* __restrict__ rcol = cols + 5 * n;
+    const T* qcol[kMaxS];
+#pragma unroll
+    for (int j = 0; j < kMaxS; ++j) qcol[j] = cols + (i64)c.rsrc[c.spos[j < c.ns ? j : 0]] * n;
     LineAcc<T, K> a;
     u64 accepted = 0;
+    auto leaf = [&](i64 k, bool active) {
+        const LeafRec<T> L = tv.leaves[k];
+        const T mb = mcol[k], rb = rcol[k];
+        T qv[kMaxS];
+#pragma unroll
+        for (int j = 0; j < kMaxS; ++j) qv[j] = j < c.ns ? qcol[j][k] : T(0);
+        const T d2 = d2_line_leaf(l, L);
+        const bool take = active && d2 <= radius2<T, PH_SCATTER>(Kvalid, T(0), L.h);
+        accepted += take ? 1 : 0;
+        a.add_sel(take, mb, rb, qv, c, d2, L.h, tab, dq);
+    };
     if (tv.root == 0) {  // n == 1: the reference's brute-force branch
-        for (i64 k = 0; k < n; ++k) {
-            const LeafRec<T> L = tv.leaves[k];
-            const T d2 = d2_line_leaf(l, L);
-            if (valid && d2 <= radius2<T, PH_SCATTER>(Kvalid, T(0), L.h)) {
-                ++accepted;
-                a.add(cols, c, n, k, d2, L.h, inorm, dq);
-            }
-        }
+        for (i64 k = 0; k < n; ++k) leaf(k, valid);
     } else {
         bool active = valid;
         i32 resume = -1;  // node at which an idle lane rejoins (0: never)
         i32 node = tv.root;
         for (i64 steps = 2 * n + 8; node != 0 && steps > 0; --steps) {
             node = __builtin_amdgcn_readfirstlane(node);
-            if (!active && node == resume) active = true;
+            active = active || node == resume;
             if (node >= (i32)n) {
                 const i64 k = (i64)node - n;
-                const LeafRec<T> L = tv.leaves[k];
-                if (active) {
-                    const T d2 = d2_line_leaf(l, L);
-                    if (d2 <= radius2<T, PH_SCATTER>(Kvalid, T(0), L.h)) {
-                        ++accepted;
-                        a.add(cols, c, n, k, d2, L.h, inorm, dq);
-                    }
-                }
+                leaf(k, active);
                 node = tv.leaf_escape[k];
             } else {
                 const NodeRec<T> N = tv.nodes[node - 1];
-                bool descend = false;
-                if (active) {
-                    const T d2n = d2_line_aabb_lb(l, N);
-                    if (d2n > radius2<T, PH_SCATTER>(Kvalid, T(0), N.scale)) {
-                        active = false;
-                        resume = N.escape;
-                    } else {
-                        descend = true;
-                    }
-                }
+                const bool prune = d2_line_aabb_lb(l, N) > radius2<T, PH_SCATTER>(Kvalid, T(0), N.scale);
+                resume = (active && prune) ? N.escape : resume;
+                const bool descend = active && !prune;
+                active = descend;
                 node = __ballot(descend) ? N.left : N.escape;
             }
         }

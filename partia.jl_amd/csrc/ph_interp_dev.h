@@ -72,7 +72,19 @@ template <class T> __device__ inline T d2_line_aabb_lb(const Line<T>& l, const N
         if (m == T(0)) {
             if (o < a || o > b) hit = false;
         } else {
-            T t0 = (a - o) / m, t1 = (b - o) / m;
+            // x / 1 == x and x / -1 == -x exactly (signed zeros included): axis-
+            // aligned rays skip the two divisions with identical results
+            T t0, t1;
+            if (m == T(1)) {
+                t0 = a - o;
+                t1 = b - o;
+            } else if (m == T(-1)) {
+                t0 = -(a - o);
+                t1 = -(b - o);
+            } else {
+                t0 = (a - o) / m;
+                t1 = (b - o) / m;
+            }
             if (t0 > t1) {
                 const T tt = t0;
                 t0 = t1;

@@ -148,4 +148,21 @@ template <class T, int K> __device__ inline T line_W(T r, T h, const T* __restri
     return invh * I;
 }
 
+// line_W without the branch: the table index is clamped for every q (a NaN or
+// huge q reads entry 1 or N-1), and the cutoff selects 0, so the value equals
+// line_W for every q.
+template <class T, int K> __device__ inline T line_W_nb(T r, T h, const T* __restrict__ inorm, T dq) {
+    T invh = T(1) / h;
+    T q = r * invh;
+    T idxf = q / dq + T(1);
+    const T lo = T(1), hi = T(PH_LUT_N_ENTRIES - 1);
+    const T cf = idxf < lo ? lo : (idxf > hi ? hi : idxf);  // NaN stays NaN: converted to 0 below, then clamped
+    i32 i = (i32)cf;
+    i = i < 1 ? 1 : (i > PH_LUT_N_ENTRIES - 1 ? PH_LUT_N_ENTRIES - 1 : i);
+    T t = idxf - T(i);
+    T I = inorm[i - 1] * (T(1) - t) + inorm[i] * t;
+    I = (q >= T(support_of<K>())) ? T(0) : I;
+    return invh * I;
+}
+
 }  // namespace ph

@@ -33,7 +33,10 @@ EXPORTS = (
     "ph_lbvh_get",
     "ph_point_samples", "ph_point_samples_device", "ph_structured_grid", "ph_structured_grid_device",
     "ph_line_samples", "ph_line_samples_device", "ph_line_grid_device", "ph_nearest_h", "ph_line_table",
+    "ph_batch_create", "ph_batch_run", "ph_batch_times", "ph_batch_destroy",
 )
+
+PH_REQ_POINTS, PH_REQ_PLANE, PH_REQ_BOX = 0, 1, 2
 
 
 class NativeLibraryError(RuntimeError):
@@ -47,6 +50,14 @@ class ph_catalog(C.Structure):
         ("n_div", C.c_int32), ("div_slots", C.c_void_p),
         ("n_curl", C.c_int32), ("curl_slots", C.c_void_p),
     ]
+
+
+class ph_request(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("count", C.c_int64),
+                ("px", C.c_void_p), ("py", C.c_void_p), ("pz", C.c_void_p),
+                ("position", C.c_double * 3), ("right", C.c_double * 3), ("up", C.c_double * 3),
+                ("forward", C.c_double * 3), ("amin", C.c_double * 3), ("amax", C.c_double * 3),
+                ("n", C.c_int64 * 3), ("out", C.c_void_p)]
 
 
 class ph_stats(C.Structure):
@@ -99,6 +110,10 @@ def lib():
         "ph_line_grid_device": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, vp, vp, vp, vp, i64, vp, i64, vp]),
         "ph_nearest_h": (i32, [vp, vp, i64, vp, vp, vp, vp]),
         "ph_line_table": (i32, [i32, vp]),
+        "ph_batch_create": (i32, [vp, vp, P(ph_catalog), i32, i32, i32, P(ph_request), P(vp)]),
+        "ph_batch_run": (i32, [vp]),
+        "ph_batch_times": (i32, [vp, vp, vp]),
+        "ph_batch_destroy": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -119,15 +134,15 @@ def f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-# Live handles, released before the interpreter exits: trees first, then their
-# contexts, while the HIP runtime is still up (objects kept alive until shutdown,
+# Live handles, released before the interpreter exits: batches and trees first,
+# then their contexts, while the HIP runtime is still up (objects kept alive until shutdown,
 # e.g. by a traceback, would otherwise be freed after the runtime's own teardown).
 _trees = weakref.WeakSet()
 _contexts = weakref.WeakSet()
 
 
 def register_tree(obj):
-    """Track an object owning a ph_lbvh handle (it must have close())."""
+    """Track an object owning a ph_lbvh or ph_batch handle (it must have close())."""
     _trees.add(obj)
 
 

@@ -64,6 +64,48 @@ class PointSamples(AbstractSamples):
         return len(self.grid)
 
 
+def _check_axis(name, params):
+    lo, hi, n = params
+    if not hi > lo:
+        raise ArgumentError("%smax must be greater than %smin." % (name, name))
+    if int(n) < 2:
+        raise ArgumentError("n%s must be at least 2." % name)
+    return float(lo), float(hi), int(n)
+
+
+def cartesian_plane_coordinates(position, right, up, xparams, yparams):
+    """_cartesian_plane_coordinates (Grids/grids/coordinate.jl:44-86) from the
+    frame's position / right / up vectors (Frames/struct/Frame.jl:145-186):
+    xi = xmin + (i - 1) dx, dx = (xmax - xmin) / (nx - 1), x = x0 + xi rx + eta ux,
+    sample k = i + (j - 1) nx.  numpy evaluates each operation rounded (no fused
+    multiply-add), as Julia does."""
+    x0, r, u = (np.asarray(v, dtype=np.float64) for v in (position, right, up))
+    xmin, xmax, nx = _check_axis("x", xparams)
+    ymin, ymax, ny = _check_axis("y", yparams)
+    dx, dy = (xmax - xmin) / float(nx - 1), (ymax - ymin) / float(ny - 1)
+    xi = xmin + np.arange(nx, dtype=np.float64) * dx
+    eta = ymin + np.arange(ny, dtype=np.float64) * dy
+    XI, ETA = np.meshgrid(xi, eta, indexing="xy")  # x fastest
+    XI, ETA = XI.ravel(), ETA.ravel()
+    return tuple((x0[a] + XI * r[a]) + ETA * u[a] for a in range(3))
+
+
+def cartesian_box_coordinates(position, right, up, forward, xparams, yparams, zparams):
+    """_cartesian_box_coordinates (Grids/grids/coordinate.jl:88-143): as the
+    plane plus zeta fx along the frame's forward vector; x fastest, then y, z."""
+    x0, r, u, f = (np.asarray(v, dtype=np.float64) for v in (position, right, up, forward))
+    xmin, xmax, nx = _check_axis("x", xparams)
+    ymin, ymax, ny = _check_axis("y", yparams)
+    zmin, zmax, nz = _check_axis("z", zparams)
+    dx, dy, dz = (xmax - xmin) / float(nx - 1), (ymax - ymin) / float(ny - 1), (zmax - zmin) / float(nz - 1)
+    xi = xmin + np.arange(nx, dtype=np.float64) * dx
+    eta = ymin + np.arange(ny, dtype=np.float64) * dy
+    zeta = zmin + np.arange(nz, dtype=np.float64) * dz
+    ZE, ET, XI = np.meshgrid(zeta, eta, xi, indexing="ij")  # x fastest
+    XI, ET, ZE = XI.ravel(), ET.ravel(), ZE.ravel()
+    return tuple(((x0[a] + XI * r[a]) + ET * u[a]) + ZE * f[a] for a in range(3))
+
+
 class LineSamples(AbstractSamples):
     def __init__(self, grid, origin, direction):
         origin, direction = tuple(origin), tuple(direction)

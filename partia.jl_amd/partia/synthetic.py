@@ -11,6 +11,11 @@ Workloads (sizes as in the benchmark configurations):
   plummer_sphere   Plummer sphere (scale radius a, truncated at 10 a), m = 1/N,
                    h = eta (m / rho_Plummer(r))^(1/3), v = solid-body rotation
                    about z plus N(0, sigma_v^2) noise, A as above
+  uniform_plus_clump  half uniform in [0,1]^3, half a Plummer clump (a = 0.05,
+                   truncated at 10 a) at the centre; m = 1/N, h from the analytic
+                   density of both components, v = rotation about the clump axis
+  request_mix      sampling requests of uneven size over [0,1]^3: point lists
+                   (count log-uniform), oriented planes (n^2) and boxes (n^3)
   lognormal_field  particles drawn with probability proportional to a log-normal
                    density exp(delta - s^2/2), delta a Gaussian random field with
                    P(k) ~ k^(-11/3) on a periodic base grid
@@ -86,6 +91,67 @@ def plummer_sphere(n, seed=2, a=0.1, rmax_over_a=10.0, eta=1.2, sigma_v=0.01):
     vy = x + sigma_v * st.normal(n)
     vz = sigma_v * st.normal(n)
     return dict(x=x, y=y, z=z, m=m, h=h, rho=rho, A=scalar_field(x, y), vx=vx, vy=vy, vz=vz)
+
+
+def uniform_plus_clump(n, seed=4, a=0.05, rmax_over_a=10.0, eta=1.2, sigma_v=0.01):
+    st = Stream(seed)
+    nu = n // 2
+    nc = n - nu
+    xu, yu, zu = st.uniform(nu), st.uniform(nu), st.uniform(nu)
+    mfrac = rmax_over_a ** 3 / (1.0 + rmax_over_a ** 2) ** 1.5
+    u = st.uniform(nc) * mfrac
+    u = np.where(u <= 0.0, 1e-300, u)
+    r = a / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+    cost = 2.0 * st.uniform(nc) - 1.0
+    phi = 2.0 * np.pi * st.uniform(nc)
+    sint = np.sqrt(np.maximum(0.0, 1.0 - cost * cost))
+    x = np.concatenate([xu, 0.5 + r * sint * np.cos(phi)])
+    y = np.concatenate([yu, 0.5 + r * sint * np.sin(phi)])
+    z = np.concatenate([zu, 0.5 + r * cost])
+    m = np.full(n, 1.0 / n)
+    rc = np.sqrt((x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2)
+    rho = nu / n + plummer_density(rc, a, M=nc / n / mfrac) * (rc <= rmax_over_a * a)
+    h = eta * (m / rho) ** (1.0 / 3.0)
+    vx = -(y - 0.5) + sigma_v * st.normal(n)
+    vy = (x - 0.5) + sigma_v * st.normal(n)
+    vz = sigma_v * st.normal(n)
+    return dict(x=x, y=y, z=z, m=m, h=h, rho=rho, A=scalar_field(x, y), vx=vx, vy=vy, vz=vz)
+
+
+def random_rotation(st):
+    """Rotation matrix of a uniformly random unit quaternion."""
+    q = st.normal(4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def request_mix(nreq, seed=4, points=(1e3, 1e6), plane=(256, 1024), box=(32, 128), half_width=0.3):
+    """Request descriptions, kind round-robin over point list / plane / box.
+    Point lists: count log-uniform in `points`, positions uniform in [0,1]^3.
+    Planes (n x n) and boxes (n^3): n log-uniform in `plane` / `box`, a random
+    frame (right, up, forward = columns 0, 2, 1 of a random rotation) at a
+    uniform position in [0.2, 0.8]^3, local axes (-w, w, n)."""
+    st = Stream(seed + 77)
+    out = []
+    for r in range(nreq):
+        kind = ("points", "plane", "box")[r % 3]
+        u = st.uniform(1)[0]
+        if kind == "points":
+            lo, hi = np.log(points[0]), np.log(points[1])
+            count = int(np.exp(lo + u * (hi - lo)))
+            out.append(dict(kind=kind, count=count, seed=int(st.uniform(1)[0] * 2 ** 31)))
+        else:
+            rng_n = plane if kind == "plane" else box
+            lo, hi = np.log(rng_n[0]), np.log(rng_n[1])
+            nn = int(np.exp(lo + u * (hi - lo)))
+            R = random_rotation(st)
+            pos = 0.2 + 0.6 * st.uniform(3)
+            axis = (-half_width, half_width, nn)
+            out.append(dict(kind=kind, n=nn, position=pos, right=R[:, 0], up=R[:, 2], forward=R[:, 1], axis=axis))
+    return out
 
 
 def gaussian_random_field(ng, seed, slope=-11.0 / 3.0, sigma_ln=1.5):

@@ -95,6 +95,31 @@ class DeviceSet:
             pass
 
 
+class DevArray:
+    """Device buffer from ph_device_alloc, optionally filled from a host array."""
+
+    def __init__(self, ctx, arr=None, nbytes=None):
+        self.ctx = ctx
+        a = None if arr is None else np.ascontiguousarray(arr)
+        size = a.nbytes if a is not None else int(nbytes)
+        self.p = C.c_void_p()
+        ctx.check(N.lib().ph_device_alloc(ctx.handle, max(size, 8), C.byref(self.p)))
+        if a is not None and a.nbytes:
+            ctx.check(N.lib().ph_memcpy_htod(ctx.handle, self.p, a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    def get(self, dtype, count):
+        out = np.zeros(count, dtype)
+        if count:
+            self.ctx.check(N.lib().ph_memcpy_dtoh(self.ctx.handle, out.ctypes.data_as(C.c_void_p), self.p,
+                                                  out.nbytes))
+        return out
+
+    def free(self):
+        if self.p:
+            N.lib().ph_device_free(self.ctx.handle, self.p)
+            self.p = C.c_void_p()
+
+
 def grid_points(ax, ay, az):
     g = np.meshgrid(ax, ay, az, indexing="ij")
     return [a.ravel(order="F") for a in g]

@@ -1,0 +1,196 @@
+"""Batched sampling (ph_batch_*) against one PointSamples call per request.
+
+Every sample of a batch runs the per-call arithmetic, so each request's outputs
+must equal those of ph_point_samples on the same coordinates bit for bit (NaN
+masks equal), in float64 and FP32C.  Plane and box coordinates are generated on
+the device; the per-request calls use the host restatement of
+Grids/grids/coordinate.jl:44-143, so the comparison also pins the generator."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from gpu_util import DevArray, DeviceSet, assert_same
+from partia import ArgumentError
+from partia import _native as N
+from partia import batch as PB
+from partia import grids
+
+pytestmark = pytest.mark.gpu
+FULL = dict(sslots=[1, 2], sflags=[0, 1], gslots=[2], dslots=[3, 4, 5], cslots=[3, 4, 5])
+SMALL = dict(sslots=[1, 2], sflags=[0, 1])
+WIDE = dict(sslots=[1, 2, 3, 4, 5], sflags=[0, 1, 1, 1, 1], gslots=[2, 3], dslots=[3, 4, 5], cslots=[3, 4, 5])
+
+
+def cloud(n, seed):
+    rng = np.random.default_rng(seed)
+    x, y, z = rng.random(n), rng.random(n), rng.random(n)
+    m = rng.random(n) + 0.05
+    h = (rng.random(n) * 0.12 + 0.04) * (200.0 / n) ** (1 / 3)
+    rho = rng.random(n) + 0.3
+    A = np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
+    return (x, y, z, m, h, rho), [rho, A, x + 0.3 * y, y - z, z * x]
+
+
+def random_frame(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    Rm = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                   [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                   [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+    return rng.random(3) * 0.6 + 0.2, Rm[:, 0], Rm[:, 2], Rm[:, 1]  # position, right, up, forward
+
+
+def make_requests(ctx, rng, L, precision, kinds):
+    """Requests of the given kinds with host copies of their coordinates."""
+    esz = 8 if precision == N.PH_PREC_FP64 else 4
+    reqs, coords, bufs, outs = [], [], [], []
+    for kind in kinds:
+        if kind == "points":
+            S = int(rng.choice([0, 1, 63, 64, 257, int(rng.integers(300, 3000))]))
+            P = [rng.random(S) * 1.2 - 0.1 for _ in range(3)]
+            d = [DevArray(ctx, a) for a in P]
+            o = DevArray(ctx, nbytes=max(1, L * S) * esz)
+            reqs.append(PB.points(d[0].p, d[1].p, d[2].p, S, o.p))
+            bufs += d
+        else:
+            pos, r, u, f = random_frame(rng)
+            ext = [(-0.3 * rng.random() - 0.05, 0.3 * rng.random() + 0.05, int(rng.integers(2, 40))) for _ in range(3)]
+            if kind == "plane":
+                P = grids.cartesian_plane_coordinates(pos, r, u, ext[0], ext[1])
+                S = len(P[0])
+                o = DevArray(ctx, nbytes=L * S * esz)
+                reqs.append(PB.plane(pos, r, u, ext[0], ext[1], o.p))
+            else:
+                ext[2] = (ext[2][0], ext[2][1], int(rng.integers(2, 12)))
+                P = grids.cartesian_box_coordinates(pos, r, u, f, ext[0], ext[1], ext[2])
+                S = len(P[0])
+                o = DevArray(ctx, nbytes=L * S * esz)
+                reqs.append(PB.box(pos, r, u, f, ext[0], ext[1], ext[2], o.p))
+        coords.append(P)
+        outs.append((o, S))
+    return reqs, coords, bufs, outs
+
+
+def read_out(o, S, L, precision):
+    dt = np.float64 if precision == N.PH_PREC_FP64 else np.float32
+    return o.get(dt, L * S).astype(np.float64).reshape(L, S)
+
+
+@pytest.mark.parametrize("precision", [N.PH_PREC_FP64, N.PH_PREC_FP32C])
+@pytest.mark.parametrize("strategy", [N.PH_GATHER, N.PH_SCATTER, N.PH_SYMMETRIC])
+def test_batch_matches_single_calls(backend, precision, strategy):
+    ctx = backend.context
+    c6, cols = cloud(20000, 7 + strategy)
+    d = DeviceSet(ctx, *c6, cols, kernel=4)
+    rng = np.random.default_rng(100 + 10 * strategy + precision)
+    for cat in (FULL, SMALL, WIDE):
+        ca = N.CatalogArgs(**cat)
+        kinds = ["points", "plane", "box"] * 4 + ["points"] * 3
+        reqs, coords, bufs, outs = make_requests(ctx, rng, ca.L, precision, kinds)
+        b = PB.Batch(ctx, d.h, ca, strategy, precision, reqs)
+        try:
+            b.run()
+            for r, (P, (o, S)) in enumerate(zip(coords, outs)):
+                got = read_out(o, S, ca.L, precision)
+                ref = d.points(cat, strategy, *P, precision=precision)
+                assert got.shape == ref.shape
+                assert_same(got, ref)
+        finally:
+            b.close()
+            for x in bufs + [o for o, _ in outs]:
+                x.free()
+
+
+def test_batch_replays_and_times(backend):
+    """Graph replays with eager calls in between give identical outputs; per-
+    request times are ordered and empty requests report NaN."""
+    ctx = backend.context
+    c6, cols = cloud(15000, 3)
+    d = DeviceSet(ctx, *c6, cols, kernel=0)
+    rng = np.random.default_rng(4)
+    ca = N.CatalogArgs(**FULL)
+    reqs, coords, bufs, outs = make_requests(ctx, rng, ca.L, N.PH_PREC_FP32C,
+                                             ["plane", "points", "box", "points", "plane"])
+    empty = DevArray(ctx, nbytes=8)
+    reqs.append(PB.points(None, None, None, 0, empty.p))
+    b = PB.Batch(ctx, d.h, ca, N.PH_SYMMETRIC, N.PH_PREC_FP32C, reqs)
+    try:
+        first = None
+        for rep in range(4):
+            b.run()
+            res = [read_out(o, S, ca.L, N.PH_PREC_FP32C) for o, S in outs]
+            if first is None:
+                first = res
+            else:
+                for a, c in zip(first, res):
+                    assert_same(a, c)
+            # eager work between replays
+            d.points(FULL, N.PH_SYMMETRIC, *coords[0], precision=N.PH_PREC_FP32C)
+            t0, t1 = b.times()
+            S = [s for _, s in outs] + [0]
+            for r in range(len(reqs)):
+                if S[r] == 0:
+                    assert np.isnan(t0[r]) and np.isnan(t1[r])
+                else:
+                    assert 0.0 <= t0[r] <= t1[r] < 1e7
+    finally:
+        b.close()
+        for x in bufs + [o for o, _ in outs] + [empty]:
+            x.free()
+
+
+def test_batch_after_column_change(backend):
+    """Runs read the current columns: same-shape updates and a reallocation
+    (another column count, which re-records the graph)."""
+    ctx = backend.context
+    c6, cols = cloud(8000, 9)
+    d = DeviceSet(ctx, *c6, cols, kernel=4)
+    rng = np.random.default_rng(12)
+    cat = dict(sslots=[1, 2], sflags=[0, 1], gslots=[2])
+    ca = N.CatalogArgs(**cat)
+    reqs, coords, bufs, outs = make_requests(ctx, rng, ca.L, N.PH_PREC_FP64, ["points", "plane", "box"])
+    b = PB.Batch(ctx, d.h, ca, N.PH_GATHER, N.PH_PREC_FP64, reqs)
+    try:
+        order = d.get("order", np.uint64, d.n).astype(np.int64) - 1  # Morton position -> source index
+        for ncol in (5, 7):
+            new = [np.cos(3.0 * c6[0] + k) for k in range(ncol)]
+            m_s, rho_s = c6[3][order], c6[5][order]
+            cs = [c[order] for c in new]
+            arr = (C.c_void_p * ncol)(*[a.ctypes.data for a in cs])
+            ctx.check(N.lib().ph_lbvh_set_columns(d.h, N.ptr(m_s), N.ptr(rho_s), ncol, arr))
+            b.run()
+            for P, (o, S) in zip(coords, outs):
+                assert_same(read_out(o, S, ca.L, N.PH_PREC_FP64), d.points(cat, N.PH_GATHER, *P))
+    finally:
+        b.close()
+        for x in bufs + [o for o, _ in outs]:
+            x.free()
+
+
+def test_batch_errors(backend):
+    ctx = backend.context
+    c6, cols = cloud(500, 1)
+    d = DeviceSet(ctx, *c6, cols, kernel=0)
+    ca = N.CatalogArgs(**SMALL)
+    o = DevArray(ctx, nbytes=1 << 16)
+    pos, r, u, f = [0.5] * 3, [1.0, 0, 0], [0, 1.0, 0], [0, 0, 1.0]
+    with pytest.raises(ArgumentError, match="at least 2"):
+        PB.Batch(ctx, d.h, ca, N.PH_GATHER, N.PH_PREC_FP64, [PB.plane(pos, r, u, (0.0, 1.0, 1), (0.0, 1.0, 4), o.p)])
+    with pytest.raises(ArgumentError, match="greater than"):
+        PB.Batch(ctx, d.h, ca, N.PH_GATHER, N.PH_PREC_FP64,
+                 [PB.box(pos, r, u, f, (0.0, 1.0, 3), (0.0, 1.0, 4), (2.0, 2.0, 3), o.p)])
+    bad = PB.points(None, None, None, 0, o.p)
+    bad.kind = 7
+    with pytest.raises(ArgumentError, match="unknown kind"):
+        PB.Batch(ctx, d.h, ca, N.PH_GATHER, N.PH_PREC_FP64, [bad])
+    with pytest.raises(ArgumentError):
+        PB.Batch(ctx, d.h, ca, N.PH_GATHER, N.PH_PREC_FP32, [PB.plane(pos, r, u, (0.0, 1.0, 2), (0.0, 1.0, 2), o.p)])
+    b = PB.Batch(ctx, d.h, ca, N.PH_GATHER, N.PH_PREC_FP64, [PB.plane(pos, r, u, (0.0, 1.0, 2), (0.0, 1.0, 2), o.p)])
+    b.run()
+    d.close()  # the tree goes away first
+    with pytest.raises(ArgumentError, match="destroyed"):
+        b.run()
+    b.close()
+    o.free()

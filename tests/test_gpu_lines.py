@@ -5,12 +5,10 @@ traversal against the per-ray one.
 Both traversals test, for every ray, exactly the nodes and leaves of the
 reference's DFS (LinearBVHTraversalMacros.jl:539-644) in the same order, so all
 results are required to be bit-identical (NaN masks equal)."""
-import ctypes as C
-
 import numpy as np
 import pytest
 
-from gpu_util import DeviceSet, assert_same, oracle_of, traversal
+from gpu_util import DevArray as Dev, DeviceSet, assert_same, oracle_of, traversal
 from partia import _native as N
 from partia import synthetic
 
@@ -28,26 +26,6 @@ def cloud(n, seed, spread=1.0):
     rho = rng.random(n) + 0.3
     A = np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
     return (x, y, z, m, h, rho), [rho, A, x + 0.3 * y]
-
-
-class Dev:
-    def __init__(self, ctx, arr):
-        self.ctx = ctx
-        a = np.ascontiguousarray(arr)
-        self.p = C.c_void_p()
-        ctx.check(N.lib().ph_device_alloc(ctx.handle, max(a.nbytes, 8), C.byref(self.p)))
-        if a.nbytes:
-            ctx.check(N.lib().ph_memcpy_htod(ctx.handle, self.p, a.ctypes.data_as(C.c_void_p), a.nbytes))
-
-    def get(self, dtype, count):
-        out = np.zeros(count, dtype)
-        if count:
-            self.ctx.check(N.lib().ph_memcpy_dtoh(self.ctx.handle, out.ctypes.data_as(C.c_void_p), self.p,
-                                                  out.nbytes))
-        return out
-
-    def free(self):
-        N.lib().ph_device_free(self.ctx.handle, self.p)
 
 
 def line_grid(d, cat, frame, ax, ay, precision=64):
