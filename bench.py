@@ -26,6 +26,13 @@ Other workloads (`--workload`), each printing one line of the same form:
               Float32 LinearBVH! + the projection; value = rays/s.
   grid64      1e5 uniform particles (seed 1), M4, gather, Float64; density and
               A (Shepard) on a 64^3 grid over [0,1]^3.
+  adjoint1024 1e6 Plummer particles (the grid workload's recipe, seed 5),
+              Float32 column density on a 1024^2 image along z over [-1,1]^2,
+              L2 loss against the map of masses m exp(N(0, 0.1^2)); one step =
+              the forward projection + its adjoint d/d(m, A, h, x, y, z);
+              value = rays/s of the adjoint, with the backward/forward time
+              ratio, and a float64 central-difference check of the adjoint on
+              64 random particles (relative step 1e-4).
   batch512    4e6 particles (half uniform, half a Plummer clump, seed 4) and 512
               sampling requests of uneven size (point lists of 1e3-1e6 random
               points, oriented planes 256^2-1024^2, oriented boxes 32^3-128^3)
@@ -73,7 +80,8 @@ def parse():
     ap.add_argument("--traversal", choices=["auto", "lane", "bucket"], default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
-    ap.add_argument("--workload", choices=["grid256", "column2048", "grid64", "batch512"], default="grid256")
+    ap.add_argument("--workload", choices=["grid256", "column2048", "grid64", "batch512", "adjoint1024"],
+                    default="grid256")
     return ap.parse_args()
 
 
@@ -452,6 +460,139 @@ def batch512(args):
     print(json.dumps(result))
 
 
+def adjoint1024(args):
+    """Forward + adjoint of a 1024^2 Float32 column-density map of 1e6 Plummer particles."""
+    npart, nimg = 1_000_000, 1024
+    print("bench: generating %d Plummer particles" % npart, file=sys.stderr, flush=True)
+    data = synthetic.plummer_sphere(npart, seed=5)
+    st = synthetic.Stream(55)
+    m_target = data["m"] * np.exp(0.1 * st.normal(npart))
+    ax = synthetic.linrange(-1.0, 1.0, nimg)
+    S = nimg * nimg
+    ctx = N.Context(0)
+    L = N.lib()
+    cat = N.CatalogArgs(sslots=[1], sflags=[0])
+    frame = [np.ascontiguousarray(np.array(v, dtype=np.float64)) for v in
+             ([0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0])]
+    dax = Dev(ctx, 8 * nimg)
+    dax.upload(ax)
+
+    def build(cols_dict, prec):
+        dc = upload_columns(ctx, cols_dict, ["x", "y", "z", "m", "h", "rho", "rho"])
+        q = (C.c_void_p * 1)(dc[6].p.value)
+        h = C.c_void_p()
+        ctx.check(L.ph_lbvh_build_device(ctx.handle, npart, *[d.p for d in dc[:6]], 1, q, 0, 63, prec, C.byref(h)))
+        for d in dc:
+            d.free()
+        return h
+
+    def forward(h, prec, dout):
+        ctx.check(L.ph_line_grid_device(ctx.handle, h, cat.ref(), N.PH_SCATTER, prec, *[N.ptr(f) for f in frame],
+                                        dax.p, nimg, dax.p, nimg, dout.p))
+        return ctx.stats()["interp_ms"]
+
+    def backward(h, prec, dcot, dgrad):
+        ctx.check(L.ph_line_grid_adjoint_device(ctx.handle, h, cat.ref(), N.PH_SCATTER, prec,
+                                                *[N.ptr(f) for f in frame], dax.p, nimg, dax.p, nimg, dcot.p,
+                                                dgrad.p))
+        return ctx.stats()["interp_ms"]
+
+    def get(dev, dtype, count):
+        out = np.zeros(count, dtype)
+        ctx.check(L.ph_memcpy_dtoh(ctx.handle, out.ctypes.data_as(C.c_void_p), dev.p, out.nbytes))
+        return out
+
+    # Float32 forward, target, cotangent
+    h32 = build(data, N.PH_PREC_FP32)
+    hT = build(dict(data, m=m_target), N.PH_PREC_FP32)
+    dmap, dT = Dev(ctx, 4 * S), Dev(ctx, 4 * S)
+    forward(hT, N.PH_PREC_FP32, dT)
+    L.ph_lbvh_destroy(hT)
+    forward(h32, N.PH_PREC_FP32, dmap)
+    sigma, target = get(dmap, np.float32, S), get(dT, np.float32, S)
+    # rays without neighbours give NaN (as in the reference) in both maps: they
+    # are left out of the loss, i.e. their cotangent is 0
+    cot = np.nan_to_num(sigma - target, nan=0.0).astype(np.float32)
+    dcot = Dev(ctx, 4 * S)
+    dcot.upload(cot)
+    dgrad = Dev(ctx, 8 * 6 * npart)
+
+    def step():
+        f = forward(h32, N.PH_PREC_FP32, dmap)
+        b = backward(h32, N.PH_PREC_FP32, dcot, dgrad)
+        return {"forward": f, "backward": b}
+
+    ms, res = timed_steps(ctx, step, args.steps, args.warmup, "adjoint1024")
+    fwd = float(np.mean([r["forward"] for r in res]))
+    bwd = float(np.mean([r["backward"] for r in res]))
+    g32 = get(dgrad, np.float64, 6 * npart).reshape(6, npart)
+    loss = 0.5 * float(np.sum(cot.astype(np.float64) ** 2))
+
+    # float64 check: adjoint vs central differences of the float64 loss
+    h64 = build(data, N.PH_PREC_FP64)
+    hT64 = build(dict(data, m=m_target), N.PH_PREC_FP64)
+    dm64, dT64 = Dev(ctx, 8 * S), Dev(ctx, 8 * S)
+    forward(hT64, N.PH_PREC_FP64, dT64)
+    L.ph_lbvh_destroy(hT64)
+    forward(h64, N.PH_PREC_FP64, dm64)
+    T64 = get(dT64, np.float64, S)
+    cot64 = np.nan_to_num(get(dm64, np.float64, S) - T64, nan=0.0)
+    dcot64 = Dev(ctx, 8 * S)
+    dcot64.upload(cot64)
+    backward(h64, N.PH_PREC_FP64, dcot64, dgrad)
+    g64 = get(dgrad, np.float64, 6 * npart).reshape(6, npart)
+    L.ph_lbvh_destroy(h64)
+
+    def loss64(dct):
+        hh = build(dct, N.PH_PREC_FP64)
+        forward(hh, N.PH_PREC_FP64, dm64)
+        L.ph_lbvh_destroy(hh)
+        return 0.5 * float(np.nansum((get(dm64, np.float64, S) - T64) ** 2))
+
+    fd_err = {}
+    fd_checks = 0
+    if not args.no_fp64_compare:
+        rng = np.random.default_rng(64)
+        inside = np.nonzero((np.abs(data["x"]) < 0.9) & (np.abs(data["y"]) < 0.9))[0]
+        picks = rng.choice(inside, 64, replace=False)
+        for row, key in ((0, "m"), (2, "h"), (3, "x")):
+            scale = float(np.sqrt(np.mean(g64[row] ** 2)))
+            worst = 0.0
+            for b in picks:
+                eps = 1e-4 * (abs(data[key][b]) if key != "x" else data["h"][b])
+                dp, dmn = dict(data), dict(data)
+                dp[key] = data[key].copy()
+                dmn[key] = data[key].copy()
+                dp[key][b] += eps
+                dmn[key][b] -= eps
+                fd = (loss64(dp) - loss64(dmn)) / (2 * eps)
+                err = abs(fd - g64[row, b]) / scale
+                worst = max(worst, err) if np.isfinite(err) else float("inf")
+                fd_checks += 1
+            fd_err[key] = worst
+        print("bench: adjoint vs central differences (max |fd - adj| / rms adj): %s" % fd_err, file=sys.stderr,
+              flush=True)
+    rel32 = {k: float(np.linalg.norm(g32[r] - g64[r]) / max(np.linalg.norm(g64[r]), 1e-300))
+             for r, k in enumerate(("m", "A", "h", "x", "y", "z"))}
+    result = {
+        "metric": "GPU adjoint rays/s, Float32 column density 1024^2 (d/dm, dA, dh, dr); backward/forward time ratio",
+        "value": S / (bwd * 1e-3), "unit": "rays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Plummer sphere, seeded mass perturbation, SplitMix64)",
+        "config": {"workload": "plummer_1e6_M4_column_density_1024_adjoint", "particles": npart,
+                   "image": [nimg, nimg], "kernel": "M4_spline", "precision": "fp32"},
+        "forward_ms": fwd, "backward_ms": bwd, "backward_over_forward": bwd / fwd, "loss": loss,
+        "fd_check": {"particles": 64, "checks": fd_checks, "step": "1e-4 relative (positions: 1e-4 h)",
+                     "max_err_over_rms": fd_err},
+        "fp32_vs_fp64_adjoint_normwise": rel32,
+    }
+    L.ph_lbvh_destroy(h32)
+    for d in (dax, dmap, dT, dcot, dgrad, dm64, dT64, dcot64):
+        d.free()
+    ctx.close()
+    print(json.dumps(result))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -464,6 +605,8 @@ def main():
         return grid64(args)
     if args.workload == "batch512":
         return batch512(args)
+    if args.workload == "adjoint1024":
+        return adjoint1024(args)
     prec = N.PH_PREC_FP64 if args.precision == "fp64" else N.PH_PREC_FP32C
     print("bench: generating %d Plummer particles" % NPART, file=sys.stderr, flush=True)
     data = synthetic.plummer_sphere(NPART, seed=2)

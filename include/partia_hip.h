@@ -217,6 +217,30 @@ int ph_line_grid_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, in
                         const double* origin3, const double* u3, const double* v3, const double* dir3,
                         const double* ax_dev, int64_t nx, const double* ay_dev, int64_t ny, void* out_dev);
 
+/* ---- adjoints of the line projection (no reference counterpart; the reverse
+ * pass a ChainRulesCore rrule of LineSamples_interpolation would call).  For a
+ * catalog of exactly one scalar A (slot s, Shepard flag as in the forward),
+ * scatter strategy, with w_b = m_b I(q_b) / (h_b rho_b), q_b = r_b / h_b:
+ *   out = sum_b A_b w_b            (no Shepard flag, e.g. column density :rho)
+ *   out = sum_b A_b w_b / sum_b w_b (Shepard flag)
+ * grad_dev (6 x n float64, overwritten, rows d/dm, d/dA, d/dh, d/dx, d/dy, d/dz,
+ * columns in the input order of the particles, i.e. before LinearBVH!'s
+ * permutation) receives the gradient of sum_s cot[s] * out[s]; rho_b is held
+ * fixed and A_b is the value of column s.  I is the table's piecewise-linear
+ * interpolant (its derivative is the segment slope).  The neighbours are
+ * recomputed with the forward's traversal; terms are evaluated in the
+ * precision (PH_PREC_FP64 or PH_PREC_FP32 trees) and summed in float64 with
+ * atomics (sums are order-dependent in the last bits).  cot_dev holds S values
+ * of the forward's output type; grad_dev must be ordinary device memory. ---- */
+int ph_line_samples_adjoint_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy,
+                                   int32_t precision, int64_t S, const double* ox_dev, const double* oy_dev,
+                                   const double* oz_dev, const double* dx_dev, const double* dy_dev,
+                                   const double* dz_dev, const void* cot_dev, double* grad_dev);
+int ph_line_grid_adjoint_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy,
+                                int32_t precision, const double* origin3, const double* u3, const double* v3,
+                                const double* dir3, const double* ax_dev, int64_t nx, const double* ay_dev, int64_t ny,
+                                const void* cot_dev, double* grad_dev);
+
 /* ---- building blocks exposed for parity tests ---- */
 /* LBVH_find_nearest_h (LBVH/LinearBVH/LinearBVHQueries.jl:118-140) per point. */
 int ph_nearest_h(ph_context* ctx, ph_lbvh* bvh, int64_t S, const double* px_host, const double* py_host,

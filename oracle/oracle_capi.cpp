@@ -204,6 +204,30 @@ int oracle_line_samples(void* hv, int nS, const int* sslots, const unsigned char
     return 0;
 }
 
+// Adjoint of oracle_line_samples for catalog scalar `slot` (scatter): grad is
+// 6 x n float64 in input order (d/dm, d/dA, d/dh, d/dx, d/dy, d/dz), summed over
+// the rays with cotangents cot[S].  Serial (the rays share particles).
+int oracle_line_adjoint(void* hv, int slot, int shepard, int64_t S, const double* ox, const double* oy,
+                        const double* oz, const double* mx, const double* my, const double* mz, const double* cot,
+                        double* grad) {
+    Handle* H = static_cast<Handle*>(hv);
+    auto run = [&](auto& in, const auto& lut) {
+        using T = std::decay_t<decltype(in.x[0])>;
+        const int64_t n = (int64_t)in.x.size();
+        std::vector<T> g(6 * (size_t)n, T(0));
+        for (int64_t s = 0; s < S; ++s) {
+            T o[3] = {T(ox[s]), T(oy[s]), T(oz[s])};
+            T m[3] = {T(mx[s]), T(my[s]), T(mz[s])};
+            line_adjoint<T>(in, lut, slot, shepard != 0, o, m, T(cot[s]), g.data());
+        }
+        for (int c = 0; c < 6; ++c)
+            for (int64_t i = 0; i < n; ++i) grad[c * n + (int64_t)in.order[i] - 1] = double(g[c * n + i]);
+    };
+    if (H->precision == 64) run(H->d, *H->lut_d);
+    else run(H->f, *H->lut_f);
+    return 0;
+}
+
 // LBVH_find_nearest_h per point; best (1-based leaf index) optional.
 int oracle_nearest_h(void* hv, int64_t S, const double* px, const double* py, const double* pz, double* ha,
                      int64_t* best) {

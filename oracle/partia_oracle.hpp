@@ -839,4 +839,62 @@ void line_interpolate(const Input<T>& in, const LineLUT<T>& lut, const Catalog& 
     for (size_t j = 0; j < N; ++j) out[j * stride] = cat.sflags[j] ? acc[j] * invS1 : acc[j];
 }
 
+// Adjoint of line_interpolate for one catalog scalar, scatter strategy (no
+// reference counterpart; pinned by central finite differences of
+// line_interpolate).  With w_b = m_b I(q_b) / (h_b rho_b), q_b = r_b / h_b
+// (r_b the distance of particle b from the line) and out = sum A_b w_b, or
+// out = sum A_b w_b / sum w_b with the Shepard flag, it adds cot * d out / d theta_b
+// for theta_b = (m_b, A_b, h_b, x_b, y_b, z_b) into g[6][n] (Morton order); rho_b
+// is held fixed.  I is the piecewise-linear table interpolant, whose derivative
+// on the segment of q is (I_i - I_{i-1}) / dq; dr/dx_b = (D - (D.m) m) / r with
+// D = x_b - o (taken as 0 at r = 0).  A ray without neighbours (NaN output)
+// contributes nothing.
+template <class T>
+void line_adjoint(const Input<T>& in, const LineLUT<T>& lut, int slot, bool shepard, const T o[3], const T m[3],
+                  T cot, T* g /* 6 x n */) {
+    const int K = in.kernel;
+    const T Kvalid = kernel_support<T>(K);
+    const i64 n = (i64)in.x.size();
+    const std::vector<T>& A = in.quant[(size_t)slot - 1];
+    T S1 = T(0), N = T(0);
+    if (shepard) {
+        line_traversal<T>(in.bvh, o, m, SCATTER, Kvalid, T(0), [&](i64 k, T d2, T hb) {
+            const size_t i = (size_t)k - 1;
+            const T dr = std::sqrt(d2 < T(0) ? T(0) : d2);
+            const T w = in.m[i] * lut.W(K, dr, hb) / in.rho[i];
+            S1 += w;
+            N += A[i] * w;
+        });
+        if (S1 == T(0)) return;
+    }
+    const T out = shepard ? N / S1 : T(0);
+    line_traversal<T>(in.bvh, o, m, SCATTER, Kvalid, T(0), [&](i64 k, T d2, T hb) {
+        const size_t i = (size_t)k - 1;
+        const T dr = std::sqrt(d2 < T(0) ? T(0) : d2);
+        const T invh = T(1) / hb, q = dr * invh;
+        T I = T(0), dI = T(0);
+        if (!(q >= Kvalid)) {
+            I = lut.lookup(K, q);
+            T idxf = q / lut.dq[K] + T(1);
+            i32 j = std::min(std::max((i32)idxf, (i32)1), (i32)(4096 - 1));
+            dI = (lut.inorm[K][j] - lut.inorm[K][j - 1]) / lut.dq[K];
+        }
+        const T mb = in.m[i], rb = in.rho[i];
+        const T w = mb * (invh * I) / rb;
+        const T c1 = shepard ? (A[i] - out) / S1 : A[i];
+        const T c2 = shepard ? T(1) / S1 : T(1);
+        const T dw_dm = (invh * I) / rb;
+        const T dw_dh = -(mb / rb) * invh * invh * (I + q * dI);
+        g[0 * n + (i64)i] += cot * c1 * dw_dm;
+        g[1 * n + (i64)i] += cot * c2 * w;
+        g[2 * n + (i64)i] += cot * c1 * dw_dh;
+        if (dr > T(0)) {
+            const T D[3] = {in.x[i] - o[0], in.y[i] - o[1], in.z[i] - o[2]};
+            const T Dm = D[0] * m[0] + D[1] * m[1] + D[2] * m[2];
+            const T f = (mb / rb) * invh * invh * dI / dr;
+            for (int a = 0; a < 3; ++a) g[(3 + a) * n + (i64)i] += cot * c1 * f * (D[a] - Dm * m[a]);
+        }
+    });
+}
+
 }  // namespace oracle

@@ -39,6 +39,7 @@ def lib():
         L.oracle_point_samples.argtypes = [vp, i32, vp, vp, i32, vp, i32, vp, i32, vp, i32, i64, vp, vp, vp, vp, i32]
         L.oracle_line_samples.argtypes = [vp, i32, vp, vp, i32, i64] + [vp] * 7 + [vp, i32]
         L.oracle_nearest_h.argtypes = [vp, i64, vp, vp, vp, vp, vp]
+        L.oracle_line_adjoint.argtypes = [vp, i32, i32, i64] + [vp] * 6 + [vp, vp]
         L.oracle_neighbors.restype = i64
         L.oracle_neighbors.argtypes = [vp, i32, i32, vp, vp, d, vp, vp, i64]
         for name in ("oracle_kernel_f", "oracle_kernel_df"):
@@ -147,6 +148,17 @@ class OracleSet:
         lib().oracle_line_samples(self.h, len(ss), _p(ss), _p(sf), strategy, S, *[_p(a) for a in o + m],
                                   _p(hv) if hv is not None else None, _p(out), nthreads)
         return out
+
+    def line_adjoint(self, slot, shepard, origin, direction, cot):
+        """6 x n gradients (input order) of sum_s cot[s] * out_s for catalog scalar
+        `slot` along the lines (scatter)."""
+        o = [_f64(a) for a in origin]
+        m = [_f64(a) for a in direction]
+        c = _f64(cot)
+        S = len(o[0])
+        g = np.zeros((6, self.n))
+        lib().oracle_line_adjoint(self.h, int(slot), int(bool(shepard)), S, *[_p(a) for a in o + m], _p(c), _p(g))
+        return g
 
     def nearest_h(self, px, py, pz):
         px, py, pz = _f64(px), _f64(py), _f64(pz)

@@ -645,6 +645,67 @@ int ph_line_grid_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int3
     });
 }
 
+// Adjoint of a one-scalar line projection (shared by the list and image-plane forms).
+static void run_line_adjoint(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                             const LineList* ll, const LineGrid* lg, i64 S, const void* cot, double* grad) {
+    need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
+    check_line_catalog(B, cat, strategy);
+    need(cat->n_scalar == 1, PH_ERR_ARGUMENT, "the line adjoint takes a catalog of exactly one scalar");
+    check_precision(B, precision);
+    need(precision != PH_PREC_FP32C, PH_ERR_ARGUMENT, "the line adjoint runs in PH_PREC_FP64 or PH_PREC_FP32");
+    need(grad != nullptr && (S == 0 || cot != nullptr), PH_ERR_ARGUMENT, "null cotangent or gradient pointer");
+    const void* lut = precision == PH_PREC_FP64 ? ctx->lut64.p : ctx->lut32.p;
+    const i32 qsrc = 5 + cat->scalar_slots[0];
+    const bool shep = cat->scalar_shepard[0] != 0;
+    PH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    launch_lines_adjoint(B, qsrc, shep, precision, ll, lg, S, cot, lut, grad, ctx->stream, &ctx->ev[4]);
+    PH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    PH_HIP(hipEventSynchronize(ctx->ev[1]));
+    ctx->stats.interp_ms = elapsed(ctx, 0, 1);
+    ctx->stats.kernel_ms = elapsed(ctx, 4, 5);
+    ctx->stats.samples = S;
+    ctx->stats.accepted = -1;
+    ctx->stats.rechecks = -1;
+}
+
+int ph_line_samples_adjoint_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy,
+                                   int32_t precision, int64_t S, const double* ox, const double* oy, const double* oz,
+                                   const double* dx, const double* dy, const double* dz, const void* cot,
+                                   double* grad) {
+    return guarded(ctx, [&] {
+        need(S >= 0, PH_ERR_ARGUMENT, "negative sample count");
+        need(S == 0 || (ox && oy && oz && dx && dy && dz), PH_ERR_ARGUMENT, "null line pointer");
+        LineList ll{ox, oy, oz, dx, dy, dz};
+        run_line_adjoint(ctx, B, cat, strategy, precision, &ll, nullptr, S, cot, grad);
+        return PH_OK;
+    });
+}
+
+int ph_line_grid_adjoint_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy,
+                                int32_t precision, const double* origin3, const double* u3, const double* v3,
+                                const double* dir3, const double* ax, int64_t nx, const double* ay, int64_t ny,
+                                const void* cot, double* grad) {
+    return guarded(ctx, [&] {
+        need(nx >= 0 && ny >= 0, PH_ERR_ARGUMENT, "negative image size");
+        need(origin3 && u3 && v3 && dir3, PH_ERR_ARGUMENT, "null frame vector");
+        const i64 S = nx * ny;
+        need(S == 0 || (ax && ay), PH_ERR_ARGUMENT, "null axis pointer");
+        LineGrid lg{};
+        for (int d = 0; d < 3; ++d) {
+            lg.o[d] = origin3[d];
+            lg.u[d] = u3[d];
+            lg.v[d] = v3[d];
+            lg.d[d] = dir3[d];
+        }
+        lg.ax = ax;
+        lg.ay = ay;
+        lg.nx = nx;
+        lg.ny = ny;
+        run_line_adjoint(ctx, B, cat, strategy, precision, nullptr, &lg, S, cot, grad);
+        return PH_OK;
+    });
+}
+
 int ph_nearest_h(ph_context* ctx, ph_lbvh* B, int64_t S, const double* px, const double* py, const double* pz,
                  double* ha) {
     return guarded(ctx, [&] {

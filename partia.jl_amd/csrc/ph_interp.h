@@ -67,6 +67,13 @@ void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, c
 // Builds the bucket tree of B if it is not built yet.
 void ensure_buckets(ph_lbvh* B, hipStream_t s);
 
+// Adjoint of the line projection of one scalar (cols[qsrc]): grad[6][n] float64
+// (d/dm, d/dA, d/dh, d/dx, d/dy, d/dz, input order) of sum cot[s] out[s];
+// cot and lut of the precision's element type.  Zeroes grad first.
+void launch_lines_adjoint(ph_lbvh* B, i32 qsrc, bool shepard, int precision, const LineList* ll, const LineGrid* lg,
+                          i64 S, const void* cot, const void* lut, double* grad, hipStream_t s,
+                          hipEvent_t* ev = nullptr);
+
 // Batched sampling (ph_batch.hip).  batch_build validates and copies the
 // requests, packs nothing yet and records the launch graph; throws Error.
 BatchState* batch_build(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int strategy, int precision, int nreq,

@@ -231,10 +231,46 @@ __global__ void __launch_bounds__(kBlock) k_lines(TreeView<T> tv, const T* __res
 // replays its own traversal inside the union: a lane that prunes node u goes
 // idle until the walk reaches escape(u), which the union walk always reaches
 // right after u's subtree, so every lane tests exactly the nodes and leaves of
-// its own DFS, in the same order, with the same arithmetic as k_lines.  The
-// results are therefore identical to k_lines, bit for bit.  The steps are
-// branch-free within the wave (idle lanes compute and discard), and the
-// line-integrated table is staged in LDS.
+// its own DFS (LinearBVHTraversalMacros.jl:539-644), in the same order, with the
+// same arithmetic as k_lines.  leaf(k, take, d2, L) is called by the whole wave
+// for every leaf of the union walk; take marks the lanes that accept it.
+template <class T, class Leaf>
+__device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, const Line<T>& l, bool valid,
+                                                         T Kvalid, Leaf&& leaf) {
+    const i64 n = tv.n;
+    auto visit = [&](i64 k, bool active) {
+        const LeafRec<T> L = tv.leaves[k];
+        const T d2 = d2_line_leaf(l, L);
+        leaf(k, active && d2 <= radius2<T, PH_SCATTER>(Kvalid, T(0), L.h), d2, L);
+    };
+    if (tv.root == 0) {  // n == 1: the reference's brute-force branch
+        for (i64 k = 0; k < n; ++k) visit(k, valid);
+        return;
+    }
+    bool active = valid;
+    i32 resume = -1;  // node at which an idle lane rejoins (0: never)
+    i32 node = tv.root;
+    for (i64 steps = 2 * n + 8; node != 0 && steps > 0; --steps) {
+        node = __builtin_amdgcn_readfirstlane(node);
+        active = active || node == resume;
+        if (node >= (i32)n) {
+            const i64 k = (i64)node - n;
+            visit(k, active);
+            node = tv.leaf_escape[k];
+        } else {
+            const NodeRec<T> N = tv.nodes[node - 1];
+            const bool prune = d2_line_aabb_lb(l, N) > radius2<T, PH_SCATTER>(Kvalid, T(0), N.scale);
+            resume = (active && prune) ? N.escape : resume;
+            const bool descend = active && !prune;
+            active = descend;
+            node = __ballot(descend) ? N.left : N.escape;
+        }
+    }
+}
+
+// Projection with the wave walk: results identical to k_lines, bit for bit.
+// The steps are branch-free within the wave (idle lanes compute and discard),
+// and the line-integrated table is staged in LDS.
 template <class T, int K, int SRC>
 __global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const T* __restrict__ cols, CatChunk c,
                                                        LineSampler<T, K, PH_SCATTER, SRC> smp,
@@ -243,11 +279,10 @@ __global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const T* 
     __shared__ T tab[PH_LUT_N_ENTRIES];
     for (int i = threadIdx.x; i < PH_LUT_N_ENTRIES; i += kBlock) tab[i] = inorm[i];
     __syncthreads();
-    i64 s;
-    Line<T> l;
+    i64 s = 0;
+    Line<T> l{};  // lanes past the end keep a zero line
     const bool valid = smp.get(s, l);
     if (__ballot(valid) == 0) return;  // wave-uniform; no barrier follows
-    const T Kvalid = T(support_of<K>());
     const i64 n = tv.n;
     const T* __restrict__ mcol = cols + 3 * n;
     const T* __restrict__ rcol = cols + 5 * n;
@@ -256,42 +291,107 @@ __global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const T* 
     for (int j = 0; j < kMaxS; ++j) qcol[j] = cols + (i64)c.rsrc[c.spos[j < c.ns ? j : 0]] * n;
     LineAcc<T, K> a;
     u64 accepted = 0;
-    auto leaf = [&](i64 k, bool active) {
-        const LeafRec<T> L = tv.leaves[k];
+    wave_walk(tv, l, valid, T(support_of<K>()), [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
         const T mb = mcol[k], rb = rcol[k];
         T qv[kMaxS];
 #pragma unroll
         for (int j = 0; j < kMaxS; ++j) qv[j] = j < c.ns ? qcol[j][k] : T(0);
-        const T d2 = d2_line_leaf(l, L);
-        const bool take = active && d2 <= radius2<T, PH_SCATTER>(Kvalid, T(0), L.h);
         accepted += take ? 1 : 0;
         a.add_sel(take, mb, rb, qv, c, d2, L.h, tab, dq);
-    };
-    if (tv.root == 0) {  // n == 1: the reference's brute-force branch
-        for (i64 k = 0; k < n; ++k) leaf(k, valid);
-    } else {
-        bool active = valid;
-        i32 resume = -1;  // node at which an idle lane rejoins (0: never)
-        i32 node = tv.root;
-        for (i64 steps = 2 * n + 8; node != 0 && steps > 0; --steps) {
-            node = __builtin_amdgcn_readfirstlane(node);
-            active = active || node == resume;
-            if (node >= (i32)n) {
-                const i64 k = (i64)node - n;
-                leaf(k, active);
-                node = tv.leaf_escape[k];
-            } else {
-                const NodeRec<T> N = tv.nodes[node - 1];
-                const bool prune = d2_line_aabb_lb(l, N) > radius2<T, PH_SCATTER>(Kvalid, T(0), N.scale);
-                resume = (active && prune) ? N.escape : resume;
-                const bool descend = active && !prune;
-                active = descend;
-                node = __ballot(descend) ? N.left : N.escape;
-            }
-        }
-    }
+    });
     if (valid) a.store(c, out, smp.S, s);
     count_add(counter, valid ? accepted : 0);
+}
+
+template <class T> __device__ inline T wave_sum(T v) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+// Adjoint of the projection of one catalog scalar A (no reference
+// counterpart): with w_b = m_b I(q_b) / (h_b rho_b), q_b = r_b / h_b and
+// out = sum A_b w_b (or, Shepard, sum A_b w_b / sum w_b), every ray adds
+// cot * d out / d theta_b for theta_b = (m_b, A_b, h_b, x_b, y_b, z_b) into
+// grad[6][n] at the particle's input index; rho_b is held fixed.  The
+// neighbours are recomputed with the forward's wave walk (a Shepard ray walks
+// twice: its sums first).  I is the table's piecewise-linear interpolant, with
+// derivative (I_i - I_{i-1}) / dq on the segment of q; dr/dx_b = (D - (D.m) m) / r
+// with D = x_b - o (0 at r = 0).  The terms are evaluated in T, summed over the
+// wave's rays in float64 and added with one float64 atomic per wave, particle
+// and parameter (so the float64 sums are order-dependent in the last bits).
+template <class T, int K, int SRC, bool SHEP>
+__global__ void __launch_bounds__(kBlock) k_lines_adjoint(TreeView<T> tv, const T* __restrict__ cols, i32 qsrc,
+                                                          LineSampler<T, K, PH_SCATTER, SRC> smp,
+                                                          const T* __restrict__ inorm, T dq,
+                                                          const T* __restrict__ cot, const u32* __restrict__ order,
+                                                          double* __restrict__ grad) {
+    __shared__ T tab[PH_LUT_N_ENTRIES];
+    for (int i = threadIdx.x; i < PH_LUT_N_ENTRIES; i += kBlock) tab[i] = inorm[i];
+    __syncthreads();
+    i64 s = 0;
+    Line<T> l{};  // lanes past the end keep a zero line
+    const bool valid = smp.get(s, l);
+    if (__ballot(valid) == 0) return;  // wave-uniform; no barrier follows
+    const T Kvalid = T(support_of<K>());
+    const i64 n = tv.n;
+    const T* __restrict__ mcol = cols + 3 * n;
+    const T* __restrict__ rcol = cols + 5 * n;
+    const T* __restrict__ acol = cols + (i64)qsrc * n;
+    T g = valid ? cot[s] : T(0);
+    T S1 = T(0), out = T(0);
+    if constexpr (SHEP) {  // the forward's sums, same arithmetic
+        T N = T(0);
+        wave_walk(tv, l, valid, Kvalid, [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
+            const T mb = mcol[k], rb = rcol[k], Ab = acol[k];
+            const T dr = sqrt(d2 < T(0) ? T(0) : d2);
+            const T t = mb * line_W_nb<T, K>(dr, L.h, tab, dq) / rb;
+            S1 += take ? t : T(0);
+            N += take ? Ab * t : T(0);
+        });
+        if (S1 == T(0)) g = T(0);  // NaN output: no gradient
+        out = S1 == T(0) ? T(0) : N * (T(1) / S1);
+    }
+    const T c2 = SHEP ? (S1 == T(0) ? T(0) : T(1) / S1) : T(1);
+    wave_walk(tv, l, valid, Kvalid, [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
+        if (__ballot(take && g != T(0)) == 0) return;  // wave-uniform
+        const T mb = mcol[k], rb = rcol[k], Ab = acol[k];
+        const T dr = sqrt(d2 < T(0) ? T(0) : d2);
+        const T invh = T(1) / L.h, q = dr * invh;
+        const T idxf = q / dq + T(1);
+        const T lo = T(1), hi = T(PH_LUT_N_ENTRIES - 1);
+        const T cf = idxf < lo ? lo : (idxf > hi ? hi : idxf);
+        i32 i = (i32)cf;
+        i = i < 1 ? 1 : (i > PH_LUT_N_ENTRIES - 1 ? PH_LUT_N_ENTRIES - 1 : i);
+        const T t = idxf - T(i);
+        const bool cut = q >= Kvalid;
+        const T I = cut ? T(0) : tab[i - 1] * (T(1) - t) + tab[i] * t;
+        const T dI = cut ? T(0) : (tab[i] - tab[i - 1]) / dq;
+        const T w = mb * (invh * I) / rb;
+        const T c1 = SHEP ? (Ab - out) * c2 : Ab;
+        const T v[3] = {g * c1 * ((invh * I) / rb), g * c2 * w, -(g * c1) * (mb / rb) * invh * invh * (I + q * dI)};
+        const T f = dr > T(0) ? g * c1 * (mb / rb) * invh * invh * dI / dr : T(0);
+        const T D[3] = {L.x - l.o[0], L.y - l.o[1], L.z - l.o[2]};
+        const T Dm = D[0] * l.m[0] + D[1] * l.m[1] + D[2] * l.m[2];
+        const u32 src = order[k];
+        // lanes not taking the leaf contribute exact zeros (selected, not
+        // multiplied: their terms may be inf or NaN)
+        double sum[6];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) sum[j] = wave_sum(take ? (double)v[j] : 0.0);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) sum[3 + a] = wave_sum(take ? (double)(f * (D[a] - Dm * l.m[a])) : 0.0);
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+            for (int j = 0; j < 6; ++j)
+                if (sum[j] != 0.0) unsafeAtomicAdd(&grad[(i64)j * n + src], sum[j]);
+        }
+    });
+}
+
+// grad[0 .. count) = 0 (a kernel, not a memset, so the call can be recorded).
+__global__ void k_zero_f64(double* __restrict__ p, i64 count) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (i64)gridDim.x * blockDim.x) p[i] = 0.0;
 }
 
 template <class T>
@@ -421,7 +521,60 @@ void launch_lines_p(ph_lbvh* B, const CatChunk& c, const LineList* ll, const Lin
     }
 }
 
+template <class T, int K, bool SHEP>
+void launch_adjoint_k(ph_lbvh* B, i32 qsrc, const LineList* ll, const LineGrid* lg, i64 S, const T* cot,
+                      const T* inorm, double* grad, hipStream_t s, hipEvent_t* ev) {
+    const TreeView<T> tv = view_of<T>(B);
+    const T* cols = B->cols.as<T>();
+    const T dq = T((1.0 * support_of<K>()) / 4095.0) - T(0.0);
+    const T* tab = inorm + (size_t)K * PH_LUT_N_ENTRIES;
+    const u32* order = B->order.as<u32>();
+    k_zero_f64<<<grid_for(6 * B->n), kBlock, 0, s>>>(grad, 6 * B->n);
+    PH_HIP(hipGetLastError());
+    if (ev) PH_HIP(hipEventRecord(ev[0], s));
+    if (lg) {
+        LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S};
+        const i64 tiles = ((lg->nx + 7) / 8) * ((lg->ny + 7) / 8);
+        k_lines_adjoint<T, K, 1, SHEP><<<(unsigned)((tiles + 3) / 4), kBlock, 0, s>>>(tv, cols, qsrc, smp, tab, dq,
+                                                                                      cot, order, grad);
+    } else {
+        LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S};
+        k_lines_adjoint<T, K, 0, SHEP><<<(unsigned)((S + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+            tv, cols, qsrc, smp, tab, dq, cot, order, grad);
+    }
+    PH_HIP(hipGetLastError());
+    if (ev) PH_HIP(hipEventRecord(ev[1], s));
+}
+
+template <class T, bool SHEP>
+void launch_adjoint_p(ph_lbvh* B, i32 qsrc, const LineList* ll, const LineGrid* lg, i64 S, const T* cot,
+                      const T* inorm, double* grad, hipStream_t s, hipEvent_t* ev) {
+    switch (B->kernel) {
+    case PH_KERNEL_M4: launch_adjoint_k<T, PH_KERNEL_M4, SHEP>(B, qsrc, ll, lg, S, cot, inorm, grad, s, ev); break;
+    case PH_KERNEL_M5: launch_adjoint_k<T, PH_KERNEL_M5, SHEP>(B, qsrc, ll, lg, S, cot, inorm, grad, s, ev); break;
+    case PH_KERNEL_M6: launch_adjoint_k<T, PH_KERNEL_M6, SHEP>(B, qsrc, ll, lg, S, cot, inorm, grad, s, ev); break;
+    case PH_KERNEL_C2: launch_adjoint_k<T, PH_KERNEL_C2, SHEP>(B, qsrc, ll, lg, S, cot, inorm, grad, s, ev); break;
+    case PH_KERNEL_C4: launch_adjoint_k<T, PH_KERNEL_C4, SHEP>(B, qsrc, ll, lg, S, cot, inorm, grad, s, ev); break;
+    default: launch_adjoint_k<T, PH_KERNEL_C6, SHEP>(B, qsrc, ll, lg, S, cot, inorm, grad, s, ev); break;
+    }
+}
+
 }  // namespace
+
+void launch_lines_adjoint(ph_lbvh* B, i32 qsrc, bool shepard, int precision, const LineList* ll, const LineGrid* lg,
+                          i64 S, const void* cot, const void* lut, double* grad, hipStream_t s, hipEvent_t* ev) {
+    if (precision == PH_PREC_FP64) {
+        auto c = static_cast<const double*>(cot);
+        auto t = static_cast<const double*>(lut);
+        if (shepard) launch_adjoint_p<double, true>(B, qsrc, ll, lg, S, c, t, grad, s, ev);
+        else launch_adjoint_p<double, false>(B, qsrc, ll, lg, S, c, t, grad, s, ev);
+    } else {
+        auto c = static_cast<const float*>(cot);
+        auto t = static_cast<const float*>(lut);
+        if (shepard) launch_adjoint_p<float, true>(B, qsrc, ll, lg, S, c, t, grad, s, ev);
+        else launch_adjoint_p<float, false>(B, qsrc, ll, lg, S, c, t, grad, s, ev);
+    }
+}
 
 void launch_points(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl,
                    const GridAxes* ga, i64 S, int L, void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {

@@ -34,6 +34,7 @@ EXPORTS = (
     "ph_point_samples", "ph_point_samples_device", "ph_structured_grid", "ph_structured_grid_device",
     "ph_line_samples", "ph_line_samples_device", "ph_line_grid_device", "ph_nearest_h", "ph_line_table",
     "ph_batch_create", "ph_batch_run", "ph_batch_times", "ph_batch_destroy",
+    "ph_line_samples_adjoint_device", "ph_line_grid_adjoint_device",
 )
 
 PH_REQ_POINTS, PH_REQ_PLANE, PH_REQ_BOX = 0, 1, 2
@@ -114,6 +115,9 @@ def lib():
         "ph_batch_run": (i32, [vp]),
         "ph_batch_times": (i32, [vp, vp, vp]),
         "ph_batch_destroy": (i32, [vp]),
+        "ph_line_samples_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, i64] + [vp] * 8),
+        "ph_line_grid_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, vp, vp, vp, vp, i64, vp, i64,
+                                              vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -1,0 +1,83 @@
+"""The oracle's line-integrated adjoint (d out / d(m, A, h, x, y, z)) against
+central finite differences of the oracle's own forward projection, which the
+other oracle tests pin to the reference.  The table interpolant is piecewise
+linear in q, so a difference step is taken small (1e-6 relative) and the
+comparison is normalised per parameter by the size of the gradient."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+KERNELS = {"M4": 0, "C4": 4, "M6": 2}
+
+
+def cloud(n, seed):
+    rng = np.random.default_rng(seed)
+    x, y, z = rng.random(n), rng.random(n), rng.random(n)
+    m = rng.random(n) + 0.2
+    h = (rng.random(n) * 0.1 + 0.08) * (100.0 / n) ** (1 / 3)
+    rho = rng.random(n) + 0.5
+    A = 1.0 + 0.5 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
+    return [x, y, z, m, h, rho, A]
+
+
+def rays(S, seed):
+    rng = np.random.default_rng(seed)
+    org = rng.random((3, S)) * 0.8 + 0.1
+    dirs = rng.normal(size=(3, S))
+    dirs /= np.linalg.norm(dirs, axis=0)
+    dirs[:, : S // 2] = np.array([[0.0], [0.0], [1.0]])  # image-plane rays along z
+    return org, dirs
+
+
+def forward(p, kernel, shepard, org, dirs, cot):
+    x, y, z, m, h, rho, A = p
+    o = O.OracleSet(x, y, z, m, h, rho, [rho, A], kernel=kernel)
+    out = o.line_samples(dict(sslots=[2], sflags=[int(shepard)]), O.SCATTER, org, dirs)[0]
+    out = np.where(np.isnan(out), 0.0, out)
+    return float(np.dot(cot, out))
+
+
+@pytest.mark.parametrize("name", list(KERNELS))
+@pytest.mark.parametrize("shepard", [False, True])
+def test_line_adjoint_matches_finite_differences(name, shepard):
+    kernel = KERNELS[name]
+    p = cloud(300, 5 + kernel)
+    n = len(p[0])
+    org, dirs = rays(160, 9)
+    cot = np.random.default_rng(3).normal(size=org.shape[1])
+    o = O.OracleSet(*p[:6], [p[5], p[6]], kernel=kernel)
+    g = o.line_adjoint(2, shepard, org, dirs, cot)  # rows m, A, h, x, y, z
+    assert g.shape == (6, n)
+    rng = np.random.default_rng(11)
+    picks = rng.choice(n, 24, replace=False)
+    # (row of g, index into p, step)
+    params = [(0, 3, "rel"), (1, 6, "rel"), (2, 4, "rel"), (3, 0, "h"), (4, 1, "h"), (5, 2, "h")]
+    for row, pi, kind in params:
+        scale = max(np.sqrt(np.mean(g[row] ** 2)), 1e-300)
+        for b in picks:
+            base = p[pi][b]
+            eps = 1e-6 * (abs(base) if kind == "rel" else p[4][b])
+            q_plus = [a.copy() for a in p]
+            q_minus = [a.copy() for a in p]
+            q_plus[pi][b] = base + eps
+            q_minus[pi][b] = base - eps
+            fd = (forward(q_plus, kernel, shepard, org, dirs, cot) - forward(q_minus, kernel, shepard, org, dirs,
+                                                                            cot)) / (2 * eps)
+            assert abs(fd - g[row, b]) <= 2e-4 * scale + 1e-6 * abs(g[row, b]), (row, b, fd, g[row, b], scale)
+
+
+def test_line_adjoint_is_linear_in_the_cotangent_and_local():
+    p = cloud(200, 1)
+    org, dirs = rays(40, 2)
+    o = O.OracleSet(*p[:6], [p[5], p[6]], kernel=0)
+    rng = np.random.default_rng(0)
+    c1, c2 = rng.normal(size=40), rng.normal(size=40)
+    g1 = o.line_adjoint(2, False, org, dirs, c1)
+    g2 = o.line_adjoint(2, False, org, dirs, c2)
+    g12 = o.line_adjoint(2, False, org, dirs, 2.0 * c1 - 3.0 * c2)
+    np.testing.assert_allclose(g12, 2.0 * g1 - 3.0 * g2, rtol=1e-12, atol=1e-12 * np.abs(g12).max())
+    # particles no ray reaches get exact zeros
+    far = o.line_adjoint(2, False, [np.full(1, 5.0), np.full(1, 5.0), np.zeros(1)],
+                         [np.zeros(1), np.zeros(1), np.ones(1)], np.ones(1))
+    assert not far.any()
