@@ -78,6 +78,8 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "fp32c"], default="fp32c")
     ap.add_argument("--no-fp64-compare", action="store_true", help="skip timing the float64 path beside FP32C")
     ap.add_argument("--traversal", choices=["auto", "lane", "bucket"], default="auto")
+    ap.add_argument("--autotune", choices=["on", "off"], default="on",
+                    help="tune the launch configuration on the first (untimed) call of each workload key")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
     ap.add_argument("--workload", choices=["grid256", "column2048", "grid64", "batch512", "adjoint1024"],
@@ -616,6 +618,7 @@ def main():
     ctx = N.Context(0)
     ctx.set_traversal({"auto": N.PH_TRAVERSAL_AUTO, "lane": N.PH_TRAVERSAL_LANE,
                        "bucket": N.PH_TRAVERSAL_BUCKET}[args.traversal])
+    ctx.set_autotune(args.autotune == "on")
     L = N.lib()
     names = ["x", "y", "z", "m", "h", "rho", "rho", "A", "vx", "vy", "vz"]
     dcols = []
@@ -717,6 +720,8 @@ def main():
         "accepted_neighbours": int(n_acc),
         "guard_band_rechecks": int(n_rechecks),
         "traversal": args.traversal,
+        "autotune": [{k: e[k] for k in ("occupancy", "xcd_map", "best_ms", "default_ms")}
+                     for e in ctx.autotune_report()],
         "fp64": fp64,
         "speedup_vs_fp64": (fp64["ms_per_step"] / ms_per_step) if fp64 else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -90,6 +90,64 @@ void check_precision(const ph_lbvh* B, int precision) {
         need(B->precision == PH_PREC_FP64, PH_ERR_ARGUMENT, "float64 evaluation needs a tree built from Float64 columns");
 }
 
+// Sets the thread's launch configuration for the duration of a scope.
+struct LaunchScope {
+    LaunchCfg saved;
+    explicit LaunchScope(const LaunchCfg& c) : saved(tl_launch) { tl_launch = c; }
+    ~LaunchScope() { tl_launch = saved; }
+};
+
+// Workload key of the autotuner: precision, strategy, kernel, catalog shape,
+// sample source and the sizes' binary orders of magnitude.
+uint64_t tune_key(const ph_lbvh* B, const CatChunk& c, int strategy, int precision, bool grid, i64 S) {
+    const uint64_t shape = (c.ng + c.nd + c.nc == 0 && c.ns <= 2) ? 0
+                         : (c.ns == 2 && c.ng == 1 && c.nd == 1 && c.nc == 1) ? 1 : 2;
+    auto lg = [](i64 v) {
+        uint64_t b = 0;
+        while (v > 1) {
+            v >>= 1;
+            ++b;
+        }
+        return b;
+    };
+    return (uint64_t)(precision == PH_PREC_FP64 ? 0 : 1) | (uint64_t)strategy << 2 | (uint64_t)B->kernel << 4 |
+           shape << 8 | (uint64_t)(grid ? 1 : 0) << 10 | lg(S) << 12 | lg(B->n) << 20;
+}
+
+// Times every candidate configuration on the call itself (outputs are the same
+// for all of them) and keeps the fastest per workload key.
+LaunchCfg autotuned(ph_context* ctx, ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl,
+                    const GridAxes* ga, i64 S, void* out) {
+    const uint64_t key = tune_key(B, c, strategy, precision, ga != nullptr, S);
+    auto it = ctx->tuned.find(key);
+    if (it != ctx->tuned.end()) return it->second.cfg;
+    std::vector<LaunchCfg> cand;
+    for (float occ : {1.f, 0.75f, 0.5f})
+        for (int xm : {1, 0}) cand.push_back(LaunchCfg{occ, xm});
+    std::vector<double> best(cand.size(), 1e300);
+    hipEvent_t* ev = &ctx->ev[2];
+    auto run = [&](const LaunchCfg& cf) {
+        LaunchScope scope(cf);
+        launch_bucket(B, c, strategy, precision, pl, ga, S, out, nullptr, ctx->stream, ev);
+        PH_HIP(hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        PH_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        return (double)ms;
+    };
+    run(cand[0]);  // warm-up: records, caches
+    for (int rep = 0; rep < 2; ++rep)
+        for (size_t i = 0; i < cand.size(); ++i) best[i] = std::min(best[i], run(cand[i]));
+    size_t w = 0;
+    for (size_t i = 1; i < cand.size(); ++i)
+        if (best[i] < best[w]) w = i;
+    ph_context::Tuned t;
+    t.cfg = cand[w];
+    t.best_ms = best[w];
+    t.default_ms = best[0];
+    ctx->tuned[key] = t;
+    return t.cfg;
+}
+
 // Point / grid launches: the bucket traversal for FP32C and (by default) float64,
 // the reference-structured per-leaf traversal on request and for Float32 trees.
 void launch_point_like(ph_context* ctx, ph_lbvh* B, const CatChunk& c, int strategy, int precision,
@@ -97,8 +155,14 @@ void launch_point_like(ph_context* ctx, ph_lbvh* B, const CatChunk& c, int strat
                        hipEvent_t* ev) {
     bool bucket = precision == PH_PREC_FP32C;
     if (precision == PH_PREC_FP64) bucket = ctx->traversal != PH_TRAVERSAL_LANE;
-    if (bucket) launch_bucket(B, c, strategy, precision, pl, ga, S, out, counter, ctx->stream, ev);
-    else launch_points(B, c, strategy, precision, pl, ga, S, L, out, counter, ctx->stream, ev);
+    if (bucket) {
+        LaunchCfg cfg = ctx->launch;
+        if (ctx->autotune && S > 0) cfg = autotuned(ctx, B, c, strategy, precision, pl, ga, S, out);
+        LaunchScope scope(cfg);
+        launch_bucket(B, c, strategy, precision, pl, ga, S, out, counter, ctx->stream, ev);
+    } else {
+        launch_points(B, c, strategy, precision, pl, ga, S, L, out, counter, ctx->stream, ev);
+    }
 }
 
 size_t out_elem(int precision) { return precision == PH_PREC_FP64 ? sizeof(double) : sizeof(float); }
@@ -243,6 +307,39 @@ int ph_set_traversal(ph_context* ctx, int mode) {
     if (mode < PH_TRAVERSAL_AUTO || mode > PH_TRAVERSAL_WAVE)
         return fail(ctx, PH_ERR_ARGUMENT, "unknown traversal mode");
     ctx->traversal = mode;
+    return PH_OK;
+}
+
+int ph_set_autotune(ph_context* ctx, int enable) {
+    if (!ctx) return PH_ERR_ARGUMENT;
+    ctx->autotune = enable ? 1 : 0;
+    return PH_OK;
+}
+
+int ph_set_launch(ph_context* ctx, double occupancy, int32_t xcd_map) {
+    if (!ctx) return PH_ERR_ARGUMENT;
+    if (!(occupancy > 0.0 && occupancy <= 1.0) || (xcd_map != 0 && xcd_map != 1))
+        return fail(ctx, PH_ERR_ARGUMENT, "occupancy must be in (0, 1] and xcd_map 0 or 1");
+    ctx->launch = LaunchCfg{(float)occupancy, xcd_map};
+    return PH_OK;
+}
+
+int ph_autotune_report(ph_context* ctx, int32_t capacity, ph_tune_entry* out, int32_t* count) {
+    if (!ctx || capacity < 0 || (capacity > 0 && !out)) return PH_ERR_ARGUMENT;
+    int32_t k = 0;
+    for (const auto& kv : ctx->tuned) {
+        if (k < capacity) {
+            ph_tune_entry& e = out[k];
+            e.key = kv.first;
+            e.occupancy = kv.second.cfg.occupancy;
+            e.xcd_map = kv.second.cfg.xcd_map;
+            e.reserved = 0;
+            e.best_ms = kv.second.best_ms;
+            e.default_ms = kv.second.default_ms;
+        }
+        ++k;
+    }
+    if (count) *count = k;
     return PH_OK;
 }
 

@@ -2,6 +2,8 @@
 // in the FP32-compensated precision: the bucket-tree build and the per-call
 // kernels and launchers.  The traversal, the accumulators and the exactness
 // argument are in ph_bucket_dev.h.
+#include <algorithm>
+
 #include "ph_bucket_dev.h"
 
 namespace ph {
@@ -148,20 +150,45 @@ inline unsigned sample_blocks(const GridAxes* ga, i64 S) {
     return (unsigned)((S + kBlock - 1) / kBlock);
 }
 
+// Dynamic LDS per workgroup that limits a kernel to `occupancy` of its default
+// workgroups per CU (0 when occupancy >= 1 or the limit needs more LDS than a
+// workgroup may hold).
+template <class F> size_t occupancy_pad(F kern, float occupancy) {
+    if (occupancy >= 1.f) return 0;
+    int per_cu = 0, dev = 0, lds_cu = 0, lds_block = 0;
+    PH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
+    PH_HIP(hipGetDevice(&dev));
+    PH_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    PH_HIP(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const int want = std::max(1, (int)(per_cu * occupancy + 0.5f));
+    if (want >= per_cu || lds_cu <= 0) return 0;
+    hipFuncAttributes fa{};
+    PH_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kern)));
+    const i64 per_block = (i64)(lds_cu / want / 1024) * 1024;  // `want` fit, one more does not
+    const i64 pad = per_block - (i64)fa.sharedSizeBytes;
+    if (pad <= 0 || per_block > lds_block) return 0;
+    return (size_t)pad;
+}
+
 template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT>
 void launch_c32_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const GridAxes* ga, i64 S, const double* ha,
                   void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
     const BucketView bv = bucket_view(B);
     const unsigned g = sample_blocks(ga, S);
-    if (ev) PH_HIP(hipEventRecord(ev[0], s));
-    if (ga)
-        k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 1><<<g, kBlock, 0, s>>>(
-            bv, B->rec32.as<double>(), c, Sampler<double, 1>{*ga, S}, ha,
-            static_cast<float*>(out), counters);
-    else
-        k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 0><<<g, kBlock, 0, s>>>(
-            bv, B->rec32.as<double>(), c, Sampler<double, 0>{*pl, S}, ha,
-            static_cast<float*>(out), counters);
+    const LaunchCfg cfg = tl_launch;
+    if (ga) {
+        auto kern = k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 1>;
+        const size_t pad = occupancy_pad(kern, cfg.occupancy);
+        if (ev) PH_HIP(hipEventRecord(ev[0], s));
+        kern<<<g, kBlock, pad, s>>>(bv, B->rec32.as<double>(), c, Sampler<double, 1>{*ga, S, cfg.xcd_map}, ha,
+                                    static_cast<float*>(out), counters);
+    } else {
+        auto kern = k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 0>;
+        const size_t pad = occupancy_pad(kern, cfg.occupancy);
+        if (ev) PH_HIP(hipEventRecord(ev[0], s));
+        kern<<<g, kBlock, pad, s>>>(bv, B->rec32.as<double>(), c, Sampler<double, 0>{*pl, S, cfg.xcd_map}, ha,
+                                    static_cast<float*>(out), counters);
+    }
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[1], s));
 }
@@ -171,15 +198,20 @@ void launch_f64_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const Grid
                   void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
     const BucketView bv = bucket_view(B);
     const unsigned g = sample_blocks(ga, S);
-    if (ev) PH_HIP(hipEventRecord(ev[0], s));
-    if (ga)
-        k_bucket_f64<K, STRAT, MS, MG, MD, MC, 1><<<g, kBlock, 0, s>>>(bv, B->rec.as<double>(), c,
-                                                                      Sampler<double, 1>{*ga, S}, ha,
-                                                                      static_cast<double*>(out), counters);
-    else
-        k_bucket_f64<K, STRAT, MS, MG, MD, MC, 0><<<g, kBlock, 0, s>>>(bv, B->rec.as<double>(), c,
-                                                                      Sampler<double, 0>{*pl, S}, ha,
-                                                                      static_cast<double*>(out), counters);
+    const LaunchCfg cfg = tl_launch;
+    if (ga) {
+        auto kern = k_bucket_f64<K, STRAT, MS, MG, MD, MC, 1>;
+        const size_t pad = occupancy_pad(kern, cfg.occupancy);
+        if (ev) PH_HIP(hipEventRecord(ev[0], s));
+        kern<<<g, kBlock, pad, s>>>(bv, B->rec.as<double>(), c, Sampler<double, 1>{*ga, S, cfg.xcd_map}, ha,
+                                    static_cast<double*>(out), counters);
+    } else {
+        auto kern = k_bucket_f64<K, STRAT, MS, MG, MD, MC, 0>;
+        const size_t pad = occupancy_pad(kern, cfg.occupancy);
+        if (ev) PH_HIP(hipEventRecord(ev[0], s));
+        kern<<<g, kBlock, pad, s>>>(bv, B->rec.as<double>(), c, Sampler<double, 0>{*pl, S, cfg.xcd_map}, ha,
+                                    static_cast<double*>(out), counters);
+    }
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[1], s));
 }
@@ -206,8 +238,9 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
     if (STRAT != PH_SCATTER) {
         const BucketView bv = bucket_view(B);
         const unsigned g = sample_blocks(ga, S);
-        if (ga) k_nearest_bucket<1><<<g, kBlock, 0, s>>>(bv, Sampler<double, 1>{*ga, S}, B->ha.as<double>());
-        else k_nearest_bucket<0><<<g, kBlock, 0, s>>>(bv, Sampler<double, 0>{*pl, S}, B->ha.as<double>());
+        const int xm = tl_launch.xcd_map;
+        if (ga) k_nearest_bucket<1><<<g, kBlock, 0, s>>>(bv, Sampler<double, 1>{*ga, S, xm}, B->ha.as<double>());
+        else k_nearest_bucket<0><<<g, kBlock, 0, s>>>(bv, Sampler<double, 0>{*pl, S, xm}, B->ha.as<double>());
         PH_HIP(hipGetLastError());
         ha = B->ha.as<double>();
     }

@@ -87,6 +87,19 @@ struct BlockCache {
     }
 };
 
+// Launch configuration of the point / grid kernels (the autotuner's knobs).
+// Neither knob changes any sample's arithmetic, so results are identical for
+// every configuration:
+//   occupancy  fraction of the default workgroups per CU (1, 0.75, 0.5 ...),
+//              enforced with unused dynamic LDS per workgroup;
+//   xcd_map    1: consecutive tiles go to one XCD (shared L2), 0: hardware order.
+struct LaunchCfg {
+    float occupancy = 1.f;
+    int xcd_map = 1;
+};
+// Configuration of the entry point running on this thread (set by the entry points).
+inline thread_local LaunchCfg tl_launch{};
+
 // Cache of the context whose entry point runs on this thread (set by the entry
 // points), or null: then buffers use hipMalloc / hipFree directly.
 inline thread_local BlockCache* tl_cache = nullptr;
@@ -171,6 +184,15 @@ struct ph_context {
     int num_cu = 256;
     std::vector<struct ph_lbvh*> trees;  // live trees; released with the context
     std::vector<struct ph_batch*> batches;  // live batches; their device state is released with the context
+    // launch configuration of point / grid calls: explicit, or chosen per
+    // workload key by the autotuner (timed on the call's own data)
+    int autotune = 0;
+    ph::LaunchCfg launch{};
+    struct Tuned {
+        ph::LaunchCfg cfg;
+        double best_ms = 0.0, default_ms = 0.0;
+    };
+    std::map<uint64_t, Tuned> tuned;
 };
 
 // Device-resident InterpolationInput after LinearBVH!.

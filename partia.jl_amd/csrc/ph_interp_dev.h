@@ -25,7 +25,8 @@ template <class T> TreeView<T> view_of(const ph_lbvh* B) {
 
 // XCD-aware workgroup index: hardware dispatches workgroup b to XCD b % 8; give
 // each XCD a contiguous range of logical blocks so neighbouring tiles share an L2.
-__device__ inline i64 xcd_block() {
+__device__ inline i64 xcd_block(bool remap = true) {
+    if (!remap) return blockIdx.x;
     const i64 nb = gridDim.x, b = blockIdx.x;
     const i64 q = nb / kNumXcd, r = nb % kNumXcd;
     const i64 x = b % kNumXcd, pos = b / kNumXcd;
@@ -371,8 +372,9 @@ template <class T, int SRC> struct Sampler;
 template <class T> struct Sampler<T, 0> {
     PointList pl;
     i64 S;
+    int xcd = 1;  // XCD-contiguous block order
     __device__ inline bool get(i64& s, T& x, T& y, T& z) const {
-        s = xcd_block() * kBlock + threadIdx.x;
+        s = xcd_block(xcd != 0) * kBlock + threadIdx.x;
         if (s >= S) return false;
         x = T(pl.x[s]);
         y = T(pl.y[s]);
@@ -383,8 +385,9 @@ template <class T> struct Sampler<T, 0> {
 template <class T> struct Sampler<T, 1> {
     GridAxes g;
     i64 S;
+    int xcd = 1;  // XCD-contiguous block order
     __device__ inline bool get(i64& s, T& x, T& y, T& z) const {
-        const i64 tile = xcd_block() * (kBlock / kWave) + threadIdx.x / kWave;
+        const i64 tile = xcd_block(xcd != 0) * (kBlock / kWave) + threadIdx.x / kWave;
         const int lane = threadIdx.x & (kWave - 1);
         const i64 tx = (g.nx + 3) / 4, ty = (g.ny + 3) / 4;
         const i64 bx = tile % tx, by = (tile / tx) % ty, bz = tile / (tx * ty);

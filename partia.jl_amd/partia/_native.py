@@ -35,6 +35,7 @@ EXPORTS = (
     "ph_line_samples", "ph_line_samples_device", "ph_line_grid_device", "ph_nearest_h", "ph_line_table",
     "ph_batch_create", "ph_batch_run", "ph_batch_times", "ph_batch_destroy",
     "ph_line_samples_adjoint_device", "ph_line_grid_adjoint_device",
+    "ph_set_autotune", "ph_set_launch", "ph_autotune_report",
 )
 
 PH_REQ_POINTS, PH_REQ_PLANE, PH_REQ_BOX = 0, 1, 2
@@ -59,6 +60,11 @@ class ph_request(C.Structure):
                 ("position", C.c_double * 3), ("right", C.c_double * 3), ("up", C.c_double * 3),
                 ("forward", C.c_double * 3), ("amin", C.c_double * 3), ("amax", C.c_double * 3),
                 ("n", C.c_int64 * 3), ("out", C.c_void_p)]
+
+
+class ph_tune_entry(C.Structure):
+    _fields_ = [("key", C.c_uint64), ("occupancy", C.c_double), ("xcd_map", C.c_int32), ("reserved", C.c_int32),
+                ("best_ms", C.c_double), ("default_ms", C.c_double)]
 
 
 class ph_stats(C.Structure):
@@ -115,6 +121,9 @@ def lib():
         "ph_batch_run": (i32, [vp]),
         "ph_batch_times": (i32, [vp, vp, vp]),
         "ph_batch_destroy": (i32, [vp]),
+        "ph_set_autotune": (i32, [vp, C.c_int]),
+        "ph_set_launch": (i32, [vp, d, i32]),
+        "ph_autotune_report": (i32, [vp, i32, P(ph_tune_entry), P(i32)]),
         "ph_line_samples_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, i64] + [vp] * 8),
         "ph_line_grid_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, vp, vp, vp, vp, i64, vp, i64,
                                               vp, vp]),
@@ -188,6 +197,21 @@ class Context:
     def set_traversal(self, mode):
         """PH_TRAVERSAL_AUTO, PH_TRAVERSAL_LANE or PH_TRAVERSAL_BUCKET (point and grid calls)."""
         self.check(lib().ph_set_traversal(self.handle, int(mode)))
+
+    def set_autotune(self, enable):
+        self.check(lib().ph_set_autotune(self.handle, 1 if enable else 0))
+
+    def set_launch(self, occupancy=1.0, xcd_map=1):
+        self.check(lib().ph_set_launch(self.handle, float(occupancy), int(xcd_map)))
+
+    def autotune_report(self):
+        """List of dicts, one per tuned workload key."""
+        n = C.c_int32()
+        self.check(lib().ph_autotune_report(self.handle, 0, None, C.byref(n)))
+        arr = (ph_tune_entry * max(1, n.value))()
+        self.check(lib().ph_autotune_report(self.handle, n.value, arr, C.byref(n)))
+        return [{k: getattr(arr[i], k) for k, _ in ph_tune_entry._fields_ if k != "reserved"}
+                for i in range(n.value)]
 
     def close(self):
         if getattr(self, "handle", None):
