@@ -692,6 +692,22 @@ def main():
         fp64 = {"ms_per_step": ms64, "samples_per_s": S / (ms64 * 1e-3), "accumulation_kernel_ms": float(np.mean(k64))}
         print("bench: float64 %.2f ms per step (accumulation kernel %.2f ms)" % (ms64, fp64["accumulation_kernel_ms"]),
               file=sys.stderr, flush=True)
+        # the same float64 steps with the reference-structured walk: one DFS per
+        # sample over the LBVH, leaf by leaf (what a one-thread-per-sample port of
+        # the reference's GPU kernel does)
+        ctx.set_traversal(N.PH_TRAVERSAL_LANE)
+        step(N.PH_PREC_FP64)
+        ctx.check(L.ph_synchronize(ctx.handle))
+        t1 = time.perf_counter()
+        kl = [step(N.PH_PREC_FP64)[2] for _ in range(args.steps)]
+        ctx.check(L.ph_synchronize(ctx.handle))
+        msl = 1e3 * (time.perf_counter() - t1) / args.steps
+        ctx.set_traversal({"auto": N.PH_TRAVERSAL_AUTO, "lane": N.PH_TRAVERSAL_LANE,
+                           "bucket": N.PH_TRAVERSAL_BUCKET}[args.traversal])
+        fp64["per_leaf_walk"] = {"ms_per_step": msl, "samples_per_s": S / (msl * 1e-3),
+                                 "accumulation_kernel_ms": float(np.mean(kl))}
+        print("bench: float64, per-leaf walk %.2f ms per step (kernel %.2f ms)" % (msl, float(np.mean(kl))),
+              file=sys.stderr, flush=True)
 
     fbytes = 8 if prec == N.PH_PREC_FP64 else 4
     alg_bytes = n_acc * FIELDS_PER_NEIGHBOUR * fbytes + S * (3 + L_OUT) * fbytes
@@ -724,6 +740,7 @@ def main():
                      for e in ctx.autotune_report()],
         "fp64": fp64,
         "speedup_vs_fp64": (fp64["ms_per_step"] / ms_per_step) if fp64 else None,
+        "speedup_vs_fp64_per_leaf_walk": (fp64["per_leaf_walk"]["ms_per_step"] / ms_per_step) if fp64 else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "algorithmic_bytes": int(alg_bytes)},

@@ -109,7 +109,7 @@ __device__ inline i64 uniform_i64(i64 v) {
 // workgroup just finished and takes the next unit.  The unit is then read back
 // into scalar registers (wave-uniform), so the loop exit is a uniform branch:
 // every wave leaves in the same iteration and executes the same barriers.
-template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT, bool C32>
+template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT, bool C32>
 __global__ void __launch_bounds__(kBlock) k_batch(BucketView bv, const double* __restrict__ rec, CatChunk c,
                                                   const BatchReq* __restrict__ reqs, i32 nreq, i64 units,
                                                   unsigned long long* __restrict__ ctr,
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(kBlock) k_batch(BucketView bv, const double* _
         if (valid) batch_point(R, s, p);
         double ha = 0.0;
         if constexpr (STRAT != PH_SCATTER) {
-            if (valid) ha = nearest_h_bucket(bv, p);
+            if (valid) ha = nearest_h_bucket(bv, p, neighbour_bound2(bv, support_of<K>()));
         }
         u64 accepted = 0;
         if constexpr (C32)
@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(kBlock) k_batch(BucketView bv, const double* _
     }
 }
 
-template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT, bool C32>
+template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT, bool C32>
 void launch_batch_t(BatchState& st, const BucketView& bv, const CatChunk& c, unsigned long long* ctr, int num_cu,
                     hipStream_t s) {
     auto kern = k_batch<K, STRAT, MS, MG, MD, MC, EXACT, C32>;
@@ -181,13 +181,12 @@ void launch_batch_ks(BatchState& st, const BucketView& bv, const CatChunk& c, un
                      hipStream_t s) {
     const bool small = c.ng + c.nd + c.nc == 0 && c.ns <= 2;
     if (st.precision == PH_PREC_FP64) {
-        if (small) launch_batch_t<K, STRAT, 2, 0, 0, 0, false, false>(st, bv, c, ctr, num_cu, s);
-        else launch_batch_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, false, false>(st, bv, c, ctr, num_cu, s);
-    } else {
-        if (small) launch_batch_t<K, STRAT, 2, 0, 0, 0, false, true>(st, bv, c, ctr, num_cu, s);
-        else if (c.ns == 2 && c.ng == 1 && c.nd == 1 && c.nc == 1)
-            launch_batch_t<K, STRAT, 2, 1, 1, 1, true, true>(st, bv, c, ctr, num_cu, s);
-        else launch_batch_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, false, true>(st, bv, c, ctr, num_cu, s);
+        if (small) launch_batch_t<K, STRAT, 2, 0, 0, 0, 0, false>(st, bv, c, ctr, num_cu, s);
+        else launch_batch_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0, false>(st, bv, c, ctr, num_cu, s);
+    } else {  // the per-call selection (ph_bucket.hip), so the arithmetic is the same
+        if (small) launch_batch_t<K, STRAT, 2, 0, 0, 0, 0, true>(st, bv, c, ctr, num_cu, s);
+        else if (c2_shared(c)) launch_batch_t<K, STRAT, 2, 1, 1, 1, 2, true>(st, bv, c, ctr, num_cu, s);
+        else launch_batch_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0, true>(st, bv, c, ctr, num_cu, s);
     }
 }
 

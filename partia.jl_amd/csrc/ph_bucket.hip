@@ -103,14 +103,17 @@ __global__ void k_bucket_root(i64 n, i32 root, const NodeRec<double>* __restrict
     recs[0] = R;
 }
 
+// Smoothing length of each sample's nearest particle, or NaN when no particle
+// lies within Kd * max(h) (then the sample has no neighbours).
 template <int SRC>
-__global__ void __launch_bounds__(kBlock) k_nearest_bucket(BucketView bv, Sampler<double, SRC> smp, double* __restrict__ ha) {
+__global__ void __launch_bounds__(kBlock) k_nearest_bucket(BucketView bv, Sampler<double, SRC> smp, double Kd,
+                                                           double* __restrict__ ha) {
     i64 s;
     double p[3];
-    if (smp.get(s, p[0], p[1], p[2])) ha[s] = nearest_h_bucket(bv, p);
+    if (smp.get(s, p[0], p[1], p[2])) ha[s] = nearest_h_bucket(bv, p, neighbour_bound2(bv, Kd));
 }
 
-template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT, int SRC>
+template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT, int SRC>
 __global__ void __launch_bounds__(kBlock) k_bucket_c32(BucketView bv, const double* __restrict__ rec, CatChunk c,
                                                        Sampler<double, SRC> smp, const double* __restrict__ ha_s,
                                                        float* __restrict__ out, u64* counter) {
@@ -170,7 +173,7 @@ template <class F> size_t occupancy_pad(F kern, float occupancy) {
     return (size_t)pad;
 }
 
-template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT>
+template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT>
 void launch_c32_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const GridAxes* ga, i64 S, const double* ha,
                   void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
     const BucketView bv = bucket_view(B);
@@ -239,8 +242,9 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
         const BucketView bv = bucket_view(B);
         const unsigned g = sample_blocks(ga, S);
         const int xm = tl_launch.xcd_map;
-        if (ga) k_nearest_bucket<1><<<g, kBlock, 0, s>>>(bv, Sampler<double, 1>{*ga, S, xm}, B->ha.as<double>());
-        else k_nearest_bucket<0><<<g, kBlock, 0, s>>>(bv, Sampler<double, 0>{*pl, S, xm}, B->ha.as<double>());
+        const double Kd = support_of<K>();
+        if (ga) k_nearest_bucket<1><<<g, kBlock, 0, s>>>(bv, Sampler<double, 1>{*ga, S, xm}, Kd, B->ha.as<double>());
+        else k_nearest_bucket<0><<<g, kBlock, 0, s>>>(bv, Sampler<double, 0>{*pl, S, xm}, Kd, B->ha.as<double>());
         PH_HIP(hipGetLastError());
         ha = B->ha.as<double>();
     }
@@ -253,10 +257,10 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
     }
     k_pack_c32<<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c32, ncol, c32.rw, B->rec32.as<double>());
     PH_HIP(hipGetLastError());
-    if (small) launch_c32_t<K, STRAT, 2, 0, 0, 0, false>(B, c32, pl, ga, S, ha, out, counters, s, ev);
-    else if (c.ns == 2 && c.ng == 1 && c.nd == 1 && c.nc == 1)  // density, scalar, gradient, divergence, curl
-        launch_c32_t<K, STRAT, 2, 1, 1, 1, true>(B, c32, pl, ga, S, ha, out, counters, s, ev);
-    else launch_c32_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, false>(B, c32, pl, ga, S, ha, out, counters, s, ev);
+    if (small) launch_c32_t<K, STRAT, 2, 0, 0, 0, 0>(B, c32, pl, ga, S, ha, out, counters, s, ev);
+    else if (c2_shared(c32))  // density, scalar A, grad A, div v, curl v
+        launch_c32_t<K, STRAT, 2, 1, 1, 1, 2>(B, c32, pl, ga, S, ha, out, counters, s, ev);
+    else launch_c32_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0>(B, c32, pl, ga, S, ha, out, counters, s, ev);
 }
 
 template <int K>

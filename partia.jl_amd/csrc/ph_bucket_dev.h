@@ -196,7 +196,11 @@ template <int STRAT> __device__ inline float radius2_32(float Kv, float r2a, flo
 // FP32-compensated record (position 0: {m/rho, 1/h}; >= 1: float64 columns).  EXACT: the catalog chunk has
 // exactly MS scalars, MG gradients, MD divergences and MC curls; otherwise the
 // unused output slots read position 0 and add zeros (no branch either).
-template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT> struct AccC {
+// EXACT = 2 additionally shares sums the C2-shaped catalog computes twice (the
+// gradient field is the second scalar's column, the curl the divergence's
+// columns): the same fma sequence over the same operands, so identical values.
+template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT> struct AccC {
+    static constexpr bool SHARE = EXACT == 2;
     static constexpr int NS = MS > 0 ? MS : 1, G = MG > 0 ? MG : 1, D = MD > 0 ? MD : 1, CC = MC > 0 ? MC : 1;
     double S1, Q[3];
     double ts[NS];
@@ -270,7 +274,7 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT> struct A
             for (int j = 0; j < MG; ++j) {
                 const bool on = EXACT || j < c.ng;
                 const double A = on ? r(c.gpos[j]) : 0.0;
-                tgs[j] = fma(A, w, tgs[j]);
+                if constexpr (!SHARE) tgs[j] = fma(A, w, tgs[j]);
                 tg[j][0] = fma(A, g0, tg[j][0]);
                 tg[j][1] = fma(A, g1, tg[j][1]);
                 tg[j][2] = fma(A, g2, tg[j][2]);
@@ -290,9 +294,11 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT> struct A
                 const bool on = EXACT || j < c.nc;
                 const double ax = on ? r(c.cpos[3 * j]) : 0.0, ay = on ? r(c.cpos[3 * j + 1]) : 0.0,
                              az = on ? r(c.cpos[3 * j + 2]) : 0.0;
-                tcs[j][0] = fma(ax, w, tcs[j][0]);
-                tcs[j][1] = fma(ay, w, tcs[j][1]);
-                tcs[j][2] = fma(az, w, tcs[j][2]);
+                if constexpr (!SHARE) {
+                    tcs[j][0] = fma(ax, w, tcs[j][0]);
+                    tcs[j][1] = fma(ay, w, tcs[j][1]);
+                    tcs[j][2] = fma(az, w, tcs[j][2]);
+                }
                 tc[j][0] = fma(-az, g1, fma(ay, g2, tc[j][0]));
                 tc[j][1] = fma(-ax, g2, fma(az, g0, tc[j][1]));
                 tc[j][2] = fma(-ay, g0, fma(ax, g1, tc[j][2]));
@@ -334,7 +340,7 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT> struct A
 #pragma unroll
         for (int j = 0; j < MG; ++j)
             if (j < c.ng) {
-                const double A = tgs[j] * invS1;
+                const double A = (SHARE ? ts[1] : tgs[j]) * invS1;
                 out[(c.gout[j] + 0) * S + s] = (float)(tg[j][0] - A * Q[0]);
                 out[(c.gout[j] + 1) * S + s] = (float)(tg[j][1] - A * Q[1]);
                 out[(c.gout[j] + 2) * S + s] = (float)(tg[j][2] - A * Q[2]);
@@ -348,7 +354,8 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT> struct A
 #pragma unroll
         for (int j = 0; j < MC; ++j)
             if (j < c.nc) {
-                const double ax = tcs[j][0] * invS1, ay = tcs[j][1] * invS1, az = tcs[j][2] * invS1;
+                const double(&cs)[3] = SHARE ? tds[j] : tcs[j];
+                const double ax = cs[0] * invS1, ay = cs[1] * invS1, az = cs[2] * invS1;
                 out[(c.cout[j] + 0) * S + s] = (float)(-(tc[j][0] - (ay * Q[2] - az * Q[1])));
                 out[(c.cout[j] + 1) * S + s] = (float)(-(tc[j][1] - (az * Q[0] - ax * Q[2])));
                 out[(c.cout[j] + 2) * S + s] = (float)(-(tc[j][2] - (ax * Q[1] - ay * Q[0])));
@@ -455,7 +462,9 @@ __device__ __attribute__((always_inline)) u64 test_buckets(const BucketView& bv,
 // lexicographic minimum of (float64 d2, leaf index), i.e. the first nearest leaf
 // in DFS order.  A greedy descent seeds the bound; pruning compares float32 box
 // distances with the bound inflated to cover rounding, so no candidate is lost.
-__device__ inline double nearest_h_bucket(const BucketView& bv, const double p[3]) {
+// Only leaves with d2 <= bound2 are candidates: NaN when there is none (with
+// bound2 = infinity, only for an empty tree).
+__device__ inline double nearest_h_bucket(const BucketView& bv, const double p[3], double bound2 = INFINITY) {
     const float px = (float)p[0], py = (float)p[1], pz = (float)p[2];
     const float eps = 2.3841858e-7f * fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz))) + 1e-30f;
     auto box_d2 = [&](const BucketRec& R) {
@@ -468,8 +477,9 @@ __device__ inline double nearest_h_bucket(const BucketView& bv, const double p[3
         }
         return d2;
     };
-    double best = INFINITY;
-    i64 bk = -1;
+    constexpr i64 kNone = INT64_MAX;
+    double best = bound2;  // a leaf at exactly the bound is a candidate (k < kNone)
+    i64 bk = kNone;
     auto scan = [&](i32 link) {
         const i64 lo = bucket_lo(link);
         const int cnt = bucket_cnt(link);
@@ -510,7 +520,17 @@ __device__ inline double nearest_h_bucket(const BucketView& bv, const double p[3
             ++i;
         }
     }
-    return bk >= 0 ? bv.leaves[bk].h : (double)NAN;
+    return bk != kNone ? bv.leaves[bk].h : (double)NAN;
+}
+
+// Bound of the nearest-h search of a sample that is interpolated with support
+// K: a particle farther than K * max(h) from the sample is no neighbour in any
+// strategy (every radius is K h_a or K h_b, h_a being some particle's h), so
+// when no particle lies within it the outputs are NaN whatever h_a is, and the
+// search may stop there.  The root record's scale is max(h) rounded up.
+__device__ inline double neighbour_bound2(const BucketView& bv, double Kd) {
+    const double r = Kd * (double)bv.recs[0].scale;
+    return r * r;
 }
 
 
@@ -519,17 +539,19 @@ __device__ inline double nearest_h_bucket(const BucketView& bv, const double p[3
 
 // FP32-compensated interpolation of sample s at p (ha: smoothing length of its
 // nearest particle, gather / symmetric).  out: L x S float32.
-template <int K, int STRAT, int MS, int MG, int MD, int MC, bool EXACT>
+template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT>
 __device__ __attribute__((always_inline)) void bucket_c32_sample(const BucketView& bv, const double* __restrict__ rec,
                                                                  const CatChunk& c, bool valid, const double p[3],
                                                                  double ha, float* __restrict__ out, i64 S, i64 s,
                                                                  u64& accepted) {
     const double Kd = support_of<K>();
-    if (!valid) ha = 0.0;
+    // no particle within K max(h): no neighbours (NaN outputs), no walk
+    const bool none = STRAT != PH_SCATTER && !(ha == ha);
+    if (!valid || none) ha = 0.0;
     const double ra = Kd * ha, r2a = ra * ra;  // float64, as the reference
     Probe<STRAT> pr;
     pr.init(p, Kd, ra);
-    const double iha = STRAT == PH_SCATTER || !valid ? 0.0 : 1.0 / ha;
+    const double iha = STRAT == PH_SCATTER || !valid || none ? 0.0 : 1.0 / ha;
     const double iha2 = iha * iha;
     const float inv_ha = (float)iha;
     const float sig = sigma3<float, K>();
@@ -546,7 +568,7 @@ __device__ __attribute__((always_inline)) void bucket_c32_sample(const BucketVie
 
     AccC<K, STRAT, MS, MG, MD, MC, EXACT> acc;
     acc.init();
-    i32 i = valid ? 0 : bv.nrec;
+    i32 i = valid && !none ? 0 : bv.nrec;
     i64 steps = 2 * (i64)bv.nrec + 8;  // a valid tree is walked in at most nrec steps
     BucketList bl{&blist[0][tid], 0};
     while (true) {
@@ -590,13 +612,14 @@ __device__ __attribute__((always_inline)) void bucket_f64_sample(const BucketVie
                                                                  double ha, double* __restrict__ out, i64 S, i64 s,
                                                                  u64& accepted) {
     const double Kd = support_of<K>();
-    if (!valid) ha = 0.0;
+    const bool none = STRAT != PH_SCATTER && !(ha == ha);  // as in bucket_c32_sample
+    if (!valid || none) ha = 0.0;
     const double ra = Kd * ha, r2a = ra * ra;
     Probe<STRAT> pr;
     pr.init(p, Kd, ra);
     Acc<double, K, STRAT, MS, MG, MD, MC> acc;
     acc.init();
-    i32 i = valid ? 0 : bv.nrec;
+    i32 i = valid && !none ? 0 : bv.nrec;
     i64 steps = 2 * (i64)bv.nrec + 8;
     __shared__ u32 blist[kQ][kBlock];
     BucketList bl{&blist[0][threadIdx.x], 0};
@@ -620,6 +643,17 @@ __device__ __attribute__((always_inline)) void bucket_f64_sample(const BucketVie
 
 BucketView bucket_view(const ph_lbvh* B) {
     return BucketView{B->n, (i32)B->nrec, B->brecs.as<BucketRec>(), B->leaves.as<LeafRec<double>>()};
+}
+
+// A compacted FP32C chunk of the C2 shape (2 scalars, 1 gradient, 1 divergence,
+// 1 curl) whose gradient field is the second scalar's column and whose curl
+// reads the divergence's columns: the shared-sum accumulator applies.
+inline bool c2_shared(const CatChunk& c) {
+    if (!(c.ns == 2 && c.ng == 1 && c.nd == 1 && c.nc == 1)) return false;
+    if (c.gpos[0] != c.spos[1]) return false;
+    for (int k = 0; k < 3; ++k)
+        if (c.cpos[k] != c.dpos[k]) return false;
+    return true;
 }
 
 // Compact FP32-compensated record layout of a catalog chunk: positions 1..ncol

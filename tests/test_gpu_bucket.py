@@ -152,3 +152,27 @@ def test_fp32c_plummer_C2_full_grid(backend):
     errs = field_errors(got, ref, fields=catalog_fields(FULL))
     print("FP32C vs FP64 (normwise, pointwise) per field:", ["%.2e/%.2e" % e for e in errs])
     assert max(e[0] for e in errs) <= FP32C_NORMWISE, errs
+
+
+@pytest.mark.parametrize("strategy", [O.GATHER, O.SYMMETRIC])
+def test_samples_beyond_every_support_are_nan(backend, strategy):
+    """Samples with no particle within K max(h) skip the exact nearest-h search
+    (their outputs are NaN whatever h_a is): float64 outputs stay bit-identical
+    to the oracle's full search, FP32C has the same NaN mask, and samples just
+    inside the bound keep their neighbours."""
+    ctx = backend.context
+    for kernel in (0, 2, 4):
+        d, o = pair(ctx, 3000, kernel, 70 + kernel)
+        rng = np.random.default_rng(kernel)
+        far = rng.random((3, 3000)) * 7.0 - 3.0  # mostly outside [0,1]^3
+        hmax = d.bvh()["leaf_scale"].max()
+        K = O.support(kernel)
+        edge = np.array([[1.0 + K * hmax * f for f in (0.2, 0.6, 0.95, 0.999)],
+                         [0.5] * 4, [0.5] * 4])  # beside the cloud, within the bound
+        P = np.concatenate([far, edge], axis=1)
+        ref = o.point_samples(FULL, strategy, *P)
+        assert np.isnan(ref[0, :3000]).mean() > 0.5
+        with traversal(ctx, N.PH_TRAVERSAL_BUCKET):
+            assert_same(d.points(FULL, strategy, *P), ref)
+            got32 = d.points(FULL, strategy, *P, precision=N.PH_PREC_FP32C)
+        assert np.array_equal(np.isnan(got32), np.isnan(ref))
