@@ -126,7 +126,8 @@ LaunchCfg autotuned(ph_context* ctx, ph_lbvh* B, const CatChunk& c, int strategy
         for (int xm : {1, 0}) cand.push_back(LaunchCfg{occ, xm});
     std::vector<double> best(cand.size(), 1e300);
     hipEvent_t* ev = &ctx->ev[2];
-    auto run = [&](const LaunchCfg& cf) {
+    auto run = [&](LaunchCfg cf) {
+        cf.tile = ctx->traversal != PH_TRAVERSAL_BUCKET;
         LaunchScope scope(cf);
         launch_bucket(B, c, strategy, precision, pl, ga, S, out, nullptr, ctx->stream, ev);
         PH_HIP(hipEventSynchronize(ev[1]));
@@ -158,6 +159,7 @@ void launch_point_like(ph_context* ctx, ph_lbvh* B, const CatChunk& c, int strat
     if (bucket) {
         LaunchCfg cfg = ctx->launch;
         if (ctx->autotune && S > 0) cfg = autotuned(ctx, B, c, strategy, precision, pl, ga, S, out);
+        cfg.tile = ctx->traversal != PH_TRAVERSAL_BUCKET;  // FP32C grids: per-wave walk unless asked otherwise
         LaunchScope scope(cfg);
         launch_bucket(B, c, strategy, precision, pl, ga, S, out, counter, ctx->stream, ev);
     } else {

@@ -257,6 +257,10 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
     }
     k_pack_c32<<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c32, ncol, c32.rw, B->rec32.as<double>());
     PH_HIP(hipGetLastError());
+    if (ga && tl_launch.tile) {  // one walk per wave of grid points
+        launch_tile_c32(B, c32, STRAT, *ga, S, ha, out, counters, s, ev);
+        return;
+    }
     if (small) launch_c32_t<K, STRAT, 2, 0, 0, 0, 0>(B, c32, pl, ga, S, ha, out, counters, s, ev);
     else if (c2_shared(c32))  // density, scalar A, grad A, div v, curl v
         launch_c32_t<K, STRAT, 2, 1, 1, 1, 2>(B, c32, pl, ga, S, ha, out, counters, s, ev);

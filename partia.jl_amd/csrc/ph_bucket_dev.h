@@ -227,7 +227,13 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT> struct Ac
     __device__ __attribute__((always_inline)) void add(bool live, const CatChunk& c, const Buf& rb, u32 voff,
                                                        double dx, double dy, double dz, double d2, double iha2,
                                                        float Wa_norm, float Ga_norm) {
-        auto r = [&](int pos) { return rb.f64(voff, (u32)pos * 8u); };
+        add_r(live, c, [&](int pos) { return rb.f64(voff, (u32)pos * 8u); }, dx, dy, dz, d2, iha2, Wa_norm, Ga_norm);
+    }
+    // The same with the record read through r(pos).
+    template <class Rec>
+    __device__ __attribute__((always_inline)) void add_r(bool live, const CatChunk& c, Rec&& r, double dx, double dy,
+                                                         double dz, double d2, double iha2, float Wa_norm,
+                                                         float Ga_norm) {
         const float2 vi = __builtin_bit_cast(float2, r(0));
         const float d2f = (float)d2;
         float W, coef;

@@ -96,6 +96,7 @@ struct BlockCache {
 struct LaunchCfg {
     float occupancy = 1.f;
     int xcd_map = 1;
+    int tile = 1;  // FP32C grids: one wave walks its tile's union (0: one walk per lane)
 };
 // Configuration of the entry point running on this thread (set by the entry points).
 inline thread_local LaunchCfg tl_launch{};

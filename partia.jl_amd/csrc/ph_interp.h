@@ -66,6 +66,10 @@ void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, c
                    const GridAxes* ga, i64 S, void* out, u64* counters, hipStream_t s, hipEvent_t* ev = nullptr);
 // Builds the bucket tree of B if it is not built yet.
 void ensure_buckets(ph_lbvh* B, hipStream_t s);
+// FP32C grid interpolation with one bucket-tree walk per wave (ph_tile.hip);
+// the caller has packed B->rec32 for c32 and computed ha (gather / symmetric).
+void launch_tile_c32(ph_lbvh* B, const CatChunk& c32, int strategy, const GridAxes& ga, i64 S, const double* ha,
+                     void* out, u64* counters, hipStream_t s, hipEvent_t* ev);
 
 // Adjoint of the line projection of one scalar (cols[qsrc]): grad[6][n] float64
 // (d/dm, d/dA, d/dh, d/dx, d/dy, d/dz, input order) of sum cot[s] out[s];

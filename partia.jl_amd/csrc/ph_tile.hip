@@ -1,0 +1,212 @@
+// ph_tile.hip -- FP32-compensated grid interpolation with one bucket-tree walk
+// per wave.
+//
+// A wave holds a 4 x 4 x 4 block of grid points.  Instead of 64 independent
+// walks (each lane loading its own records, leaves and particle records: 64
+// cache lines per load instruction), the wave walks the bucket tree once and
+// keeps a bucket when any lane's conservative test keeps it: records are read
+// at wave-uniform addresses.  The leaves of the kept buckets are staged, 64 at
+// a time, in LDS together with their particle records (one coalesced load per
+// lane); every lane then applies the reference's float64 leaf test to all of
+// them and accumulates its accepted leaves in lockstep from LDS.
+//
+// Exactness: the union of the lanes' kept buckets contains each lane's own
+// kept buckets, hence all of its accepted leaves; staged leaves are in
+// increasing Morton index (preorder), each lane accepts exactly the leaves the
+// reference accepts (same float64 test) and accumulates them in that order with
+// the same arithmetic (AccC) as the per-lane kernel k_bucket_c32.  The two
+// kernels therefore give identical results, bit for bit.
+#include <algorithm>
+
+#include "ph_bucket_dev.h"
+
+namespace ph {
+namespace {
+
+// Cross-lane LDS exchange within one wave: writes of every lane are complete
+// and visible to the wave's other lanes after this point.
+__device__ __attribute__((always_inline)) inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Dynamic LDS of k_tile_c32 for record width rw: per wave 64 staged leaves of
+// (x, y, z, h, record) doubles, then 64 leaf indices per wave.
+inline size_t tile_lds_bytes(int rw) {
+    return (size_t)(kBlock / kWave) * kWave * (4 + (size_t)rw) * sizeof(double) +
+           (size_t)(kBlock / kWave) * kWave * sizeof(u32);
+}
+
+template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT>
+__global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const double* __restrict__ rec, CatChunk c,
+                                                     Sampler<double, 1> smp, const double* __restrict__ ha_s,
+                                                     float* __restrict__ out, u64* counter) {
+    extern __shared__ double tile_smem[];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int rw = c.rw;
+    const int slot = 4 + rw;  // doubles per staged leaf
+    double* st = tile_smem + (size_t)wv * kWave * slot;
+    u32* lidx = reinterpret_cast<u32*>(tile_smem + (size_t)(kBlock / kWave) * kWave * slot) + wv * kWave;
+
+    i64 s = 0;
+    double p[3] = {0.0, 0.0, 0.0};
+    const bool valid = smp.get(s, p[0], p[1], p[2]);
+    const double Kd = support_of<K>();
+    double ha = 0.0;
+    if constexpr (STRAT != PH_SCATTER) {
+        if (valid) ha = ha_s[s];
+    }
+    // no particle within K max(h): no neighbours (NaN outputs)
+    const bool none = STRAT != PH_SCATTER && !(ha == ha);
+    const bool live_lane = valid && !none;
+    if (!live_lane) ha = 0.0;
+    const double ra = Kd * ha, r2a = ra * ra;  // float64, as the reference
+    Probe<STRAT> pr;
+    pr.init(p, Kd, ra);
+    const double iha = STRAT == PH_SCATTER || !live_lane ? 0.0 : 1.0 / ha;
+    const double iha2 = iha * iha;
+    const float inv_ha = (float)iha;
+    const float sig = sigma3<float, K>();
+    const float Wa_norm = sig * inv_ha * inv_ha * inv_ha, Ga_norm = Wa_norm * inv_ha;
+
+    AccC<K, STRAT, MS, MG, MD, MC, EXACT> acc;
+    acc.init();
+    u64 accepted = 0;
+    i32 i = __ballot(live_lane) ? 0 : bv.nrec;  // wave-uniform from here on
+    i64 steps = (i64)bv.nrec + 8;               // the union walk visits every record at most once
+    for (;;) {
+        // union walk: the leaves of up to 64 kept buckets' worth of leaves
+        int nl = 0;
+        while (i < bv.nrec && steps > 0) {
+            --steps;
+            const BucketRec R = bv.recs[i];
+            const bool keep = __ballot(live_lane && pr.keep(R)) != 0;
+            if (R.link >= 0) {
+                i = __builtin_amdgcn_readfirstlane(keep ? i + 1 : R.link);
+                continue;
+            }
+            if (keep) {
+                const int cnt = bucket_cnt(R.link);
+                if (nl + cnt > kWave) {  // starts the next batch
+                    ++steps;
+                    break;
+                }
+                if (lane < cnt) lidx[nl + lane] = (u32)(bucket_lo(R.link) + lane);
+                nl += cnt;
+            }
+            i = __builtin_amdgcn_readfirstlane(i + 1);
+        }
+        if (nl == 0) break;
+        wave_sync();
+        // stage leaf lane and its record (consecutive leaves: coalesced loads)
+        if (lane < nl) {
+            const u32 k = lidx[lane];
+            const LeafRec<double> L = bv.leaves[k];
+            double* d = st + (size_t)lane * slot;
+            d[0] = L.x;
+            d[1] = L.y;
+            d[2] = L.z;
+            d[3] = L.h;
+            const double* r = rec + (size_t)k * rw;
+            for (int q = 0; q < rw; ++q) d[4 + q] = r[q];
+        }
+        wave_sync();
+        // the reference's float64 leaf test of every staged leaf, per lane
+        u64 mask = 0;
+        for (int j = 0; j < nl; ++j) {
+            const double* d = st + (size_t)j * slot;
+            const LeafRec<double> L{d[0], d[1], d[2], d[3]};
+            const bool acc_j = live_lane && d2_point(p[0], p[1], p[2], L) <= radius2<double, STRAT>(Kd, r2a, L.h);
+            mask |= acc_j ? (1ull << j) : 0ull;
+        }
+        accepted += __popcll(mask);
+        // lockstep: one neighbour per lane and iteration, lowest bit first
+        // (increasing leaf index); a lane with none left adds zeros
+        int nit = __popcll(mask);
+        for (int d = kWave / 2; d > 0; d >>= 1) nit = max(nit, __shfl_xor(nit, d, kWave));
+        nit = __builtin_amdgcn_readfirstlane(nit);
+        for (int it = 0; it < nit; ++it) {
+            const bool live = mask != 0;
+            const int b = __builtin_ctzll(mask | (1ull << 63));
+            mask &= mask - 1;
+            const double* d = st + (size_t)(b < nl ? b : nl - 1) * slot;  // a staged (finite) leaf
+            const double dx = p[0] - d[0], dy = p[1] - d[1], dz = p[2] - d[2];
+            acc.add_r(live, c, [&](int pos) { return d[4 + pos]; }, dx, dy, dz, (dx * dx + dy * dy) + dz * dz, iha2,
+                      Wa_norm, Ga_norm);
+        }
+        wave_sync();  // every lane is done with the staged leaves
+        if (!(i < bv.nrec && steps > 0)) break;
+    }
+    if (valid) acc.finish(c, out, smp.S, s);
+    count_add(counter, accepted);
+}
+
+template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT>
+void launch_tile_t(const BucketView& bv, const double* rec, const CatChunk& c, const GridAxes& ga, i64 S,
+                   const double* ha, float* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+    const LaunchCfg cfg = tl_launch;
+    auto kern = k_tile_c32<K, STRAT, MS, MG, MD, MC, EXACT>;
+    const size_t dyn = tile_lds_bytes(c.rw);
+    size_t pad = 0;
+    if (cfg.occupancy < 1.f) {  // as occupancy_pad, with this kernel's dynamic LDS
+        int per_cu = 0, dev = 0, lds_cu = 0, lds_block = 0;
+        PH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, dyn));
+        PH_HIP(hipGetDevice(&dev));
+        PH_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+        PH_HIP(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const int want = std::max(1, (int)(per_cu * cfg.occupancy + 0.5f));
+        if (want < per_cu && lds_cu > 0) {
+            hipFuncAttributes fa{};
+            PH_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kern)));
+            const i64 per_block = (i64)(lds_cu / want / 1024) * 1024;
+            const i64 extra = per_block - (i64)fa.sharedSizeBytes - (i64)dyn;
+            if (extra > 0 && per_block <= lds_block) pad = (size_t)extra;
+        }
+    }
+    const i64 tiles = ((ga.nx + 3) / 4) * ((ga.ny + 3) / 4) * ((ga.nz + 3) / 4);
+    const unsigned g = (unsigned)((tiles + 3) / 4);
+    if (ev) PH_HIP(hipEventRecord(ev[0], s));
+    kern<<<g, kBlock, dyn + pad, s>>>(bv, rec, c, Sampler<double, 1>{ga, S, cfg.xcd_map}, ha, out, counters);
+    PH_HIP(hipGetLastError());
+    if (ev) PH_HIP(hipEventRecord(ev[1], s));
+}
+
+template <int K, int STRAT>
+void launch_tile_ks(const BucketView& bv, const double* rec, const CatChunk& c32, const GridAxes& ga, i64 S,
+                    const double* ha, float* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+    // the per-lane kernel's selection (ph_bucket.hip), so the arithmetic is the same
+    const bool small = c32.ng + c32.nd + c32.nc == 0 && c32.ns <= 2;
+    if (small) launch_tile_t<K, STRAT, 2, 0, 0, 0, 0>(bv, rec, c32, ga, S, ha, out, counters, s, ev);
+    else if (c2_shared(c32)) launch_tile_t<K, STRAT, 2, 1, 1, 1, 2>(bv, rec, c32, ga, S, ha, out, counters, s, ev);
+    else launch_tile_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0>(bv, rec, c32, ga, S, ha, out, counters, s, ev);
+}
+
+template <int K>
+void launch_tile_k(int strategy, const BucketView& bv, const double* rec, const CatChunk& c32, const GridAxes& ga,
+                   i64 S, const double* ha, float* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+    switch (strategy) {
+    case PH_GATHER: launch_tile_ks<K, PH_GATHER>(bv, rec, c32, ga, S, ha, out, counters, s, ev); break;
+    case PH_SCATTER: launch_tile_ks<K, PH_SCATTER>(bv, rec, c32, ga, S, ha, out, counters, s, ev); break;
+    default: launch_tile_ks<K, PH_SYMMETRIC>(bv, rec, c32, ga, S, ha, out, counters, s, ev); break;
+    }
+}
+
+}  // namespace
+
+void launch_tile_c32(ph_lbvh* B, const CatChunk& c32, int strategy, const GridAxes& ga, i64 S, const double* ha,
+                     void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+    const BucketView bv = bucket_view(B);
+    const double* rec = B->rec32.as<double>();
+    float* o = static_cast<float*>(out);
+    switch (B->kernel) {
+    case PH_KERNEL_M4: launch_tile_k<PH_KERNEL_M4>(strategy, bv, rec, c32, ga, S, ha, o, counters, s, ev); break;
+    case PH_KERNEL_M5: launch_tile_k<PH_KERNEL_M5>(strategy, bv, rec, c32, ga, S, ha, o, counters, s, ev); break;
+    case PH_KERNEL_M6: launch_tile_k<PH_KERNEL_M6>(strategy, bv, rec, c32, ga, S, ha, o, counters, s, ev); break;
+    case PH_KERNEL_C2: launch_tile_k<PH_KERNEL_C2>(strategy, bv, rec, c32, ga, S, ha, o, counters, s, ev); break;
+    case PH_KERNEL_C4: launch_tile_k<PH_KERNEL_C4>(strategy, bv, rec, c32, ga, S, ha, o, counters, s, ev); break;
+    default: launch_tile_k<PH_KERNEL_C6>(strategy, bv, rec, c32, ga, S, ha, o, counters, s, ev); break;
+    }
+}
+
+}  // namespace ph

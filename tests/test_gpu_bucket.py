@@ -176,3 +176,50 @@ def test_samples_beyond_every_support_are_nan(backend, strategy):
             assert_same(d.points(FULL, strategy, *P), ref)
             got32 = d.points(FULL, strategy, *P, precision=N.PH_PREC_FP32C)
         assert np.array_equal(np.isnan(got32), np.isnan(ref))
+
+
+WIDE = dict(sslots=[1, 2, 3, 4, 5], sflags=[0, 1, 1, 1, 0], gslots=[0, 3], dslots=[3, 4, 5], cslots=[5, 4, 2])
+SMALL = dict(sslots=[1, 2], sflags=[0, 1])
+
+
+@pytest.mark.parametrize("kernel", range(6))
+@pytest.mark.parametrize("strategy", STRATS)
+def test_tile_walk_equals_lane_walk_fp32c_grids(backend, kernel, strategy):
+    """FP32C grids: the per-wave walk (default) and the per-lane walk give
+    identical outputs and neighbour counts, for the C2 catalog (shared sums),
+    a two-scalar catalog and a catalog split into chunks; ragged tile edges."""
+    ctx = backend.context
+    d, o = pair(ctx, 9000, kernel, 300 + kernel)
+    axes = (synthetic.linrange(-0.15, 1.1, 19), synthetic.linrange(0.0, 1.0, 11), synthetic.linrange(0.1, 0.95, 7))
+    for cat in (FULL, SMALL, WIDE):
+        res = {}
+        for mode in (N.PH_TRAVERSAL_AUTO, N.PH_TRAVERSAL_BUCKET):
+            with traversal(ctx, mode):
+                ctx.set_count_neighbours(True)
+                try:
+                    res[mode] = (d.grid(cat, strategy, *axes, precision=N.PH_PREC_FP32C), ctx.stats()["accepted"])
+                finally:
+                    ctx.set_count_neighbours(False)
+        assert_same(res[N.PH_TRAVERSAL_AUTO][0], res[N.PH_TRAVERSAL_BUCKET][0])
+        assert res[N.PH_TRAVERSAL_AUTO][1] == res[N.PH_TRAVERSAL_BUCKET][1]
+    # and against the float64 oracle
+    ref = o.point_samples(FULL, strategy, *grid_points(*axes))
+    got = d.grid(FULL, strategy, *axes, precision=N.PH_PREC_FP32C)
+    assert max(e[0] for e in field_errors(got, ref, fields=catalog_fields(FULL))) <= FP32C_NORMWISE
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 17, 65, 300])
+def test_tile_walk_small_trees_and_far_grids(backend, n):
+    """Tiny trees (a single bucket, n = 1), more staged leaves than one batch
+    holds (large h), and grids reaching far beyond the particles."""
+    ctx = backend.context
+    c6, cols = cloud(n, 90 + n)
+    c6 = (c6[0], c6[1], c6[2], c6[3], c6[4] * 0 + (0.9 if n > 60 else 0.3), c6[5])
+    d = DeviceSet(ctx, *c6, cols, kernel=4)
+    axes = (synthetic.linrange(-2.0, 3.0, 9), synthetic.linrange(-0.2, 1.2, 10), synthetic.linrange(0.0, 1.0, 5))
+    for strategy in STRATS:
+        res = {}
+        for mode in (N.PH_TRAVERSAL_AUTO, N.PH_TRAVERSAL_BUCKET):
+            with traversal(ctx, mode):
+                res[mode] = d.grid(FULL, strategy, *axes, precision=N.PH_PREC_FP32C)
+        assert_same(res[N.PH_TRAVERSAL_AUTO], res[N.PH_TRAVERSAL_BUCKET])
