@@ -38,8 +38,13 @@ inline size_t tile_lds_bytes(int rw) {
            (size_t)(kBlock / kWave) * kWave * sizeof(u32);
 }
 
+// recs is bv.recs again as a restrict-qualified argument: the records are then
+// known not to be written by the kernel, and the wave-uniform record loads of
+// the walk become scalar loads (one round trip per record instead of two vector
+// loads).
 template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT>
-__global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const double* __restrict__ rec, CatChunk c,
+__global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const BucketRec* __restrict__ recs,
+                                                     const double* __restrict__ rec, CatChunk c,
                                                      Sampler<double, 1> smp, const double* __restrict__ ha_s,
                                                      float* __restrict__ out, u64* counter) {
     extern __shared__ double tile_smem[];
@@ -75,15 +80,21 @@ __global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const double
     u64 accepted = 0;
     i32 i = __ballot(live_lane) ? 0 : bv.nrec;  // wave-uniform from here on
     i64 steps = (i64)bv.nrec + 8;               // the union walk visits every record at most once
+    BucketRec R = recs[i];                      // the record array has one padding record at nrec
     for (;;) {
         // union walk: the leaves of up to 64 kept buckets' worth of leaves
         int nl = 0;
         while (i < bv.nrec && steps > 0) {
             --steps;
-            const BucketRec R = bv.recs[i];
-            const bool keep = __ballot(live_lane && pr.keep(R)) != 0;
+            // both possible successors are loaded before the test: the next
+            // record (first child, or the record after a bucket) and the escape
+            const i32 jmp = R.link >= 0 ? R.link : i + 1;
+            const BucketRec Rn = recs[i + 1], Rj = recs[jmp];
+            const bool kl = pr.keep(R);
+            const bool keep = __ballot(live_lane && kl) != 0;
             if (R.link >= 0) {
                 i = __builtin_amdgcn_readfirstlane(keep ? i + 1 : R.link);
+                R = keep ? Rn : Rj;
                 continue;
             }
             if (keep) {
@@ -96,6 +107,7 @@ __global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const double
                 nl += cnt;
             }
             i = __builtin_amdgcn_readfirstlane(i + 1);
+            R = Rn;
         }
         if (nl == 0) break;
         wave_sync();
@@ -167,7 +179,7 @@ void launch_tile_t(const BucketView& bv, const double* rec, const CatChunk& c, c
     const i64 tiles = ((ga.nx + 3) / 4) * ((ga.ny + 3) / 4) * ((ga.nz + 3) / 4);
     const unsigned g = (unsigned)((tiles + 3) / 4);
     if (ev) PH_HIP(hipEventRecord(ev[0], s));
-    kern<<<g, kBlock, dyn + pad, s>>>(bv, rec, c, Sampler<double, 1>{ga, S, cfg.xcd_map}, ha, out, counters);
+    kern<<<g, kBlock, dyn + pad, s>>>(bv, bv.recs, rec, c, Sampler<double, 1>{ga, S, cfg.xcd_map}, ha, out, counters);
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[1], s));
 }

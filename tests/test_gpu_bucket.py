@@ -50,7 +50,7 @@ def test_bucket_and_lane_grids_bitwise(backend, strategy):
     d, o = pair(ctx, 7000, 4, 7)
     axes = (synthetic.linrange(-0.1, 1.1, 13), synthetic.linrange(0.0, 1.0, 9), synthetic.linrange(0.2, 0.9, 6))
     ref = o.point_samples(FULL, strategy, *grid_points(*axes))
-    for mode in (N.PH_TRAVERSAL_BUCKET, N.PH_TRAVERSAL_LANE):
+    for mode in (N.PH_TRAVERSAL_AUTO, N.PH_TRAVERSAL_BUCKET, N.PH_TRAVERSAL_LANE):
         with traversal(ctx, mode):
             assert_same(d.grid(FULL, strategy, *axes), ref)
 
@@ -223,3 +223,7 @@ def test_tile_walk_small_trees_and_far_grids(backend, n):
             with traversal(ctx, mode):
                 res[mode] = d.grid(FULL, strategy, *axes, precision=N.PH_PREC_FP32C)
         assert_same(res[N.PH_TRAVERSAL_AUTO], res[N.PH_TRAVERSAL_BUCKET])
+        # float64: the per-wave nearest-h search (default) against the per-leaf walk
+        with traversal(ctx, N.PH_TRAVERSAL_LANE):
+            lane = d.grid(FULL, strategy, *axes)
+        assert_same(d.grid(FULL, strategy, *axes), lane)
