@@ -80,6 +80,9 @@ def parse():
     ap.add_argument("--traversal", choices=["auto", "lane", "bucket"], default="auto")
     ap.add_argument("--autotune", choices=["on", "off"], default="on",
                     help="tune the launch configuration on the first (untimed) call of each workload key")
+    ap.add_argument("--launch", default=None, metavar="OCC,XCD[,ORDER]",
+                    help="fixed launch configuration (occupancy, block order, tile order) of the point / grid kernels; "
+                         "turns the autotuner off")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
     ap.add_argument("--workload", choices=["grid256", "column2048", "grid64", "batch512", "adjoint1024"],
@@ -103,6 +106,16 @@ class Dev:
         if self.p:
             N.lib().ph_device_free(self.ctx.handle, self.p)
             self.p = C.c_void_p()
+
+
+def apply_launch(ctx, args):
+    """--launch OCC,XCD fixes the launch configuration; otherwise --autotune decides."""
+    if args.launch:
+        v = args.launch.split(",")
+        ctx.set_autotune(False)
+        ctx.set_launch(float(v[0]), int(v[1]), int(v[2]) if len(v) > 2 else 0)
+    else:
+        ctx.set_autotune(args.autotune == "on")
 
 
 def measured_traffic(kernel, workload="plummer2e6_c4_symmetric_grid256"):
@@ -618,7 +631,7 @@ def main():
     ctx = N.Context(0)
     ctx.set_traversal({"auto": N.PH_TRAVERSAL_AUTO, "lane": N.PH_TRAVERSAL_LANE,
                        "bucket": N.PH_TRAVERSAL_BUCKET}[args.traversal])
-    ctx.set_autotune(args.autotune == "on")
+    apply_launch(ctx, args)
     L = N.lib()
     names = ["x", "y", "z", "m", "h", "rho", "rho", "A", "vx", "vy", "vz"]
     dcols = []
@@ -736,7 +749,8 @@ def main():
         "accepted_neighbours": int(n_acc),
         "guard_band_rechecks": int(n_rechecks),
         "traversal": args.traversal,
-        "autotune": [{k: e[k] for k in ("occupancy", "xcd_map", "best_ms", "default_ms")}
+        "launch": args.launch,
+        "autotune": [{k: e[k] for k in ("occupancy", "xcd_map", "tile_order", "best_ms", "default_ms")}
                      for e in ctx.autotune_report()],
         "fp64": fp64,
         "speedup_vs_fp64": (fp64["ms_per_step"] / ms_per_step) if fp64 else None,

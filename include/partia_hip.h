@@ -133,24 +133,30 @@ int ph_set_traversal(ph_context* ctx, int mode);
 /* ---- autotuner (no reference counterpart): launch configuration of the point
  * and grid kernels of the bucket traversal.  The knobs change neither the
  * neighbour order nor any arithmetic, so every configuration gives identical
- * results; only the speed changes.  Knobs: occupancy (fraction of the default
- * workgroups per CU, enforced with unused dynamic LDS: 1, 0.75, 0.5) and the
- * block order (1: consecutive sample tiles on one XCD so they share its L2;
- * 0: hardware order).  With autotuning on, the first call of each workload key
- * (precision, strategy, kernel, catalog shape, grid or list, log2 of the
- * sample and particle counts) times every configuration twice on the call's
- * own data and keeps the fastest for later calls of that key; otherwise the
- * ph_set_launch configuration (default 1, 1) is used. ---- */
+ * results; only the speed changes.  Knobs:
+ *   occupancy   fraction of the default workgroups per CU, enforced with
+ *               unused dynamic LDS (1, 0.75, 0.5 ...);
+ *   xcd_map     block order over the 8 XCDs: 0 hardware order; 1 one
+ *               contiguous range of blocks per XCD; k = 2..7 runs of 2^(k+2)
+ *               consecutive blocks dealt round-robin to the XCDs (each XCD's
+ *               blocks are neighbours, dense regions spread over every XCD);
+ *   tile_order  grid tiles (4x4x4 samples, one per wavefront) in x-fastest
+ *               rows (0) or in bricks of 8x8x8 tiles, Morton order inside (1).
+ * With autotuning on, the first call of each workload key (precision,
+ * strategy, kernel, catalog shape, grid or list, log2 of the sample and
+ * particle counts) times the candidate configurations twice on the call's own
+ * data and keeps the fastest for later calls of that key; otherwise the
+ * ph_set_launch configuration (default 1, 2, 0) is used. ---- */
 typedef struct ph_tune_entry {
     uint64_t key;
     double occupancy;
     int32_t xcd_map;
-    int32_t reserved;
+    int32_t tile_order;
     double best_ms;    /* kernel time of the chosen configuration   */
     double default_ms; /* kernel time of the default configuration  */
 } ph_tune_entry;
 int ph_set_autotune(ph_context* ctx, int enable);
-int ph_set_launch(ph_context* ctx, double occupancy, int32_t xcd_map);
+int ph_set_launch(ph_context* ctx, double occupancy, int32_t xcd_map, int32_t tile_order);
 /* Copies up to capacity entries of the context's tuned keys; *count = all entries. */
 int ph_autotune_report(ph_context* ctx, int32_t capacity, ph_tune_entry* out, int32_t* count);
 

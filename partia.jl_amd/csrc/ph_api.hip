@@ -121,9 +121,17 @@ LaunchCfg autotuned(ph_context* ctx, ph_lbvh* B, const CatChunk& c, int strategy
     const uint64_t key = tune_key(B, c, strategy, precision, ga != nullptr, S);
     auto it = ctx->tuned.find(key);
     if (it != ctx->tuned.end()) return it->second.cfg;
-    std::vector<LaunchCfg> cand;
-    for (float occ : {1.f, 0.75f, 0.5f})
-        for (int xm : {1, 0}) cand.push_back(LaunchCfg{occ, xm});
+    // the default first; full occupancy with the block orders that balance
+    // dense regions over the XCDs, both tile orders; two lower occupancies
+    std::vector<LaunchCfg> cand{ctx->launch};
+    auto add = [&](LaunchCfg a) {
+        for (const LaunchCfg& b : cand)
+            if (a.occupancy == b.occupancy && a.xcd_map == b.xcd_map && a.order == b.order) return;
+        cand.push_back(a);
+    };
+    for (int order : {0, 1})
+        for (int xm : {2, 3, 7, 0}) add(LaunchCfg{1.f, xm, order});
+    for (float occ : {0.75f, 0.5f}) add(LaunchCfg{occ, 2, 0});
     std::vector<double> best(cand.size(), 1e300);
     hipEvent_t* ev = &ctx->ev[2];
     auto run = [&](LaunchCfg cf) {
@@ -318,11 +326,11 @@ int ph_set_autotune(ph_context* ctx, int enable) {
     return PH_OK;
 }
 
-int ph_set_launch(ph_context* ctx, double occupancy, int32_t xcd_map) {
+int ph_set_launch(ph_context* ctx, double occupancy, int32_t xcd_map, int32_t tile_order) {
     if (!ctx) return PH_ERR_ARGUMENT;
-    if (!(occupancy > 0.0 && occupancy <= 1.0) || (xcd_map != 0 && xcd_map != 1))
-        return fail(ctx, PH_ERR_ARGUMENT, "occupancy must be in (0, 1] and xcd_map 0 or 1");
-    ctx->launch = LaunchCfg{(float)occupancy, xcd_map};
+    if (!(occupancy > 0.0 && occupancy <= 1.0) || xcd_map < 0 || xcd_map > 7 || tile_order < 0 || tile_order > 1)
+        return fail(ctx, PH_ERR_ARGUMENT, "occupancy must be in (0, 1], xcd_map in 0..7 and tile_order 0 or 1");
+    ctx->launch = LaunchCfg{(float)occupancy, xcd_map, tile_order};
     return PH_OK;
 }
 
@@ -335,7 +343,7 @@ int ph_autotune_report(ph_context* ctx, int32_t capacity, ph_tune_entry* out, in
             e.key = kv.first;
             e.occupancy = kv.second.cfg.occupancy;
             e.xcd_map = kv.second.cfg.xcd_map;
-            e.reserved = 0;
+            e.tile_order = kv.second.cfg.order;
             e.best_ms = kv.second.best_ms;
             e.default_ms = kv.second.default_ms;
         }

@@ -92,10 +92,14 @@ struct BlockCache {
 // every configuration:
 //   occupancy  fraction of the default workgroups per CU (1, 0.75, 0.5 ...),
 //              enforced with unused dynamic LDS per workgroup;
-//   xcd_map    1: consecutive tiles go to one XCD (shared L2), 0: hardware order.
+//   xcd_map    block order: 0 hardware order, 1 one contiguous range of tiles
+//              per XCD (shared L2), 2..7 runs of 2^(xcd_map+2) blocks dealt
+//              round-robin to the XCDs (shared L2 and balanced dense regions);
+//   order      grid tile order: 0 x-fastest rows, 1 bricks of 8^3 tiles.
 struct LaunchCfg {
     float occupancy = 1.f;
-    int xcd_map = 1;
+    int xcd_map = 2;
+    int order = 0;
     int tile = 1;  // FP32C grids: one wave walks its tile's union (0: one walk per lane)
 };
 // Configuration of the entry point running on this thread (set by the entry points).

@@ -23,14 +23,27 @@ template <class T> TreeView<T> view_of(const ph_lbvh* B) {
                        B->leaf_escape.as<i32>()};
 }
 
-// XCD-aware workgroup index: hardware dispatches workgroup b to XCD b % 8; give
-// each XCD a contiguous range of logical blocks so neighbouring tiles share an L2.
-__device__ inline i64 xcd_block(bool remap = true) {
-    if (!remap) return blockIdx.x;
+// XCD-aware workgroup index: hardware dispatches workgroup b to XCD b % 8.
+//   mode 0: hardware order (logical block b);
+//   mode 1: each XCD takes one contiguous range of logical blocks, so
+//           neighbouring tiles share an L2 (uneven work per range is not
+//           balanced);
+//   mode k in 2..7: each XCD takes runs of 2^(k+2) consecutive logical blocks,
+//           run j going to XCD j % 8: the blocks one XCD holds at a time are
+//           neighbours, and dense regions spread over every XCD.  Blocks past
+//           the last whole round of 8 runs keep the hardware order.
+// Every mode is a bijection of [0, gridDim.x).
+__device__ inline i64 xcd_block(int mode = 2) {
     const i64 nb = gridDim.x, b = blockIdx.x;
-    const i64 q = nb / kNumXcd, r = nb % kNumXcd;
+    if (mode == 0) return b;
     const i64 x = b % kNumXcd, pos = b / kNumXcd;
-    return x * q + (x < r ? x : r) + pos;
+    if (mode == 1) {
+        const i64 q = nb / kNumXcd, r = nb % kNumXcd;
+        return x * q + (x < r ? x : r) + pos;
+    }
+    const i64 run = i64(1) << (mode + 2), span = run * kNumXcd;
+    if (b >= nb / span * span) return b;
+    return ((pos / run) * kNumXcd + x) * run + pos % run;
 }
 
 // ---- geometry (LinearBVHTraversalMacros.jl:800-953) ----
@@ -372,9 +385,9 @@ template <class T, int SRC> struct Sampler;
 template <class T> struct Sampler<T, 0> {
     PointList pl;
     i64 S;
-    int xcd = 1;  // XCD-contiguous block order
+    int xcd = 2;  // block order (xcd_block mode)
     __device__ inline bool get(i64& s, T& x, T& y, T& z) const {
-        s = xcd_block(xcd != 0) * kBlock + threadIdx.x;
+        s = xcd_block(xcd) * kBlock + threadIdx.x;
         if (s >= S) return false;
         x = T(pl.x[s]);
         y = T(pl.y[s]);
@@ -382,15 +395,51 @@ template <class T> struct Sampler<T, 0> {
         return true;
     }
 };
+// Position of tile t in a (tx, ty, tz) tile grid walked in bricks of kBrick^3
+// tiles: z slabs of kBrick tile planes, each in y bands of kBrick rows, each in
+// bricks of kBrick columns (slabs, bands and bricks at the far edges are
+// clipped); Morton order inside a whole brick, x-fastest inside a clipped one.
+// A bijection of [0, tx*ty*tz): aligned runs of 8^k tiles are cubes.
+constexpr i64 kBrick = 8;
+__device__ inline void brick_tile(i64 t, i64 tx, i64 ty, i64 tz, i64& bx, i64& by, i64& bz) {
+    const i64 sz = t / (tx * ty * kBrick);  // z slab
+    t -= sz * tx * ty * kBrick;
+    const i64 hz = min(kBrick, tz - sz * kBrick);
+    const i64 sy = t / (tx * kBrick * hz);  // y band
+    t -= sy * tx * kBrick * hz;
+    const i64 hy = min(kBrick, ty - sy * kBrick);
+    const i64 sx = t / (kBrick * hy * hz);  // brick
+    t -= sx * kBrick * hy * hz;
+    const i64 wx = min(kBrick, tx - sx * kBrick);
+    if (wx == kBrick && hy == kBrick && hz == kBrick) {
+        bx = sx * kBrick + ((t & 1) | ((t >> 2) & 2) | ((t >> 4) & 4));
+        by = sy * kBrick + (((t >> 1) & 1) | ((t >> 3) & 2) | ((t >> 5) & 4));
+        bz = sz * kBrick + (((t >> 2) & 1) | ((t >> 4) & 2) | ((t >> 6) & 4));
+        return;
+    }
+    bx = sx * kBrick + t % wx;
+    by = sy * kBrick + (t / wx) % hy;
+    bz = sz * kBrick + t / (wx * hy);
+}
+
 template <class T> struct Sampler<T, 1> {
     GridAxes g;
     i64 S;
-    int xcd = 1;  // XCD-contiguous block order
+    int xcd = 2;    // block order (xcd_block mode)
+    int order = 0;  // tile order: 0 x-fastest rows, 1 bricks
     __device__ inline bool get(i64& s, T& x, T& y, T& z) const {
-        const i64 tile = xcd_block(xcd != 0) * (kBlock / kWave) + threadIdx.x / kWave;
+        // one 4x4x4 tile of samples per wave
+        const i64 tile = xcd_block(xcd) * (kBlock / kWave) + threadIdx.x / kWave;
         const int lane = threadIdx.x & (kWave - 1);
-        const i64 tx = (g.nx + 3) / 4, ty = (g.ny + 3) / 4;
-        const i64 bx = tile % tx, by = (tile / tx) % ty, bz = tile / (tx * ty);
+        const i64 tx = (g.nx + 3) / 4, ty = (g.ny + 3) / 4, tz = (g.nz + 3) / 4;
+        if (tile >= tx * ty * tz) return false;
+        i64 bx, by, bz;
+        if (order) brick_tile(tile, tx, ty, tz, bx, by, bz);
+        else {
+            bx = tile % tx;
+            by = (tile / tx) % ty;
+            bz = tile / (tx * ty);
+        }
         const i64 ix = bx * 4 + (lane & 3), iy = by * 4 + ((lane >> 2) & 3), iz = bz * 4 + (lane >> 4);
         if (ix >= g.nx || iy >= g.ny || iz >= g.nz) return false;
         s = ix + g.nx * (iy + g.ny * iz);

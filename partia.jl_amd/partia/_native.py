@@ -63,7 +63,7 @@ class ph_request(C.Structure):
 
 
 class ph_tune_entry(C.Structure):
-    _fields_ = [("key", C.c_uint64), ("occupancy", C.c_double), ("xcd_map", C.c_int32), ("reserved", C.c_int32),
+    _fields_ = [("key", C.c_uint64), ("occupancy", C.c_double), ("xcd_map", C.c_int32), ("tile_order", C.c_int32),
                 ("best_ms", C.c_double), ("default_ms", C.c_double)]
 
 
@@ -122,7 +122,7 @@ def lib():
         "ph_batch_times": (i32, [vp, vp, vp]),
         "ph_batch_destroy": (i32, [vp]),
         "ph_set_autotune": (i32, [vp, C.c_int]),
-        "ph_set_launch": (i32, [vp, d, i32]),
+        "ph_set_launch": (i32, [vp, d, i32, i32]),
         "ph_autotune_report": (i32, [vp, i32, P(ph_tune_entry), P(i32)]),
         "ph_line_samples_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, i64] + [vp] * 8),
         "ph_line_grid_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, vp, vp, vp, vp, i64, vp, i64,
@@ -201,8 +201,8 @@ class Context:
     def set_autotune(self, enable):
         self.check(lib().ph_set_autotune(self.handle, 1 if enable else 0))
 
-    def set_launch(self, occupancy=1.0, xcd_map=1):
-        self.check(lib().ph_set_launch(self.handle, float(occupancy), int(xcd_map)))
+    def set_launch(self, occupancy=1.0, xcd_map=2, tile_order=0):
+        self.check(lib().ph_set_launch(self.handle, float(occupancy), int(xcd_map), int(tile_order)))
 
     def autotune_report(self):
         """List of dicts, one per tuned workload key."""
@@ -210,7 +210,7 @@ class Context:
         self.check(lib().ph_autotune_report(self.handle, 0, None, C.byref(n)))
         arr = (ph_tune_entry * max(1, n.value))()
         self.check(lib().ph_autotune_report(self.handle, n.value, arr, C.byref(n)))
-        return [{k: getattr(arr[i], k) for k, _ in ph_tune_entry._fields_ if k != "reserved"}
+        return [{k: getattr(arr[i], k) for k, _ in ph_tune_entry._fields_}
                 for i in range(n.value)]
 
     def close(self):

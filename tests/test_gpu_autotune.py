@@ -11,7 +11,8 @@ from partia import synthetic
 
 pytestmark = pytest.mark.gpu
 FULL = dict(sslots=[1, 2], sflags=[0, 1], gslots=[2], dslots=[3, 4, 5], cslots=[3, 4, 5])
-CONFIGS = [(occ, xm) for occ in (1.0, 0.75, 0.5) for xm in (1, 0)]
+CONFIGS = ([(occ, xm, 0) for occ in (1.0, 0.75, 0.5) for xm in (2, 1, 0)]
+           + [(1.0, xm, o) for xm in range(8) for o in (0, 1) if (xm, o) not in ((2, 0), (1, 0), (0, 0))])
 
 
 def plummer_set(ctx, n=60000):
@@ -31,12 +32,12 @@ def test_every_launch_configuration_gives_identical_outputs(backend, precision):
     try:
         ref_g = d.grid(FULL, N.PH_SYMMETRIC, ax, ax, ax, precision=precision)
         ref_p = d.points(FULL, N.PH_GATHER, *P, precision=precision)
-        for occ, xm in CONFIGS:
-            ctx.set_launch(occ, xm)
+        for occ, xm, order in CONFIGS:
+            ctx.set_launch(occ, xm, order)
             assert_same(d.grid(FULL, N.PH_SYMMETRIC, ax, ax, ax, precision=precision), ref_g)
             assert_same(d.points(FULL, N.PH_GATHER, *P, precision=precision), ref_p)
     finally:
-        ctx.set_launch(1.0, 1)
+        ctx.set_launch()
 
 
 def test_autotuner_keeps_results_and_records_one_choice_per_key(backend):
@@ -51,7 +52,7 @@ def test_autotuner_keeps_results_and_records_one_choice_per_key(backend):
         rep = [e for e in ctx.autotune_report() if e["key"] not in before]
         assert len(rep) == 1
         e = rep[0]
-        assert (float(e["occupancy"]), int(e["xcd_map"])) in CONFIGS
+        assert (float(e["occupancy"]), int(e["xcd_map"]), int(e["tile_order"])) in CONFIGS
         assert 0.0 < e["best_ms"] <= e["default_ms"]
         second = d.grid(FULL, N.PH_SYMMETRIC, ax, ax, ax, precision=N.PH_PREC_FP32C)  # uses the choice
         assert len([e for e in ctx.autotune_report() if e["key"] not in before]) == 1
@@ -66,4 +67,8 @@ def test_set_launch_rejects_bad_values(backend):
     with pytest.raises(ArgumentError):
         ctx.set_launch(0.0, 1)
     with pytest.raises(ArgumentError):
-        ctx.set_launch(1.0, 2)
+        ctx.set_launch(1.0, 8)
+    with pytest.raises(ArgumentError):
+        ctx.set_launch(1.0, -1)
+    with pytest.raises(ArgumentError):
+        ctx.set_launch(1.0, 1, 2)
