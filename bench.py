@@ -725,7 +725,13 @@ def main():
     fbytes = 8 if prec == N.PH_PREC_FP64 else 4
     alg_bytes = n_acc * FIELDS_PER_NEIGHBOUR * fbytes + S * (3 + L_OUT) * fbytes
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic("k_bucket_f64" if prec == N.PH_PREC_FP64 else "k_bucket_c32")
+    # the accumulation kernel the call ran: FP32C grids walk one bucket-tree
+    # union per wave (k_tile_c32) unless the per-lane bucket walk is forced
+    if prec == N.PH_PREC_FP64:
+        kname = "k_points" if args.traversal == "lane" else "k_bucket_f64"
+    else:
+        kname = "k_bucket_c32" if args.traversal == "bucket" else "k_tile_c32"
+    traffic, traffic_src = measured_traffic(kname)
     result = {
         "metric": METRIC,
         "value": S / (ms_per_step * 1e-3),
@@ -755,7 +761,7 @@ def main():
         "fp64": fp64,
         "speedup_vs_fp64": (fp64["ms_per_step"] / ms_per_step) if fp64 else None,
         "speedup_vs_fp64_per_leaf_walk": (fp64["per_leaf_walk"]["ms_per_step"] / ms_per_step) if fp64 else None,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "algorithmic_bytes": int(alg_bytes)},
     }

@@ -243,7 +243,10 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
         const unsigned g = sample_blocks(ga, S);
         const int xm = tl_launch.xcd_map;
         const double Kd = support_of<K>();
-        if (ga) k_nearest_bucket<1><<<g, kBlock, 0, s>>>(bv, Sampler<double, 1>{*ga, S, xm, tl_launch.order}, Kd, B->ha.as<double>());
+        if (ga && tl_launch.tile) launch_nearest_tile(B, *ga, S, Kd, B->ha.as<double>(), s);
+        else if (ga)
+            k_nearest_bucket<1><<<g, kBlock, 0, s>>>(bv, Sampler<double, 1>{*ga, S, xm, tl_launch.order}, Kd,
+                                                     B->ha.as<double>());
         else k_nearest_bucket<0><<<g, kBlock, 0, s>>>(bv, Sampler<double, 0>{*pl, S, xm}, Kd, B->ha.as<double>());
         PH_HIP(hipGetLastError());
         ha = B->ha.as<double>();

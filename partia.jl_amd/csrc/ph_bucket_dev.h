@@ -470,63 +470,80 @@ __device__ __attribute__((always_inline)) u64 test_buckets(const BucketView& bv,
 // distances with the bound inflated to cover rounding, so no candidate is lost.
 // Only leaves with d2 <= bound2 are candidates: NaN when there is none (with
 // bound2 = infinity, only for an empty tree).
-__device__ inline double nearest_h_bucket(const BucketView& bv, const double p[3], double bound2 = INFINITY) {
+// Nearest-particle search state: the lexicographic minimum of (d2, leaf
+// index) over the leaves seen so far with d2 <= the initial bound.  The result
+// does not depend on the order in which leaves are offered, so any superset of
+// the leaves the pruned walk must see gives the same answer.
+struct Nearest {
+    static constexpr i64 kNone = INT64_MAX;
+    double best;  // a leaf at exactly the bound is a candidate (k < kNone)
+    i64 bk = kNone;
+    __device__ inline void offer(const double p[3], const LeafRec<double>& L, i64 k) {
+        const double d2 = d2_point(p[0], p[1], p[2], L);
+        if (d2 < best || (d2 == best && k < bk)) {
+            best = d2;
+            bk = k;
+        }
+    }
+    __device__ inline double h(const BucketView& bv) const { return bk != kNone ? bv.leaves[bk].h : (double)NAN; }
+};
+
+__device__ inline float box_d2_32(const BucketRec& R, float px, float py, float pz) {
+    const float q[3] = {px, py, pz};
+    float d2 = 0.f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float g = fmaxf(fmaxf(R.lo[d] - q[d], q[d] - R.hi[d]), 0.f);
+        d2 += g * g;
+    }
+    return d2;
+}
+
+__device__ inline void nearest_scan(const BucketView& bv, const double p[3], i32 link, Nearest& nr) {
+    const i64 lo = bucket_lo(link);
+    const int cnt = bucket_cnt(link);
+    for (int j = 0; j < cnt; ++j) nr.offer(p, bv.leaves[lo + j], lo + j);
+}
+
+// The pruned walk from the root: every bucket whose box may hold a leaf within
+// the current best distance (float32 test inflated past rounding) is scanned.
+__device__ inline void nearest_walk(const BucketView& bv, const double p[3], Nearest& nr) {
     const float px = (float)p[0], py = (float)p[1], pz = (float)p[2];
     const float eps = 2.3841858e-7f * fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz))) + 1e-30f;
-    auto box_d2 = [&](const BucketRec& R) {
-        const float q[3] = {px, py, pz};
-        float d2 = 0.f;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const float g = fmaxf(fmaxf(R.lo[d] - q[d], q[d] - R.hi[d]), 0.f);
-            d2 += g * g;
+    i32 i = 0;
+    for (i64 steps = 2 * (i64)bv.nrec + 8; i < bv.nrec && steps > 0; --steps) {
+        const BucketRec R = bv.recs[i];
+        const float bound = (float)nr.best;
+        const float lim = (sqrtf(bound) * kInflate + eps);
+        const bool keep = !(box_d2_32(R, px, py, pz) > lim * lim * kInflate);
+        if (R.link >= 0) {
+            i = keep ? i + 1 : R.link;
+        } else {
+            if (keep) nearest_scan(bv, p, R.link, nr);
+            ++i;
         }
-        return d2;
-    };
-    constexpr i64 kNone = INT64_MAX;
-    double best = bound2;  // a leaf at exactly the bound is a candidate (k < kNone)
-    i64 bk = kNone;
-    auto scan = [&](i32 link) {
-        const i64 lo = bucket_lo(link);
-        const int cnt = bucket_cnt(link);
-        for (int j = 0; j < cnt; ++j) {
-            const LeafRec<double> L = bv.leaves[lo + j];
-            const double d2 = d2_point(p[0], p[1], p[2], L);
-            const i64 k = lo + j;
-            if (d2 < best || (d2 == best && k < bk)) {
-                best = d2;
-                bk = k;
-            }
-        }
-    };
+    }
+}
+
+__device__ inline double nearest_h_bucket(const BucketView& bv, const double p[3], double bound2 = INFINITY) {
+    const float px = (float)p[0], py = (float)p[1], pz = (float)p[2];
+    Nearest nr{bound2};
     // greedy descent: nearer child first (left child = next record; right child
-    // = the record after the left subtree)
+    // = the record after the left subtree); its bucket seeds the bound
     i32 i = 0;
     for (i64 steps = bv.nrec; steps > 0; --steps) {
         const BucketRec R = bv.recs[i];
         if (R.link < 0) {
-            scan(R.link);
+            nearest_scan(bv, p, R.link, nr);
             break;
         }
         const i32 l = i + 1;
         const BucketRec RL = bv.recs[l];
         const i32 r = RL.link >= 0 ? RL.link : l + 1;
-        i = box_d2(bv.recs[r]) < box_d2(RL) ? r : l;
+        i = box_d2_32(bv.recs[r], px, py, pz) < box_d2_32(RL, px, py, pz) ? r : l;
     }
-    i = 0;
-    for (i64 steps = 2 * (i64)bv.nrec + 8; i < bv.nrec && steps > 0; --steps) {
-        const BucketRec R = bv.recs[i];
-        const float bound = (float)best;
-        const float lim = (sqrtf(bound) * kInflate + eps);
-        const bool keep = !(box_d2(R) > lim * lim * kInflate);
-        if (R.link >= 0) {
-            i = keep ? i + 1 : R.link;
-        } else {
-            if (keep) scan(R.link);
-            ++i;
-        }
-    }
-    return bk != kNone ? bv.leaves[bk].h : (double)NAN;
+    nearest_walk(bv, p, nr);
+    return nr.h(bv);
 }
 
 // Bound of the nearest-h search of a sample that is interpolated with support

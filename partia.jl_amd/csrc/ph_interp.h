@@ -70,6 +70,9 @@ void ensure_buckets(ph_lbvh* B, hipStream_t s);
 // the caller has packed B->rec32 for c32 and computed ha (gather / symmetric).
 void launch_tile_c32(ph_lbvh* B, const CatChunk& c32, int strategy, const GridAxes& ga, i64 S, const double* ha,
                      void* out, u64* counters, hipStream_t s, hipEvent_t* ev);
+// Nearest-particle smoothing length of every grid point (NaN: no particle
+// within Kd max(h)), one wave per tile seeding its lanes' searches (ph_tile.hip).
+void launch_nearest_tile(ph_lbvh* B, const GridAxes& ga, i64 S, double Kd, double* ha, hipStream_t s);
 
 // Adjoint of the line projection of one scalar (cols[qsrc]): grad[6][n] float64
 // (d/dm, d/dA, d/dh, d/dx, d/dy, d/dz, input order) of sum cot[s] out[s];

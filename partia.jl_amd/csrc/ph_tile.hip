@@ -154,6 +154,63 @@ __global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const Bucket
     count_add(counter, accepted);
 }
 
+// Nearest-particle smoothing length of every grid point, one wave per 4x4x4
+// tile.  The wave descends the bucket tree once (scalar loads) toward the
+// centre of its points; the 64 leaves around the bucket it reaches (Morton
+// neighbours) are staged in LDS and offered to every lane, which seeds each
+// lane's bound with a nearby leaf; each lane then runs the pruned walk of
+// nearest_h_bucket from the root.  The result is the lexicographic minimum of
+// (d2, leaf index) over the leaves within the bound whatever leaves are offered
+// first, so it equals the per-lane search's (k_nearest_bucket) bit for bit.
+constexpr int kNnCand = 64;
+__global__ void __launch_bounds__(kBlock) k_nearest_tile(BucketView bv, const BucketRec* __restrict__ recs,
+                                                         Sampler<double, 1> smp, double Kd,
+                                                         double* __restrict__ ha) {
+    __shared__ LeafRec<double> cand[kBlock / kWave][kNnCand];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    i64 s = 0;
+    double p[3] = {0.0, 0.0, 0.0};
+    const bool valid = smp.get(s, p[0], p[1], p[2]);
+    if (__ballot(valid) == 0) return;  // wave-uniform
+    // centre of the wave's valid points (all lanes hold the same value)
+    float c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        float lo = valid ? (float)p[d] : INFINITY, hi = valid ? (float)p[d] : -INFINITY;
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            lo = fminf(lo, __shfl_xor(lo, o, kWave));
+            hi = fmaxf(hi, __shfl_xor(hi, o, kWave));
+        }
+        c[d] = 0.5f * lo + 0.5f * hi;
+    }
+    // greedy descent toward the centre; decisions made wave-uniform
+    i32 i = 0;
+    i32 link = -1;  // bucket 0, one leaf: a valid fallback
+    for (i64 steps = bv.nrec; steps > 0; --steps) {
+        const BucketRec R = recs[i];
+        if (R.link < 0) {
+            link = R.link;
+            break;
+        }
+        const i32 l = i + 1;
+        const BucketRec RL = recs[l];
+        const i32 r = RL.link >= 0 ? RL.link : l + 1;
+        const BucketRec RR = recs[r];
+        i = __builtin_amdgcn_readfirstlane(box_d2_32(RR, c[0], c[1], c[2]) < box_d2_32(RL, c[0], c[1], c[2]) ? r
+                                                                                                             : l);
+    }
+    const i64 n = bv.n;
+    const int m = (int)(n < kNnCand ? n : kNnCand);
+    i64 first = bucket_lo(link) + bucket_cnt(link) / 2 - kNnCand / 2;
+    first = first < 0 ? 0 : (first > n - m ? n - m : first);
+    if (lane < m) cand[wv][lane] = bv.leaves[first + lane];
+    wave_sync();
+    Nearest nr{neighbour_bound2(bv, Kd)};
+    for (int j = 0; j < m; ++j) nr.offer(p, cand[wv][j], first + j);
+    nearest_walk(bv, p, nr);
+    if (valid) ha[s] = nr.h(bv);
+}
+
 template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT>
 void launch_tile_t(const BucketView& bv, const double* rec, const CatChunk& c, const GridAxes& ga, i64 S,
                    const double* ha, float* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
@@ -205,6 +262,16 @@ void launch_tile_k(int strategy, const BucketView& bv, const double* rec, const 
 }
 
 }  // namespace
+
+void launch_nearest_tile(ph_lbvh* B, const GridAxes& ga, i64 S, double Kd, double* ha, hipStream_t s) {
+    const BucketView bv = bucket_view(B);
+    const LaunchCfg cfg = tl_launch;
+    const i64 tiles = ((ga.nx + 3) / 4) * ((ga.ny + 3) / 4) * ((ga.nz + 3) / 4);
+    const unsigned g = (unsigned)((tiles + 3) / 4);
+    hipLaunchKernelGGL(k_nearest_tile, dim3(g), dim3(kBlock), 0, s, bv, bv.recs,
+                       Sampler<double, 1>{ga, S, cfg.xcd_map, cfg.order}, Kd, ha);
+    PH_HIP(hipGetLastError());
+}
 
 void launch_tile_c32(ph_lbvh* B, const CatChunk& c32, int strategy, const GridAxes& ga, i64 S, const double* ha,
                      void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
