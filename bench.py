@@ -214,6 +214,7 @@ def column2048(args):
     ax = synthetic.linrange(0.0, 1.0, nimg)
     S = nimg * nimg
     ctx = N.Context(0)
+    apply_launch(ctx, args)
     if args.traversal == "lane":
         ctx.set_traversal(N.PH_TRAVERSAL_LANE)  # one DFS per ray instead of one per 8 x 8 tile
     L = N.lib()
@@ -309,6 +310,7 @@ def grid64(args):
     ax = synthetic.linrange(0.0, 1.0, ng)
     S = ng ** 3
     ctx = N.Context(0)
+    apply_launch(ctx, args)
     L = N.lib()
     dcols = upload_columns(ctx, data, ["x", "y", "z", "m", "h", "rho", "rho", "A"])
     dax = Dev(ctx, 8 * ng)
@@ -485,6 +487,7 @@ def adjoint1024(args):
     ax = synthetic.linrange(-1.0, 1.0, nimg)
     S = nimg * nimg
     ctx = N.Context(0)
+    apply_launch(ctx, args)
     L = N.lib()
     cat = N.CatalogArgs(sslots=[1], sflags=[0])
     frame = [np.ascontiguousarray(np.array(v, dtype=np.float64)) for v in

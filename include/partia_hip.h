@@ -140,13 +140,16 @@ int ph_set_traversal(ph_context* ctx, int mode);
  *               contiguous range of blocks per XCD; k = 2..7 runs of 2^(k+2)
  *               consecutive blocks dealt round-robin to the XCDs (each XCD's
  *               blocks are neighbours, dense regions spread over every XCD);
+ *               -1 each kernel family's measured best (2 for point and grid
+ *               kernels, 5 for line kernels).  Line kernels use this knob
+ *               only; the autotuner does not time them;
  *   tile_order  grid tiles (4x4x4 samples, one per wavefront) in x-fastest
  *               rows (0) or in bricks of 8x8x8 tiles, Morton order inside (1).
  * With autotuning on, the first call of each workload key (precision,
  * strategy, kernel, catalog shape, grid or list, log2 of the sample and
  * particle counts) times the candidate configurations twice on the call's own
  * data and keeps the fastest for later calls of that key; otherwise the
- * ph_set_launch configuration (default 1, 2, 0) is used. ---- */
+ * ph_set_launch configuration (default 1, -1, 0) is used. ---- */
 typedef struct ph_tune_entry {
     uint64_t key;
     double occupancy;

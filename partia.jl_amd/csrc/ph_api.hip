@@ -123,7 +123,9 @@ LaunchCfg autotuned(ph_context* ctx, ph_lbvh* B, const CatChunk& c, int strategy
     if (it != ctx->tuned.end()) return it->second.cfg;
     // the default first; full occupancy with the block orders that balance
     // dense regions over the XCDs, both tile orders; two lower occupancies
-    std::vector<LaunchCfg> cand{ctx->launch};
+    LaunchCfg first = ctx->launch;
+    first.xcd_map = point_xcd(first);
+    std::vector<LaunchCfg> cand{first};
     auto add = [&](LaunchCfg a) {
         for (const LaunchCfg& b : cand)
             if (a.occupancy == b.occupancy && a.xcd_map == b.xcd_map && a.order == b.order) return;
@@ -328,8 +330,8 @@ int ph_set_autotune(ph_context* ctx, int enable) {
 
 int ph_set_launch(ph_context* ctx, double occupancy, int32_t xcd_map, int32_t tile_order) {
     if (!ctx) return PH_ERR_ARGUMENT;
-    if (!(occupancy > 0.0 && occupancy <= 1.0) || xcd_map < 0 || xcd_map > 7 || tile_order < 0 || tile_order > 1)
-        return fail(ctx, PH_ERR_ARGUMENT, "occupancy must be in (0, 1], xcd_map in 0..7 and tile_order 0 or 1");
+    if (!(occupancy > 0.0 && occupancy <= 1.0) || xcd_map < -1 || xcd_map > 7 || tile_order < 0 || tile_order > 1)
+        return fail(ctx, PH_ERR_ARGUMENT, "occupancy must be in (0, 1], xcd_map in -1..7 and tile_order 0 or 1");
     ctx->launch = LaunchCfg{(float)occupancy, xcd_map, tile_order};
     return PH_OK;
 }
@@ -689,6 +691,7 @@ int ph_line_samples_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, i
         need(S == 0 || (ox && oy && oz && dx && dy && dz && out), PH_ERR_ARGUMENT, "null ray or output pointer");
         LineList ll{ox, oy, oz, dx, dy, dz};
         const void* lut = precision == PH_PREC_FP64 ? ctx->lut64.p : ctx->lut32.p;
+        LaunchScope scope(ctx->launch);  // block order
         run_chunks(ctx, cat, S, [&](const CatChunk& c, int L, u64* counter, hipEvent_t* ev) {
             launch_lines(B, c, strategy, precision, &ll, nullptr, S, L, out, counter, lut,
                          ctx->traversal == PH_TRAVERSAL_WAVE, ctx->stream, ev);
@@ -744,6 +747,7 @@ int ph_line_grid_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int3
         lg.nx = nx;
         lg.ny = ny;
         const void* lut = precision == PH_PREC_FP64 ? ctx->lut64.p : ctx->lut32.p;
+        LaunchScope scope(ctx->launch);  // block order
         run_chunks(ctx, cat, S, [&](const CatChunk& c, int L, u64* counter, hipEvent_t* ev) {
             launch_lines(B, c, strategy, precision, nullptr, &lg, S, L, out, counter, lut,
                          ctx->traversal != PH_TRAVERSAL_LANE, ctx->stream, ev);
@@ -764,6 +768,7 @@ static void run_line_adjoint(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat,
     const void* lut = precision == PH_PREC_FP64 ? ctx->lut64.p : ctx->lut32.p;
     const i32 qsrc = 5 + cat->scalar_slots[0];
     const bool shep = cat->scalar_shepard[0] != 0;
+    LaunchScope scope(ctx->launch);  // block order
     PH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
     launch_lines_adjoint(B, qsrc, shep, precision, ll, lg, S, cot, lut, grad, ctx->stream, &ctx->ev[4]);
     PH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));

@@ -183,13 +183,13 @@ void launch_c32_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const Grid
         auto kern = k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 1>;
         const size_t pad = occupancy_pad(kern, cfg.occupancy);
         if (ev) PH_HIP(hipEventRecord(ev[0], s));
-        kern<<<g, kBlock, pad, s>>>(bv, B->rec32.as<double>(), c, Sampler<double, 1>{*ga, S, cfg.xcd_map, cfg.order}, ha,
+        kern<<<g, kBlock, pad, s>>>(bv, B->rec32.as<double>(), c, Sampler<double, 1>{*ga, S, point_xcd(cfg), cfg.order}, ha,
                                     static_cast<float*>(out), counters);
     } else {
         auto kern = k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 0>;
         const size_t pad = occupancy_pad(kern, cfg.occupancy);
         if (ev) PH_HIP(hipEventRecord(ev[0], s));
-        kern<<<g, kBlock, pad, s>>>(bv, B->rec32.as<double>(), c, Sampler<double, 0>{*pl, S, cfg.xcd_map}, ha,
+        kern<<<g, kBlock, pad, s>>>(bv, B->rec32.as<double>(), c, Sampler<double, 0>{*pl, S, point_xcd(cfg)}, ha,
                                     static_cast<float*>(out), counters);
     }
     PH_HIP(hipGetLastError());
@@ -206,13 +206,13 @@ void launch_f64_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const Grid
         auto kern = k_bucket_f64<K, STRAT, MS, MG, MD, MC, 1>;
         const size_t pad = occupancy_pad(kern, cfg.occupancy);
         if (ev) PH_HIP(hipEventRecord(ev[0], s));
-        kern<<<g, kBlock, pad, s>>>(bv, B->rec.as<double>(), c, Sampler<double, 1>{*ga, S, cfg.xcd_map, cfg.order}, ha,
+        kern<<<g, kBlock, pad, s>>>(bv, B->rec.as<double>(), c, Sampler<double, 1>{*ga, S, point_xcd(cfg), cfg.order}, ha,
                                     static_cast<double*>(out), counters);
     } else {
         auto kern = k_bucket_f64<K, STRAT, MS, MG, MD, MC, 0>;
         const size_t pad = occupancy_pad(kern, cfg.occupancy);
         if (ev) PH_HIP(hipEventRecord(ev[0], s));
-        kern<<<g, kBlock, pad, s>>>(bv, B->rec.as<double>(), c, Sampler<double, 0>{*pl, S, cfg.xcd_map}, ha,
+        kern<<<g, kBlock, pad, s>>>(bv, B->rec.as<double>(), c, Sampler<double, 0>{*pl, S, point_xcd(cfg)}, ha,
                                     static_cast<double*>(out), counters);
     }
     PH_HIP(hipGetLastError());
@@ -241,7 +241,7 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
     if (STRAT != PH_SCATTER) {
         const BucketView bv = bucket_view(B);
         const unsigned g = sample_blocks(ga, S);
-        const int xm = tl_launch.xcd_map;
+        const int xm = point_xcd(tl_launch);
         const double Kd = support_of<K>();
         if (ga && tl_launch.tile) launch_nearest_tile(B, *ga, S, Kd, B->ha.as<double>(), s);
         else if (ga)

@@ -95,15 +95,20 @@ struct BlockCache {
 //   xcd_map    block order: 0 hardware order, 1 one contiguous range of tiles
 //              per XCD (shared L2), 2..7 runs of 2^(xcd_map+2) blocks dealt
 //              round-robin to the XCDs (shared L2 and balanced dense regions);
+//              -1: each kernel family's measured best (runs of 16 blocks for
+//              point / grid kernels, of 128 for line kernels);
 //   order      grid tile order: 0 x-fastest rows, 1 bricks of 8^3 tiles.
 struct LaunchCfg {
     float occupancy = 1.f;
-    int xcd_map = 2;
+    int xcd_map = -1;
     int order = 0;
     int tile = 1;  // FP32C grids: one wave walks its tile's union (0: one walk per lane)
 };
 // Configuration of the entry point running on this thread (set by the entry points).
 inline thread_local LaunchCfg tl_launch{};
+// Block order of a point / grid kernel and of a line kernel under cfg.
+inline int point_xcd(const LaunchCfg& c) { return c.xcd_map < 0 ? 2 : c.xcd_map; }
+inline int line_xcd(const LaunchCfg& c) { return c.xcd_map < 0 ? 5 : c.xcd_map; }
 
 // Cache of the context whose entry point runs on this thread (set by the entry
 // points), or null: then buffers use hipMalloc / hipFree directly.

@@ -125,8 +125,9 @@ template <class T, int K, int STRAT, int SRC> struct LineSampler;
 template <class T, int K, int STRAT> struct LineSampler<T, K, STRAT, 0> {
     LineList ll;
     i64 S;
+    int xcd = 2;  // block order (xcd_block mode)
     __device__ inline bool get(i64& s, Line<T>& l) const {
-        s = xcd_block() * kBlock + threadIdx.x;
+        s = xcd_block(xcd) * kBlock + threadIdx.x;
         if (s >= S) return false;
         l.o[0] = T(ll.ox[s]); l.o[1] = T(ll.oy[s]); l.o[2] = T(ll.oz[s]);
         l.m[0] = T(ll.dx[s]); l.m[1] = T(ll.dy[s]); l.m[2] = T(ll.dz[s]);
@@ -136,8 +137,9 @@ template <class T, int K, int STRAT> struct LineSampler<T, K, STRAT, 0> {
 template <class T, int K, int STRAT> struct LineSampler<T, K, STRAT, 1> {
     LineGrid lg;
     i64 S;
+    int xcd = 2;  // block order (xcd_block mode)
     __device__ inline bool get(i64& s, Line<T>& l) const {
-        const i64 tile = xcd_block() * (kBlock / kWave) + threadIdx.x / kWave;
+        const i64 tile = xcd_block(xcd) * (kBlock / kWave) + threadIdx.x / kWave;
         const int lane = threadIdx.x & (kWave - 1);
         const i64 tx = (lg.nx + 7) / 8;
         const i64 i = (tile % tx) * 8 + (lane & 7), j = (tile / tx) * 8 + (lane >> 3);
@@ -487,7 +489,7 @@ void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const Lin
     const T* tab = inorm + (size_t)K * PH_LUT_N_ENTRIES;
     if (ev) PH_HIP(hipEventRecord(ev[0], s));
     if (lg) {
-        LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S};
+        LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S, line_xcd(tl_launch)};
         const i64 tiles = ((lg->nx + 7) / 8) * ((lg->ny + 7) / 8);
         const unsigned g = (unsigned)((tiles + 3) / 4);
         if (wave)
@@ -496,7 +498,7 @@ void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const Lin
             k_lines<T, K, PH_SCATTER, 1><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out),
                                                               counters);
     } else {
-        LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S};
+        LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S, line_xcd(tl_launch)};
         const unsigned g = (unsigned)((S + kBlock - 1) / kBlock);
         if (wave)
             k_lines_wave<T, K, 0><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out), counters);
@@ -533,12 +535,12 @@ void launch_adjoint_k(ph_lbvh* B, i32 qsrc, const LineList* ll, const LineGrid* 
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[0], s));
     if (lg) {
-        LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S};
+        LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S, line_xcd(tl_launch)};
         const i64 tiles = ((lg->nx + 7) / 8) * ((lg->ny + 7) / 8);
         k_lines_adjoint<T, K, 1, SHEP><<<(unsigned)((tiles + 3) / 4), kBlock, 0, s>>>(tv, cols, qsrc, smp, tab, dq,
                                                                                       cot, order, grad);
     } else {
-        LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S};
+        LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S, line_xcd(tl_launch)};
         k_lines_adjoint<T, K, 0, SHEP><<<(unsigned)((S + kBlock - 1) / kBlock), kBlock, 0, s>>>(
             tv, cols, qsrc, smp, tab, dq, cot, order, grad);
     }

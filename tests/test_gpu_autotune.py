@@ -11,7 +11,7 @@ from partia import synthetic
 
 pytestmark = pytest.mark.gpu
 FULL = dict(sslots=[1, 2], sflags=[0, 1], gslots=[2], dslots=[3, 4, 5], cslots=[3, 4, 5])
-CONFIGS = ([(occ, xm, 0) for occ in (1.0, 0.75, 0.5) for xm in (2, 1, 0)]
+CONFIGS = ([(occ, xm, 0) for occ in (1.0, 0.75, 0.5) for xm in (-1, 2, 1, 0)]
            + [(1.0, xm, o) for xm in range(8) for o in (0, 1) if (xm, o) not in ((2, 0), (1, 0), (0, 0))])
 
 
@@ -69,6 +69,6 @@ def test_set_launch_rejects_bad_values(backend):
     with pytest.raises(ArgumentError):
         ctx.set_launch(1.0, 8)
     with pytest.raises(ArgumentError):
-        ctx.set_launch(1.0, -1)
+        ctx.set_launch(1.0, -2)
     with pytest.raises(ArgumentError):
         ctx.set_launch(1.0, 1, 2)

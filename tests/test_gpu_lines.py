@@ -146,6 +146,32 @@ def test_line_list_wave_same_as_lane(backend, precision):
     assert res[N.PH_TRAVERSAL_WAVE][1] == res[N.PH_TRAVERSAL_LANE][1] > 0
 
 
+def test_line_block_orders_give_identical_images(backend):
+    """The block order (ph_set_launch xcd_map) only moves work between XCDs:
+    image-plane and list projections are bit-identical for every order, with
+    both traversals, and equal to the oracle."""
+    c6, cols = cloud(20000, 95)
+    d = DeviceSet(backend.context, *c6, cols, kernel=0)
+    o = oracle_of(*c6, cols, kernel=0)
+    cat = dict(sslots=[1, 2], sflags=[0, 1])
+    ax, ay = synthetic.linrange(0.0, 1.0, 203), synthetic.linrange(0.0, 1.0, 77)  # 26 x 10 ragged tiles
+    ref = o.line_samples(cat, O.SCATTER, *frame_rays(Z_FRAME, ax, ay))
+    rng = np.random.default_rng(8)
+    org, dirs = rng.random((3, 2500)), rng.normal(size=(3, 2500))
+    dirs /= np.linalg.norm(dirs, axis=0)
+    ctx = d.ctx
+    try:
+        ref_l = d.lines(cat, O.SCATTER, org, dirs)
+        for xm in (-1, 0, 1, 2, 5, 7):
+            ctx.set_launch(1.0, xm)
+            for mode in (N.PH_TRAVERSAL_AUTO, N.PH_TRAVERSAL_LANE, N.PH_TRAVERSAL_WAVE):
+                with traversal(ctx, mode):
+                    assert_same(line_grid(d, cat, Z_FRAME, ax, ay), ref)
+                    assert_same(d.lines(cat, O.SCATTER, org, dirs), ref_l)
+    finally:
+        ctx.set_launch()
+
+
 def test_line_grid_empty_image(backend):
     c6, cols = cloud(100, 3)
     d = DeviceSet(backend.context, *c6, cols)
