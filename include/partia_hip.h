@@ -274,6 +274,32 @@ int ph_line_grid_adjoint_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog*
                                 const double* dir3, const double* ax_dev, int64_t nx, const double* ay_dev, int64_t ny,
                                 const void* cot_dev, double* grad_dev);
 
+/* ---- adjoints of point / grid interpolation (no reference counterpart; the
+ * reverse pass a ChainRulesCore rrule of PointSamples_interpolation /
+ * StructuredGrid_interpolation would call).  For a catalog of exactly one
+ * scalar A (slot s, Shepard flag as in the forward), any strategy, with
+ * v_b = m_b / rho_b and W_ab the strategy's kernel (gather W(r, h_a), scatter
+ * W(r, h_b), symmetric their mean; general_interpolation.jl:1-669):
+ *   out = sum_b A_b v_b W_ab                          (no Shepard flag, e.g. :rho)
+ *   out = sum_b A_b v_b W_ab / sum_b v_b W_ab         (Shepard flag)
+ * grad_dev (6 x n float64, overwritten, rows d/dm, d/dA, d/dh, d/dx, d/dy,
+ * d/dz, columns in the input order of the particles) receives the gradient of
+ * sum_s cot[s] * out[s]; rho_b is held fixed and A_b is the value of column s.
+ * For gather and symmetric, h_a is the smoothing length of the sample's
+ * nearest particle (LinearBVHQueries.jl:118-140), whose h row receives
+ * d out / d h_a.  Samples with a NaN output contribute nothing.  Neighbours
+ * are recomputed with the forward's float64 test; terms are evaluated in
+ * float64 and summed with float64 atomics (sums are order-dependent in the last
+ * bits).  precision: PH_PREC_FP64 (cot_dev float64) or PH_PREC_FP32C (cot_dev
+ * float32), on a tree built from Float64 columns. ---- */
+int ph_point_samples_adjoint_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy,
+                                    int32_t precision, int64_t S, const double* px_dev, const double* py_dev,
+                                    const double* pz_dev, const void* cot_dev, double* grad_dev);
+int ph_structured_grid_adjoint_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy,
+                                      int32_t precision, const double* ax_dev, int64_t nx, const double* ay_dev,
+                                      int64_t ny, const double* az_dev, int64_t nz, const void* cot_dev,
+                                      double* grad_dev);
+
 /* ---- building blocks exposed for parity tests ---- */
 /* LBVH_find_nearest_h (LBVH/LinearBVH/LinearBVHQueries.jl:118-140) per point. */
 int ph_nearest_h(ph_context* ctx, ph_lbvh* bvh, int64_t S, const double* px_host, const double* py_host,

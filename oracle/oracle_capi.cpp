@@ -228,6 +228,25 @@ int oracle_line_adjoint(void* hv, int slot, int shepard, int64_t S, const double
     return 0;
 }
 
+int oracle_point_adjoint(void* hv, int slot, int shepard, int strategy, int64_t S, const double* px,
+                         const double* py, const double* pz, const double* cot, double* grad) {
+    Handle* H = static_cast<Handle*>(hv);
+    auto run = [&](auto& in) {
+        using T = std::decay_t<decltype(in.x[0])>;
+        const int64_t n = (int64_t)in.x.size();
+        std::vector<T> g(6 * (size_t)n, T(0));
+        for (int64_t s = 0; s < S; ++s) {
+            T p[3] = {T(px[s]), T(py[s]), T(pz[s])};
+            point_adjoint<T>(in, slot, shepard != 0, strategy, p, T(cot[s]), g.data());
+        }
+        for (int c = 0; c < 6; ++c)
+            for (int64_t i = 0; i < n; ++i) grad[c * n + (int64_t)in.order[i] - 1] = double(g[c * n + i]);
+    };
+    if (H->precision == 64) run(H->d);
+    else run(H->f);
+    return 0;
+}
+
 // LBVH_find_nearest_h per point; best (1-based leaf index) optional.
 int oracle_nearest_h(void* hv, int64_t S, const double* px, const double* py, const double* pz, double* ha,
                      int64_t* best) {

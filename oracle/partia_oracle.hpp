@@ -897,4 +897,79 @@ void line_adjoint(const Input<T>& in, const LineLUT<T>& lut, int slot, bool shep
     });
 }
 
+// Adjoint of one scalar output of the general interpolation kernel at point p
+// (general_interpolation.jl:1-669, scalar terms of accumulations/
+// scalar_accumulation.jl:6-111 and the epilogue :128-140), for the gradient of
+// cot * out with respect to m_b, A_b, h_b and x_b, y_b, z_b of every particle
+// (rho_b held fixed).  With v_b = m_b / rho_b and W_ab the strategy's kernel
+// (gather W(r, h_a), scatter W(r, h_b), symmetric their mean):
+//   out = N = sum_b A_b v_b W_ab          (no Shepard flag)
+//   out = N / S1, S1 = sum_b v_b W_ab     (Shepard flag)
+// d out = sum_b c_b d(v_b W_ab) (+ dA_b terms), c_b = A_b or (A_b - out) / S1.
+// With W = s h^-3 f(q), q = r / h: dW/dr = s h^-4 f'(q) and
+// dW/dh = -s h^-4 (3 f(q) + q f'(q)); dr/dx_b = -(p - x_b) / r (0 at r = 0).
+// For gather and symmetric, h_a is the smoothing length of p's nearest
+// particle (LinearBVHQueries.jl:118-140), so d out / d h_a goes to that
+// particle's h.  A point without neighbours (NaN output) contributes nothing.
+// g rows: 0 m, 1 A, 2 h, 3 x, 4 y, 5 z (Morton order, like the input).
+template <class T>
+void point_adjoint(const Input<T>& in, int slot, bool shepard, int strategy, const T p[3], T cot, T* g) {
+    const int K = in.kernel;
+    const T Kvalid = kernel_support<T>(K);
+    const T sig = kernel_sigma3<T>(K);
+    const i64 n = (i64)in.x.size();
+    const std::vector<T>& A = in.quant[(size_t)slot - 1];
+    const bool sym = strategy == SYMMETRIC;
+    T ha = T(0);
+    i64 nn = 0;
+    if (strategy != SCATTER) {
+        ha = nearest_h<T>(in.bvh, p, &nn);
+        if (nn == 0) return;
+    }
+    auto dWdr = [&](T r, T h) {
+        const T ih = T(1) / h;
+        return sig * (ih * ih * ih * ih) * kernel_df<T>(K, r * ih);
+    };
+    auto dWdh = [&](T r, T h) {
+        const T ih = T(1) / h, q = r * ih;
+        return -sig * (ih * ih * ih * ih) * (T(3) * kernel_f<T>(K, q) + q * kernel_df<T>(K, q));
+    };
+    auto Wsel = [&](T r, T hb) {
+        if (sym) return T(0.5) * (kernel_W<T>(K, r, ha) + kernel_W<T>(K, r, hb));
+        return kernel_W<T>(K, r, strategy == GATHER ? ha : hb);
+    };
+    const T wa = sym ? T(0.5) : (strategy == GATHER ? T(1) : T(0));  // weight of the h_a kernel
+    const T wb = sym ? T(0.5) : (strategy == SCATTER ? T(1) : T(0));  // weight of the h_b kernel
+    T S1 = T(0), N = T(0), HN = T(0), HS = T(0);
+    point_traversal<T>(in.bvh, p, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
+        const size_t i = (size_t)k - 1;
+        const T r = std::sqrt(d2), v = in.m[i] / in.rho[i];
+        const T W = Wsel(r, hb);
+        S1 += v * W;
+        N += A[i] * v * W;
+        if (wa != T(0)) {
+            const T dh = wa * dWdh(r, ha);
+            HN += A[i] * v * dh;
+            HS += v * dh;
+        }
+    });
+    if (S1 == T(0)) return;
+    const T out = shepard ? N / S1 : N;
+    point_traversal<T>(in.bvh, p, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
+        const size_t i = (size_t)k - 1;
+        const T r = std::sqrt(d2), mb = in.m[i], rb = in.rho[i], v = mb / rb;
+        const T W = Wsel(r, hb);
+        const T c = shepard ? (A[i] - out) / S1 : A[i];
+        g[0 * n + (i64)i] += cot * c * W / rb;
+        g[1 * n + (i64)i] += cot * (shepard ? v * W / S1 : v * W);
+        if (wb != T(0)) g[2 * n + (i64)i] += cot * c * v * (wb * dWdh(r, hb));
+        if (r > T(0)) {
+            const T dr = wa * (wa != T(0) ? dWdr(r, ha) : T(0)) + wb * (wb != T(0) ? dWdr(r, hb) : T(0));
+            const T D[3] = {p[0] - in.x[i], p[1] - in.y[i], p[2] - in.z[i]};
+            for (int a = 0; a < 3; ++a) g[(3 + a) * n + (i64)i] += cot * c * v * dr * (-D[a] / r);
+        }
+    });
+    if (strategy != SCATTER) g[2 * n + (nn - 1)] += cot * (shepard ? (HN - out * HS) / S1 : HN);
+}
+
 }  // namespace oracle

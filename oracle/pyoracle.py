@@ -40,6 +40,7 @@ def lib():
         L.oracle_line_samples.argtypes = [vp, i32, vp, vp, i32, i64] + [vp] * 7 + [vp, i32]
         L.oracle_nearest_h.argtypes = [vp, i64, vp, vp, vp, vp, vp]
         L.oracle_line_adjoint.argtypes = [vp, i32, i32, i64] + [vp] * 6 + [vp, vp]
+        L.oracle_point_adjoint.argtypes = [vp, i32, i32, i32, i64] + [vp] * 3 + [vp, vp]
         L.oracle_neighbors.restype = i64
         L.oracle_neighbors.argtypes = [vp, i32, i32, vp, vp, d, vp, vp, i64]
         for name in ("oracle_kernel_f", "oracle_kernel_df"):
@@ -158,6 +159,16 @@ class OracleSet:
         S = len(o[0])
         g = np.zeros((6, self.n))
         lib().oracle_line_adjoint(self.h, int(slot), int(bool(shepard)), S, *[_p(a) for a in o + m], _p(c), _p(g))
+        return g
+
+    def point_adjoint(self, slot, shepard, strategy, px, py, pz, cot):
+        """6 x n gradients (input order) of sum_s cot[s] * out_s for catalog scalar
+        `slot` at the points (rows m, A, h, x, y, z; rho held fixed)."""
+        px, py, pz, c = _f64(px), _f64(py), _f64(pz), _f64(cot)
+        S = len(px)
+        g = np.zeros((6, self.n))
+        lib().oracle_point_adjoint(self.h, int(slot), int(bool(shepard)), int(strategy), S, _p(px), _p(py), _p(pz),
+                                   _p(c), _p(g))
         return g
 
     def nearest_h(self, px, py, pz):

@@ -1,0 +1,290 @@
+// ph_adjoint.hip -- reverse pass of one scalar output of point / grid
+// interpolation (no reference counterpart: what a ChainRulesCore rrule of
+// PointSamples_interpolation / StructuredGrid_interpolation would call).
+//
+// For a catalog of one scalar A (Shepard flag s) the forward of
+// general_interpolation.jl:1-669 computes, per sample a,
+//   N = sum_b A_b v_b W_ab, S1 = sum_b v_b W_ab, out = s ? N / S1 : N,
+// v_b = m_b / rho_b and W_ab = W(r, h_a) (gather), W(r, h_b) (scatter) or their
+// mean (symmetric), h_a the smoothing length of a's nearest particle.  Given
+// the cotangent g_a of every output, the kernel adds to each particle
+//   d/dm_b  g c_b W_ab / rho_b            d/dA_b  g v_b W_ab (/ S1)
+//   d/dh_b  g c_b v_b dW_ab/dh_b          d/dx_b  g c_b v_b dW/dr (-(x_a - x_b) / r)
+// with c_b = A_b, or (A_b - out) / S1 under Shepard, and to a's nearest
+// particle's h the term g d out / d h_a.  W = s3 h^-3 f(q), q = r / h:
+// dW/dr = s3 h^-4 f'(q), dW/dh = -s3 h^-4 (3 f(q) + q f'(q)).  rho_b is held
+// fixed.  The oracle's point_adjoint states the same sums.
+//
+// One wave walks the bucket tree once for its 64 samples (as k_tile_c32), with
+// the forward's float64 leaf test, so every lane sees exactly its forward
+// neighbours.  Pass 1 recomputes N, S1 and the h_a sums; pass 2 forms each
+// staged leaf's six partials on every lane, sums them over the wave in float64
+// and adds them with one float64 atomic per wave, particle and parameter
+// (so the float64 totals are order-dependent in the last bits).
+#include <algorithm>
+
+#include "ph_bucket_dev.h"
+
+namespace ph {
+namespace {
+
+// Nearest particle of every sample: smoothing length and Morton index (-1 and
+// NaN when no particle lies within K max(h)).
+template <int SRC>
+__global__ void __launch_bounds__(kBlock) k_nearest_index(BucketView bv, Sampler<double, SRC> smp, double Kd,
+                                                          double* __restrict__ ha, i64* __restrict__ nn) {
+    i64 s;
+    double p[3];
+    if (!smp.get(s, p[0], p[1], p[2])) return;
+    const Nearest nr = nearest_bucket(bv, p, neighbour_bound2(bv, Kd));
+    ha[s] = nr.h(bv);
+    nn[s] = nr.bk != Nearest::kNone ? nr.bk : -1;
+}
+
+__global__ void k_zero_grad(double* __restrict__ p, i64 count) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (i64)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// The union walk of k_tile_c32: calls batch(nl) with lidx[0, nl) holding the
+// Morton indices of up to 64 leaves of buckets some live lane keeps, in
+// increasing order over the whole walk.  Wave-uniform throughout.
+template <int STRAT, class Batch>
+__device__ __attribute__((always_inline)) void union_walk(const BucketView& bv, const BucketRec* __restrict__ recs,
+                                                          const Probe<STRAT>& pr, bool live, u32* lidx,
+                                                          Batch&& batch) {
+    const int lane = threadIdx.x & (kWave - 1);
+    i32 i = __ballot(live) ? 0 : bv.nrec;
+    i64 steps = (i64)bv.nrec + 8;  // every record at most once
+    BucketRec R = recs[i];         // one padding record follows the tree
+    for (;;) {
+        int nl = 0;
+        while (i < bv.nrec && steps > 0) {
+            --steps;
+            const i32 jmp = R.link >= 0 ? R.link : i + 1;
+            const BucketRec Rn = recs[i + 1], Rj = recs[jmp];
+            const bool keep = __ballot(live && pr.keep(R)) != 0;
+            if (R.link >= 0) {
+                i = __builtin_amdgcn_readfirstlane(keep ? i + 1 : R.link);
+                R = keep ? Rn : Rj;
+                continue;
+            }
+            if (keep) {
+                const int cnt = bucket_cnt(R.link);
+                if (nl + cnt > kWave) {  // starts the next batch
+                    ++steps;
+                    break;
+                }
+                if (lane < cnt) lidx[nl + lane] = (u32)(bucket_lo(R.link) + lane);
+                nl += cnt;
+            }
+            i = __builtin_amdgcn_readfirstlane(i + 1);
+            R = Rn;
+        }
+        if (nl == 0) break;
+        wave_sync();
+        batch(nl);
+        wave_sync();  // lidx is rewritten by the next batch
+        if (!(i < bv.nrec && steps > 0)) break;
+    }
+}
+
+template <int K> __device__ inline double dW_dr(double r, double h) {
+    const double ih = 1.0 / h, ih2 = ih * ih;
+    return sigma3<double, K>() * (ih2 * ih2) * shape_df<double, K>(r * ih);
+}
+template <int K> __device__ inline double dW_dh(double r, double h) {
+    const double ih = 1.0 / h, ih2 = ih * ih, q = r * ih;
+    return -sigma3<double, K>() * (ih2 * ih2) * (3.0 * shape_f<double, K>(q) + q * shape_df<double, K>(q));
+}
+
+template <int K, int STRAT, int SRC, bool SHEP, class CT>
+__global__ void __launch_bounds__(kBlock) k_points_adjoint(BucketView bv, const BucketRec* __restrict__ recs,
+                                                           const LeafRec<double>* __restrict__ leaves,
+                                                           const double* __restrict__ cols, i32 qsrc,
+                                                           Sampler<double, SRC> smp, const double* __restrict__ ha_s,
+                                                           const i64* __restrict__ nn_s, const CT* __restrict__ cot,
+                                                           const u32* __restrict__ order, double* __restrict__ grad) {
+    __shared__ u32 lidx_all[kBlock];
+    u32* lidx = lidx_all + (threadIdx.x / kWave) * kWave;
+    i64 s = 0;
+    double p[3] = {0.0, 0.0, 0.0};
+    const bool valid = smp.get(s, p[0], p[1], p[2]);
+    if (__ballot(valid) == 0) return;  // wave-uniform; no block barrier follows
+    const double Kd = support_of<K>();
+    const i64 n = bv.n;
+    double ha = 0.0;
+    i64 nn = -1;
+    if constexpr (STRAT != PH_SCATTER) {
+        if (valid) {
+            ha = ha_s[s];
+            nn = nn_s[s];
+        }
+    }
+    const bool live = valid && (STRAT == PH_SCATTER || nn >= 0);
+    if (!live) ha = 0.0;
+    const double ra = Kd * ha, r2a = ra * ra;  // the forward's radius
+    Probe<STRAT> pr;
+    pr.init(p, Kd, ra);
+    constexpr double wa = STRAT == PH_SYMMETRIC ? 0.5 : (STRAT == PH_GATHER ? 1.0 : 0.0);  // h_a kernel weight
+    constexpr double wb = STRAT == PH_SYMMETRIC ? 0.5 : (STRAT == PH_SCATTER ? 1.0 : 0.0);  // h_b kernel weight
+    const double* __restrict__ mcol = cols + 3 * n;
+    const double* __restrict__ rcol = cols + 5 * n;
+    const double* __restrict__ acol = cols + (i64)qsrc * n;
+
+    // pass 1: the forward's sums and the h_a sums
+    double S1 = 0.0, N = 0.0, HN = 0.0, HS = 0.0;
+    union_walk<STRAT>(bv, recs, pr, live, lidx, [&](int nl) {
+        for (int j = 0; j < nl; ++j) {
+            const u32 k = __builtin_amdgcn_readfirstlane(lidx[j]);
+            const LeafRec<double> L = leaves[k];
+            const double d2 = d2_point(p[0], p[1], p[2], L);
+            const bool take = live && d2 <= radius2<double, STRAT>(Kd, r2a, L.h);
+            const double r = sqrt(d2), v = mcol[k] / rcol[k], A = acol[k];
+            double W = 0.0;
+            if constexpr (wa != 0.0) W += wa * kernel_W<double, K>(r, take ? ha : 1.0);
+            if constexpr (wb != 0.0) W += wb * kernel_W<double, K>(r, L.h);
+            S1 += take ? v * W : 0.0;
+            N += take ? A * v * W : 0.0;
+            if constexpr (wa != 0.0) {
+                const double dh = wa * dW_dh<K>(r, take ? ha : 1.0);
+                HN += take ? A * v * dh : 0.0;
+                HS += take ? v * dh : 0.0;
+            }
+        }
+    });
+    const bool has = live && S1 != 0.0;  // otherwise the output is NaN: no gradient
+    const double g = has ? (double)cot[s] : 0.0;
+    const double out = has ? (SHEP ? N / S1 : N) : 0.0;
+    const double iS1 = has && SHEP ? 1.0 / S1 : 1.0;
+
+    // pass 2: per staged leaf, six partials summed over the wave
+    union_walk<STRAT>(bv, recs, pr, live, lidx, [&](int nl) {
+        for (int j = 0; j < nl; ++j) {
+            const u32 k = __builtin_amdgcn_readfirstlane(lidx[j]);
+            const LeafRec<double> L = leaves[k];
+            const double d2 = d2_point(p[0], p[1], p[2], L);
+            const bool take = has && g != 0.0 && d2 <= radius2<double, STRAT>(Kd, r2a, L.h);
+            if (__ballot(take) == 0) continue;  // wave-uniform
+            const double r = sqrt(d2), mb = mcol[k], rb = rcol[k], v = mb / rb, A = acol[k];
+            const double hA = take ? ha : 1.0;
+            double W = 0.0, dr = 0.0;
+            if constexpr (wa != 0.0) {
+                W += wa * kernel_W<double, K>(r, hA);
+                dr += wa * dW_dr<K>(r, hA);
+            }
+            if constexpr (wb != 0.0) {
+                W += wb * kernel_W<double, K>(r, L.h);
+                dr += wb * dW_dr<K>(r, L.h);
+            }
+            const double c = SHEP ? (A - out) * iS1 : A;
+            const double gc = g * c;
+            const double dhb = wb != 0.0 ? gc * v * (wb * dW_dh<K>(r, L.h)) : 0.0;
+            const double fx = r > 0.0 ? -gc * v * dr / r : 0.0;
+            // lanes not taking the leaf contribute exact zeros (selected, not
+            // multiplied: their terms may be inf or NaN)
+            double sum[6];
+            sum[0] = wave_sum(take ? gc * W / rb : 0.0);
+            sum[1] = wave_sum(take ? g * v * W * iS1 : 0.0);
+            sum[2] = wave_sum(take ? dhb : 0.0);
+            sum[3] = wave_sum(take ? fx * (p[0] - L.x) : 0.0);
+            sum[4] = wave_sum(take ? fx * (p[1] - L.y) : 0.0);
+            sum[5] = wave_sum(take ? fx * (p[2] - L.z) : 0.0);
+            if ((threadIdx.x & (kWave - 1)) == 0) {
+                const u32 src = order[k];
+#pragma unroll
+                for (int q = 0; q < 6; ++q)
+                    if (sum[q] != 0.0) unsafeAtomicAdd(&grad[(i64)q * n + src], sum[q]);
+            }
+        }
+    });
+    // d out / d h_a to the nearest particle's h
+    if constexpr (wa != 0.0) {
+        if (has && g != 0.0) {
+            const double dha = g * (SHEP ? (HN - out * HS) * iS1 : HN);
+            if (dha != 0.0) unsafeAtomicAdd(&grad[2 * n + order[nn]], dha);
+        }
+    }
+}
+
+template <int K, int STRAT, bool SHEP, class CT>
+void launch_adjoint_t(ph_lbvh* B, i32 qsrc, const PointList* pl, const GridAxes* ga, i64 S, const CT* cot,
+                      double* grad, hipStream_t s, hipEvent_t* ev) {
+    const BucketView bv = bucket_view(B);
+    const LaunchCfg cfg = tl_launch;
+    const int xm = point_xcd(cfg);
+    const double Kd = support_of<K>();
+    unsigned g = (unsigned)((S + kBlock - 1) / kBlock);
+    if (ga) {
+        const i64 tiles = ((ga->nx + 3) / 4) * ((ga->ny + 3) / 4) * ((ga->nz + 3) / 4);
+        g = (unsigned)((tiles + 3) / 4);
+    }
+    double* ha = nullptr;
+    i64* nn = nullptr;
+    if (STRAT != PH_SCATTER) {
+        B->ha.alloc(sizeof(double) * (size_t)S);
+        B->nn.alloc(sizeof(i64) * (size_t)S);
+        ha = B->ha.as<double>();
+        nn = B->nn.as<i64>();
+        if (ga) k_nearest_index<1><<<g, kBlock, 0, s>>>(bv, Sampler<double, 1>{*ga, S, xm, cfg.order}, Kd, ha, nn);
+        else k_nearest_index<0><<<g, kBlock, 0, s>>>(bv, Sampler<double, 0>{*pl, S, xm}, Kd, ha, nn);
+        PH_HIP(hipGetLastError());
+    }
+    const double* cols = B->cols.as<double>();
+    const u32* order = B->order.as<u32>();
+    if (ev) PH_HIP(hipEventRecord(ev[0], s));
+    if (ga)
+        k_points_adjoint<K, STRAT, 1, SHEP, CT><<<g, kBlock, 0, s>>>(bv, bv.recs, bv.leaves, cols, qsrc,
+                                                                     Sampler<double, 1>{*ga, S, xm, cfg.order}, ha,
+                                                                     nn, cot, order, grad);
+    else
+        k_points_adjoint<K, STRAT, 0, SHEP, CT><<<g, kBlock, 0, s>>>(bv, bv.recs, bv.leaves, cols, qsrc,
+                                                                     Sampler<double, 0>{*pl, S, xm}, ha, nn, cot,
+                                                                     order, grad);
+    PH_HIP(hipGetLastError());
+    if (ev) PH_HIP(hipEventRecord(ev[1], s));
+}
+
+template <int K, bool SHEP, class CT>
+void launch_adjoint_s(ph_lbvh* B, int strategy, i32 qsrc, const PointList* pl, const GridAxes* ga, i64 S,
+                      const CT* cot, double* grad, hipStream_t s, hipEvent_t* ev) {
+    switch (strategy) {
+    case PH_GATHER: launch_adjoint_t<K, PH_GATHER, SHEP, CT>(B, qsrc, pl, ga, S, cot, grad, s, ev); break;
+    case PH_SCATTER: launch_adjoint_t<K, PH_SCATTER, SHEP, CT>(B, qsrc, pl, ga, S, cot, grad, s, ev); break;
+    default: launch_adjoint_t<K, PH_SYMMETRIC, SHEP, CT>(B, qsrc, pl, ga, S, cot, grad, s, ev); break;
+    }
+}
+
+template <bool SHEP, class CT>
+void launch_adjoint_k(ph_lbvh* B, int strategy, i32 qsrc, const PointList* pl, const GridAxes* ga, i64 S,
+                      const CT* cot, double* grad, hipStream_t s, hipEvent_t* ev) {
+    switch (B->kernel) {
+    case PH_KERNEL_M4: launch_adjoint_s<PH_KERNEL_M4, SHEP, CT>(B, strategy, qsrc, pl, ga, S, cot, grad, s, ev); break;
+    case PH_KERNEL_M5: launch_adjoint_s<PH_KERNEL_M5, SHEP, CT>(B, strategy, qsrc, pl, ga, S, cot, grad, s, ev); break;
+    case PH_KERNEL_M6: launch_adjoint_s<PH_KERNEL_M6, SHEP, CT>(B, strategy, qsrc, pl, ga, S, cot, grad, s, ev); break;
+    case PH_KERNEL_C2: launch_adjoint_s<PH_KERNEL_C2, SHEP, CT>(B, strategy, qsrc, pl, ga, S, cot, grad, s, ev); break;
+    case PH_KERNEL_C4: launch_adjoint_s<PH_KERNEL_C4, SHEP, CT>(B, strategy, qsrc, pl, ga, S, cot, grad, s, ev); break;
+    default: launch_adjoint_s<PH_KERNEL_C6, SHEP, CT>(B, strategy, qsrc, pl, ga, S, cot, grad, s, ev); break;
+    }
+}
+
+}  // namespace
+
+void launch_points_adjoint(ph_lbvh* B, int strategy, i32 qsrc, bool shepard, int precision, const PointList* pl,
+                           const GridAxes* ga, i64 S, const void* cot, double* grad, hipStream_t s, hipEvent_t* ev) {
+    const i64 n = B->n;
+    k_zero_grad<<<(unsigned)std::min<i64>((6 * n + kBlock - 1) / kBlock, 4096), kBlock, 0, s>>>(grad, 6 * n);
+    PH_HIP(hipGetLastError());
+    if (S <= 0) return;
+    ensure_buckets(B, s);
+    if (precision == PH_PREC_FP64) {
+        auto c = static_cast<const double*>(cot);
+        if (shepard) launch_adjoint_k<true, double>(B, strategy, qsrc, pl, ga, S, c, grad, s, ev);
+        else launch_adjoint_k<false, double>(B, strategy, qsrc, pl, ga, S, c, grad, s, ev);
+    } else {
+        auto c = static_cast<const float*>(cot);
+        if (shepard) launch_adjoint_k<true, float>(B, strategy, qsrc, pl, ga, S, c, grad, s, ev);
+        else launch_adjoint_k<false, float>(B, strategy, qsrc, pl, ga, S, c, grad, s, ev);
+    }
+}
+
+}  // namespace ph

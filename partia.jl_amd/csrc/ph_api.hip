@@ -818,6 +818,56 @@ int ph_line_grid_adjoint_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* c
     });
 }
 
+// Adjoint of a one-scalar point / grid interpolation (shared by both forms).
+static void run_points_adjoint(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                               const PointList* pl, const GridAxes* ga, i64 S, const void* cot, double* grad) {
+    need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
+    need(strategy >= PH_GATHER && strategy <= PH_SYMMETRIC, PH_ERR_ARGUMENT, "unknown strategy");
+    check_catalog(B, cat);
+    need(cat->n_scalar == 1 && cat->n_grad == 0 && cat->n_div == 0 && cat->n_curl == 0, PH_ERR_ARGUMENT,
+         "the point adjoint takes a catalog of exactly one scalar");
+    check_precision(B, precision);
+    need(precision != PH_PREC_FP32, PH_ERR_ARGUMENT, "the point adjoint runs in PH_PREC_FP64 or PH_PREC_FP32C");
+    need(grad != nullptr && (S == 0 || cot != nullptr), PH_ERR_ARGUMENT, "null cotangent or gradient pointer");
+    const i32 qsrc = 5 + cat->scalar_slots[0];
+    const bool shep = cat->scalar_shepard[0] != 0;
+    LaunchScope scope(ctx->launch);  // block order
+    PH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    launch_points_adjoint(B, strategy, qsrc, shep, precision, pl, ga, S, cot, grad, ctx->stream, &ctx->ev[4]);
+    PH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    PH_HIP(hipEventSynchronize(ctx->ev[1]));
+    ctx->stats.interp_ms = elapsed(ctx, 0, 1);
+    ctx->stats.kernel_ms = S > 0 ? elapsed(ctx, 4, 5) : 0.0;
+    ctx->stats.samples = S;
+    ctx->stats.accepted = -1;
+    ctx->stats.rechecks = -1;
+}
+
+int ph_point_samples_adjoint_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy,
+                                    int32_t precision, int64_t S, const double* px, const double* py,
+                                    const double* pz, const void* cot, double* grad) {
+    return guarded(ctx, [&] {
+        need(S >= 0, PH_ERR_ARGUMENT, "negative sample count");
+        need(S == 0 || (px && py && pz), PH_ERR_ARGUMENT, "null sample pointer");
+        PointList pl{px, py, pz};
+        run_points_adjoint(ctx, B, cat, strategy, precision, &pl, nullptr, S, cot, grad);
+        return PH_OK;
+    });
+}
+
+int ph_structured_grid_adjoint_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy,
+                                      int32_t precision, const double* ax, int64_t nx, const double* ay, int64_t ny,
+                                      const double* az, int64_t nz, const void* cot, double* grad) {
+    return guarded(ctx, [&] {
+        need(nx >= 0 && ny >= 0 && nz >= 0, PH_ERR_ARGUMENT, "negative grid size");
+        const i64 S = nx * ny * nz;
+        need(S == 0 || (ax && ay && az), PH_ERR_ARGUMENT, "null axis pointer");
+        GridAxes ga{ax, ay, az, nx, ny, nz};
+        run_points_adjoint(ctx, B, cat, strategy, precision, nullptr, &ga, S, cot, grad);
+        return PH_OK;
+    });
+}
+
 int ph_nearest_h(ph_context* ctx, ph_lbvh* B, int64_t S, const double* px, const double* py, const double* pz,
                  double* ha) {
     return guarded(ctx, [&] {

@@ -525,7 +525,7 @@ __device__ inline void nearest_walk(const BucketView& bv, const double p[3], Nea
     }
 }
 
-__device__ inline double nearest_h_bucket(const BucketView& bv, const double p[3], double bound2 = INFINITY) {
+__device__ inline Nearest nearest_bucket(const BucketView& bv, const double p[3], double bound2 = INFINITY) {
     const float px = (float)p[0], py = (float)p[1], pz = (float)p[2];
     Nearest nr{bound2};
     // greedy descent: nearer child first (left child = next record; right child
@@ -543,7 +543,10 @@ __device__ inline double nearest_h_bucket(const BucketView& bv, const double p[3
         i = box_d2_32(bv.recs[r], px, py, pz) < box_d2_32(RL, px, py, pz) ? r : l;
     }
     nearest_walk(bv, p, nr);
-    return nr.h(bv);
+    return nr;
+}
+__device__ inline double nearest_h_bucket(const BucketView& bv, const double p[3], double bound2 = INFINITY) {
+    return nearest_bucket(bv, p, bound2).h(bv);
 }
 
 // Bound of the nearest-h search of a sample that is interpolated with support

@@ -225,6 +225,7 @@ struct ph_lbvh {
     ph::DevBuf leaf_escape;  // i32 [n]
     ph::DevBuf rec;          // packed per-particle records of the last interpolation call
     ph::DevBuf ha;           // per-sample smoothing length of the nearest particle
+    ph::DevBuf nn;           // per-sample Morton index of the nearest particle (adjoints)
     // Bucket tree (built on first use): the LBVH in DFS preorder with every
     // subtree of at most 4 leaves collapsed into one bucket.
     bool buckets_ready = false;
@@ -236,7 +237,7 @@ struct ph_lbvh {
     // valid for ph_lbvh_destroy only).
     void release_device() {
         for (ph::DevBuf* b : {&cols, &codes, &order, &left, &right, &escape, &parent, &range, &nodes, &leaves,
-                              &leaf_escape, &rec, &ha, &brecs, &rec32})
+                              &leaf_escape, &rec, &ha, &nn, &brecs, &rec32})
             b->release();
         buckets_ready = false;
     }

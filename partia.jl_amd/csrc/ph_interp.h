@@ -70,6 +70,12 @@ void ensure_buckets(ph_lbvh* B, hipStream_t s);
 // the caller has packed B->rec32 for c32 and computed ha (gather / symmetric).
 void launch_tile_c32(ph_lbvh* B, const CatChunk& c32, int strategy, const GridAxes& ga, i64 S, const double* ha,
                      void* out, u64* counters, hipStream_t s, hipEvent_t* ev);
+// Adjoint of one scalar output of point (pl) or grid (ga) interpolation
+// (ph_adjoint.hip): grad[6][n] float64, overwritten, input order; cot holds S
+// values of the forward's output type (double for PH_PREC_FP64, float for
+// PH_PREC_FP32C).  qsrc: the scalar's column in B->cols.
+void launch_points_adjoint(ph_lbvh* B, int strategy, i32 qsrc, bool shepard, int precision, const PointList* pl,
+                           const GridAxes* ga, i64 S, const void* cot, double* grad, hipStream_t s, hipEvent_t* ev);
 // Nearest-particle smoothing length of every grid point (NaN: no particle
 // within Kd max(h)), one wave per tile seeding its lanes' searches (ph_tile.hip).
 void launch_nearest_tile(ph_lbvh* B, const GridAxes& ga, i64 S, double Kd, double* ha, hipStream_t s);

@@ -305,12 +305,6 @@ __global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const T* 
     count_add(counter, valid ? accepted : 0);
 }
 
-template <class T> __device__ inline T wave_sum(T v) {
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
-    return v;
-}
-
 // Adjoint of the projection of one catalog scalar A (no reference
 // counterpart): with w_b = m_b I(q_b) / (h_b rho_b), q_b = r_b / h_b and
 // out = sum A_b w_b (or, Shepard, sum A_b w_b / sum w_b), every ray adds

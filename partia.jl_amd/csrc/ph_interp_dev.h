@@ -450,6 +450,21 @@ template <class T> struct Sampler<T, 1> {
     }
 };
 
+// Cross-lane LDS exchange within one wave: writes of every lane are complete
+// and visible to the wave's other lanes after this point.
+__device__ __attribute__((always_inline)) inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sum over the wave's 64 lanes (every lane gets the total).
+template <class T> __device__ inline T wave_sum(T v) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
 __device__ inline void count_add(u64* counter, u64 v) {
     if (!counter) return;
     // wave reduction, one atomic per wave

@@ -23,14 +23,6 @@
 namespace ph {
 namespace {
 
-// Cross-lane LDS exchange within one wave: writes of every lane are complete
-// and visible to the wave's other lanes after this point.
-__device__ __attribute__((always_inline)) inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Dynamic LDS of k_tile_c32 for record width rw: per wave 64 staged leaves of
 // (x, y, z, h, record) doubles, then 64 leaf indices per wave.
 inline size_t tile_lds_bytes(int rw) {

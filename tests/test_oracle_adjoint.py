@@ -81,3 +81,74 @@ def test_line_adjoint_is_linear_in_the_cotangent_and_local():
     far = o.line_adjoint(2, False, [np.full(1, 5.0), np.full(1, 5.0), np.zeros(1)],
                          [np.zeros(1), np.zeros(1), np.ones(1)], np.ones(1))
     assert not far.any()
+
+
+# ---- scalar outputs of the general interpolation kernel at points ----
+
+STRATS = {"gather": O.GATHER, "scatter": O.SCATTER, "symmetric": O.SYMMETRIC}
+
+
+def points(S, seed):
+    return np.random.default_rng(seed).random((3, S)) * 0.8 + 0.1
+
+
+def point_forward(p, kernel, strategy, shepard, pts, cot):
+    x, y, z, m, h, rho, A = p
+    o = O.OracleSet(x, y, z, m, h, rho, [rho, A], kernel=kernel)
+    out = o.point_samples(dict(sslots=[2], sflags=[int(shepard)]), strategy, *pts)[0]
+    out = np.where(np.isnan(out), 0.0, out)
+    return float(np.dot(cot, out))
+
+
+@pytest.mark.parametrize("strategy", list(STRATS))
+@pytest.mark.parametrize("name,shepard", [("M4", False), ("M4", True), ("C4", True), ("M6", False)])
+def test_point_adjoint_matches_finite_differences(strategy, name, shepard):
+    """d/d(m, A, h, x, y, z) of sum_s cot_s out_s against central differences of
+    the oracle's forward; for gather and symmetric the h row includes the
+    dependence of h_a on the nearest particle's h."""
+    kernel, st = KERNELS[name], STRATS[strategy]
+    p = cloud(300, 15 + kernel)
+    n = len(p[0])
+    pts = points(120, 19)
+    cot = np.random.default_rng(4).normal(size=pts.shape[1])
+    o = O.OracleSet(*p[:6], [p[5], p[6]], kernel=kernel)
+    g = o.point_adjoint(2, shepard, st, *pts, cot)
+    assert g.shape == (6, n)
+    rng = np.random.default_rng(12)
+    # particles the points reach (nonzero rows) plus a few others
+    reached = np.flatnonzero(np.abs(g).sum(axis=0))
+    assert len(reached) > 20
+    picks = np.concatenate([rng.choice(reached, 20, replace=False), rng.choice(n, 4, replace=False)])
+    ha, best = o.nearest_h(*pts)
+    if st != O.SCATTER:  # include nearest particles, whose h moves h_a
+        order = o.tree()["order"].astype(np.int64)  # 1-based input index per leaf
+        picks = np.concatenate([picks, order[best[:4] - 1] - 1])
+    params = [(0, 3, "rel"), (1, 6, "rel"), (2, 4, "rel"), (3, 0, "h"), (4, 1, "h"), (5, 2, "h")]
+    for row, pi, kind in params:
+        scale = max(np.sqrt(np.mean(g[row] ** 2)), 1e-300)
+        for b in picks:
+            base = p[pi][b]
+            eps = 1e-6 * (abs(base) if kind == "rel" else p[4][b])
+            q_plus = [a.copy() for a in p]
+            q_minus = [a.copy() for a in p]
+            q_plus[pi][b] = base + eps
+            q_minus[pi][b] = base - eps
+            fd = (point_forward(q_plus, kernel, st, shepard, pts, cot) -
+                  point_forward(q_minus, kernel, st, shepard, pts, cot)) / (2 * eps)
+            assert abs(fd - g[row, b]) <= 2e-4 * scale + 1e-6 * abs(g[row, b]), (row, b, fd, g[row, b], scale)
+
+
+def test_point_adjoint_linear_in_the_cotangent_and_empty_points():
+    p = cloud(200, 3)
+    o = O.OracleSet(*p[:6], [p[5], p[6]], kernel=0)
+    pts = points(50, 4)
+    rng = np.random.default_rng(1)
+    c1, c2 = rng.normal(size=50), rng.normal(size=50)
+    for st in STRATS.values():
+        g1 = o.point_adjoint(2, True, st, *pts, c1)
+        g2 = o.point_adjoint(2, True, st, *pts, c2)
+        g12 = o.point_adjoint(2, True, st, *pts, 2.0 * c1 - 3.0 * c2)
+        np.testing.assert_allclose(g12, 2.0 * g1 - 3.0 * g2, rtol=1e-12, atol=1e-12 * np.abs(g12).max())
+        # a point far from every particle (NaN output) contributes nothing
+        far = o.point_adjoint(2, True, st, [5.0], [5.0], [5.0], [1.0])
+        assert not far.any()
