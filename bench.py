@@ -85,7 +85,7 @@ def parse():
                          "turns the autotuner off")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
-    ap.add_argument("--workload", choices=["grid256", "column2048", "grid64", "batch512", "adjoint1024"],
+    ap.add_argument("--workload", choices=["grid256", "column2048", "grid64", "batch512", "adjoint1024", "gridadjoint256"],
                     default="grid256")
     return ap.parse_args()
 
@@ -611,6 +611,108 @@ def adjoint1024(args):
     print(json.dumps(result))
 
 
+def gridadjoint256(args):
+    """Forward + adjoint of the Shepard-normalised scalar A of the 2e6-particle
+    Plummer sphere on the 256^3 grid (C4, symmetric), float64."""
+    npart, ng = NPART, NGRID
+    print("bench: generating %d Plummer particles" % npart, file=sys.stderr, flush=True)
+    data = synthetic.plummer_sphere(npart, seed=2)
+    axes = synthetic.linrange(-1.0, 1.0, ng)
+    S = ng ** 3
+    ctx = N.Context(0)
+    apply_launch(ctx, args)
+    L = N.lib()
+    cat = N.CatalogArgs(sslots=[2], sflags=[1])
+    dax = Dev(ctx, 8 * ng)
+    dax.upload(axes)
+    prec = N.PH_PREC_FP64
+
+    def build(dct):
+        dc = upload_columns(ctx, dct, ["x", "y", "z", "m", "h", "rho", "rho", "A"])
+        q = (C.c_void_p * 2)(dc[6].p.value, dc[7].p.value)
+        h = C.c_void_p()
+        ctx.check(L.ph_lbvh_build_device(ctx.handle, npart, *[d.p for d in dc[:6]], 2, q, KERNEL_C4, 63, prec,
+                                         C.byref(h)))
+        for d in dc:
+            d.free()
+        return h
+
+    dout = Dev(ctx, 8 * S)
+    cot = synthetic.Stream(77).normal(S)
+    dcot = Dev(ctx, 8 * S)
+    dcot.upload(cot)
+    dgrad = Dev(ctx, 8 * 6 * npart)
+
+    def forward(h):
+        ctx.check(L.ph_structured_grid_device(ctx.handle, h, cat.ref(), N.PH_SYMMETRIC, prec, dax.p, ng, dax.p, ng,
+                                              dax.p, ng, dout.p))
+        return ctx.stats()["interp_ms"]
+
+    def backward(h):
+        ctx.check(L.ph_structured_grid_adjoint_device(ctx.handle, h, cat.ref(), N.PH_SYMMETRIC, prec, dax.p, ng,
+                                                      dax.p, ng, dax.p, ng, dcot.p, dgrad.p))
+        return ctx.stats()["interp_ms"]
+
+    def get(dev, count):
+        out = np.zeros(count)
+        ctx.check(L.ph_memcpy_dtoh(ctx.handle, out.ctypes.data_as(C.c_void_p), dev.p, out.nbytes))
+        return out
+
+    h = build(data)
+
+    def step():
+        return {"forward": forward(h), "backward": backward(h)}
+
+    ms, res = timed_steps(ctx, step, args.steps, args.warmup, "gridadjoint256")
+    fwd = float(np.mean([r["forward"] for r in res]))
+    bwd = float(np.mean([r["backward"] for r in res]))
+    g = get(dgrad, 6 * npart).reshape(6, npart)
+    L.ph_lbvh_destroy(h)
+
+    def objective(dct):  # sum_s cot_s out_s over samples with neighbours
+        hh = build(dct)
+        forward(hh)
+        L.ph_lbvh_destroy(hh)
+        return float(np.nansum(cot * get(dout, S)))
+
+    fd_err, fd_checks = {}, 0
+    if not args.no_fp64_compare:
+        rng = np.random.default_rng(256)
+        picks = rng.choice(np.flatnonzero(np.abs(g[0]) > 0), 12, replace=False)
+        for row, key in ((0, "m"), (1, "A"), (2, "h"), (3, "x")):
+            scale = float(np.sqrt(np.mean(g[row] ** 2)))
+            worst = 0.0
+            for b in picks:
+                eps = 1e-5 * (abs(data[key][b]) if key != "x" else data["h"][b])
+                dp, dmn = dict(data), dict(data)
+                dp[key] = data[key].copy()
+                dmn[key] = data[key].copy()
+                dp[key][b] += eps
+                dmn[key][b] -= eps
+                fd = (objective(dp) - objective(dmn)) / (2 * eps)
+                err = abs(fd - g[row, b]) / scale
+                worst = max(worst, err) if np.isfinite(err) else float("inf")
+                fd_checks += 1
+            fd_err[key] = worst
+        print("bench: adjoint vs central differences (max |fd - adj| / rms adj): %s" % fd_err, file=sys.stderr,
+              flush=True)
+    result = {
+        "metric": "GPU adjoint samples/s, float64 grid scalar 256^3 (d/dm, dA, dh, dr); backward/forward time ratio",
+        "value": S / (bwd * 1e-3), "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Plummer sphere, seeded cotangent, SplitMix64)",
+        "config": {"workload": "plummer_2e6_C4_symmetric_grid256_adjoint", "particles": npart, "grid": [ng] * 3,
+                   "kernel": "C4_Wendland", "strategy": "symmetric", "output": "A (Shepard)", "precision": "fp64"},
+        "forward_ms": fwd, "backward_ms": bwd, "backward_over_forward": bwd / fwd,
+        "fd_check": {"particles": 12, "checks": fd_checks, "step": "1e-5 relative (positions: 1e-5 h)",
+                     "max_err_over_rms": fd_err},
+    }
+    for d in (dax, dout, dcot, dgrad):
+        d.free()
+    ctx.close()
+    print(json.dumps(result))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -625,6 +727,8 @@ def main():
         return batch512(args)
     if args.workload == "adjoint1024":
         return adjoint1024(args)
+    if args.workload == "gridadjoint256":
+        return gridadjoint256(args)
     prec = N.PH_PREC_FP64 if args.precision == "fp64" else N.PH_PREC_FP32C
     print("bench: generating %d Plummer particles" % NPART, file=sys.stderr, flush=True)
     data = synthetic.plummer_sphere(NPART, seed=2)

@@ -88,13 +88,15 @@ __device__ __attribute__((always_inline)) void union_walk(const BucketView& bv, 
     }
 }
 
-template <int K> __device__ inline double dW_dr(double r, double h) {
-    const double ih = 1.0 / h, ih2 = ih * ih;
-    return sigma3<double, K>() * (ih2 * ih2) * shape_df<double, K>(r * ih);
-}
-template <int K> __device__ inline double dW_dh(double r, double h) {
-    const double ih = 1.0 / h, ih2 = ih * ih, q = r * ih;
-    return -sigma3<double, K>() * (ih2 * ih2) * (3.0 * shape_f<double, K>(q) + q * shape_df<double, K>(q));
+// W, dW/dr and dW/dh at (r, h) from one evaluation of f and f'.
+struct KernelD {
+    double W, dr, dh;
+};
+template <int K> __device__ inline KernelD kernel_d(double r, double h) {
+    const double ih = 1.0 / h, q = r * ih;
+    const double f = shape_f<double, K>(q), df = shape_df<double, K>(q);
+    const double s3 = sigma3<double, K>() * (ih * ih * ih), s4 = s3 * ih;
+    return KernelD{s3 * f, s4 * df, -s4 * (3.0 * f + q * df)};
 }
 
 template <int K, int STRAT, int SRC, bool SHEP, class CT>
@@ -141,15 +143,15 @@ __global__ void __launch_bounds__(kBlock) k_points_adjoint(BucketView bv, const 
             const bool take = live && d2 <= radius2<double, STRAT>(Kd, r2a, L.h);
             const double r = sqrt(d2), v = mcol[k] / rcol[k], A = acol[k];
             double W = 0.0;
-            if constexpr (wa != 0.0) W += wa * kernel_W<double, K>(r, take ? ha : 1.0);
+            if constexpr (wa != 0.0) {
+                const KernelD a = kernel_d<K>(r, take ? ha : 1.0);
+                W += wa * a.W;
+                HN += take ? A * v * (wa * a.dh) : 0.0;
+                HS += take ? v * (wa * a.dh) : 0.0;
+            }
             if constexpr (wb != 0.0) W += wb * kernel_W<double, K>(r, L.h);
             S1 += take ? v * W : 0.0;
             N += take ? A * v * W : 0.0;
-            if constexpr (wa != 0.0) {
-                const double dh = wa * dW_dh<K>(r, take ? ha : 1.0);
-                HN += take ? A * v * dh : 0.0;
-                HS += take ? v * dh : 0.0;
-            }
         }
     });
     const bool has = live && S1 != 0.0;  // otherwise the output is NaN: no gradient
@@ -166,35 +168,30 @@ __global__ void __launch_bounds__(kBlock) k_points_adjoint(BucketView bv, const 
             const bool take = has && g != 0.0 && d2 <= radius2<double, STRAT>(Kd, r2a, L.h);
             if (__ballot(take) == 0) continue;  // wave-uniform
             const double r = sqrt(d2), mb = mcol[k], rb = rcol[k], v = mb / rb, A = acol[k];
-            const double hA = take ? ha : 1.0;
-            double W = 0.0, dr = 0.0;
+            double W = 0.0, dr = 0.0, dhb = 0.0;
             if constexpr (wa != 0.0) {
-                W += wa * kernel_W<double, K>(r, hA);
-                dr += wa * dW_dr<K>(r, hA);
+                const KernelD a = kernel_d<K>(r, take ? ha : 1.0);
+                W += wa * a.W;
+                dr += wa * a.dr;
             }
             if constexpr (wb != 0.0) {
-                W += wb * kernel_W<double, K>(r, L.h);
-                dr += wb * dW_dr<K>(r, L.h);
+                const KernelD b = kernel_d<K>(r, L.h);
+                W += wb * b.W;
+                dr += wb * b.dr;
+                dhb = wb * b.dh;
             }
             const double c = SHEP ? (A - out) * iS1 : A;
             const double gc = g * c;
-            const double dhb = wb != 0.0 ? gc * v * (wb * dW_dh<K>(r, L.h)) : 0.0;
             const double fx = r > 0.0 ? -gc * v * dr / r : 0.0;
             // lanes not taking the leaf contribute exact zeros (selected, not
             // multiplied: their terms may be inf or NaN)
-            double sum[6];
-            sum[0] = wave_sum(take ? gc * W / rb : 0.0);
-            sum[1] = wave_sum(take ? g * v * W * iS1 : 0.0);
-            sum[2] = wave_sum(take ? dhb : 0.0);
-            sum[3] = wave_sum(take ? fx * (p[0] - L.x) : 0.0);
-            sum[4] = wave_sum(take ? fx * (p[1] - L.y) : 0.0);
-            sum[5] = wave_sum(take ? fx * (p[2] - L.z) : 0.0);
-            if ((threadIdx.x & (kWave - 1)) == 0) {
-                const u32 src = order[k];
-#pragma unroll
-                for (int q = 0; q < 6; ++q)
-                    if (sum[q] != 0.0) unsafeAtomicAdd(&grad[(i64)q * n + src], sum[q]);
-            }
+            const double t[6] = {take ? gc * W / rb : 0.0,         take ? g * v * W * iS1 : 0.0,
+                                 take ? gc * v * dhb : 0.0,        take ? fx * (p[0] - L.x) : 0.0,
+                                 take ? fx * (p[1] - L.y) : 0.0,   take ? fx * (p[2] - L.z) : 0.0};
+            const double sum = wave_sum6(t);  // parameter lane >> 3
+            const int lane = threadIdx.x & (kWave - 1);
+            if ((lane & 7) == 0 && lane < 48 && sum != 0.0)
+                unsafeAtomicAdd(&grad[(i64)(lane >> 3) * n + order[k]], sum);
         }
     });
     // d out / d h_a to the nearest particle's h
@@ -225,7 +222,7 @@ void launch_adjoint_t(ph_lbvh* B, i32 qsrc, const PointList* pl, const GridAxes*
         B->nn.alloc(sizeof(i64) * (size_t)S);
         ha = B->ha.as<double>();
         nn = B->nn.as<i64>();
-        if (ga) k_nearest_index<1><<<g, kBlock, 0, s>>>(bv, Sampler<double, 1>{*ga, S, xm, cfg.order}, Kd, ha, nn);
+        if (ga) launch_nearest_tile(B, *ga, S, Kd, ha, s, nn);
         else k_nearest_index<0><<<g, kBlock, 0, s>>>(bv, Sampler<double, 0>{*pl, S, xm}, Kd, ha, nn);
         PH_HIP(hipGetLastError());
     }

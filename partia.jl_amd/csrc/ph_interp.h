@@ -77,8 +77,10 @@ void launch_tile_c32(ph_lbvh* B, const CatChunk& c32, int strategy, const GridAx
 void launch_points_adjoint(ph_lbvh* B, int strategy, i32 qsrc, bool shepard, int precision, const PointList* pl,
                            const GridAxes* ga, i64 S, const void* cot, double* grad, hipStream_t s, hipEvent_t* ev);
 // Nearest-particle smoothing length of every grid point (NaN: no particle
-// within Kd max(h)), one wave per tile seeding its lanes' searches (ph_tile.hip).
-void launch_nearest_tile(ph_lbvh* B, const GridAxes& ga, i64 S, double Kd, double* ha, hipStream_t s);
+// within Kd max(h)), one wave per tile seeding its lanes' searches
+// (ph_tile.hip); with nn, also the nearest leaf's Morton index (-1: none).
+void launch_nearest_tile(ph_lbvh* B, const GridAxes& ga, i64 S, double Kd, double* ha, hipStream_t s,
+                         i64* nn = nullptr);
 
 // Adjoint of the line projection of one scalar (cols[qsrc]): grad[6][n] float64
 // (d/dm, d/dA, d/dh, d/dx, d/dy, d/dz, input order) of sum cot[s] out[s];

@@ -369,19 +369,16 @@ __global__ void __launch_bounds__(kBlock) k_lines_adjoint(TreeView<T> tv, const 
         const T f = dr > T(0) ? g * c1 * (mb / rb) * invh * invh * dI / dr : T(0);
         const T D[3] = {L.x - l.o[0], L.y - l.o[1], L.z - l.o[2]};
         const T Dm = D[0] * l.m[0] + D[1] * l.m[1] + D[2] * l.m[2];
-        const u32 src = order[k];
         // lanes not taking the leaf contribute exact zeros (selected, not
         // multiplied: their terms may be inf or NaN)
-        double sum[6];
+        double part[6];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) sum[j] = wave_sum(take ? (double)v[j] : 0.0);
+        for (int j = 0; j < 3; ++j) part[j] = take ? (double)v[j] : 0.0;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) sum[3 + a] = wave_sum(take ? (double)(f * (D[a] - Dm * l.m[a])) : 0.0);
-        if ((threadIdx.x & (kWave - 1)) == 0) {
-#pragma unroll
-            for (int j = 0; j < 6; ++j)
-                if (sum[j] != 0.0) unsafeAtomicAdd(&grad[(i64)j * n + src], sum[j]);
-        }
+        for (int a = 0; a < 3; ++a) part[3 + a] = take ? (double)(f * (D[a] - Dm * l.m[a])) : 0.0;
+        const double sum = wave_sum6(part);  // parameter lane >> 3
+        const int lane = threadIdx.x & (kWave - 1);
+        if ((lane & 7) == 0 && lane < 48 && sum != 0.0) unsafeAtomicAdd(&grad[(i64)(lane >> 3) * n + order[k]], sum);
     });
 }
 

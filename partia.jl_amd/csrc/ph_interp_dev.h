@@ -465,6 +465,43 @@ template <class T> __device__ inline T wave_sum(T v) {
     return v;
 }
 
+// Sums of six per-lane values over the wave, reduce-scatter style: value v
+// (0..5) ends up, summed over all 64 lanes, in lanes 8v .. 8v+7 (returned by
+// each lane for v = lane >> 3; lanes 48..63 return 0).  10 cross-lane moves
+// instead of 36 for six separate wave sums; the additions are exact
+// float64 sums in a fixed tree order.
+__device__ inline double wave_sum6(const double in[6]) {
+    const int lane = threadIdx.x & (kWave - 1);
+    double v[8] = {in[0], in[1], in[2], in[3], in[4], in[5], 0.0, 0.0};
+    {  // 8 -> 4 values: lanes with bit 5 set keep values 4..7
+        const bool hi = lane & 32;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double send = hi ? v[k] : v[k + 4];
+            const double keep = hi ? v[k + 4] : v[k];
+            v[k] = keep + __shfl_xor(send, 32, kWave);
+        }
+    }
+    {  // 4 -> 2 (bit 4)
+        const bool hi = lane & 16;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const double send = hi ? v[k] : v[k + 2];
+            const double keep = hi ? v[k + 2] : v[k];
+            v[k] = keep + __shfl_xor(send, 16, kWave);
+        }
+    }
+    {  // 2 -> 1 (bit 3)
+        const bool hi = lane & 8;
+        const double send = hi ? v[0] : v[1];
+        const double keep = hi ? v[1] : v[0];
+        v[0] = keep + __shfl_xor(send, 8, kWave);
+    }
+#pragma unroll
+    for (int d = 4; d > 0; d >>= 1) v[0] += __shfl_xor(v[0], d, kWave);
+    return v[0];
+}
+
 __device__ inline void count_add(u64* counter, u64 v) {
     if (!counter) return;
     // wave reduction, one atomic per wave

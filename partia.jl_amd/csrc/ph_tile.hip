@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const Bucket
 constexpr int kNnCand = 64;
 __global__ void __launch_bounds__(kBlock) k_nearest_tile(BucketView bv, const BucketRec* __restrict__ recs,
                                                          Sampler<double, 1> smp, double Kd,
-                                                         double* __restrict__ ha) {
+                                                         double* __restrict__ ha, i64* __restrict__ nn) {
     __shared__ LeafRec<double> cand[kBlock / kWave][kNnCand];
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     i64 s = 0;
@@ -200,7 +200,10 @@ __global__ void __launch_bounds__(kBlock) k_nearest_tile(BucketView bv, const Bu
     Nearest nr{neighbour_bound2(bv, Kd)};
     for (int j = 0; j < m; ++j) nr.offer(p, cand[wv][j], first + j);
     nearest_walk(bv, p, nr);
-    if (valid) ha[s] = nr.h(bv);
+    if (valid) {
+        ha[s] = nr.h(bv);
+        if (nn) nn[s] = nr.bk != Nearest::kNone ? nr.bk : -1;
+    }
 }
 
 template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT>
@@ -255,13 +258,13 @@ void launch_tile_k(int strategy, const BucketView& bv, const double* rec, const 
 
 }  // namespace
 
-void launch_nearest_tile(ph_lbvh* B, const GridAxes& ga, i64 S, double Kd, double* ha, hipStream_t s) {
+void launch_nearest_tile(ph_lbvh* B, const GridAxes& ga, i64 S, double Kd, double* ha, hipStream_t s, i64* nn) {
     const BucketView bv = bucket_view(B);
     const LaunchCfg cfg = tl_launch;
     const i64 tiles = ((ga.nx + 3) / 4) * ((ga.ny + 3) / 4) * ((ga.nz + 3) / 4);
     const unsigned g = (unsigned)((tiles + 3) / 4);
     hipLaunchKernelGGL(k_nearest_tile, dim3(g), dim3(kBlock), 0, s, bv, bv.recs,
-                       Sampler<double, 1>{ga, S, point_xcd(cfg), cfg.order}, Kd, ha);
+                       Sampler<double, 1>{ga, S, point_xcd(cfg), cfg.order}, Kd, ha, nn);
     PH_HIP(hipGetLastError());
 }
 

@@ -116,8 +116,10 @@ def test_grid_adjoint_equals_point_adjoint_on_the_same_points(backend):
 
 
 def test_point_adjoint_tiny_trees_and_empty(backend):
-    """Trees of 1 and 3 particles (a single bucket) and zero samples (the
-    gradient is cleared)."""
+    """Trees of 1, 3 and 9 particles (one or a few buckets) and zero samples
+    (the gradient is cleared).  With one particle a Shepard-normalised output
+    equals A_b exactly, so its m, h and x rows are rounding noise: that tree is
+    checked without the Shepard flag."""
     for n in (1, 3, 9):
         c6, cols = cloud(n, 70 + n)
         c6[4] = np.full(n, 0.6)
@@ -126,9 +128,10 @@ def test_point_adjoint_tiny_trees_and_empty(backend):
         rng = np.random.default_rng(n)
         pts = rng.random((3, 300))
         cot = rng.normal(size=300)
-        for st in STRATS:
-            ref = o.point_adjoint(2, True, st, *pts, cot)
-            assert_rows_close(gpu_point_adjoint(d, 2, True, st, pts, cot), ref, 1e-10)
+        for shepard in ((False,) if n == 1 else (False, True)):
+            for st in STRATS:
+                ref = o.point_adjoint(2, shepard, st, *pts, cot)
+                assert_rows_close(gpu_point_adjoint(d, 2, shepard, st, pts, cot), ref, 1e-10)
     got = gpu_point_adjoint(d, 2, False, O.GATHER, np.zeros((3, 0)), np.zeros(0))
     assert not got.any()
 
