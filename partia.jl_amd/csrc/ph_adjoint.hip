@@ -53,19 +53,17 @@ __device__ __attribute__((always_inline)) void union_walk(const BucketView& bv, 
                                                           const Probe<STRAT>& pr, bool live, u32* lidx,
                                                           Batch&& batch) {
     const int lane = threadIdx.x & (kWave - 1);
-    i32 i = __ballot(live) ? 0 : bv.nrec;
-    i64 steps = (i64)bv.nrec + 8;  // every record at most once
-    BucketRec R = recs[i];         // one padding record follows the tree
+    const u64 live_mask = __ballot(live);
+    i32 i = live_mask ? 0 : bv.nrec;
+    i32 steps = bv.nrec + 8;  // every record at most once (nrec < 2^27)
     for (;;) {
         int nl = 0;
         while (i < bv.nrec && steps > 0) {
             --steps;
-            const i32 jmp = R.link >= 0 ? R.link : i + 1;
-            const BucketRec Rn = recs[i + 1], Rj = recs[jmp];
-            const bool keep = __ballot(live && pr.keep(R)) != 0;
+            const BucketRec R = recs[i];  // wave-uniform: a scalar load
+            const bool keep = (pr.keep_mask(R) & live_mask) != 0;
             if (R.link >= 0) {
                 i = __builtin_amdgcn_readfirstlane(keep ? i + 1 : R.link);
-                R = keep ? Rn : Rj;
                 continue;
             }
             if (keep) {
@@ -78,7 +76,6 @@ __device__ __attribute__((always_inline)) void union_walk(const BucketView& bv, 
                 nl += cnt;
             }
             i = __builtin_amdgcn_readfirstlane(i + 1);
-            R = Rn;
         }
         if (nl == 0) break;
         wave_sync();

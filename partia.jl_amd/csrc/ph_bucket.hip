@@ -288,8 +288,7 @@ void ensure_buckets(ph_lbvh* B, hipStream_t s) {
     if (n > kMaxBucketLeaves) throw Error{PH_ERR_CAPACITY, "the bucket traversal supports at most 2^26 particles"};
     i64 nrec = 1;
     if (n <= kBucket) {
-        B->brecs.alloc(2 * sizeof(BucketRec));  // + one padding record
-        PH_HIP(hipMemsetAsync(B->brecs.as<BucketRec>() + 1, 0, sizeof(BucketRec), s));
+        B->brecs.alloc(sizeof(BucketRec));
         k_bucket_root<<<1, 1, 0, s>>>(n, B->root, B->nodes.as<NodeRec<double>>(), B->leaves.as<LeafRec<double>>(),
                                       B->brecs.as<BucketRec>());
         PH_HIP(hipGetLastError());
@@ -308,8 +307,7 @@ void ensure_buckets(ph_lbvh* B, hipStream_t s) {
         PH_HIP(hipMemcpyAsync(&tot[1], bs + n, sizeof(u32), hipMemcpyDeviceToHost, s));
         PH_HIP(hipStreamSynchronize(s));
         nrec = (i64)tot[0] + (i64)tot[1];
-        B->brecs.alloc(sizeof(BucketRec) * (size_t)(nrec + 1));  // + one padding record: wave walks read ahead
-        PH_HIP(hipMemsetAsync(B->brecs.as<BucketRec>() + nrec, 0, sizeof(BucketRec), s));
+        B->brecs.alloc(sizeof(BucketRec) * (size_t)nrec);
         k_bucket_emit<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->root, B->left.as<i32>(), B->parent.as<i32>(),
                                                         B->range.as<i32>(), B->nodes.as<NodeRec<double>>(),
                                                         B->leaves.as<LeafRec<double>>(), big_s, bs,

@@ -404,13 +404,28 @@ template <int STRAT> struct Probe {
             const float g = fmaxf(fmaxf(R.lo[d] - p[d], p[d] - R.hi[d]), 0.f);
             d2 += g * g;
         }
-        float R2 = Rg2;
-        if constexpr (STRAT != PH_GATHER) {
-            const float rs = Kf * R.scale * kInflate + eps;
-            const float rs2 = rs * rs * kInflate;
-            R2 = STRAT == PH_SCATTER ? rs2 : fmaxf(rs2, Rg2);
+        if constexpr (STRAT == PH_GATHER) return !(d2 > Rg2);
+        const float rs = Kf * R.scale * kInflate + eps;
+        const float rs2 = rs * rs * kInflate;
+        if constexpr (STRAT == PH_SCATTER) return !(d2 > rs2);
+        return !(d2 > rs2) || !(d2 > Rg2);  // d2 <= max(rs2, Rg2)
+    }
+    // The same test as a wave mask (every lane active): the compares give the
+    // lane mask directly, with no boolean round trip through a vector register.
+    __device__ inline u64 keep_mask(const BucketRec& R) const {
+        constexpr int kUle = 13;  // unordered or <=, i.e. !(d2 > R2)
+        const float p[3] = {px, py, pz};
+        float d2 = 0.f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float g = fmaxf(fmaxf(R.lo[d] - p[d], p[d] - R.hi[d]), 0.f);
+            d2 += g * g;
         }
-        return !(d2 > R2);
+        if constexpr (STRAT == PH_GATHER) return __builtin_amdgcn_fcmpf(d2, Rg2, kUle);
+        const float rs = Kf * R.scale * kInflate + eps;
+        const float rs2 = rs * rs * kInflate;
+        if constexpr (STRAT == PH_SCATTER) return __builtin_amdgcn_fcmpf(d2, rs2, kUle);
+        return __builtin_amdgcn_fcmpf(d2, rs2, kUle) | __builtin_amdgcn_fcmpf(d2, Rg2, kUle);
     }
 };
 

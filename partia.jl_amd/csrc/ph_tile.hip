@@ -70,23 +70,18 @@ __global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const Bucket
     AccC<K, STRAT, MS, MG, MD, MC, EXACT> acc;
     acc.init();
     u64 accepted = 0;
-    i32 i = __ballot(live_lane) ? 0 : bv.nrec;  // wave-uniform from here on
-    i64 steps = (i64)bv.nrec + 8;               // the union walk visits every record at most once
-    BucketRec R = recs[i];                      // the record array has one padding record at nrec
+    const u64 live_mask = __ballot(live_lane);
+    i32 i = live_mask ? 0 : bv.nrec;  // wave-uniform from here on
+    i32 steps = bv.nrec + 8;  // the union walk visits every record at most once (nrec < 2^27)
     for (;;) {
         // union walk: the leaves of up to 64 kept buckets' worth of leaves
         int nl = 0;
         while (i < bv.nrec && steps > 0) {
             --steps;
-            // both possible successors are loaded before the test: the next
-            // record (first child, or the record after a bucket) and the escape
-            const i32 jmp = R.link >= 0 ? R.link : i + 1;
-            const BucketRec Rn = recs[i + 1], Rj = recs[jmp];
-            const bool kl = pr.keep(R);
-            const bool keep = __ballot(live_lane && kl) != 0;
+            const BucketRec R = recs[i];
+            const bool keep = (pr.keep_mask(R) & live_mask) != 0;
             if (R.link >= 0) {
                 i = __builtin_amdgcn_readfirstlane(keep ? i + 1 : R.link);
-                R = keep ? Rn : Rj;
                 continue;
             }
             if (keep) {
@@ -99,7 +94,6 @@ __global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const Bucket
                 nl += cnt;
             }
             i = __builtin_amdgcn_readfirstlane(i + 1);
-            R = Rn;
         }
         if (nl == 0) break;
         wave_sync();
