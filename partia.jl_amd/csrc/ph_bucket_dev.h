@@ -198,9 +198,16 @@ template <int STRAT> __device__ inline float radius2_32(float Kv, float r2a, flo
 // unused output slots read position 0 and add zeros (no branch either).
 // EXACT = 2 additionally shares sums the C2-shaped catalog computes twice (the
 // gradient field is the second scalar's column, the curl the divergence's
-// columns): the same fma sequence over the same operands, so identical values.
+// columns): the same fma sequence over the same operands, so identical values;
+// its record positions are the fixed compact layout c2_shared checks, so the
+// record reads have constant offsets.
 template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT> struct AccC {
     static constexpr bool SHARE = EXACT == 2;
+    // record positions: the chunk's, or the C2 layout {vi, s0, s1 = g0, d0..d2 = c0..c2}
+    __device__ static int spos(const CatChunk& c, int j) { return SHARE ? 1 + j : c.spos[j]; }
+    __device__ static int gpos(const CatChunk& c, int j) { return SHARE ? 2 : c.gpos[j]; }
+    __device__ static int dpos(const CatChunk& c, int k) { return SHARE ? 3 + k : c.dpos[k]; }
+    __device__ static int cpos(const CatChunk& c, int k) { return SHARE ? 3 + k : c.cpos[k]; }
     static constexpr int NS = MS > 0 ? MS : 1, G = MG > 0 ? MG : 1, D = MD > 0 ? MD : 1, CC = MC > 0 ? MC : 1;
     double S1, Q[3];
     double ts[NS];
@@ -267,7 +274,7 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT> struct Ac
 #pragma unroll
         for (int j = 0; j < MS; ++j) {
             const double wj = (EXACT || j < c.ns) ? w : 0.0;
-            ts[j] = fma(r(c.spos[j]), wj, ts[j]);
+            ts[j] = fma(r(spos(c, j)), wj, ts[j]);
         }
         if constexpr (MG + MD + MC > 0) {
             // grad W = delta f'(q) sigma / (r h^4); zero at r = 0 (kernel.jl:273-285)
@@ -279,7 +286,7 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT> struct Ac
 #pragma unroll
             for (int j = 0; j < MG; ++j) {
                 const bool on = EXACT || j < c.ng;
-                const double A = on ? r(c.gpos[j]) : 0.0;
+                const double A = on ? r(gpos(c, j)) : 0.0;
                 if constexpr (!SHARE) tgs[j] = fma(A, w, tgs[j]);
                 tg[j][0] = fma(A, g0, tg[j][0]);
                 tg[j][1] = fma(A, g1, tg[j][1]);
@@ -288,8 +295,8 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT> struct Ac
 #pragma unroll
             for (int j = 0; j < MD; ++j) {
                 const bool on = EXACT || j < c.nd;
-                const double ax = on ? r(c.dpos[3 * j]) : 0.0, ay = on ? r(c.dpos[3 * j + 1]) : 0.0,
-                             az = on ? r(c.dpos[3 * j + 2]) : 0.0;
+                const double ax = on ? r(dpos(c, 3 * j)) : 0.0, ay = on ? r(dpos(c, 3 * j + 1)) : 0.0,
+                             az = on ? r(dpos(c, 3 * j + 2)) : 0.0;
                 tds[j][0] = fma(ax, w, tds[j][0]);
                 tds[j][1] = fma(ay, w, tds[j][1]);
                 tds[j][2] = fma(az, w, tds[j][2]);
@@ -298,8 +305,8 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT> struct Ac
 #pragma unroll
             for (int j = 0; j < MC; ++j) {
                 const bool on = EXACT || j < c.nc;
-                const double ax = on ? r(c.cpos[3 * j]) : 0.0, ay = on ? r(c.cpos[3 * j + 1]) : 0.0,
-                             az = on ? r(c.cpos[3 * j + 2]) : 0.0;
+                const double ax = on ? r(cpos(c, 3 * j)) : 0.0, ay = on ? r(cpos(c, 3 * j + 1)) : 0.0,
+                             az = on ? r(cpos(c, 3 * j + 2)) : 0.0;
                 if constexpr (!SHARE) {
                     tcs[j][0] = fma(ax, w, tcs[j][0]);
                     tcs[j][1] = fma(ay, w, tcs[j][1]);
@@ -690,10 +697,12 @@ BucketView bucket_view(const ph_lbvh* B) {
 // 1 curl) whose gradient field is the second scalar's column and whose curl
 // reads the divergence's columns: the shared-sum accumulator applies.
 inline bool c2_shared(const CatChunk& c) {
+    // two scalars, the second one's gradient, a divergence and a curl of the
+    // same three columns, in the compact layout {vi, s0, s1, d0, d1, d2}
     if (!(c.ns == 2 && c.ng == 1 && c.nd == 1 && c.nc == 1)) return false;
-    if (c.gpos[0] != c.spos[1]) return false;
+    if (c.spos[0] != 1 || c.spos[1] != 2 || c.gpos[0] != 2) return false;
     for (int k = 0; k < 3; ++k)
-        if (c.cpos[k] != c.dpos[k]) return false;
+        if (c.dpos[k] != 3 + k || c.cpos[k] != 3 + k) return false;
     return true;
 }
 
