@@ -97,7 +97,8 @@ __global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const Bucket
         }
         if (nl == 0) break;
         wave_sync();
-        // stage leaf lane and its record (consecutive leaves: coalesced loads)
+        // stage leaf lane, the square of its radius (K h)^2 and its record
+        // (consecutive leaves: coalesced loads)
         if (lane < nl) {
             const u32 k = lidx[lane];
             const LeafRec<double> L = bv.leaves[k];
@@ -105,19 +106,28 @@ __global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const Bucket
             d[0] = L.x;
             d[1] = L.y;
             d[2] = L.z;
-            d[3] = L.h;
+            const double rb = Kd * L.h;  // float64, as the reference
+            d[3] = rb * rb;
             const double* r = rec + (size_t)k * rw;
             for (int q = 0; q < rw; ++q) d[4 + q] = r[q];
         }
         wave_sync();
-        // the reference's float64 leaf test of every staged leaf, per lane
-        u64 mask = 0;
-        for (int j = 0; j < nl; ++j) {
+        // the reference's float64 leaf test of every staged leaf, per lane:
+        // d2 <= max((K h_a)^2, (K h_b)^2) as two compares; the mask is built a
+        // 32-bit half at a time (uniform bit constants)
+        auto take = [&](int j) {
             const double* d = st + (size_t)j * slot;
-            const LeafRec<double> L{d[0], d[1], d[2], d[3]};
-            const bool acc_j = live_lane && d2_point(p[0], p[1], p[2], L) <= radius2<double, STRAT>(Kd, r2a, L.h);
-            mask |= acc_j ? (1ull << j) : 0ull;
-        }
+            const double dx = p[0] - d[0], dy = p[1] - d[1], dz = p[2] - d[2];
+            const double d2 = (dx * dx + dy * dy) + dz * dz;  // d2_point's order
+            if constexpr (STRAT == PH_GATHER) return live_lane && d2 <= r2a;
+            if constexpr (STRAT == PH_SCATTER) return live_lane && d2 <= d[3];
+            return live_lane && (d2 <= d[3] || d2 <= r2a);
+        };
+        u32 mlo = 0, mhi = 0;
+        const int nlo = nl < 32 ? nl : 32;
+        for (int j = 0; j < nlo; ++j) mlo |= take(j) ? (1u << j) : 0u;
+        for (int j = 32; j < nl; ++j) mhi |= take(j) ? (1u << (j - 32)) : 0u;
+        u64 mask = ((u64)mhi << 32) | mlo;
         accepted += __popcll(mask);
         // lockstep: one neighbour per lane and iteration, lowest bit first
         // (increasing leaf index); a lane with none left adds zeros
