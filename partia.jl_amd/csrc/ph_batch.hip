@@ -32,6 +32,7 @@ struct BatchReq {
     double amin[3], d[3];  // axis minimum and step (max - min) / (n - 1)
     i64 n[3];
     void* out;
+    const u32* perm;  // PH_REQ_POINTS: walk order (Morton order of the points), or null
 };
 
 struct BatchState {
@@ -42,6 +43,7 @@ struct BatchState {
     std::vector<int> ncols;        // FP32C: distinct columns per chunk
     int rw_max = 2;
     DevBuf reqs, rec, ctl, times;  // ctl: per-chunk counters + launch timestamp
+    DevBuf perm;                   // walk orders of the point-list requests
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const void* cols_at_capture = nullptr;
@@ -79,6 +81,74 @@ __device__ inline void batch_point(const BatchReq& R, i64 s, double p[3]) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) p[a] = ((R.o[a] + xi * R.r[a]) + eta * R.u[a]) + zeta * R.f[a];
     }
+}
+
+// Work units of a request and the sample each thread of a unit takes: point
+// lists in blocks of kBlock consecutive points (in walk order); planes in
+// 8 x 8 tiles and boxes in 4 x 4 x 4 tiles of samples per wave, so that the
+// lanes of a wave sample neighbouring points.  The sample index is the
+// request's own (outputs land where a single call puts them); -1: none.
+constexpr int kPlaneTile = 8, kBoxTile = 4;
+inline i64 batch_units(const BatchReq& R) {
+    if (R.kind == PH_REQ_POINTS) return (R.S + kBlock - 1) / kBlock;
+    i64 tiles;
+    if (R.kind == PH_REQ_PLANE)
+        tiles = ((R.n[0] + kPlaneTile - 1) / kPlaneTile) * ((R.n[1] + kPlaneTile - 1) / kPlaneTile);
+    else
+        tiles = ((R.n[0] + kBoxTile - 1) / kBoxTile) * ((R.n[1] + kBoxTile - 1) / kBoxTile) *
+                ((R.n[2] + kBoxTile - 1) / kBoxTile);
+    return (tiles + kBlock / kWave - 1) / (kBlock / kWave);
+}
+__device__ inline i64 batch_sample(const BatchReq& R, i64 unit, int tid) {
+    if (R.kind == PH_REQ_POINTS) {
+        const i64 j = unit * kBlock + tid;
+        if (j >= R.S) return -1;
+        return R.perm ? (i64)R.perm[j] : j;
+    }
+    const i64 tile = unit * (kBlock / kWave) + tid / kWave;
+    const int lane = tid & (kWave - 1);
+    if (R.kind == PH_REQ_PLANE) {
+        const i64 tx = (R.n[0] + kPlaneTile - 1) / kPlaneTile, ty = (R.n[1] + kPlaneTile - 1) / kPlaneTile;
+        if (tile >= tx * ty) return -1;
+        const i64 i = (tile % tx) * kPlaneTile + (lane % kPlaneTile), j = (tile / tx) * kPlaneTile + lane / kPlaneTile;
+        if (i >= R.n[0] || j >= R.n[1]) return -1;
+        return i + R.n[0] * j;
+    }
+    const i64 tx = (R.n[0] + kBoxTile - 1) / kBoxTile, ty = (R.n[1] + kBoxTile - 1) / kBoxTile,
+              tz = (R.n[2] + kBoxTile - 1) / kBoxTile;
+    if (tile >= tx * ty * tz) return -1;
+    const i64 i = (tile % tx) * kBoxTile + (lane & 3), j = ((tile / tx) % ty) * kBoxTile + ((lane >> 2) & 3),
+              k = (tile / (tx * ty)) * kBoxTile + (lane >> 4);
+    if (i >= R.n[0] || j >= R.n[1] || k >= R.n[2]) return -1;
+    return i + R.n[0] * (j + R.n[1] * k);
+}
+
+// 30-bit Morton code of each point in the list's bounding box (box6 =
+// {xmin, ymin, zmin, xmax, ymax, zmax}, device) and its index, for the walk
+// order of a point-list request.  Any order gives the same outputs.
+__device__ inline u32 spread10(u32 v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__global__ void k_point_codes(const double* __restrict__ px, const double* __restrict__ py,
+                              const double* __restrict__ pz, i64 n, const double* __restrict__ box6,
+                              u32* __restrict__ keys, u32* __restrict__ vals) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v[3] = {px[i], py[i], pz[i]};
+    u32 q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double ext = box6[3 + a] - box6[a];
+        const double f = ext > 0.0 ? (v[a] - box6[a]) / ext : 0.5;
+        q[a] = (u32)(fmin(fmax(f, 0.0), 1.0) * 1023.0);  // NaN -> 0
+    }
+    keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    vals[i] = (u32)i;
 }
 
 // Clears a chunk's work counter; for the first chunk also stamps the launch
@@ -142,8 +212,9 @@ __global__ void __launch_bounds__(kBlock) k_batch(BucketView bv, const double* _
         }
         lo = __builtin_amdgcn_readfirstlane(lo);
         const BatchReq& R = reqs[lo];
-        const i64 s = (u - R.unit0) * kBlock + threadIdx.x;
-        const bool valid = s < R.S;
+        const i64 sv = batch_sample(R, u - R.unit0, threadIdx.x);
+        const bool valid = sv >= 0;
+        const i64 s = valid ? sv : 0;
         double p[3] = {0.0, 0.0, 0.0};
         if (valid) batch_point(R, s, p);
         double ha = 0.0;
@@ -299,10 +370,36 @@ BatchState* batch_build(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int 
         if (d.S > 0 && !q.out) throw Error{PH_ERR_ARGUMENT, "request " + std::to_string(r) + ": null output pointer"};
         d.out = q.out;
         d.unit0 = unit;
-        unit += (d.S + kBlock - 1) / kBlock;
+        unit += batch_units(d);
         st->samples += d.S;
     }
     st->units = unit;
+    // walk orders of the point lists: Morton order within each list's box
+    {
+        i64 total = 0;
+        for (const BatchReq& d : st->host)
+            if (d.kind == PH_REQ_POINTS && d.S >= 2 * kBlock) total += d.S;
+        if (total > (i64)UINT32_MAX) throw Error{PH_ERR_CAPACITY, "point-list requests exceed 2^32 points"};
+        if (total > 0) {
+            st->perm.alloc(sizeof(u32) * (size_t)total);
+            DevBuf keys, box, tmp;
+            keys.alloc(sizeof(u32) * (size_t)total);
+            box.alloc(sizeof(double) * 6);
+            i64 off = 0;
+            for (BatchReq& d : st->host) {
+                if (!(d.kind == PH_REQ_POINTS && d.S >= 2 * kBlock)) continue;
+                u32* vals = st->perm.as<u32>() + off;
+                minmax3<double>(d.px, d.py, d.pz, d.S, box.as<double>(), tmp, ctx->stream);
+                k_point_codes<<<grid_for(d.S), kBlock, 0, ctx->stream>>>(d.px, d.py, d.pz, d.S, box.as<double>(),
+                                                                          keys.as<u32>() + off, vals);
+                PH_HIP(hipGetLastError());
+                radix_sort_pairs(keys.as<u32>() + off, vals, d.S, 30, tmp, ctx->stream);
+                d.perm = vals;
+                off += d.S;
+            }
+            PH_HIP(hipStreamSynchronize(ctx->stream));  // the temporaries go out of scope
+        }
+    }
     split_catalog(cat, st->chunks);
     if (st->chunks.empty()) st->chunks.push_back(CatChunk{});
     st->ncols.assign(st->chunks.size(), 0);
