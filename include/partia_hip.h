@@ -141,15 +141,19 @@ int ph_set_traversal(ph_context* ctx, int mode);
  *               consecutive blocks dealt round-robin to the XCDs (each XCD's
  *               blocks are neighbours, dense regions spread over every XCD);
  *               -1 each kernel family's measured best (2 for point and grid
- *               kernels, 5 for line kernels).  Line kernels use this knob
- *               only; the autotuner does not time them;
- *   tile_order  grid tiles (4x4x4 samples, one per wavefront) in x-fastest
- *               rows (0) or in bricks of 8x8x8 tiles, Morton order inside (1).
+ *               kernels; for line kernels 7 when the launch has at least
+ *               16384 blocks, else 0);
+ *   tile_order  order of the tiles (one per wavefront: 4x4x4 grid points or
+ *               8x8 image-plane rays): 0 x-fastest rows, 1 bricks of 8x8x8 or
+ *               8x8 tiles with Morton order inside, -1 each family's best
+ *               (rows for grids, bricks for images).
+ * Line kernels follow xcd_map and tile_order; the autotuner times the point
+ * and grid kernels only.
  * With autotuning on, the first call of each workload key (precision,
  * strategy, kernel, catalog shape, grid or list, log2 of the sample and
  * particle counts) times the candidate configurations twice on the call's own
  * data and keeps the fastest for later calls of that key; otherwise the
- * ph_set_launch configuration (default 1, -1, 0) is used. ---- */
+ * ph_set_launch configuration (default 1, -1, -1) is used. ---- */
 typedef struct ph_tune_entry {
     uint64_t key;
     double occupancy;

@@ -228,7 +228,7 @@ void launch_adjoint_t(ph_lbvh* B, i32 qsrc, const PointList* pl, const GridAxes*
     if (ev) PH_HIP(hipEventRecord(ev[0], s));
     if (ga)
         k_points_adjoint<K, STRAT, 1, SHEP, CT><<<g, kBlock, 0, s>>>(bv, bv.recs, bv.leaves, cols, qsrc,
-                                                                     Sampler<double, 1>{*ga, S, xm, cfg.order}, ha,
+                                                                     Sampler<double, 1>{*ga, S, xm, point_order(cfg)}, ha,
                                                                      nn, cot, order, grad);
     else
         k_points_adjoint<K, STRAT, 0, SHEP, CT><<<g, kBlock, 0, s>>>(bv, bv.recs, bv.leaves, cols, qsrc,

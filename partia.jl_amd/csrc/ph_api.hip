@@ -125,6 +125,7 @@ LaunchCfg autotuned(ph_context* ctx, ph_lbvh* B, const CatChunk& c, int strategy
     // dense regions over the XCDs, both tile orders; two lower occupancies
     LaunchCfg first = ctx->launch;
     first.xcd_map = point_xcd(first);
+    first.order = point_order(first);
     std::vector<LaunchCfg> cand{first};
     auto add = [&](LaunchCfg a) {
         for (const LaunchCfg& b : cand)
@@ -330,8 +331,9 @@ int ph_set_autotune(ph_context* ctx, int enable) {
 
 int ph_set_launch(ph_context* ctx, double occupancy, int32_t xcd_map, int32_t tile_order) {
     if (!ctx) return PH_ERR_ARGUMENT;
-    if (!(occupancy > 0.0 && occupancy <= 1.0) || xcd_map < -1 || xcd_map > 7 || tile_order < 0 || tile_order > 1)
-        return fail(ctx, PH_ERR_ARGUMENT, "occupancy must be in (0, 1], xcd_map in -1..7 and tile_order 0 or 1");
+    if (!(occupancy > 0.0 && occupancy <= 1.0) || xcd_map < -1 || xcd_map > 7 || tile_order < -1 ||
+        tile_order > 1)
+        return fail(ctx, PH_ERR_ARGUMENT, "occupancy must be in (0, 1], xcd_map in -1..7 and tile_order in -1..1");
     ctx->launch = LaunchCfg{(float)occupancy, xcd_map, tile_order};
     return PH_OK;
 }

@@ -183,7 +183,7 @@ void launch_c32_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const Grid
         auto kern = k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 1>;
         const size_t pad = occupancy_pad(kern, cfg.occupancy);
         if (ev) PH_HIP(hipEventRecord(ev[0], s));
-        kern<<<g, kBlock, pad, s>>>(bv, B->rec32.as<double>(), c, Sampler<double, 1>{*ga, S, point_xcd(cfg), cfg.order}, ha,
+        kern<<<g, kBlock, pad, s>>>(bv, B->rec32.as<double>(), c, Sampler<double, 1>{*ga, S, point_xcd(cfg), point_order(cfg)}, ha,
                                     static_cast<float*>(out), counters);
     } else {
         auto kern = k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 0>;
@@ -206,7 +206,7 @@ void launch_f64_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const Grid
         auto kern = k_bucket_f64<K, STRAT, MS, MG, MD, MC, 1>;
         const size_t pad = occupancy_pad(kern, cfg.occupancy);
         if (ev) PH_HIP(hipEventRecord(ev[0], s));
-        kern<<<g, kBlock, pad, s>>>(bv, B->rec.as<double>(), c, Sampler<double, 1>{*ga, S, point_xcd(cfg), cfg.order}, ha,
+        kern<<<g, kBlock, pad, s>>>(bv, B->rec.as<double>(), c, Sampler<double, 1>{*ga, S, point_xcd(cfg), point_order(cfg)}, ha,
                                     static_cast<double*>(out), counters);
     } else {
         auto kern = k_bucket_f64<K, STRAT, MS, MG, MD, MC, 0>;
@@ -245,7 +245,7 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
         const double Kd = support_of<K>();
         if (ga && tl_launch.tile) launch_nearest_tile(B, *ga, S, Kd, B->ha.as<double>(), s);
         else if (ga)
-            k_nearest_bucket<1><<<g, kBlock, 0, s>>>(bv, Sampler<double, 1>{*ga, S, xm, tl_launch.order}, Kd,
+            k_nearest_bucket<1><<<g, kBlock, 0, s>>>(bv, Sampler<double, 1>{*ga, S, xm, point_order(tl_launch)}, Kd,
                                                      B->ha.as<double>());
         else k_nearest_bucket<0><<<g, kBlock, 0, s>>>(bv, Sampler<double, 0>{*pl, S, xm}, Kd, B->ha.as<double>());
         PH_HIP(hipGetLastError());

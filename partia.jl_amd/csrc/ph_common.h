@@ -96,19 +96,29 @@ struct BlockCache {
 //              per XCD (shared L2), 2..7 runs of 2^(xcd_map+2) blocks dealt
 //              round-robin to the XCDs (shared L2 and balanced dense regions);
 //              -1: each kernel family's measured best (runs of 16 blocks for
-//              point / grid kernels, of 128 for line kernels);
-//   order      grid tile order: 0 x-fastest rows, 1 bricks of 8^3 tiles.
+//              point / grid kernels; line kernels: line_xcd);
+//   order      tile order: 0 x-fastest rows, 1 bricks of 8^3 (grids) or 8^2
+//              (images) tiles; -1: each family's measured best (rows for
+//              grids, bricks for images).
 struct LaunchCfg {
     float occupancy = 1.f;
     int xcd_map = -1;
-    int order = 0;
+    int order = -1;
     int tile = 1;  // FP32C grids: one wave walks its tile's union (0: one walk per lane)
 };
 // Configuration of the entry point running on this thread (set by the entry points).
 inline thread_local LaunchCfg tl_launch{};
 // Block order of a point / grid kernel and of a line kernel under cfg.
 inline int point_xcd(const LaunchCfg& c) { return c.xcd_map < 0 ? 2 : c.xcd_map; }
-inline int line_xcd(const LaunchCfg& c) { return c.xcd_map < 0 ? 5 : c.xcd_map; }
+// Line kernels: runs of 512 blocks when the launch holds at least four
+// rounds of them (one run per XCD per round), else the hardware order; longer
+// runs on smaller images put whole dense regions on a few XCDs.
+inline int line_xcd(const LaunchCfg& c, i64 blocks) {
+    if (c.xcd_map >= 0) return c.xcd_map;
+    return blocks >= 4 * 8 * 512 ? 7 : 0;
+}
+inline int point_order(const LaunchCfg& c) { return c.order < 0 ? 0 : c.order; }
+inline int line_order(const LaunchCfg& c) { return c.order < 0 ? 1 : c.order; }
 
 // Cache of the context whose entry point runs on this thread (set by the entry
 // points), or null: then buffers use hipMalloc / hipFree directly.

@@ -137,12 +137,20 @@ template <class T, int K, int STRAT> struct LineSampler<T, K, STRAT, 0> {
 template <class T, int K, int STRAT> struct LineSampler<T, K, STRAT, 1> {
     LineGrid lg;
     i64 S;
-    int xcd = 2;  // block order (xcd_block mode)
+    int xcd = 2;    // block order (xcd_block mode)
+    int order = 0;  // tile order: 0 x-fastest rows, 1 bricks
     __device__ inline bool get(i64& s, Line<T>& l) const {
         const i64 tile = xcd_block(xcd) * (kBlock / kWave) + threadIdx.x / kWave;
         const int lane = threadIdx.x & (kWave - 1);
-        const i64 tx = (lg.nx + 7) / 8;
-        const i64 i = (tile % tx) * 8 + (lane & 7), j = (tile / tx) * 8 + (lane >> 3);
+        const i64 tx = (lg.nx + 7) / 8, ty = (lg.ny + 7) / 8;
+        if (tile >= tx * ty) return false;
+        i64 bx, by;
+        if (order) brick_tile2(tile, tx, ty, bx, by);
+        else {
+            bx = tile % tx;
+            by = tile / tx;
+        }
+        const i64 i = bx * 8 + (lane & 7), j = by * 8 + (lane >> 3);
         if (i >= lg.nx || j >= lg.ny) return false;
         s = i + lg.nx * j;
         const double a = lg.ax[i], b = lg.ay[j];
@@ -480,17 +488,17 @@ void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const Lin
     const T* tab = inorm + (size_t)K * PH_LUT_N_ENTRIES;
     if (ev) PH_HIP(hipEventRecord(ev[0], s));
     if (lg) {
-        LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S, line_xcd(tl_launch)};
         const i64 tiles = ((lg->nx + 7) / 8) * ((lg->ny + 7) / 8);
         const unsigned g = (unsigned)((tiles + 3) / 4);
+        LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S, line_xcd(tl_launch, g), line_order(tl_launch)};
         if (wave)
             k_lines_wave<T, K, 1><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out), counters);
         else
             k_lines<T, K, PH_SCATTER, 1><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out),
                                                               counters);
     } else {
-        LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S, line_xcd(tl_launch)};
         const unsigned g = (unsigned)((S + kBlock - 1) / kBlock);
+        LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S, line_xcd(tl_launch, g)};
         if (wave)
             k_lines_wave<T, K, 0><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out), counters);
         else
@@ -526,14 +534,14 @@ void launch_adjoint_k(ph_lbvh* B, i32 qsrc, const LineList* ll, const LineGrid* 
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[0], s));
     if (lg) {
-        LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S, line_xcd(tl_launch)};
         const i64 tiles = ((lg->nx + 7) / 8) * ((lg->ny + 7) / 8);
-        k_lines_adjoint<T, K, 1, SHEP><<<(unsigned)((tiles + 3) / 4), kBlock, 0, s>>>(tv, cols, qsrc, smp, tab, dq,
-                                                                                      cot, order, grad);
+        const unsigned g = (unsigned)((tiles + 3) / 4);
+        LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S, line_xcd(tl_launch, g), line_order(tl_launch)};
+        k_lines_adjoint<T, K, 1, SHEP><<<g, kBlock, 0, s>>>(tv, cols, qsrc, smp, tab, dq, cot, order, grad);
     } else {
-        LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S, line_xcd(tl_launch)};
-        k_lines_adjoint<T, K, 0, SHEP><<<(unsigned)((S + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-            tv, cols, qsrc, smp, tab, dq, cot, order, grad);
+        const unsigned g = (unsigned)((S + kBlock - 1) / kBlock);
+        LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S, line_xcd(tl_launch, g)};
+        k_lines_adjoint<T, K, 0, SHEP><<<g, kBlock, 0, s>>>(tv, cols, qsrc, smp, tab, dq, cot, order, grad);
     }
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[1], s));

@@ -422,6 +422,25 @@ __device__ inline void brick_tile(i64 t, i64 tx, i64 ty, i64 tz, i64& bx, i64& b
     bz = sz * kBrick + t / (wx * hy);
 }
 
+// The same for a (tx, ty) grid of image tiles: y bands of kBrick rows, bricks
+// of kBrick columns, Morton order inside a whole brick (x-fastest in a clipped
+// one).  A bijection of [0, tx*ty).
+__device__ inline void brick_tile2(i64 t, i64 tx, i64 ty, i64& bx, i64& by) {
+    const i64 sy = t / (tx * kBrick);  // y band
+    t -= sy * tx * kBrick;
+    const i64 hy = min(kBrick, ty - sy * kBrick);
+    const i64 sx = t / (kBrick * hy);  // brick
+    t -= sx * kBrick * hy;
+    const i64 wx = min(kBrick, tx - sx * kBrick);
+    if (wx == kBrick && hy == kBrick) {
+        bx = sx * kBrick + ((t & 1) | ((t >> 1) & 2) | ((t >> 2) & 4));
+        by = sy * kBrick + (((t >> 1) & 1) | ((t >> 2) & 2) | ((t >> 3) & 4));
+        return;
+    }
+    bx = sx * kBrick + t % wx;
+    by = sy * kBrick + t / wx;
+}
+
 template <class T> struct Sampler<T, 1> {
     GridAxes g;
     i64 S;

@@ -235,7 +235,7 @@ void launch_tile_t(const BucketView& bv, const double* rec, const CatChunk& c, c
     const i64 tiles = ((ga.nx + 3) / 4) * ((ga.ny + 3) / 4) * ((ga.nz + 3) / 4);
     const unsigned g = (unsigned)((tiles + 3) / 4);
     if (ev) PH_HIP(hipEventRecord(ev[0], s));
-    kern<<<g, kBlock, dyn + pad, s>>>(bv, bv.recs, rec, c, Sampler<double, 1>{ga, S, point_xcd(cfg), cfg.order}, ha, out, counters);
+    kern<<<g, kBlock, dyn + pad, s>>>(bv, bv.recs, rec, c, Sampler<double, 1>{ga, S, point_xcd(cfg), point_order(cfg)}, ha, out, counters);
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[1], s));
 }
@@ -268,7 +268,7 @@ void launch_nearest_tile(ph_lbvh* B, const GridAxes& ga, i64 S, double Kd, doubl
     const i64 tiles = ((ga.nx + 3) / 4) * ((ga.ny + 3) / 4) * ((ga.nz + 3) / 4);
     const unsigned g = (unsigned)((tiles + 3) / 4);
     hipLaunchKernelGGL(k_nearest_tile, dim3(g), dim3(kBlock), 0, s, bv, bv.recs,
-                       Sampler<double, 1>{ga, S, point_xcd(cfg), cfg.order}, Kd, ha, nn);
+                       Sampler<double, 1>{ga, S, point_xcd(cfg), point_order(cfg)}, Kd, ha, nn);
     PH_HIP(hipGetLastError());
 }
 

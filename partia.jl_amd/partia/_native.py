@@ -205,7 +205,7 @@ class Context:
     def set_autotune(self, enable):
         self.check(lib().ph_set_autotune(self.handle, 1 if enable else 0))
 
-    def set_launch(self, occupancy=1.0, xcd_map=-1, tile_order=0):
+    def set_launch(self, occupancy=1.0, xcd_map=-1, tile_order=-1):
         self.check(lib().ph_set_launch(self.handle, float(occupancy), int(xcd_map), int(tile_order)))
 
     def autotune_report(self):

@@ -11,7 +11,7 @@ from partia import synthetic
 
 pytestmark = pytest.mark.gpu
 FULL = dict(sslots=[1, 2], sflags=[0, 1], gslots=[2], dslots=[3, 4, 5], cslots=[3, 4, 5])
-CONFIGS = ([(occ, xm, 0) for occ in (1.0, 0.75, 0.5) for xm in (-1, 2, 1, 0)]
+CONFIGS = ([(occ, xm, o) for occ in (1.0, 0.75, 0.5) for xm in (-1, 2, 1, 0) for o in (-1, 0)]
            + [(1.0, xm, o) for xm in range(8) for o in (0, 1) if (xm, o) not in ((2, 0), (1, 0), (0, 0))])
 
 
@@ -72,3 +72,5 @@ def test_set_launch_rejects_bad_values(backend):
         ctx.set_launch(1.0, -2)
     with pytest.raises(ArgumentError):
         ctx.set_launch(1.0, 1, 2)
+    with pytest.raises(ArgumentError):
+        ctx.set_launch(1.0, 1, -2)

@@ -147,9 +147,10 @@ def test_line_list_wave_same_as_lane(backend, precision):
 
 
 def test_line_block_orders_give_identical_images(backend):
-    """The block order (ph_set_launch xcd_map) only moves work between XCDs:
-    image-plane and list projections are bit-identical for every order, with
-    both traversals, and equal to the oracle."""
+    """The block and tile orders (ph_set_launch xcd_map, tile_order) only move
+    work between XCDs and waves: image-plane and list projections are
+    bit-identical for every order, with both traversals, and equal to the
+    oracle."""
     c6, cols = cloud(20000, 95)
     d = DeviceSet(backend.context, *c6, cols, kernel=0)
     o = oracle_of(*c6, cols, kernel=0)
@@ -162,8 +163,8 @@ def test_line_block_orders_give_identical_images(backend):
     ctx = d.ctx
     try:
         ref_l = d.lines(cat, O.SCATTER, org, dirs)
-        for xm in (-1, 0, 1, 2, 5, 7):
-            ctx.set_launch(1.0, xm)
+        for xm, order in ((-1, 0), (0, 0), (1, 0), (2, 0), (5, 0), (7, 0), (-1, 1), (3, 1), (5, 1)):
+            ctx.set_launch(1.0, xm, order)
             for mode in (N.PH_TRAVERSAL_AUTO, N.PH_TRAVERSAL_LANE, N.PH_TRAVERSAL_WAVE):
                 with traversal(ctx, mode):
                     assert_same(line_grid(d, cat, Z_FRAME, ax, ay), ref)
