@@ -328,6 +328,8 @@ int ph_line_table(int32_t kernel, double* inorm_host /* 4096 */);
 #define PH_REQ_POINTS 0 /* count points at px, py, pz (device float64)                  */
 #define PH_REQ_PLANE 1  /* n[0] x n[1] points: position + xi right + eta up              */
 #define PH_REQ_BOX 2    /* n[0] x n[1] x n[2] points: ... + zeta forward                 */
+#define PH_REQ_POLAR 3  /* n[0] x n[1] points: position + s cos(phi) right + s sin(phi) up */
+#define PH_REQ_CYLINDER 4 /* n[0] x n[1] x n[2]: ... + zeta forward                       */
 typedef struct ph_request {
     int32_t kind;
     int32_t reserved;
@@ -339,6 +341,8 @@ typedef struct ph_request {
     double amin[3], amax[3];            /* (xmin, ymin, zmin), (xmax, ymax, zmax)         */
     int64_t n[3];                       /* (nx, ny, nz), each >= 2; nz unused by PLANE    */
     void* out;                          /* device, L x S: double (FP64) or float (FP32C)  */
+    const double* trig;                 /* POLAR / CYLINDER: host (sin phi_j, cos phi_j)  */
+                                        /* pairs for j < n[1], or NULL (C library sin/cos) */
 } ph_request;
 typedef struct ph_batch ph_batch;
 /* Validates the requests (the reference's ArgumentError rules of coordinate.jl:

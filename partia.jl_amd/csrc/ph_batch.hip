@@ -13,6 +13,7 @@
 // Device timestamps (s_memrealtime) give each request's first-unit start and
 // last-unit end.
 #include <algorithm>
+#include <cmath>
 #include <memory>
 
 #include "ph_bucket_dev.h"
@@ -32,7 +33,8 @@ struct BatchReq {
     double amin[3], d[3];  // axis minimum and step (max - min) / (n - 1)
     i64 n[3];
     void* out;
-    const u32* perm;  // PH_REQ_POINTS: walk order (Morton order of the points), or null
+    const u32* perm;    // PH_REQ_POINTS: walk order (Morton order of the points), or null
+    const double* trig;  // POLAR / CYLINDER: (sin phi_j, cos phi_j) pairs, device
 };
 
 struct BatchState {
@@ -44,6 +46,7 @@ struct BatchState {
     int rw_max = 2;
     DevBuf reqs, rec, ctl, times;  // ctl: per-chunk counters + launch timestamp
     DevBuf perm;                   // walk orders of the point-list requests
+    DevBuf trig;                   // angle tables of the polar / cylindrical requests
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const void* cols_at_capture = nullptr;
@@ -62,6 +65,9 @@ namespace {
 // (Grids/grids/coordinate.jl:44-143): xi = xmin + (i - 1) dx with
 // dx = (xmax - xmin) / (nx - 1), x = x0 + xi rx + eta ux [+ zeta fx], evaluated
 // left to right without contraction; sample k = i + (j - 1) nx + (k3 - 1) nx ny.
+// _polar_plane_coordinates / _cylindrical_coordinates (coordinate.jl:153-276):
+// s = smin + (i - 1) ds, xi = s cos(phi_j), eta = s sin(phi_j) with the angle
+// table's values, then as the plane / box.
 __device__ inline void batch_point(const BatchReq& R, i64 s, double p[3]) {
     if (R.kind == PH_REQ_POINTS) {
         p[0] = R.px[s];
@@ -71,9 +77,16 @@ __device__ inline void batch_point(const BatchReq& R, i64 s, double p[3]) {
     }
     const i64 nx = R.n[0], ny = R.n[1];
     const i64 i = s % nx, j = (s / nx) % ny, k3 = s / (nx * ny);
-    const double xi = R.amin[0] + (double)i * R.d[0];
-    const double eta = R.amin[1] + (double)j * R.d[1];
-    if (R.kind == PH_REQ_PLANE) {
+    double xi, eta;
+    if (R.kind == PH_REQ_POLAR || R.kind == PH_REQ_CYLINDER) {
+        const double sv = R.amin[0] + (double)i * R.d[0];
+        xi = sv * R.trig[2 * j + 1];
+        eta = sv * R.trig[2 * j];
+    } else {
+        xi = R.amin[0] + (double)i * R.d[0];
+        eta = R.amin[1] + (double)j * R.d[1];
+    }
+    if (R.kind == PH_REQ_PLANE || R.kind == PH_REQ_POLAR) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) p[a] = (R.o[a] + xi * R.r[a]) + eta * R.u[a];
     } else {
@@ -92,7 +105,7 @@ constexpr int kPlaneTile = 8, kBoxTile = 4;
 inline i64 batch_units(const BatchReq& R) {
     if (R.kind == PH_REQ_POINTS) return (R.S + kBlock - 1) / kBlock;
     i64 tiles;
-    if (R.kind == PH_REQ_PLANE)
+    if (R.kind == PH_REQ_PLANE || R.kind == PH_REQ_POLAR)
         tiles = ((R.n[0] + kPlaneTile - 1) / kPlaneTile) * ((R.n[1] + kPlaneTile - 1) / kPlaneTile);
     else
         tiles = ((R.n[0] + kBoxTile - 1) / kBoxTile) * ((R.n[1] + kBoxTile - 1) / kBoxTile) *
@@ -107,7 +120,7 @@ __device__ inline i64 batch_sample(const BatchReq& R, i64 unit, int tid) {
     }
     const i64 tile = unit * (kBlock / kWave) + tid / kWave;
     const int lane = tid & (kWave - 1);
-    if (R.kind == PH_REQ_PLANE) {
+    if (R.kind == PH_REQ_PLANE || R.kind == PH_REQ_POLAR) {
         const i64 tx = (R.n[0] + kPlaneTile - 1) / kPlaneTile, ty = (R.n[1] + kPlaneTile - 1) / kPlaneTile;
         if (tile >= tx * ty) return -1;
         const i64 i = (tile % tx) * kPlaneTile + (lane % kPlaneTile), j = (tile / tx) * kPlaneTile + lane / kPlaneTile;
@@ -364,6 +377,39 @@ BatchState* batch_build(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int 
                 d.u[a] = q.up[a];
                 d.f[a] = q.forward[a];
             }
+        } else if (q.kind == PH_REQ_POLAR || q.kind == PH_REQ_CYLINDER) {
+            // coordinate.jl:153-172, 209-234
+            const double smin = q.amin[0], smax = q.amax[0], pmin = q.amin[1], pmax = q.amax[1];
+            const double two_pi = 2.0 * M_PI;
+            if (!(smin >= 0.0)) throw Error{PH_ERR_ARGUMENT, "smin must be nonnegative."};
+            if (!(smax > smin)) throw Error{PH_ERR_ARGUMENT, "smax must be greater than smin."};
+            if (q.kind == PH_REQ_CYLINDER && !(q.amax[2] > q.amin[2]))
+                throw Error{PH_ERR_ARGUMENT, "zmax must be greater than zmin."};
+            // isapprox(pmax, 2 pi) with Julia's default rtol = sqrt(eps)
+            const bool near_2pi = std::fabs(pmax - two_pi) <= 1.4901161193847656e-8 * std::max(std::fabs(pmax), two_pi);
+            if (!(pmin >= 0.0 && (pmax <= two_pi || near_2pi) && pmax > pmin))
+                throw Error{PH_ERR_ARGUMENT, "angular range must satisfy 0 ≤ ϕmin < ϕmax ≤ 2π."};
+            if (q.n[0] < 2) throw Error{PH_ERR_ARGUMENT, "ns must be at least 2."};
+            if (q.n[1] < 1) throw Error{PH_ERR_ARGUMENT, "nϕ must be at least 1."};
+            if (q.kind == PH_REQ_CYLINDER && q.n[2] < 2) throw Error{PH_ERR_ARGUMENT, "nz must be at least 2."};
+            const int axes = q.kind == PH_REQ_POLAR ? 2 : 3;
+            d.S = 1;
+            for (int a = 0; a < axes; ++a) {
+                if (d.S > (i64(1) << 40) / q.n[a]) throw Error{PH_ERR_ARGUMENT, "request too large"};
+                d.S *= q.n[a];
+                d.amin[a] = q.amin[a];
+                d.n[a] = q.n[a];
+            }
+            d.d[0] = (smax - smin) / (double)(q.n[0] - 1);
+            d.d[1] = (pmax - pmin) / (double)q.n[1];  // half-open in phi
+            if (axes == 3) d.d[2] = (q.amax[2] - q.amin[2]) / (double)(q.n[2] - 1);
+            else d.n[2] = 1;
+            for (int a = 0; a < 3; ++a) {
+                d.o[a] = q.position[a];
+                d.r[a] = q.right[a];
+                d.u[a] = q.up[a];
+                d.f[a] = q.forward[a];
+            }
         } else {
             throw Error{PH_ERR_ARGUMENT, "request " + std::to_string(r) + ": unknown kind"};
         }
@@ -374,6 +420,35 @@ BatchState* batch_build(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int 
         st->samples += d.S;
     }
     st->units = unit;
+    // angle tables of the polar / cylindrical requests: the caller's pairs, or
+    // sin / cos of phi_j = phimin + (j - 1) dphi from the C library
+    {
+        std::vector<double> tab;
+        std::vector<size_t> at((size_t)nreq, 0);
+        for (int r = 0; r < nreq; ++r) {
+            const BatchReq& d = st->host[(size_t)r];
+            if (d.kind != PH_REQ_POLAR && d.kind != PH_REQ_CYLINDER) continue;
+            at[(size_t)r] = tab.size();
+            for (i64 j = 0; j < d.n[1]; ++j) {
+                if (reqs[r].trig) {
+                    tab.push_back(reqs[r].trig[2 * j]);
+                    tab.push_back(reqs[r].trig[2 * j + 1]);
+                } else {
+                    const double phi = d.amin[1] + (double)j * d.d[1];
+                    tab.push_back(std::sin(phi));
+                    tab.push_back(std::cos(phi));
+                }
+            }
+        }
+        if (!tab.empty()) {
+            st->trig.alloc(sizeof(double) * tab.size());
+            PH_HIP(hipMemcpy(st->trig.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+            for (int r = 0; r < nreq; ++r) {
+                BatchReq& d = st->host[(size_t)r];
+                if (d.kind == PH_REQ_POLAR || d.kind == PH_REQ_CYLINDER) d.trig = st->trig.as<double>() + at[(size_t)r];
+            }
+        }
+    }
     // walk orders of the point lists: Morton order within each list's box
     {
         i64 total = 0;

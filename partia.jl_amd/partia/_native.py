@@ -39,7 +39,7 @@ EXPORTS = (
     "ph_set_autotune", "ph_set_launch", "ph_autotune_report",
 )
 
-PH_REQ_POINTS, PH_REQ_PLANE, PH_REQ_BOX = 0, 1, 2
+PH_REQ_POINTS, PH_REQ_PLANE, PH_REQ_BOX, PH_REQ_POLAR, PH_REQ_CYLINDER = 0, 1, 2, 3, 4
 
 
 class NativeLibraryError(RuntimeError):
@@ -60,7 +60,7 @@ class ph_request(C.Structure):
                 ("px", C.c_void_p), ("py", C.c_void_p), ("pz", C.c_void_p),
                 ("position", C.c_double * 3), ("right", C.c_double * 3), ("up", C.c_double * 3),
                 ("forward", C.c_double * 3), ("amin", C.c_double * 3), ("amax", C.c_double * 3),
-                ("n", C.c_int64 * 3), ("out", C.c_void_p)]
+                ("n", C.c_int64 * 3), ("out", C.c_void_p), ("trig", C.c_void_p)]
 
 
 class ph_tune_entry(C.Structure):

@@ -3,10 +3,12 @@ one graph-recorded launch of a persistent work-queue kernel (ph_batch_*).
 
 Each request computes what one external-LBVH PointSamples_interpolation call
 (KI/grid_interpolation/drivers/PointSamples_driver.jl:98-118) computes, with
-identical outputs.  Samples are a device point list or the frame-based
+identical outputs.  Samples are a device point list, the frame-based
 Cartesian plane / box of PointSamples(Cartesian, frame, xparams, yparams[,
-zparams]) (Grids/grids/PointSamples.jl:328-356), generated on the device.
-Outputs stay on the device (L x S per request).
+zparams]) or the polar plane / cylinder of PointSamples(Polar, frame, sparams,
+phiparams) / PointSamples(Cylindrical, frame, sparams, phiparams, zparams)
+(Grids/grids/PointSamples.jl:328-397), generated on the device.  Outputs stay
+on the device (L x S per request).
 """
 import ctypes as C
 
@@ -35,10 +37,24 @@ def box(position, right, up, forward, xparams, yparams, zparams, out):
     return _frame(N.PH_REQ_BOX, position, right, up, forward, (xparams, yparams, zparams), out)
 
 
+def polar(position, right, up, sparams, phiparams, out, trig=None):
+    """PointSamples(Polar, frame, sparams, phiparams): ns * nphi samples, s fastest,
+    phi half-open.  trig: optional (nphi, 2) array of (sin phi_j, cos phi_j), e.g.
+    the caller's own sincos values; by default the library uses the C library's."""
+    r = _frame(N.PH_REQ_POLAR, position, right, up, (0.0, 0.0, 0.0), (sparams, phiparams, (0.0, 1.0, 2)), out)
+    return _trig(r, trig)
+
+
+def cylinder(position, right, up, forward, sparams, phiparams, zparams, out, trig=None):
+    """PointSamples(Cylindrical, frame, sparams, phiparams, zparams): ns * nphi * nz samples."""
+    r = _frame(N.PH_REQ_CYLINDER, position, right, up, forward, (sparams, phiparams, zparams), out)
+    return _trig(r, trig)
+
+
 def sample_count(r):
     if r.kind == N.PH_REQ_POINTS:
         return int(r.count)
-    if r.kind == N.PH_REQ_PLANE:
+    if r.kind in (N.PH_REQ_PLANE, N.PH_REQ_POLAR):
         return int(r.n[0]) * int(r.n[1])
     return int(r.n[0]) * int(r.n[1]) * int(r.n[2])
 
@@ -83,6 +99,15 @@ def _p(v):
     if isinstance(v, C.c_void_p):
         return v.value
     return int(v)
+
+
+def _trig(r, trig):
+    """The table is read by ph_batch_create; the request object keeps it alive."""
+    if trig is not None:
+        t = np.ascontiguousarray(np.asarray(trig, dtype=np.float64).reshape(-1, 2))
+        r.trig = t.ctypes.data
+        r.trig_table = t
+    return r
 
 
 def _frame(kind, position, right, up, forward, params, out):

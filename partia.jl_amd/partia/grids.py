@@ -6,6 +6,8 @@ Grids/griddataset/GridBundle.jl:1-4.  Arrays follow Julia's column-major layout:
 a StructuredGrid's values are an order='F' numpy array and flattening is x
 fastest.  Non-Cartesian coordinate systems are converted on the host.
 """
+import math
+
 import numpy as np
 
 from .errors import ArgumentError
@@ -62,6 +64,67 @@ class PointSamples(AbstractSamples):
 
     def __len__(self):
         return len(self.grid)
+
+
+def angle_table(phiparams):
+    """(nphi, 2) array of (sin phi_j, cos phi_j), phi_j = phimin + (j - 1) dphi,
+    dphi = (phimax - phimin) / nphi (half-open), with Python's math.sin /
+    math.cos (the C library's, as ph_batch_create uses without a table)."""
+    pmin, pmax, nphi = float(phiparams[0]), float(phiparams[1]), int(phiparams[2])
+    dphi = (pmax - pmin) / float(nphi)
+    phi = [pmin + float(j) * dphi for j in range(nphi)]
+    return np.array([[math.sin(a), math.cos(a)] for a in phi], dtype=np.float64).reshape(nphi, 2)
+
+
+def _check_polar(sparams, phiparams):
+    smin, smax, ns = float(sparams[0]), float(sparams[1]), int(sparams[2])
+    pmin, pmax, nphi = float(phiparams[0]), float(phiparams[1]), int(phiparams[2])
+    two_pi = 2.0 * math.pi
+    if not smin >= 0.0:
+        raise ArgumentError("smin must be nonnegative.")
+    if not smax > smin:
+        raise ArgumentError("smax must be greater than smin.")
+    near = abs(pmax - two_pi) <= math.sqrt(np.finfo(np.float64).eps) * max(abs(pmax), two_pi)
+    if not (pmin >= 0.0 and (pmax <= two_pi or near) and pmax > pmin):
+        raise ArgumentError("angular range must satisfy 0 ≤ ϕmin < ϕmax ≤ 2π.")
+    if ns < 2:
+        raise ArgumentError("ns must be at least 2.")
+    if nphi < 1:
+        raise ArgumentError("nϕ must be at least 1.")
+    return smin, smax, ns, nphi
+
+
+def polar_plane_coordinates(position, right, up, sparams, phiparams, trig=None):
+    """_polar_plane_coordinates (Grids/grids/coordinate.jl:153-206): s = smin +
+    (i - 1) ds, ds = (smax - smin) / (ns - 1); xi = s cos phi_j, eta = s sin phi_j;
+    x = x0 + xi rx + eta ux; sample k = i + (j - 1) ns.  trig: the (sin, cos)
+    table (angle_table by default)."""
+    x0, r, u = (np.asarray(v, dtype=np.float64) for v in (position, right, up))
+    smin, smax, ns, nphi = _check_polar(sparams, phiparams)
+    t = angle_table(phiparams) if trig is None else np.asarray(trig, dtype=np.float64).reshape(nphi, 2)
+    s = smin + np.arange(ns, dtype=np.float64) * ((smax - smin) / float(ns - 1))
+    XI = (s[None, :] * t[:, 1:2]).ravel()  # s fastest, then phi
+    ETA = (s[None, :] * t[:, 0:1]).ravel()
+    return tuple((x0[a] + XI * r[a]) + ETA * u[a] for a in range(3))
+
+
+def cylindrical_coordinates(position, right, up, forward, sparams, phiparams, zparams, trig=None):
+    """_cylindrical_coordinates (Grids/grids/coordinate.jl:208-276): the polar
+    plane plus zeta fx, zeta = zmin + (k3 - 1) dz; s fastest, then phi, z."""
+    x0, r, u, f = (np.asarray(v, dtype=np.float64) for v in (position, right, up, forward))
+    smin, smax, ns, nphi = _check_polar(sparams, phiparams)
+    zmin, zmax, nz = float(zparams[0]), float(zparams[1]), int(zparams[2])
+    if not zmax > zmin:
+        raise ArgumentError("zmax must be greater than zmin.")
+    if nz < 2:
+        raise ArgumentError("nz must be at least 2.")
+    t = angle_table(phiparams) if trig is None else np.asarray(trig, dtype=np.float64).reshape(nphi, 2)
+    s = smin + np.arange(ns, dtype=np.float64) * ((smax - smin) / float(ns - 1))
+    zeta = zmin + np.arange(nz, dtype=np.float64) * ((zmax - zmin) / float(nz - 1))
+    XI = np.tile((s[None, :] * t[:, 1:2]).ravel(), nz)
+    ETA = np.tile((s[None, :] * t[:, 0:1]).ravel(), nz)
+    ZE = np.repeat(zeta, ns * nphi)
+    return tuple(((x0[a] + XI * r[a]) + ETA * u[a]) + ZE * f[a] for a in range(3))
 
 
 def _check_axis(name, params):

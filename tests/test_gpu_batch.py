@@ -143,6 +143,74 @@ def test_batch_point_lists_in_walk_order(backend):
                 x.free()
 
 
+@pytest.mark.parametrize("precision", [N.PH_PREC_FP64, N.PH_PREC_FP32C])
+def test_batch_polar_and_cylindrical_requests(backend, precision):
+    """PointSamples(Polar, ...) and PointSamples(Cylindrical, ...) generated on
+    the device (coordinate.jl:153-276): equal, bit for bit, to single calls on
+    the host restatement's coordinates with the same angle table (the C
+    library's, or one the caller passes)."""
+    ctx = backend.context
+    c6, cols = cloud(20000, 27)
+    d = DeviceSet(ctx, *c6, cols, kernel=4)
+    rng = np.random.default_rng(8)
+    ca = N.CatalogArgs(**FULL)
+    esz = 8 if precision == N.PH_PREC_FP64 else 4
+    specs = [("polar", (0.0, 0.45, 13), (0.0, 2 * np.pi, 21), None, None),
+             ("polar", (0.05, 0.3, 2), (np.pi / 4, 3 * np.pi / 4, 1), None, None),
+             ("cylinder", (0.0, 0.4, 9), (0.5, 6.0, 11), (-0.2, 0.3, 5), None),
+             ("polar", (0.1, 0.4, 30), (0.0, 2 * np.pi, 17), None, "table"),
+             ("cylinder", (0.0, 0.35, 6), (0.0, 2 * np.pi * (1 + 1e-10), 7), (-0.1, 0.1, 2), "table")]
+    reqs, coords, outs = [], [], []
+    for kind, sp, pp, zp, tab in specs:
+        pos, r, u, f = random_frame(rng)
+        trig = None
+        if tab:
+            phi = pp[0] + np.arange(pp[2]) * ((pp[1] - pp[0]) / pp[2])
+            trig = np.stack([np.sin(phi), np.cos(phi)], axis=1)  # numpy's, not the C library's
+        if kind == "polar":
+            P = grids.polar_plane_coordinates(pos, r, u, sp, pp, trig=trig)
+        else:
+            P = grids.cylindrical_coordinates(pos, r, u, f, sp, pp, zp, trig=trig)
+        o = DevArray(ctx, nbytes=ca.L * len(P[0]) * esz)
+        if kind == "polar":
+            reqs.append(PB.polar(pos, r, u, sp, pp, o.p, trig=trig))
+        else:
+            reqs.append(PB.cylinder(pos, r, u, f, sp, pp, zp, o.p, trig=trig))
+        assert PB.sample_count(reqs[-1]) == len(P[0])
+        coords.append(P)
+        outs.append(o)
+    b = PB.Batch(ctx, d.h, ca, O_SYM, precision, reqs)
+    try:
+        b.run()
+        for P, o in zip(coords, outs):
+            assert_same(read_out(o, len(P[0]), ca.L, precision), d.points(FULL, O_SYM, *P, precision=precision))
+    finally:
+        b.close()
+        for o in outs:
+            o.free()
+
+
+def test_batch_polar_argument_errors(backend):
+    ctx = backend.context
+    c6, cols = cloud(500, 2)
+    d = DeviceSet(ctx, *c6, cols, kernel=0)
+    ca = N.CatalogArgs(**SMALL)
+    o = DevArray(ctx, nbytes=4096)
+    fr = ((0.5, 0.5, 0.5), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0))
+    try:
+        bad = [PB.polar(*fr[:3], (-0.1, 0.3, 3), (0.0, 1.0, 2), o.p),
+               PB.polar(*fr[:3], (0.0, 0.3, 3), (0.0, 7.0, 2), o.p),
+               PB.polar(*fr[:3], (0.0, 0.3, 1), (0.0, 1.0, 2), o.p),
+               PB.polar(*fr[:3], (0.0, 0.3, 3), (0.0, 1.0, 0), o.p),
+               PB.cylinder(*fr, (0.0, 0.3, 3), (0.0, 1.0, 2), (0.2, 0.2, 3), o.p),
+               PB.cylinder(*fr, (0.0, 0.3, 3), (0.0, 1.0, 2), (0.0, 0.2, 1), o.p)]
+        for r in bad:
+            with pytest.raises(ArgumentError):
+                PB.Batch(ctx, d.h, ca, O_SYM, N.PH_PREC_FP64, [r])
+    finally:
+        o.free()
+
+
 def test_batch_replays_and_times(backend):
     """Graph replays with eager calls in between give identical outputs; per-
     request times are ordered and empty requests report NaN."""
