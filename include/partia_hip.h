@@ -253,6 +253,16 @@ int ph_line_samples_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat,
 int ph_line_grid_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy, int32_t precision,
                         const double* origin3, const double* u3, const double* v3, const double* dir3,
                         const double* ax_dev, int64_t nx, const double* ay_dev, int64_t ny, void* out_dev);
+/* LineSamples(Polar, ParallelBeam, frame, sparams, phiparams)
+ * (Grids/grids/LineSamples.jl:512-537): origins on the polar sampling of the
+ * frame plane (_polar_plane_coordinates, Grids/grids/coordinate.jl:153-206),
+ * origin3 + (s_i cos phi_j) u3 + (s_i sin phi_j) v3, i fastest, all along dir3.
+ * s_dev: the ns radii s_i = smin + (i - 1) ds; trig_dev: nphi pairs
+ * (sin phi_j, cos phi_j) of the half-open angles, as the caller computes them
+ * (device float64).  Outputs and rules as ph_line_grid_device. */
+int ph_line_polar_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                         const double* origin3, const double* u3, const double* v3, const double* dir3,
+                         const double* s_dev, int64_t ns, const double* trig_dev, int64_t nphi, void* out_dev);
 
 /* ---- adjoints of the line projection (no reference counterpart; the reverse
  * pass a ChainRulesCore rrule of LineSamples_interpolation would call).  For a
@@ -277,6 +287,11 @@ int ph_line_grid_adjoint_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog*
                                 int32_t precision, const double* origin3, const double* u3, const double* v3,
                                 const double* dir3, const double* ax_dev, int64_t nx, const double* ay_dev, int64_t ny,
                                 const void* cot_dev, double* grad_dev);
+/* The same for the polar rays of ph_line_polar_device. */
+int ph_line_polar_adjoint_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy,
+                                 int32_t precision, const double* origin3, const double* u3, const double* v3,
+                                 const double* dir3, const double* s_dev, int64_t ns, const double* trig_dev,
+                                 int64_t nphi, const void* cot_dev, double* grad_dev);
 
 /* ---- adjoints of point / grid interpolation (no reference counterpart; the
  * reverse pass a ChainRulesCore rrule of PointSamples_interpolation /

@@ -726,9 +726,11 @@ int ph_line_samples(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t 
     });
 }
 
-int ph_line_grid_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
-                        const double* origin3, const double* u3, const double* v3, const double* dir3,
-                        const double* ax, int64_t nx, const double* ay, int64_t ny, void* out) {
+// Image-plane rays: origin + a u + b v along dir, (a, b) = (ax[i], ay[j]) or,
+// polar, (ax[i] ay[2j + 1], ax[i] ay[2j]) with ay the (sin, cos) pairs.
+static int line_grid(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                     const double* origin3, const double* u3, const double* v3, const double* dir3, const double* ax,
+                     int64_t nx, const double* ay, int64_t ny, int polar, void* out) {
     return guarded(ctx, [&] {
         need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
         need(nx >= 0 && ny >= 0, PH_ERR_ARGUMENT, "negative image size");
@@ -748,6 +750,7 @@ int ph_line_grid_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int3
         lg.ay = ay;
         lg.nx = nx;
         lg.ny = ny;
+        lg.polar = polar;
         const void* lut = precision == PH_PREC_FP64 ? ctx->lut64.p : ctx->lut32.p;
         LaunchScope scope(ctx->launch);  // block order
         run_chunks(ctx, cat, S, [&](const CatChunk& c, int L, u64* counter, hipEvent_t* ev) {
@@ -756,6 +759,18 @@ int ph_line_grid_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int3
         });
         return PH_OK;
     });
+}
+
+int ph_line_grid_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                        const double* origin3, const double* u3, const double* v3, const double* dir3,
+                        const double* ax, int64_t nx, const double* ay, int64_t ny, void* out) {
+    return line_grid(ctx, B, cat, strategy, precision, origin3, u3, v3, dir3, ax, nx, ay, ny, 0, out);
+}
+
+int ph_line_polar_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                         const double* origin3, const double* u3, const double* v3, const double* dir3,
+                         const double* s_dev, int64_t ns, const double* trig_dev, int64_t nphi, void* out) {
+    return line_grid(ctx, B, cat, strategy, precision, origin3, u3, v3, dir3, s_dev, ns, trig_dev, nphi, 1, out);
 }
 
 // Adjoint of a one-scalar line projection (shared by the list and image-plane forms).
@@ -795,10 +810,10 @@ int ph_line_samples_adjoint_device(ph_context* ctx, ph_lbvh* B, const ph_catalog
     });
 }
 
-int ph_line_grid_adjoint_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy,
+static int line_grid_adjoint(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy,
                                 int32_t precision, const double* origin3, const double* u3, const double* v3,
                                 const double* dir3, const double* ax, int64_t nx, const double* ay, int64_t ny,
-                                const void* cot, double* grad) {
+                                const void* cot, double* grad, int polar) {
     return guarded(ctx, [&] {
         need(nx >= 0 && ny >= 0, PH_ERR_ARGUMENT, "negative image size");
         need(origin3 && u3 && v3 && dir3, PH_ERR_ARGUMENT, "null frame vector");
@@ -815,9 +830,25 @@ int ph_line_grid_adjoint_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* c
         lg.ay = ay;
         lg.nx = nx;
         lg.ny = ny;
+        lg.polar = polar;
         run_line_adjoint(ctx, B, cat, strategy, precision, nullptr, &lg, S, cot, grad);
         return PH_OK;
     });
+}
+
+int ph_line_grid_adjoint_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy,
+                                int32_t precision, const double* origin3, const double* u3, const double* v3,
+                                const double* dir3, const double* ax, int64_t nx, const double* ay, int64_t ny,
+                                const void* cot, double* grad) {
+    return line_grid_adjoint(ctx, B, cat, strategy, precision, origin3, u3, v3, dir3, ax, nx, ay, ny, cot, grad, 0);
+}
+
+int ph_line_polar_adjoint_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy,
+                                 int32_t precision, const double* origin3, const double* u3, const double* v3,
+                                 const double* dir3, const double* s_dev, int64_t ns, const double* trig_dev,
+                                 int64_t nphi, const void* cot, double* grad) {
+    return line_grid_adjoint(ctx, B, cat, strategy, precision, origin3, u3, v3, dir3, s_dev, ns, trig_dev, nphi, cot,
+                             grad, 1);
 }
 
 // Adjoint of a one-scalar point / grid interpolation (shared by both forms).

@@ -46,9 +46,10 @@ struct LineList {
 };
 struct LineGrid {
     double o[3], u[3], v[3], d[3];
-    const double* ax;
-    const double* ay;
+    const double* ax;  // polar: the radii s_i
+    const double* ay;  // polar: (sin phi_j, cos phi_j) pairs
     i64 nx, ny;
+    int polar = 0;     // origins o + (s_i cos phi_j) u + (s_i sin phi_j) v
 };
 
 // Point samples (list or grid); out is L x S of T (double for FP64, float for FP32 / FP32C).

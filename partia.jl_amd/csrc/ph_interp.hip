@@ -153,7 +153,14 @@ template <class T, int K, int STRAT> struct LineSampler<T, K, STRAT, 1> {
         const i64 i = bx * 8 + (lane & 7), j = by * 8 + (lane >> 3);
         if (i >= lg.nx || j >= lg.ny) return false;
         s = i + lg.nx * j;
-        const double a = lg.ax[i], b = lg.ay[j];
+        double a, b;
+        if (lg.polar) {  // _polar_plane_coordinates (coordinate.jl:189-205): xi = s cos, eta = s sin
+            a = lg.ax[i] * lg.ay[2 * j + 1];
+            b = lg.ax[i] * lg.ay[2 * j];
+        } else {
+            a = lg.ax[i];
+            b = lg.ay[j];
+        }
         for (int d = 0; d < 3; ++d) {
             l.o[d] = T((lg.o[d] + lg.u[d] * a) + lg.v[d] * b);
             l.m[d] = T(lg.d[d]);

@@ -35,6 +35,7 @@ EXPORTS = (
     "ph_line_samples", "ph_line_samples_device", "ph_line_grid_device", "ph_nearest_h", "ph_line_table",
     "ph_batch_create", "ph_batch_run", "ph_batch_times", "ph_batch_destroy",
     "ph_line_samples_adjoint_device", "ph_line_grid_adjoint_device",
+    "ph_line_polar_device", "ph_line_polar_adjoint_device",
     "ph_point_samples_adjoint_device", "ph_structured_grid_adjoint_device",
     "ph_set_autotune", "ph_set_launch", "ph_autotune_report",
 )
@@ -129,6 +130,9 @@ def lib():
         "ph_line_grid_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, vp, vp, vp, vp, i64, vp, i64,
                                               vp, vp]),
         "ph_point_samples_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, i64, vp, vp, vp, vp, vp]),
+        "ph_line_polar_device": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, vp, vp, vp, vp, i64, vp, i64, vp]),
+        "ph_line_polar_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, vp, vp, vp, vp, i64, vp, i64,
+                                               vp, vp]),
         "ph_structured_grid_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, i64, vp, i64, vp, i64,
                                                     vp, vp]),
     }

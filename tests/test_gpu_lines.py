@@ -173,6 +173,64 @@ def test_line_block_orders_give_identical_images(backend):
         ctx.set_launch()
 
 
+def test_line_polar_rays_match_the_ray_list(backend):
+    """LineSamples(Polar, ParallelBeam, frame, sparams, phiparams): rays from the
+    polar sampling of the frame plane along the frame's forward vector, equal
+    bit for bit to the same rays as an explicit list (host restatement of the
+    origins with the same angle table) and to the oracle; the adjoint equals
+    the list adjoint to rounding."""
+    from partia import grids
+    c6, cols = cloud(6000, 97)
+    d = DeviceSet(backend.context, *c6, cols, kernel=0)
+    o = oracle_of(*c6, cols, kernel=0)
+    ctx = d.ctx
+    cat = dict(sslots=[1, 2], sflags=[0, 1])
+    pos, u, v, fw = np.array([0.5, 0.5, -0.5]), np.array([1.0, 0.0, 0.0]), np.array([0.0, 1.0, 0.0]), \
+        np.array([0.0, 0.0, 1.0])
+    sp, pp = (0.0, 0.45, 23), (0.0, 2 * np.pi, 37)
+    tab = grids.angle_table(pp)
+    org = grids.polar_plane_coordinates(pos, u, v, sp, pp, trig=tab)
+    S = len(org[0])
+    dirs = [np.full(S, fw[k]) for k in range(3)]
+    ref = o.line_samples(cat, O.SCATTER, org, dirs)
+    s_vals = sp[0] + np.arange(sp[2]) * ((sp[1] - sp[0]) / (sp[2] - 1))
+    ds, dt = Dev(ctx, s_vals), Dev(ctx, np.ascontiguousarray(tab.ravel()))
+    ca = N.CatalogArgs(**cat)
+    dout = Dev(ctx, np.zeros(ca.L * S))
+    frame = [np.ascontiguousarray(a, dtype=np.float64) for a in (pos, u, v, fw)]
+    try:
+        for mode in (N.PH_TRAVERSAL_AUTO, N.PH_TRAVERSAL_LANE):
+            with traversal(ctx, mode):
+                ctx.check(N.lib().ph_line_polar_device(ctx.handle, d.h, ca.ref(), O.SCATTER, 64,
+                                                       *[N.ptr(f) for f in frame], ds.p, sp[2], dt.p, pp[2], dout.p))
+                got = dout.get(np.float64, ca.L * S).reshape(ca.L, S)
+                assert_same(got, ref)
+                assert_same(got, d.lines(cat, O.SCATTER, org, dirs))
+        # adjoint of the Shepard scalar: the polar form against the list form
+        one = N.CatalogArgs(sslots=[2], sflags=[1])
+        cot = np.random.default_rng(1).normal(size=S)
+        dc = Dev(ctx, cot)
+        dg1, dg2 = Dev(ctx, np.zeros(6 * d.n)), Dev(ctx, np.zeros(6 * d.n))
+        arrs = [Dev(ctx, np.ascontiguousarray(a)) for a in list(org) + dirs]
+        try:
+            ctx.check(N.lib().ph_line_polar_adjoint_device(ctx.handle, d.h, one.ref(), O.SCATTER, 64,
+                                                           *[N.ptr(f) for f in frame], ds.p, sp[2], dt.p, pp[2],
+                                                           dc.p, dg1.p))
+            ctx.check(N.lib().ph_line_samples_adjoint_device(ctx.handle, d.h, one.ref(), O.SCATTER, 64, S,
+                                                             *[a.p for a in arrs], dc.p, dg2.p))
+            g1 = dg1.get(np.float64, 6 * d.n).reshape(6, d.n)
+            g2 = dg2.get(np.float64, 6 * d.n).reshape(6, d.n)
+            for r in range(6):
+                scale = max(np.sqrt(np.mean(g2[r] ** 2)), 1e-300)
+                assert np.max(np.abs(g1[r] - g2[r])) <= 1e-10 * scale
+        finally:
+            for a in arrs + [dc, dg1, dg2]:
+                a.free()
+    finally:
+        for b in (ds, dt, dout):
+            b.free()
+
+
 def test_line_grid_empty_image(backend):
     c6, cols = cloud(100, 3)
     d = DeviceSet(backend.context, *c6, cols)
