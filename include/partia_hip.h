@@ -233,6 +233,28 @@ int ph_structured_grid(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int
 int ph_structured_grid_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy,
                               int32_t precision, const double* ax_dev, int64_t nx, const double* ay_dev, int64_t ny,
                               const double* az_dev, int64_t nz, void* out_dev);
+/* StructuredGrid_interpolation for Cylindrical and Spherical grids
+ * (KI/grid_interpolation/drivers/StructuredGrid_driver.jl:46-64 through
+ * flatten / coordinate_grid, Grids/grids/StructuredGrid.jl:176-222): the grid
+ * points are generated on the device, first axis fastest, with _cylin2cart /
+ * _sph2cart (Tools/coordinate_transformations.jl:85-89,198-204) in the tree's
+ * precision.  Angle axes are passed as (sin, cos) pairs of the axis values,
+ * as the caller computes them (Julia's sin / cos for exact parity):
+ *   PH_COORDS_CYLINDRICAL  a1: s (n1); a2: pairs of phi (n2); a3: z (n3)
+ *   PH_COORDS_SPHERICAL    a1: r (n1); a2: pairs of phi (n2); a3: pairs of theta (n3)
+ *   PH_COORDS_CARTESIAN    as ph_structured_grid.
+ * Outputs as ph_structured_grid, L x (n1 n2 n3). */
+#define PH_COORDS_CARTESIAN 0
+#define PH_COORDS_CYLINDRICAL 1
+#define PH_COORDS_SPHERICAL 2
+int ph_structured_grid_coords_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy,
+                                     int32_t precision, int32_t coords, const double* a1_dev, int64_t n1,
+                                     const double* a2_dev, int64_t n2, const double* a3_dev, int64_t n3,
+                                     void* out_dev);
+/* The same with host axes / tables and host outputs (float64 values). */
+int ph_structured_grid_coords(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy,
+                              int32_t precision, int32_t coords, const double* a1, int64_t n1, const double* a2,
+                              int64_t n2, const double* a3, int64_t n3, double* out_host);
 
 /* ---- LineSamples_interpolation: replaces KI/grid_interpolation/drivers/
  * LineSamples_driver.jl:39-57,116-139 (and the CUDA twin LineSamples_driver.jl:

@@ -634,7 +634,16 @@ int ph_point_samples(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t
 int ph_structured_grid_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
                               const double* ax, int64_t nx, const double* ay, int64_t ny, const double* az, int64_t nz,
                               void* out) {
+    return ph_structured_grid_coords_device(ctx, B, cat, strategy, precision, PH_COORDS_CARTESIAN, ax, nx, ay, ny, az,
+                                            nz, out);
+}
+
+int ph_structured_grid_coords_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy,
+                                     int32_t precision, int32_t coords, const double* ax, int64_t nx,
+                                     const double* ay, int64_t ny, const double* az, int64_t nz, void* out) {
     return guarded(ctx, [&] {
+        need(coords >= PH_COORDS_CARTESIAN && coords <= PH_COORDS_SPHERICAL, PH_ERR_ARGUMENT,
+             "unknown coordinate system");
         need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
         need(strategy >= PH_GATHER && strategy <= PH_SYMMETRIC, PH_ERR_ARGUMENT, "unknown strategy");
         need(nx >= 0 && ny >= 0 && nz >= 0, PH_ERR_ARGUMENT, "negative grid size");
@@ -642,7 +651,7 @@ int ph_structured_grid_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat
         check_catalog(B, cat);
         const i64 S = nx * ny * nz;
         need(S == 0 || (ax && ay && az && out), PH_ERR_ARGUMENT, "null axis or output pointer");
-        GridAxes ga{ax, ay, az, nx, ny, nz};
+        GridAxes ga{ax, ay, az, nx, ny, nz, coords};
         run_chunks(ctx, cat, S, [&](const CatChunk& c, int L, u64* counter, hipEvent_t* ev) {
             launch_point_like(ctx, B, c, strategy, precision, nullptr, &ga, S, L, out, counter, ev);
         });
@@ -653,20 +662,31 @@ int ph_structured_grid_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat
 int ph_structured_grid(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
                        const double* ax, int64_t nx, const double* ay, int64_t ny, const double* az, int64_t nz,
                        double* out_host) {
+    return ph_structured_grid_coords(ctx, B, cat, strategy, precision, PH_COORDS_CARTESIAN, ax, nx, ay, ny, az, nz,
+                                     out_host);
+}
+
+int ph_structured_grid_coords(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy,
+                              int32_t precision, int32_t coords, const double* ax, int64_t nx, const double* ay,
+                              int64_t ny, const double* az, int64_t nz, double* out_host) {
     return guarded(ctx, [&] {
+        // angle axes hold (sin, cos) pairs
+        const int64_t my = coords == PH_COORDS_CARTESIAN ? ny : 2 * ny;
+        const int64_t mz = coords == PH_COORDS_SPHERICAL ? 2 * nz : nz;
         need(cat != nullptr, PH_ERR_ARGUMENT, "catalog is null");
         need(nx >= 0 && ny >= 0 && nz >= 0, PH_ERR_ARGUMENT, "negative grid size");
         const i64 S = nx * ny * nz;
         need(S == 0 || (ax && ay && az && out_host), PH_ERR_ARGUMENT, "null axis or output pointer");
         const int L = catalog_L(cat);
         DevBuf da, dout;
-        da.alloc(sizeof(double) * (size_t)(nx + ny + nz + 1));
+        da.alloc(sizeof(double) * (size_t)(nx + my + mz + 1));
         dout.alloc(out_elem(precision) * (size_t)L * (size_t)(S > 0 ? S : 1));
         double* d = da.as<double>();
         if (nx) PH_HIP(hipMemcpy(d, ax, sizeof(double) * nx, hipMemcpyHostToDevice));
-        if (ny) PH_HIP(hipMemcpy(d + nx, ay, sizeof(double) * ny, hipMemcpyHostToDevice));
-        if (nz) PH_HIP(hipMemcpy(d + nx + ny, az, sizeof(double) * nz, hipMemcpyHostToDevice));
-        int rc = ph_structured_grid_device(ctx, B, cat, strategy, precision, d, nx, d + nx, ny, d + nx + ny, nz, dout.p);
+        if (my) PH_HIP(hipMemcpy(d + nx, ay, sizeof(double) * my, hipMemcpyHostToDevice));
+        if (mz) PH_HIP(hipMemcpy(d + nx + my, az, sizeof(double) * mz, hipMemcpyHostToDevice));
+        int rc = ph_structured_grid_coords_device(ctx, B, cat, strategy, precision, coords, d, nx, d + nx, ny,
+                                                  d + nx + my, nz, dout.p);
         if (rc != PH_OK) return rc;
         if (S > 0) download(ctx, dout.p, precision, (int64_t)L * S, out_host);
         return PH_OK;

@@ -40,6 +40,9 @@ struct GridAxes {
     const double* ay;
     const double* az;
     i64 nx, ny, nz;
+    // PH_COORDS_CARTESIAN: axes x, y, z.  PH_COORDS_CYLINDRICAL: s, (sin, cos)
+    // pairs of phi, z.  PH_COORDS_SPHERICAL: r, pairs of phi, pairs of theta.
+    int coords = 0;
 };
 struct LineList {
     const double *ox, *oy, *oz, *dx, *dy, *dz;

@@ -462,9 +462,22 @@ template <class T> struct Sampler<T, 1> {
         const i64 ix = bx * 4 + (lane & 3), iy = by * 4 + ((lane >> 2) & 3), iz = bz * 4 + (lane >> 4);
         if (ix >= g.nx || iy >= g.ny || iz >= g.nz) return false;
         s = ix + g.nx * (iy + g.ny * iz);
-        x = T(g.ax[ix]);
-        y = T(g.ay[iy]);
-        z = T(g.az[iz]);
+        if (g.coords == PH_COORDS_CARTESIAN) {
+            x = T(g.ax[ix]);
+            y = T(g.ay[iy]);
+            z = T(g.az[iz]);
+        } else if (g.coords == PH_COORDS_CYLINDRICAL) {  // _cylin2cart (coordinate_transformations.jl:85-89)
+            const T sv = T(g.ax[ix]), sp = T(g.ay[2 * iy]), cp = T(g.ay[2 * iy + 1]);
+            x = sv * cp;
+            y = sv * sp;
+            z = T(g.az[iz]);
+        } else {  // _sph2cart (coordinate_transformations.jl:198-204)
+            const T r = T(g.ax[ix]), sp = T(g.ay[2 * iy]), cp = T(g.ay[2 * iy + 1]), st = T(g.az[2 * iz]),
+                    ct = T(g.az[2 * iz + 1]);
+            x = (r * st) * cp;
+            y = (r * st) * sp;
+            z = r * ct;
+        }
         return true;
     }
 };

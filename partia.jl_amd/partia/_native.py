@@ -36,11 +36,13 @@ EXPORTS = (
     "ph_batch_create", "ph_batch_run", "ph_batch_times", "ph_batch_destroy",
     "ph_line_samples_adjoint_device", "ph_line_grid_adjoint_device",
     "ph_line_polar_device", "ph_line_polar_adjoint_device",
+    "ph_structured_grid_coords", "ph_structured_grid_coords_device",
     "ph_point_samples_adjoint_device", "ph_structured_grid_adjoint_device",
     "ph_set_autotune", "ph_set_launch", "ph_autotune_report",
 )
 
 PH_REQ_POINTS, PH_REQ_PLANE, PH_REQ_BOX, PH_REQ_POLAR, PH_REQ_CYLINDER = 0, 1, 2, 3, 4
+PH_COORDS_CARTESIAN, PH_COORDS_CYLINDRICAL, PH_COORDS_SPHERICAL = 0, 1, 2
 
 
 class NativeLibraryError(RuntimeError):
@@ -130,6 +132,9 @@ def lib():
         "ph_line_grid_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, vp, vp, vp, vp, i64, vp, i64,
                                               vp, vp]),
         "ph_point_samples_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, i64, vp, vp, vp, vp, vp]),
+        "ph_structured_grid_coords": (i32, [vp, vp, P(ph_catalog), i32, i32, i32, vp, i64, vp, i64, vp, i64, vp]),
+        "ph_structured_grid_coords_device": (i32, [vp, vp, P(ph_catalog), i32, i32, i32, vp, i64, vp, i64, vp, i64,
+                                                   vp]),
         "ph_line_polar_device": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, vp, vp, vp, vp, i64, vp, i64, vp]),
         "ph_line_polar_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, vp, vp, vp, vp, i64, vp, i64,
                                                vp, vp]),

@@ -18,7 +18,8 @@ import numpy as np
 from . import _native as N
 from .catalog import InterpolationCatalog, to_concise_catalog
 from .errors import ArgumentError, MethodError
-from .grids import Cartesian, GridBundle, LineSamples, PointSamples, StructuredGrid, flatten, restore_struct
+from .grids import (Cartesian, Cylindrical, GridBundle, LineSamples, PointSamples, Spherical, StructuredGrid, device_axes,
+                    flatten, restore_struct)
 from .input import InterpolationInput, LinearBVH, LinearBVH_, matches_lbvh_leaf_order
 from .kernels import AbstractInterpolationStrategy, HIPComputeBackend, itpScatter, itpSymmetric
 
@@ -152,22 +153,21 @@ def StructuredGrid_interpolation(backend, coord, grid_template, inp, catalog, st
     _check_strategy(strategy)
     if not isinstance(grid_template, StructuredGrid) or len(grid_template.axes) != 3:
         raise MethodError("StructuredGrid_interpolation needs a 3D StructuredGrid template")
-    if coord is not Cartesian:
-        flat = flatten(coord, grid_template)
-        bundle = PointSamples_interpolation(backend, flat, inp, catalog, strategy, precision=precision)
-        T = bundle.datatype()
-        axes = tuple(np.asarray(a, dtype=T) for a in grid_template.axes)
-        return GridBundle(tuple(restore_struct(coord, g, axes) for g in bundle.grids), bundle.names)
-    # Cartesian: the flattened coordinates are generated on the device from the axes.
+    if coord not in (Cartesian, Cylindrical, Spherical):
+        raise MethodError("StructuredGrid_interpolation takes Cartesian, Cylindrical or Spherical grids")
+    # the flattened coordinates are generated on the device from the axes
+    # (angle axes as sin / cos pairs)
     lbvh = LinearBVH_(inp, np.uint64, backend)
     ctx = backend.context
     cn = to_concise_catalog(catalog).native()
     nx, ny, nz = grid_template.size
     S = nx * ny * nz
     out = np.zeros(max(cn.L, 1) * S)
-    ax, ay, az = (N.f64(a) for a in grid_template.axes)
-    ctx.check(N.lib().ph_structured_grid(ctx.handle, lbvh.handle, cn.ref(), strategy.id, _precision_of(inp, precision),
-                                         N.ptr(ax), nx, N.ptr(ay), ny, N.ptr(az), nz, N.ptr(out)))
+    cid, a1, a2, a3 = device_axes(coord, grid_template)
+    ax, ay, az = N.f64(a1), N.f64(a2), N.f64(a3)
+    ctx.check(N.lib().ph_structured_grid_coords(ctx.handle, lbvh.handle, cn.ref(), strategy.id,
+                                                _precision_of(inp, precision), cid, N.ptr(ax), nx, N.ptr(ay), ny,
+                                                N.ptr(az), nz, N.ptr(out)))
     lbvh.close()
     T = grid_template.dtype
     axes = tuple(np.asarray(a, dtype=T) for a in grid_template.axes)

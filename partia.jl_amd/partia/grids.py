@@ -267,6 +267,40 @@ def _mesh(axes):
     return np.meshgrid(*axes, indexing="ij")
 
 
+def trig_pairs(values):
+    """(sin v, cos v) pairs of axis values, flattened (math.sin / math.cos)."""
+    return np.array([f(float(v)) for v in values for f in (math.sin, math.cos)], dtype=np.float64)
+
+
+def device_axes(coord, grid):
+    """Axes in the layout of ph_structured_grid_coords: angle axes as
+    (sin, cos) pairs.  Returns (coords id, a1, a2, a3)."""
+    from . import _native as N
+    a = [np.asarray(x, dtype=np.float64) for x in grid.axes]
+    if coord is Cylindrical:
+        return N.PH_COORDS_CYLINDRICAL, a[0], trig_pairs(a[1]), a[2]
+    if coord is Spherical:
+        return N.PH_COORDS_SPHERICAL, a[0], trig_pairs(a[1]), trig_pairs(a[2])
+    return N.PH_COORDS_CARTESIAN, a[0], a[1], a[2]
+
+
+def coordinates_from_pairs(coord, grid):
+    """coordinate_grid with the sin / cos of device_axes, in the reference's
+    order of operations (the values the device generator produces)."""
+    c, a1, a2, a3 = device_axes(coord, grid)
+    n1, n2, n3 = grid.size
+    if c == 0:
+        return coordinate_grid(coord, grid)
+    I, J, K = np.meshgrid(np.arange(n1), np.arange(n2), np.arange(n3), indexing="ij")
+    I, J, K = I.ravel(order="F"), J.ravel(order="F"), K.ravel(order="F")
+    sp, cp = a2[2 * J], a2[2 * J + 1]
+    if coord is Cylindrical:
+        s = a1[I]
+        return (s * cp, s * sp, a3[K])
+    r, st, ct = a1[I], a3[2 * K], a3[2 * K + 1]
+    return ((r * st) * cp, (r * st) * sp, r * ct)
+
+
 def coordinate_grid(coord, grid):
     """Flattened (column-major) Cartesian coordinates of every grid point
     (StructuredGrid.jl:140-222)."""

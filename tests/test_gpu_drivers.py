@@ -180,6 +180,31 @@ def test_structured_grid_analytic(backend, coord, params):
     assert np.allclose(res.grids[4].grid.ravel(order="F"), 3.0, atol=1e-1, rtol=2e-2)
 
 
+@pytest.mark.parametrize("coord,params", [
+    (Cylindrical, [(0.0, 0.45, 9), (0.0, 2 * np.pi, 13), (0.2, 0.8, 6)]),
+    (Spherical, [(0.05, 0.45, 7), (0.3, 6.0, 11), (0.1, 3.0, 5)]),
+])
+def test_structured_grid_coords_generated_on_the_device(backend, coord, params):
+    """Cylindrical and spherical grids generated on the device
+    (ph_structured_grid_coords: _cylin2cart / _sph2cart with the axes' sin /
+    cos pairs) equal point samples on the same coordinates computed on the host
+    in the reference's order, bit for bit, and match coordinate_grid's
+    closed form."""
+    from partia import grids as G
+    g = StructuredGrid(coord, *params)
+    inp, cat = uniform_cloud()
+    res = StructuredGrid_interpolation(backend, coord, g, copy.deepcopy(inp), cat, itpSymmetric)
+    xyz = G.coordinates_from_pairs(coord, g)  # the part in x, y > 0 lies in the cloud
+    for a, b in zip(xyz, G.coordinate_grid(coord, g)):
+        np.testing.assert_allclose(a, b, atol=1e-12)
+    pts = PointSamples(np.zeros(len(xyz[0])), tuple(np.asarray(c) for c in xyz))
+    pres = PointSamples_interpolation(backend, pts, inp, cat, itpSymmetric)
+    for sg, pg in zip(res.grids, pres.grids):
+        assert sg.size == g.size
+        assert np.array_equal(sg.grid.ravel(order="F"), pg.grid, equal_nan=True)
+    assert np.isfinite(res.grids[0].grid).mean() > 0.2
+
+
 def test_empty_input_all_nan(fixtures, backend):
     fx = fixtures["empty_input"]
     inp = InterpolationInput(tuple(np.array(fx[k]) for k in "xyz"), np.array(fx["m"]), np.array(fx["h"]),
