@@ -247,6 +247,30 @@ int oracle_point_adjoint(void* hv, int slot, int shepard, int strategy, int64_t 
     return 0;
 }
 
+// field_adjoint over S points; cot: S x (1 or 3) row-major; grad: (5 + c) x n, input order.
+int oracle_field_adjoint(void* hv, int mode, const int* slots, int shepard, int strategy, int64_t S,
+                         const double* px, const double* py, const double* pz, const double* cot, double* grad) {
+    Handle* H = static_cast<Handle*>(hv);
+    const int c = mode >= 2 ? 3 : 1, w = mode == 0 || mode == 2 ? 1 : 3, rows = 5 + c;
+    auto run = [&](auto& in) {
+        using T = std::decay_t<decltype(in.x[0])>;
+        const int64_t n = (int64_t)in.x.size();
+        std::vector<T> g((size_t)rows * (size_t)n, T(0));
+        for (int64_t s = 0; s < S; ++s) {
+            T p[3] = {T(px[s]), T(py[s]), T(pz[s])};
+            T ct[3] = {T(cot[s * w]), T(w == 3 ? cot[s * w + 1] : 0), T(w == 3 ? cot[s * w + 2] : 0)};
+            field_adjoint<T>(in, mode, slots, shepard != 0, strategy, p, ct, g.data());
+        }
+        for (int r = 0; r < rows; ++r)
+            for (int64_t i = 0; i < n; ++i) grad[r * n + (int64_t)in.order[i] - 1] = double(g[r * n + i]);
+    };
+    if (H->precision == 64) run(H->d);
+    else run(H->f);
+    return 0;
+}
+
+double oracle_kernel_d2f(int K, double q) { return kernel_d2f<double>(K, q); }
+
 // LBVH_find_nearest_h per point; best (1-based leaf index) optional.
 int oracle_nearest_h(void* hv, int64_t S, const double* px, const double* py, const double* pz, double* ha,
                      int64_t* best) {

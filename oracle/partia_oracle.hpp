@@ -126,6 +126,47 @@ template <class T> inline T kernel_df(int K, T q) {
     return nan;
 }
 
+// Second derivatives f''(q) of the shape functions, from the mathematics of
+// the forms above (no reference counterpart: the adjoints of derivative
+// outputs need them).  M-splines: sums of c (a - q)^p per piece; Wendland
+// f = t^k g(q) with t = 1 - q/2: f'' = t^(k-2) (t^2 g'' - k t g' + k(k-1)/4 g).
+template <class T> inline T kernel_d2f(int K, T q) {
+    const T nan = std::numeric_limits<T>::quiet_NaN();
+    if (q < T(0)) return nan;
+    const T t = T(1) - T(0.5) * q;
+    switch (K) {
+    case M4:
+        if (q < T(1)) return T(1.5) * (T(2) - q) - T(6) * (T(1) - q);
+        if (q < T(2)) return T(1.5) * (T(2) - q);
+        return T(0);
+    case M5:
+        if (q < T(0.5)) return T(12) * p2(T(2.5) - q) - T(60) * p2(T(1.5) - q) + T(120) * p2(T(0.5) - q);
+        if (q < T(1.5)) return T(12) * p2(T(2.5) - q) - T(60) * p2(T(1.5) - q);
+        if (q < T(2.5)) return T(12) * p2(T(2.5) - q);
+        return T(0);
+    case M6:
+        if (q < T(1)) return T(20) * p3(T(3) - q) - T(120) * p3(T(2) - q) + T(300) * p3(T(1) - q);
+        if (q < T(2)) return T(20) * p3(T(3) - q) - T(120) * p3(T(2) - q);
+        if (q < T(3)) return T(20) * p3(T(3) - q);
+        return T(0);
+    case C2:  // k = 4, g = 2q + 1
+        if (q < T(2)) return p2(t) * (T(3) * (T(2) * q + T(1)) - T(8) * t);
+        return T(0);
+    case C4: {  // k = 6, g = 35/12 q^2 + 3q + 1
+        if (!(q < T(2))) return T(0);
+        const T g = (T(35) / T(12)) * p2(q) + T(3) * q + T(1), g1 = (T(35) / T(6)) * q + T(3), g2 = T(35) / T(6);
+        return p4(t) * (p2(t) * g2 - T(6) * t * g1 + T(7.5) * g);
+    }
+    case C6: {  // k = 8, g = 4q^3 + 6.25q^2 + 4q + 1
+        if (!(q < T(2))) return T(0);
+        const T g = T(4) * p3(q) + T(6.25) * p2(q) + T(4) * q + T(1), g1 = T(12) * p2(q) + T(12.5) * q + T(4),
+                g2 = T(24) * q + T(12.5);
+        return p6(t) * (p2(t) * g2 - T(8) * t * g1 + T(14) * g);
+    }
+    }
+    return nan;
+}
+
 // KernelFunctionValid (M4_spline.jl:66 and siblings).
 template <class T> inline T kernel_support(int K) {
     switch (K) {
@@ -970,6 +1011,150 @@ void point_adjoint(const Input<T>& in, int slot, bool shepard, int strategy, con
         }
     });
     if (strategy != SCATTER) g[2 * n + (nn - 1)] += cot * (shepard ? (HN - out * HS) / S1 : HN);
+}
+
+// Adjoint of one output group of the general interpolation kernel
+// (general_interpolation.jl:1-669; epilogue :128-214) at point p: mode 0 a
+// scalar of column slots[0] (Shepard flag shepard), 1 the gradient of
+// slots[0], 2 the divergence and 3 the curl of slots[0..2].  cot holds the
+// output's cotangent (1 or 3 values).  With v = m / rho, W_ab and its gradient
+// G_ab = grad_a W_ab of the strategy, S1 = sum v W and the Shepard means
+// At = sum A v W / S1, every output's first-order change for a change of
+// particle b is v_b (A_b - At) . u_b with u = P(G) + q W:
+//   scalar     P = 0,         q = g / S1 (Shepard; g, At = 0 without)
+//   gradient   P(G) = g . G,  q = -(g . b) / S1,       b = sum v G
+//   divergence P(G) = g G,    q = -g b / S1
+//   curl       P(G) = g x G,  q = (b x g) / S1
+// so d/dA_b = v u, d/dm_b = (A - At) . u / rho_b, and the h_b, x_b and h_a
+// rows follow from omega . dG + kappa dW with omega = P^T (A - At),
+// kappa = (A - At) . q.  G = D F(r, h), D = p - x_b, F = s3 h^-4 f'(q) / r:
+//   dF/dh = -s3 h^-5 (4 f' + q f'') / r,  dF/dr = s3 h^-4 (q f'' - f') / r^2,
+//   d(omega . G)/dx_b = -omega F - (omega . D) F_r D / r.
+// rho_b is held fixed; a particle at r = 0 contributes no position terms.
+// g rows (Morton order): m, the c field columns, h, x, y, z (c = 1 or 3).
+template <class T>
+void field_adjoint(const Input<T>& in, int mode, const int* slots, bool shepard, int strategy, const T p[3],
+                   const T* cot, T* g) {
+    const int K = in.kernel;
+    const T Kvalid = kernel_support<T>(K);
+    const T sig = kernel_sigma3<T>(K);
+    const i64 n = (i64)in.x.size();
+    const int c = mode >= 2 ? 3 : 1;
+    const std::vector<T>* A[3] = {&in.quant[(size_t)slots[0] - 1], c == 3 ? &in.quant[(size_t)slots[1] - 1] : nullptr,
+                                  c == 3 ? &in.quant[(size_t)slots[2] - 1] : nullptr};
+    const bool sym = strategy == SYMMETRIC;
+    const T wa = sym ? T(0.5) : (strategy == GATHER ? T(1) : T(0));
+    const T wb = sym ? T(0.5) : (strategy == SCATTER ? T(1) : T(0));
+    T ha = T(0);
+    i64 nn = 0;
+    if (strategy != SCATTER) {
+        ha = nearest_h<T>(in.bvh, p, &nn);
+        if (nn == 0) return;
+    }
+    // kernel pieces at (r, h): W, W_r, F, F_r, dW/dh, dF/dh
+    struct KP { T W, Wr, F, Fr, Wh, Fh; };
+    auto kp = [&](T r, T h) {
+        const T ih = T(1) / h, q = r * ih, f = kernel_f<T>(K, q), f1 = kernel_df<T>(K, q), f2 = kernel_d2f<T>(K, q);
+        const T s3 = sig * ih * ih * ih, s4 = s3 * ih, s5 = s4 * ih;
+        KP o{};
+        o.W = s3 * f;
+        o.Wr = s4 * f1;
+        o.Wh = -s4 * (T(3) * f + q * f1);
+        if (r > T(0)) {
+            o.F = s4 * f1 / r;
+            o.Fr = s4 * (q * f2 - f1) / (r * r);
+            o.Fh = -s5 * (T(4) * f1 + q * f2) / r;
+        }
+        return o;
+    };
+    auto field = [&](size_t i, T a[3]) {
+        for (int j = 0; j < c; ++j) a[j] = (*A[j])[i];
+    };
+    // pass 1: S1, the field sums and b = sum v G
+    T S1 = T(0), SA[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
+    point_traversal<T>(in.bvh, p, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
+        const size_t i = (size_t)k - 1;
+        const T r = std::sqrt(d2), v = in.m[i] / in.rho[i];
+        const T D[3] = {p[0] - in.x[i], p[1] - in.y[i], p[2] - in.z[i]};
+        KP ka{}, kb{};
+        if (wa != T(0)) ka = kp(r, ha);
+        if (wb != T(0)) kb = kp(r, hb);
+        const T W = wa * ka.W + wb * kb.W, F = wa * ka.F + wb * kb.F;
+        T a[3];
+        field(i, a);
+        S1 += v * W;
+        for (int j = 0; j < c; ++j) SA[j] += a[j] * v * W;
+        for (int d = 0; d < 3; ++d) b[d] += v * D[d] * F;
+    });
+    if (S1 == T(0)) return;
+    T At[3] = {T(0), T(0), T(0)}, q[3] = {T(0), T(0), T(0)};
+    const T gs = cot[0];
+    if (mode == 0) {
+        if (shepard) At[0] = SA[0] / S1;
+        q[0] = shepard ? gs / S1 : gs;
+    } else if (mode == 1) {
+        At[0] = SA[0] / S1;
+        q[0] = -(cot[0] * b[0] + cot[1] * b[1] + cot[2] * b[2]) / S1;
+    } else {
+        for (int j = 0; j < 3; ++j) At[j] = SA[j] / S1;
+        if (mode == 2) {
+            for (int j = 0; j < 3; ++j) q[j] = -gs * b[j] / S1;
+        } else {
+            const T* gv = cot;  // (b x g) / S1
+            q[0] = (b[1] * gv[2] - b[2] * gv[1]) / S1;
+            q[1] = (b[2] * gv[0] - b[0] * gv[2]) / S1;
+            q[2] = (b[0] * gv[1] - b[1] * gv[0]) / S1;
+        }
+    }
+    T dha = T(0);
+    point_traversal<T>(in.bvh, p, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
+        const size_t i = (size_t)k - 1;
+        const T r = std::sqrt(d2), rb = in.rho[i], v = in.m[i] / rb;
+        const T D[3] = {p[0] - in.x[i], p[1] - in.y[i], p[2] - in.z[i]};
+        KP ka{}, kb{};
+        if (wa != T(0)) ka = kp(r, ha);
+        if (wb != T(0)) kb = kp(r, hb);
+        const T W = wa * ka.W + wb * kb.W, F = wa * ka.F + wb * kb.F;
+        const T G[3] = {D[0] * F, D[1] * F, D[2] * F};
+        T a[3];
+        field(i, a);
+        for (int j = 0; j < c; ++j) a[j] -= At[j];
+        // u = P(G) + q W and omega = P^T a
+        T u[3] = {T(0), T(0), T(0)}, om[3] = {T(0), T(0), T(0)};
+        if (mode == 0) {
+            u[0] = q[0] * W;
+        } else if (mode == 1) {
+            u[0] = cot[0] * G[0] + cot[1] * G[1] + cot[2] * G[2] + q[0] * W;
+            for (int d = 0; d < 3; ++d) om[d] = a[0] * cot[d];
+        } else if (mode == 2) {
+            for (int d = 0; d < 3; ++d) u[d] = cot[0] * G[d] + q[d] * W;
+            for (int d = 0; d < 3; ++d) om[d] = cot[0] * a[d];
+        } else {
+            u[0] = cot[1] * G[2] - cot[2] * G[1] + q[0] * W;
+            u[1] = cot[2] * G[0] - cot[0] * G[2] + q[1] * W;
+            u[2] = cot[0] * G[1] - cot[1] * G[0] + q[2] * W;
+            om[0] = a[1] * cot[2] - a[2] * cot[1];  // a x g
+            om[1] = a[2] * cot[0] - a[0] * cot[2];
+            om[2] = a[0] * cot[1] - a[1] * cot[0];
+        }
+        T au = T(0), kap = T(0);
+        for (int j = 0; j < c; ++j) {
+            au += a[j] * u[j];
+            kap += a[j] * q[j];
+        }
+        const T omD = om[0] * D[0] + om[1] * D[1] + om[2] * D[2];
+        g[0 * n + (i64)i] += au / rb;
+        for (int j = 0; j < c; ++j) g[(1 + j) * n + (i64)i] += v * u[j];
+        // h_b and h_a: omega . dG/dh + kappa dW/dh, dG/dh = D dF/dh
+        if (wb != T(0)) g[(1 + c) * n + (i64)i] += v * wb * (omD * kb.Fh + kap * kb.Wh);
+        if (wa != T(0)) dha += v * wa * (omD * ka.Fh + kap * ka.Wh);
+        if (r > T(0)) {
+            const T Fr = wa * ka.Fr + wb * kb.Fr, Wr = wa * ka.Wr + wb * kb.Wr;
+            for (int d = 0; d < 3; ++d)
+                g[(2 + c + d) * n + (i64)i] += v * (-om[d] * F - omD * Fr * D[d] / r - kap * Wr * D[d] / r);
+        }
+    });
+    if (strategy != SCATTER) g[(1 + c) * n + (nn - 1)] += dha;
 }
 
 }  // namespace oracle

@@ -41,6 +41,9 @@ def lib():
         L.oracle_nearest_h.argtypes = [vp, i64, vp, vp, vp, vp, vp]
         L.oracle_line_adjoint.argtypes = [vp, i32, i32, i64] + [vp] * 6 + [vp, vp]
         L.oracle_point_adjoint.argtypes = [vp, i32, i32, i32, i64] + [vp] * 3 + [vp, vp]
+        L.oracle_field_adjoint.argtypes = [vp, i32, vp, i32, i32, i64] + [vp] * 3 + [vp, vp]
+        L.oracle_kernel_d2f.argtypes = [i32, d]
+        L.oracle_kernel_d2f.restype = d
         L.oracle_neighbors.restype = i64
         L.oracle_neighbors.argtypes = [vp, i32, i32, vp, vp, d, vp, vp, i64]
         for name in ("oracle_kernel_f", "oracle_kernel_df"):
@@ -171,6 +174,20 @@ class OracleSet:
                                    _p(c), _p(g))
         return g
 
+    def field_adjoint(self, mode, slots, shepard, strategy, px, py, pz, cot):
+        """(5 + c) x n gradients (input order; rows m, the c field columns, h, x,
+        y, z) of sum_s cot_s . out_s for one output group: mode 0 scalar, 1
+        gradient, 2 divergence, 3 curl of the 1-based column slots."""
+        px, py, pz = _f64(px), _f64(py), _f64(pz)
+        c = 3 if mode >= 2 else 1
+        S = len(px)
+        ct = _f64(np.asarray(cot, dtype=np.float64).reshape(S, -1))
+        sl = _i32(list(slots) + [0] * (3 - len(slots)))
+        g = np.zeros((5 + c, self.n))
+        lib().oracle_field_adjoint(self.h, int(mode), _p(sl), int(bool(shepard)), int(strategy), S, _p(px), _p(py),
+                                   _p(pz), _p(ct), _p(g))
+        return g
+
     def nearest_h(self, px, py, pz):
         px, py, pz = _f64(px), _f64(py), _f64(pz)
         S = len(px)
@@ -186,6 +203,10 @@ class OracleSet:
         idx, d2 = np.zeros(cap, np.int64), np.zeros(cap)
         cnt = lib().oracle_neighbors(self.h, mode, strategy, _p(a), _p(dv), float(ha), _p(idx), _p(d2), cap)
         return idx[:cnt], d2[:cnt]
+
+
+def kernel_d2f(K, q):
+    return lib().oracle_kernel_d2f(KERNELS.get(K, K) if isinstance(K, str) else K, q)
 
 
 def kernel_f(K, q):

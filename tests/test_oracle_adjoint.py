@@ -152,3 +152,89 @@ def test_point_adjoint_linear_in_the_cotangent_and_empty_points():
         # a point far from every particle (NaN output) contributes nothing
         far = o.point_adjoint(2, True, st, [5.0], [5.0], [5.0], [1.0])
         assert not far.any()
+
+
+# ---- gradient, divergence and curl outputs at points ----
+
+FIELDS = {1: dict(gslots=[2]), 2: dict(dslots=[[2, 3, 4]]), 3: dict(cslots=[[2, 3, 4]])}
+
+
+def field_cloud(n, seed):
+    p = cloud(n, seed)
+    x, y, z = p[0], p[1], p[2]
+    B = 0.5 + np.cos(2 * np.pi * z) * x
+    Cq = y * y - 0.3 * np.sin(2 * np.pi * x * z)
+    return p + [B, Cq]  # x y z m h rho A B C
+
+
+def field_forward(p, kernel, strategy, mode, pts, cot):
+    x, y, z, m, h, rho, A, B, Cq = p
+    o = O.OracleSet(x, y, z, m, h, rho, [rho, A, B, Cq], kernel=kernel)
+    out = o.point_samples(FIELDS[mode], strategy, *pts)  # (1 or 3) x S
+    out = np.where(np.isnan(out), 0.0, out)
+    return float(np.sum(cot.T * out))
+
+
+def test_kernel_second_derivative_matches_differences_of_the_first():
+    """f'' of every kernel against central differences of f' (kernel.jl's
+    derivative functions), away from the knots where f'' jumps."""
+    for K in range(6):
+        R = O.support(K)
+        for q in np.linspace(0.013, R - 0.011, 97):
+            e = 1e-6
+            fd = (O.kernel_df(K, q + e) - O.kernel_df(K, q - e)) / (2 * e)
+            assert abs(fd - O.kernel_d2f(K, q)) <= 1e-6 * (1 + abs(fd)), (K, q, fd, O.kernel_d2f(K, q))
+        assert O.kernel_d2f(K, R + 0.1) == 0.0
+
+
+@pytest.mark.parametrize("strategy", list(STRATS))
+@pytest.mark.parametrize("mode,name", [(1, "M4"), (2, "C4"), (3, "M4"), (1, "M6"), (3, "C4")])
+def test_field_adjoint_matches_finite_differences(strategy, mode, name):
+    """d/d(m, field columns, h, x, y, z) of sum_s cot_s . out_s for the gradient,
+    divergence and curl outputs against central differences of the oracle's
+    forward interpolation."""
+    kernel, st = KERNELS[name], STRATS[strategy]
+    p = field_cloud(260, 31 + kernel + 7 * mode)
+    n = len(p[0])
+    pts = points(90, 23 + mode)
+    c = 1 if mode == 2 else 3
+    cot = np.random.default_rng(5 + mode).normal(size=(pts.shape[1], c))
+    o = O.OracleSet(*p[:6], [p[5], p[6], p[7], p[8]], kernel=kernel)
+    slots = [2] if mode == 1 else [2, 3, 4]
+    g = o.field_adjoint(mode, slots, False, st, *pts, cot)
+    nc = len(slots)
+    assert g.shape == (5 + nc, n)
+    rng = np.random.default_rng(17)
+    reached = np.flatnonzero(np.abs(g).sum(axis=0))
+    assert len(reached) > 20
+    picks = np.concatenate([rng.choice(reached, 14, replace=False), rng.choice(n, 3, replace=False)])
+    if st != O.SCATTER:
+        ha, best = o.nearest_h(*pts)
+        order = o.tree()["order"].astype(np.int64)
+        picks = np.concatenate([picks, order[best[:3] - 1] - 1])
+    params = [(0, 3, "rel")] + [(1 + j, 6 + j, "rel") for j in range(nc)] + [(1 + nc, 4, "rel")] + \
+        [(2 + nc + d, d, "h") for d in range(3)]
+    for row, pi, kind in params:
+        scale = max(np.sqrt(np.mean(g[row] ** 2)), 1e-300)
+        for b in picks:
+            base = p[pi][b]
+            eps = 1e-6 * (abs(base) if kind == "rel" else p[4][b])
+            q_plus = [a.copy() for a in p]
+            q_minus = [a.copy() for a in p]
+            q_plus[pi][b] = base + eps
+            q_minus[pi][b] = base - eps
+            fd = (field_forward(q_plus, kernel, st, mode, pts, cot) -
+                  field_forward(q_minus, kernel, st, mode, pts, cot)) / (2 * eps)
+            assert abs(fd - g[row, b]) <= 3e-4 * scale + 1e-6 * abs(g[row, b]), (row, b, fd, g[row, b], scale)
+
+
+def test_field_adjoint_scalar_mode_is_the_point_adjoint():
+    p = field_cloud(200, 8)
+    o = O.OracleSet(*p[:6], [p[5], p[6], p[7], p[8]], kernel=0)
+    pts = points(40, 9)
+    cot = np.random.default_rng(2).normal(size=40)
+    for st in STRATS.values():
+        for shep in (False, True):
+            a = o.point_adjoint(2, shep, st, *pts, cot)
+            b = o.field_adjoint(0, [2], shep, st, *pts, cot)
+            np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-13 * np.abs(a).max())
