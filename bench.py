@@ -87,7 +87,19 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
     ap.add_argument("--workload", choices=["grid256", "column2048", "grid64", "batch512", "adjoint1024", "gridadjoint256"],
                     default="grid256")
+    ap.add_argument("--output", choices=list(ADJ_OUTPUTS), default="scalar",
+                    help="output group of gridadjoint256: the Shepard scalar A, grad A, div v or curl v")
     return ap.parse_args()
+
+
+# gridadjoint256's output groups: catalog over the columns (rho, A, vx, vy, vz),
+# values per sample, field columns (gradient rows between m and h) and their names
+ADJ_OUTPUTS = {
+    "scalar": (dict(sslots=[2], sflags=[1]), 1, ["A"], "A (Shepard)"),
+    "gradient": (dict(gslots=[2]), 3, ["A"], "grad A"),
+    "divergence": (dict(dslots=[3, 4, 5]), 1, ["vx", "vy", "vz"], "div v"),
+    "curl": (dict(cslots=[3, 4, 5]), 3, ["vx", "vy", "vz"], "curl v"),
+}
 
 
 class Dev:
@@ -612,8 +624,9 @@ def adjoint1024(args):
 
 
 def gridadjoint256(args):
-    """Forward + adjoint of the Shepard-normalised scalar A of the 2e6-particle
-    Plummer sphere on the 256^3 grid (C4, symmetric), float64."""
+    """Forward + adjoint of one output group (--output: the Shepard-normalised
+    scalar A, grad A, div v or curl v) of the 2e6-particle Plummer sphere on the
+    256^3 grid (C4, symmetric), float64."""
     npart, ng = NPART, NGRID
     print("bench: generating %d Plummer particles" % npart, file=sys.stderr, flush=True)
     data = synthetic.plummer_sphere(npart, seed=2)
@@ -622,26 +635,28 @@ def gridadjoint256(args):
     ctx = N.Context(0)
     apply_launch(ctx, args)
     L = N.lib()
-    cat = N.CatalogArgs(sslots=[2], sflags=[1])
+    catkw, width, fields, oname = ADJ_OUTPUTS[args.output]
+    cat = N.CatalogArgs(**catkw)
+    rows = 5 + len(fields)
     dax = Dev(ctx, 8 * ng)
     dax.upload(axes)
     prec = N.PH_PREC_FP64
 
     def build(dct):
-        dc = upload_columns(ctx, dct, ["x", "y", "z", "m", "h", "rho", "rho", "A"])
-        q = (C.c_void_p * 2)(dc[6].p.value, dc[7].p.value)
+        dc = upload_columns(ctx, dct, ["x", "y", "z", "m", "h", "rho", "rho", "A", "vx", "vy", "vz"])
+        q = (C.c_void_p * 5)(*[d.p.value for d in dc[6:]])
         h = C.c_void_p()
-        ctx.check(L.ph_lbvh_build_device(ctx.handle, npart, *[d.p for d in dc[:6]], 2, q, KERNEL_C4, 63, prec,
+        ctx.check(L.ph_lbvh_build_device(ctx.handle, npart, *[d.p for d in dc[:6]], 5, q, KERNEL_C4, 63, prec,
                                          C.byref(h)))
         for d in dc:
             d.free()
         return h
 
-    dout = Dev(ctx, 8 * S)
-    cot = synthetic.Stream(77).normal(S)
-    dcot = Dev(ctx, 8 * S)
+    dout = Dev(ctx, 8 * width * S)
+    cot = synthetic.Stream(77).normal(width * S)  # component-major, as the forward's output rows
+    dcot = Dev(ctx, 8 * width * S)
     dcot.upload(cot)
-    dgrad = Dev(ctx, 8 * 6 * npart)
+    dgrad = Dev(ctx, 8 * rows * npart)
 
     def forward(h):
         ctx.check(L.ph_structured_grid_device(ctx.handle, h, cat.ref(), N.PH_SYMMETRIC, prec, dax.p, ng, dax.p, ng,
@@ -666,24 +681,24 @@ def gridadjoint256(args):
     ms, res = timed_steps(ctx, step, args.steps, args.warmup, "gridadjoint256")
     fwd = float(np.mean([r["forward"] for r in res]))
     bwd = float(np.mean([r["backward"] for r in res]))
-    g = get(dgrad, 6 * npart).reshape(6, npart)
+    g = get(dgrad, rows * npart).reshape(rows, npart)
     L.ph_lbvh_destroy(h)
 
     def objective(dct):  # sum_s cot_s out_s over samples with neighbours
         hh = build(dct)
         forward(hh)
         L.ph_lbvh_destroy(hh)
-        return float(np.nansum(cot * get(dout, S)))
+        return float(np.nansum(cot * get(dout, width * S)))
 
     fd_err, fd_checks = {}, 0
     if not args.no_fp64_compare:
         rng = np.random.default_rng(256)
         picks = rng.choice(np.flatnonzero(np.abs(g[0]) > 0), 12, replace=False)
-        for row, key in ((0, "m"), (1, "A"), (2, "h"), (3, "x")):
+        for row, key in [(0, "m")] + [(1 + j, f) for j, f in enumerate(fields)] + [(rows - 4, "h"), (rows - 3, "x")]:
             scale = float(np.sqrt(np.mean(g[row] ** 2)))
             worst = 0.0
             for b in picks:
-                eps = 1e-5 * (abs(data[key][b]) if key != "x" else data["h"][b])
+                eps = 1e-5 * (abs(data[key][b]) if key not in ("x", "vx", "vy", "vz") else data["h"][b])
                 dp, dmn = dict(data), dict(data)
                 dp[key] = data[key].copy()
                 dmn[key] = data[key].copy()
@@ -697,12 +712,13 @@ def gridadjoint256(args):
         print("bench: adjoint vs central differences (max |fd - adj| / rms adj): %s" % fd_err, file=sys.stderr,
               flush=True)
     result = {
-        "metric": "GPU adjoint samples/s, float64 grid scalar 256^3 (d/dm, dA, dh, dr); backward/forward time ratio",
+        "metric": "GPU adjoint samples/s, float64 grid %s 256^3 (d/dm, dA, dh, dr); backward/forward time ratio" %
+                  args.output,
         "value": S / (bwd * 1e-3), "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded Plummer sphere, seeded cotangent, SplitMix64)",
         "config": {"workload": "plummer_2e6_C4_symmetric_grid256_adjoint", "particles": npart, "grid": [ng] * 3,
-                   "kernel": "C4_Wendland", "strategy": "symmetric", "output": "A (Shepard)", "precision": "fp64"},
+                   "kernel": "C4_Wendland", "strategy": "symmetric", "output": oname, "precision": "fp64"},
         "forward_ms": fwd, "backward_ms": bwd, "backward_over_forward": bwd / fwd,
         "fd_check": {"particles": 12, "checks": fd_checks, "step": "1e-5 relative (positions: 1e-5 h)",
                      "max_err_over_rms": fd_err},

@@ -332,7 +332,16 @@ int ph_line_polar_adjoint_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog
  * are recomputed with the forward's float64 test; terms are evaluated in
  * float64 and summed with float64 atomics (sums are order-dependent in the last
  * bits).  precision: PH_PREC_FP64 (cot_dev float64) or PH_PREC_FP32C (cot_dev
- * float32), on a tree built from Float64 columns. ---- */
+ * float32), on a tree built from Float64 columns.
+ * The catalog may instead hold exactly one gradient (of a column, slot >= 1;
+ * the density-gradient branch of slot 0 is not provided), one divergence or
+ * one curl (general_interpolation.jl:128-214 with accumulations/{gradient,
+ * divergence,curl}_accumulation.jl).  cot_dev then holds the output's 3 (or,
+ * for the divergence, 1) rows of S values, laid out as the forward writes
+ * them, and grad_dev has 5 + c rows: d/dm, d/d of each of the c field columns
+ * (c = 1 for a gradient, 3 for a divergence or curl, in slot order), d/dh,
+ * d/dx, d/dy, d/dz.  The Shepard means inside these outputs are
+ * differentiated too. ---- */
 int ph_point_samples_adjoint_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy,
                                     int32_t precision, int64_t S, const double* px_dev, const double* py_dev,
                                     const double* pz_dev, const void* cot_dev, double* grad_dev);

@@ -871,22 +871,33 @@ int ph_line_polar_adjoint_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* 
                              grad, 1);
 }
 
-// Adjoint of a one-scalar point / grid interpolation (shared by both forms).
+// Adjoint of one output group of point / grid interpolation (shared by both forms).
 static void run_points_adjoint(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
                                const PointList* pl, const GridAxes* ga, i64 S, const void* cot, double* grad) {
     need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
     need(strategy >= PH_GATHER && strategy <= PH_SYMMETRIC, PH_ERR_ARGUMENT, "unknown strategy");
     check_catalog(B, cat);
-    need(cat->n_scalar == 1 && cat->n_grad == 0 && cat->n_div == 0 && cat->n_curl == 0, PH_ERR_ARGUMENT,
-         "the point adjoint takes a catalog of exactly one scalar");
+    need(cat->n_scalar + cat->n_grad + cat->n_div + cat->n_curl == 1, PH_ERR_ARGUMENT,
+         "the point adjoint takes a catalog of exactly one scalar, gradient, divergence or curl");
     check_precision(B, precision);
     need(precision != PH_PREC_FP32, PH_ERR_ARGUMENT, "the point adjoint runs in PH_PREC_FP64 or PH_PREC_FP32C");
     need(grad != nullptr && (S == 0 || cot != nullptr), PH_ERR_ARGUMENT, "null cotangent or gradient pointer");
-    const i32 qsrc = 5 + cat->scalar_slots[0];
-    const bool shep = cat->scalar_shepard[0] != 0;
+    AdjointOut ao;
+    if (cat->n_scalar == 1) {
+        ao.q[0] = 5 + cat->scalar_slots[0];
+        ao.shepard = cat->scalar_shepard[0] != 0;
+    } else if (cat->n_grad == 1) {
+        need(cat->grad_slots[0] >= 1, PH_ERR_ARGUMENT, "the adjoint of the density gradient (slot 0) is not provided");
+        ao.mode = 1;
+        ao.q[0] = 5 + cat->grad_slots[0];
+    } else {
+        const int32_t* sl = cat->n_div == 1 ? cat->div_slots : cat->curl_slots;
+        ao.mode = cat->n_div == 1 ? 2 : 3;
+        for (int j = 0; j < 3; ++j) ao.q[j] = 5 + sl[j];
+    }
     LaunchScope scope(ctx->launch);  // block order
     PH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
-    launch_points_adjoint(B, strategy, qsrc, shep, precision, pl, ga, S, cot, grad, ctx->stream, &ctx->ev[4]);
+    launch_points_adjoint(B, strategy, ao, precision, pl, ga, S, cot, grad, ctx->stream, &ctx->ev[4]);
     PH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
     PH_HIP(hipEventSynchronize(ctx->ev[1]));
     ctx->stats.interp_ms = elapsed(ctx, 0, 1);

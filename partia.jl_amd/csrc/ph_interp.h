@@ -74,11 +74,22 @@ void ensure_buckets(ph_lbvh* B, hipStream_t s);
 // the caller has packed B->rec32 for c32 and computed ha (gather / symmetric).
 void launch_tile_c32(ph_lbvh* B, const CatChunk& c32, int strategy, const GridAxes& ga, i64 S, const double* ha,
                      void* out, u64* counters, hipStream_t s, hipEvent_t* ev);
-// Adjoint of one scalar output of point (pl) or grid (ga) interpolation
-// (ph_adjoint.hip): grad[6][n] float64, overwritten, input order; cot holds S
-// values of the forward's output type (double for PH_PREC_FP64, float for
-// PH_PREC_FP32C).  qsrc: the scalar's column in B->cols.
-void launch_points_adjoint(ph_lbvh* B, int strategy, i32 qsrc, bool shepard, int precision, const PointList* pl,
+// One output group of point / grid interpolation for the adjoint: a scalar
+// (mode 0, Shepard flag shepard), the gradient of a column (1), or the
+// divergence (2) or curl (3) of three; q: the columns' indices in B->cols.
+struct AdjointOut {
+    int mode = 0;
+    i32 q[3] = {0, 0, 0};
+    bool shepard = false;
+    __host__ __device__ int fields() const { return mode >= 2 ? 3 : 1; }  // field columns c
+    __host__ __device__ int width() const { return mode == 1 || mode == 3 ? 3 : 1; }  // values per sample
+};
+// Adjoint of one output group of point (pl) or grid (ga) interpolation
+// (ph_adjoint.hip): grad[5 + c][n] float64, overwritten, input order; cot
+// holds width() x S values of the forward's output type (double for
+// PH_PREC_FP64, float for PH_PREC_FP32C), component-major like the forward's
+// output rows.
+void launch_points_adjoint(ph_lbvh* B, int strategy, const AdjointOut& ao, int precision, const PointList* pl,
                            const GridAxes* ga, i64 S, const void* cot, double* grad, hipStream_t s, hipEvent_t* ev);
 // Nearest-particle smoothing length of every grid point (NaN: no particle
 // within Kd max(h)), one wave per tile seeding its lanes' searches

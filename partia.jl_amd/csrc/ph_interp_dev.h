@@ -497,14 +497,13 @@ template <class T> __device__ inline T wave_sum(T v) {
     return v;
 }
 
-// Sums of six per-lane values over the wave, reduce-scatter style: value v
-// (0..5) ends up, summed over all 64 lanes, in lanes 8v .. 8v+7 (returned by
-// each lane for v = lane >> 3; lanes 48..63 return 0).  10 cross-lane moves
-// instead of 36 for six separate wave sums; the additions are exact
-// float64 sums in a fixed tree order.
-__device__ inline double wave_sum6(const double in[6]) {
+// Sums of eight per-lane values over the wave, reduce-scatter style: value v
+// (0..7) ends up, summed over all 64 lanes, in lanes 8v .. 8v+7 (returned by
+// each lane for v = lane >> 3).  10 cross-lane moves instead of 48 for eight
+// separate wave sums; the additions are float64 sums in a fixed tree order.
+__device__ inline double wave_sum8(const double in[8]) {
     const int lane = threadIdx.x & (kWave - 1);
-    double v[8] = {in[0], in[1], in[2], in[3], in[4], in[5], 0.0, 0.0};
+    double v[8] = {in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7]};
     {  // 8 -> 4 values: lanes with bit 5 set keep values 4..7
         const bool hi = lane & 32;
 #pragma unroll
@@ -532,6 +531,12 @@ __device__ inline double wave_sum6(const double in[6]) {
 #pragma unroll
     for (int d = 4; d > 0; d >>= 1) v[0] += __shfl_xor(v[0], d, kWave);
     return v[0];
+}
+
+// The same for six values (lanes 48..63 return 0).
+__device__ inline double wave_sum6(const double in[6]) {
+    const double v[8] = {in[0], in[1], in[2], in[3], in[4], in[5], 0.0, 0.0};
+    return wave_sum8(v);
 }
 
 __device__ inline void count_add(u64* counter, u64 v) {

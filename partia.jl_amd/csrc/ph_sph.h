@@ -91,6 +91,41 @@ template <class T, int K> __host__ __device__ inline T shape_df(T q) {
     }
 }
 
+// Second derivative f''(q), the derivative of shape_df (used by the adjoints of
+// gradient, divergence and curl outputs).  Wendland shapes t^k g(q), t = 1 - q/2:
+// f'' = t^(k-2) (t^2 g'' - k t g' + k (k - 1) / 4 g).
+template <class T, int K> __host__ __device__ inline T shape_d2f(T q) {
+    if (q < T(0)) return sph_nan<T>();
+    const T t = T(1) - T(0.5) * q;
+    if constexpr (K == PH_KERNEL_M4) {
+        if (q < T(1)) return T(1.5) * (T(2) - q) - T(6) * (T(1) - q);
+        if (q < T(2)) return T(1.5) * (T(2) - q);
+        return T(0);
+    } else if constexpr (K == PH_KERNEL_M5) {
+        if (q < T(0.5)) return T(12) * q2(T(2.5) - q) - T(60) * q2(T(1.5) - q) + T(120) * q2(T(0.5) - q);
+        if (q < T(1.5)) return T(12) * q2(T(2.5) - q) - T(60) * q2(T(1.5) - q);
+        if (q < T(2.5)) return T(12) * q2(T(2.5) - q);
+        return T(0);
+    } else if constexpr (K == PH_KERNEL_M6) {
+        if (q < T(1)) return T(20) * q3(T(3) - q) - T(120) * q3(T(2) - q) + T(300) * q3(T(1) - q);
+        if (q < T(2)) return T(20) * q3(T(3) - q) - T(120) * q3(T(2) - q);
+        if (q < T(3)) return T(20) * q3(T(3) - q);
+        return T(0);
+    } else if constexpr (K == PH_KERNEL_C2) {  // k = 4, g = 2q + 1
+        if (q < T(2)) return q2(t) * (T(3) * (T(2) * q + T(1)) - T(8) * t);
+        return T(0);
+    } else if constexpr (K == PH_KERNEL_C4) {  // k = 6, g = 35/12 q^2 + 3q + 1
+        if (!(q < T(2))) return T(0);
+        const T g = (T(35) / T(12)) * q2(q) + T(3) * q + T(1), g1 = (T(35) / T(6)) * q + T(3);
+        return q4(t) * (q2(t) * (T(35) / T(6)) - T(6) * t * g1 + T(7.5) * g);
+    } else {  // k = 8, g = 4q^3 + 6.25q^2 + 4q + 1
+        if (!(q < T(2))) return T(0);
+        const T g = T(4) * q3(q) + T(6.25) * q2(q) + T(4) * q + T(1), g1 = T(12) * q2(q) + T(12.5) * q + T(4),
+                g2 = T(24) * q + T(12.5);
+        return q6(t) * (q2(t) * g2 - T(8) * t * g1 + T(14) * g);
+    }
+}
+
 template <int K> __host__ __device__ constexpr double support_of() {
     return K == PH_KERNEL_M5 ? 2.5 : (K == PH_KERNEL_M6 ? 3.0 : 2.0);
 }
