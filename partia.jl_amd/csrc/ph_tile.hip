@@ -23,6 +23,12 @@
 namespace ph {
 namespace {
 
+// A double in LDS (address space 3).
+typedef __attribute__((address_space(3))) double LdsDouble;
+
+// Record width of the C2 layout {vi, s0, s1, d0, d1, d2}.
+constexpr int kC2Rec = 6;
+
 // Dynamic LDS of k_tile_c32 for record width rw: per wave 64 staged leaves of
 // (x, y, z, h, record) doubles, then 64 leaf indices per wave.
 inline size_t tile_lds_bytes(int rw) {
@@ -41,9 +47,10 @@ __global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const Bucket
                                                      float* __restrict__ out, u64* counter) {
     extern __shared__ double tile_smem[];
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    const int rw = c.rw;
+    const int rw = EXACT == 2 ? kC2Rec : c.rw;  // the C2 layout: constant LDS offsets
     const int slot = 4 + rw;  // doubles per staged leaf
-    double* st = tile_smem + (size_t)wv * kWave * slot;
+    // an LDS-typed pointer: 32-bit offsets in the inner loops
+    LdsDouble* st = (LdsDouble*)tile_smem + wv * kWave * slot;
     u32* lidx = reinterpret_cast<u32*>(tile_smem + (size_t)(kBlock / kWave) * kWave * slot) + wv * kWave;
 
     i64 s = 0;
@@ -102,7 +109,7 @@ __global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const Bucket
         if (lane < nl) {
             const u32 k = lidx[lane];
             const LeafRec<double> L = bv.leaves[k];
-            double* d = st + (size_t)lane * slot;
+            LdsDouble* d = st + lane * slot;
             d[0] = L.x;
             d[1] = L.y;
             d[2] = L.z;
@@ -113,21 +120,23 @@ __global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const Bucket
         }
         wave_sync();
         // the reference's float64 leaf test of every staged leaf, per lane:
-        // d2 <= max((K h_a)^2, (K h_b)^2) as two compares; the mask is built a
-        // 32-bit half at a time (uniform bit constants)
-        auto take = [&](int j) {
-            const double* d = st + (size_t)j * slot;
+        // d2 <= (K h_a)^2 or d2 <= (K h_b)^2, both compares evaluated (no
+        // control flow).  The leaves are tested from the last to the first and
+        // shifted into 32-bit halves, so leaf j lands at bit j; lanes without
+        // neighbours clear their mask afterwards.
+        auto take = [&](int j) -> u32 {
+            const LdsDouble* d = st + j * slot;
             const double dx = p[0] - d[0], dy = p[1] - d[1], dz = p[2] - d[2];
             const double d2 = (dx * dx + dy * dy) + dz * dz;  // d2_point's order
-            if constexpr (STRAT == PH_GATHER) return live_lane && d2 <= r2a;
-            if constexpr (STRAT == PH_SCATTER) return live_lane && d2 <= d[3];
-            return live_lane && (d2 <= d[3] || d2 <= r2a);
+            if constexpr (STRAT == PH_GATHER) return d2 <= r2a ? 1u : 0u;
+            if constexpr (STRAT == PH_SCATTER) return d2 <= d[3] ? 1u : 0u;
+            return (d2 <= d[3] ? 1u : 0u) | (d2 <= r2a ? 1u : 0u);
         };
         u32 mlo = 0, mhi = 0;
         const int nlo = nl < 32 ? nl : 32;
-        for (int j = 0; j < nlo; ++j) mlo |= take(j) ? (1u << j) : 0u;
-        for (int j = 32; j < nl; ++j) mhi |= take(j) ? (1u << (j - 32)) : 0u;
-        u64 mask = ((u64)mhi << 32) | mlo;
+        for (int j = nl - 1; j >= 32; --j) mhi = mhi + mhi + take(j);
+        for (int j = nlo - 1; j >= 0; --j) mlo = mlo + mlo + take(j);
+        u64 mask = live_lane ? ((u64)mhi << 32) | mlo : 0;
         accepted += __popcll(mask);
         // lockstep: one neighbour per lane and iteration, lowest bit first
         // (increasing leaf index); a lane with none left adds zeros
@@ -138,7 +147,7 @@ __global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const Bucket
             const bool live = mask != 0;
             const int b = __builtin_ctzll(mask | (1ull << 63));
             mask &= mask - 1;
-            const double* d = st + (size_t)(b < nl ? b : nl - 1) * slot;  // a staged (finite) leaf
+            const LdsDouble* d = st + ((b < nl ? b : nl - 1) & (kWave - 1)) * slot;  // a staged (finite) leaf
             const double dx = p[0] - d[0], dy = p[1] - d[1], dz = p[2] - d[2];
             acc.add_r(live, c, [&](int pos) { return d[4 + pos]; }, dx, dy, dz, (dx * dx + dy * dy) + dz * dz, iha2,
                       Wa_norm, Ga_norm);
@@ -246,7 +255,8 @@ void launch_tile_ks(const BucketView& bv, const double* rec, const CatChunk& c32
     // the per-lane kernel's selection (ph_bucket.hip), so the arithmetic is the same
     const bool small = c32.ng + c32.nd + c32.nc == 0 && c32.ns <= 2;
     if (small) launch_tile_t<K, STRAT, 2, 0, 0, 0, 0>(bv, rec, c32, ga, S, ha, out, counters, s, ev);
-    else if (c2_shared(c32)) launch_tile_t<K, STRAT, 2, 1, 1, 1, 2>(bv, rec, c32, ga, S, ha, out, counters, s, ev);
+    else if (c2_shared(c32) && c32.rw == kC2Rec)
+        launch_tile_t<K, STRAT, 2, 1, 1, 1, 2>(bv, rec, c32, ga, S, ha, out, counters, s, ev);
     else launch_tile_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0>(bv, rec, c32, ga, S, ha, out, counters, s, ev);
 }
 
