@@ -193,6 +193,21 @@ int ph_lbvh_build_device(ph_context* ctx, int64_t n, const double* x_dev, const 
                          const double* m_dev, const double* h_dev, const double* rho_dev, int32_t ncol,
                          const double* const* cols_dev, int32_t kernel, int32_t code_bits, int32_t precision,
                          ph_lbvh** out);
+/* LinearBVH! of an InterpolationInput{2} (InterpolationInput.jl:224-242):
+ * MortonEncoding(x, y; CodeType) with 16 (code_bits 30, UInt32) or 32 (code_bits
+ * 63, UInt64) bits per axis (MortonEncoding.jl:80-86, toolbox.jl:77-92,
+ * 166-168, 264-272), the same radix tree and refit.  The tree stores z = 0 for
+ * every particle, so its boxes and distances are the 2D ones; ph_nearest_h
+ * answers 2D queries (pz may be null).  The interpolation kernels are 3D only
+ * (README.md:152): interpolation entry points reject a 2D tree. */
+int ph_lbvh_build2d(ph_context* ctx, int64_t n, const double* x_host, const double* y_host, const double* m_host,
+                    const double* h_host, const double* rho_host, int32_t ncol, const double* const* cols_host,
+                    int32_t kernel, int32_t code_bits, int32_t precision, ph_lbvh** out);
+int ph_lbvh_build2d_device(ph_context* ctx, int64_t n, const double* x_dev, const double* y_dev, const double* m_dev,
+                           const double* h_dev, const double* rho_dev, int32_t ncol, const double* const* cols_dev,
+                           int32_t kernel, int32_t code_bits, int32_t precision, ph_lbvh** out);
+/* Dimension of a tree's input: 3, or 2 for ph_lbvh_build2d[_device]. */
+int ph_lbvh_dims(const ph_lbvh* bvh, int32_t* dims);
 int ph_lbvh_destroy(ph_lbvh* bvh);
 /* Replace the m, rho and quantity columns of a built tree (host arrays already
  * in Morton order, i.e. permuted as LinearBVH! left them).  The external-LBVH

@@ -92,6 +92,35 @@ void* oracle_build(int precision, int64_t n, const double* x, const double* y, c
     }
 }
 
+// LinearBVH! of an InterpolationInput{2}: as oracle_build with z all zeros
+// and 2D Morton codes.
+void* oracle_build2d(int precision, int64_t n, const double* x, const double* y, const double* m, const double* h,
+                     const double* rho, int ncol, const double* const* cols, int kernel, int code_bits) {
+    try {
+        auto H = std::make_unique<Handle>();
+        H->precision = precision;
+        H->code_bits = code_bits;
+        const std::vector<double> z((size_t)std::max<int64_t>(n, 0), 0.0);
+        if (precision == 64) {
+            fill_input(H->d, n, x, y, z.data(), m, h, rho, ncol, cols, kernel);
+            H->d.dims = 2;
+            if (code_bits == 30) linear_bvh_inplace<double, u32>(H->d);
+            else linear_bvh_inplace<double, u64>(H->d);
+            H->lut_d.reset(new LineLUT<double>(ph_lut_inorm));
+        } else {
+            fill_input(H->f, n, x, y, z.data(), m, h, rho, ncol, cols, kernel);
+            H->f.dims = 2;
+            if (code_bits == 30) linear_bvh_inplace<float, u32>(H->f);
+            else linear_bvh_inplace<float, u64>(H->f);
+            H->lut_f.reset(new LineLUT<float>(ph_lut_inorm));
+        }
+        return H.release();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
 void oracle_free(void* h) { delete static_cast<Handle*>(h); }
 
 // Tree and encoding arrays (unified IDs, reference layout).  Arrays of length
@@ -333,6 +362,8 @@ float oracle_line_W32(int K, float r, float h) {
 }
 uint32_t oracle_expand_bits3_u32(uint32_t x) { return expand_bits3(x); }
 uint64_t oracle_expand_bits3_u64(uint64_t x) { return expand_bits3(x); }
+uint32_t oracle_expand_bits2_u32(uint32_t x) { return expand_bits2(x); }
+uint64_t oracle_expand_bits2_u64(uint64_t x) { return expand_bits2(x); }
 
 // _find_range / _split_position over a caller-provided sorted code array.
 void oracle_find_range_u64(const uint64_t* codes, int64_t n, int64_t i, int64_t* lo, int64_t* hi) {

@@ -280,13 +280,33 @@ inline u64 expand_bits3(u64 x) {  // toolbox.jl:47-55
 template <class TI> inline TI encode3(TI ix, TI iy, TI iz) {  // toolbox.jl:113-118
     return (expand_bits3(ix) << 2) | (expand_bits3(iy) << 1) | expand_bits3(iz);
 }
-template <class TI> constexpr int axis_bits() { return (int(sizeof(TI) * 8) - int(sizeof(TI) * 8) % 3) / 3; }
+// 2D interleave (x0)(y0)(x1)(y1)...: toolbox.jl:77-92, 166-168.
+inline u32 expand_bits2(u32 x) {
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+inline u64 expand_bits2(u64 x) {
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+template <class TI> inline TI encode2(TI ix, TI iy) { return (expand_bits2(ix) << 1) | expand_bits2(iy); }
+// _axis_bits(Val(D), T) (toolbox.jl:264-267): bits per axis of a D-dimensional code.
+template <class TI> constexpr int axis_bits(int D = 3) {
+    return (int(sizeof(TI) * 8) - int(sizeof(TI) * 8) % D) / D;
+}
 
-// _normalize_vector + _quantize_coords (toolbox.jl:281-294, 311-336).
-template <class T, class TI> void quantize_axis(const std::vector<T>& v, std::vector<TI>& out) {
+// _normalize_vector + _quantize_coords (toolbox.jl:281-294, 311-336, 366-391).
+template <class T, class TI> void quantize_axis(const std::vector<T>& v, std::vector<TI>& out, int D = 3) {
     size_t n = v.size();
     out.resize(n);
-    T scale = T(std::exp2(T(axis_bits<TI>()))) - T(1);
+    T scale = T(std::exp2(T(axis_bits<TI>(D)))) - T(1);
     T vmin = v[0], vmax = v[0];
     for (size_t i = 1; i < n; ++i) {  // minimum / maximum
         vmin = std::min(vmin, v[i]);
@@ -324,19 +344,22 @@ template <class T, class TI> struct Encoding {
 
 // MortonEncoding(x, y, z; CodeType) -- MortonEncoding.jl:36-43 + sort_by_morton!
 // (toolbox.jl:20-33): stable sort by code (sortperm's Perm ordering breaks ties
-// by index), permute codes/order/coords, then rewrite duplicates upward.
+// by index), permute codes/order/coords, then rewrite duplicates upward.  With
+// D = 2, MortonEncoding(x, y; CodeType) (MortonEncoding.jl:80-86): z is not
+// encoded (the caller's z is all zeros and stays so).
 template <class T, class TI>
-Encoding<T, TI> morton_encoding(const std::vector<T>& x, const std::vector<T>& y, const std::vector<T>& z) {
+Encoding<T, TI> morton_encoding(const std::vector<T>& x, const std::vector<T>& y, const std::vector<T>& z,
+                                int D = 3) {
     size_t n = x.size();
     // minimum/maximum of an empty column throw in the reference (and the tree needs n >= 1)
     if (n == 0) throw std::invalid_argument("BinaryRadixTree: enc.codes must be non-empty (got n=0).");
     std::vector<TI> ix, iy, iz;
-    quantize_axis<T, TI>(x, ix);
-    quantize_axis<T, TI>(y, iy);
-    quantize_axis<T, TI>(z, iz);
+    quantize_axis<T, TI>(x, ix, D);
+    quantize_axis<T, TI>(y, iy, D);
+    if (D == 3) quantize_axis<T, TI>(z, iz, D);
     std::vector<TI> code(n), order(n);
     for (size_t i = 0; i < n; ++i) {
-        code[i] = encode3<TI>(ix[i], iy[i], iz[i]);
+        code[i] = D == 3 ? encode3<TI>(ix[i], iy[i], iz[i]) : encode2<TI>(ix[i], iy[i]);
         order[i] = TI(i + 1);
     }
     std::vector<size_t> p(n);
@@ -699,14 +722,16 @@ template <class T> inline T nearest_h(const LBVH<T>& b, const T p[3], i64* best_
 // ---------------------------------------------------------------------------
 template <class T> struct Input {
     int kernel = M4;
+    int dims = 3;  // 2: InterpolationInput{2} (z all zeros)
     std::vector<T> x, y, z, m, h, rho;  // Morton order after LinearBVH!
     std::vector<std::vector<T>> quant;
     std::vector<u64> order, codes;
     LBVH<T> bvh;
 };
 
+// LinearBVH! (InterpolationInput.jl:201-221; 2D :224-242).
 template <class T, class TI> void linear_bvh_inplace(Input<T>& in) {
-    Encoding<T, TI> e = morton_encoding<T, TI>(in.x, in.y, in.z);
+    Encoding<T, TI> e = morton_encoding<T, TI>(in.x, in.y, in.z, in.dims);
     size_t n = in.x.size();
     auto perm = [&](std::vector<T>& v) {
         std::vector<T> w(n);

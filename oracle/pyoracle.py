@@ -63,6 +63,12 @@ def lib():
         L.oracle_expand_bits3_u32.argtypes = [C.c_uint32]
         L.oracle_expand_bits3_u64.restype = C.c_uint64
         L.oracle_expand_bits3_u64.argtypes = [C.c_uint64]
+        L.oracle_expand_bits2_u32.restype = C.c_uint32
+        L.oracle_expand_bits2_u32.argtypes = [C.c_uint32]
+        L.oracle_expand_bits2_u64.restype = C.c_uint64
+        L.oracle_expand_bits2_u64.argtypes = [C.c_uint64]
+        L.oracle_build2d.restype = vp
+        L.oracle_build2d.argtypes = [i32, i64] + [vp] * 5 + [i32, vp, i32, i32]
         L.oracle_find_range_u64.argtypes = [vp, i64, i64, vp, vp]
         L.oracle_split_position_u64.restype = i64
         L.oracle_split_position_u64.argtypes = [vp, i64, i64, i64]
@@ -85,17 +91,26 @@ def _i32(a):
 class OracleSet:
     """One InterpolationInput after LinearBVH! on the CPU oracle."""
 
-    def __init__(self, x, y, z, m, h, rho, cols=(), kernel="M4_spline", code_bits=63, precision=64):
+    def __init__(self, x, y, z, m, h, rho, cols=(), kernel="M4_spline", code_bits=63, precision=64, dims=3):
+        """dims=2: an InterpolationInput{2} (z is ignored; the tree's z is 0)."""
         L = lib()
         self.n = len(x)
         self.ncol = len(cols)
         self.kernel = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
         self.precision = precision
-        arrs = [_f64(a) for a in (x, y, z, m, h, rho)]
+        self.dims = dims
         cl = [_f64(c) for c in cols]
         ptrs = (C.c_void_p * max(1, len(cl)))(*[c.ctypes.data for c in cl]) if cl else None
-        self._keep = arrs + cl
-        self.h = L.oracle_build(precision, self.n, *[_p(a) for a in arrs], len(cl), ptrs, self.kernel, code_bits)
+        if dims == 2:
+            arrs = [_f64(a) for a in (x, y, m, h, rho)]
+            self._keep = arrs + cl
+            self.h = L.oracle_build2d(precision, self.n, *[_p(a) for a in arrs], len(cl), ptrs, self.kernel,
+                                      code_bits)
+        else:
+            arrs = [_f64(a) for a in (x, y, z, m, h, rho)]
+            self._keep = arrs + cl
+            self.h = L.oracle_build(precision, self.n, *[_p(a) for a in arrs], len(cl), ptrs, self.kernel,
+                                    code_bits)
         if not self.h:
             raise ValueError(L.oracle_last_error().decode())
 

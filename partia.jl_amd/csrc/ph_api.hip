@@ -67,6 +67,9 @@ float elapsed(ph_context* ctx, int a, int b) {
 }
 
 void check_catalog(const ph_lbvh* B, const ph_catalog* cat) {
+    // the reference's interpolation kernels are 3D only (README.md:152)
+    need(B->dims == 3, PH_ERR_ARGUMENT,
+         "interpolation needs a 3D tree: this one was built from an InterpolationInput{2}");
     need(cat != nullptr, PH_ERR_ARGUMENT, "catalog is null");
     need(cat->n_scalar >= 0 && cat->n_grad >= 0 && cat->n_div >= 0 && cat->n_curl >= 0, PH_ERR_ARGUMENT,
          "negative catalog size");
@@ -409,7 +412,7 @@ int ph_synchronize(ph_context* ctx) {
 }
 
 static int build_common(ph_context* ctx, int64_t n, const double* const* src, bool host_src, int32_t ncol,
-                        int32_t kernel, int32_t code_bits, int32_t precision, ph_lbvh** out) {
+                        int32_t kernel, int32_t code_bits, int32_t precision, ph_lbvh** out, int32_t dims = 3) {
     return guarded(ctx, [&] {
         need(ctx != nullptr && out != nullptr, PH_ERR_ARGUMENT, "null context or output");
         *out = nullptr;
@@ -421,9 +424,11 @@ static int build_common(ph_context* ctx, int64_t n, const double* const* src, bo
         need(code_bits == 30 || code_bits == 63, PH_ERR_ARGUMENT, "code_bits must be 30 (UInt32) or 63 (UInt64)");
         need(precision == PH_PREC_FP64 || precision == PH_PREC_FP32, PH_ERR_ARGUMENT,
              "tree precision must be PH_PREC_FP64 or PH_PREC_FP32");
-        for (int c = 0; c < 6 + ncol; ++c) need(src[c] != nullptr, PH_ERR_ARGUMENT, "null column pointer");
+        for (int c = 0; c < 6 + ncol; ++c)
+            need(src[c] != nullptr || (c == 2 && dims == 2), PH_ERR_ARGUMENT, "null column pointer");
         auto B = std::make_unique<ph_lbvh>();
         B->ctx = ctx;
+        B->dims = dims;
         B->n = n;
         B->ncol = ncol;
         B->kernel = kernel;
@@ -455,6 +460,28 @@ int ph_lbvh_build_device(ph_context* ctx, int64_t n, const double* x, const doub
     std::vector<const double*> src = {x, y, z, m, h, rho};
     for (int c = 0; c < ncol; ++c) src.push_back(cols ? cols[c] : nullptr);
     return build_common(ctx, n, src.data(), false, ncol, kernel, code_bits, precision, out);
+}
+
+int ph_lbvh_build2d(ph_context* ctx, int64_t n, const double* x, const double* y, const double* m,
+                    const double* h, const double* rho, int32_t ncol, const double* const* cols, int32_t kernel,
+                    int32_t code_bits, int32_t precision, ph_lbvh** out) {
+    std::vector<const double*> src = {x, y, nullptr, m, h, rho};
+    for (int c = 0; c < ncol; ++c) src.push_back(cols ? cols[c] : nullptr);
+    return build_common(ctx, n, src.data(), true, ncol, kernel, code_bits, precision, out, 2);
+}
+
+int ph_lbvh_build2d_device(ph_context* ctx, int64_t n, const double* x, const double* y, const double* m,
+                           const double* h, const double* rho, int32_t ncol, const double* const* cols,
+                           int32_t kernel, int32_t code_bits, int32_t precision, ph_lbvh** out) {
+    std::vector<const double*> src = {x, y, nullptr, m, h, rho};
+    for (int c = 0; c < ncol; ++c) src.push_back(cols ? cols[c] : nullptr);
+    return build_common(ctx, n, src.data(), false, ncol, kernel, code_bits, precision, out, 2);
+}
+
+int ph_lbvh_dims(const ph_lbvh* B, int32_t* dims) {
+    if (!B || !dims) return PH_ERR_ARGUMENT;
+    *dims = B->dims;
+    return PH_OK;
 }
 
 int ph_lbvh_destroy(ph_lbvh* B) {
@@ -936,14 +963,16 @@ int ph_nearest_h(ph_context* ctx, ph_lbvh* B, int64_t S, const double* px, const
                  double* ha) {
     return guarded(ctx, [&] {
         need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
-        need(S >= 0 && (S == 0 || (px && py && pz && ha)), PH_ERR_ARGUMENT, "bad arguments");
+        // pz may be null for a 2D tree (points in the plane z = 0)
+        need(S >= 0 && (S == 0 || (px && py && (pz || B->dims == 2) && ha)), PH_ERR_ARGUMENT, "bad arguments");
         if (S == 0) return PH_OK;
         DevBuf d;
         d.alloc(sizeof(double) * 4 * (size_t)S);
         double* p = d.as<double>();
         PH_HIP(hipMemcpy(p, px, sizeof(double) * S, hipMemcpyHostToDevice));
         PH_HIP(hipMemcpy(p + S, py, sizeof(double) * S, hipMemcpyHostToDevice));
-        PH_HIP(hipMemcpy(p + 2 * S, pz, sizeof(double) * S, hipMemcpyHostToDevice));
+        if (pz) PH_HIP(hipMemcpy(p + 2 * S, pz, sizeof(double) * S, hipMemcpyHostToDevice));
+        else PH_HIP(hipMemset(p + 2 * S, 0, sizeof(double) * S));
         PointList pl{p, p + S, p + 2 * S};
         launch_nearest_h(B, pl, S, p + 3 * S, ctx->stream);
         PH_HIP(hipStreamSynchronize(ctx->stream));

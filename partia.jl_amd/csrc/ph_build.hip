@@ -21,9 +21,18 @@ namespace ph {
 namespace {
 
 template <class TI> struct CodeTraits;
+// expand / expand2: the 3D and 2D bit spreads of toolbox.jl:38-55, 77-92.
 template <> struct CodeTraits<u32> {
     static constexpr int axis_bits = 10;
+    static constexpr int axis_bits2 = 16;
     static constexpr int sort_bits = 32;
+    __device__ static u32 expand2(u32 x) {
+        x = (x | (x << 8)) & 0x00FF00FFu;
+        x = (x | (x << 4)) & 0x0F0F0F0Fu;
+        x = (x | (x << 2)) & 0x33333333u;
+        x = (x | (x << 1)) & 0x55555555u;
+        return x;
+    }
     __device__ static u32 expand(u32 x) {
         x = (x | (x << 16)) & 0x30000ffu;
         x = (x | (x << 8)) & 0x300f00fu;
@@ -35,7 +44,16 @@ template <> struct CodeTraits<u32> {
 };
 template <> struct CodeTraits<u64> {
     static constexpr int axis_bits = 21;
+    static constexpr int axis_bits2 = 32;
     static constexpr int sort_bits = 64;
+    __device__ static u64 expand2(u64 x) {
+        x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+        x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+        x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+        x = (x | (x << 2)) & 0x3333333333333333ull;
+        x = (x | (x << 1)) & 0x5555555555555555ull;
+        return x;
+    }
     __device__ static u64 expand(u64 x) {
         x = (x | (x << 32)) & 0x1f00000000ffffull;
         x = (x | (x << 16)) & 0x1f0000ff0000ffull;
@@ -49,9 +67,12 @@ template <> struct CodeTraits<u64> {
 
 __device__ inline int clz64(u64 v) { return v ? __clzll((long long)v) : 64; }
 
-// Quantise one coordinate: floor(scale * ((v - vmin) * inv(vmax - vmin))), 0.5 when flat.
-template <class T, class TI> __device__ inline TI quantise(T v, T vmin, T vmax) {
-    const T scale = T((1ull << CodeTraits<TI>::axis_bits) - 1);  // exp2(b) - 1, exact in T
+// Quantise one coordinate: floor(scale * ((v - vmin) * inv(vmax - vmin))), 0.5
+// when flat; scale = exp2(b) - 1 rounded to T (as Julia's exp2(T(b)) - one(T):
+// 2^32 - 1 becomes 2^32 in Float32).
+template <class T, class TI, int BITS = CodeTraits<TI>::axis_bits>
+__device__ inline TI quantise(T v, T vmin, T vmax) {
+    const T scale = T((1ull << BITS) - 1);
     T dv = vmax - vmin;
     T f;
     if (dv == T(0)) {
@@ -75,6 +96,20 @@ __global__ void __launch_bounds__(kBlock) k_encode(const T* __restrict__ x, cons
     idx[i] = (u32)i;
 }
 
+// 2D codes (MortonEncoding(x, y), MortonEncoding.jl:80-86; toolbox.jl:166-168).
+template <class T, class TI>
+__global__ void __launch_bounds__(kBlock) k_encode2(const T* __restrict__ x, const T* __restrict__ y, i64 n,
+                                                    const T* __restrict__ mm6, TI* __restrict__ codes,
+                                                    u32* __restrict__ idx) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    constexpr int b = CodeTraits<TI>::axis_bits2;
+    TI ix = quantise<T, TI, b>(x[i], mm6[0], mm6[3]);
+    TI iy = quantise<T, TI, b>(y[i], mm6[1], mm6[4]);
+    codes[i] = (CodeTraits<TI>::expand2(ix) << 1) | CodeTraits<TI>::expand2(iy);
+    idx[i] = (u32)i;
+}
+
 // Gather every column into Morton order: dst[c][i] = src[c][idx[i]].
 template <class TS, class TD>
 __global__ void __launch_bounds__(kBlock) k_permute(const TS* const* __restrict__ src, TD* __restrict__ dst, int ncols, i64 n,
@@ -86,10 +121,39 @@ __global__ void __launch_bounds__(kBlock) k_permute(const TS* const* __restrict_
 }
 
 // Dedup as a prefix max: key_i = c_i + 2^31 - i (never negative for i < 2^31).
-template <class TI> __global__ void k_dedup_keys(const TI* codes, u64* keys, i64 n) {
+// Codes from here up (2D UInt64 codes use all 64 bits) may make c + 2^31 - i
+// overflow; they are rewritten by k_dedup_tail instead.
+constexpr u64 kTailCode = ~0ull - (1ull << 32) + 1;  // 2^64 - 2^32
+
+template <class TI> __global__ void k_dedup_keys(const TI* codes, u64* keys, i64 n, u32* first) {
     const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     keys[i] = (u64)codes[i] + (1ull << 31) - (u64)i;
+    if (sizeof(TI) == 8 && (u64)codes[i] >= kTailCode) atomicMin(first, (u32)i);
+}
+// The reference's sequential rewrite (codes[i] = codes[i-1] + 1, wrapping at
+// 2^64, when codes[i] <= codes[i-1]) from the first tail code on, seeded with
+// the scan's exact value before it; writes keys in k_dedup_apply's form.  One
+// thread: the tail holds only the codes within 2^32 of the top.
+template <class TI> __global__ void k_dedup_tail(const TI* codes, u64* keys, i64 n, const u32* first) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const i64 f = (i64)*first;
+    if (f >= n) return;
+    i64 i = f;
+    u64 prev;
+    if (i == 0) {
+        prev = (u64)codes[0];
+        keys[0] = prev + (1ull << 31);
+        i = 1;
+    } else {
+        prev = keys[i - 1] - (1ull << 31) + (u64)(i - 1);
+    }
+    for (; i < n; ++i) {
+        const u64 c = (u64)codes[i];
+        const u64 v = c <= prev ? prev + 1ull : c;
+        keys[i] = v + (1ull << 31) - (u64)i;
+        prev = v;
+    }
 }
 template <class TI> __global__ void k_dedup_apply(TI* codes, const u64* keys, u64* codes64, i64 n) {
     const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
@@ -324,6 +388,7 @@ void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipS
     W.mm.alloc(sizeof(T) * 8);
     W.keys.alloc(align(sizeof(TI) * n));
     W.dk.alloc(sizeof(u64) * n);
+    W.flag.alloc(sizeof(u32) * 64);
     W.adj.alloc(sizeof(i32) * n);
     W.visited.alloc(sizeof(u32) * (n - 1) + sizeof(u32) * 64 + sizeof(i32) * n);
     W.tmp.alloc(align(16 * (size_t)n) + (size_t(1) << 20));  // radix sort, scans, min/max
@@ -341,8 +406,12 @@ void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipS
 
     // 1-2: min/max, encode
     minmax3<T>(src_cols[0], src_cols[1], src_cols[2], n, W.mm.as<T>(), W.tmp, s);
-    k_encode<T, TI><<<g, kBlock, 0, s>>>(src_cols[0], src_cols[1], src_cols[2], n, W.mm.as<T>(), W.keys.as<TI>(),
-                                         B->order.as<u32>());
+    if (B->dims == 2)
+        k_encode2<T, TI><<<g, kBlock, 0, s>>>(src_cols[0], src_cols[1], n, W.mm.as<T>(), W.keys.as<TI>(),
+                                              B->order.as<u32>());
+    else
+        k_encode<T, TI><<<g, kBlock, 0, s>>>(src_cols[0], src_cols[1], src_cols[2], n, W.mm.as<T>(),
+                                             W.keys.as<TI>(), B->order.as<u32>());
     PH_HIP(hipGetLastError());
     // 3: stable sort by code
     radix_sort_pairs(W.keys.as<TI>(), B->order.as<u32>(), n, CodeTraits<TI>::sort_bits, W.tmp, s);
@@ -350,9 +419,14 @@ void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipS
     k_permute<T, T><<<g, kBlock, 0, s>>>(W.ptrs.as<const T*>(), B->cols.as<T>(), ncols, n, B->order.as<u32>());
     PH_HIP(hipGetLastError());
     // 5: duplicate rewrite
-    k_dedup_keys<TI><<<g, kBlock, 0, s>>>(W.keys.as<TI>(), W.dk.as<u64>(), n);
+    u32* first = W.flag.as<u32>();
+    k_fill_u32<<<1, kBlock, 0, s>>>(first, 1, 0xffffffffu);
+    PH_HIP(hipGetLastError());
+    k_dedup_keys<TI><<<g, kBlock, 0, s>>>(W.keys.as<TI>(), W.dk.as<u64>(), n, first);
     PH_HIP(hipGetLastError());
     inclusive_max_u64(W.dk.as<u64>(), n, W.tmp, s);
+    k_dedup_tail<TI><<<1, 64, 0, s>>>(W.keys.as<TI>(), W.dk.as<u64>(), n, first);
+    PH_HIP(hipGetLastError());
     k_dedup_apply<TI><<<g, kBlock, 0, s>>>(W.keys.as<TI>(), W.dk.as<u64>(), B->codes.as<u64>(), n);
     PH_HIP(hipGetLastError());
 
@@ -409,16 +483,24 @@ void build_lbvh(ph_lbvh* B, const double* const* src64, bool host_src, hipStream
     const i64 n = B->n;
     const int ncols = 6 + B->ncol;
     std::vector<const double*> dptr(ncols);
+    // a 2D input has no z column: the tree's z is a column of zeros
+    const bool zero_z = B->dims == 2;
     if (host_src) {
         // stage host columns on the device (float64, original order)
         W.stage.alloc(sizeof(double) * (size_t)ncols * n);
         for (int c = 0; c < ncols; ++c) {
             double* d = W.stage.as<double>() + (size_t)c * n;
-            PH_HIP(hipMemcpyAsync(d, src64[c], sizeof(double) * n, hipMemcpyHostToDevice, s));
+            if (c == 2 && zero_z) PH_HIP(hipMemsetAsync(d, 0, sizeof(double) * n, s));
+            else PH_HIP(hipMemcpyAsync(d, src64[c], sizeof(double) * n, hipMemcpyHostToDevice, s));
             dptr[c] = d;
         }
     } else {
         for (int c = 0; c < ncols; ++c) dptr[c] = src64[c];
+        if (zero_z) {
+            W.stage.alloc(sizeof(double) * n);
+            PH_HIP(hipMemsetAsync(W.stage.p, 0, sizeof(double) * n, s));
+            dptr[2] = W.stage.as<double>();
+        }
     }
     if (B->precision == PH_PREC_FP32) {
         W.stage32.alloc(sizeof(float) * (size_t)ncols * n);

@@ -223,6 +223,7 @@ struct ph_lbvh {
     int32_t kernel = 0;
     int32_t code_bits = 63;
     int32_t precision = PH_PREC_FP64;  // PH_PREC_FP64 or PH_PREC_FP32 columns
+    int32_t dims = 3;   // 2: InterpolationInput{2} (2D Morton codes, z column all zeros)
     int32_t root = 0;
     // Morton order columns: x,y,z,m,h,rho then ncol quantity columns, (6+ncol) x n
     ph::DevBuf cols;

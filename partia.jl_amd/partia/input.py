@@ -18,12 +18,15 @@ from .kernels import M5_spline, AbstractSPHKernel, HIPComputeBackend, CPUCompute
 
 
 class InterpolationInput:
-    """InterpolationInput{3, T, V, K, NCOLUMN}: SoA particle columns."""
+    """InterpolationInput{D, T, V, K, NCOLUMN}: SoA particle columns, D = 3 or 2.
+
+    A 2D input supports LinearBVH! and the tree queries; the interpolation
+    kernels are 3D only (README.md:152) and reject its tree."""
 
     def __init__(self, coord, m, h, rho, quant=(), smoothed_kernel=M5_spline):
         coord = tuple(coord)
-        if len(coord) != 3:
-            raise MethodError("only 3D inputs are supported by the interpolation kernels")
+        if len(coord) not in (2, 3):
+            raise MethodError("InterpolationInput takes 2D or 3D coordinates")
         if not (isinstance(smoothed_kernel, type) and issubclass(smoothed_kernel, AbstractSPHKernel)):
             raise MethodError("smoothed_kernel must be an SPH kernel type")
         npart = len(m)
@@ -38,6 +41,7 @@ class InterpolationInput:
             if len(q) != npart:
                 raise DimensionMismatch("quant[%d] length %d != Nparticles %d" % (j + 1, len(q), npart))
         self.Npart = npart
+        self.D = len(coord)
         self.smoothed_kernel = smoothed_kernel
         self.coord = coord
         self.m = m
@@ -128,8 +132,9 @@ class LinearBVH:
     the device on first access.
     """
 
-    def __init__(self, backend, handle, n, ncol, kernel, code_bits, dtype):
+    def __init__(self, backend, handle, n, ncol, kernel, code_bits, dtype, dims=3):
         self.backend = backend
+        self.D = dims
         self.handle = handle
         N.register_tree(self)
         self.n = n
@@ -157,7 +162,7 @@ class LinearBVH:
     @property
     def leaf_coor(self):
         a = self._get("leaf_coor", np.float64, 3 * self.n).reshape(3, self.n).astype(self.dtype)
-        return (a[0], a[1], a[2])
+        return tuple(a[d] for d in range(self.D))
 
     @property
     def leaf_scale(self):
@@ -168,7 +173,7 @@ class LinearBVH:
         ni = max(self.n - 1, 0)
         mn = self._get("node_min", np.float64, 3 * ni).reshape(3, ni).astype(self.dtype)
         mx = self._get("node_max", np.float64, 3 * ni).reshape(3, ni).astype(self.dtype)
-        return AABB((mn[0], mn[1], mn[2]), (mx[0], mx[1], mx[2]))
+        return AABB(tuple(mn[d] for d in range(self.D)), tuple(mx[d] for d in range(self.D)))
 
     @property
     def node_scale(self):
@@ -208,10 +213,11 @@ UInt32 = np.uint32
 
 
 def LinearBVH_(inp, CodeType=np.uint64, backend=None):
-    """LinearBVH!(input; CodeType) (InterpolationInput.jl:201-221) on the GPU.
+    """LinearBVH!(input; CodeType) (InterpolationInput.jl:201-221; the 2D
+    method :224-242) on the GPU.
 
-    Builds the tree from the input's columns, then permutes x, y, z, m, h, ρ and
-    every quantity column of `inp` in place into Morton order.
+    Builds the tree from the input's columns, then permutes the coordinates, m,
+    h, ρ and every quantity column of `inp` in place into Morton order.
     """
     backend = backend or default_backend()
     if not isinstance(backend, HIPComputeBackend):
@@ -225,13 +231,15 @@ def LinearBVH_(inp, CodeType=np.uint64, backend=None):
     ctx = backend.context
     dt = inp.m.dtype
     prec = N.PH_PREC_FP32 if dt == np.float32 else N.PH_PREC_FP64
-    base = [N.f64(a) for a in (inp.coord[0], inp.coord[1], inp.coord[2], inp.m, inp.h, inp.ρ)]
+    dims = len(inp.coord)
+    base = [N.f64(a) for a in tuple(inp.coord) + (inp.m, inp.h, inp.ρ)]
     cols = [N.f64(q) for q in inp.quant]
     cptr = (C.c_void_p * max(1, len(cols)))(*[c.ctypes.data for c in cols])
     h = C.c_void_p()
-    ctx.check(N.lib().ph_lbvh_build(ctx.handle, inp.Npart, *[N.ptr(a) for a in base], len(cols), cptr,
-                                    inp.smoothed_kernel.id, bits, prec, C.byref(h)))
-    lbvh = LinearBVH(backend, h, inp.Npart, len(cols), inp.smoothed_kernel, bits, dt)
+    build = N.lib().ph_lbvh_build if dims == 3 else N.lib().ph_lbvh_build2d
+    ctx.check(build(ctx.handle, inp.Npart, *[N.ptr(a) for a in base], len(cols), cptr, inp.smoothed_kernel.id, bits,
+                    prec, C.byref(h)))
+    lbvh = LinearBVH(backend, h, inp.Npart, len(cols), inp.smoothed_kernel, bits, dt, dims)
     p = lbvh.order.astype(np.int64) - 1
     for arr in list(inp.coord) + [inp.m, inp.h, inp.ρ] + list(inp.quant):
         arr[:] = arr[p]
@@ -241,5 +249,6 @@ def LinearBVH_(inp, CodeType=np.uint64, backend=None):
 def matches_lbvh_leaf_order(inp, lbvh):
     """input.coord[d] == lbvh.leaf_coor[d] for every d and input.h == lbvh.leaf_scale (:265-267)."""
     lc = lbvh.leaf_coor
-    return all(np.array_equal(np.asarray(inp.coord[d]), lc[d]) for d in range(3)) and \
+    return len(inp.coord) == lbvh.D and \
+        all(np.array_equal(np.asarray(inp.coord[d]), lc[d]) for d in range(lbvh.D)) and \
         np.array_equal(np.asarray(inp.h), lbvh.leaf_scale)

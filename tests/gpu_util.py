@@ -12,17 +12,19 @@ import pyoracle as O
 class DeviceSet:
     """One ph_lbvh built from host float64 columns (x, y, z, m, h, rho, cols...)."""
 
-    def __init__(self, ctx, x, y, z, m, h, rho, cols=(), kernel=0, code_bits=63, precision=64):
+    def __init__(self, ctx, x, y, z, m, h, rho, cols=(), kernel=0, code_bits=63, precision=64, dims=3):
+        """dims=2: ph_lbvh_build2d (z is ignored)."""
         self.ctx = ctx
         self.n = len(x)
         self.ncol = len(cols)
-        arrs = [N.f64(a) for a in (x, y, z, m, h, rho)]
+        arrs = [N.f64(a) for a in ((x, y, z, m, h, rho) if dims == 3 else (x, y, m, h, rho))]
         cl = [N.f64(c) for c in cols]
         cptr = (C.c_void_p * max(1, len(cl)))(*[c.ctypes.data for c in cl])
         self.h = C.c_void_p()
         N.register_tree(self)
-        ctx.check(N.lib().ph_lbvh_build(ctx.handle, self.n, *[N.ptr(a) for a in arrs], len(cl), cptr, kernel,
-                                        code_bits, precision, C.byref(self.h)))
+        build = N.lib().ph_lbvh_build if dims == 3 else N.lib().ph_lbvh_build2d
+        ctx.check(build(ctx.handle, self.n, *[N.ptr(a) for a in arrs], len(cl), cptr, kernel, code_bits, precision,
+                        C.byref(self.h)))
 
     def get(self, which, dtype, size):
         out = np.zeros(size, dtype)
@@ -143,8 +145,8 @@ def assert_same(got, ref, bitwise=True, rtol=1e-12):
                 assert np.max(np.abs(got[k][f] - ref[k][f])) <= rtol * scale, "output %d" % k
 
 
-def oracle_of(x, y, z, m, h, rho, cols=(), kernel=0, code_bits=63, precision=64):
-    return O.OracleSet(x, y, z, m, h, rho, cols, kernel=kernel, code_bits=code_bits, precision=precision)
+def oracle_of(x, y, z, m, h, rho, cols=(), kernel=0, code_bits=63, precision=64, dims=3):
+    return O.OracleSet(x, y, z, m, h, rho, cols, kernel=kernel, code_bits=code_bits, precision=precision, dims=dims)
 
 
 def catalog_fields(cat):

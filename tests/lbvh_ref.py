@@ -27,6 +27,15 @@ def interleave_bits(ix, iy, iz, bits):
     return code
 
 
+def interleave_bits2(ix, iy, bits):
+    """2D: x at bit 2k+1, y at 2k, for k = 0..bits-1 (bit by bit)."""
+    code = 0
+    for k in range(bits):
+        code |= ((int(ix) >> k) & 1) << (2 * k + 1)
+        code |= ((int(iy) >> k) & 1) << (2 * k)
+    return code
+
+
 def quantise(v, bits):
     v = np.asarray(v, dtype=np.float64)
     scale = float(2 ** bits - 1)
@@ -46,6 +55,20 @@ def morton_pipeline(x, y, z, code_bits):
     order = np.argsort(np.array(codes, dtype=np.uint64), kind="stable")
     c = [int(codes[i]) for i in order]
     # dedup as a prefix maximum: c'[i] = i + max_{j<=i}(c[j] - j)
+    out, run = [], None
+    for i, cj in enumerate(c):
+        run = cj - i if run is None else max(run, cj - i)
+        out.append(run + i)
+    return np.array(out, dtype=np.uint64), (order + 1).astype(np.uint64)
+
+
+def morton_pipeline2(x, y, code_bits):
+    """The same for a 2D input: 16 (UInt32) or 32 (UInt64) bits per axis."""
+    bits = 16 if code_bits == 30 else 32
+    ix, iy = quantise(x, bits), quantise(y, bits)
+    codes = np.array([interleave_bits2(a, b, bits) for a, b in zip(ix, iy)], dtype=object)
+    order = np.argsort(np.array(codes, dtype=np.uint64), kind="stable")
+    c = [int(codes[i]) for i in order]
     out, run = [], None
     for i, cj in enumerate(c):
         run = cj - i if run is None else max(run, cj - i)

@@ -177,3 +177,83 @@ def test_nearest_h_brute_force_and_ties():
 def test_empty_tree_rejected():
     with pytest.raises(ValueError):
         O.OracleSet(np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0))
+
+
+# ---- the 2D path: MortonEncoding(x, y) and LinearBVH! of an InterpolationInput{2} ----
+
+def test_expand_bits2_is_bit_interleave():
+    for _ in range(200):
+        v = int(RNG.integers(0, 2 ** 32))
+        assert O.lib().oracle_expand_bits2_u64(v) == R.interleave_bits2(v, 0, 32) >> 1
+        v16 = v & 0xFFFF
+        assert O.lib().oracle_expand_bits2_u32(v16) == R.interleave_bits2(v16, 0, 16) >> 1
+
+
+@pytest.mark.parametrize("code_bits", [30, 63])
+@pytest.mark.parametrize("n", [1, 2, 7, 300, 5000])
+def test_2d_codes_and_order_match_independent_pipeline(code_bits, n):
+    P = np.concatenate([RNG.random((2, n // 2)), 0.5 + 1e-6 * RNG.random((2, n - n // 2))], axis=1)
+    o = O.OracleSet(P[0], P[1], None, np.ones(n), np.full(n, 0.01), np.ones(n), code_bits=code_bits, dims=2)
+    t = o.tree()
+    codes, order = R.morton_pipeline2(P[0], P[1], code_bits)
+    assert np.array_equal(t["order"], order)
+    assert np.array_equal(t["codes"], codes)
+    assert not o.columns()[2].any()  # z stays zero
+
+
+def _tree_invariants_2d(P, code_bits=63):
+    n = len(P[0])
+    o = O.OracleSet(P[0], P[1], None, np.ones(n), np.full(n, 0.1), np.ones(n), code_bits=code_bits, dims=2)
+    t = o.tree()
+    left, right, parent, root = R.topdown_tree(t["codes"])
+    assert t["root"] == root
+    assert np.array_equal(t["left"], left) and np.array_equal(t["right"], right)
+    assert np.array_equal(t["parent"], parent)
+    assert np.array_equal(t["escape"], R.expected_escape(left, right, n))
+    b = o.bvh()
+    mn, mx, sc = R.recursive_boxes(left, right, b["leaf_coor"], b["leaf_scale"], n)
+    assert np.array_equal(b["node_min"], mn) and np.array_equal(b["node_max"], mx)
+    assert np.array_equal(b["node_scale"], sc)
+    return o, t
+
+
+@pytest.mark.parametrize("offset", [0, 1, 2, 3, 4])
+def test_2d_prime_hash_trees(offset, fixtures):
+    """neighbor_search_tests.jl:177-232 with D = 2."""
+    for n in range(1, fixtures["prime_hash"]["n_max"] + 1):
+        for bits in (30, 63):
+            _tree_invariants_2d(R.prime_hash_coords(2, n, offset), bits)
+
+
+@pytest.mark.parametrize("n", [1, 2, 8, 16])
+def test_2d_identical_codes(n):
+    o, t = _tree_invariants_2d([np.full(n, 0.5)] * 2)
+    assert t["root"] == (1 if n >= 2 else 0)
+    assert np.count_nonzero(o.tree()["parent"] == 0) == 1
+
+
+def test_2d_lattice_boxes_contain_children(fixtures):
+    for n in fixtures["aabb_lattice"]["n"]:
+        c = np.linspace(0.0, 1.0, n)
+        _tree_invariants_2d([c, c])
+
+
+@pytest.mark.parametrize("strategy", [O.GATHER, O.SCATTER, O.SYMMETRIC])
+def test_2d_point_traversal_and_nearest_h_equal_brute_force(strategy):
+    n = 3000
+    P = RNG.random((2, n))
+    h = RNG.random(n) * 0.05 + 0.01
+    o = O.OracleSet(P[0], P[1], None, np.ones(n), h, np.ones(n), dims=2)
+    cols = o.columns()
+    for _ in range(40):
+        p = np.array([*(RNG.random(2) * 1.2 - 0.1), 0.0])
+        ha = float(RNG.random() * 0.05 + 0.005)
+        got, _ = o.neighbors(p, strategy, ha=ha)
+        exp = R.brute_point_neighbours(cols[:3], cols[4], p, strategy, 2.0, ha)
+        assert np.array_equal(got, exp)
+    Q = RNG.random((2, 300)) * 1.4 - 0.2
+    ha, best = o.nearest_h(Q[0], Q[1], np.zeros(300))
+    for s in range(Q.shape[1]):
+        d2 = (Q[0, s] - cols[0]) ** 2 + (Q[1, s] - cols[1]) ** 2
+        k = int(np.argmin(d2))
+        assert best[s] == k + 1 and ha[s] == cols[4][k]
