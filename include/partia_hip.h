@@ -366,6 +366,22 @@ int ph_structured_grid_adjoint_device(ph_context* ctx, ph_lbvh* bvh, const ph_ca
                                       double* grad_dev);
 
 /* ---- building blocks exposed for parity tests ---- */
+/* LBVH_probe_neighbors and LBVH_query! (LBVH/LinearBVH/LinearBVHQueries.jl:22-42,
+ * 183-211) for S points with radii radius_dev[S], on a float64 tree, device
+ * arrays (any output may be null): count_dev[s] = leaves with d2 <= radius^2;
+ * closest_dev[s] = 1-based Morton index of the closest of them, first in DFS
+ * order on ties (0 if none); closest_d2_dev[s] = its d2 (+Inf if none).  With
+ * pool_dev, the accepted leaves' 1-based indices are written in DFS (Morton)
+ * order from pool_dev[offsets_dev[s]] (the caller sizes offsets from a first
+ * call's counts, as LBVH_query! fills its pool).  For a 2D tree pz may be null. */
+int ph_query_device(ph_context* ctx, ph_lbvh* bvh, int64_t S, const double* px_dev, const double* py_dev,
+                    const double* pz_dev, const double* radius_dev, int64_t* count_dev, int64_t* closest_dev,
+                    double* closest_d2_dev, const int64_t* offsets_dev, int64_t* pool_dev);
+/* LBVH_find_nearest (LinearBVHQueries.jl:79-95): idx_dev[s] = 1-based Morton index of
+ * the closest leaf (first in DFS order on ties), d2_dev[s] its squared distance
+ * (may be null).  Float64 tree, device arrays; for a 2D tree pz may be null. */
+int ph_find_nearest_device(ph_context* ctx, ph_lbvh* bvh, int64_t S, const double* px_dev, const double* py_dev,
+                           const double* pz_dev, int64_t* idx_dev, double* d2_dev);
 /* LBVH_find_nearest_h (LBVH/LinearBVH/LinearBVHQueries.jl:118-140) per point. */
 int ph_nearest_h(ph_context* ctx, ph_lbvh* bvh, int64_t S, const double* px_host, const double* py_host,
                  const double* pz_host, double* ha_host);

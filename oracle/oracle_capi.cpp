@@ -300,6 +300,31 @@ int oracle_field_adjoint(void* hv, int mode, const int* slots, int shepard, int 
 
 double oracle_kernel_d2f(int K, double q) { return kernel_d2f<double>(K, q); }
 
+// query_leaves per point: counts, closest, closest d2; with pool (sized by the
+// caller from the counts) the accepted indices point after point.
+int oracle_query(void* hv, int64_t S, const double* px, const double* py, const double* pz, const double* radius,
+                 int64_t* count, int64_t* closest, double* cd2, int64_t* pool) {
+    Handle* H = static_cast<Handle*>(hv);
+    auto run = [&](auto& in) {
+        using T = std::decay_t<decltype(in.x[0])>;
+        int64_t off = 0;
+        for (int64_t s = 0; s < S; ++s) {
+            T p[3] = {T(px[s]), T(py[s]), T(pz[s])};
+            std::vector<i64> v;
+            i64 best = 0;
+            T bd = T(0);
+            count[s] = query_leaves<T>(in.bvh, p, T(radius[s]), pool ? &v : nullptr, &best, &bd);
+            closest[s] = best;
+            cd2[s] = double(bd);
+            if (pool)
+                for (i64 k : v) pool[off++] = k;
+        }
+    };
+    if (H->precision == 64) run(H->d);
+    else run(H->f);
+    return 0;
+}
+
 // LBVH_find_nearest_h per point; best (1-based leaf index) optional.
 int oracle_nearest_h(void* hv, int64_t S, const double* px, const double* py, const double* pz, double* ha,
                      int64_t* best) {

@@ -685,6 +685,30 @@ inline void line_traversal(const LBVH<T>& b, const T o[3], const T m[3], int str
         [&](i64 k) { return R.leaf(k); }, [&](i64 i) { return R.node(i); }, hit);
 }
 
+// LBVH_probe_neighbors / LBVH_query! (LinearBVHQueries.jl:22-42, 183-211): the
+// gather traversal with r2 = radius^2; accepted leaves in DFS order, the
+// closest by strict < (first in DFS order on ties).  Returns the count.
+template <class T>
+inline i64 query_leaves(const LBVH<T>& b, const T p[3], T radius, std::vector<i64>* pool, i64* closest, T* cd2) {
+    const T r2 = radius * radius;
+    i64 count = 0, best = 0;
+    T bd = std::numeric_limits<T>::infinity();
+    traverse<T>(
+        b, [&](i64 k) { return d2_point_leaf(p, b, k); }, [&](i64 i) { return d2_point_aabb(p, b, i); },
+        [&](i64) { return r2; }, [&](i64) { return r2; },
+        [&](i64 k, T d2, T) {
+            ++count;
+            if (pool) pool->push_back(k);
+            if (d2 < bd) {
+                bd = d2;
+                best = k;
+            }
+        });
+    if (closest) *closest = best;
+    if (cd2) *cd2 = bd;
+    return count;
+}
+
 // LBVH_find_nearest_h (LBVH/LinearBVH/LinearBVHQueries.jl:118-140): the bound
 // shrinks as the traversal proceeds; strict `<` keeps the first leaf in DFS order.
 template <class T> inline T nearest_h(const LBVH<T>& b, const T p[3], i64* best_out = nullptr) {

@@ -43,6 +43,7 @@ def lib():
         L.oracle_point_adjoint.argtypes = [vp, i32, i32, i32, i64] + [vp] * 3 + [vp, vp]
         L.oracle_field_adjoint.argtypes = [vp, i32, vp, i32, i32, i64] + [vp] * 3 + [vp, vp]
         L.oracle_kernel_d2f.argtypes = [i32, d]
+        L.oracle_query.argtypes = [vp, i64] + [vp] * 8
         L.oracle_kernel_d2f.restype = d
         L.oracle_neighbors.restype = i64
         L.oracle_neighbors.argtypes = [vp, i32, i32, vp, vp, d, vp, vp, i64]
@@ -202,6 +203,20 @@ class OracleSet:
         lib().oracle_field_adjoint(self.h, int(mode), _p(sl), int(bool(shepard)), int(strategy), S, _p(px), _p(py),
                                    _p(pz), _p(ct), _p(g))
         return g
+
+    def query(self, px, py, pz, radius, with_pool=False):
+        """LBVH_probe_neighbors / LBVH_query! per point: (counts, closest
+        1-based index or 0, closest d2 or inf, pool or None)."""
+        px, py, pz, r = _f64(px), _f64(py), _f64(pz), _f64(radius)
+        S = len(px)
+        cnt, best, cd2 = np.zeros(S, np.int64), np.zeros(S, np.int64), np.zeros(S)
+        lib().oracle_query(self.h, S, _p(px), _p(py), _p(pz), _p(r), _p(cnt), _p(best), _p(cd2), None)
+        pool = None
+        if with_pool:
+            pool = np.zeros(max(int(cnt.sum()), 1), np.int64)
+            lib().oracle_query(self.h, S, _p(px), _p(py), _p(pz), _p(r), _p(cnt), _p(best), _p(cd2), _p(pool))
+            pool = pool[: int(cnt.sum())]
+        return cnt, best, cd2, pool
 
     def nearest_h(self, px, py, pz):
         px, py, pz = _f64(px), _f64(py), _f64(pz)

@@ -981,6 +981,48 @@ int ph_nearest_h(ph_context* ctx, ph_lbvh* B, int64_t S, const double* px, const
     });
 }
 
+// The query points of the LBVH queries; a 2D tree takes pz = null (zeros).
+static PointList query_points(ph_lbvh* B, i64 S, const double* px, const double* py, const double* pz, DevBuf& zero,
+                              hipStream_t s) {
+    need(B->precision == PH_PREC_FP64, PH_ERR_ARGUMENT, "the LBVH queries need a tree built from Float64 columns");
+    need(S >= 0, PH_ERR_ARGUMENT, "negative point count");
+    need(S == 0 || (px && py && (pz || B->dims == 2)), PH_ERR_ARGUMENT, "null point pointer");
+    if (S > 0 && !pz) {
+        zero.alloc(sizeof(double) * (size_t)S);
+        PH_HIP(hipMemsetAsync(zero.p, 0, sizeof(double) * (size_t)S, s));
+        pz = zero.as<double>();
+    }
+    return PointList{px, py, pz};
+}
+
+int ph_query_device(ph_context* ctx, ph_lbvh* B, int64_t S, const double* px, const double* py, const double* pz,
+                    const double* radius, int64_t* count, int64_t* closest, double* closest_d2,
+                    const int64_t* offsets, int64_t* pool) {
+    return guarded(ctx, [&] {
+        need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
+        need(S == 0 || radius, PH_ERR_ARGUMENT, "null radius pointer");
+        need(!pool || offsets, PH_ERR_ARGUMENT, "a pool needs offsets");
+        DevBuf zero;
+        const PointList pl = query_points(B, S, px, py, pz, zero, ctx->stream);
+        if (S > 0) launch_query(B, pl, S, radius, count, closest, closest_d2, offsets, pool, ctx->stream);
+        PH_HIP(hipStreamSynchronize(ctx->stream));
+        return PH_OK;
+    });
+}
+
+int ph_find_nearest_device(ph_context* ctx, ph_lbvh* B, int64_t S, const double* px, const double* py,
+                           const double* pz, int64_t* idx, double* d2) {
+    return guarded(ctx, [&] {
+        need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
+        need(S == 0 || idx, PH_ERR_ARGUMENT, "null index pointer");
+        DevBuf zero;
+        const PointList pl = query_points(B, S, px, py, pz, zero, ctx->stream);
+        if (S > 0) launch_find_nearest(B, pl, S, idx, d2, ctx->stream);
+        PH_HIP(hipStreamSynchronize(ctx->stream));
+        return PH_OK;
+    });
+}
+
 int ph_line_table(int32_t kernel, double* inorm) {
     if (kernel < 0 || kernel > 5 || !inorm) return fail(nullptr, PH_ERR_ARGUMENT, "bad arguments");
     std::memcpy(inorm, ph_lut_inorm[kernel], sizeof(double) * PH_LUT_N);

@@ -104,6 +104,14 @@ void launch_lines_adjoint(ph_lbvh* B, i32 qsrc, bool shepard, int precision, con
                           i64 S, const void* cot, const void* lut, double* grad, hipStream_t s,
                           hipEvent_t* ev = nullptr);
 
+// LBVH queries (ph_query.hip), float64 trees, device arrays.  launch_query:
+// per point the leaves within radius[s] (count, closest 1-based Morton index or
+// 0, its d2 or +Inf; with pool, their indices from pool[offsets[s]] in DFS
+// order); launch_find_nearest: the closest leaf (1-based) and its d2.
+void launch_query(ph_lbvh* B, const PointList& pl, i64 S, const double* radius, i64* count, i64* closest,
+                  double* closest_d2, const i64* offsets, i64* pool, hipStream_t s);
+void launch_find_nearest(ph_lbvh* B, const PointList& pl, i64 S, i64* idx, double* d2, hipStream_t s);
+
 // Batched sampling (ph_batch.hip).  batch_build validates and copies the
 // requests, packs nothing yet and records the launch graph; throws Error.
 BatchState* batch_build(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int strategy, int precision, int nreq,

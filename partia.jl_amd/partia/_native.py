@@ -29,7 +29,7 @@ EXPORTS = (
     "ph_ctx_create", "ph_ctx_destroy", "ph_last_error", "ph_get_stats", "ph_set_count_neighbours",
     "ph_set_traversal",
     "ph_device_alloc", "ph_device_free", "ph_memcpy_htod", "ph_memcpy_dtoh", "ph_synchronize",
-    "ph_lbvh_build", "ph_lbvh_build_device", "ph_lbvh_build2d", "ph_lbvh_build2d_device", "ph_lbvh_dims",
+    "ph_lbvh_build", "ph_lbvh_build_device", "ph_lbvh_build2d", "ph_lbvh_build2d_device", "ph_lbvh_dims", "ph_query_device", "ph_find_nearest_device",
     "ph_lbvh_destroy", "ph_lbvh_set_columns", "ph_lbvh_info",
     "ph_lbvh_get",
     "ph_point_samples", "ph_point_samples_device", "ph_structured_grid", "ph_structured_grid_device",
@@ -112,6 +112,8 @@ def lib():
         "ph_lbvh_build2d": (i32, [vp, i64] + [vp] * 5 + [i32, vp, i32, i32, i32, P(vp)]),
         "ph_lbvh_build2d_device": (i32, [vp, i64] + [vp] * 5 + [i32, vp, i32, i32, i32, P(vp)]),
         "ph_lbvh_dims": (i32, [vp, P(i32)]),
+        "ph_query_device": (i32, [vp, vp, i64] + [vp] * 9),
+        "ph_find_nearest_device": (i32, [vp, vp, i64] + [vp] * 5),
         "ph_lbvh_destroy": (i32, [vp]),
         "ph_lbvh_set_columns": (i32, [vp, vp, vp, i32, vp]),
         "ph_lbvh_info": (i32, [vp, P(i64), P(i32), P(i32), P(i32), P(i32)]),

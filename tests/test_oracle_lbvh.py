@@ -257,3 +257,52 @@ def test_2d_point_traversal_and_nearest_h_equal_brute_force(strategy):
         d2 = (Q[0, s] - cols[0]) ** 2 + (Q[1, s] - cols[1]) ** 2
         k = int(np.argmin(d2))
         assert best[s] == k + 1 and ha[s] == cols[4][k]
+
+
+# ---- LBVH_probe_neighbors / LBVH_query! / LBVH_find_nearest ----
+
+@pytest.mark.parametrize("dims", [3, 2])
+def test_query_equals_brute_force(dims):
+    """Counts, DFS-ordered pools (increasing Morton index) and the closest
+    accepted leaf (first index on ties) against brute force over all leaves;
+    radius 0 and radii past every leaf included."""
+    n = 4000
+    P = RNG.random((3, n))
+    if dims == 2:
+        P[2] = 0.0
+    o = O.OracleSet(*P, np.ones(n), np.full(n, 0.01), np.ones(n), dims=dims)
+    c = o.columns()
+    S = 300
+    Q = RNG.random((3, S)) * 1.2 - 0.1
+    if dims == 2:
+        Q[2] = 0.0
+    Q[:, :5] = c[:3, :5]  # on particles: d2 = 0
+    r = RNG.random(S) * 0.08
+    r[5], r[6] = 0.0, 10.0
+    cnt, best, cd2, pool = o.query(*Q, r, with_pool=True)
+    off = 0
+    for s in range(S):
+        d2 = ((Q[0, s] - c[0]) * (Q[0, s] - c[0]) + (Q[1, s] - c[1]) * (Q[1, s] - c[1])) + \
+            (Q[2, s] - c[2]) * (Q[2, s] - c[2])
+        acc = np.flatnonzero(d2 <= r[s] * r[s])
+        assert cnt[s] == len(acc)
+        assert np.array_equal(pool[off:off + cnt[s]], acc + 1)
+        off += cnt[s]
+        if len(acc):
+            k = acc[np.argmin(d2[acc])]
+            assert best[s] == k + 1 and cd2[s] == d2[k]
+        else:
+            assert best[s] == 0 and cd2[s] == np.inf
+    assert cnt[6] == n
+
+
+def test_query_on_the_reference_fixture(fixtures):
+    """LBVH_query! on test/neighbor_search_tests.jl's point set and radii."""
+    fx = fixtures["neighbour_query"]
+    x, y, z = (np.array(fx[k]) for k in "xyz")
+    o = O.OracleSet(x, y, z, np.ones(4), np.full(4, fx["h"]), np.ones(4))
+    cols = o.columns()
+    for p, r in fx["cases_3d"]:
+        cnt, best, cd2, pool = o.query([p[0]], [p[1]], [p[2]], [r], with_pool=True)
+        exp = R.brute_point_neighbours(cols[:3], cols[4], p, 0, 2.0, r / 2.0)
+        assert sorted(pool) == sorted(exp) and cnt[0] == len(exp) >= 1
