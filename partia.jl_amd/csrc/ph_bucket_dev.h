@@ -48,10 +48,10 @@ namespace {
 
 // Bucket geometry (compile-time; kQ * kBucket mask bits per lane and batch).
 #ifndef PH_BUCKET_LEAVES
-#define PH_BUCKET_LEAVES 4
+#define PH_BUCKET_LEAVES 8
 #endif
 #ifndef PH_BUCKET_BATCH
-#define PH_BUCKET_BATCH 16
+#define PH_BUCKET_BATCH 8
 #endif
 #ifndef PH_ACC_UNROLL
 #define PH_ACC_UNROLL 1
