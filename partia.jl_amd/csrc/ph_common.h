@@ -238,7 +238,7 @@ struct ph_lbvh {
     ph::DevBuf ha;           // per-sample smoothing length of the nearest particle
     ph::DevBuf nn;           // per-sample Morton index of the nearest particle (adjoints)
     // Bucket tree (built on first use): the LBVH in DFS preorder with every
-    // subtree of at most 4 leaves collapsed into one bucket.
+    // subtree of at most kBucket (8) leaves collapsed into one bucket.
     bool buckets_ready = false;
     int64_t nrec = 0;
     ph::DevBuf brecs;     // bucket-tree records [nrec]
