@@ -15,17 +15,17 @@ namespace {
 // bs[first leaf - 1] = 1 for each child that is a bucket (at most kBucket leaves).
 __global__ void __launch_bounds__(kBlock) k_bucket_flags(i64 n, const i32* __restrict__ left,
                                                  const i32* __restrict__ range, u32* __restrict__ big_s,
-                                                 u32* __restrict__ bs) {
+                                                 u32* __restrict__ bs, int bmax) {
     const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
     if (i > n - 1) return;
     const i64 lo = range[2 * (i - 1)], hi = range[2 * (i - 1) + 1];
     const i64 l = left[i - 1];
     const i64 s = l >= n ? lo : l;  // split: the left child covers [lo, s]
-    const bool big = hi - lo + 1 > kBucket;
+    const bool big = hi - lo + 1 > bmax;
     big_s[s - 1] = big ? 1u : 0u;
     if (big) {
-        if (s - lo + 1 <= kBucket) bs[lo - 1] = 1u;
-        if (hi - s <= kBucket) bs[s] = 1u;
+        if (s - lo + 1 <= bmax) bs[lo - 1] = 1u;
+        if (hi - s <= bmax) bs[s] = 1u;
     }
 }
 
@@ -61,11 +61,11 @@ __global__ void __launch_bounds__(kBlock) k_bucket_emit(i64 n, i32 root, const i
                                                 const NodeRec<double>* __restrict__ nodes,
                                                 const LeafRec<double>* __restrict__ leaves,
                                                 const u32* __restrict__ PB, const u32* __restrict__ PS,
-                                                BucketRec* __restrict__ recs) {
+                                                BucketRec* __restrict__ recs, int bmax) {
     const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
     if (i > n - 1) return;
     const i64 lo = range[2 * (i - 1)], hi = range[2 * (i - 1) + 1];
-    if (hi - lo + 1 <= kBucket) return;
+    if (hi - lo + 1 <= bmax) return;
     i64 Lc = 0;
     for (i64 c = i, steps = n; c != root && steps > 0; --steps) {
         const i64 p = parent[c - 1];
@@ -80,12 +80,12 @@ __global__ void __launch_bounds__(kBlock) k_bucket_emit(i64 n, i32 root, const i
     recs[pre] = R;
     const i64 l = left[i - 1];
     const i64 s = l >= n ? lo : l;
-    if (s - lo + 1 <= kBucket) {  // left child is a bucket, right after its parent
+    if (s - lo + 1 <= bmax) {  // left child is a bucket, right after its parent
         node_box(s == lo ? (lo + n - 1) : s, n, nodes, leaves, R);
         R.link = bucket_link(lo - 1, s - lo + 1);
         recs[pre + 1] = R;
     }
-    if (hi - s <= kBucket) {
+    if (hi - s <= bmax) {
         const i64 rpre = Lc + PB[s] + PS[s];
         node_box(s + 1 == hi ? (hi + n - 1) : s + 1, n, nodes, leaves, R);
         R.link = bucket_link(s, hi - s);
@@ -282,15 +282,22 @@ void launch_bucket_k(ph_lbvh* B, const CatChunk& c, int strategy, int precision,
 
 }  // namespace
 
-void ensure_buckets(ph_lbvh* B, hipStream_t s) {
-    if (B->buckets_ready) return;
+void ensure_buckets(ph_lbvh* B, hipStream_t s, int leaves) {
+    // two bucket trees per LBVH: buckets of at most kBucket leaves (the grid
+    // tile kernels) and of at most kBucketSmall (the per-lane kernels, the
+    // batch queue, adjoints and queries)
+    const bool big_buckets = leaves == kBucket;
+    bool& ready = big_buckets ? B->buckets8_ready : B->buckets_ready;
+    DevBuf& brecs = big_buckets ? B->brecs8 : B->brecs;
+    if (ready) return;
+    const int bmax = big_buckets ? kBucket : kBucketSmall;
     const i64 n = B->n;
     if (n > kMaxBucketLeaves) throw Error{PH_ERR_CAPACITY, "the bucket traversal supports at most 2^26 particles"};
     i64 nrec = 1;
-    if (n <= kBucket) {
-        B->brecs.alloc(sizeof(BucketRec));
+    if (n <= bmax) {
+        brecs.alloc(sizeof(BucketRec));
         k_bucket_root<<<1, 1, 0, s>>>(n, B->root, B->nodes.as<NodeRec<double>>(), B->leaves.as<LeafRec<double>>(),
-                                      B->brecs.as<BucketRec>());
+                                      brecs.as<BucketRec>());
         PH_HIP(hipGetLastError());
     } else {
         DevBuf flags, tmp;
@@ -298,7 +305,7 @@ void ensure_buckets(ph_lbvh* B, hipStream_t s) {
         u32* big_s = flags.as<u32>();  // n entries: n - 1 split flags + the total
         u32* bs = big_s + n;           // n + 1 entries: n bucket-start flags + the total
         PH_HIP(hipMemsetAsync(flags.p, 0, sizeof(u32) * (size_t)(2 * n + 1), s));
-        k_bucket_flags<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->left.as<i32>(), B->range.as<i32>(), big_s, bs);
+        k_bucket_flags<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->left.as<i32>(), B->range.as<i32>(), big_s, bs, bmax);
         PH_HIP(hipGetLastError());
         exclusive_sum_u32(big_s, n, tmp, s);
         exclusive_sum_u32(bs, n + 1, tmp, s);
@@ -307,16 +314,16 @@ void ensure_buckets(ph_lbvh* B, hipStream_t s) {
         PH_HIP(hipMemcpyAsync(&tot[1], bs + n, sizeof(u32), hipMemcpyDeviceToHost, s));
         PH_HIP(hipStreamSynchronize(s));
         nrec = (i64)tot[0] + (i64)tot[1];
-        B->brecs.alloc(sizeof(BucketRec) * (size_t)nrec);
+        brecs.alloc(sizeof(BucketRec) * (size_t)nrec);
         k_bucket_emit<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->root, B->left.as<i32>(), B->parent.as<i32>(),
                                                         B->range.as<i32>(), B->nodes.as<NodeRec<double>>(),
                                                         B->leaves.as<LeafRec<double>>(), big_s, bs,
-                                                        B->brecs.as<BucketRec>());
+                                                        brecs.as<BucketRec>(), bmax);
         PH_HIP(hipGetLastError());
         PH_HIP(hipStreamSynchronize(s));  // flags / tmp are released on return
     }
-    B->nrec = nrec;
-    B->buckets_ready = true;
+    (big_buckets ? B->nrec8 : B->nrec) = nrec;
+    ready = true;
 }
 
 void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl, const GridAxes* ga,

@@ -51,16 +51,20 @@ namespace {
 #define PH_BUCKET_LEAVES 8
 #endif
 #ifndef PH_BUCKET_BATCH
-#define PH_BUCKET_BATCH 8
+#define PH_BUCKET_BATCH 16
 #endif
 #ifndef PH_ACC_UNROLL
 #define PH_ACC_UNROLL 1
 #endif
 constexpr int kBucket = PH_BUCKET_LEAVES;  // leaves per bucket at most (count stored in 5 bits)
-constexpr int kLog2Bucket = kBucket == 4 ? 2 : (kBucket == 8 ? 3 : (kBucket == 16 ? 4 : 5));
+// The bucket tree of the per-lane kernels, batches, adjoints and queries
+// (kBucket-leaf buckets serve the grid tile kernels).
+constexpr int kBucketSmall = kBucket < 4 ? kBucket : 4;
+// The per-lane kernels walk the small-bucket tree: mask bit q * kBucketSmall + j.
+constexpr int kLog2BucketSmall = kBucketSmall == 4 ? 2 : (kBucketSmall == 8 ? 3 : (kBucketSmall == 16 ? 4 : 5));
 constexpr int kQ = PH_BUCKET_BATCH;        // buckets tested per batch
 static_assert(kBucket == 4 || kBucket == 8 || kBucket == 16 || kBucket == 32, "bucket size 4, 8, 16 or 32");
-static_assert(kQ * kBucket <= 64, "one 64-bit mask per lane and batch");
+static_assert(kQ * kBucketSmall <= 64, "one 64-bit mask per lane and batch");
 constexpr i64 kMaxBucketLeaves = i64(1) << 26;  // leaf index of a bucket stored in 26 bits
 
 // Preorder record: link >= 0 -> internal node, link = index after its subtree;
@@ -479,7 +483,7 @@ __device__ __attribute__((always_inline)) u64 test_buckets(const BucketView& bv,
             for (int j = 0; j < cnt; ++j) {
                 const LeafRec<double> L = bv.leaves[lo + j];
                 if (d2_point(p[0], p[1], p[2], L) <= radius2<double, STRAT>(Kd, r2a, L.h))
-                    mask |= 1ull << (q * kBucket + j);
+                    mask |= 1ull << (q * kBucketSmall + j);
             }
         }
     }
@@ -637,7 +641,7 @@ __device__ __attribute__((always_inline)) void bucket_c32_sample(const BucketVie
                 live[u] = mask != 0;
                 const int b = __builtin_ctzll(mask | (1ull << 63));
                 mask &= mask - 1;
-                const u32 kk = bl.lo(b >> kLog2Bucket) + (u32)(b & (kBucket - 1));
+                const u32 kk = bl.lo(b >> kLog2BucketSmall) + (u32)(b & (kBucketSmall - 1));
                 k[u] = kk < nlast ? kk : nlast;
                 L[u] = lb.leaf(k[u] * (u32)sizeof(LeafRec<double>));
             }
@@ -679,7 +683,7 @@ __device__ __attribute__((always_inline)) void bucket_f64_sample(const BucketVie
             if (mask != 0) {
                 const int b = __builtin_ctzll(mask);
                 mask &= mask - 1;
-                const i64 k = (i64)bl.lo(b >> kLog2Bucket) + (b & (kBucket - 1));
+                const i64 k = (i64)bl.lo(b >> kLog2BucketSmall) + (b & (kBucketSmall - 1));
                 const LeafRec<double> L = bv.leaves[k];
                 acc.add(c, rec + k * c.rw, p[0] - L.x, p[1] - L.y, p[2] - L.z, d2_point(p[0], p[1], p[2], L), ha, L.h);
             }
@@ -689,7 +693,9 @@ __device__ __attribute__((always_inline)) void bucket_f64_sample(const BucketVie
     if (valid) acc.finish(c, out, S, s);
 }
 
-BucketView bucket_view(const ph_lbvh* B) {
+BucketView bucket_view(const ph_lbvh* B, int leaves = kBucketSmall) {
+    if (leaves == kBucket)
+        return BucketView{B->n, (i32)B->nrec8, B->brecs8.as<BucketRec>(), B->leaves.as<LeafRec<double>>()};
     return BucketView{B->n, (i32)B->nrec, B->brecs.as<BucketRec>(), B->leaves.as<LeafRec<double>>()};
 }
 

@@ -241,6 +241,9 @@ struct ph_lbvh {
     // subtree of at most kBucket (8) leaves collapsed into one bucket.
     bool buckets_ready = false;
     int64_t nrec = 0;
+    bool buckets8_ready = false;  // the same with buckets of up to kBucket leaves (brecs8)
+    int64_t nrec8 = 0;
+    ph::DevBuf brecs8;
     ph::DevBuf brecs;     // bucket-tree records [nrec]
     ph::DevBuf rec32;     // FP32-compensated records of the last call ({m/rho, 1/h}, columns)
 

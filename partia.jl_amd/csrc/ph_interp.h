@@ -68,8 +68,9 @@ void launch_nearest_h(ph_lbvh* B, const PointList& pl, i64 S, double* ha, hipStr
 // the preorder bucket tree, leaves of reached buckets tested with float64.
 void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl,
                    const GridAxes* ga, i64 S, void* out, u64* counters, hipStream_t s, hipEvent_t* ev = nullptr);
-// Builds the bucket tree of B if it is not built yet.
-void ensure_buckets(ph_lbvh* B, hipStream_t s);
+// Builds the bucket tree of B if it is not built yet: buckets of at most
+// `leaves` leaves, 4 (the default) or kBucket (8, the grid tile kernels).
+void ensure_buckets(ph_lbvh* B, hipStream_t s, int leaves = 4);
 // FP32C grid interpolation with one bucket-tree walk per wave (ph_tile.hip);
 // the caller has packed B->rec32 for c32 and computed ha (gather / symmetric).
 void launch_tile_c32(ph_lbvh* B, const CatChunk& c32, int strategy, const GridAxes& ga, i64 S, const double* ha,

@@ -273,6 +273,7 @@ void launch_tile_k(int strategy, const BucketView& bv, const double* rec, const 
 }  // namespace
 
 void launch_nearest_tile(ph_lbvh* B, const GridAxes& ga, i64 S, double Kd, double* ha, hipStream_t s, i64* nn) {
+    ensure_buckets(B, s);  // the per-lane searches prefer small buckets
     const BucketView bv = bucket_view(B);
     const LaunchCfg cfg = tl_launch;
     const i64 tiles = ((ga.nx + 3) / 4) * ((ga.ny + 3) / 4) * ((ga.nz + 3) / 4);
@@ -284,7 +285,8 @@ void launch_nearest_tile(ph_lbvh* B, const GridAxes& ga, i64 S, double Kd, doubl
 
 void launch_tile_c32(ph_lbvh* B, const CatChunk& c32, int strategy, const GridAxes& ga, i64 S, const double* ha,
                      void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
-    const BucketView bv = bucket_view(B);
+    ensure_buckets(B, s, kBucket);
+    const BucketView bv = bucket_view(B, kBucket);
     const double* rec = B->rec32.as<double>();
     float* o = static_cast<float*>(out);
     switch (B->kernel) {
