@@ -1,6 +1,9 @@
 // ph_api.hip -- extern "C" entry points of libpartia_hip (declared in include/partia_hip.h).
+#include <algorithm>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 
 #include "ph_build.h"
 #include "ph_interp.h"
@@ -437,6 +440,9 @@ static int build_common(ph_context* ctx, int64_t n, const double* const* src, bo
         BuildScratch W;
         PH_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
         build_lbvh(B.get(), src, host_src, ctx->stream, W);
+        // the bucket trees of the point / grid kernels, built with the tree so
+        // that interpolation calls allocate only per-call buffers
+        if (buckets_supported(B.get())) ensure_buckets(B.get(), ctx->stream);
         PH_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
         PH_HIP(hipEventSynchronize(ctx->ev[3]));
         ctx->stats.build_ms = elapsed(ctx, 2, 3);

@@ -174,8 +174,8 @@ template <class F> size_t occupancy_pad(F kern, float occupancy) {
 }
 
 template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT>
-void launch_c32_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const GridAxes* ga, i64 S, const double* ha,
-                  void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+void launch_c32_t(ph_lbvh* B, const double* rec, const CatChunk& c, const PointList* pl, const GridAxes* ga, i64 S,
+                  const double* ha, void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
     const BucketView bv = bucket_view(B);
     const unsigned g = sample_blocks(ga, S);
     const LaunchCfg cfg = tl_launch;
@@ -183,13 +183,13 @@ void launch_c32_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const Grid
         auto kern = k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 1>;
         const size_t pad = occupancy_pad(kern, cfg.occupancy);
         if (ev) PH_HIP(hipEventRecord(ev[0], s));
-        kern<<<g, kBlock, pad, s>>>(bv, B->rec32.as<double>(), c, Sampler<double, 1>{*ga, S, point_xcd(cfg), point_order(cfg)}, ha,
+        kern<<<g, kBlock, pad, s>>>(bv, rec, c, Sampler<double, 1>{*ga, S, point_xcd(cfg), point_order(cfg)}, ha,
                                     static_cast<float*>(out), counters);
     } else {
         auto kern = k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 0>;
         const size_t pad = occupancy_pad(kern, cfg.occupancy);
         if (ev) PH_HIP(hipEventRecord(ev[0], s));
-        kern<<<g, kBlock, pad, s>>>(bv, B->rec32.as<double>(), c, Sampler<double, 0>{*pl, S, point_xcd(cfg)}, ha,
+        kern<<<g, kBlock, pad, s>>>(bv, rec, c, Sampler<double, 0>{*pl, S, point_xcd(cfg)}, ha,
                                     static_cast<float*>(out), counters);
     }
     PH_HIP(hipGetLastError());
@@ -197,8 +197,8 @@ void launch_c32_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const Grid
 }
 
 template <int K, int STRAT, int MS, int MG, int MD, int MC>
-void launch_f64_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const GridAxes* ga, i64 S, const double* ha,
-                  void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+void launch_f64_t(ph_lbvh* B, const double* rec, const CatChunk& c, const PointList* pl, const GridAxes* ga, i64 S,
+                  const double* ha, void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
     const BucketView bv = bucket_view(B);
     const unsigned g = sample_blocks(ga, S);
     const LaunchCfg cfg = tl_launch;
@@ -206,13 +206,13 @@ void launch_f64_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const Grid
         auto kern = k_bucket_f64<K, STRAT, MS, MG, MD, MC, 1>;
         const size_t pad = occupancy_pad(kern, cfg.occupancy);
         if (ev) PH_HIP(hipEventRecord(ev[0], s));
-        kern<<<g, kBlock, pad, s>>>(bv, B->rec.as<double>(), c, Sampler<double, 1>{*ga, S, point_xcd(cfg), point_order(cfg)}, ha,
+        kern<<<g, kBlock, pad, s>>>(bv, rec, c, Sampler<double, 1>{*ga, S, point_xcd(cfg), point_order(cfg)}, ha,
                                     static_cast<double*>(out), counters);
     } else {
         auto kern = k_bucket_f64<K, STRAT, MS, MG, MD, MC, 0>;
         const size_t pad = occupancy_pad(kern, cfg.occupancy);
         if (ev) PH_HIP(hipEventRecord(ev[0], s));
-        kern<<<g, kBlock, pad, s>>>(bv, B->rec.as<double>(), c, Sampler<double, 0>{*pl, S, point_xcd(cfg)}, ha,
+        kern<<<g, kBlock, pad, s>>>(bv, rec, c, Sampler<double, 0>{*pl, S, point_xcd(cfg)}, ha,
                                     static_cast<double*>(out), counters);
     }
     PH_HIP(hipGetLastError());
@@ -221,21 +221,29 @@ void launch_f64_t(ph_lbvh* B, const CatChunk& c, const PointList* pl, const Grid
 
 template <int K, int STRAT>
 void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointList* pl, const GridAxes* ga, i64 S,
-                      void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+                      void* out, u64* counters, hipStream_t s, hipEvent_t* ev, const CallWorkspace* ws) {
     const i64 n = B->n;
     const bool small = c.ng + c.nd + c.nc == 0 && c.ns <= 2;
-    // buffers first (the pool's allocations wait for the stream), then kernels
+    // buffers first (the pool's allocations wait for the stream), then kernels;
+    // a caller's workspace replaces the tree's per-call buffers
     int ncol = 0;
     CatChunk c32{};
+    double* recp = nullptr;
+    double* hap = nullptr;
     if (precision == PH_PREC_FP64) {
-        B->rec.alloc(sizeof(double) * (size_t)c.rw * (size_t)n);
+        if (!ws) B->rec.alloc(sizeof(double) * (size_t)c.rw * (size_t)n);
+        recp = ws ? ws->rec : B->rec.as<double>();
     } else {
         c32 = compact_c32(c, ncol);
         if ((i64)c32.rw * 8 * n >= (i64(1) << 32))
             throw Error{PH_ERR_CAPACITY, "FP32C records exceed 4 GiB; split the catalog or use PH_PREC_FP64"};
-        B->rec32.alloc(sizeof(double) * (size_t)c32.rw * (size_t)n);
+        if (!ws) B->rec32.alloc(sizeof(double) * (size_t)c32.rw * (size_t)n);
+        recp = ws ? ws->rec : B->rec32.as<double>();
     }
-    if (STRAT != PH_SCATTER) B->ha.alloc(sizeof(double) * (size_t)S);
+    if (STRAT != PH_SCATTER) {
+        if (!ws) B->ha.alloc(sizeof(double) * (size_t)S);
+        hap = ws ? ws->ha : B->ha.as<double>();
+    }
     // nearest-particle smoothing length of every sample (gather / symmetric)
     const double* ha = nullptr;
     if (STRAT != PH_SCATTER) {
@@ -243,101 +251,126 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
         const unsigned g = sample_blocks(ga, S);
         const int xm = point_xcd(tl_launch);
         const double Kd = support_of<K>();
-        if (ga && tl_launch.tile) launch_nearest_tile(B, *ga, S, Kd, B->ha.as<double>(), s);
+        if (ga && tl_launch.tile) launch_nearest_tile(B, *ga, S, Kd, hap, s);
         else if (ga)
             k_nearest_bucket<1><<<g, kBlock, 0, s>>>(bv, Sampler<double, 1>{*ga, S, xm, point_order(tl_launch)}, Kd,
-                                                     B->ha.as<double>());
-        else k_nearest_bucket<0><<<g, kBlock, 0, s>>>(bv, Sampler<double, 0>{*pl, S, xm}, Kd, B->ha.as<double>());
+                                                     hap);
+        else k_nearest_bucket<0><<<g, kBlock, 0, s>>>(bv, Sampler<double, 0>{*pl, S, xm}, Kd, hap);
         PH_HIP(hipGetLastError());
-        ha = B->ha.as<double>();
+        ha = hap;
     }
     if (precision == PH_PREC_FP64) {
-        k_pack_records<double><<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c, B->rec.as<double>());
+        k_pack_records<double><<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c, recp);
         PH_HIP(hipGetLastError());
-        if (small) launch_f64_t<K, STRAT, 2, 0, 0, 0>(B, c, pl, ga, S, ha, out, counters, s, ev);
-        else launch_f64_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC>(B, c, pl, ga, S, ha, out, counters, s, ev);
+        if (small) launch_f64_t<K, STRAT, 2, 0, 0, 0>(B, recp, c, pl, ga, S, ha, out, counters, s, ev);
+        else launch_f64_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC>(B, recp, c, pl, ga, S, ha, out, counters, s, ev);
         return;
     }
-    k_pack_c32<<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c32, ncol, c32.rw, B->rec32.as<double>());
+    k_pack_c32<<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c32, ncol, c32.rw, recp);
     PH_HIP(hipGetLastError());
     if (ga && tl_launch.tile) {  // one walk per wave of grid points
-        launch_tile_c32(B, c32, STRAT, *ga, S, ha, out, counters, s, ev);
+        launch_tile_c32(B, recp, c32, STRAT, *ga, S, ha, out, counters, s, ev);
         return;
     }
-    if (small) launch_c32_t<K, STRAT, 2, 0, 0, 0, 0>(B, c32, pl, ga, S, ha, out, counters, s, ev);
+    if (small) launch_c32_t<K, STRAT, 2, 0, 0, 0, 0>(B, recp, c32, pl, ga, S, ha, out, counters, s, ev);
     else if (c2_shared(c32))  // density, scalar A, grad A, div v, curl v
-        launch_c32_t<K, STRAT, 2, 1, 1, 1, 2>(B, c32, pl, ga, S, ha, out, counters, s, ev);
-    else launch_c32_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0>(B, c32, pl, ga, S, ha, out, counters, s, ev);
+        launch_c32_t<K, STRAT, 2, 1, 1, 1, 2>(B, recp, c32, pl, ga, S, ha, out, counters, s, ev);
+    else launch_c32_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0>(B, recp, c32, pl, ga, S, ha, out, counters, s, ev);
 }
 
 template <int K>
 void launch_bucket_k(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl,
-                     const GridAxes* ga, i64 S, void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+                     const GridAxes* ga, i64 S, void* out, u64* counters, hipStream_t s, hipEvent_t* ev,
+                     const CallWorkspace* ws) {
     switch (strategy) {
-    case PH_GATHER: launch_bucket_ks<K, PH_GATHER>(B, c, precision, pl, ga, S, out, counters, s, ev); break;
-    case PH_SCATTER: launch_bucket_ks<K, PH_SCATTER>(B, c, precision, pl, ga, S, out, counters, s, ev); break;
-    default: launch_bucket_ks<K, PH_SYMMETRIC>(B, c, precision, pl, ga, S, out, counters, s, ev); break;
+    case PH_GATHER: launch_bucket_ks<K, PH_GATHER>(B, c, precision, pl, ga, S, out, counters, s, ev, ws); break;
+    case PH_SCATTER: launch_bucket_ks<K, PH_SCATTER>(B, c, precision, pl, ga, S, out, counters, s, ev, ws); break;
+    default: launch_bucket_ks<K, PH_SYMMETRIC>(B, c, precision, pl, ga, S, out, counters, s, ev, ws); break;
     }
 }
 
 }  // namespace
 
-void ensure_buckets(ph_lbvh* B, hipStream_t s, int leaves) {
-    // two bucket trees per LBVH: buckets of at most kBucket leaves (the grid
-    // tile kernels) and of at most kBucketSmall (the per-lane kernels, the
-    // batch queue, adjoints and queries)
-    const bool big_buckets = leaves == kBucket;
-    bool& ready = big_buckets ? B->buckets8_ready : B->buckets_ready;
-    DevBuf& brecs = big_buckets ? B->brecs8 : B->brecs;
-    if (ready) return;
-    const int bmax = big_buckets ? kBucket : kBucketSmall;
+i64 bucket_record_doubles(const CatChunk& c, int precision) {
+    if (precision == PH_PREC_FP64) return c.rw;
+    int ncol = 0;
+    return compact_c32(c, ncol).rw;
+}
+
+bool buckets_supported(const ph_lbvh* B) {
+    return B->dims == 3 && B->precision == PH_PREC_FP64 && B->n <= kMaxBucketLeaves;
+}
+
+void ensure_buckets(ph_lbvh* B, hipStream_t s) {
+    // Two bucket trees per LBVH, built together (one flag pass, one wait for
+    // the record counts): buckets of at most kBucket leaves (the grid tile
+    // kernels) and of at most kBucketSmall (the per-lane kernels, the batch
+    // queue, adjoints and queries).
+    if (B->buckets_ready && B->buckets8_ready) return;
     const i64 n = B->n;
     if (n > kMaxBucketLeaves) throw Error{PH_ERR_CAPACITY, "the bucket traversal supports at most 2^26 particles"};
-    i64 nrec = 1;
-    if (n <= bmax) {
-        brecs.alloc(sizeof(BucketRec));
-        k_bucket_root<<<1, 1, 0, s>>>(n, B->root, B->nodes.as<NodeRec<double>>(), B->leaves.as<LeafRec<double>>(),
-                                      brecs.as<BucketRec>());
-        PH_HIP(hipGetLastError());
-    } else {
-        DevBuf flags, tmp;
-        flags.alloc(sizeof(u32) * (size_t)(2 * n + 1));
-        u32* big_s = flags.as<u32>();  // n entries: n - 1 split flags + the total
-        u32* bs = big_s + n;           // n + 1 entries: n bucket-start flags + the total
-        PH_HIP(hipMemsetAsync(flags.p, 0, sizeof(u32) * (size_t)(2 * n + 1), s));
-        k_bucket_flags<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->left.as<i32>(), B->range.as<i32>(), big_s, bs, bmax);
-        PH_HIP(hipGetLastError());
-        exclusive_sum_u32(big_s, n, tmp, s);
-        exclusive_sum_u32(bs, n + 1, tmp, s);
-        u32 tot[2];
-        PH_HIP(hipMemcpyAsync(&tot[0], big_s + (n - 1), sizeof(u32), hipMemcpyDeviceToHost, s));
-        PH_HIP(hipMemcpyAsync(&tot[1], bs + n, sizeof(u32), hipMemcpyDeviceToHost, s));
+    DevBuf* recs[2] = {&B->brecs, &B->brecs8};
+    const int bmax[2] = {kBucketSmall, kBucket};
+    i64 nrec[2] = {1, 1};
+    // buffers first: flags of both trees (big splits: n entries, n - 1 flags +
+    // the total; bucket starts: n + 1 entries, n flags + the total)
+    const bool split[2] = {n > bmax[0], n > bmax[1]};
+    DevBuf flags, tmp;
+    const size_t per_tree = (size_t)(2 * n + 1);
+    if (split[0] || split[1]) flags.alloc(sizeof(u32) * 2 * per_tree);
+    u32* big_s[2] = {flags.as<u32>(), flags.as<u32>() + per_tree};
+    u32* bs[2] = {big_s[0] + n, big_s[1] + n};
+    u32 tot[2][2] = {{0, 0}, {0, 0}};
+    if (split[0] || split[1]) {
+        PH_HIP(hipMemsetAsync(flags.p, 0, sizeof(u32) * 2 * per_tree, s));
+        for (int t = 0; t < 2; ++t) {
+            if (!split[t]) continue;
+            k_bucket_flags<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->left.as<i32>(), B->range.as<i32>(), big_s[t],
+                                                             bs[t], bmax[t]);
+            PH_HIP(hipGetLastError());
+            exclusive_sum_u32(big_s[t], n, tmp, s);
+            exclusive_sum_u32(bs[t], n + 1, tmp, s);
+            PH_HIP(hipMemcpyAsync(&tot[t][0], big_s[t] + (n - 1), sizeof(u32), hipMemcpyDeviceToHost, s));
+            PH_HIP(hipMemcpyAsync(&tot[t][1], bs[t] + n, sizeof(u32), hipMemcpyDeviceToHost, s));
+        }
         PH_HIP(hipStreamSynchronize(s));
-        nrec = (i64)tot[0] + (i64)tot[1];
-        brecs.alloc(sizeof(BucketRec) * (size_t)nrec);
-        k_bucket_emit<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->root, B->left.as<i32>(), B->parent.as<i32>(),
-                                                        B->range.as<i32>(), B->nodes.as<NodeRec<double>>(),
-                                                        B->leaves.as<LeafRec<double>>(), big_s, bs,
-                                                        brecs.as<BucketRec>(), bmax);
-        PH_HIP(hipGetLastError());
-        PH_HIP(hipStreamSynchronize(s));  // flags / tmp are released on return
     }
-    (big_buckets ? B->nrec8 : B->nrec) = nrec;
-    ready = true;
+    for (int t = 0; t < 2; ++t) {
+        if (split[t]) nrec[t] = (i64)tot[t][0] + (i64)tot[t][1];
+        recs[t]->alloc(sizeof(BucketRec) * (size_t)nrec[t]);
+    }
+    for (int t = 0; t < 2; ++t) {
+        if (!split[t]) {  // the whole tree is one bucket
+            k_bucket_root<<<1, 1, 0, s>>>(n, B->root, B->nodes.as<NodeRec<double>>(),
+                                          B->leaves.as<LeafRec<double>>(), recs[t]->as<BucketRec>());
+        } else {
+            k_bucket_emit<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->root, B->left.as<i32>(), B->parent.as<i32>(),
+                                                            B->range.as<i32>(), B->nodes.as<NodeRec<double>>(),
+                                                            B->leaves.as<LeafRec<double>>(), big_s[t], bs[t],
+                                                            recs[t]->as<BucketRec>(), bmax[t]);
+        }
+        PH_HIP(hipGetLastError());
+    }
+    // flags / tmp are released on return: into the context's cache, reusable
+    // once the stream has drained, or freed directly (wait for the kernels)
+    if (!tl_cache) PH_HIP(hipStreamSynchronize(s));
+    B->nrec = nrec[0];
+    B->nrec8 = nrec[1];
+    B->buckets_ready = B->buckets8_ready = true;
 }
 
 void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl, const GridAxes* ga,
-                   i64 S, void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+                   i64 S, void* out, u64* counters, hipStream_t s, hipEvent_t* ev, const CallWorkspace* ws) {
     if (S <= 0) return;
     if (B->precision != PH_PREC_FP64) throw Error{PH_ERR_ARGUMENT, "the bucket traversal needs a float64 tree"};
     ensure_buckets(B, s);
     switch (B->kernel) {
-    case PH_KERNEL_M4: launch_bucket_k<PH_KERNEL_M4>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev); break;
-    case PH_KERNEL_M5: launch_bucket_k<PH_KERNEL_M5>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev); break;
-    case PH_KERNEL_M6: launch_bucket_k<PH_KERNEL_M6>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev); break;
-    case PH_KERNEL_C2: launch_bucket_k<PH_KERNEL_C2>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev); break;
-    case PH_KERNEL_C4: launch_bucket_k<PH_KERNEL_C4>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev); break;
-    default: launch_bucket_k<PH_KERNEL_C6>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev); break;
+    case PH_KERNEL_M4: launch_bucket_k<PH_KERNEL_M4>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev, ws); break;
+    case PH_KERNEL_M5: launch_bucket_k<PH_KERNEL_M5>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev, ws); break;
+    case PH_KERNEL_M6: launch_bucket_k<PH_KERNEL_M6>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev, ws); break;
+    case PH_KERNEL_C2: launch_bucket_k<PH_KERNEL_C2>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev, ws); break;
+    case PH_KERNEL_C4: launch_bucket_k<PH_KERNEL_C4>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev, ws); break;
+    default: launch_bucket_k<PH_KERNEL_C6>(B, c, strategy, precision, pl, ga, S, out, counters, s, ev, ws); break;
     }
 }
 

@@ -237,23 +237,27 @@ struct ph_lbvh {
     ph::DevBuf rec;          // packed per-particle records of the last interpolation call
     ph::DevBuf ha;           // per-sample smoothing length of the nearest particle
     ph::DevBuf nn;           // per-sample Morton index of the nearest particle (adjoints)
-    // Bucket tree (built on first use): the LBVH in DFS preorder with every
-    // subtree of at most kBucket (8) leaves collapsed into one bucket.
+    // Bucket trees (built with the tree, ensure_buckets): the LBVH in DFS
+    // preorder with every subtree of at most kBucketSmall (4) leaves, resp.
+    // kBucket (8) leaves for the grid tile kernels, collapsed into one bucket.
     bool buckets_ready = false;
     int64_t nrec = 0;
-    bool buckets8_ready = false;  // the same with buckets of up to kBucket leaves (brecs8)
+    bool buckets8_ready = false;
     int64_t nrec8 = 0;
-    ph::DevBuf brecs8;
-    ph::DevBuf brecs;     // bucket-tree records [nrec]
+    ph::DevBuf brecs8;    // 8-leaf bucket-tree records [nrec8]
+    ph::DevBuf brecs;     // 4-leaf bucket-tree records [nrec]
     ph::DevBuf rec32;     // FP32-compensated records of the last call ({m/rho, 1/h}, columns)
 
     // Frees every device buffer (the context is going away; the handle stays
     // valid for ph_lbvh_destroy only).
     void release_device() {
         for (ph::DevBuf* b : {&cols, &codes, &order, &left, &right, &escape, &parent, &range, &nodes, &leaves,
-                              &leaf_escape, &rec, &ha, &nn, &brecs, &rec32})
+                              &leaf_escape, &rec, &ha, &nn, &brecs, &brecs8, &rec32})
             b->release();
         buckets_ready = false;
+        buckets8_ready = false;
+        nrec = 0;
+        nrec8 = 0;
     }
 };
 

@@ -64,17 +64,32 @@ void launch_lines(ph_lbvh* B, const CatChunk& c, int strategy, int precision, co
                   i64 S, int L, void* out, u64* counters, const void* lut, bool wave, hipStream_t s,
                   hipEvent_t* ev = nullptr);
 void launch_nearest_h(ph_lbvh* B, const PointList& pl, i64 S, double* ha, hipStream_t s);
+// Per-call buffers of the bucket launchers supplied by the caller instead of
+// the tree's own (which are allocated on demand): rec holds n x rw doubles of
+// the chunk's packed records, ha S doubles (gather / symmetric).
+struct CallWorkspace {
+    double* rec;
+    double* ha;
+};
+// Doubles per particle of the packed records the bucket launchers use for
+// chunk c at the given precision (PH_PREC_FP64 or PH_PREC_FP32C).
+i64 bucket_record_doubles(const CatChunk& c, int precision);
 // Bucket traversal (float64 or FP32-compensated evaluation): per lane, a DFS over
 // the preorder bucket tree, leaves of reached buckets tested with float64.
 void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl,
-                   const GridAxes* ga, i64 S, void* out, u64* counters, hipStream_t s, hipEvent_t* ev = nullptr);
-// Builds the bucket tree of B if it is not built yet: buckets of at most
-// `leaves` leaves, 4 (the default) or kBucket (8, the grid tile kernels).
-void ensure_buckets(ph_lbvh* B, hipStream_t s, int leaves = 4);
+                   const GridAxes* ga, i64 S, void* out, u64* counters, hipStream_t s, hipEvent_t* ev = nullptr,
+                   const CallWorkspace* ws = nullptr);
+// Builds B's two bucket trees if they are not built yet (buckets of at most
+// 4 leaves: per-lane kernels, batches, adjoints, queries; of at most 8: the
+// grid tile kernels).  ph_lbvh_build calls it for every tree that
+// buckets_supported accepts (3D, float64, at most 2^26 particles).
+void ensure_buckets(ph_lbvh* B, hipStream_t s);
+bool buckets_supported(const ph_lbvh* B);
 // FP32C grid interpolation with one bucket-tree walk per wave (ph_tile.hip);
-// the caller has packed B->rec32 for c32 and computed ha (gather / symmetric).
-void launch_tile_c32(ph_lbvh* B, const CatChunk& c32, int strategy, const GridAxes& ga, i64 S, const double* ha,
-                     void* out, u64* counters, hipStream_t s, hipEvent_t* ev);
+// the caller has packed rec (FP32C records of c32) and computed ha (gather /
+// symmetric).
+void launch_tile_c32(ph_lbvh* B, const double* rec, const CatChunk& c32, int strategy, const GridAxes& ga, i64 S,
+                     const double* ha, void* out, u64* counters, hipStream_t s, hipEvent_t* ev);
 // One output group of point / grid interpolation for the adjoint: a scalar
 // (mode 0, Shepard flag shepard), the gradient of a column (1), or the
 // divergence (2) or curl (3) of three; q: the columns' indices in B->cols.

@@ -273,7 +273,7 @@ void launch_tile_k(int strategy, const BucketView& bv, const double* rec, const 
 }  // namespace
 
 void launch_nearest_tile(ph_lbvh* B, const GridAxes& ga, i64 S, double Kd, double* ha, hipStream_t s, i64* nn) {
-    ensure_buckets(B, s);  // the per-lane searches prefer small buckets
+    ensure_buckets(B, s);
     const BucketView bv = bucket_view(B);
     const LaunchCfg cfg = tl_launch;
     const i64 tiles = ((ga.nx + 3) / 4) * ((ga.ny + 3) / 4) * ((ga.nz + 3) / 4);
@@ -283,11 +283,10 @@ void launch_nearest_tile(ph_lbvh* B, const GridAxes& ga, i64 S, double Kd, doubl
     PH_HIP(hipGetLastError());
 }
 
-void launch_tile_c32(ph_lbvh* B, const CatChunk& c32, int strategy, const GridAxes& ga, i64 S, const double* ha,
-                     void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
-    ensure_buckets(B, s, kBucket);
+void launch_tile_c32(ph_lbvh* B, const double* rec, const CatChunk& c32, int strategy, const GridAxes& ga, i64 S,
+                     const double* ha, void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+    ensure_buckets(B, s);
     const BucketView bv = bucket_view(B, kBucket);
-    const double* rec = B->rec32.as<double>();
     float* o = static_cast<float*>(out);
     switch (B->kernel) {
     case PH_KERNEL_M4: launch_tile_k<PH_KERNEL_M4>(strategy, bv, rec, c32, ga, S, ha, o, counters, s, ev); break;

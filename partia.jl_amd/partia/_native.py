@@ -18,6 +18,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libpartia_hip.so"))
 
 PH_OK = 0
+PH_ERR_ARGUMENT, PH_ERR_DIMENSION, PH_ERR_STRATEGY, PH_ERR_LEAF_ORDER, PH_ERR_CAPACITY = -1, -2, -3, -4, -5
 PH_GATHER, PH_SCATTER, PH_SYMMETRIC = 0, 1, 2
 PH_PREC_FP64, PH_PREC_FP32, PH_PREC_FP32C = 64, 32, 33
 PH_TRAVERSAL_AUTO, PH_TRAVERSAL_LANE, PH_TRAVERSAL_BUCKET, PH_TRAVERSAL_WAVE = 0, 1, 2, 3

@@ -208,10 +208,11 @@ def test_tile_walk_equals_lane_walk_fp32c_grids(backend, kernel, strategy):
     assert max(e[0] for e in field_errors(got, ref, fields=catalog_fields(FULL))) <= FP32C_NORMWISE
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 17, 65, 300])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 16, 17, 65, 300])
 def test_tile_walk_small_trees_and_far_grids(backend, n):
-    """Tiny trees (a single bucket, n = 1), more staged leaves than one batch
-    holds (large h), and grids reaching far beyond the particles."""
+    """Tiny trees (a single bucket, n = 1; n = 8 is one full 8-leaf bucket and
+    n = 9 the first split of the tile kernel's tree), more staged leaves than
+    one batch holds (large h), and grids reaching far beyond the particles."""
     ctx = backend.context
     c6, cols = cloud(n, 90 + n)
     c6 = (c6[0], c6[1], c6[2], c6[3], c6[4] * 0 + (0.9 if n > 60 else 0.3), c6[5])
@@ -227,3 +228,22 @@ def test_tile_walk_small_trees_and_far_grids(backend, n):
         with traversal(ctx, N.PH_TRAVERSAL_LANE):
             lane = d.grid(FULL, strategy, *axes)
         assert_same(d.grid(FULL, strategy, *axes), lane)
+
+
+def test_tree_outliving_its_context_after_fp32c_grid():
+    """A tree whose context was destroyed after an FP32C grid call (which uses
+    both bucket trees) can still be destroyed: the context released every
+    device buffer of the tree, the 8-leaf bucket tree included."""
+    ctx = N.Context(0)
+    c6, cols = cloud(3000, 77)
+    d = DeviceSet(ctx, *c6, cols, kernel=4)
+    axes = (synthetic.linrange(0.0, 1.0, 8),) * 3
+    got = d.grid(FULL, O.SYMMETRIC, *axes, precision=N.PH_PREC_FP32C)
+    assert np.isfinite(got).any()
+    ctx.close()
+    d.close()  # ph_lbvh_destroy of an orphaned tree
+    ctx2 = N.Context(0)  # the device is still usable
+    d2 = DeviceSet(ctx2, *c6, cols, kernel=4)
+    assert_same(d2.grid(FULL, O.SYMMETRIC, *axes, precision=N.PH_PREC_FP32C), got)
+    d2.close()
+    ctx2.close()
