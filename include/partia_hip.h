@@ -236,6 +236,23 @@ int ph_point_samples(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32
 int ph_point_samples_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy,
                             int32_t precision, int64_t S, const double* px_dev, const double* py_dev,
                             const double* pz_dev, void* out_dev);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ph_point_samples_device with the per-call buffers (the
+ * nearest-particle h of every sample and the packed particle records) in a
+ * static device workspace of 64 MiB per GPU instead of allocations.  It
+ * allocates nothing, needs 8 S (gather / symmetric) + 8 n rw bytes of that
+ * workspace (rw = record width, 2..14 doubles) and returns PH_ERR_CAPACITY
+ * beyond it.  PH_PREC_FP64 or PH_PREC_FP32C on a float64 tree of at most 2^26
+ * particles (whose bucket trees ph_lbvh_build made).  The results equal
+ * ph_point_samples_device's bit for bit.  The kernels are only enqueued on the
+ * context's stream (no wait; ph_synchronize before reading out_dev), and
+ * ph_get_stats is not updated.  Every call on one GPU uses that GPU's one
+ * workspace: calls that overlap on one GPU (other contexts' streams, other
+ * threads) race. */
+int ph_point_samples_device_noalloc(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy,
+                                    int32_t precision, int64_t S, const double* px_dev, const double* py_dev,
+                                    const double* pz_dev, void* out_dev);
 
 /* ---- StructuredGrid_interpolation (KI/grid_interpolation/drivers/
  * StructuredGrid_driver.jl:46-64) for Cartesian grids: the flattened sample

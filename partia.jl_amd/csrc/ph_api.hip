@@ -11,7 +11,32 @@
 
 using namespace ph;
 
+// Static workspace of ph_point_samples_device_noalloc: the per-call buffers of
+// the bucket kernels (nearest-particle h per sample, packed particle records)
+// for callers that cannot allocate device memory.  Every GPU the library runs
+// on holds its own copy until the process ends; in host code `scratch` is only
+// a stand-in for the symbol, so the per-device address from
+// hipGetSymbolAddress is what kernels receive.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_addr;  // device -> address of its copy of `scratch`
+
+// Address of `scratch` on the current device (which must be `device`), looked
+// up once per device.
+void* scratch_address(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) return it->second;
+    void* ptr = nullptr;
+    PH_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error{PH_ERR_HIP, "hipGetSymbolAddress returned a null address for the static workspace"};
+    g_scratch_addr[device] = ptr;
+    return ptr;
+}
 
 thread_local std::string g_noctx_error;
 
@@ -261,6 +286,7 @@ int ph_ctx_create(int device, ph_context** out) {
         hipDeviceProp_t prop;
         PH_HIP(hipGetDeviceProperties(&prop, device));
         ctx->num_cu = prop.multiProcessorCount;
+        ctx->noalloc_scratch = scratch_address(device);
         // line-integrated tables, float64 and float32 copies
         const size_t nt = 6 * (size_t)PH_LUT_N;
         ctx->lut64.alloc(sizeof(double) * nt);
@@ -637,6 +663,56 @@ int ph_point_samples_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, 
         run_chunks(ctx, cat, S, [&](const CatChunk& c, int L, u64* counter, hipEvent_t* ev) {
             launch_point_like(ctx, B, c, strategy, precision, &pl, nullptr, S, L, out, counter, ev);
         });
+        return PH_OK;
+    });
+}
+
+int ph_point_samples_device_noalloc(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy,
+                                    int32_t precision, int64_t S, const double* px, const double* py,
+                                    const double* pz, void* out) {
+    if (!ctx) return fail(nullptr, PH_ERR_ARGUMENT, "context is null");
+    return guarded(ctx, [&] {
+        need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
+        need(strategy >= PH_GATHER && strategy <= PH_SYMMETRIC, PH_ERR_ARGUMENT, "unknown strategy");
+        need(S >= 0, PH_ERR_ARGUMENT, "negative sample count");
+        need(precision == PH_PREC_FP64 || precision == PH_PREC_FP32C, PH_ERR_ARGUMENT,
+             "ph_point_samples_device_noalloc evaluates in PH_PREC_FP64 or PH_PREC_FP32C");
+        check_precision(B, precision);
+        check_catalog(B, cat);
+        need(S == 0 || (px && py && pz && out), PH_ERR_ARGUMENT, "null sample or output pointer");
+        need(B->buckets_ready && B->buckets8_ready, PH_ERR_ARGUMENT,
+             "ph_point_samples_device_noalloc needs a tree with bucket trees (3D, float64, at most 2^26 particles)");
+        need(ctx->noalloc_scratch != nullptr, PH_ERR_INTERNAL, "static workspace address missing");
+        if (S == 0) return PH_OK;
+        // workspace: ha (gather / symmetric) at offset 0, then the largest
+        // chunk's packed records, both 256-byte aligned
+        std::vector<CatChunk> chunks;
+        split_catalog(cat, chunks);
+        const i64 cap = (i64)kNoAllocScratchBytes;
+        auto up = [](i64 b) { return (b + 255) & ~i64(255); };
+        need(S <= cap / 8, PH_ERR_CAPACITY, "ph_point_samples_device_noalloc: S exceeds the static workspace");
+        const i64 ha_bytes = strategy != PH_SCATTER ? up(8 * S) : 0;
+        i64 rec_bytes = 0;
+        for (const CatChunk& c : chunks) {
+            const i64 rw = bucket_record_doubles(c, precision);
+            need(B->n <= cap / (8 * rw), PH_ERR_CAPACITY, "ph_point_samples_device_noalloc: records exceed the static workspace");
+            rec_bytes = std::max(rec_bytes, up(8 * rw * B->n));
+        }
+        need(ha_bytes + rec_bytes <= cap, PH_ERR_CAPACITY,
+             "ph_point_samples_device_noalloc needs " + std::to_string(ha_bytes + rec_bytes) +
+                 " bytes of workspace (8 S + 8 n rw); the static workspace holds " + std::to_string(cap) +
+                 " bytes: use ph_point_samples_device");
+        unsigned char* base = static_cast<unsigned char*>(ctx->noalloc_scratch);
+        CallWorkspace ws{reinterpret_cast<double*>(base + ha_bytes), reinterpret_cast<double*>(base)};
+        struct NoAlloc {
+            bool saved = tl_no_alloc;
+            NoAlloc() { tl_no_alloc = true; }
+            ~NoAlloc() { tl_no_alloc = saved; }
+        } no_alloc;
+        PointList pl{px, py, pz};
+        LaunchScope scope(ctx->launch);
+        for (const CatChunk& c : chunks)
+            launch_bucket(B, c, strategy, precision, &pl, nullptr, S, out, nullptr, ctx->stream, nullptr, &ws);
         return PH_OK;
     });
 }

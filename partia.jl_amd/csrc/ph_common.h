@@ -123,6 +123,9 @@ inline int line_order(const LaunchCfg& c) { return c.order < 0 ? 1 : c.order; }
 // Cache of the context whose entry point runs on this thread (set by the entry
 // points), or null: then buffers use hipMalloc / hipFree directly.
 inline thread_local BlockCache* tl_cache = nullptr;
+// Set while an entry point that must not allocate device memory runs: any
+// DevBuf allocation then fails with PH_ERR_INTERNAL instead of allocating.
+inline thread_local bool tl_no_alloc = false;
 
 // Device buffer owned by the library.
 struct DevBuf {
@@ -144,6 +147,7 @@ struct DevBuf {
     }
     void alloc(size_t b) {
         if (b <= bytes && p) return;
+        if (tl_no_alloc) throw Error{PH_ERR_INTERNAL, "device allocation inside an entry point that allocates nothing"};
         release();
         if (b == 0) b = 16;
         b = (b + 255) & ~size_t(255);
@@ -213,6 +217,9 @@ struct ph_context {
         double best_ms = 0.0, default_ms = 0.0;
     };
     std::map<uint64_t, Tuned> tuned;
+    // device address of this GPU's copy of the library's static workspace
+    // (ph_point_samples_device_noalloc), looked up at context creation
+    void* noalloc_scratch = nullptr;
 };
 
 // Device-resident InterpolationInput after LinearBVH!.

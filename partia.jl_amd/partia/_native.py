@@ -33,7 +33,7 @@ EXPORTS = (
     "ph_lbvh_build", "ph_lbvh_build_device", "ph_lbvh_build2d", "ph_lbvh_build2d_device", "ph_lbvh_dims", "ph_query_device", "ph_find_nearest_device",
     "ph_lbvh_destroy", "ph_lbvh_set_columns", "ph_lbvh_info",
     "ph_lbvh_get",
-    "ph_point_samples", "ph_point_samples_device", "ph_structured_grid", "ph_structured_grid_device",
+    "ph_point_samples", "ph_point_samples_device", "ph_point_samples_device_noalloc", "ph_structured_grid", "ph_structured_grid_device",
     "ph_line_samples", "ph_line_samples_device", "ph_line_grid_device", "ph_nearest_h", "ph_line_table",
     "ph_batch_create", "ph_batch_run", "ph_batch_times", "ph_batch_destroy",
     "ph_line_samples_adjoint_device", "ph_line_grid_adjoint_device",
@@ -121,6 +121,7 @@ def lib():
         "ph_lbvh_get": (i32, [vp, i32, vp]),
         "ph_point_samples": (i32, [vp, vp, P(ph_catalog), i32, i32, i64, vp, vp, vp, vp]),
         "ph_point_samples_device": (i32, [vp, vp, P(ph_catalog), i32, i32, i64, vp, vp, vp, vp]),
+        "ph_point_samples_device_noalloc": (i32, [vp, vp, P(ph_catalog), i32, i32, i64, vp, vp, vp, vp]),
         "ph_structured_grid": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, i64, vp, i64, vp, i64, vp]),
         "ph_structured_grid_device": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, i64, vp, i64, vp, i64, vp]),
         "ph_line_samples": (i32, [vp, vp, P(ph_catalog), i32, i32, i64] + [vp] * 7),
