@@ -44,6 +44,7 @@ Other workloads (`--workload`), each printing one line of the same form:
 import argparse
 import ctypes as C
 import json
+import platform
 import os
 import sys
 import time
@@ -144,8 +145,34 @@ def measured_traffic(kernel, workload="plummer2e6_c4_symmetric_grid256"):
     return None, None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def oracle_rate(run, S, target_seconds, stride0):
+    """Samples/s of run(idx) on a strided subset of S samples, the subset grown
+    until it takes about a quarter of target_seconds; returns (rate, count, stride, seconds)."""
+    stride = stride0
+    while True:
+        idx = np.arange(0, S, stride)
+        t0 = time.perf_counter()
+        run(idx)
+        dt = time.perf_counter() - t0
+        if dt * 4 > target_seconds or stride <= 16:
+            return len(idx) / dt, len(idx), stride, dt
+        stride //= 4
+
+
 def cpu_baseline(data, axes, target_seconds):
-    """Oracle (C++ restatement of the reference CPU path) on this host's cores."""
+    """Oracle (C++ restatement of the reference CPU path) on this host: all the
+    threads this process may use, and one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
 
@@ -153,41 +180,72 @@ def cpu_baseline(data, axes, target_seconds):
     t0 = time.perf_counter()
     o = O.OracleSet(data["x"], data["y"], data["z"], data["m"], data["h"], data["rho"], cols,
                     kernel="C4_Wendland", code_bits=63)
-    t_build = time.perf_counter() - t0
+    t_build = time.perf_counter() - t0  # serial, as the reference's sort and permutation
     nthreads = threads()
     S = NGRID ** 3
     ax = axes
-    # strided subset of the flattened grid: spread over the whole volume
-    done, t_interp, stride = 0, 0.0, 4096
-    while True:
-        idx = np.arange(0, S, stride)
-        ix, iy, iz = idx % NGRID, (idx // NGRID) % NGRID, idx // (NGRID * NGRID)
-        t0 = time.perf_counter()
-        o.point_samples(CATALOG, O.SYMMETRIC, ax[ix], ax[iy], ax[iz], nthreads=nthreads)
-        dt = time.perf_counter() - t0
-        done, t_interp = len(idx), dt
-        if dt * 4 > target_seconds or stride <= 16:
-            break
-        stride //= 4
-    rate_interp = done / t_interp
-    total = t_build + S / rate_interp
+
+    def run(nt):
+        def f(idx):
+            ix, iy, iz = idx % NGRID, (idx // NGRID) % NGRID, idx // (NGRID * NGRID)
+            o.point_samples(CATALOG, O.SYMMETRIC, ax[ix], ax[iy], ax[iz], nthreads=nt)
+        return f
+
+    rate, done, stride, t_interp = oracle_rate(run(nthreads), S, target_seconds, 4096)
+    rate1, done1, stride1, t1 = oracle_rate(run(1), S, target_seconds / 2, 65536)
     return {
-        "value": S / total,
+        "value": S / (t_build + S / rate),
         "unit": "samples/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": ("C++ oracle (reference algorithm restated) on %d host threads: LinearBVH! on all %d particles "
-                   "(%.2f s, serial sort as in the reference) + %d of the %d grid samples (every %d-th, %.2f s); "
-                   "whole-grid time extrapolated from that rate" % (nthreads, NPART, t_build, done, S, stride,
-                                                                   t_interp)),
+        "sample": ("C++ oracle (reference algorithm restated) on %d host threads of a %s: LinearBVH! on all %d "
+                   "particles (%.2f s, serial sort as in the reference) + %d of the %d grid samples (every %d-th, "
+                   "%.2f s); whole-grid time extrapolated from that rate" % (nthreads, cpu_model(), NPART, t_build,
+                                                                           done, S, stride, t_interp)),
+        "cpu_model": cpu_model(),
+        "host_cpus": os.cpu_count(),
+        "threads_available": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
         "build_s": t_build,
-        "interp_samples_per_s": rate_interp,
+        "interp_samples_per_s": rate,
+        "one_core": {"value": S / (t_build + S / rate1), "unit": "samples/s", "interp_samples_per_s": rate1,
+                     "sample": "%d of the %d grid samples (every %d-th, %.2f s) on 1 thread" % (done1, S, stride1,
+                                                                                             t1)},
     }
 
 
+FIELDS = (("rho", [0]), ("A", [1]), ("grad_A", [2, 3, 4]), ("div_v", [5]), ("curl_v", [6, 7, 8]))
+
+
+def field_errors(got, ref):
+    """Per output field of the C2 catalog (a vector field counted whole), over
+    the samples with neighbours: max and RMS of |got - ref|, the RMS of ref,
+    the normwise error ||got - ref|| / ||ref|| and the largest pointwise
+    |got - ref| / max(|ref|, 1e-3 rms ref); NaN masks must agree."""
+    out = {}
+    for name, rows in FIELDS:
+        g, r = got[rows].astype(np.float64), ref[rows]
+        same_nan = bool(np.array_equal(np.isnan(g), np.isnan(r)))
+        f = ~np.isnan(r[0])
+        d = np.abs(g[:, f] - r[:, f])
+        rr = r[:, f]
+        rms_ref = float(np.sqrt(np.mean(rr * rr))) if rr.size else 0.0
+        out[name] = {
+            "nan_masks_equal": same_nan,
+            "max_abs": float(d.max()) if d.size else 0.0,
+            "rms_abs": float(np.sqrt(np.mean(d * d))) if d.size else 0.0,
+            "rms_ref": rms_ref,
+            "normwise": float(np.linalg.norm(d) / max(np.linalg.norm(rr), 1e-300)),
+            "max_rel_pointwise": float(np.max(d / np.maximum(np.abs(rr), max(1e-3 * rms_ref, 1e-300))))
+            if d.size else 0.0,
+        }
+    return out
+
+
 def threads():
+    """Threads of the CPU baseline: the CPUs this process may use, at most the
+    16 CPUs a one-GPU job of the GPU box is given (OMP_NUM_THREADS there)."""
     n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    return max(1, min(int(n), 16))
+    return max(1, min(int(n), int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16))
 
 
 def timed_steps(ctx, step, steps, warmup, label):
@@ -818,7 +876,10 @@ def main():
     build_ms = float(np.mean(builds))
 
     fp64 = None
+    fp32c_error = None
     if prec != N.PH_PREC_FP64 and not args.no_fp64_compare:
+        got32 = np.zeros(L_OUT * S, np.float32)
+        ctx.check(L.ph_memcpy_dtoh(ctx.handle, got32.ctypes.data_as(C.c_void_p), dout.p, got32.nbytes))
         step(N.PH_PREC_FP64)  # warm-up
         ctx.check(L.ph_synchronize(ctx.handle))
         t1 = time.perf_counter()
@@ -828,6 +889,23 @@ def main():
         fp64 = {"ms_per_step": ms64, "samples_per_s": S / (ms64 * 1e-3), "accumulation_kernel_ms": float(np.mean(k64))}
         print("bench: float64 %.2f ms per step (accumulation kernel %.2f ms)" % (ms64, fp64["accumulation_kernel_ms"]),
               file=sys.stderr, flush=True)
+        ref64 = np.zeros(L_OUT * S)
+        ctx.check(L.ph_memcpy_dtoh(ctx.handle, ref64.ctypes.data_as(C.c_void_p), dout.p, ref64.nbytes))
+        fp32c_error = field_errors(got32.reshape(L_OUT, S), ref64.reshape(L_OUT, S))
+        del got32, ref64
+        print("bench: FP32C vs float64 per field: %s" % ", ".join(
+            "%s normwise %.2e max %.2e" % (k, v["normwise"], v["max_rel_pointwise"]) for k, v in fp32c_error.items()),
+            file=sys.stderr, flush=True)
+        # the same float64 steps with one bucket walk per sample (k_bucket_f64)
+        ctx.set_traversal(N.PH_TRAVERSAL_BUCKET)
+        step(N.PH_PREC_FP64)
+        ctx.check(L.ph_synchronize(ctx.handle))
+        t1 = time.perf_counter()
+        kb = [step(N.PH_PREC_FP64)[2] for _ in range(args.steps)]
+        ctx.check(L.ph_synchronize(ctx.handle))
+        msb = 1e3 * (time.perf_counter() - t1) / args.steps
+        fp64["per_lane_bucket_walk"] = {"ms_per_step": msb, "samples_per_s": S / (msb * 1e-3),
+                                        "accumulation_kernel_ms": float(np.mean(kb))}
         # the same float64 steps with the reference-structured walk: one DFS per
         # sample over the LBVH, leaf by leaf (what a one-thread-per-sample port of
         # the reference's GPU kernel does)
@@ -851,7 +929,7 @@ def main():
     # the accumulation kernel the call ran: FP32C grids walk one bucket-tree
     # union per wave (k_tile_c32) unless the per-lane bucket walk is forced
     if prec == N.PH_PREC_FP64:
-        kname = "k_points" if args.traversal == "lane" else "k_bucket_f64"
+        kname = {"lane": "k_points", "bucket": "k_bucket_f64"}.get(args.traversal, "k_tile_f64")
     else:
         kname = "k_bucket_c32" if args.traversal == "bucket" else "k_tile_c32"
     traffic, traffic_src = measured_traffic(kname)
@@ -883,6 +961,8 @@ def main():
                      for e in ctx.autotune_report()],
         "fp64": fp64,
         "speedup_vs_fp64": (fp64["ms_per_step"] / ms_per_step) if fp64 else None,
+        "speedup_vs_fp64_kernel": (fp64["accumulation_kernel_ms"] / kernel_ms) if fp64 else None,
+        "fp32c_error_vs_fp64": fp32c_error,
         "speedup_vs_fp64_per_leaf_walk": (fp64["per_leaf_walk"]["ms_per_step"] / ms_per_step) if fp64 else None,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,

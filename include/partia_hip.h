@@ -241,8 +241,9 @@ int ph_point_samples_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat
  * forbid hipMalloc): ph_point_samples_device with the per-call buffers (the
  * nearest-particle h of every sample and the packed particle records) in a
  * static device workspace of 64 MiB per GPU instead of allocations.  It
- * allocates nothing, needs 8 S (gather / symmetric) + 8 n rw bytes of that
- * workspace (rw = record width, 2..14 doubles) and returns PH_ERR_CAPACITY
+ * allocates nothing, needs 8 S (gather / symmetric) + 8 n rw (FP32C:
+ * 8 (n + 1) rw) bytes of that workspace (rw = record width, 2..14 doubles)
+ * and returns PH_ERR_CAPACITY
  * beyond it.  PH_PREC_FP64 or PH_PREC_FP32C on a float64 tree of at most 2^26
  * particles (whose bucket trees ph_lbvh_build made).  The results equal
  * ph_point_samples_device's bit for bit.  The kernels are only enqueued on the

@@ -695,12 +695,14 @@ int ph_point_samples_device_noalloc(ph_context* ctx, ph_lbvh* B, const ph_catalo
         i64 rec_bytes = 0;
         for (const CatChunk& c : chunks) {
             const i64 rw = bucket_record_doubles(c, precision);
-            need(B->n <= cap / (8 * rw), PH_ERR_CAPACITY, "ph_point_samples_device_noalloc: records exceed the static workspace");
-            rec_bytes = std::max(rec_bytes, up(8 * rw * B->n));
+            const i64 nrec = bucket_records(B->n, precision);
+            need(nrec <= cap / (8 * rw), PH_ERR_CAPACITY,
+                 "ph_point_samples_device_noalloc: records exceed the static workspace");
+            rec_bytes = std::max(rec_bytes, up(8 * rw * nrec));
         }
         need(ha_bytes + rec_bytes <= cap, PH_ERR_CAPACITY,
              "ph_point_samples_device_noalloc needs " + std::to_string(ha_bytes + rec_bytes) +
-                 " bytes of workspace (8 S + 8 n rw); the static workspace holds " + std::to_string(cap) +
+                 " bytes of workspace (8 S + 8 rw records); the static workspace holds " + std::to_string(cap) +
                  " bytes: use ph_point_samples_device");
         unsigned char* base = static_cast<unsigned char*>(ctx->noalloc_scratch);
         CallWorkspace ws{reinterpret_cast<double*>(base + ha_bytes), reinterpret_cast<double*>(base)};

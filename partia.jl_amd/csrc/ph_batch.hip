@@ -299,7 +299,7 @@ void record(BatchState& st, ph_lbvh* B, int num_cu, hipStream_t s) {
         if (st.precision == PH_PREC_FP64)
             k_pack_records<double><<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c, st.rec.as<double>());
         else
-            k_pack_c32<<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c, st.ncols[q], c.rw,
+            k_pack_c32<<<grid_for(n + 1), kBlock, 0, s>>>(B->cols.as<double>(), n, c, st.ncols[q], c.rw,
                                                       st.rec.as<double>());
         PH_HIP(hipGetLastError());
         if (st.units == 0) continue;
@@ -486,7 +486,7 @@ BatchState* batch_build(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int 
     if (precision != PH_PREC_FP64 && (i64)st->rw_max * 8 * n >= (i64(1) << 32))
         throw Error{PH_ERR_CAPACITY, "FP32C records exceed 4 GiB; split the catalog or use PH_PREC_FP64"};
     ensure_buckets(B, ctx->stream);
-    st->rec.alloc(sizeof(double) * (size_t)st->rw_max * (size_t)n);
+    st->rec.alloc(sizeof(double) * (size_t)st->rw_max * (size_t)(n + 1));  // FP32C: + the zero record
     st->ctl.alloc(sizeof(unsigned long long) * (1 + st->chunks.size()));
     st->times.alloc(sizeof(unsigned long long) * 2 * (size_t)std::max(nreq, 1));
     st->reqs.alloc(sizeof(BatchReq) * (size_t)std::max(nreq, 1));

@@ -235,9 +235,9 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
         recp = ws ? ws->rec : B->rec.as<double>();
     } else {
         c32 = compact_c32(c, ncol);
-        if ((i64)c32.rw * 8 * n >= (i64(1) << 32))
+        if ((i64)c32.rw * 8 * (n + 1) >= (i64(1) << 32))
             throw Error{PH_ERR_CAPACITY, "FP32C records exceed 4 GiB; split the catalog or use PH_PREC_FP64"};
-        if (!ws) B->rec32.alloc(sizeof(double) * (size_t)c32.rw * (size_t)n);
+        if (!ws) B->rec32.alloc(sizeof(double) * (size_t)c32.rw * (size_t)(n + 1));  // + the zero record
         recp = ws ? ws->rec : B->rec32.as<double>();
     }
     if (STRAT != PH_SCATTER) {
@@ -262,14 +262,18 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
     if (precision == PH_PREC_FP64) {
         k_pack_records<double><<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c, recp);
         PH_HIP(hipGetLastError());
+        if (ga && tl_launch.tile) {  // one walk per wave of grid points
+            launch_tile(B, recp, c, precision, STRAT, *ga, S, ha, out, counters, s, ev);
+            return;
+        }
         if (small) launch_f64_t<K, STRAT, 2, 0, 0, 0>(B, recp, c, pl, ga, S, ha, out, counters, s, ev);
         else launch_f64_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC>(B, recp, c, pl, ga, S, ha, out, counters, s, ev);
         return;
     }
-    k_pack_c32<<<grid_for(n), kBlock, 0, s>>>(B->cols.as<double>(), n, c32, ncol, c32.rw, recp);
+    k_pack_c32<<<grid_for(n + 1), kBlock, 0, s>>>(B->cols.as<double>(), n, c32, ncol, c32.rw, recp);
     PH_HIP(hipGetLastError());
     if (ga && tl_launch.tile) {  // one walk per wave of grid points
-        launch_tile_c32(B, recp, c32, STRAT, *ga, S, ha, out, counters, s, ev);
+        launch_tile(B, recp, c32, precision, STRAT, *ga, S, ha, out, counters, s, ev);
         return;
     }
     if (small) launch_c32_t<K, STRAT, 2, 0, 0, 0, 0>(B, recp, c32, pl, ga, S, ha, out, counters, s, ev);
@@ -296,6 +300,8 @@ i64 bucket_record_doubles(const CatChunk& c, int precision) {
     int ncol = 0;
     return compact_c32(c, ncol).rw;
 }
+
+i64 bucket_records(i64 n, int precision) { return precision == PH_PREC_FP64 ? n : n + 1; }
 
 bool buckets_supported(const ph_lbvh* B) {
     return B->dims == 3 && B->precision == PH_PREC_FP64 && B->n <= kMaxBucketLeaves;

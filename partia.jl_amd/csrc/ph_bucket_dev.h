@@ -128,11 +128,20 @@ __device__ inline float f32_up(double x) {
 // FP32-compensated record per particle, rw doubles: element 0 holds {m/rho, 1/h}
 // as two float32 (rounded once from float64), elements 1..ncol the float64
 // values of the distinct columns the catalog chunk uses (source c32.rsrc[1 + j]).
+// Record n (one past the particles) is the zero record {0, 1}, 0, ...: lanes
+// without a pending neighbour read it in the lockstep loops, so every term they
+// add is 0 whatever the particles' columns hold (a NaN included).  Launch with
+// n + 1 threads; rec holds (n + 1) rw doubles.
 __global__ void __launch_bounds__(kBlock) k_pack_c32(const double* __restrict__ cols, i64 n, CatChunk c32, int ncol,
                                              int rw, double* __restrict__ rec) {
     const i64 b = (i64)blockIdx.x * kBlock + threadIdx.x;
-    if (b >= n) return;
+    if (b > n) return;
     double* r = rec + b * rw;
+    if (b == n) {
+        r[0] = __builtin_bit_cast(double, make_float2(0.f, 1.f));
+        for (int j = 1; j < rw; ++j) r[j] = 0.0;
+        return;
+    }
     const float2 vi = make_float2((float)(cols[3 * n + b] / cols[5 * n + b]), (float)(1.0 / cols[4 * n + b]));
     r[0] = __builtin_bit_cast(double, vi);
     for (int j = 0; j < ncol; ++j) r[1 + j] = cols[(i64)c32.rsrc[1 + j] * n + b];
@@ -615,7 +624,7 @@ __device__ __attribute__((always_inline)) void bucket_c32_sample(const BucketVie
     __shared__ u32 blist[kQ][kBlock];
     const int tid = threadIdx.x;
     const u32 nlast = (u32)(bv.n - 1);
-    const Buf rb = Buf::of(rec, (i64)bv.n * rw * 8);
+    const Buf rb = Buf::of(rec, ((i64)bv.n + 1) * rw * 8);  // with the zero record n
     const Buf lb = Buf::of(bv.leaves, (i64)bv.n * (i64)sizeof(LeafRec<double>));
 
     AccC<K, STRAT, MS, MG, MD, MC, EXACT> acc;
@@ -648,8 +657,9 @@ __device__ __attribute__((always_inline)) void bucket_c32_sample(const BucketVie
 #pragma unroll
             for (int u = 0; u < PH_ACC_UNROLL; ++u) {
                 const double dx = p[0] - L[u].x, dy = p[1] - L[u].y, dz = p[2] - L[u].z;
-                acc.add(live[u], c, rb, k[u] * (u32)rw * 8u, dx, dy, dz, (dx * dx + dy * dy) + dz * dz, iha2, Wa_norm,
-                        Ga_norm);
+                // a lane without a pending neighbour reads the zero record
+                acc.add(live[u], c, rb, (live[u] ? k[u] : (u32)bv.n) * (u32)rw * 8u, dx, dy, dz,
+                        (dx * dx + dy * dy) + dz * dz, iha2, Wa_norm, Ga_norm);
             }
         }
         if (!__any(i < bv.nrec && steps > 0)) break;

@@ -74,6 +74,9 @@ struct CallWorkspace {
 // Doubles per particle of the packed records the bucket launchers use for
 // chunk c at the given precision (PH_PREC_FP64 or PH_PREC_FP32C).
 i64 bucket_record_doubles(const CatChunk& c, int precision);
+// Records the bucket launchers pack for n particles at that precision (the
+// FP32C layout adds a zero record at index n).
+i64 bucket_records(i64 n, int precision);
 // Bucket traversal (float64 or FP32-compensated evaluation): per lane, a DFS over
 // the preorder bucket tree, leaves of reached buckets tested with float64.
 void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl,
@@ -85,11 +88,12 @@ void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, c
 // buckets_supported accepts (3D, float64, at most 2^26 particles).
 void ensure_buckets(ph_lbvh* B, hipStream_t s);
 bool buckets_supported(const ph_lbvh* B);
-// FP32C grid interpolation with one bucket-tree walk per wave (ph_tile.hip);
-// the caller has packed rec (FP32C records of c32) and computed ha (gather /
-// symmetric).
-void launch_tile_c32(ph_lbvh* B, const double* rec, const CatChunk& c32, int strategy, const GridAxes& ga, i64 S,
-                     const double* ha, void* out, u64* counters, hipStream_t s, hipEvent_t* ev);
+// Grid interpolation with one bucket-tree walk per wave (ph_tile.hip), FP32C
+// (float outputs) or float64 (precision PH_PREC_FP64, double outputs); the
+// caller has packed rec (the chunk's records in that precision's layout) and
+// computed ha (gather / symmetric).
+void launch_tile(ph_lbvh* B, const double* rec, const CatChunk& c, int precision, int strategy, const GridAxes& ga,
+                 i64 S, const double* ha, void* out, u64* counters, hipStream_t s, hipEvent_t* ev);
 // One output group of point / grid interpolation for the adjoint: a scalar
 // (mode 0, Shepard flag shepard), the gradient of a column (1), or the
 // divergence (2) or curl (3) of three; q: the columns' indices in B->cols.

@@ -330,6 +330,84 @@ template <class T, int K, int STRAT, int MS, int MG, int MD, int MC> struct Acc 
         }
     }
 
+    // add() for lanes stepping in lockstep, the record read through r(pos): a
+    // lane without a pending neighbour (live = false) passes a record whose
+    // mass and columns are 0 and whose density is 1, and its kernel value and
+    // gradient are selected to 0 (not multiplied: a dead lane's h_a may be 0),
+    // so every term it adds is +0 or -0.  The accumulators start at +0 and a
+    // sum that starts at +0 never becomes -0, so adding a signed zero leaves it
+    // unchanged: live lanes get add()'s bits, in add()'s order.
+    template <class Rec>
+    __device__ __attribute__((always_inline)) void add_live(bool live, const CatChunk& c, Rec&& r, T dx, T dy, T dz,
+                                                            T d2, T ha, T hb) {
+        const T dr = sqrt(d2);
+        const T mb = r(0), rb = r(1);
+        const T W = live ? Wsel(dr, ha, hb) : T(0);
+        S1 += mb * W / rb;
+#pragma unroll
+        for (int j = 0; j < MS; ++j)
+            if (j < c.ns) sc[j] += r(c.spos[j]) * (mb * W / rb);
+        if constexpr (MG + MD + MC > 0) {
+            if (c.ng + c.nd + c.nc == 0) return;
+            T g[3];
+            Gsel(dx, dy, dz, ha, hb, g);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) g[d] = live ? g[d] : T(0);
+            const T invr = T(1) / rb;
+#pragma unroll
+            for (int j = 0; j < MG; ++j) {
+                if (j >= c.ng) break;
+                const int pos = c.gpos[j];
+                if (pos < 0) {  // density gradient (gradient_accumulation.jl:7-35)
+                    for (int d = 0; d < 3; ++d) {
+                        const T mg = mb * g[d];
+                        gf[j][d] += mg;
+                        gb[j][d] += mg * invr;
+                    }
+                    gs[j] += mb * W;
+                } else {
+                    const T A = r(pos);
+                    for (int d = 0; d < 3; ++d) {
+                        const T w = mb * invr * g[d];
+                        gf[j][d] += w * A;
+                        gb[j][d] += w;
+                    }
+                    gs[j] += A * (mb * W / rb);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < MD; ++j) {
+                if (j >= c.nd) break;
+                T A[3], w[3];
+                for (int d = 0; d < 3; ++d) {
+                    A[d] = r(c.dpos[3 * j + d]);
+                    w[d] = mb * invr * g[d];
+                }
+                df[j] += w[0] * A[0] + w[1] * A[1] + w[2] * A[2];
+                for (int d = 0; d < 3; ++d) {
+                    db[j][d] += w[d];
+                    ds[j][d] += A[d] * (mb * W / rb);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < MC; ++j) {
+                if (j >= c.nc) break;
+                T A[3], w[3];
+                for (int d = 0; d < 3; ++d) {
+                    A[d] = r(c.cpos[3 * j + d]);
+                    w[d] = mb * invr * g[d];
+                }
+                cf[j][0] += A[1] * w[2] - A[2] * w[1];
+                cf[j][1] += A[2] * w[0] - A[0] * w[2];
+                cf[j][2] += A[0] * w[1] - A[1] * w[0];
+                for (int d = 0; d < 3; ++d) {
+                    cb[j][d] += w[d];
+                    cs[j][d] += A[d] * (mb * W / rb);
+                }
+            }
+        }
+    }
+
     // Epilogue (general_interpolation.jl:128-214); out[k*S + s].
     __device__ inline void finish(const CatChunk& c, T* __restrict__ out, i64 S, i64 s) const {
         if (S1 == T(0)) {

@@ -247,3 +247,55 @@ def test_tree_outliving_its_context_after_fp32c_grid():
     assert_same(d2.grid(FULL, O.SYMMETRIC, *axes, precision=N.PH_PREC_FP32C), got)
     d2.close()
     ctx2.close()
+
+
+@pytest.mark.parametrize("kernel", range(6))
+@pytest.mark.parametrize("strategy", STRATS)
+def test_tile_walk_f64_grids_bitwise(backend, kernel, strategy):
+    """Float64 grids: the per-wave walk (k_tile_f64, default) equals the
+    per-lane walk and the oracle bit for bit with the same neighbour counts,
+    for the C2 catalog, a two-scalar catalog and a catalog split into chunks
+    with a density gradient; ragged tile edges."""
+    ctx = backend.context
+    d, o = pair(ctx, 9000, kernel, 400 + kernel)
+    axes = (synthetic.linrange(-0.15, 1.1, 19), synthetic.linrange(0.0, 1.0, 11), synthetic.linrange(0.1, 0.95, 7))
+    pts = grid_points(*axes)
+    for cat in (FULL, SMALL, WIDE):
+        res = {}
+        for mode in (N.PH_TRAVERSAL_AUTO, N.PH_TRAVERSAL_BUCKET):
+            with traversal(ctx, mode):
+                ctx.set_count_neighbours(True)
+                try:
+                    res[mode] = (d.grid(cat, strategy, *axes), ctx.stats()["accepted"])
+                finally:
+                    ctx.set_count_neighbours(False)
+        assert_same(res[N.PH_TRAVERSAL_AUTO][0], res[N.PH_TRAVERSAL_BUCKET][0])
+        assert res[N.PH_TRAVERSAL_AUTO][1] == res[N.PH_TRAVERSAL_BUCKET][1]
+        assert_same(res[N.PH_TRAVERSAL_AUTO][0], o.point_samples(cat, strategy, *pts))
+
+
+@pytest.mark.parametrize("strategy", STRATS)
+def test_nan_column_values_stay_local(backend, strategy):
+    """A NaN in a column reaches only the samples that have that particle as a
+    neighbour, as in the reference: lanes stepping in lockstep without a
+    pending neighbour must not read it (the Morton-last particle, which
+    per-lane kernels once used as a stand-in, is among the NaN ones)."""
+    ctx = backend.context
+    c6, cols = cloud(6000, 91)
+    order = DeviceSet(ctx, *c6, cols, kernel=4).get("order", np.uint64, 6000)
+    bad = np.concatenate([[int(order[-1]) - 1, int(order[0]) - 1], np.random.default_rng(3).integers(0, 6000, 6)])
+    cols = [c.copy() for c in cols]
+    cols[1][bad] = np.nan
+    d, o = DeviceSet(ctx, *c6, cols, kernel=4), oracle_of(*c6, cols, kernel=4)
+    axes = (synthetic.linrange(-0.05, 1.05, 21), synthetic.linrange(0.0, 1.0, 13), synthetic.linrange(0.0, 1.0, 9))
+    P = np.random.default_rng(4).random((3, 3000)) * 1.1 - 0.05
+    fields = catalog_fields(FULL)
+    ref_g, ref_p = o.point_samples(FULL, strategy, *grid_points(*axes)), o.point_samples(FULL, strategy, *P)
+    assert np.isnan(ref_g[1]).any() and np.isfinite(ref_g[1]).sum() > ref_g.shape[1] // 2
+    for mode in (N.PH_TRAVERSAL_AUTO, N.PH_TRAVERSAL_BUCKET):
+        with traversal(ctx, mode):
+            assert_same(d.grid(FULL, strategy, *axes), ref_g)
+            assert_same(d.points(FULL, strategy, *P), ref_p)
+            for got, ref in ((d.grid(FULL, strategy, *axes, precision=N.PH_PREC_FP32C), ref_g),
+                             (d.points(FULL, strategy, *P, precision=N.PH_PREC_FP32C), ref_p)):
+                assert max(e[0] for e in field_errors(got, ref, fields=fields)) <= FP32C_NORMWISE
