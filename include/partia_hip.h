@@ -383,6 +383,51 @@ int ph_structured_grid_adjoint_device(ph_context* ctx, ph_lbvh* bvh, const ph_ca
                                       int64_t ny, const double* az_dev, int64_t nz, const void* cot_dev,
                                       double* grad_dev);
 
+/* ---- the reference's single-quantity kernels (its building blocks outside
+ * the drivers, used by its tests): one lane per sample walks the reference's
+ * DFS and accumulates in the tree's float type, so float64 results equal the
+ * reference's arithmetic bit for bit.  precision: PH_PREC_FP64 on a float64
+ * tree or PH_PREC_FP32 on a Float32 tree; outputs of that type (device
+ * variants) or double (host variants). ---- */
+#define PH_SINGLE_DENSITY 0          /* _density_kernel: sum m_b W_ab (0 without neighbours); 1 x S */
+#define PH_SINGLE_NUMBER_DENSITY 1   /* _number_density_kernel: sum W_ab; 1 x S                      */
+#define PH_SINGLE_GRADIENT_DENSITY 2 /* _gradient_density_kernel: 3 x S (x, y, z rows), NaN without  */
+/* _density_kernel, _number_density_kernel (KI/single_point_interpolation/
+ * kernels/scalar_interpolation.jl:1-166) and _gradient_density_kernel
+ * (gradient_interpolation.jl:1-170, whose mean term uses the unnormalised
+ * density sum m_b W, :47-55, unlike the fused kernel's density gradient) at S
+ * points.  ha (one per point, gather / symmetric) is the kernels' h_a argument;
+ * null takes each point's nearest-particle h (LinearBVHQueries.jl:118-140), as
+ * the drivers do. */
+int ph_point_single(ph_context* ctx, ph_lbvh* bvh, int32_t kind, int32_t strategy, int32_t precision, int64_t S,
+                    const double* px_host, const double* py_host, const double* pz_host, const double* ha_host,
+                    double* out_host);
+int ph_point_single_device(ph_context* ctx, ph_lbvh* bvh, int32_t kind, int32_t strategy, int32_t precision,
+                           int64_t S, const double* px_dev, const double* py_dev, const double* pz_dev,
+                           const double* ha_dev, void* out_dev);
+/* _line_integrated_quantities_interpolate_kernel in every strategy
+ * (line_integrated_scalar_interpolation.jl:76-214): the catalog's scalars
+ * along S lines with the caller's h_a per line (ha required for gather and
+ * symmetric, unused by scatter).  The drivers accept scatter only
+ * (LineSamples_driver.jl:40-44); this is the kernel-level call. */
+int ph_line_samples_ha(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                       int64_t S, const double* ox_host, const double* oy_host, const double* oz_host,
+                       const double* dx_host, const double* dy_host, const double* dz_host, const double* ha_host,
+                       double* out_host);
+int ph_line_samples_ha_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat, int32_t strategy,
+                              int32_t precision, int64_t S, const double* ox_dev, const double* oy_dev,
+                              const double* oz_dev, const double* dx_dev, const double* dy_dev, const double* dz_dev,
+                              const double* ha_dev, void* out_dev);
+/* _line_integrated_density_kernel (line_integrated_scalar_interpolation.jl:1-70):
+ * the column sum m_b W along each line (no rho, no Shepard; 0 without
+ * neighbours), 1 x S, ha as for ph_line_samples_ha. */
+int ph_line_density(ph_context* ctx, ph_lbvh* bvh, int32_t strategy, int32_t precision, int64_t S,
+                    const double* ox_host, const double* oy_host, const double* oz_host, const double* dx_host,
+                    const double* dy_host, const double* dz_host, const double* ha_host, double* out_host);
+int ph_line_density_device(ph_context* ctx, ph_lbvh* bvh, int32_t strategy, int32_t precision, int64_t S,
+                           const double* ox_dev, const double* oy_dev, const double* oz_dev, const double* dx_dev,
+                           const double* dy_dev, const double* dz_dev, const double* ha_dev, void* out_dev);
+
 /* ---- building blocks exposed for parity tests ---- */
 /* LBVH_probe_neighbors and LBVH_query! (LBVH/LinearBVH/LinearBVHQueries.jl:22-42,
  * 183-211) for S points with radii radius_dev[S], on a float64 tree, device

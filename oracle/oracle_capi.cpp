@@ -233,6 +233,53 @@ int oracle_line_samples(void* hv, int nS, const int* sslots, const unsigned char
     return 0;
 }
 
+// The single-quantity point kernels (single_interpolate): out is 1 x S (kind 0
+// density, 1 number density) or 3 x S (kind 2 density gradient).  ha (per
+// point, may be null) is the caller's smoothing length for gather / symmetric;
+// null takes each point's nearest-particle h, as the drivers do.
+int oracle_point_single(void* hv, int kind, int strategy, int64_t S, const double* px, const double* py,
+                        const double* pz, const double* ha, double* out, int nthreads) {
+    Handle* H = static_cast<Handle*>(hv);
+    const int L = kind == 2 ? 3 : 1;
+    auto run = [&](auto& in) {
+        using T = std::decay_t<decltype(in.x[0])>;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+        {
+            T tmp[3];
+#pragma omp for schedule(dynamic, 64)
+            for (int64_t s = 0; s < S; ++s) {
+                T p[3] = {T(px[s]), T(py[s]), T(pz[s])};
+                T h = strategy == SCATTER ? T(0) : ha ? T(ha[s]) : nearest_h<T>(in.bvh, p);
+                single_interpolate<T>(in, kind, strategy, p, h, tmp, 1);
+                for (int k = 0; k < L; ++k) out[(int64_t)k * S + s] = double(tmp[k]);
+            }
+        }
+    };
+    if (H->precision == 64) run(H->d);
+    else run(H->f);
+    return 0;
+}
+
+// _line_integrated_density_kernel per line (line_density); ha as in
+// oracle_line_samples.
+int oracle_line_density(void* hv, int strategy, int64_t S, const double* ox, const double* oy, const double* oz,
+                        const double* mx, const double* my, const double* mz, const double* ha, double* out,
+                        int nthreads) {
+    Handle* H = static_cast<Handle*>(hv);
+    auto run = [&](auto& in, const auto& lut) {
+        using T = std::decay_t<decltype(in.x[0])>;
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : omp_get_max_threads()) schedule(dynamic, 64)
+        for (int64_t s = 0; s < S; ++s) {
+            T o[3] = {T(ox[s]), T(oy[s]), T(oz[s])};
+            T m[3] = {T(mx[s]), T(my[s]), T(mz[s])};
+            out[s] = double(line_density<T>(in, lut, strategy, o, m, ha ? T(ha[s]) : T(0)));
+        }
+    };
+    if (H->precision == 64) run(H->d, *H->lut_d);
+    else run(H->f, *H->lut_f);
+    return 0;
+}
+
 // Adjoint of oracle_line_samples for catalog scalar `slot` (scatter): grad is
 // 6 x n float64 in input order (d/dm, d/dA, d/dh, d/dx, d/dy, d/dz), summed over
 // the rays with cotangents cot[S].  Serial (the rays share particles).

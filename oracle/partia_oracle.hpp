@@ -929,6 +929,95 @@ void line_interpolate(const Input<T>& in, const LineLUT<T>& lut, const Catalog& 
     for (size_t j = 0; j < N; ++j) out[j * stride] = cat.sflags[j] ? acc[j] * invS1 : acc[j];
 }
 
+// Single-quantity kernels of the reference's test API at point p (ha: gather /
+// symmetric), the neighbours of point_traversal in DFS order:
+//  kind 0, _density_kernel (KI/single_point_interpolation/kernels/
+//    scalar_interpolation.jl:1-85, _density_accumulation of
+//    accumulations/scalar_accumulation.jl:6-27): sum m_b W, 0 without neighbours;
+//  kind 1, _number_density_kernel (:86-166, _number_density_accumulation
+//    :29-49): sum W;
+//  kind 2, _gradient_density_kernel (kernels/gradient_interpolation.jl:1-170,
+//    _gradient_density_accumulation of accumulations/gradient_accumulation.jl:
+//    1-60): f = sum m_b grad W and b = sum m_b grad W * inv(rho_b) with
+//    grad W of ra - rb (kernel.jl:470-477), rho = sum m_b W of the same walk
+//    (the unnormalised density: no Shepard division), then b *= rho and
+//    grad rho = f - b; NaN x 3 when rho == 0.
+// W and grad W of symmetric are the halves of the ha and hb terms.  The
+// distance r = sqrt(sum (ra - rb)^2) of kernel.jl:183-190 sums the squares from
+// zero in x, y, z order, which is d2_point_leaf's value.
+template <class T>
+void single_interpolate(const Input<T>& in, int kind, int strategy, const T p[3], T ha, T* out, i64 stride) {
+    const int K = in.kernel;
+    const T Kvalid = kernel_support<T>(K);
+    const bool sym = strategy == SYMMETRIC;
+    T rho = T(0), f[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
+    auto Wsel = [&](T dr, T hb) -> T {
+        if (sym) return T(0.5) * (kernel_W<T>(K, dr, ha) + kernel_W<T>(K, dr, hb));
+        return kernel_W<T>(K, dr, strategy == GATHER ? ha : hb);
+    };
+    point_traversal<T>(in.bvh, p, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
+        const size_t i = (size_t)k - 1;
+        const T dr = std::sqrt(d2);
+        const T mb = in.m[i];
+        if (kind == 0) {
+            rho += mb * Wsel(dr, hb);
+        } else if (kind == 1) {
+            rho += Wsel(dr, hb);
+        } else {
+            const T dx = p[0] - in.x[i], dy = p[1] - in.y[i], dz = p[2] - in.z[i];
+            T g[3];
+            if (sym) {
+                T ga[3], gb_[3];
+                kernel_gradW<T>(K, dx, dy, dz, ha, ga);
+                kernel_gradW<T>(K, dx, dy, dz, hb, gb_);
+                for (int d = 0; d < 3; ++d) g[d] = T(0.5) * (ga[d] + gb_[d]);
+            } else {
+                kernel_gradW<T>(K, dx, dy, dz, strategy == GATHER ? ha : hb, g);
+            }
+            const T invrb = T(1) / in.rho[i];
+            for (int d = 0; d < 3; ++d) {
+                const T mg = mb * g[d];
+                f[d] += mg;
+                b[d] += mg * invrb;
+            }
+            rho += mb * Wsel(dr, hb);
+        }
+    });
+    if (kind != 2) {
+        out[0] = rho;
+        return;
+    }
+    if (rho == T(0)) {
+        for (int d = 0; d < 3; ++d) out[d * stride] = std::numeric_limits<T>::quiet_NaN();
+        return;
+    }
+    for (int d = 0; d < 3; ++d) {
+        b[d] *= rho;
+        out[d * stride] = f[d] - b[d];
+    }
+}
+
+// _line_integrated_density_kernel (KI/line_integrated_interpolation/kernels/
+// line_integrated_scalar_interpolation.jl:1-70, _line_integrated_density_accumulation
+// of accumulations/line_integrated_scalar_accumulation.jl:6-16): the column
+// sum m_b W(r, h) along the line, 0 without neighbours (no rho, no Shepard).
+template <class T>
+T line_density(const Input<T>& in, const LineLUT<T>& lut, int strategy, const T o[3], const T m[3], T ha) {
+    const int K = in.kernel;
+    const T Kvalid = kernel_support<T>(K);
+    const bool sym = strategy == SYMMETRIC;
+    T sigma = T(0);
+    line_traversal<T>(in.bvh, o, m, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
+        const size_t i = (size_t)k - 1;
+        const T dr = std::sqrt(d2 < T(0) ? T(0) : d2);
+        T W;
+        if (sym) W = T(0.5) * (lut.W(K, dr, ha) + lut.W(K, dr, hb));
+        else W = lut.W(K, dr, strategy == GATHER ? ha : hb);
+        sigma += in.m[i] * W;
+    });
+    return sigma;
+}
+
 // Adjoint of line_interpolate for one catalog scalar, scatter strategy (no
 // reference counterpart; pinned by central finite differences of
 // line_interpolate).  With w_b = m_b I(q_b) / (h_b rho_b), q_b = r_b / h_b

@@ -38,6 +38,8 @@ def lib():
         L.oracle_get_columns.argtypes = [vp, vp]
         L.oracle_point_samples.argtypes = [vp, i32, vp, vp, i32, vp, i32, vp, i32, vp, i32, i64, vp, vp, vp, vp, i32]
         L.oracle_line_samples.argtypes = [vp, i32, vp, vp, i32, i64] + [vp] * 7 + [vp, i32]
+        L.oracle_point_single.argtypes = [vp, i32, i32, i64, vp, vp, vp, vp, vp, i32]
+        L.oracle_line_density.argtypes = [vp, i32, i64] + [vp] * 6 + [vp, vp, i32]
         L.oracle_nearest_h.argtypes = [vp, i64, vp, vp, vp, vp, vp]
         L.oracle_line_adjoint.argtypes = [vp, i32, i32, i64] + [vp] * 6 + [vp, vp]
         L.oracle_point_adjoint.argtypes = [vp, i32, i32, i32, i64] + [vp] * 3 + [vp, vp]
@@ -166,6 +168,28 @@ class OracleSet:
         hv = _f64(ha) if ha is not None else None
         out = np.zeros((len(ss), S))
         lib().oracle_line_samples(self.h, len(ss), _p(ss), _p(sf), strategy, S, *[_p(a) for a in o + m],
+                                  _p(hv) if hv is not None else None, _p(out), nthreads)
+        return out
+
+    def point_single(self, kind, strategy, px, py, pz, ha=None, nthreads=0):
+        """Single-quantity kernels: kind 0 density, 1 number density (1 x S), 2
+        density gradient (3 x S); ha None: each point's nearest-particle h."""
+        px, py, pz = _f64(px), _f64(py), _f64(pz)
+        S = len(px)
+        hv = _f64(ha) if ha is not None else None
+        out = np.zeros((3 if kind == 2 else 1, S))
+        lib().oracle_point_single(self.h, int(kind), int(strategy), S, _p(px), _p(py), _p(pz),
+                                  _p(hv) if hv is not None else None, _p(out), nthreads)
+        return out
+
+    def line_density(self, strategy, origin, direction, ha=None, nthreads=0):
+        """_line_integrated_density_kernel per line (S values)."""
+        o = [_f64(a) for a in origin]
+        m = [_f64(a) for a in direction]
+        S = len(o[0])
+        hv = _f64(ha) if ha is not None else None
+        out = np.zeros(S)
+        lib().oracle_line_density(self.h, int(strategy), S, *[_p(a) for a in o + m],
                                   _p(hv) if hv is not None else None, _p(out), nthreads)
         return out
 

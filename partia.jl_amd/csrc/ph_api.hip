@@ -1107,6 +1107,142 @@ int ph_find_nearest_device(ph_context* ctx, ph_lbvh* B, int64_t S, const double*
     });
 }
 
+// ---- single-quantity kernels ----
+
+// Launches on the context's stream between the call events; stats as for the
+// other interpolation calls (no neighbour count).
+extern "C++" template <class F> static void timed_call(ph_context* ctx, i64 S, F&& launch) {
+    PH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    launch();
+    PH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    PH_HIP(hipEventSynchronize(ctx->ev[1]));
+    ctx->stats.interp_ms = ctx->stats.kernel_ms = elapsed(ctx, 0, 1);
+    ctx->stats.samples = S;
+    ctx->stats.accepted = -1;
+    ctx->stats.rechecks = -1;
+}
+
+static void check_single(ph_context* ctx, const ph_lbvh* B, int32_t strategy, int32_t precision, int64_t S) {
+    need(B != nullptr && B->ctx == ctx, PH_ERR_ARGUMENT, "tree does not belong to this context");
+    need(B->dims == 3, PH_ERR_ARGUMENT, "interpolation needs a 3D tree: this one was built from an InterpolationInput{2}");
+    need(strategy >= PH_GATHER && strategy <= PH_SYMMETRIC, PH_ERR_ARGUMENT, "unknown strategy");
+    need(S >= 0, PH_ERR_ARGUMENT, "negative sample count");
+    need(precision == PH_PREC_FP64 || precision == PH_PREC_FP32, PH_ERR_ARGUMENT,
+         "single-quantity kernels evaluate in the tree's float type: PH_PREC_FP64 or PH_PREC_FP32");
+    check_precision(B, precision);
+}
+
+int ph_point_single_device(ph_context* ctx, ph_lbvh* B, int32_t kind, int32_t strategy, int32_t precision, int64_t S,
+                           const double* px, const double* py, const double* pz, const double* ha, void* out) {
+    return guarded(ctx, [&] {
+        check_single(ctx, B, strategy, precision, S);
+        need(kind >= PH_SINGLE_DENSITY && kind <= PH_SINGLE_GRADIENT_DENSITY, PH_ERR_ARGUMENT,
+             "unknown single-quantity kind");
+        need(S == 0 || (px && py && pz && out), PH_ERR_ARGUMENT, "null sample or output pointer");
+        PointList pl{px, py, pz};
+        timed_call(ctx, S, [&] { launch_point_single(B, kind, strategy, pl, ha, S, out, ctx->stream); });
+        return PH_OK;
+    });
+}
+
+int ph_point_single(ph_context* ctx, ph_lbvh* B, int32_t kind, int32_t strategy, int32_t precision, int64_t S,
+                    const double* px, const double* py, const double* pz, const double* ha, double* out_host) {
+    return guarded(ctx, [&] {
+        check_single(ctx, B, strategy, precision, S);
+        need(S == 0 || (px && py && pz && out_host), PH_ERR_ARGUMENT, "null sample or output pointer");
+        const int L = kind == PH_SINGLE_GRADIENT_DENSITY ? 3 : 1;
+        const size_t n = (size_t)(S > 0 ? S : 1);
+        DevBuf dp, dout;
+        dp.alloc(sizeof(double) * 4 * n);
+        dout.alloc(out_elem(precision) * (size_t)L * n);
+        double* d = dp.as<double>();
+        const double* src[4] = {px, py, pz, ha};
+        for (int k = 0; k < 4 && S > 0; ++k)
+            if (src[k]) PH_HIP(hipMemcpy(d + (size_t)k * S, src[k], sizeof(double) * S, hipMemcpyHostToDevice));
+        int rc = ph_point_single_device(ctx, B, kind, strategy, precision, S, d, d + S, d + 2 * S,
+                                        ha ? d + 3 * S : nullptr, dout.p);
+        if (rc != PH_OK) return rc;
+        if (S > 0) download(ctx, dout.p, precision, (int64_t)L * S, out_host);
+        return PH_OK;
+    });
+}
+
+// Line kernels with the caller's h_a: the catalog's quantities (cat) or, with
+// cat null, the line density.
+static int line_single_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                              int64_t S, const double* ox, const double* oy, const double* oz, const double* dx,
+                              const double* dy, const double* dz, const double* ha, void* out) {
+    return guarded(ctx, [&] {
+        check_single(ctx, B, strategy, precision, S);
+        if (cat) {
+            check_catalog(B, cat);
+            need(cat->n_grad == 0 && cat->n_div == 0 && cat->n_curl == 0, PH_ERR_ARGUMENT,
+                 "line-integrated interpolation supports scalar quantities only");
+        }
+        need(S == 0 || (ox && oy && oz && dx && dy && dz && out), PH_ERR_ARGUMENT, "null line or output pointer");
+        need(S == 0 || strategy == PH_SCATTER || ha != nullptr, PH_ERR_ARGUMENT,
+             "itpGather and itpSymmetric take the smoothing length ha of every line");
+        LineList ll{ox, oy, oz, dx, dy, dz};
+        const void* lut = precision == PH_PREC_FP64 ? ctx->lut64.p : ctx->lut32.p;
+        std::vector<CatChunk> chunks;
+        if (cat) split_catalog(cat, chunks);
+        timed_call(ctx, S, [&] {
+            if (!cat) launch_line_single(B, nullptr, strategy, ll, ha, S, out, lut, ctx->stream);
+            for (const CatChunk& c : chunks) launch_line_single(B, &c, strategy, ll, ha, S, out, lut, ctx->stream);
+        });
+        return PH_OK;
+    });
+}
+
+static int line_single_host(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                            int64_t S, const double* ox, const double* oy, const double* oz, const double* dx,
+                            const double* dy, const double* dz, const double* ha, double* out_host) {
+    return guarded(ctx, [&] {
+        need(S >= 0, PH_ERR_ARGUMENT, "negative sample count");
+        need(S == 0 || (ox && oy && oz && dx && dy && dz && out_host), PH_ERR_ARGUMENT, "null line or output pointer");
+        const int L = cat ? catalog_L(cat) : 1;
+        const size_t n = (size_t)(S > 0 ? S : 1);
+        DevBuf dr, dout;
+        dr.alloc(sizeof(double) * 7 * n);
+        dout.alloc(out_elem(precision) * (size_t)(L > 0 ? L : 1) * n);
+        double* d = dr.as<double>();
+        const double* src[7] = {ox, oy, oz, dx, dy, dz, ha};
+        for (int k = 0; k < 7 && S > 0; ++k)
+            if (src[k]) PH_HIP(hipMemcpy(d + (size_t)k * S, src[k], sizeof(double) * S, hipMemcpyHostToDevice));
+        int rc = line_single_device(ctx, B, cat, strategy, precision, S, d, d + S, d + 2 * S, d + 3 * S, d + 4 * S,
+                                    d + 5 * S, ha ? d + 6 * S : nullptr, dout.p);
+        if (rc != PH_OK) return rc;
+        if (S > 0 && L > 0) download(ctx, dout.p, precision, (int64_t)L * S, out_host);
+        return PH_OK;
+    });
+}
+
+int ph_line_samples_ha_device(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                              int64_t S, const double* ox, const double* oy, const double* oz, const double* dx,
+                              const double* dy, const double* dz, const double* ha, void* out) {
+    if (!cat) return fail(ctx, PH_ERR_ARGUMENT, "catalog is null");
+    return line_single_device(ctx, B, cat, strategy, precision, S, ox, oy, oz, dx, dy, dz, ha, out);
+}
+
+int ph_line_samples_ha(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t strategy, int32_t precision,
+                       int64_t S, const double* ox, const double* oy, const double* oz, const double* dx,
+                       const double* dy, const double* dz, const double* ha, double* out_host) {
+    if (!cat) return fail(ctx, PH_ERR_ARGUMENT, "catalog is null");
+    return line_single_host(ctx, B, cat, strategy, precision, S, ox, oy, oz, dx, dy, dz, ha, out_host);
+}
+
+int ph_line_density_device(ph_context* ctx, ph_lbvh* B, int32_t strategy, int32_t precision, int64_t S,
+                           const double* ox, const double* oy, const double* oz, const double* dx, const double* dy,
+                           const double* dz, const double* ha, void* out) {
+    return line_single_device(ctx, B, nullptr, strategy, precision, S, ox, oy, oz, dx, dy, dz, ha, out);
+}
+
+int ph_line_density(ph_context* ctx, ph_lbvh* B, int32_t strategy, int32_t precision, int64_t S, const double* ox,
+                    const double* oy, const double* oz, const double* dx, const double* dy, const double* dz,
+                    const double* ha, double* out_host) {
+    return line_single_host(ctx, B, nullptr, strategy, precision, S, ox, oy, oz, dx, dy, dz, ha, out_host);
+}
+
 int ph_line_table(int32_t kernel, double* inorm) {
     if (kernel < 0 || kernel > 5 || !inorm) return fail(nullptr, PH_ERR_ARGUMENT, "bad arguments");
     std::memcpy(inorm, ph_lut_inorm[kernel], sizeof(double) * PH_LUT_N);

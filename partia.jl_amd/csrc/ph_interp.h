@@ -124,6 +124,15 @@ void launch_lines_adjoint(ph_lbvh* B, i32 qsrc, bool shepard, int precision, con
                           i64 S, const void* cot, const void* lut, double* grad, hipStream_t s,
                           hipEvent_t* ev = nullptr);
 
+// Single-quantity kernels (ph_single.hip), one lane per sample, out of the
+// tree's float type: point kind PH_SINGLE_* (1 or 3 x S; ha null: nearest h);
+// line quantities of chunk c with the caller's ha (L x S), or with c null the
+// line density (1 x S); lut is the tables of the tree's type.
+void launch_point_single(ph_lbvh* B, int kind, int strategy, const PointList& pl, const double* ha, i64 S, void* out,
+                         hipStream_t s);
+void launch_line_single(ph_lbvh* B, const CatChunk* c, int strategy, const LineList& ll, const double* ha, i64 S,
+                        void* out, const void* lut, hipStream_t s);
+
 // LBVH queries (ph_query.hip), float64 trees, device arrays.  launch_query:
 // per point the leaves within radius[s] (count, closest 1-based Morton index or
 // 0, its d2 or +Inf; with pool, their indices from pool[offsets[s]] in DFS

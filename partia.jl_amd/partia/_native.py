@@ -41,9 +41,12 @@ EXPORTS = (
     "ph_structured_grid_coords", "ph_structured_grid_coords_device",
     "ph_point_samples_adjoint_device", "ph_structured_grid_adjoint_device",
     "ph_set_autotune", "ph_set_launch", "ph_autotune_report",
+    "ph_point_single", "ph_point_single_device", "ph_line_samples_ha", "ph_line_samples_ha_device",
+    "ph_line_density", "ph_line_density_device",
 )
 
 PH_REQ_POINTS, PH_REQ_PLANE, PH_REQ_BOX, PH_REQ_POLAR, PH_REQ_CYLINDER = 0, 1, 2, 3, 4
+PH_SINGLE_DENSITY, PH_SINGLE_NUMBER_DENSITY, PH_SINGLE_GRADIENT_DENSITY = 0, 1, 2
 PH_COORDS_CARTESIAN, PH_COORDS_CYLINDRICAL, PH_COORDS_SPHERICAL = 0, 1, 2
 
 
@@ -122,6 +125,12 @@ def lib():
         "ph_point_samples": (i32, [vp, vp, P(ph_catalog), i32, i32, i64, vp, vp, vp, vp]),
         "ph_point_samples_device": (i32, [vp, vp, P(ph_catalog), i32, i32, i64, vp, vp, vp, vp]),
         "ph_point_samples_device_noalloc": (i32, [vp, vp, P(ph_catalog), i32, i32, i64, vp, vp, vp, vp]),
+        "ph_point_single": (i32, [vp, vp, i32, i32, i32, i64, vp, vp, vp, vp, vp]),
+        "ph_point_single_device": (i32, [vp, vp, i32, i32, i32, i64, vp, vp, vp, vp, vp]),
+        "ph_line_samples_ha": (i32, [vp, vp, P(ph_catalog), i32, i32, i64] + [vp] * 8),
+        "ph_line_samples_ha_device": (i32, [vp, vp, P(ph_catalog), i32, i32, i64] + [vp] * 8),
+        "ph_line_density": (i32, [vp, vp, i32, i32, i64] + [vp] * 8),
+        "ph_line_density_device": (i32, [vp, vp, i32, i32, i64] + [vp] * 8),
         "ph_structured_grid": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, i64, vp, i64, vp, i64, vp]),
         "ph_structured_grid_device": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, i64, vp, i64, vp, i64, vp]),
         "ph_line_samples": (i32, [vp, vp, P(ph_catalog), i32, i32, i64] + [vp] * 7),

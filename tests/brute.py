@@ -182,3 +182,56 @@ def line_quantity(inp, o, m, col, K, lineW, shepard=True):
         return np.nan
     num = np.sum(inp["cols"][col][sel] * wt)
     return float(num / S1) if shepard else float(num)
+
+
+def number_density(inp, p, ha, K, strategy):
+    """brute_number_density (test/interpolation_test_common.jl:104-118): sum W."""
+    sel, w, _ = _weights(inp, p, ha, K, strategy, False)
+    return float(np.sum(w))
+
+
+def gradient_density(inp, p, ha, K, strategy):
+    """brute_gradient_density (test/interpolation_test_common.jl:156-186): the
+    unnormalised rho = sum m W weights the mean term."""
+    sel, w, g = _weights(inp, p, ha, K, strategy, True)
+    m, rb = inp["m"][sel], inp["rho"][sel]
+    rho = np.sum(m * w)
+    if rho == 0:
+        return (np.nan,) * 3
+    return tuple(float(np.sum(m * gd) - rho * np.sum(m / rb * gd)) for gd in g)
+
+
+def line_density(inp, o, m, ha, K, lineW, strategy):
+    """brute_line_integrated_density (test/interpolation_test_common.jl:139-154):
+    sum m_b W along the line with the strategy's radius and kernel."""
+    d2 = line_d2(inp, o, m)
+    hb = inp["h"]
+    Kv = SUPPORT[K]
+    rad = Kv * ha * np.ones_like(hb) if strategy == 0 else (Kv * hb if strategy == 1 else Kv * np.maximum(ha, hb))
+    sel = d2 <= rad * rad
+    r = np.sqrt(np.maximum(d2[sel], 0))
+    if strategy == 2:
+        w = np.array([0.5 * (lineW(K, ri, ha) + lineW(K, ri, hi)) for ri, hi in zip(r, hb[sel])])
+    else:
+        w = np.array([lineW(K, ri, ha if strategy == 0 else hi) for ri, hi in zip(r, hb[sel])])
+    return float(np.sum(inp["m"][sel] * w)) if w.size else 0.0
+
+
+def line_quantity_ha(inp, o, m, ha, col, K, lineW, strategy, shepard=True):
+    """brute_line_integrated_quantity for any strategy (caller's ha)."""
+    d2 = line_d2(inp, o, m)
+    hb = inp["h"]
+    Kv = SUPPORT[K]
+    rad = Kv * ha * np.ones_like(hb) if strategy == 0 else (Kv * hb if strategy == 1 else Kv * np.maximum(ha, hb))
+    sel = d2 <= rad * rad
+    r = np.sqrt(np.maximum(d2[sel], 0))
+    if strategy == 2:
+        w = np.array([0.5 * (lineW(K, ri, ha) + lineW(K, ri, hi)) for ri, hi in zip(r, hb[sel])])
+    else:
+        w = np.array([lineW(K, ri, ha if strategy == 0 else hi) for ri, hi in zip(r, hb[sel])])
+    wt = inp["m"][sel] * w / inp["rho"][sel]
+    S1 = np.sum(wt)
+    if S1 == 0:
+        return np.nan
+    num = np.sum(inp["cols"][col][sel] * wt)
+    return float(num / S1) if shepard else float(num)
