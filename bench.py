@@ -446,6 +446,46 @@ def grid64(args):
     print(json.dumps(result))
 
 
+def batch_cpu_baseline(data, mix, host_pts, sizes, target_seconds):
+    """Oracle on every 32nd request, strided samples, FP64 (the reference's
+    arithmetic); whole-batch time extrapolated from the sample rate."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    cols = [data["rho"], data["A"], data["vx"], data["vy"], data["vz"]]
+    t0 = time.perf_counter()
+    o = O.OracleSet(data["x"], data["y"], data["z"], data["m"], data["h"], data["rho"], cols,
+                    kernel="C4_Wendland", code_bits=63)
+    t_build = time.perf_counter() - t0
+    nt = threads()
+    picks = list(range(0, len(mix), 32))
+
+    def run(stride):
+        done = 0
+        for r in picks:
+            P = host_pts[r]
+            k = np.arange(0, sizes[r], stride)
+            o.point_samples(CATALOG, O.SYMMETRIC, P[0][k], P[1][k], P[2][k], nthreads=nt)
+            done += len(k)
+        return done
+
+    stride = 4096
+    while True:
+        t0 = time.perf_counter()
+        done = run(stride)
+        dt = time.perf_counter() - t0
+        if dt * 4 > target_seconds or stride <= 16:
+            break
+        stride //= 4
+    rate = done / dt
+    S_tot = int(sum(sizes))
+    return {"value": S_tot / (t_build + S_tot / rate), "unit": "samples/s", "cores": nt, "kind": "port",
+            "sample": ("C++ oracle (reference algorithm restated, float64) on %d host threads of a %s: LinearBVH! on "
+                       "all %d particles (%.2f s) + %d samples of %d requests (every %d-th sample of every 32nd "
+                       "request, %.2f s); whole-batch time extrapolated" % (nt, cpu_model(), len(data["x"]), t_build,
+                                                                           done, len(picks), stride, dt)),
+            "build_s": t_build, "interp_samples_per_s": rate}
+
+
 def batch512(args):
     """512 uneven requests over a 4e6-particle uniform-plus-clump set in one graph launch."""
     from partia import batch as PB
@@ -504,6 +544,7 @@ def batch512(args):
     span = (t_last - t_first) / 1e3
     # the same requests as separate calls (plane / box coordinates uploaded beforehand)
     sep_ms = None
+    n_acc = 0
     if not args.no_fp64_compare:
         pre = []
         for q, P, S, r in zip(mix, host_pts, sizes, reqs):
@@ -523,6 +564,13 @@ def batch512(args):
         ctx.check(L.ph_synchronize(ctx.handle))
         sep_ms = 1e3 * (time.perf_counter() - t0)
         print("bench: the same requests as %d separate calls: %.1f ms" % (nreq, sep_ms), file=sys.stderr, flush=True)
+        # accepted neighbours of every request (the batch's are the same), untimed
+        ctx.set_count_neighbours(True)
+        for (px, py, pz), S, r in zip(pre, sizes, reqs):
+            ctx.check(L.ph_point_samples_device(ctx.handle, h, cat.ref(), N.PH_SYMMETRIC, N.PH_PREC_FP32C, S,
+                                                C.c_void_p(px), C.c_void_p(py), C.c_void_p(pz), C.c_void_p(r.out)))
+            n_acc += ctx.stats()["accepted"]
+        ctx.set_count_neighbours(False)
     result = {
         "metric": "GPU samples/s over 512 uneven requests in one graph launch; per-request p50/p99 latency",
         "value": S_tot / (ms * 1e-3), "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -539,6 +587,20 @@ def batch512(args):
         "separate_calls_ms": sep_ms,
         "speedup_vs_separate_calls": (sep_ms / ms) if sep_ms else None,
     }
+    if n_acc:
+        # per accepted neighbour the 10 float32 fields of the C2 catalog; per
+        # sample 9 float32 outputs, plus 3 float64 coordinates read for point lists
+        n_pts = int(sum(S for q, S in zip(mix, sizes) if q["kind"] == "points"))
+        alg_bytes = n_acc * FIELDS_PER_NEIGHBOUR * 4 + S_tot * L_OUT * 4 + n_pts * 24
+        batch_ms = result["batch_ms"]
+        achieved = alg_bytes / (batch_ms * 1e-3) / 1e9
+        traffic, traffic_src = measured_traffic("k_batch", "uniform_clump4e6_512_requests")
+        result["accepted_neighbours"] = int(n_acc)
+        result["roofline"] = {"bound": "hbm", "kernel": "k_batch", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                              "traffic_source": traffic_src, "algorithmic_bytes": int(alg_bytes)}
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = batch_cpu_baseline(data, mix, host_pts, sizes, args.cpu_seconds)
     b.close()
     L.ph_lbvh_destroy(h)
     for d in dcols + dev:
@@ -579,11 +641,12 @@ def adjoint1024(args):
                                         dax.p, nimg, dax.p, nimg, dout.p))
         return ctx.stats()["interp_ms"]
 
-    def backward(h, prec, dcot, dgrad):
+    def backward(h, prec, dcot, dgrad, kernel=False):
         ctx.check(L.ph_line_grid_adjoint_device(ctx.handle, h, cat.ref(), N.PH_SCATTER, prec,
                                                 *[N.ptr(f) for f in frame], dax.p, nimg, dax.p, nimg, dcot.p,
                                                 dgrad.p))
-        return ctx.stats()["interp_ms"]
+        st = ctx.stats()
+        return (st["interp_ms"], st["kernel_ms"]) if kernel else st["interp_ms"]
 
     def get(dev, dtype, count):
         out = np.zeros(count, dtype)
@@ -607,12 +670,17 @@ def adjoint1024(args):
 
     def step():
         f = forward(h32, N.PH_PREC_FP32, dmap)
-        b = backward(h32, N.PH_PREC_FP32, dcot, dgrad)
-        return {"forward": f, "backward": b}
+        b, bk = backward(h32, N.PH_PREC_FP32, dcot, dgrad, kernel=True)
+        return {"forward": f, "backward": b, "backward_kernel": bk}
 
+    ctx.set_count_neighbours(True)
+    forward(h32, N.PH_PREC_FP32, dmap)
+    n_acc = ctx.stats()["accepted"]
+    ctx.set_count_neighbours(False)
     ms, res = timed_steps(ctx, step, args.steps, args.warmup, "adjoint1024")
     fwd = float(np.mean([r["forward"] for r in res]))
     bwd = float(np.mean([r["backward"] for r in res]))
+    bwd_kernel = float(np.mean([r["backward_kernel"] for r in res]))
     g32 = get(dgrad, np.float64, 6 * npart).reshape(6, npart)
     loss = 0.5 * float(np.sum(cot.astype(np.float64) ** 2))
 
@@ -674,6 +742,41 @@ def adjoint1024(args):
                      "max_err_over_rms": fd_err},
         "fp32_vs_fp64_adjoint_normwise": rel32,
     }
+    # per accepted (ray, particle) pair the 7 float32 fields read (x, y, z, h, m,
+    # rho, A); per ray the float32 cotangent; per particle 6 float64 gradients
+    alg_bytes = n_acc * 7 * 4 + S * 4 + npart * 6 * 8
+    achieved = alg_bytes / (bwd_kernel * 1e-3) / 1e9
+    result["accepted_neighbours"] = int(n_acc)
+    result["backward_kernel_ms"] = bwd_kernel
+    result["roofline"] = {"bound": "hbm", "kernel": "k_lines_adjoint", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                          "algorithmic_bytes": int(alg_bytes)}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        t0 = time.perf_counter()
+        o = O.OracleSet(data["x"], data["y"], data["z"], data["m"], data["h"], data["rho"], [data["rho"]],
+                        kernel="M4_spline", code_bits=63, precision=32)
+        t_build = time.perf_counter() - t0
+        stride = 1 << 14
+        while True:
+            idx = np.arange(0, S, stride)
+            i, j = idx % nimg, idx // nimg
+            z = np.zeros(len(idx))
+            org = [ax[i].astype(np.float32).astype(np.float64), ax[j].astype(np.float32).astype(np.float64), z]
+            t0 = time.perf_counter()
+            o.line_adjoint(1, False, org, [z, z, z + 1.0], cot[idx].astype(np.float64))
+            dt = time.perf_counter() - t0
+            if dt * 4 > args.cpu_seconds or stride <= 16:
+                break
+            stride //= 4
+        rate = len(idx) / dt
+        result["cpu_baseline"] = {
+            "value": S / (t_build + S / rate), "unit": "rays/s", "cores": 1, "kind": "port",
+            "sample": ("C++ oracle adjoint (serial: the rays share particles) on 1 host thread of a %s: LinearBVH! on "
+                       "all %d particles (%.2f s) + the adjoint of %d of the %d rays (every %d-th, %.2f s); "
+                       "whole-map time extrapolated" % (cpu_model(), npart, t_build, len(idx), S, stride, dt)),
+            "build_s": t_build, "adjoint_rays_per_s": rate}
     L.ph_lbvh_destroy(h32)
     for d in (dax, dmap, dT, dcot, dgrad, dm64, dT64, dcot64):
         d.free()
