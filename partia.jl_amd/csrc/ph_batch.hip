@@ -7,7 +7,12 @@
 // the sample coordinates (explicit list, or the frame-based Cartesian plane /
 // box of Grids/grids/coordinate.jl:44-143 with the reference's arithmetic),
 // computes the nearest-particle smoothing length (gather / symmetric) and runs
-// the same per-sample body as the per-call bucket kernels.  Every sample is
+// the per-sample body of the per-call kernels: frame-based requests (planes,
+// boxes, polar planes, cylinders: each wave holds neighbouring samples) the
+// grid tile kernels' per-wave union walk of the 8-leaf bucket tree or the
+// per-lane bucket walk of the 4-leaf tree, chosen per wave from its samples'
+// spread against their support radii (union_pays); PH_TRAVERSAL_WAVE forces the
+// union walk, PH_TRAVERSAL_BUCKET / LANE the per-lane walk.  Every sample is
 // computed with exactly the per-call arithmetic, so each request's outputs equal
 // those of a ph_point_samples_device call on the same coordinates, bit for bit.
 // Device timestamps (s_memrealtime) give each request's first-unit start and
@@ -16,16 +21,34 @@
 #include <cmath>
 #include <memory>
 
-#include "ph_bucket_dev.h"
+#include "ph_tile_dev.h"
 
 namespace ph {
 
 // Device descriptor of one request.
+// The walk of a request's waves: one bucket walk per lane (4-leaf tree), one
+// union walk per wave (8-leaf tree, ph_tile_dev.h), or per wave whichever
+// union_pays picks.
+constexpr i32 kWalkLane = 0, kWalkUnion = 1, kWalkChoose = 2;
+
+// The union walk pays while the box around a wave's samples, grown by the
+// mean support diameter 2R, holds at most kUnionRatio times the volume of one
+// sample's support cube (2R)^3: every lane tests every leaf the union holds,
+// while the per-lane walk tests each lane's own leaves but waits on one chain
+// of dependent loads per lane.  16 was the fastest of 0.5, 2, 4, 8, 16, 32, 64,
+// 128 and 1000 on 512 mixed requests over 4e6 particles (uniform + Plummer
+// clump; MI355X): 247 ms against 282 ms with the union walk for every frame
+// request and 342 ms with per-lane walks only.
+#ifndef PH_BATCH_UNION_RATIO
+#define PH_BATCH_UNION_RATIO 16.0f
+#endif
+constexpr float kUnionRatio = PH_BATCH_UNION_RATIO;
+
 struct BatchReq {
     i64 S;      // samples
     i64 unit0;  // first work unit
     i32 kind;
-    i32 pad;
+    i32 walk;   // kWalkLane, kWalkUnion or kWalkChoose
     const double* px;
     const double* py;
     const double* pz;
@@ -44,6 +67,7 @@ struct BatchState {
     std::vector<CatChunk> chunks;  // as launched (FP32C: compacted)
     std::vector<int> ncols;        // FP32C: distinct columns per chunk
     int rw_max = 2;
+    int wave_doubles = 0;  // LDS per wave of the persistent kernel, in doubles
     DevBuf reqs, rec, ctl, times;  // ctl: per-chunk counters + launch timestamp
     DevBuf perm;                   // walk orders of the point-list requests
     DevBuf trig;                   // angle tables of the polar / cylindrical requests
@@ -192,13 +216,43 @@ __device__ inline i64 uniform_i64(i64 v) {
 // workgroup just finished and takes the next unit.  The unit is then read back
 // into scalar registers (wave-uniform), so the loop exit is a uniform branch:
 // every wave leaves in the same iteration and executes the same barriers.
+// Wave-uniform: whether the union walk pays for this wave (kUnionRatio).  live
+// lanes hold a sample with neighbours (support radius r = K h_a); the sums and
+// the decision are taken in float, broadcast from lane 0.
+__device__ inline bool union_pays(bool live, const double p[3], double r) {
+    float lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = live ? (float)p[a] : __builtin_inff();
+        hi[a] = live ? (float)p[a] : -__builtin_inff();
+    }
+    float rs = live ? (float)r : 0.f, cnt = live ? 1.f : 0.f;
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], __shfl_xor(lo[a], d, kWave));
+            hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], d, kWave));
+        }
+        rs += __shfl_xor(rs, d, kWave);
+        cnt += __shfl_xor(cnt, d, kWave);
+    }
+    const float dm = 2.f * rs / fmaxf(cnt, 1.f);
+    const float v = (hi[0] - lo[0] + dm) * (hi[1] - lo[1] + dm) * (hi[2] - lo[2] + dm);
+    const bool pays = cnt > 0.f && v <= kUnionRatio * dm * dm * dm;
+    return __builtin_amdgcn_readfirstlane(pays ? 1 : 0) != 0;
+}
+
 template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT, bool C32>
-__global__ void __launch_bounds__(kBlock) k_batch(BucketView bv, const double* __restrict__ rec, CatChunk c,
-                                                  const BatchReq* __restrict__ reqs, i32 nreq, i64 units,
+__global__ void __launch_bounds__(kBlock) k_batch(BucketView bv, const BucketRec* __restrict__ recs, BucketView bv8,
+                                                  const BucketRec* __restrict__ recs8, const double* __restrict__ rec,
+                                                  CatChunk c, const BatchReq* __restrict__ reqs, i32 nreq, i64 units,
                                                   unsigned long long* __restrict__ ctr,
-                                                  unsigned long long* __restrict__ times) {
+                                                  unsigned long long* __restrict__ times, int wave_doubles) {
+    extern __shared__ double batch_smem[];
     __shared__ i64 s_unit;
     __shared__ unsigned long long s_t0;
+    const int lane = threadIdx.x & (kWave - 1);
+    // this wave's LDS: the nearest-h candidates, then the union walk's staging
+    // (or the per-lane bucket lists)
+    double* wl = batch_smem + (threadIdx.x / kWave) * wave_doubles;
     i32 done_req = -1;              // thread 0: request of the unit just finished
     unsigned long long done_t0 = 0;  // thread 0: its start time
     for (;;) {
@@ -225,69 +279,101 @@ __global__ void __launch_bounds__(kBlock) k_batch(BucketView bv, const double* _
         }
         lo = __builtin_amdgcn_readfirstlane(lo);
         const BatchReq& R = reqs[lo];
+        const i32 mode = __builtin_amdgcn_readfirstlane(R.walk);  // wave-uniform
         const i64 sv = batch_sample(R, u - R.unit0, threadIdx.x);
         const bool valid = sv >= 0;
         const i64 s = valid ? sv : 0;
         double p[3] = {0.0, 0.0, 0.0};
         if (valid) batch_point(R, s, p);
+        const double Kd = support_of<K>();
         double ha = 0.0;
+        bool walk = mode != kWalkLane;  // union walk (scatter: no h_a to choose by)
         if constexpr (STRAT != PH_SCATTER) {
-            if (valid) ha = nearest_h_bucket(bv, p, neighbour_bound2(bv, support_of<K>()));
+            if (mode == kWalkUnion) {
+                i64 bk = 0;
+                ha = nearest_tile_h(bv, recs, valid, p, Kd, reinterpret_cast<LeafRec<double>*>(wl), bk);
+                wave_sync();  // the candidates are read before the region is reused
+            } else if (valid) {
+                ha = nearest_h_bucket(bv, p, neighbour_bound2(bv, Kd));
+            }
+            if (!valid) ha = 0.0;
+            if (mode == kWalkChoose) walk = union_pays(valid && ha == ha, p, Kd * ha);
         }
         u64 accepted = 0;
-        if constexpr (C32)
-            bucket_c32_sample<K, STRAT, MS, MG, MD, MC, EXACT>(bv, rec, c, valid, p, ha, static_cast<float*>(R.out),
-                                                               R.S, s, accepted);
-        else
-            bucket_f64_sample<K, STRAT, MS, MG, MD, MC>(bv, rec, c, valid, p, ha, static_cast<double*>(R.out), R.S, s,
-                                                        accepted);
+        if (walk) {
+            if constexpr (C32)
+                tile_c32_sample<K, STRAT, MS, MG, MD, MC, EXACT>(bv8, recs8, rec, c, valid, p, ha,
+                                                                 static_cast<float*>(R.out), R.S, s, wl, accepted);
+            else
+                tile_f64_sample<K, STRAT, MS, MG, MD, MC>(bv8, recs8, rec, c, valid, p, ha,
+                                                          static_cast<double*>(R.out), R.S, s, wl, accepted);
+        } else {
+            u32* list = reinterpret_cast<u32*>(wl) + lane;
+            if constexpr (C32)
+                bucket_c32_sample<K, STRAT, MS, MG, MD, MC, EXACT>(bv, rec, c, valid, p, ha,
+                                                                   static_cast<float*>(R.out), R.S, s, accepted, list,
+                                                                   kWave);
+            else
+                bucket_f64_sample<K, STRAT, MS, MG, MD, MC>(bv, rec, c, valid, p, ha, static_cast<double*>(R.out),
+                                                            R.S, s, accepted, list, kWave);
+        }
         __syncthreads();  // the unit is complete
         done_req = lo;
         done_t0 = t0;
     }
 }
 
+// LDS per wave of the persistent kernel for chunk c, in doubles: the union
+// walk's staging, the per-lane bucket lists (kQ entries per lane) and the
+// nearest-h candidates (64 leaves of 4 doubles) share it.
+inline int batch_wave_doubles(const CatChunk& c, bool c32) {
+    const int slot = c32 ? 4 + (c2_shared(c) && c.rw == kC2Rec ? kC2Rec : c.rw) : 5 + c.rw;
+    return std::max(std::max(tile_wave_doubles(slot), kQ * kWave / 2), kNnCand * 4);
+}
+
 template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT, bool C32>
-void launch_batch_t(BatchState& st, const BucketView& bv, const CatChunk& c, unsigned long long* ctr, int num_cu,
-                    hipStream_t s) {
+void launch_batch_t(BatchState& st, const BucketView& bv, const BucketView& bv8, const CatChunk& c,
+                    unsigned long long* ctr, int num_cu, hipStream_t s) {
     auto kern = k_batch<K, STRAT, MS, MG, MD, MC, EXACT, C32>;
+    const size_t dyn = (size_t)(kBlock / kWave) * st.wave_doubles * sizeof(double);
     int per_cu = 0;
-    PH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
+    PH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, dyn));
     const i64 want = (i64)(per_cu > 0 ? per_cu : 1) * num_cu;
     const unsigned g = (unsigned)(want < st.units ? want : (st.units > 0 ? st.units : 1));
-    kern<<<g, kBlock, 0, s>>>(bv, st.rec.as<double>(), c, st.reqs.as<BatchReq>(), st.nreq, st.units, ctr,
-                              st.times.as<unsigned long long>());
+    kern<<<g, kBlock, dyn, s>>>(bv, bv.recs, bv8, bv8.recs, st.rec.as<double>(), c, st.reqs.as<BatchReq>(), st.nreq,
+                                st.units, ctr, st.times.as<unsigned long long>(), st.wave_doubles);
     PH_HIP(hipGetLastError());
 }
 
 template <int K, int STRAT>
-void launch_batch_ks(BatchState& st, const BucketView& bv, const CatChunk& c, unsigned long long* ctr, int num_cu,
-                     hipStream_t s) {
+void launch_batch_ks(BatchState& st, const BucketView& bv, const BucketView& bv8, const CatChunk& c,
+                     unsigned long long* ctr, int num_cu, hipStream_t s) {
     const bool small = c.ng + c.nd + c.nc == 0 && c.ns <= 2;
     if (st.precision == PH_PREC_FP64) {
-        if (small) launch_batch_t<K, STRAT, 2, 0, 0, 0, 0, false>(st, bv, c, ctr, num_cu, s);
-        else launch_batch_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0, false>(st, bv, c, ctr, num_cu, s);
-    } else {  // the per-call selection (ph_bucket.hip), so the arithmetic is the same
-        if (small) launch_batch_t<K, STRAT, 2, 0, 0, 0, 0, true>(st, bv, c, ctr, num_cu, s);
-        else if (c2_shared(c)) launch_batch_t<K, STRAT, 2, 1, 1, 1, 2, true>(st, bv, c, ctr, num_cu, s);
-        else launch_batch_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0, true>(st, bv, c, ctr, num_cu, s);
+        if (small) launch_batch_t<K, STRAT, 2, 0, 0, 0, 0, false>(st, bv, bv8, c, ctr, num_cu, s);
+        else launch_batch_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0, false>(st, bv, bv8, c, ctr, num_cu, s);
+    } else {  // the per-call selection (ph_bucket.hip, ph_tile.hip), so the arithmetic is the same
+        if (small) launch_batch_t<K, STRAT, 2, 0, 0, 0, 0, true>(st, bv, bv8, c, ctr, num_cu, s);
+        else if (c2_shared(c) && c.rw == kC2Rec)
+            launch_batch_t<K, STRAT, 2, 1, 1, 1, 2, true>(st, bv, bv8, c, ctr, num_cu, s);
+        else launch_batch_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0, true>(st, bv, bv8, c, ctr, num_cu, s);
     }
 }
 
 template <int K>
-void launch_batch_k(BatchState& st, const BucketView& bv, const CatChunk& c, unsigned long long* ctr, int num_cu,
-                    hipStream_t s) {
+void launch_batch_k(BatchState& st, const BucketView& bv, const BucketView& bv8, const CatChunk& c,
+                    unsigned long long* ctr, int num_cu, hipStream_t s) {
     switch (st.strategy) {
-    case PH_GATHER: launch_batch_ks<K, PH_GATHER>(st, bv, c, ctr, num_cu, s); break;
-    case PH_SCATTER: launch_batch_ks<K, PH_SCATTER>(st, bv, c, ctr, num_cu, s); break;
-    default: launch_batch_ks<K, PH_SYMMETRIC>(st, bv, c, ctr, num_cu, s); break;
+    case PH_GATHER: launch_batch_ks<K, PH_GATHER>(st, bv, bv8, c, ctr, num_cu, s); break;
+    case PH_SCATTER: launch_batch_ks<K, PH_SCATTER>(st, bv, bv8, c, ctr, num_cu, s); break;
+    default: launch_batch_ks<K, PH_SYMMETRIC>(st, bv, bv8, c, ctr, num_cu, s); break;
     }
 }
 
 // Records the launch sequence of one run on stream s.  ctl[0] = launch
 // timestamp, ctl[1 + q] = work counter of catalog chunk q.
 void record(BatchState& st, ph_lbvh* B, int num_cu, hipStream_t s) {
-    const BucketView bv = bucket_view(B);
+    const BucketView bv = bucket_view(B), bv8 = bucket_view(B, kBucket);
     unsigned long long* ctl = st.ctl.as<unsigned long long>();
     unsigned long long* times = st.times.as<unsigned long long>();
     const i64 n = B->n;
@@ -304,12 +390,12 @@ void record(BatchState& st, ph_lbvh* B, int num_cu, hipStream_t s) {
         PH_HIP(hipGetLastError());
         if (st.units == 0) continue;
         switch (B->kernel) {
-        case PH_KERNEL_M4: launch_batch_k<PH_KERNEL_M4>(st, bv, c, ctr, num_cu, s); break;
-        case PH_KERNEL_M5: launch_batch_k<PH_KERNEL_M5>(st, bv, c, ctr, num_cu, s); break;
-        case PH_KERNEL_M6: launch_batch_k<PH_KERNEL_M6>(st, bv, c, ctr, num_cu, s); break;
-        case PH_KERNEL_C2: launch_batch_k<PH_KERNEL_C2>(st, bv, c, ctr, num_cu, s); break;
-        case PH_KERNEL_C4: launch_batch_k<PH_KERNEL_C4>(st, bv, c, ctr, num_cu, s); break;
-        default: launch_batch_k<PH_KERNEL_C6>(st, bv, c, ctr, num_cu, s); break;
+        case PH_KERNEL_M4: launch_batch_k<PH_KERNEL_M4>(st, bv, bv8, c, ctr, num_cu, s); break;
+        case PH_KERNEL_M5: launch_batch_k<PH_KERNEL_M5>(st, bv, bv8, c, ctr, num_cu, s); break;
+        case PH_KERNEL_M6: launch_batch_k<PH_KERNEL_M6>(st, bv, bv8, c, ctr, num_cu, s); break;
+        case PH_KERNEL_C2: launch_batch_k<PH_KERNEL_C2>(st, bv, bv8, c, ctr, num_cu, s); break;
+        case PH_KERNEL_C4: launch_batch_k<PH_KERNEL_C4>(st, bv, bv8, c, ctr, num_cu, s); break;
+        default: launch_batch_k<PH_KERNEL_C6>(st, bv, bv8, c, ctr, num_cu, s); break;
         }
     }
 }
@@ -481,7 +567,14 @@ BatchState* batch_build(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int 
     for (size_t q = 0; q < st->chunks.size(); ++q) {
         if (precision != PH_PREC_FP64) st->chunks[q] = compact_c32(st->chunks[q], st->ncols[q]);
         st->rw_max = std::max(st->rw_max, st->chunks[q].rw);
+        st->wave_doubles = std::max(st->wave_doubles, batch_wave_doubles(st->chunks[q], precision != PH_PREC_FP64));
     }
+    // the walk of every request's waves: chosen per wave unless the context
+    // asks for the union walk (PH_TRAVERSAL_WAVE) or per-lane walks
+    const i32 walk = ctx->traversal == PH_TRAVERSAL_WAVE ? kWalkUnion
+                     : ctx->traversal == PH_TRAVERSAL_BUCKET || ctx->traversal == PH_TRAVERSAL_LANE ? kWalkLane
+                                                                                                 : kWalkChoose;
+    for (BatchReq& d : st->host) d.walk = walk;
     const i64 n = B->n;
     if (precision != PH_PREC_FP64 && (i64)st->rw_max * 8 * n >= (i64(1) << 32))
         throw Error{PH_ERR_CAPACITY, "FP32C records exceed 4 GiB; split the catalog or use PH_PREC_FP64"};

@@ -125,7 +125,9 @@ __global__ void __launch_bounds__(kBlock) k_bucket_c32(BucketView bv, const doub
         if (valid) ha = ha_s[s];
     }
     u64 accepted = 0;
-    bucket_c32_sample<K, STRAT, MS, MG, MD, MC, EXACT>(bv, rec, c, valid, p, ha, out, smp.S, s, accepted);
+    __shared__ u32 blist[kQ][kBlock];
+    bucket_c32_sample<K, STRAT, MS, MG, MD, MC, EXACT>(bv, rec, c, valid, p, ha, out, smp.S, s, accepted,
+                                                       &blist[0][threadIdx.x], kBlock);
     count_add(counter, accepted);
 }
 
@@ -141,7 +143,9 @@ __global__ void __launch_bounds__(kBlock) k_bucket_f64(BucketView bv, const doub
         if (valid) ha = ha_s[s];
     }
     u64 accepted = 0;
-    bucket_f64_sample<K, STRAT, MS, MG, MD, MC>(bv, rec, c, valid, p, ha, out, smp.S, s, accepted);
+    __shared__ u32 blist[kQ][kBlock];
+    bucket_f64_sample<K, STRAT, MS, MG, MD, MC>(bv, rec, c, valid, p, ha, out, smp.S, s, accepted,
+                                                &blist[0][threadIdx.x], kBlock);
     count_add(counter, accepted);
 }
 

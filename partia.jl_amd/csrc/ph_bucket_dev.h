@@ -449,13 +449,14 @@ template <int STRAT> struct Probe {
     }
 };
 
-// Per-lane list of up to kQ buckets in LDS (stride kBlock): the bucket code
-// -link - 1 = first leaf << 5 | (count - 1).
+// Per-lane list of up to kQ buckets in LDS (entry q at slot[q * stride]): the
+// bucket code -link - 1 = first leaf << 5 | (count - 1).
 struct BucketList {
     u32* slot;
     int n;
-    __device__ __attribute__((always_inline)) void push(i32 link) { slot[n++ * kBlock] = (u32)(-link - 1); }
-    __device__ __attribute__((always_inline)) u32 code(int q) const { return slot[q * kBlock]; }
+    int stride = kBlock;
+    __device__ __attribute__((always_inline)) void push(i32 link) { slot[n++ * stride] = (u32)(-link - 1); }
+    __device__ __attribute__((always_inline)) u32 code(int q) const { return slot[q * stride]; }
     __device__ __attribute__((always_inline)) u32 lo(int q) const { return code(q) >> 5; }
 };
 
@@ -600,11 +601,12 @@ __device__ inline double neighbour_bound2(const BucketView& bv, double Kd) {
 
 // FP32-compensated interpolation of sample s at p (ha: smoothing length of its
 // nearest particle, gather / symmetric).  out: L x S float32.
+// list: this lane's kQ bucket-list entries in LDS, entry q at list[q * stride].
 template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT>
 __device__ __attribute__((always_inline)) void bucket_c32_sample(const BucketView& bv, const double* __restrict__ rec,
                                                                  const CatChunk& c, bool valid, const double p[3],
                                                                  double ha, float* __restrict__ out, i64 S, i64 s,
-                                                                 u64& accepted) {
+                                                                 u64& accepted, u32* list, int stride) {
     const double Kd = support_of<K>();
     // no particle within K max(h): no neighbours (NaN outputs), no walk
     const bool none = STRAT != PH_SCATTER && !(ha == ha);
@@ -621,8 +623,6 @@ __device__ __attribute__((always_inline)) void bucket_c32_sample(const BucketVie
 
     // per-lane bucket list in LDS: read back with a per-lane index in the
     // accumulation loop, which thus has no divergent branch
-    __shared__ u32 blist[kQ][kBlock];
-    const int tid = threadIdx.x;
     const u32 nlast = (u32)(bv.n - 1);
     const Buf rb = Buf::of(rec, ((i64)bv.n + 1) * rw * 8);  // with the zero record n
     const Buf lb = Buf::of(bv.leaves, (i64)bv.n * (i64)sizeof(LeafRec<double>));
@@ -631,7 +631,7 @@ __device__ __attribute__((always_inline)) void bucket_c32_sample(const BucketVie
     acc.init();
     i32 i = valid && !none ? 0 : bv.nrec;
     i64 steps = 2 * (i64)bv.nrec + 8;  // a valid tree is walked in at most nrec steps
-    BucketList bl{&blist[0][tid], 0};
+    BucketList bl{list, 0, stride};
     while (true) {
         collect<STRAT>(bv, pr, i, steps, bl);
         u64 mask = test_buckets<STRAT>(bv, bl, p, Kd, r2a);
@@ -672,7 +672,7 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC>
 __device__ __attribute__((always_inline)) void bucket_f64_sample(const BucketView& bv, const double* __restrict__ rec,
                                                                  const CatChunk& c, bool valid, const double p[3],
                                                                  double ha, double* __restrict__ out, i64 S, i64 s,
-                                                                 u64& accepted) {
+                                                                 u64& accepted, u32* list, int stride) {
     const double Kd = support_of<K>();
     const bool none = STRAT != PH_SCATTER && !(ha == ha);  // as in bucket_c32_sample
     if (!valid || none) ha = 0.0;
@@ -683,8 +683,7 @@ __device__ __attribute__((always_inline)) void bucket_f64_sample(const BucketVie
     acc.init();
     i32 i = valid && !none ? 0 : bv.nrec;
     i64 steps = 2 * (i64)bv.nrec + 8;
-    __shared__ u32 blist[kQ][kBlock];
-    BucketList bl{&blist[0][threadIdx.x], 0};
+    BucketList bl{list, 0, stride};
     while (true) {
         collect<STRAT>(bv, pr, i, steps, bl);
         u64 mask = test_buckets<STRAT>(bv, bl, p, Kd, r2a);

@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from gpu_util import DevArray, DeviceSet, assert_same
+from gpu_util import DevArray, DeviceSet, assert_same, traversal
 from partia import ArgumentError
 from partia import _native as N
 from partia import batch as PB
@@ -98,6 +98,34 @@ def test_batch_matches_single_calls(backend, precision, strategy):
                 ref = d.points(cat, strategy, *P, precision=precision)
                 assert got.shape == ref.shape
                 assert_same(got, ref)
+        finally:
+            b.close()
+            for x in bufs + [o for o, _ in outs]:
+                x.free()
+
+
+@pytest.mark.parametrize("precision", [N.PH_PREC_FP64, N.PH_PREC_FP32C])
+@pytest.mark.parametrize("mode", [N.PH_TRAVERSAL_AUTO, N.PH_TRAVERSAL_WAVE, N.PH_TRAVERSAL_BUCKET,
+                                  N.PH_TRAVERSAL_LANE])
+def test_batch_walks_under_every_traversal(backend, precision, mode):
+    """By default each wave takes the per-wave union walk of the 8-leaf bucket
+    tree or the per-lane bucket walk, by its samples' spread; PH_TRAVERSAL_WAVE
+    puts every wave on the union walk (with the per-wave nearest-h search),
+    BUCKET and LANE on the per-lane walk.  Every choice gives the per-call
+    results bit for bit (gather and symmetric)."""
+    ctx = backend.context
+    c6, cols = cloud(12000, 31)
+    d = DeviceSet(ctx, *c6, cols, kernel=1)
+    rng = np.random.default_rng(41 + mode + 7 * precision)
+    for cat, strategy in ((FULL, N.PH_GATHER), (WIDE, O_SYM), (SMALL, O_SYM)):
+        ca = N.CatalogArgs(**cat)
+        reqs, coords, bufs, outs = make_requests(ctx, rng, ca.L, precision, ["points", "plane", "box", "points"] * 2)
+        with traversal(ctx, mode):
+            b = PB.Batch(ctx, d.h, ca, strategy, precision, reqs)
+        try:
+            b.run()
+            for P, (o, S) in zip(coords, outs):
+                assert_same(read_out(o, S, ca.L, precision), d.points(cat, strategy, *P, precision=precision))
         finally:
             b.close()
             for x in bufs + [o for o, _ in outs]:
