@@ -7,7 +7,7 @@ namespace ph {
 
 // Scratch buffers of one build (freed with the object).
 struct BuildScratch {
-    DevBuf ptrs, mm, keys, dk, adj, visited, tmp, stage, stage32, flag;
+    DevBuf ptrs, mm, keys, dk, adj, visited, tmp, stage, stage32, flag, aos;
 };
 
 // Builds the tree of B (n, ncol, kernel, code_bits, precision set by the caller)

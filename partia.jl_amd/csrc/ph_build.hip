@@ -14,6 +14,8 @@
 //      count arrivals in LDS; the few nodes spanning chunks are finished by a
 //      second pass with agent-scope acq_rel counters; then a visit-count check.
 // Tree arrays keep the reference's unified node IDs so they compare with memcmp.
+#include <algorithm>
+
 #include "ph_build.h"
 #include "ph_primitives.h"
 
@@ -110,14 +112,45 @@ __global__ void __launch_bounds__(kBlock) k_encode2(const T* __restrict__ x, con
     idx[i] = (u32)i;
 }
 
-// Gather every column into Morton order: dst[c][i] = src[c][idx[i]].
-template <class TS, class TD>
-__global__ void __launch_bounds__(kBlock) k_permute(const TS* const* __restrict__ src, TD* __restrict__ dst, int ncols, i64 n,
-                                            const u32* __restrict__ idx) {
+// Gather every column into Morton order, dst[c][i] = src[c][idx[i]], in two
+// passes: the columns are first interleaved into rows of rs >= ncols values
+// (rs even: 16-byte aligned rows), in input order, through an LDS tile of
+// `rows` rows (coalesced reads of each column, coalesced writes of the rows);
+// then each sorted position reads its particle's row (one or two cache lines
+// instead of one line per column) and writes the columns coalesced.
+inline int interleave_stride(int ncols) { return (ncols + 1) & ~1; }
+inline int interleave_rows(int rs) { return std::min(kBlock, 4096 / rs > 0 ? 4096 / rs : 1); }
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_interleave(const T* const* __restrict__ src, T* __restrict__ aos, int ncols,
+                                                       int rs, int rows, i64 n) {
+    extern __shared__ double interleave_lds[];
+    T* tile = reinterpret_cast<T*>(interleave_lds);
+    const i64 i0 = (i64)blockIdx.x * rows;
+    const int m = (int)(n - i0 < rows ? n - i0 : rows);
+    for (int c = 0; c < rs; ++c)
+        for (int r = threadIdx.x; r < m; r += kBlock) tile[r * rs + c] = c < ncols ? src[c][i0 + r] : T(0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < m * rs; e += kBlock) aos[i0 * rs + e] = tile[e];
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_permute(const T* __restrict__ aos, T* __restrict__ dst, int ncols, int rs,
+                                                    i64 n, const u32* __restrict__ idx) {
     const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const u32 s = idx[i];
-    for (int c = 0; c < ncols; ++c) dst[(i64)c * n + i] = (TD)src[c][s];
+    const T* row = aos + (i64)idx[i] * rs;
+    if constexpr (sizeof(T) == 8) {  // rows of doubles are 16-byte aligned
+        int c = 0;
+        for (; c + 1 < ncols; c += 2) {
+            const double2 v = *reinterpret_cast<const double2*>(row + c);
+            dst[(i64)c * n + i] = v.x;
+            dst[(i64)(c + 1) * n + i] = v.y;
+        }
+        if (c < ncols) dst[(i64)c * n + i] = row[c];
+    } else {
+        for (int c = 0; c < ncols; ++c) dst[(i64)c * n + i] = row[c];
+    }
 }
 
 // Dedup as a prefix max: key_i = c_i + 2^31 - i (never negative for i < 2^31).
@@ -392,6 +425,8 @@ void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipS
     W.adj.alloc(sizeof(i32) * n);
     W.visited.alloc(sizeof(u32) * (n - 1) + sizeof(u32) * 64 + sizeof(i32) * n);
     W.tmp.alloc(align(16 * (size_t)n) + (size_t(1) << 20));  // radix sort, scans, min/max
+    const int rs = interleave_stride(ncols), rows = interleave_rows(rs);
+    W.aos.alloc(sizeof(T) * (size_t)rs * n);  // the interleaved columns
     B->order.alloc(sizeof(u32) * n);
     B->cols.alloc(sizeof(T) * (size_t)ncols * n);
     B->codes.alloc(sizeof(u64) * n);
@@ -413,10 +448,13 @@ void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipS
         k_encode<T, TI><<<g, kBlock, 0, s>>>(src_cols[0], src_cols[1], src_cols[2], n, W.mm.as<T>(),
                                              W.keys.as<TI>(), B->order.as<u32>());
     PH_HIP(hipGetLastError());
+    k_interleave<T><<<(unsigned)((n + rows - 1) / rows), kBlock, sizeof(T) * (size_t)rows * rs, s>>>(
+        W.ptrs.as<const T*>(), W.aos.as<T>(), ncols, rs, rows, n);
+    PH_HIP(hipGetLastError());
     // 3: stable sort by code
     radix_sort_pairs(W.keys.as<TI>(), B->order.as<u32>(), n, CodeTraits<TI>::sort_bits, W.tmp, s);
     // 4: permute the columns
-    k_permute<T, T><<<g, kBlock, 0, s>>>(W.ptrs.as<const T*>(), B->cols.as<T>(), ncols, n, B->order.as<u32>());
+    k_permute<T><<<g, kBlock, 0, s>>>(W.aos.as<T>(), B->cols.as<T>(), ncols, rs, n, B->order.as<u32>());
     PH_HIP(hipGetLastError());
     // 5: duplicate rewrite
     u32* first = W.flag.as<u32>();

@@ -165,14 +165,30 @@ __global__ void __launch_bounds__(kBlock) k_minmax3(const T* x, const T* y, cons
     }
 }
 
-template <class T> __global__ void k_minmax_final(const T* partial, int nparts, T* out6) {
+// One block: every thread folds a strided share of the partials, then the
+// wave and block reductions of k_minmax3.
+template <class T> __global__ void __launch_bounds__(kBlock) k_minmax_final(const T* partial, int nparts, T* out6) {
+    __shared__ T lds[6][kBlock / kWave];
+    const T inf = T(INFINITY);
+    T a[6] = {inf, inf, inf, -inf, -inf, -inf};
+    for (int i = threadIdx.x; i < nparts; i += kBlock)
+        for (int d = 0; d < 3; ++d) {
+            a[d] = fmin(a[d], partial[(i64)i * 6 + d]);
+            a[3 + d] = fmax(a[3 + d], partial[(i64)i * 6 + 3 + d]);
+        }
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    for (int d = 0; d < 3; ++d) {
+        T mn = wave_min(a[d]), mx = wave_max(a[3 + d]);
+        if (lane == 0) {
+            lds[d][w] = mn;
+            lds[3 + d][w] = mx;
+        }
+    }
+    __syncthreads();
     if (threadIdx.x < 6) {
         const int d = threadIdx.x;
-        T r = partial[d];
-        for (int i = 1; i < nparts; ++i) {
-            T v = partial[(i64)i * 6 + d];
-            r = d < 3 ? fmin(r, v) : fmax(r, v);
-        }
+        T r = lds[d][0];
+        for (int i = 1; i < kBlock / kWave; ++i) r = d < 3 ? fmin(r, lds[d][i]) : fmax(r, lds[d][i]);
         out6[d] = r;
     }
 }
@@ -194,25 +210,37 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, i64 n, int
     counts[(i64)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+// Scatter of one tile of kSortTile keys.  Each key's rank among the tile's
+// keys of its digit is found chunk by chunk (256 keys; stable: chunk, wave and
+// lane order); the tile is then sorted by digit in LDS and written out from
+// LDS, so consecutive threads write consecutive addresses of one digit's run.
+// The positions equal those of a direct stable scatter.
 template <class K>
-__global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* __restrict__ keys, const u32* __restrict__ vals, K* __restrict__ okeys,
-                                                  u32* __restrict__ ovals, i64 n, int shift, const u32* __restrict__ offsets,
-                                                  int nblocks) {
+__global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* __restrict__ keys, const u32* __restrict__ vals,
+                                                          K* __restrict__ okeys, u32* __restrict__ ovals, i64 n,
+                                                          int shift, const u32* __restrict__ offsets, int nblocks) {
     __shared__ u32 wave_cnt[kBlock / kWave][kRadix];
+    __shared__ u32 tcnt[kRadix];   // keys of each digit in the chunks so far, then the tile's digit starts
     __shared__ u32 base_of[kRadix];
-    __shared__ u32 chunk_tot[kRadix];
+    __shared__ u32 wsum[kBlock / kWave];
+    __shared__ K skey[kSortTile];
+    __shared__ u32 sval[kSortTile];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
     base_of[tid] = offsets[(i64)tid * nblocks + blockIdx.x];
+    tcnt[tid] = 0;
     for (int i = 0; i < kBlock / kWave; ++i) wave_cnt[i][tid] = 0;
     __syncthreads();
     const u64 lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (kWave - lane));
     const i64 tile = (i64)blockIdx.x * kSortTile;
+    K key[kSortChunks];
+    u32 val[kSortChunks], rnk[kSortChunks];
+#pragma unroll
     for (int c = 0; c < kSortChunks; ++c) {
         const i64 i = tile + (i64)c * kBlock + tid;
         const bool active = i < n;
-        K key = active ? keys[i] : K(0);
-        u32 val = active ? vals[i] : 0u;
-        const u32 digit = (u32)(key >> shift) & (kRadix - 1);
+        key[c] = active ? keys[i] : K(0);
+        val[c] = active ? vals[i] : 0u;
+        const u32 digit = (u32)(key[c] >> shift) & (kRadix - 1);
         u64 match = __ballot(active);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -223,25 +251,57 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* __restrict__ 
         const u32 rank = (u32)__popcll(match & lt_mask);
         if (active && rank == 0) wave_cnt[w][digit] = (u32)__popcll(match);
         __syncthreads();
-        {
-            u32 run = 0;
+        {  // per digit: earlier waves' counts, on top of the earlier chunks'
+            u32 run = tcnt[tid];
             for (int q = 0; q < kBlock / kWave; ++q) {
-                u32 cq = wave_cnt[q][tid];
+                const u32 cq = wave_cnt[q][tid];
                 wave_cnt[q][tid] = run;
                 run += cq;
             }
-            chunk_tot[tid] = run;
+            tcnt[tid] = run;
         }
         __syncthreads();
-        if (active) {
-            const u32 pos = base_of[digit] + wave_cnt[w][digit] + rank;
-            okeys[pos] = key;
-            ovals[pos] = val;
-        }
+        rnk[c] = active ? wave_cnt[w][digit] + rank : ~0u;
         __syncthreads();
-        base_of[tid] += chunk_tot[tid];
         for (int q = 0; q < kBlock / kWave; ++q) wave_cnt[q][tid] = 0;
         __syncthreads();
+    }
+    // the tile's digit starts: exclusive scan of the digit totals
+    const u32 tot = tcnt[tid];
+    u32 incl = tot;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const u32 u = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += u;
+    }
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    u32 pre = 0;
+    for (int q = 0; q < w; ++q) pre += wsum[q];
+    tcnt[tid] = pre + incl - tot;
+    __syncthreads();
+    // the tile sorted by digit, in LDS
+#pragma unroll
+    for (int c = 0; c < kSortChunks; ++c) {
+        if (rnk[c] != ~0u) {
+            const u32 digit = (u32)(key[c] >> shift) & (kRadix - 1);
+            const u32 at = tcnt[digit] + rnk[c];
+            skey[at] = key[c];
+            sval[at] = val[c];
+        }
+    }
+    __syncthreads();
+    const i64 m = n - tile < kSortTile ? n - tile : kSortTile;
+#pragma unroll
+    for (int c = 0; c < kSortChunks; ++c) {
+        const int e = c * kBlock + tid;
+        if (e < m) {
+            const K k = skey[e];
+            const u32 digit = (u32)(k >> shift) & (kRadix - 1);
+            const u32 pos = base_of[digit] + (u32)e - tcnt[digit];
+            okeys[pos] = k;
+            ovals[pos] = sval[e];
+        }
     }
 }
 
@@ -296,7 +356,7 @@ template <class T> void minmax3(const T* x, const T* y, const T* z, i64 n, T* ou
     tmp.alloc(sizeof(T) * 6 * (size_t)g);
     k_minmax3<T><<<g, kBlock, 0, s>>>(x, y, z, n, tmp.as<T>());
     PH_HIP(hipGetLastError());
-    k_minmax_final<T><<<1, 64, 0, s>>>(tmp.as<T>(), g, out6);
+    k_minmax_final<T><<<1, kBlock, 0, s>>>(tmp.as<T>(), g, out6);
     PH_HIP(hipGetLastError());
 }
 template void minmax3<double>(const double*, const double*, const double*, i64, double*, DevBuf&, hipStream_t);
