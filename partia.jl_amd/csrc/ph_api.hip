@@ -465,10 +465,13 @@ static int build_common(ph_context* ctx, int64_t n, const double* const* src, bo
         B->precision = precision;
         BuildScratch W;
         PH_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-        build_lbvh(B.get(), src, host_src, ctx->stream, W);
         // the bucket trees of the point / grid kernels, built with the tree so
-        // that interpolation calls allocate only per-call buffers
-        if (buckets_supported(B.get())) ensure_buckets(B.get(), ctx->stream);
+        // that interpolation calls allocate only per-call buffers; their
+        // read-back of the record counts also reads the build's refit check
+        const bool buckets = buckets_supported(B.get());
+        u32* refit_bad = nullptr;
+        build_lbvh(B.get(), src, host_src, ctx->stream, W, buckets ? &refit_bad : nullptr);
+        if (buckets) ensure_buckets(B.get(), ctx->stream, refit_bad);
         PH_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
         PH_HIP(hipEventSynchronize(ctx->ev[3]));
         ctx->stats.build_ms = elapsed(ctx, 2, 3);

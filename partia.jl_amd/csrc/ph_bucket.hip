@@ -14,19 +14,36 @@ namespace {
 // Per internal node i: big_s[split - 1] = (leaves of i > kBucket); for a big node,
 // bs[first leaf - 1] = 1 for each child that is a bucket (at most kBucket leaves).
 __global__ void __launch_bounds__(kBlock) k_bucket_flags(i64 n, const i32* __restrict__ left,
-                                                 const i32* __restrict__ range, u32* __restrict__ big_s,
-                                                 u32* __restrict__ bs, int bmax) {
+                                                 const i32* __restrict__ range, u32* __restrict__ big_s0,
+                                                 u32* __restrict__ bs0, u32* __restrict__ big_s1,
+                                                 u32* __restrict__ bs1) {
     const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
     if (i > n - 1) return;
     const i64 lo = range[2 * (i - 1)], hi = range[2 * (i - 1) + 1];
     const i64 l = left[i - 1];
     const i64 s = l >= n ? lo : l;  // split: the left child covers [lo, s]
-    const bool big = hi - lo + 1 > bmax;
-    big_s[s - 1] = big ? 1u : 0u;
-    if (big) {
-        if (s - lo + 1 <= bmax) bs[lo - 1] = 1u;
-        if (hi - s <= bmax) bs[s] = 1u;
+    u32* const big_s[2] = {big_s0, big_s1};
+    u32* const bs[2] = {bs0, bs1};
+    const int bmax[2] = {kBucketSmall, kBucket};
+    for (int t = 0; t < 2; ++t) {
+        const bool big = hi - lo + 1 > bmax[t];
+        big_s[t][s - 1] = big ? 1u : 0u;
+        if (big) {
+            if (s - lo + 1 <= bmax[t]) bs[t][lo - 1] = 1u;
+            if (hi - s <= bmax[t]) bs[t][s] = 1u;
+        }
     }
+}
+
+// The scanned flag array's values at the five segment starts (info[0..4]) and
+// the build's refit check count (info[5], 0 without one).
+__global__ void k_bucket_info(const u32* __restrict__ scanned, i64 n, const u32* __restrict__ refit_bad,
+                              u32* __restrict__ info) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const i64 per_tree = 2 * n + 1;
+    const i64 at[5] = {0, n, per_tree, per_tree + n, 2 * per_tree};
+    for (int k = 0; k < 5; ++k) info[k] = scanned ? scanned[at[k]] : 0u;
+    info[5] = refit_bad ? *refit_bad : 0u;
 }
 
 __device__ inline void round_box(const double lo[3], const double hi[3], double scale, BucketRec& R) {
@@ -56,40 +73,56 @@ __device__ inline void node_box(i64 u, i64 n, const NodeRec<double>* __restrict_
 // prefix counts of big splits / bucket starts: the nodes before u in preorder are
 // its ancestors (all big) and the big nodes and buckets entirely left of lo; big
 // nodes with split < lo are the latter plus the ancestors holding u on the right.
+// One bucket tree's prefix arrays (segments of the scanned flag array: their
+// values minus the value at the segment start), records and bucket size.
+struct EmitTree {
+    const u32* PB;
+    const u32* PS;
+    u32 pb0, ps0;
+    BucketRec* recs;
+    int bmax;
+};
+
+// Both trees in one pass: L(u) is a property of the LBVH, found once.
 __global__ void __launch_bounds__(kBlock) k_bucket_emit(i64 n, i32 root, const i32* __restrict__ left,
                                                 const i32* __restrict__ parent, const i32* __restrict__ range,
                                                 const NodeRec<double>* __restrict__ nodes,
-                                                const LeafRec<double>* __restrict__ leaves,
-                                                const u32* __restrict__ PB, const u32* __restrict__ PS,
-                                                BucketRec* __restrict__ recs, int bmax) {
+                                                const LeafRec<double>* __restrict__ leaves, EmitTree t0, EmitTree t1,
+                                                int ntrees) {
     const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
     if (i > n - 1) return;
     const i64 lo = range[2 * (i - 1)], hi = range[2 * (i - 1) + 1];
-    if (hi - lo + 1 <= bmax) return;
+    if (hi - lo + 1 <= t0.bmax) return;  // a bucket (or inside one) in both trees
     i64 Lc = 0;
     for (i64 c = i, steps = n; c != root && steps > 0; --steps) {
         const i64 p = parent[c - 1];
         if (left[p - 1] == c) ++Lc;
         c = p;
     }
-    const i64 pre = Lc + PB[lo - 1] + PS[lo - 1];
-    const i64 cnt = (i64)(PB[hi - 1] - PB[lo - 1]) + (i64)(PS[hi] - PS[lo - 1]);
-    BucketRec R;
-    node_box(i, n, nodes, leaves, R);
-    R.link = (i32)(pre + cnt);
-    recs[pre] = R;
     const i64 l = left[i - 1];
     const i64 s = l >= n ? lo : l;
-    if (s - lo + 1 <= bmax) {  // left child is a bucket, right after its parent
-        node_box(s == lo ? (lo + n - 1) : s, n, nodes, leaves, R);
-        R.link = bucket_link(lo - 1, s - lo + 1);
-        recs[pre + 1] = R;
-    }
-    if (hi - s <= bmax) {
-        const i64 rpre = Lc + PB[s] + PS[s];
-        node_box(s + 1 == hi ? (hi + n - 1) : s + 1, n, nodes, leaves, R);
-        R.link = bucket_link(s, hi - s);
-        recs[rpre] = R;
+    for (int t = 0; t < ntrees; ++t) {
+        const EmitTree& T = t == 0 ? t0 : t1;
+        if (hi - lo + 1 <= T.bmax) continue;
+        auto PB = [&](i64 k) { return (i64)(T.PB[k] - T.pb0); };
+        auto PS = [&](i64 k) { return (i64)(T.PS[k] - T.ps0); };
+        const i64 pre = Lc + PB(lo - 1) + PS(lo - 1);
+        const i64 cnt = (PB(hi - 1) - PB(lo - 1)) + (PS(hi) - PS(lo - 1));
+        BucketRec R;
+        node_box(i, n, nodes, leaves, R);
+        R.link = (i32)(pre + cnt);
+        T.recs[pre] = R;
+        if (s - lo + 1 <= T.bmax) {  // left child is a bucket, right after its parent
+            node_box(s == lo ? (lo + n - 1) : s, n, nodes, leaves, R);
+            R.link = bucket_link(lo - 1, s - lo + 1);
+            T.recs[pre + 1] = R;
+        }
+        if (hi - s <= T.bmax) {
+            const i64 rpre = Lc + PB(s) + PS(s);
+            node_box(s + 1 == hi ? (hi + n - 1) : s + 1, n, nodes, leaves, R);
+            R.link = bucket_link(s, hi - s);
+            T.recs[rpre] = R;
+        }
     }
 }
 
@@ -311,58 +344,69 @@ bool buckets_supported(const ph_lbvh* B) {
     return B->dims == 3 && B->precision == PH_PREC_FP64 && B->n <= kMaxBucketLeaves;
 }
 
-void ensure_buckets(ph_lbvh* B, hipStream_t s) {
-    // Two bucket trees per LBVH, built together (one flag pass, one wait for
-    // the record counts): buckets of at most kBucket leaves (the grid tile
-    // kernels) and of at most kBucketSmall (the per-lane kernels, the batch
-    // queue, adjoints and queries).
+void ensure_buckets(ph_lbvh* B, hipStream_t s, const u32* refit_bad) {
+    // Two bucket trees per LBVH, built together: buckets of at most kBucket
+    // leaves (the grid tile kernels) and of at most kBucketSmall (the per-lane
+    // kernels, the batch queue, adjoints and queries).  One flag pass, one scan,
+    // one read-back of the record counts (with the build's refit check when the
+    // build left it here), one emit pass.
     if (B->buckets_ready && B->buckets8_ready) return;
     const i64 n = B->n;
     if (n > kMaxBucketLeaves) throw Error{PH_ERR_CAPACITY, "the bucket traversal supports at most 2^26 particles"};
     DevBuf* recs[2] = {&B->brecs, &B->brecs8};
     const int bmax[2] = {kBucketSmall, kBucket};
     i64 nrec[2] = {1, 1};
-    // buffers first: flags of both trees (big splits: n entries, n - 1 flags +
-    // the total; bucket starts: n + 1 entries, n flags + the total)
-    const bool split[2] = {n > bmax[0], n > bmax[1]};
-    DevBuf flags, tmp;
-    const size_t per_tree = (size_t)(2 * n + 1);
-    if (split[0] || split[1]) flags.alloc(sizeof(u32) * 2 * per_tree);
-    u32* big_s[2] = {flags.as<u32>(), flags.as<u32>() + per_tree};
-    u32* bs[2] = {big_s[0] + n, big_s[1] + n};
-    u32 tot[2][2] = {{0, 0}, {0, 0}};
-    if (split[0] || split[1]) {
-        PH_HIP(hipMemsetAsync(flags.p, 0, sizeof(u32) * 2 * per_tree, s));
-        for (int t = 0; t < 2; ++t) {
-            if (!split[t]) continue;
-            k_bucket_flags<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->left.as<i32>(), B->range.as<i32>(), big_s[t],
-                                                             bs[t], bmax[t]);
+    const bool split[2] = {n > bmax[0], n > bmax[1]};  // split[1] implies split[0]
+    // buffers first.  Flags of both trees in one array, scanned as one: per
+    // tree the big splits (n entries: n - 1 flags + the total) and the bucket
+    // starts (n + 1 entries: n flags + the total), then one closing entry; a
+    // segment's exclusive prefix is its scanned value minus that at its start.
+    const i64 per_tree = 2 * n + 1, m = 2 * per_tree + 1;
+    DevBuf flags, tmp, info;
+    if (split[0]) flags.alloc(sizeof(u32) * (size_t)m);
+    if (split[0] || refit_bad) info.alloc(sizeof(u32) * 8);
+    u32* f = flags.as<u32>();
+    u32* big_s[2] = {f, f + per_tree};
+    u32* bs[2] = {f + n, f + per_tree + n};
+    u32 h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (split[0] || refit_bad) {
+        if (split[0]) {
+            PH_HIP(hipMemsetAsync(f, 0, sizeof(u32) * (size_t)m, s));
+            k_bucket_flags<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->left.as<i32>(), B->range.as<i32>(), big_s[0],
+                                                             bs[0], big_s[1], bs[1]);
             PH_HIP(hipGetLastError());
-            exclusive_sum_u32(big_s[t], n, tmp, s);
-            exclusive_sum_u32(bs[t], n + 1, tmp, s);
-            PH_HIP(hipMemcpyAsync(&tot[t][0], big_s[t] + (n - 1), sizeof(u32), hipMemcpyDeviceToHost, s));
-            PH_HIP(hipMemcpyAsync(&tot[t][1], bs[t] + n, sizeof(u32), hipMemcpyDeviceToHost, s));
+            exclusive_sum_u32(f, m, tmp, s);
         }
+        k_bucket_info<<<1, 1, 0, s>>>(split[0] ? f : nullptr, n, refit_bad, info.as<u32>());
+        PH_HIP(hipGetLastError());
+        PH_HIP(hipMemcpyAsync(h, info.p, sizeof(h), hipMemcpyDeviceToHost, s));
         PH_HIP(hipStreamSynchronize(s));
+        if (h[5] != 0)
+            throw Error{PH_ERR_INTERNAL,
+                        "LinearBVH refit: " + std::to_string(h[5]) + " internal nodes were not reached twice"};
     }
     for (int t = 0; t < 2; ++t) {
-        if (split[t]) nrec[t] = (i64)tot[t][0] + (i64)tot[t][1];
+        if (split[t]) nrec[t] = (i64)(h[2 * t + 1] - h[2 * t]) + (i64)(h[2 * t + 2] - h[2 * t + 1]);
         recs[t]->alloc(sizeof(BucketRec) * (size_t)nrec[t]);
     }
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < 2; ++t)
         if (!split[t]) {  // the whole tree is one bucket
             k_bucket_root<<<1, 1, 0, s>>>(n, B->root, B->nodes.as<NodeRec<double>>(),
                                           B->leaves.as<LeafRec<double>>(), recs[t]->as<BucketRec>());
-        } else {
-            k_bucket_emit<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->root, B->left.as<i32>(), B->parent.as<i32>(),
-                                                            B->range.as<i32>(), B->nodes.as<NodeRec<double>>(),
-                                                            B->leaves.as<LeafRec<double>>(), big_s[t], bs[t],
-                                                            recs[t]->as<BucketRec>(), bmax[t]);
+            PH_HIP(hipGetLastError());
         }
+    if (split[0]) {
+        EmitTree et[2];
+        for (int t = 0; t < 2; ++t)
+            et[t] = EmitTree{big_s[t], bs[t], h[2 * t], h[2 * t + 1], recs[t]->as<BucketRec>(), bmax[t]};
+        k_bucket_emit<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->root, B->left.as<i32>(), B->parent.as<i32>(),
+                                                        B->range.as<i32>(), B->nodes.as<NodeRec<double>>(),
+                                                        B->leaves.as<LeafRec<double>>(), et[0], et[1],
+                                                        split[1] ? 2 : 1);
         PH_HIP(hipGetLastError());
     }
-    // flags / tmp are released on return: into the context's cache, reusable
-    // once the stream has drained, or freed directly (wait for the kernels)
+    // flags / tmp / info are released on return: into the context's cache,
+    // reusable once the stream has drained, or freed directly (wait for the kernels)
     if (!tl_cache) PH_HIP(hipStreamSynchronize(s));
     B->nrec = nrec[0];
     B->nrec8 = nrec[1];

@@ -408,7 +408,8 @@ __global__ void k_d2f(const double* __restrict__ s, float* __restrict__ d, i64 m
 }
 
 template <class T, class TI>
-void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipStream_t s, BuildScratch& W) {
+void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipStream_t s, BuildScratch& W,
+                 u32** deferred_bad) {
     const i64 n = B->n;
     const int ncols = 6 + B->ncol;
     const int g = grid_for(n);
@@ -506,6 +507,10 @@ void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipS
         PH_HIP(hipGetLastError());
         k_check_visited<<<grid_for(n - 1), kBlock, 0, s>>>(visited, n - 1, bad);
         PH_HIP(hipGetLastError());
+        if (deferred_bad) {  // the caller reads the count back with its own results
+            *deferred_bad = bad;
+            return;
+        }
         u32 hbad = 0;
         PH_HIP(hipMemcpyAsync(&hbad, bad, sizeof(u32), hipMemcpyDeviceToHost, s));
         PH_HIP(hipStreamSynchronize(s));
@@ -517,7 +522,8 @@ void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipS
 
 }  // namespace
 
-void build_lbvh(ph_lbvh* B, const double* const* src64, bool host_src, hipStream_t s, BuildScratch& W) {
+void build_lbvh(ph_lbvh* B, const double* const* src64, bool host_src, hipStream_t s, BuildScratch& W,
+                u32** deferred_bad) {
     const i64 n = B->n;
     const int ncols = 6 + B->ncol;
     std::vector<const double*> dptr(ncols);
@@ -549,11 +555,11 @@ void build_lbvh(ph_lbvh* B, const double* const* src64, bool host_src, hipStream
             PH_HIP(hipGetLastError());
             fptr[c] = d;
         }
-        if (B->code_bits == 30) build_typed<float, u32>(B, fptr.data(), s, W);
-        else build_typed<float, u64>(B, fptr.data(), s, W);
+        if (B->code_bits == 30) build_typed<float, u32>(B, fptr.data(), s, W, deferred_bad);
+        else build_typed<float, u64>(B, fptr.data(), s, W, deferred_bad);
     } else {
-        if (B->code_bits == 30) build_typed<double, u32>(B, dptr.data(), s, W);
-        else build_typed<double, u64>(B, dptr.data(), s, W);
+        if (B->code_bits == 30) build_typed<double, u32>(B, dptr.data(), s, W, deferred_bad);
+        else build_typed<double, u64>(B, dptr.data(), s, W, deferred_bad);
     }
 }
 

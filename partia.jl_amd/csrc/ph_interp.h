@@ -86,7 +86,9 @@ void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, c
 // 4 leaves: per-lane kernels, batches, adjoints, queries; of at most 8: the
 // grid tile kernels).  ph_lbvh_build calls it for every tree that
 // buckets_supported accepts (3D, float64, at most 2^26 particles).
-void ensure_buckets(ph_lbvh* B, hipStream_t s);
+// refit_bad: the build's refit check count on the device, read back with the
+// record counts (0 expected), or null.
+void ensure_buckets(ph_lbvh* B, hipStream_t s, const u32* refit_bad = nullptr);
 bool buckets_supported(const ph_lbvh* B);
 // Grid interpolation with one bucket-tree walk per wave (ph_tile.hip), FP32C
 // (float outputs) or float64 (precision PH_PREC_FP64, double outputs); the
