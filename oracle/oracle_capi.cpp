@@ -18,6 +18,7 @@
 #include <string>
 
 #include "partia_oracle.hpp"
+#include "oracle_dual.hpp"
 #include "../partia.jl_amd/csrc/line_lut_tables.inc"
 
 using namespace oracle;
@@ -55,6 +56,41 @@ template <class T> void fill_input(Input<T>& in, int64_t n, const double* x, con
     in.y = cp(y); in.z = cp(z); in.m = cp(m); in.h = cp(h); in.rho = cp(rho);
     in.quant.clear();
     for (int c = 0; c < ncol; ++c) in.quant.push_back(cp(cols[c]));
+}
+
+// The float64 input with a derivative direction: tan holds (6 + ncol) rows of n
+// derivatives in input order (x, y, z, m, h, rho, the quantity columns).  The
+// tree keeps its topology; its leaves carry the coordinate and h derivatives,
+// its node boxes none (they only steer the walk).
+Input<Dual> dual_input(const Input<double>& in, const double* tan) {
+    const int64_t n = (int64_t)in.x.size();
+    auto col = [&](const std::vector<double>& v, int row) {
+        std::vector<Dual> w((size_t)n);
+        for (int64_t i = 0; i < n; ++i) w[(size_t)i] = Dual{v[(size_t)i], tan[(int64_t)row * n + (int64_t)in.order[(size_t)i] - 1]};
+        return w;
+    };
+    auto val = [](const std::vector<double>& v) { return std::vector<Dual>(v.begin(), v.end()); };
+    Input<Dual> o;
+    o.kernel = in.kernel;
+    o.dims = in.dims;
+    o.order = in.order;
+    o.codes = in.codes;
+    o.x = col(in.x, 0);
+    o.y = col(in.y, 1);
+    o.z = col(in.z, 2);
+    o.m = col(in.m, 3);
+    o.h = col(in.h, 4);
+    o.rho = col(in.rho, 5);
+    for (size_t c = 0; c < in.quant.size(); ++c) o.quant.push_back(col(in.quant[c], 6 + (int)c));
+    o.bvh.brt = in.bvh.brt;
+    for (int d = 0; d < 3; ++d) {
+        o.bvh.leaf[d] = col(in.bvh.leaf[d], d);
+        o.bvh.nmin[d] = val(in.bvh.nmin[d]);
+        o.bvh.nmax[d] = val(in.bvh.nmax[d]);
+    }
+    o.bvh.leaf_scale = col(in.bvh.leaf_scale, 4);
+    o.bvh.nscale = val(in.bvh.nscale);
+    return o;
 }
 
 template <class T> const LBVH<T>& bvh_of(const Handle* h);
@@ -346,6 +382,69 @@ int oracle_field_adjoint(void* hv, int mode, const int* slots, int shepard, int 
 }
 
 double oracle_kernel_d2f(int K, double q) { return kernel_d2f<double>(K, q); }
+
+// Forward mode through the float64 point path (nearest-particle h_a included):
+// out (L x S) the outputs, dout (L x S) their derivatives along tan ((6 + ncol)
+// x n, input order: x, y, z, m, h, rho, quantities).  Float64 handles only.
+int oracle_point_jvp(void* hv, int nS, const int* sslots, const unsigned char* sflags, int nG, const int* gslots,
+                     int nD, const int* dslots, int nC, const int* cslots, int strategy, int64_t S, const double* px,
+                     const double* py, const double* pz, const double* tan, double* out, double* dout, int nthreads) {
+    Handle* H = static_cast<Handle*>(hv);
+    if (H->precision != 64) {
+        g_err = "oracle_point_jvp needs a float64 handle";
+        return 1;
+    }
+    Catalog cat = make_catalog(nS, sslots, sflags, nG, gslots, nD, dslots, nC, cslots);
+    const int L = cat.L();
+    const Input<Dual> in = dual_input(H->d, tan);
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+    {
+        std::vector<Dual> tmp((size_t)L);
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t s = 0; s < S; ++s) {
+            Dual p[3] = {Dual(px[s]), Dual(py[s]), Dual(pz[s])};
+            Dual ha = strategy == SCATTER ? Dual(0.0) : nearest_h<Dual>(in.bvh, p);
+            general_interpolate<Dual>(in, cat, strategy, p, ha, tmp.data(), 1);
+            for (int k = 0; k < L; ++k) {
+                out[(int64_t)k * S + s] = tmp[(size_t)k].v;
+                dout[(int64_t)k * S + s] = tmp[(size_t)k].d;
+            }
+        }
+    }
+    return 0;
+}
+
+// Forward mode through the float64 line path (the line traversal and the
+// line-integrated table, whose cells are linear in q): as oracle_line_samples,
+// with dout the derivatives along tan.
+int oracle_line_jvp(void* hv, int nS, const int* sslots, const unsigned char* sflags, int strategy, int64_t S,
+                    const double* ox, const double* oy, const double* oz, const double* mx, const double* my,
+                    const double* mz, const double* ha, const double* tan, double* out, double* dout, int nthreads) {
+    Handle* H = static_cast<Handle*>(hv);
+    if (H->precision != 64) {
+        g_err = "oracle_line_jvp needs a float64 handle";
+        return 1;
+    }
+    Catalog cat = make_catalog(nS, sslots, sflags, 0, nullptr, 0, nullptr, 0, nullptr);
+    const Input<Dual> in = dual_input(H->d, tan);
+    const LineLUT<Dual> lut(ph_lut_inorm);
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+    {
+        std::vector<Dual> tmp((size_t)(nS > 0 ? nS : 1));
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t s = 0; s < S; ++s) {
+            Dual o[3] = {Dual(ox[s]), Dual(oy[s]), Dual(oz[s])};
+            Dual m[3] = {Dual(mx[s]), Dual(my[s]), Dual(mz[s])};
+            Dual h = ha ? Dual(ha[s]) : Dual(0.0);
+            line_interpolate<Dual>(in, lut, cat, strategy, o, m, h, tmp.data(), 1);
+            for (int k = 0; k < nS; ++k) {
+                out[(int64_t)k * S + s] = tmp[(size_t)k].v;
+                dout[(int64_t)k * S + s] = tmp[(size_t)k].d;
+            }
+        }
+    }
+    return 0;
+}
 
 // query_leaves per point: counts, closest, closest d2; with pool (sized by the
 // caller from the counts) the accepted indices point after point.

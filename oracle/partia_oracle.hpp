@@ -23,6 +23,10 @@
 //     table; the two agree to <= 1.6e-10 of I(0).
 // Everything else is evaluated in the same order as the reference so that, for a
 // given float type, results are reproducible to the last bit on this CPU.
+// Like the reference (generic over T <: AbstractFloat, kernel.jl:141), the
+// interpolation templates also run on a forward-mode dual number type
+// (oracle_dual.hpp): comparisons use the value part, so the traversal and the
+// accepted neighbours are those of the float64 run.
 #pragma once
 
 #include <algorithm>
@@ -34,6 +38,11 @@
 #include <vector>
 
 namespace oracle {
+
+// Square roots are called unqualified so that a scalar type of the oracle's own
+// (the forward-mode Dual of oracle_dual.hpp) brings its sqrt by argument-
+// dependent lookup; for float and double this is std::sqrt.
+using std::sqrt;
 
 using i32 = int32_t;
 using i64 = int64_t;
@@ -206,7 +215,7 @@ template <class T> inline void kernel_gradW(int K, T dx, T dy, T dz, T h, T g[3]
     T r2 = dx * dx + dy * dy + dz * dz;
     T gx = T(0), gy = T(0), gz = T(0);
     if (!(r2 == T(0))) {
-        T r = std::sqrt(r2);
+        T r = sqrt(r2);
         T invr = T(1) / r;
         T q = r / h;
         T coeff = kernel_df<T>(K, q) * kernel_sigma3<T>(K) * invr;
@@ -608,7 +617,7 @@ template <class T> inline T d2_line_aabb_lb(const T o[3], const T m[3], const LB
     }
     T dc2 = d2_point_line(c, o, m);
     if (dc2 <= r2) return T(0);
-    T dl = std::sqrt(dc2) - std::sqrt(r2);
+    T dl = sqrt(dc2) - sqrt(r2);
     return dl * dl;
 }
 
@@ -816,7 +825,7 @@ void general_interpolate(const Input<T>& in, const Catalog& cat, int strategy, c
     point_traversal<T>(in.bvh, p, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
         const size_t i = (size_t)k - 1;
         T dx = p[0] - in.x[i], dy = p[1] - in.y[i], dz = p[2] - in.z[i];
-        T dr = std::sqrt(d2);
+        T dr = sqrt(d2);
         T mb = in.m[i], rb = in.rho[i];
         T W = Wsel(dr, hb);
         S1 += mb * W / rb;
@@ -913,7 +922,7 @@ void line_interpolate(const Input<T>& in, const LineLUT<T>& lut, const Catalog& 
     const bool sym = strategy == SYMMETRIC;
     line_traversal<T>(in.bvh, o, m, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
         const size_t i = (size_t)k - 1;
-        T dr = std::sqrt(d2 < T(0) ? T(0) : d2);
+        T dr = sqrt(d2 < T(0) ? T(0) : d2);
         T mb = in.m[i], rb = in.rho[i];
         T W;
         if (sym) W = T(0.5) * (lut.W(K, dr, ha) + lut.W(K, dr, hb));
@@ -957,7 +966,7 @@ void single_interpolate(const Input<T>& in, int kind, int strategy, const T p[3]
     };
     point_traversal<T>(in.bvh, p, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
         const size_t i = (size_t)k - 1;
-        const T dr = std::sqrt(d2);
+        const T dr = sqrt(d2);
         const T mb = in.m[i];
         if (kind == 0) {
             rho += mb * Wsel(dr, hb);
@@ -1009,7 +1018,7 @@ T line_density(const Input<T>& in, const LineLUT<T>& lut, int strategy, const T 
     T sigma = T(0);
     line_traversal<T>(in.bvh, o, m, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
         const size_t i = (size_t)k - 1;
-        const T dr = std::sqrt(d2 < T(0) ? T(0) : d2);
+        const T dr = sqrt(d2 < T(0) ? T(0) : d2);
         T W;
         if (sym) W = T(0.5) * (lut.W(K, dr, ha) + lut.W(K, dr, hb));
         else W = lut.W(K, dr, strategy == GATHER ? ha : hb);
@@ -1039,7 +1048,7 @@ void line_adjoint(const Input<T>& in, const LineLUT<T>& lut, int slot, bool shep
     if (shepard) {
         line_traversal<T>(in.bvh, o, m, SCATTER, Kvalid, T(0), [&](i64 k, T d2, T hb) {
             const size_t i = (size_t)k - 1;
-            const T dr = std::sqrt(d2 < T(0) ? T(0) : d2);
+            const T dr = sqrt(d2 < T(0) ? T(0) : d2);
             const T w = in.m[i] * lut.W(K, dr, hb) / in.rho[i];
             S1 += w;
             N += A[i] * w;
@@ -1049,7 +1058,7 @@ void line_adjoint(const Input<T>& in, const LineLUT<T>& lut, int slot, bool shep
     const T out = shepard ? N / S1 : T(0);
     line_traversal<T>(in.bvh, o, m, SCATTER, Kvalid, T(0), [&](i64 k, T d2, T hb) {
         const size_t i = (size_t)k - 1;
-        const T dr = std::sqrt(d2 < T(0) ? T(0) : d2);
+        const T dr = sqrt(d2 < T(0) ? T(0) : d2);
         const T invh = T(1) / hb, q = dr * invh;
         T I = T(0), dI = T(0);
         if (!(q >= Kvalid)) {
@@ -1122,7 +1131,7 @@ void point_adjoint(const Input<T>& in, int slot, bool shepard, int strategy, con
     T S1 = T(0), N = T(0), HN = T(0), HS = T(0);
     point_traversal<T>(in.bvh, p, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
         const size_t i = (size_t)k - 1;
-        const T r = std::sqrt(d2), v = in.m[i] / in.rho[i];
+        const T r = sqrt(d2), v = in.m[i] / in.rho[i];
         const T W = Wsel(r, hb);
         S1 += v * W;
         N += A[i] * v * W;
@@ -1136,7 +1145,7 @@ void point_adjoint(const Input<T>& in, int slot, bool shepard, int strategy, con
     const T out = shepard ? N / S1 : N;
     point_traversal<T>(in.bvh, p, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
         const size_t i = (size_t)k - 1;
-        const T r = std::sqrt(d2), mb = in.m[i], rb = in.rho[i], v = mb / rb;
+        const T r = sqrt(d2), mb = in.m[i], rb = in.rho[i], v = mb / rb;
         const T W = Wsel(r, hb);
         const T c = shepard ? (A[i] - out) / S1 : A[i];
         g[0 * n + (i64)i] += cot * c * W / rb;
@@ -1212,7 +1221,7 @@ void field_adjoint(const Input<T>& in, int mode, const int* slots, bool shepard,
     T S1 = T(0), SA[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
     point_traversal<T>(in.bvh, p, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
         const size_t i = (size_t)k - 1;
-        const T r = std::sqrt(d2), v = in.m[i] / in.rho[i];
+        const T r = sqrt(d2), v = in.m[i] / in.rho[i];
         const T D[3] = {p[0] - in.x[i], p[1] - in.y[i], p[2] - in.z[i]};
         KP ka{}, kb{};
         if (wa != T(0)) ka = kp(r, ha);
@@ -1247,7 +1256,7 @@ void field_adjoint(const Input<T>& in, int mode, const int* slots, bool shepard,
     T dha = T(0);
     point_traversal<T>(in.bvh, p, strategy, Kvalid, ha, [&](i64 k, T d2, T hb) {
         const size_t i = (size_t)k - 1;
-        const T r = std::sqrt(d2), rb = in.rho[i], v = in.m[i] / rb;
+        const T r = sqrt(d2), rb = in.rho[i], v = in.m[i] / rb;
         const T D[3] = {p[0] - in.x[i], p[1] - in.y[i], p[2] - in.z[i]};
         KP ka{}, kb{};
         if (wa != T(0)) ka = kp(r, ha);

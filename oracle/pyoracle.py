@@ -44,6 +44,8 @@ def lib():
         L.oracle_line_adjoint.argtypes = [vp, i32, i32, i64] + [vp] * 6 + [vp, vp]
         L.oracle_point_adjoint.argtypes = [vp, i32, i32, i32, i64] + [vp] * 3 + [vp, vp]
         L.oracle_field_adjoint.argtypes = [vp, i32, vp, i32, i32, i64] + [vp] * 3 + [vp, vp]
+        L.oracle_point_jvp.argtypes = [vp, i32, vp, vp, i32, vp, i32, vp, i32, vp, i32, i64] + [vp] * 6 + [i32]
+        L.oracle_line_jvp.argtypes = [vp, i32, vp, vp, i32, i64] + [vp] * 10 + [i32]
         L.oracle_kernel_d2f.argtypes = [i32, d]
         L.oracle_query.argtypes = [vp, i64] + [vp] * 8
         L.oracle_kernel_d2f.restype = d
@@ -227,6 +229,42 @@ class OracleSet:
         lib().oracle_field_adjoint(self.h, int(mode), _p(sl), int(bool(shepard)), int(strategy), S, _p(px), _p(py),
                                    _p(pz), _p(ct), _p(g))
         return g
+
+    def point_jvp(self, cat, strategy, px, py, pz, tan, nthreads=0):
+        """Forward mode (dual numbers) through the float64 point path: the
+        outputs (L x S) and their derivatives along tan ((6 + ncol) x n, input
+        order: x, y, z, m, h, rho, the quantity columns)."""
+        px, py, pz = _f64(px), _f64(py), _f64(pz)
+        S = len(px)
+        ss, gs = _i32(cat.get("sslots", ())), _i32(cat.get("gslots", ()))
+        sf = np.ascontiguousarray(np.asarray(cat.get("sflags", ()), dtype=np.uint8))
+        ds, cs = _i32(np.ravel(cat.get("dslots", ()))), _i32(np.ravel(cat.get("cslots", ())))
+        L = len(ss) + 3 * len(gs) + len(ds) // 3 + len(cs)
+        t = _f64(np.asarray(tan, dtype=np.float64).reshape(6 + self.ncol, self.n))
+        out, dout = np.zeros((L, S)), np.zeros((L, S))
+        rc = lib().oracle_point_jvp(self.h, len(ss), _p(ss), _p(sf), len(gs), _p(gs), len(ds) // 3, _p(ds),
+                                    len(cs) // 3, _p(cs), int(strategy), S, _p(px), _p(py), _p(pz), _p(t), _p(out),
+                                    _p(dout), nthreads)
+        if rc != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        return out, dout
+
+    def line_jvp(self, cat, strategy, origin, direction, tan, ha=None, nthreads=0):
+        """Forward mode through the float64 line path: (outputs, derivatives
+        along tan), each nS x S."""
+        o = [_f64(a) for a in origin]
+        m = [_f64(a) for a in direction]
+        S = len(o[0])
+        ss = _i32(cat.get("sslots", ()))
+        sf = np.ascontiguousarray(np.asarray(cat.get("sflags", ()), dtype=np.uint8))
+        hv = _f64(ha) if ha is not None else None
+        t = _f64(np.asarray(tan, dtype=np.float64).reshape(6 + self.ncol, self.n))
+        out, dout = np.zeros((len(ss), S)), np.zeros((len(ss), S))
+        rc = lib().oracle_line_jvp(self.h, len(ss), _p(ss), _p(sf), int(strategy), S, *[_p(a) for a in o + m],
+                                   _p(hv) if hv is not None else None, _p(t), _p(out), _p(dout), nthreads)
+        if rc != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        return out, dout
 
     def query(self, px, py, pz, radius, with_pool=False):
         """LBVH_probe_neighbors / LBVH_query! per point: (counts, closest
