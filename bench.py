@@ -1060,7 +1060,8 @@ def main():
         "guard_band_rechecks": int(n_rechecks),
         "traversal": args.traversal,
         "launch": args.launch,
-        "autotune": [{k: e[k] for k in ("occupancy", "xcd_map", "tile_order", "best_ms", "default_ms")}
+        "autotune": [{k: e[k] for k in ("occupancy", "xcd_map", "tile_order", "bucket_leaves", "block_size", "best_ms",
+                                         "default_ms")}
                      for e in ctx.autotune_report()],
         "fp64": fp64,
         "speedup_vs_fp64": (fp64["ms_per_step"] / ms_per_step) if fp64 else None,

@@ -146,9 +146,16 @@ int ph_set_traversal(ph_context* ctx, int mode);
  *   tile_order  order of the tiles (one per wavefront: 4x4x4 grid points or
  *               8x8 image-plane rays): 0 x-fastest rows, 1 bricks of 8x8x8 or
  *               8x8 tiles with Morton order inside, -1 each family's best
- *               (rows for grids, bricks for images).
+ *               (rows for grids, bricks for images);
+ *   bucket_leaves  grid tile walks (one walk per wavefront): leaves per
+ *               bucket of the walked tree, 4, 8 (default) or 16 (the 16-leaf
+ *               tree is built on first use; an entry point that allocates
+ *               nothing walks the 8-leaf tree until it exists);
+ *   block_size  grid tile walks: threads per workgroup, 64, 128, 256
+ *               (default) or 512.
  * Line kernels follow xcd_map and tile_order; the autotuner times the point
- * and grid kernels only.
+ * and grid kernels only, and the grid tile walks' bucket_leaves and
+ * block_size after their other knobs.
  * With autotuning on, the first call of each workload key (precision,
  * strategy, kernel, catalog shape, grid or list, log2 of the sample and
  * particle counts) times the candidate configurations twice on the call's own
@@ -161,9 +168,14 @@ typedef struct ph_tune_entry {
     int32_t tile_order;
     double best_ms;    /* kernel time of the chosen configuration   */
     double default_ms; /* kernel time of the default configuration  */
+    int32_t bucket_leaves;
+    int32_t block_size;
 } ph_tune_entry;
 int ph_set_autotune(ph_context* ctx, int enable);
 int ph_set_launch(ph_context* ctx, double occupancy, int32_t xcd_map, int32_t tile_order);
+/* Grid tile walks: bucket_leaves 4, 8 or 16 and block_size 64, 128, 256 or
+ * 512; 0 keeps the default (8, 256).  Used when autotuning is off. */
+int ph_set_tile_shape(ph_context* ctx, int32_t bucket_leaves, int32_t block_size);
 /* Copies up to capacity entries of the context's tuned keys; *count = all entries. */
 int ph_autotune_report(ph_context* ctx, int32_t capacity, ph_tune_entry* out, int32_t* count);
 

@@ -183,9 +183,31 @@ LaunchCfg autotuned(ph_context* ctx, ph_lbvh* B, const CatChunk& c, int strategy
     size_t w = 0;
     for (size_t i = 1; i < cand.size(); ++i)
         if (best[i] < best[w]) w = i;
+    LaunchCfg chosen = cand[w];
+    double chosen_ms = best[w];
+    // grid tile walks: then the walked tree's bucket size and the block size,
+    // one at a time from the best so far
+    if (ga && ctx->traversal != PH_TRAVERSAL_BUCKET && !tl_no_alloc) {
+        auto stage = [&](auto set, std::initializer_list<int> values) {
+            LaunchCfg base = chosen;
+            for (int v : values) {
+                LaunchCfg cf = base;
+                set(cf, v);
+                if (cf.leaves == base.leaves && cf.block == base.block) continue;
+                run(cf);  // warm-up (a 16-leaf tree is built here)
+                const double ms = std::min(run(cf), run(cf));
+                if (ms < chosen_ms) {
+                    chosen = cf;
+                    chosen_ms = ms;
+                }
+            }
+        };
+        stage([](LaunchCfg& cf, int v) { cf.leaves = v; }, {4, 8, 16});
+        stage([](LaunchCfg& cf, int v) { cf.block = v; }, {128, 256, 512});
+    }
     ph_context::Tuned t;
-    t.cfg = cand[w];
-    t.best_ms = best[w];
+    t.cfg = chosen;
+    t.best_ms = chosen_ms;
     t.default_ms = best[0];
     ctx->tuned[key] = t;
     return t.cfg;
@@ -366,7 +388,19 @@ int ph_set_launch(ph_context* ctx, double occupancy, int32_t xcd_map, int32_t ti
     if (!(occupancy > 0.0 && occupancy <= 1.0) || xcd_map < -1 || xcd_map > 7 || tile_order < -1 ||
         tile_order > 1)
         return fail(ctx, PH_ERR_ARGUMENT, "occupancy must be in (0, 1], xcd_map in -1..7 and tile_order in -1..1");
-    ctx->launch = LaunchCfg{(float)occupancy, xcd_map, tile_order};
+    ctx->launch.occupancy = (float)occupancy;
+    ctx->launch.xcd_map = xcd_map;
+    ctx->launch.order = tile_order;
+    return PH_OK;
+}
+
+int ph_set_tile_shape(ph_context* ctx, int32_t bucket_leaves, int32_t block_size) {
+    if (!ctx) return PH_ERR_ARGUMENT;
+    if (!(bucket_leaves == 0 || bucket_leaves == 4 || bucket_leaves == 8 || bucket_leaves == 16) ||
+        !(block_size == 0 || block_size == 64 || block_size == 128 || block_size == 256 || block_size == 512))
+        return fail(ctx, PH_ERR_ARGUMENT, "bucket_leaves must be 0, 4, 8 or 16 and block_size 0, 64, 128, 256 or 512");
+    ctx->launch.leaves = bucket_leaves ? bucket_leaves : 8;
+    ctx->launch.block = block_size ? block_size : kBlock;
     return PH_OK;
 }
 
@@ -382,6 +416,8 @@ int ph_autotune_report(ph_context* ctx, int32_t capacity, ph_tune_entry* out, in
             e.tile_order = kv.second.cfg.order;
             e.best_ms = kv.second.best_ms;
             e.default_ms = kv.second.default_ms;
+            e.bucket_leaves = kv.second.cfg.leaves;
+            e.block_size = kv.second.cfg.block;
         }
         ++k;
     }

@@ -13,10 +13,12 @@ namespace {
 
 // Per internal node i: big_s[split - 1] = (leaves of i > kBucket); for a big node,
 // bs[first leaf - 1] = 1 for each child that is a bucket (at most kBucket leaves).
+// Split and bucket-start flags of up to two bucket trees (bucket sizes b0 and
+// b1; b1 = 0: one tree), each tree's segments at big_s[t] and bs[t].
 __global__ void __launch_bounds__(kBlock) k_bucket_flags(i64 n, const i32* __restrict__ left,
                                                  const i32* __restrict__ range, u32* __restrict__ big_s0,
                                                  u32* __restrict__ bs0, u32* __restrict__ big_s1,
-                                                 u32* __restrict__ bs1) {
+                                                 u32* __restrict__ bs1, int b0, int b1) {
     const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
     if (i > n - 1) return;
     const i64 lo = range[2 * (i - 1)], hi = range[2 * (i - 1) + 1];
@@ -24,8 +26,8 @@ __global__ void __launch_bounds__(kBlock) k_bucket_flags(i64 n, const i32* __res
     const i64 s = l >= n ? lo : l;  // split: the left child covers [lo, s]
     u32* const big_s[2] = {big_s0, big_s1};
     u32* const bs[2] = {bs0, bs1};
-    const int bmax[2] = {kBucketSmall, kBucket};
-    for (int t = 0; t < 2; ++t) {
+    const int bmax[2] = {b0, b1};
+    for (int t = 0; t < (b1 > 0 ? 2 : 1); ++t) {
         const bool big = hi - lo + 1 > bmax[t];
         big_s[t][s - 1] = big ? 1u : 0u;
         if (big) {
@@ -35,14 +37,18 @@ __global__ void __launch_bounds__(kBlock) k_bucket_flags(i64 n, const i32* __res
     }
 }
 
-// The scanned flag array's values at the five segment starts (info[0..4]) and
-// the build's refit check count (info[5], 0 without one).
-__global__ void k_bucket_info(const u32* __restrict__ scanned, i64 n, const u32* __restrict__ refit_bad,
+// The scanned flag array's values at the segment starts of ntrees trees and
+// at its closing entry (info[0 .. 2 ntrees]), and the build's refit check
+// count (info[5], 0 without one).
+__global__ void k_bucket_info(const u32* __restrict__ scanned, i64 n, int ntrees, const u32* __restrict__ refit_bad,
                               u32* __restrict__ info) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     const i64 per_tree = 2 * n + 1;
-    const i64 at[5] = {0, n, per_tree, per_tree + n, 2 * per_tree};
-    for (int k = 0; k < 5; ++k) info[k] = scanned ? scanned[at[k]] : 0u;
+    for (int t = 0; t < ntrees; ++t) {
+        info[2 * t] = scanned ? scanned[t * per_tree] : 0u;
+        info[2 * t + 1] = scanned ? scanned[t * per_tree + n] : 0u;
+    }
+    info[2 * ntrees] = scanned ? scanned[ntrees * per_tree] : 0u;
     info[5] = refit_bad ? *refit_bad : 0u;
 }
 
@@ -344,24 +350,23 @@ bool buckets_supported(const ph_lbvh* B) {
     return B->dims == 3 && B->precision == PH_PREC_FP64 && B->n <= kMaxBucketLeaves;
 }
 
-void ensure_buckets(ph_lbvh* B, hipStream_t s, const u32* refit_bad) {
-    // Two bucket trees per LBVH, built together: buckets of at most kBucket
-    // leaves (the grid tile kernels) and of at most kBucketSmall (the per-lane
-    // kernels, the batch queue, adjoints and queries).  One flag pass, one scan,
-    // one read-back of the record counts (with the build's refit check when the
-    // build left it here), one emit pass.
-    if (B->buckets_ready && B->buckets8_ready) return;
+// Builds one or two bucket trees of B (bucket sizes bmax[0] < bmax[1]) into
+// recs[t], nrec[t]: one flag pass, one scan, one read-back of the record counts
+// (with the build's refit check when the build left it here), one emit pass.
+void build_bucket_trees(ph_lbvh* B, hipStream_t s, int ntrees, const int* bmax, DevBuf* const* recs, i64* nrec,
+                        const u32* refit_bad) {
     const i64 n = B->n;
     if (n > kMaxBucketLeaves) throw Error{PH_ERR_CAPACITY, "the bucket traversal supports at most 2^26 particles"};
-    DevBuf* recs[2] = {&B->brecs, &B->brecs8};
-    const int bmax[2] = {kBucketSmall, kBucket};
-    i64 nrec[2] = {1, 1};
-    const bool split[2] = {n > bmax[0], n > bmax[1]};  // split[1] implies split[0]
-    // buffers first.  Flags of both trees in one array, scanned as one: per
+    bool split[2] = {false, false};  // split[1] implies split[0]
+    for (int t = 0; t < ntrees; ++t) {
+        split[t] = n > bmax[t];
+        nrec[t] = 1;
+    }
+    // buffers first.  Flags of the trees in one array, scanned as one: per
     // tree the big splits (n entries: n - 1 flags + the total) and the bucket
     // starts (n + 1 entries: n flags + the total), then one closing entry; a
     // segment's exclusive prefix is its scanned value minus that at its start.
-    const i64 per_tree = 2 * n + 1, m = 2 * per_tree + 1;
+    const i64 per_tree = 2 * n + 1, m = ntrees * per_tree + 1;
     DevBuf flags, tmp, info;
     if (split[0]) flags.alloc(sizeof(u32) * (size_t)m);
     if (split[0] || refit_bad) info.alloc(sizeof(u32) * 8);
@@ -373,11 +378,12 @@ void ensure_buckets(ph_lbvh* B, hipStream_t s, const u32* refit_bad) {
         if (split[0]) {
             PH_HIP(hipMemsetAsync(f, 0, sizeof(u32) * (size_t)m, s));
             k_bucket_flags<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->left.as<i32>(), B->range.as<i32>(), big_s[0],
-                                                             bs[0], big_s[1], bs[1]);
+                                                             bs[0], big_s[1], bs[1], bmax[0],
+                                                             ntrees > 1 ? bmax[1] : 0);
             PH_HIP(hipGetLastError());
             exclusive_sum_u32(f, m, tmp, s);
         }
-        k_bucket_info<<<1, 1, 0, s>>>(split[0] ? f : nullptr, n, refit_bad, info.as<u32>());
+        k_bucket_info<<<1, 1, 0, s>>>(split[0] ? f : nullptr, n, ntrees, refit_bad, info.as<u32>());
         PH_HIP(hipGetLastError());
         PH_HIP(hipMemcpyAsync(h, info.p, sizeof(h), hipMemcpyDeviceToHost, s));
         PH_HIP(hipStreamSynchronize(s));
@@ -385,11 +391,11 @@ void ensure_buckets(ph_lbvh* B, hipStream_t s, const u32* refit_bad) {
             throw Error{PH_ERR_INTERNAL,
                         "LinearBVH refit: " + std::to_string(h[5]) + " internal nodes were not reached twice"};
     }
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < ntrees; ++t) {
         if (split[t]) nrec[t] = (i64)(h[2 * t + 1] - h[2 * t]) + (i64)(h[2 * t + 2] - h[2 * t + 1]);
         recs[t]->alloc(sizeof(BucketRec) * (size_t)nrec[t]);
     }
-    for (int t = 0; t < 2; ++t)
+    for (int t = 0; t < ntrees; ++t)
         if (!split[t]) {  // the whole tree is one bucket
             k_bucket_root<<<1, 1, 0, s>>>(n, B->root, B->nodes.as<NodeRec<double>>(),
                                           B->leaves.as<LeafRec<double>>(), recs[t]->as<BucketRec>());
@@ -398,7 +404,8 @@ void ensure_buckets(ph_lbvh* B, hipStream_t s, const u32* refit_bad) {
     if (split[0]) {
         EmitTree et[2];
         for (int t = 0; t < 2; ++t)
-            et[t] = EmitTree{big_s[t], bs[t], h[2 * t], h[2 * t + 1], recs[t]->as<BucketRec>(), bmax[t]};
+            et[t] = t < ntrees ? EmitTree{big_s[t], bs[t], h[2 * t], h[2 * t + 1], recs[t]->as<BucketRec>(), bmax[t]}
+                               : et[0];
         k_bucket_emit<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->root, B->left.as<i32>(), B->parent.as<i32>(),
                                                         B->range.as<i32>(), B->nodes.as<NodeRec<double>>(),
                                                         B->leaves.as<LeafRec<double>>(), et[0], et[1],
@@ -408,9 +415,30 @@ void ensure_buckets(ph_lbvh* B, hipStream_t s, const u32* refit_bad) {
     // flags / tmp / info are released on return: into the context's cache,
     // reusable once the stream has drained, or freed directly (wait for the kernels)
     if (!tl_cache) PH_HIP(hipStreamSynchronize(s));
+}
+
+void ensure_buckets(ph_lbvh* B, hipStream_t s, const u32* refit_bad) {
+    // Two bucket trees per LBVH, built together: buckets of at most kBucket
+    // leaves (the grid tile kernels) and of at most kBucketSmall (the per-lane
+    // kernels, the batch queue, adjoints and queries).
+    if (B->buckets_ready && B->buckets8_ready) return;
+    const int bmax[2] = {kBucketSmall, kBucket};
+    DevBuf* recs[2] = {&B->brecs, &B->brecs8};
+    i64 nrec[2] = {1, 1};
+    build_bucket_trees(B, s, 2, bmax, recs, nrec, refit_bad);
     B->nrec = nrec[0];
     B->nrec8 = nrec[1];
     B->buckets_ready = B->buckets8_ready = true;
+}
+
+void ensure_buckets16(ph_lbvh* B, hipStream_t s) {
+    if (B->buckets16_ready) return;
+    const int bmax[1] = {kBucketLarge};
+    DevBuf* recs[1] = {&B->brecs16};
+    i64 nrec[1] = {1};
+    build_bucket_trees(B, s, 1, bmax, recs, nrec, nullptr);
+    B->nrec16 = nrec[0];
+    B->buckets16_ready = true;
 }
 
 void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl, const GridAxes* ga,

@@ -702,7 +702,11 @@ __device__ __attribute__((always_inline)) void bucket_f64_sample(const BucketVie
     if (valid) acc.finish(c, out, S, s);
 }
 
+constexpr int kBucketLarge = 16;  // the third tree, for grid tile walks only
+
 BucketView bucket_view(const ph_lbvh* B, int leaves = kBucketSmall) {
+    if (leaves == kBucketLarge)
+        return BucketView{B->n, (i32)B->nrec16, B->brecs16.as<BucketRec>(), B->leaves.as<LeafRec<double>>()};
     if (leaves == kBucket)
         return BucketView{B->n, (i32)B->nrec8, B->brecs8.as<BucketRec>(), B->leaves.as<LeafRec<double>>()};
     return BucketView{B->n, (i32)B->nrec, B->brecs.as<BucketRec>(), B->leaves.as<LeafRec<double>>()};

@@ -105,6 +105,8 @@ struct LaunchCfg {
     int xcd_map = -1;
     int order = -1;
     int tile = 1;  // FP32C grids: one wave walks its tile's union (0: one walk per lane)
+    int leaves = 8;    // grid tile walks: leaves per bucket of the walked tree (4, 8 or 16)
+    int block = 256;   // grid tile walks: threads per block (64, 128, 256 or 512)
 };
 // Configuration of the entry point running on this thread (set by the entry points).
 inline thread_local LaunchCfg tl_launch{};
@@ -252,6 +254,9 @@ struct ph_lbvh {
     bool buckets8_ready = false;
     int64_t nrec8 = 0;
     ph::DevBuf brecs8;    // 8-leaf bucket-tree records [nrec8]
+    bool buckets16_ready = false;  // built on first use (a 16-leaf tile walk)
+    int64_t nrec16 = 0;
+    ph::DevBuf brecs16;   // 16-leaf bucket-tree records [nrec16]
     ph::DevBuf brecs;     // 4-leaf bucket-tree records [nrec]
     ph::DevBuf rec32;     // FP32-compensated records of the last call ({m/rho, 1/h}, columns)
 
@@ -259,12 +264,14 @@ struct ph_lbvh {
     // valid for ph_lbvh_destroy only).
     void release_device() {
         for (ph::DevBuf* b : {&cols, &codes, &order, &left, &right, &escape, &parent, &range, &nodes, &leaves,
-                              &leaf_escape, &rec, &ha, &nn, &brecs, &brecs8, &rec32})
+                              &leaf_escape, &rec, &ha, &nn, &brecs, &brecs8, &brecs16, &rec32})
             b->release();
         buckets_ready = false;
         buckets8_ready = false;
+        buckets16_ready = false;
         nrec = 0;
         nrec8 = 0;
+        nrec16 = 0;
     }
 };
 

@@ -525,8 +525,8 @@ template <class T> struct Sampler<T, 1> {
     int xcd = 2;    // block order (xcd_block mode)
     int order = 0;  // tile order: 0 x-fastest rows, 1 bricks
     __device__ inline bool get(i64& s, T& x, T& y, T& z) const {
-        // one 4x4x4 tile of samples per wave
-        const i64 tile = xcd_block(xcd) * (kBlock / kWave) + threadIdx.x / kWave;
+        // one 4x4x4 tile of samples per wave (any whole number of waves per block)
+        const i64 tile = xcd_block(xcd) * (i64)(blockDim.x / kWave) + threadIdx.x / kWave;
         const int lane = threadIdx.x & (kWave - 1);
         const i64 tx = (g.nx + 3) / 4, ty = (g.ny + 3) / 4, tz = (g.nz + 3) / 4;
         if (tile >= tx * ty * tz) return false;

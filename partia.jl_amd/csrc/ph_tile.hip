@@ -25,7 +25,7 @@ namespace {
 
 // Grid kernels: one wave per 4 x 4 x 4 tile of grid points (Sampler<double, 1>).
 template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT>
-__global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const BucketRec* __restrict__ recs,
+__global__ void __launch_bounds__(kMaxTileBlock) k_tile_c32(BucketView bv, const BucketRec* __restrict__ recs,
                                                      const double* __restrict__ rec, CatChunk c,
                                                      Sampler<double, 1> smp, const double* __restrict__ ha_s,
                                                      float* __restrict__ out, u64* counter) {
@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(kBlock) k_tile_c32(BucketView bv, const Bucket
 }
 
 template <int K, int STRAT, int MS, int MG, int MD, int MC>
-__global__ void __launch_bounds__(kBlock) k_tile_f64(BucketView bv, const BucketRec* __restrict__ recs,
+__global__ void __launch_bounds__(kMaxTileBlock) k_tile_f64(BucketView bv, const BucketRec* __restrict__ recs,
                                                      const double* __restrict__ rec, CatChunk c,
                                                      Sampler<double, 1> smp, const double* __restrict__ ha_s,
                                                      double* __restrict__ out, u64* counter) {
@@ -80,17 +80,20 @@ __global__ void __launch_bounds__(kBlock) k_nearest_tile(BucketView bv, const Bu
     }
 }
 
-// Launches a tile kernel over the grid's 4x4x4 tiles (one wave each), with
-// dyn bytes of dynamic LDS, limited to the configured occupancy.
+// Launches a tile kernel over the grid's 4x4x4 tiles (one wave each) in blocks
+// of the configured size, with `slot` doubles per staged leaf of dynamic LDS,
+// limited to the configured occupancy.
 template <class Kern, class OutT>
-void launch_tile_kernel(Kern kern, size_t dyn, const BucketView& bv, const double* rec, const CatChunk& c,
+void launch_tile_kernel(Kern kern, int slot, const BucketView& bv, const double* rec, const CatChunk& c,
                         const GridAxes& ga, i64 S, const double* ha, OutT* out, u64* counters, hipStream_t s,
                         hipEvent_t* ev) {
     const LaunchCfg cfg = tl_launch;
+    const int block = cfg.block, waves = block / kWave;
+    const size_t dyn = tile_lds_bytes(slot, block);
     size_t pad = 0;
     if (cfg.occupancy < 1.f) {  // as occupancy_pad, with this kernel's dynamic LDS
         int per_cu = 0, dev = 0, lds_cu = 0, lds_block = 0;
-        PH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, dyn));
+        PH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, dyn));
         PH_HIP(hipGetDevice(&dev));
         PH_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
         PH_HIP(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
@@ -104,9 +107,9 @@ void launch_tile_kernel(Kern kern, size_t dyn, const BucketView& bv, const doubl
         }
     }
     const i64 tiles = ((ga.nx + 3) / 4) * ((ga.ny + 3) / 4) * ((ga.nz + 3) / 4);
-    const unsigned g = (unsigned)((tiles + 3) / 4);
+    const unsigned g = (unsigned)((tiles + waves - 1) / waves);
     if (ev) PH_HIP(hipEventRecord(ev[0], s));
-    kern<<<g, kBlock, dyn + pad, s>>>(bv, bv.recs, rec, c, Sampler<double, 1>{ga, S, point_xcd(cfg), point_order(cfg)},
+    kern<<<g, block, dyn + pad, s>>>(bv, bv.recs, rec, c, Sampler<double, 1>{ga, S, point_xcd(cfg), point_order(cfg)},
                                       ha, out, counters);
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[1], s));
@@ -119,23 +122,21 @@ void launch_tile_ks(const BucketView& bv, const double* rec, const CatChunk& c, 
     const bool small = c.ng + c.nd + c.nc == 0 && c.ns <= 2;
     if (f64) {
         double* o = static_cast<double*>(out);
-        const size_t dyn = tile_lds_bytes(5 + c.rw);
-        if (small) launch_tile_kernel(k_tile_f64<K, STRAT, 2, 0, 0, 0>, dyn, bv, rec, c, ga, S, ha, o, counters, s, ev);
+        const int slot = 5 + c.rw;
+        if (small) launch_tile_kernel(k_tile_f64<K, STRAT, 2, 0, 0, 0>, slot, bv, rec, c, ga, S, ha, o, counters, s, ev);
         else
-            launch_tile_kernel(k_tile_f64<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC>, dyn, bv, rec, c, ga, S, ha, o,
+            launch_tile_kernel(k_tile_f64<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC>, slot, bv, rec, c, ga, S, ha, o,
                                counters, s, ev);
         return;
     }
     float* o = static_cast<float*>(out);
     if (small)
-        launch_tile_kernel(k_tile_c32<K, STRAT, 2, 0, 0, 0, 0>, tile_lds_bytes(4 + c.rw), bv, rec, c, ga, S, ha, o,
-                           counters, s, ev);
+        launch_tile_kernel(k_tile_c32<K, STRAT, 2, 0, 0, 0, 0>, 4 + c.rw, bv, rec, c, ga, S, ha, o, counters, s, ev);
     else if (c2_shared(c) && c.rw == kC2Rec)
-        launch_tile_kernel(k_tile_c32<K, STRAT, 2, 1, 1, 1, 2>, tile_lds_bytes(4 + kC2Rec), bv, rec, c, ga, S, ha, o,
-                           counters, s, ev);
+        launch_tile_kernel(k_tile_c32<K, STRAT, 2, 1, 1, 1, 2>, 4 + kC2Rec, bv, rec, c, ga, S, ha, o, counters, s, ev);
     else
-        launch_tile_kernel(k_tile_c32<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0>, tile_lds_bytes(4 + c.rw), bv, rec, c,
-                           ga, S, ha, o, counters, s, ev);
+        launch_tile_kernel(k_tile_c32<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0>, 4 + c.rw, bv, rec, c, ga, S, ha, o,
+                           counters, s, ev);
 }
 
 template <int K>
@@ -164,8 +165,14 @@ void launch_nearest_tile(ph_lbvh* B, const GridAxes& ga, i64 S, double Kd, doubl
 
 void launch_tile(ph_lbvh* B, const double* rec, const CatChunk& c, int precision, int strategy, const GridAxes& ga,
                  i64 S, const double* ha, void* out, u64* counters, hipStream_t s, hipEvent_t* ev) {
+    // the walked tree: buckets of the configured size (4, 8 or 16 leaves)
     ensure_buckets(B, s);
-    const BucketView bv = bucket_view(B, kBucket);
+    int leaves = tl_launch.leaves;
+    if (leaves == kBucketLarge) {
+        if (!B->buckets16_ready && tl_no_alloc) leaves = kBucket;  // building it would allocate
+        else ensure_buckets16(B, s);
+    }
+    const BucketView bv = bucket_view(B, leaves == kBucketSmall || leaves == kBucketLarge ? leaves : kBucket);
     const bool f64 = precision == PH_PREC_FP64;
     switch (B->kernel) {
     case PH_KERNEL_M4: launch_tile_k<PH_KERNEL_M4>(strategy, bv, rec, c, f64, ga, S, ha, out, counters, s, ev); break;

@@ -28,10 +28,12 @@ constexpr int kC2Rec = 6;
 // doubles each, then 64 leaf indices.  FP32C: slot = 4 + rw (x, y, z,
 // (K h)^2, record); float64: slot = 5 + rw (x, y, z, (K h)^2, h, record).
 __host__ __device__ constexpr int tile_wave_doubles(int slot) { return (kWave + 1) * slot + kWave / 2; }
-// Dynamic LDS of a tile kernel's workgroup.
-inline size_t tile_lds_bytes(int slot) {
-    return (size_t)(kBlock / kWave) * (size_t)tile_wave_doubles(slot) * sizeof(double);
+// Dynamic LDS of a tile kernel's workgroup of `block` threads.
+inline size_t tile_lds_bytes(int slot, int block = kBlock) {
+    return (size_t)(block / kWave) * (size_t)tile_wave_doubles(slot) * sizeof(double);
 }
+// Largest workgroup of the grid tile kernels (their block size is a launch knob).
+constexpr int kMaxTileBlock = 512;
 
 // The next batch of the wave's union walk: the leaves of the kept buckets
 // (kept when any live lane's conservative test keeps them), in preorder = in

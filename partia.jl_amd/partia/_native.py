@@ -40,7 +40,7 @@ EXPORTS = (
     "ph_line_polar_device", "ph_line_polar_adjoint_device",
     "ph_structured_grid_coords", "ph_structured_grid_coords_device",
     "ph_point_samples_adjoint_device", "ph_structured_grid_adjoint_device",
-    "ph_set_autotune", "ph_set_launch", "ph_autotune_report",
+    "ph_set_autotune", "ph_set_launch", "ph_autotune_report", "ph_set_tile_shape",
     "ph_point_single", "ph_point_single_device", "ph_line_samples_ha", "ph_line_samples_ha_device",
     "ph_line_density", "ph_line_density_device",
 )
@@ -73,7 +73,8 @@ class ph_request(C.Structure):
 
 class ph_tune_entry(C.Structure):
     _fields_ = [("key", C.c_uint64), ("occupancy", C.c_double), ("xcd_map", C.c_int32), ("tile_order", C.c_int32),
-                ("best_ms", C.c_double), ("default_ms", C.c_double)]
+                ("best_ms", C.c_double), ("default_ms", C.c_double), ("bucket_leaves", C.c_int32),
+                ("block_size", C.c_int32)]
 
 
 class ph_stats(C.Structure):
@@ -144,6 +145,7 @@ def lib():
         "ph_batch_destroy": (i32, [vp]),
         "ph_set_autotune": (i32, [vp, C.c_int]),
         "ph_set_launch": (i32, [vp, d, i32, i32]),
+        "ph_set_tile_shape": (i32, [vp, i32, i32]),
         "ph_autotune_report": (i32, [vp, i32, P(ph_tune_entry), P(i32)]),
         "ph_line_samples_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, i64] + [vp] * 8),
         "ph_line_grid_adjoint_device": (i32, [vp, vp, P(ph_catalog), i32, i32, vp, vp, vp, vp, vp, i64, vp, i64,
@@ -233,6 +235,11 @@ class Context:
 
     def set_launch(self, occupancy=1.0, xcd_map=-1, tile_order=-1):
         self.check(lib().ph_set_launch(self.handle, float(occupancy), int(xcd_map), int(tile_order)))
+
+    def set_tile_shape(self, bucket_leaves=0, block_size=0):
+        """Grid tile walks: leaves per bucket (4, 8, 16) and threads per block
+        (64, 128, 256, 512); 0 restores the default."""
+        self.check(lib().ph_set_tile_shape(self.handle, int(bucket_leaves), int(block_size)))
 
     def autotune_report(self):
         """List of dicts, one per tuned workload key."""
