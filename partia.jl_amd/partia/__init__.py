@@ -18,7 +18,8 @@ from .catalog import (InterpolationCatalog, InterpolationCatalogConcise, curl_sl
 from .drivers import (LBVH_find_nearest_h, LineSamples_interpolation, PointSamples_interpolation,
                       StructuredGrid_interpolation, initialize_interpolation)
 from .errors import ArgumentError, DeviceError, DimensionMismatch, JuliaAssertionError, MethodError
-from .grids import (AbstractGrid, AbstractSamples, Cartesian, Cylindrical, GridBundle, LineSamples, PointSamples,
+from .grids import (AbstractGrid, AbstractSamples, Cartesian, Cylindrical, Frame, GridBundle, LineSamples,
+                    ParallelBeam, PointSamples, frame_forward, frame_position, frame_right, frame_up,
                     Polar, Spherical, StructuredGrid, coordinate_grid, flatten, restore_struct)
 from .input import (AABB, BinaryRadixTree, InterpolationInput, LinearBVH, LinearBVH_, UInt32, UInt64, build_input,
                     get_coord, get_xcoord, get_ycoord, get_zcoord, matches_lbvh_leaf_order)

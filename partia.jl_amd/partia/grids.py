@@ -34,6 +34,64 @@ class Spherical(AbstractCoordinateSystem):
     pass
 
 
+class ParallelBeam:
+    """Beam-model tag of the frame-plane LineSamples constructors: every line
+    along the frame's forward direction (Grids/grids/LineSamples.jl:441-537)."""
+
+
+class Frame:
+    """Frame(x, f0, u0) (Frames/struct/Frame.jl:42-78) in its initial
+    orientation (identity quaternion): forward and up normalised (hypot), their
+    dot product within sqrt(eps) of 0, right = forward x up normalised.  The
+    accessors frame_position / frame_right / frame_up / frame_forward
+    (Frame.jl:145-186) return the basis vectors as tuples."""
+
+    def __init__(self, x, f0, u0):
+        x, f0, u0 = (tuple(float(v) for v in a) for a in (x, f0, u0))
+        nf0, nu0 = math.hypot(*f0), math.hypot(*u0)
+        if nf0 == 0.0:
+            raise ArgumentError("f0 must be nonzero.")
+        if nu0 == 0.0:
+            raise ArgumentError("u0 must be nonzero.")
+        inf0, inu0 = 1.0 / nf0, 1.0 / nu0
+        f = tuple(v * inf0 for v in f0)
+        u = tuple(v * inu0 for v in u0)
+        if abs(f[0] * u[0] + f[1] * u[1] + f[2] * u[2]) > math.sqrt(np.finfo(np.float64).eps):
+            raise ArgumentError("f0 and u0 must be orthogonal.")
+        r = (f[1] * u[2] - f[2] * u[1], f[2] * u[0] - f[0] * u[2], f[0] * u[1] - f[1] * u[0])
+        nr = math.sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2])
+        self.x, self.f0, self.u0 = x, f, u
+        self.r0 = tuple(v / nr for v in r)
+
+    def position(self):
+        return self.x
+
+    def right(self):
+        return self.r0
+
+    def up(self):
+        return self.u0
+
+    def forward(self):
+        return self.f0
+
+
+def frame_position(frame):
+    return frame.position()
+
+
+def frame_right(frame):
+    return frame.right()
+
+
+def frame_up(frame):
+    return frame.up()
+
+
+def frame_forward(frame):
+    return frame.forward()
+
+
 class AbstractGrid:
     pass
 
@@ -51,6 +109,25 @@ class PointSamples(AbstractSamples):
                 raise ArgumentError("coor[%d] length mismatch" % (d + 1))
         self.grid = grid
         self.coor = coor
+
+    @classmethod
+    def from_frame(cls, coord, frame, *params):
+        """PointSamples(COORD, frame, params...) (Grids/grids/PointSamples.jl:328-418):
+        Cartesian plane (xparams, yparams) or box (+ zparams), Polar plane
+        (sparams, phiparams) or Cylindrical (+ zparams) in the frame's basis;
+        values zero."""
+        pos, r, u, f = frame.position(), frame.right(), frame.up(), frame.forward()
+        if coord is Cartesian and len(params) == 2:
+            c = cartesian_plane_coordinates(pos, r, u, *params)
+        elif coord is Cartesian and len(params) == 3:
+            c = cartesian_box_coordinates(pos, r, u, f, *params)
+        elif coord is Polar and len(params) == 2:
+            c = polar_plane_coordinates(pos, r, u, *params)
+        elif coord is Cylindrical and len(params) == 3:
+            c = cylindrical_coordinates(pos, r, u, f, *params)
+        else:
+            raise ArgumentError("no frame-based PointSamples constructor for these arguments")
+        return cls(np.zeros(len(c[0])), c)
 
     def similar(self, dtype=None):
         """similar(grid) shares the coordinates; similar(grid, T) copies them as T."""
@@ -181,6 +258,21 @@ class LineSamples(AbstractSamples):
         self.grid = grid
         self.origin = origin
         self.direction = direction
+
+    @classmethod
+    def from_frame(cls, coord, beam, frame, *params):
+        """LineSamples(COORD, ParallelBeam, frame, params...)
+        (Grids/grids/LineSamples.jl:462-537): origins on the frame's Cartesian
+        (xparams, yparams) or Polar (sparams, phiparams) plane, every direction
+        the frame's forward vector; values zero."""
+        if beam is not ParallelBeam:
+            raise ArgumentError("only the ParallelBeam model is defined")
+        if not (coord in (Cartesian, Polar) and len(params) == 2):
+            raise ArgumentError("no frame-based LineSamples constructor for these arguments")
+        o = PointSamples.from_frame(coord, frame, *params).coor
+        n = len(o[0])
+        d = tuple(np.full(n, v) for v in frame.forward())
+        return cls(np.zeros(n), o, d)
 
     def similar(self, dtype=None):
         if dtype is None or np.dtype(dtype) == self.grid.dtype:
