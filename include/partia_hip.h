@@ -117,7 +117,8 @@ int ph_set_count_neighbours(ph_context* ctx, int enable);
  *                        outward, float64 leaf tests);
  *   PH_TRAVERSAL_AUTO    bucket.
  * PH_PREC_FP32C always uses the bucket traversal; Float32 trees the lane one.
- * The bucket traversal supports up to 2^26 particles (PH_ERR_CAPACITY beyond).
+ * The bucket traversal supports up to 2^30 particles, the LinearBVH limit
+ * (BinaryRadixTree.jl:86-91; PH_ERR_CAPACITY beyond).
  * Line calls (LinearBVHTraversalMacros.jl:539-644) give identical results in
  * every mode: PH_TRAVERSAL_LANE walks one DFS per ray; otherwise the rays of an
  * image plane (ph_line_grid_device) are walked 64 at a time by one wave over
@@ -256,7 +257,7 @@ int ph_point_samples_device(ph_context* ctx, ph_lbvh* bvh, const ph_catalog* cat
  * allocates nothing, needs 8 S (gather / symmetric) + 8 n rw (FP32C:
  * 8 (n + 1) rw) bytes of that workspace (rw = record width, 2..14 doubles)
  * and returns PH_ERR_CAPACITY
- * beyond it.  PH_PREC_FP64 or PH_PREC_FP32C on a float64 tree of at most 2^26
+ * beyond it.  PH_PREC_FP64 or PH_PREC_FP32C on a float64 tree of at most 2^30
  * particles (whose bucket trees ph_lbvh_build made).  The results equal
  * ph_point_samples_device's bit for bit.  The kernels are only enqueued on the
  * context's stream (no wait; ph_synchronize before reading out_dev), and

@@ -86,7 +86,7 @@ __device__ __attribute__((always_inline)) void union_walk(const BucketView& bv, 
                 continue;
             }
             if (keep) {
-                const int cnt = bucket_cnt(R.link);
+                const int cnt = bucket_cnt(R);
                 if (nl + cnt > kWave) {  // starts the next batch
                     ++steps;
                     break;

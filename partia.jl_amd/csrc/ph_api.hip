@@ -720,7 +720,7 @@ int ph_point_samples_device_noalloc(ph_context* ctx, ph_lbvh* B, const ph_catalo
         check_catalog(B, cat);
         need(S == 0 || (px && py && pz && out), PH_ERR_ARGUMENT, "null sample or output pointer");
         need(B->buckets_ready && B->buckets8_ready, PH_ERR_ARGUMENT,
-             "ph_point_samples_device_noalloc needs a tree with bucket trees (3D, float64, at most 2^26 particles)");
+             "ph_point_samples_device_noalloc needs a tree with bucket trees (3D, float64, at most 2^30 particles)");
         need(ctx->noalloc_scratch != nullptr, PH_ERR_INTERNAL, "static workspace address missing");
         if (S == 0) return PH_OK;
         // workspace: ha (gather / symmetric) at offset 0, then the largest
@@ -1300,7 +1300,7 @@ int ph_batch_create(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int32_t 
              "batches run in PH_PREC_FP64 or PH_PREC_FP32C");
         check_precision(B, precision);
         check_catalog(B, cat);
-        need(B->n <= (int64_t(1) << 26), PH_ERR_CAPACITY, "batches support up to 2^26 particles");
+        need(B->n <= (int64_t(1) << 30), PH_ERR_CAPACITY, "batches support up to 2^30 particles");
         auto b = std::make_unique<ph_batch>();
         b->ctx = ctx;
         b->B = B;

@@ -571,13 +571,16 @@ BatchState* batch_build(ph_context* ctx, ph_lbvh* B, const ph_catalog* cat, int 
     }
     // the walk of every request's waves: chosen per wave unless the context
     // asks for the union walk (PH_TRAVERSAL_WAVE) or per-lane walks
-    const i32 walk = ctx->traversal == PH_TRAVERSAL_WAVE ? kWalkUnion
-                     : ctx->traversal == PH_TRAVERSAL_BUCKET || ctx->traversal == PH_TRAVERSAL_LANE ? kWalkLane
-                                                                                                 : kWalkChoose;
+    i32 walk = ctx->traversal == PH_TRAVERSAL_WAVE ? kWalkUnion
+               : ctx->traversal == PH_TRAVERSAL_BUCKET || ctx->traversal == PH_TRAVERSAL_LANE ? kWalkLane
+                                                                                           : kWalkChoose;
+    // the per-lane FP32C walk reads through 32-bit buffer offsets: past them
+    // every wave takes the union walk (64-bit addresses; the same results)
+    if (precision != PH_PREC_FP64)
+        for (const CatChunk& c : st->chunks)
+            if (c32_wide(B->n, c.rw)) walk = kWalkUnion;
     for (BatchReq& d : st->host) d.walk = walk;
     const i64 n = B->n;
-    if (precision != PH_PREC_FP64 && (i64)st->rw_max * 8 * n >= (i64(1) << 32))
-        throw Error{PH_ERR_CAPACITY, "FP32C records exceed 4 GiB; split the catalog or use PH_PREC_FP64"};
     ensure_buckets(B, ctx->stream);
     st->rec.alloc(sizeof(double) * (size_t)st->rw_max * (size_t)(n + 1));  // FP32C: + the zero record
     st->ctl.alloc(sizeof(unsigned long long) * (1 + st->chunks.size()));

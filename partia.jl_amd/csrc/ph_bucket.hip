@@ -120,13 +120,13 @@ __global__ void __launch_bounds__(kBlock) k_bucket_emit(i64 n, i32 root, const i
         T.recs[pre] = R;
         if (s - lo + 1 <= T.bmax) {  // left child is a bucket, right after its parent
             node_box(s == lo ? (lo + n - 1) : s, n, nodes, leaves, R);
-            R.link = bucket_link(lo - 1, s - lo + 1);
+            encode_bucket(R, lo - 1, s - lo + 1);
             T.recs[pre + 1] = R;
         }
         if (hi - s <= T.bmax) {
             const i64 rpre = Lc + PB(s) + PS(s);
             node_box(s + 1 == hi ? (hi + n - 1) : s + 1, n, nodes, leaves, R);
-            R.link = bucket_link(s, hi - s);
+            encode_bucket(R, s, hi - s);
             T.recs[rpre] = R;
         }
     }
@@ -138,7 +138,7 @@ __global__ void k_bucket_root(i64 n, i32 root, const NodeRec<double>* __restrict
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     BucketRec R;
     node_box(root != 0 ? root : n, n, nodes, leaves, R);
-    R.link = bucket_link(0, n);
+    encode_bucket(R, 0, n);
     recs[0] = R;
 }
 
@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(kBlock) k_nearest_bucket(BucketView bv, Sample
     if (smp.get(s, p[0], p[1], p[2])) ha[s] = nearest_h_bucket(bv, p, neighbour_bound2(bv, Kd));
 }
 
-template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT, int SRC>
+template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT, int SRC, bool WIDE = false>
 __global__ void __launch_bounds__(kBlock) k_bucket_c32(BucketView bv, const double* __restrict__ rec, CatChunk c,
                                                        Sampler<double, SRC> smp, const double* __restrict__ ha_s,
                                                        float* __restrict__ out, u64* counter) {
@@ -165,8 +165,8 @@ __global__ void __launch_bounds__(kBlock) k_bucket_c32(BucketView bv, const doub
     }
     u64 accepted = 0;
     __shared__ u32 blist[kQ][kBlock];
-    bucket_c32_sample<K, STRAT, MS, MG, MD, MC, EXACT>(bv, rec, c, valid, p, ha, out, smp.S, s, accepted,
-                                                       &blist[0][threadIdx.x], kBlock);
+    bucket_c32_sample<K, STRAT, MS, MG, MD, MC, EXACT, WIDE>(bv, rec, c, valid, p, ha, out, smp.S, s, accepted,
+                                                             &blist[0][threadIdx.x], kBlock);
     count_add(counter, accepted);
 }
 
@@ -229,7 +229,10 @@ void launch_c32_t(ph_lbvh* B, const double* rec, const CatChunk& c, const PointL
         kern<<<g, kBlock, pad, s>>>(bv, rec, c, Sampler<double, 1>{*ga, S, point_xcd(cfg), point_order(cfg)}, ha,
                                     static_cast<float*>(out), counters);
     } else {
-        auto kern = k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 0>;
+        // point lists past 32-bit offsets: the 64-bit-address variant (grids
+        // then take the tile walk, launch_bucket_ks)
+        auto kern = c32_wide(B->n, c.rw) ? k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 0, true>
+                                         : k_bucket_c32<K, STRAT, MS, MG, MD, MC, EXACT, 0>;
         const size_t pad = occupancy_pad(kern, cfg.occupancy);
         if (ev) PH_HIP(hipEventRecord(ev[0], s));
         kern<<<g, kBlock, pad, s>>>(bv, rec, c, Sampler<double, 0>{*pl, S, point_xcd(cfg)}, ha,
@@ -278,8 +281,6 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
         recp = ws ? ws->rec : B->rec.as<double>();
     } else {
         c32 = compact_c32(c, ncol);
-        if ((i64)c32.rw * 8 * (n + 1) >= (i64(1) << 32))
-            throw Error{PH_ERR_CAPACITY, "FP32C records exceed 4 GiB; split the catalog or use PH_PREC_FP64"};
         if (!ws) B->rec32.alloc(sizeof(double) * (size_t)c32.rw * (size_t)(n + 1));  // + the zero record
         recp = ws ? ws->rec : B->rec32.as<double>();
     }
@@ -315,7 +316,7 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
     }
     k_pack_c32<<<grid_for(n + 1), kBlock, 0, s>>>(B->cols.as<double>(), n, c32, ncol, c32.rw, recp);
     PH_HIP(hipGetLastError());
-    if (ga && tl_launch.tile) {  // one walk per wave of grid points
+    if (ga && (tl_launch.tile || c32_wide(n, c32.rw))) {  // one walk per wave of grid points
         launch_tile(B, recp, c32, precision, STRAT, *ga, S, ha, out, counters, s, ev);
         return;
     }
@@ -356,7 +357,7 @@ bool buckets_supported(const ph_lbvh* B) {
 void build_bucket_trees(ph_lbvh* B, hipStream_t s, int ntrees, const int* bmax, DevBuf* const* recs, i64* nrec,
                         const u32* refit_bad) {
     const i64 n = B->n;
-    if (n > kMaxBucketLeaves) throw Error{PH_ERR_CAPACITY, "the bucket traversal supports at most 2^26 particles"};
+    if (n > kMaxBucketLeaves) throw Error{PH_ERR_CAPACITY, "the bucket traversal supports at most 2^30 particles"};
     bool split[2] = {false, false};  // split[1] implies split[0]
     for (int t = 0; t < ntrees; ++t) {
         split[t] = n > bmax[t];
@@ -393,8 +394,11 @@ void build_bucket_trees(ph_lbvh* B, hipStream_t s, int ntrees, const int* bmax, 
     }
     for (int t = 0; t < ntrees; ++t) {
         if (split[t]) nrec[t] = (i64)(h[2 * t + 1] - h[2 * t]) + (i64)(h[2 * t + 2] - h[2 * t + 1]);
-        recs[t]->alloc(sizeof(BucketRec) * (size_t)nrec[t]);
+        // record indices and escape links are int32
+        if (nrec[t] > (i64)INT32_MAX - 16)
+            throw Error{PH_ERR_CAPACITY, "bucket tree of more than 2^31 - 16 records"};
     }
+    for (int t = 0; t < ntrees; ++t) recs[t]->alloc(sizeof(BucketRec) * (size_t)nrec[t]);
     for (int t = 0; t < ntrees; ++t)
         if (!split[t]) {  // the whole tree is one bucket
             k_bucket_root<<<1, 1, 0, s>>>(n, B->root, B->nodes.as<NodeRec<double>>(),

@@ -65,7 +65,9 @@ constexpr int kLog2BucketSmall = kBucketSmall == 4 ? 2 : (kBucketSmall == 8 ? 3 
 constexpr int kQ = PH_BUCKET_BATCH;        // buckets tested per batch
 static_assert(kBucket == 4 || kBucket == 8 || kBucket == 16 || kBucket == 32, "bucket size 4, 8, 16 or 32");
 static_assert(kQ * kBucketSmall <= 64, "one 64-bit mask per lane and batch");
-constexpr i64 kMaxBucketLeaves = i64(1) << 26;  // leaf index of a bucket stored in 26 bits
+// Leaves of a tree with bucket trees: the LinearBVH's own limit
+// (LinearBoundingVolumeHierarchy/BinaryRadixTree/BinaryRadixTree.jl:86-91).
+constexpr i64 kMaxBucketLeaves = i64(1) << 30;
 
 // Preorder record: link >= 0 -> internal node, link = index after its subtree;
 // link < 0 -> bucket of leaves [lo, lo + cnt), link = -(1 + (lo << 5 | (cnt - 1))).
@@ -96,9 +98,20 @@ struct Buf {
     }
 };
 
-__device__ inline i32 bucket_link(i64 lo0, i64 cnt) { return -(i32)(1 + ((lo0 << 5) | (cnt - 1))); }
-__device__ inline i64 bucket_lo(i32 link) { return (i64)((u32)(-link - 1) >> 5); }
-__device__ inline int bucket_cnt(i32 link) { return (int)((u32)(-link - 1) & 31u) + 1; }
+// A bucket record: link = -(1 + first leaf index) (any index below 2^31 - 1),
+// and the leaf count - 1 in the five low bits of the scale, which is first
+// rounded up to a multiple of 32 units in the last place: the scale stays an
+// upper bound of the bucket's h (only pruning tests read it; the box and the
+// leaf tests are untouched).  An internal record's link is its escape index
+// (>= 0) and its scale is the plain rounded-up maximum.
+__device__ inline void encode_bucket(BucketRec& R, i64 lo0, i64 cnt) {
+    R.link = -(i32)(1 + lo0);
+    u32 b = __builtin_bit_cast(u32, R.scale);
+    if (b < 0x7f800000u) b = b > 0x7f7fffe0u ? 0x7f7fffe0u : ((b + 31u) & ~31u);  // finite: round up
+    R.scale = __builtin_bit_cast(float, (b & ~31u) | (u32)(cnt - 1));
+}
+__device__ inline i64 bucket_lo(i32 link) { return -(i64)link - 1; }
+__device__ inline int bucket_cnt(const BucketRec& R) { return (int)(__builtin_bit_cast(u32, R.scale) & 31u) + 1; }
 
 __device__ inline float next_down(float f) {
     if (f != f || f == -INFINITY) return f;
@@ -451,13 +464,20 @@ template <int STRAT> struct Probe {
 
 // Per-lane list of up to kQ buckets in LDS (entry q at slot[q * stride]): the
 // bucket code -link - 1 = first leaf << 5 | (count - 1).
+// A lane's listed buckets of the kBucketSmall-leaf tree: (first leaf <<
+// kLog2BucketSmall) | (count - 1) per entry (first leaf < 2^30).
 struct BucketList {
     u32* slot;
     int n;
     int stride = kBlock;
-    __device__ __attribute__((always_inline)) void push(i32 link) { slot[n++ * stride] = (u32)(-link - 1); }
+    __device__ __attribute__((always_inline)) void push(const BucketRec& R) {
+        slot[n++ * stride] = ((u32)bucket_lo(R.link) << kLog2BucketSmall) | (u32)(bucket_cnt(R) - 1);
+    }
     __device__ __attribute__((always_inline)) u32 code(int q) const { return slot[q * stride]; }
-    __device__ __attribute__((always_inline)) u32 lo(int q) const { return code(q) >> 5; }
+    __device__ __attribute__((always_inline)) u32 lo(int q) const { return code(q) >> kLog2BucketSmall; }
+    __device__ __attribute__((always_inline)) int cnt(int q) const {
+        return (int)(code(q) & (u32)(kBucketSmall - 1)) + 1;
+    }
 };
 
 // Walks from record i until kQ buckets are collected or the tree is done.
@@ -472,7 +492,7 @@ __device__ __attribute__((always_inline)) void collect(const BucketView& bv, con
         if (R.link >= 0) {
             i = k ? i + 1 : R.link;
         } else {
-            if (k) bl.push(R.link);
+            if (k) bl.push(R);
             ++i;
         }
     }
@@ -486,9 +506,12 @@ __device__ __attribute__((always_inline)) u64 test_buckets(const BucketView& bv,
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
         if (q < bl.n) {
-            const u32 code = bl.code(q);
-            const i64 lo = code >> 5;
-            const int cnt = (int)(code & 31u) + 1;
+            const i64 lo = bl.lo(q);
+            int cnt = bl.cnt(q);
+            // the count's range (1..4) is hidden from the optimiser: the loop
+            // below keeps its partial unrolling instead of four loads in flight
+            // per lane (registers: occupancy of the per-lane and batch kernels)
+            __asm__ volatile("" : "+v"(cnt));
 #pragma unroll 4
             for (int j = 0; j < cnt; ++j) {
                 const LeafRec<double> L = bv.leaves[lo + j];
@@ -535,9 +558,9 @@ __device__ inline float box_d2_32(const BucketRec& R, float px, float py, float 
     return d2;
 }
 
-__device__ inline void nearest_scan(const BucketView& bv, const double p[3], i32 link, Nearest& nr) {
-    const i64 lo = bucket_lo(link);
-    const int cnt = bucket_cnt(link);
+__device__ inline void nearest_scan(const BucketView& bv, const double p[3], const BucketRec& R, Nearest& nr) {
+    const i64 lo = bucket_lo(R.link);
+    const int cnt = bucket_cnt(R);
     for (int j = 0; j < cnt; ++j) nr.offer(p, bv.leaves[lo + j], lo + j);
 }
 
@@ -555,7 +578,7 @@ __device__ inline void nearest_walk(const BucketView& bv, const double p[3], Nea
         if (R.link >= 0) {
             i = keep ? i + 1 : R.link;
         } else {
-            if (keep) nearest_scan(bv, p, R.link, nr);
+            if (keep) nearest_scan(bv, p, R, nr);
             ++i;
         }
     }
@@ -570,7 +593,7 @@ __device__ inline Nearest nearest_bucket(const BucketView& bv, const double p[3]
     for (i64 steps = bv.nrec; steps > 0; --steps) {
         const BucketRec R = bv.recs[i];
         if (R.link < 0) {
-            nearest_scan(bv, p, R.link, nr);
+            nearest_scan(bv, p, R, nr);
             break;
         }
         const i32 l = i + 1;
@@ -602,7 +625,7 @@ __device__ inline double neighbour_bound2(const BucketView& bv, double Kd) {
 // FP32-compensated interpolation of sample s at p (ha: smoothing length of its
 // nearest particle, gather / symmetric).  out: L x S float32.
 // list: this lane's kQ bucket-list entries in LDS, entry q at list[q * stride].
-template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT>
+template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT, bool WIDE = false>
 __device__ __attribute__((always_inline)) void bucket_c32_sample(const BucketView& bv, const double* __restrict__ rec,
                                                                  const CatChunk& c, bool valid, const double p[3],
                                                                  double ha, float* __restrict__ out, i64 S, i64 s,
@@ -626,6 +649,8 @@ __device__ __attribute__((always_inline)) void bucket_c32_sample(const BucketVie
     const u32 nlast = (u32)(bv.n - 1);
     const Buf rb = Buf::of(rec, ((i64)bv.n + 1) * rw * 8);  // with the zero record n
     const Buf lb = Buf::of(bv.leaves, (i64)bv.n * (i64)sizeof(LeafRec<double>));
+    // WIDE: the leaves or the records pass 32-bit byte offsets (more than 2^27
+    // leaves, or records of 4 GiB): the same loads through 64-bit addresses
 
     AccC<K, STRAT, MS, MG, MD, MC, EXACT> acc;
     acc.init();
@@ -652,14 +677,21 @@ __device__ __attribute__((always_inline)) void bucket_c32_sample(const BucketVie
                 mask &= mask - 1;
                 const u32 kk = bl.lo(b >> kLog2BucketSmall) + (u32)(b & (kBucketSmall - 1));
                 k[u] = kk < nlast ? kk : nlast;
-                L[u] = lb.leaf(k[u] * (u32)sizeof(LeafRec<double>));
+                if constexpr (WIDE) L[u] = bv.leaves[k[u]];
+                else L[u] = lb.leaf(k[u] * (u32)sizeof(LeafRec<double>));
             }
 #pragma unroll
             for (int u = 0; u < PH_ACC_UNROLL; ++u) {
                 const double dx = p[0] - L[u].x, dy = p[1] - L[u].y, dz = p[2] - L[u].z;
+                const double d2 = (dx * dx + dy * dy) + dz * dz;
                 // a lane without a pending neighbour reads the zero record
-                acc.add(live[u], c, rb, (live[u] ? k[u] : (u32)bv.n) * (u32)rw * 8u, dx, dy, dz,
-                        (dx * dx + dy * dy) + dz * dz, iha2, Wa_norm, Ga_norm);
+                const u32 kr = live[u] ? k[u] : (u32)bv.n;
+                if constexpr (WIDE) {
+                    const double* r = rec + (i64)kr * rw;
+                    acc.add_r(live[u], c, [&](int pos) { return r[pos]; }, dx, dy, dz, d2, iha2, Wa_norm, Ga_norm);
+                } else {
+                    acc.add(live[u], c, rb, kr * (u32)rw * 8u, dx, dy, dz, d2, iha2, Wa_norm, Ga_norm);
+                }
             }
         }
         if (!__any(i < bv.nrec && steps > 0)) break;
@@ -703,6 +735,12 @@ __device__ __attribute__((always_inline)) void bucket_f64_sample(const BucketVie
 }
 
 constexpr int kBucketLarge = 16;  // the third tree, for grid tile walks only
+
+// Whether the per-lane FP32C walk needs 64-bit addresses: the leaves or the
+// records (rw doubles each, plus the zero record) pass 4 GiB.
+inline bool c32_wide(i64 n, int rw) {
+    return (n + 1) * (i64)rw * 8 > (i64)0xffffffff || n * (i64)sizeof(LeafRec<double>) > (i64)0xffffffff;
+}
 
 BucketView bucket_view(const ph_lbvh* B, int leaves = kBucketSmall) {
     if (leaves == kBucketLarge)

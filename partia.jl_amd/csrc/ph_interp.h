@@ -85,7 +85,7 @@ void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, c
 // Builds B's two bucket trees if they are not built yet (buckets of at most
 // 4 leaves: per-lane kernels, batches, adjoints, queries; of at most 8: the
 // grid tile kernels).  ph_lbvh_build calls it for every tree that
-// buckets_supported accepts (3D, float64, at most 2^26 particles).
+// buckets_supported accepts (3D, float64, at most 2^30 particles).
 // refit_bad: the build's refit check count on the device, read back with the
 // record counts (0 expected), or null.
 void ensure_buckets(ph_lbvh* B, hipStream_t s, const u32* refit_bad = nullptr);

@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(kBlock) k_query(BucketView bv, PointList pl, i
         }
         if (keep) {
             const i64 lo = bucket_lo(R.link);
-            const int c = bucket_cnt(R.link);
+            const int c = bucket_cnt(R);
             for (int j = 0; j < c; ++j) {
                 const double d2 = d2_point(p[0], p[1], p[2], bv.leaves[lo + j]);
                 if (d2 <= r2) {

@@ -54,7 +54,7 @@ __device__ __attribute__((always_inline)) int union_batch(const BucketView& bv, 
             continue;
         }
         if (keep) {
-            const int cnt = bucket_cnt(R.link);
+            const int cnt = bucket_cnt(R);
             if (nl + cnt > kWave) {  // starts the next batch
                 ++steps;
                 break;
@@ -136,7 +136,7 @@ __device__ __attribute__((always_inline)) void tile_c32_sample(const BucketView&
     acc.init();
     const u64 live_mask = __ballot(live_lane);
     i32 i = live_mask ? 0 : bv.nrec;  // wave-uniform from here on
-    i32 steps = bv.nrec + 8;  // the union walk visits every record at most once (nrec < 2^27)
+    i32 steps = bv.nrec + 8;  // the union walk visits every record at most once (nrec < 2^31 - 16)
     for (;;) {
         const int nl = union_batch<STRAT>(bv, recs, pr, live_mask, i, steps, lidx, lane);
         if (nl == 0) break;
@@ -273,11 +273,13 @@ __device__ __attribute__((always_inline)) double nearest_tile_h(const BucketView
     }
     // greedy descent toward the centre; decisions made wave-uniform
     i32 i = 0;
-    i32 link = -1;  // bucket 0, one leaf: a valid fallback
+    i64 lo = 0;  // bucket 0, one leaf: a valid fallback
+    int cnt = 1;
     for (i64 steps = bv.nrec; steps > 0; --steps) {
         const BucketRec R = recs[i];
         if (R.link < 0) {
-            link = R.link;
+            lo = bucket_lo(R.link);
+            cnt = bucket_cnt(R);
             break;
         }
         const i32 l = i + 1;
@@ -289,7 +291,7 @@ __device__ __attribute__((always_inline)) double nearest_tile_h(const BucketView
     }
     const i64 n = bv.n;
     const int m = (int)(n < kNnCand ? n : kNnCand);
-    i64 first = bucket_lo(link) + bucket_cnt(link) / 2 - kNnCand / 2;
+    i64 first = lo + cnt / 2 - kNnCand / 2;
     first = first < 0 ? 0 : (first > n - m ? n - m : first);
     if (lane < m) cand[lane] = bv.leaves[first + lane];
     wave_sync();
