@@ -149,9 +149,9 @@ int ph_set_traversal(ph_context* ctx, int mode);
  *               8x8 tiles with Morton order inside, -1 each family's best
  *               (rows for grids, bricks for images);
  *   bucket_leaves  grid tile walks (one walk per wavefront): leaves per
- *               bucket of the walked tree, 4, 8 (default) or 16 (the 16-leaf
- *               tree is built on first use; an entry point that allocates
- *               nothing walks the 8-leaf tree until it exists);
+ *               bucket of the walked tree, 4, 8 (default) or 16 (ph_lbvh_build
+ *               builds all three trees, and a one-leaf tree for the nearest-h
+ *               searches);
  *   block_size  grid tile walks: threads per workgroup, 64, 128, 256
  *               (default) or 512.
  * Line kernels follow xcd_map and tile_order; the autotuner times the point

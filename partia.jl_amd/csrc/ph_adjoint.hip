@@ -420,7 +420,7 @@ void adjoint_nearest(ph_lbvh* B, const BucketView& bv, const PointList* pl, cons
     ha = B->ha.as<double>();
     nn = B->nn.as<i64>();
     if (ga) launch_nearest_tile(B, *ga, S, Kd, ha, s, nn);
-    else k_nearest_index<0><<<g, kBlock, 0, s>>>(bv, Sampler<double, 0>{*pl, S, xm}, Kd, ha, nn);
+    else k_nearest_index<0><<<g, kBlock, 0, s>>>(nearest_view(B), Sampler<double, 0>{*pl, S, xm}, Kd, ha, nn);
     PH_HIP(hipGetLastError());
 }
 

@@ -187,14 +187,14 @@ LaunchCfg autotuned(ph_context* ctx, ph_lbvh* B, const CatChunk& c, int strategy
     double chosen_ms = best[w];
     // grid tile walks: then the walked tree's bucket size and the block size,
     // one at a time from the best so far
-    if (ga && ctx->traversal != PH_TRAVERSAL_BUCKET && !tl_no_alloc) {
+    if (ga && ctx->traversal != PH_TRAVERSAL_BUCKET) {
         auto stage = [&](auto set, std::initializer_list<int> values) {
             LaunchCfg base = chosen;
             for (int v : values) {
                 LaunchCfg cf = base;
                 set(cf, v);
                 if (cf.leaves == base.leaves && cf.block == base.block) continue;
-                run(cf);  // warm-up (a 16-leaf tree is built here)
+                run(cf);  // warm-up
                 const double ms = std::min(run(cf), run(cf));
                 if (ms < chosen_ms) {
                     chosen = cf;
@@ -719,7 +719,7 @@ int ph_point_samples_device_noalloc(ph_context* ctx, ph_lbvh* B, const ph_catalo
         check_precision(B, precision);
         check_catalog(B, cat);
         need(S == 0 || (px && py && pz && out), PH_ERR_ARGUMENT, "null sample or output pointer");
-        need(B->buckets_ready && B->buckets8_ready, PH_ERR_ARGUMENT,
+        need(B->buckets_ready, PH_ERR_ARGUMENT,
              "ph_point_samples_device_noalloc needs a tree with bucket trees (3D, float64, at most 2^30 particles)");
         need(ctx->noalloc_scratch != nullptr, PH_ERR_INTERNAL, "static workspace address missing");
         if (S == 0) return PH_OK;

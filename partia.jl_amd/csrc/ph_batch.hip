@@ -242,7 +242,8 @@ __device__ inline bool union_pays(bool live, const double p[3], double r) {
 
 template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT, bool C32>
 __global__ void __launch_bounds__(kBlock) k_batch(BucketView bv, const BucketRec* __restrict__ recs, BucketView bv8,
-                                                  const BucketRec* __restrict__ recs8, const double* __restrict__ rec,
+                                                  const BucketRec* __restrict__ recs8, BucketView nv,
+                                                  const BucketRec* __restrict__ nrecs, const double* __restrict__ rec,
                                                   CatChunk c, const BatchReq* __restrict__ reqs, i32 nreq, i64 units,
                                                   unsigned long long* __restrict__ ctr,
                                                   unsigned long long* __restrict__ times, int wave_doubles) {
@@ -291,10 +292,10 @@ __global__ void __launch_bounds__(kBlock) k_batch(BucketView bv, const BucketRec
         if constexpr (STRAT != PH_SCATTER) {
             if (mode == kWalkUnion) {
                 i64 bk = 0;
-                ha = nearest_tile_h(bv, recs, valid, p, Kd, reinterpret_cast<LeafRec<double>*>(wl), bk);
+                ha = nearest_tile_h(nv, nrecs, valid, p, Kd, reinterpret_cast<LeafRec<double>*>(wl), bk);
                 wave_sync();  // the candidates are read before the region is reused
             } else if (valid) {
-                ha = nearest_h_bucket(bv, p, neighbour_bound2(bv, Kd));
+                ha = nearest_h_bucket(nv, p, neighbour_bound2(nv, Kd));
             }
             if (!valid) ha = 0.0;
             if (mode == kWalkChoose) walk = union_pays(valid && ha == ha, p, Kd * ha);
@@ -332,7 +333,7 @@ inline int batch_wave_doubles(const CatChunk& c, bool c32) {
 }
 
 template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT, bool C32>
-void launch_batch_t(BatchState& st, const BucketView& bv, const BucketView& bv8, const CatChunk& c,
+void launch_batch_t(BatchState& st, const BucketView& bv, const BucketView& bv8, const BucketView& nv, const CatChunk& c,
                     unsigned long long* ctr, int num_cu, hipStream_t s) {
     auto kern = k_batch<K, STRAT, MS, MG, MD, MC, EXACT, C32>;
     const size_t dyn = (size_t)(kBlock / kWave) * st.wave_doubles * sizeof(double);
@@ -340,40 +341,40 @@ void launch_batch_t(BatchState& st, const BucketView& bv, const BucketView& bv8,
     PH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, dyn));
     const i64 want = (i64)(per_cu > 0 ? per_cu : 1) * num_cu;
     const unsigned g = (unsigned)(want < st.units ? want : (st.units > 0 ? st.units : 1));
-    kern<<<g, kBlock, dyn, s>>>(bv, bv.recs, bv8, bv8.recs, st.rec.as<double>(), c, st.reqs.as<BatchReq>(), st.nreq,
+    kern<<<g, kBlock, dyn, s>>>(bv, bv.recs, bv8, bv8.recs, nv, nv.recs, st.rec.as<double>(), c, st.reqs.as<BatchReq>(), st.nreq,
                                 st.units, ctr, st.times.as<unsigned long long>(), st.wave_doubles);
     PH_HIP(hipGetLastError());
 }
 
 template <int K, int STRAT>
-void launch_batch_ks(BatchState& st, const BucketView& bv, const BucketView& bv8, const CatChunk& c,
+void launch_batch_ks(BatchState& st, const BucketView& bv, const BucketView& bv8, const BucketView& nv, const CatChunk& c,
                      unsigned long long* ctr, int num_cu, hipStream_t s) {
     const bool small = c.ng + c.nd + c.nc == 0 && c.ns <= 2;
     if (st.precision == PH_PREC_FP64) {
-        if (small) launch_batch_t<K, STRAT, 2, 0, 0, 0, 0, false>(st, bv, bv8, c, ctr, num_cu, s);
-        else launch_batch_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0, false>(st, bv, bv8, c, ctr, num_cu, s);
+        if (small) launch_batch_t<K, STRAT, 2, 0, 0, 0, 0, false>(st, bv, bv8, nv, c, ctr, num_cu, s);
+        else launch_batch_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0, false>(st, bv, bv8, nv, c, ctr, num_cu, s);
     } else {  // the per-call selection (ph_bucket.hip, ph_tile.hip), so the arithmetic is the same
-        if (small) launch_batch_t<K, STRAT, 2, 0, 0, 0, 0, true>(st, bv, bv8, c, ctr, num_cu, s);
+        if (small) launch_batch_t<K, STRAT, 2, 0, 0, 0, 0, true>(st, bv, bv8, nv, c, ctr, num_cu, s);
         else if (c2_shared(c) && c.rw == kC2Rec)
-            launch_batch_t<K, STRAT, 2, 1, 1, 1, 2, true>(st, bv, bv8, c, ctr, num_cu, s);
-        else launch_batch_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0, true>(st, bv, bv8, c, ctr, num_cu, s);
+            launch_batch_t<K, STRAT, 2, 1, 1, 1, 2, true>(st, bv, bv8, nv, c, ctr, num_cu, s);
+        else launch_batch_t<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0, true>(st, bv, bv8, nv, c, ctr, num_cu, s);
     }
 }
 
 template <int K>
-void launch_batch_k(BatchState& st, const BucketView& bv, const BucketView& bv8, const CatChunk& c,
+void launch_batch_k(BatchState& st, const BucketView& bv, const BucketView& bv8, const BucketView& nv, const CatChunk& c,
                     unsigned long long* ctr, int num_cu, hipStream_t s) {
     switch (st.strategy) {
-    case PH_GATHER: launch_batch_ks<K, PH_GATHER>(st, bv, bv8, c, ctr, num_cu, s); break;
-    case PH_SCATTER: launch_batch_ks<K, PH_SCATTER>(st, bv, bv8, c, ctr, num_cu, s); break;
-    default: launch_batch_ks<K, PH_SYMMETRIC>(st, bv, bv8, c, ctr, num_cu, s); break;
+    case PH_GATHER: launch_batch_ks<K, PH_GATHER>(st, bv, bv8, nv, c, ctr, num_cu, s); break;
+    case PH_SCATTER: launch_batch_ks<K, PH_SCATTER>(st, bv, bv8, nv, c, ctr, num_cu, s); break;
+    default: launch_batch_ks<K, PH_SYMMETRIC>(st, bv, bv8, nv, c, ctr, num_cu, s); break;
     }
 }
 
 // Records the launch sequence of one run on stream s.  ctl[0] = launch
 // timestamp, ctl[1 + q] = work counter of catalog chunk q.
 void record(BatchState& st, ph_lbvh* B, int num_cu, hipStream_t s) {
-    const BucketView bv = bucket_view(B), bv8 = bucket_view(B, kBucket);
+    const BucketView bv = bucket_view(B), bv8 = bucket_view(B, kBucket), nv = nearest_view(B);
     unsigned long long* ctl = st.ctl.as<unsigned long long>();
     unsigned long long* times = st.times.as<unsigned long long>();
     const i64 n = B->n;
@@ -390,12 +391,12 @@ void record(BatchState& st, ph_lbvh* B, int num_cu, hipStream_t s) {
         PH_HIP(hipGetLastError());
         if (st.units == 0) continue;
         switch (B->kernel) {
-        case PH_KERNEL_M4: launch_batch_k<PH_KERNEL_M4>(st, bv, bv8, c, ctr, num_cu, s); break;
-        case PH_KERNEL_M5: launch_batch_k<PH_KERNEL_M5>(st, bv, bv8, c, ctr, num_cu, s); break;
-        case PH_KERNEL_M6: launch_batch_k<PH_KERNEL_M6>(st, bv, bv8, c, ctr, num_cu, s); break;
-        case PH_KERNEL_C2: launch_batch_k<PH_KERNEL_C2>(st, bv, bv8, c, ctr, num_cu, s); break;
-        case PH_KERNEL_C4: launch_batch_k<PH_KERNEL_C4>(st, bv, bv8, c, ctr, num_cu, s); break;
-        default: launch_batch_k<PH_KERNEL_C6>(st, bv, bv8, c, ctr, num_cu, s); break;
+        case PH_KERNEL_M4: launch_batch_k<PH_KERNEL_M4>(st, bv, bv8, nv, c, ctr, num_cu, s); break;
+        case PH_KERNEL_M5: launch_batch_k<PH_KERNEL_M5>(st, bv, bv8, nv, c, ctr, num_cu, s); break;
+        case PH_KERNEL_M6: launch_batch_k<PH_KERNEL_M6>(st, bv, bv8, nv, c, ctr, num_cu, s); break;
+        case PH_KERNEL_C2: launch_batch_k<PH_KERNEL_C2>(st, bv, bv8, nv, c, ctr, num_cu, s); break;
+        case PH_KERNEL_C4: launch_batch_k<PH_KERNEL_C4>(st, bv, bv8, nv, c, ctr, num_cu, s); break;
+        default: launch_batch_k<PH_KERNEL_C6>(st, bv, bv8, nv, c, ctr, num_cu, s); break;
         }
     }
 }

@@ -13,33 +13,38 @@ namespace {
 
 // Per internal node i: big_s[split - 1] = (leaves of i > kBucket); for a big node,
 // bs[first leaf - 1] = 1 for each child that is a bucket (at most kBucket leaves).
-// Split and bucket-start flags of up to two bucket trees (bucket sizes b0 and
-// b1; b1 = 0: one tree), each tree's segments at big_s[t] and bs[t].
+// Up to four bucket trees are built together (kMaxTrees); tree t's flag
+// segments (see build_bucket_trees) are big_s[t] and bs[t].
+constexpr int kMaxTrees = 4;
+struct FlagTrees {
+    u32* big_s[kMaxTrees];
+    u32* bs[kMaxTrees];
+    int bmax[kMaxTrees];
+    int ntrees;
+};
+
+// Split and bucket-start flags of every tree.
 __global__ void __launch_bounds__(kBlock) k_bucket_flags(i64 n, const i32* __restrict__ left,
-                                                 const i32* __restrict__ range, u32* __restrict__ big_s0,
-                                                 u32* __restrict__ bs0, u32* __restrict__ big_s1,
-                                                 u32* __restrict__ bs1, int b0, int b1) {
+                                                 const i32* __restrict__ range, FlagTrees ft) {
     const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
     if (i > n - 1) return;
     const i64 lo = range[2 * (i - 1)], hi = range[2 * (i - 1) + 1];
     const i64 l = left[i - 1];
     const i64 s = l >= n ? lo : l;  // split: the left child covers [lo, s]
-    u32* const big_s[2] = {big_s0, big_s1};
-    u32* const bs[2] = {bs0, bs1};
-    const int bmax[2] = {b0, b1};
-    for (int t = 0; t < (b1 > 0 ? 2 : 1); ++t) {
-        const bool big = hi - lo + 1 > bmax[t];
-        big_s[t][s - 1] = big ? 1u : 0u;
+    for (int t = 0; t < ft.ntrees; ++t) {
+        const int bmax = ft.bmax[t];
+        const bool big = hi - lo + 1 > bmax;
+        ft.big_s[t][s - 1] = big ? 1u : 0u;
         if (big) {
-            if (s - lo + 1 <= bmax[t]) bs[t][lo - 1] = 1u;
-            if (hi - s <= bmax[t]) bs[t][s] = 1u;
+            if (s - lo + 1 <= bmax) ft.bs[t][lo - 1] = 1u;
+            if (hi - s <= bmax) ft.bs[t][s] = 1u;
         }
     }
 }
 
 // The scanned flag array's values at the segment starts of ntrees trees and
 // at its closing entry (info[0 .. 2 ntrees]), and the build's refit check
-// count (info[5], 0 without one).
+// count (info[11], 0 without one).
 __global__ void k_bucket_info(const u32* __restrict__ scanned, i64 n, int ntrees, const u32* __restrict__ refit_bad,
                               u32* __restrict__ info) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
@@ -49,7 +54,7 @@ __global__ void k_bucket_info(const u32* __restrict__ scanned, i64 n, int ntrees
         info[2 * t + 1] = scanned ? scanned[t * per_tree + n] : 0u;
     }
     info[2 * ntrees] = scanned ? scanned[ntrees * per_tree] : 0u;
-    info[5] = refit_bad ? *refit_bad : 0u;
+    info[11] = refit_bad ? *refit_bad : 0u;
 }
 
 __device__ inline void round_box(const double lo[3], const double hi[3], double scale, BucketRec& R) {
@@ -89,16 +94,20 @@ struct EmitTree {
     int bmax;
 };
 
-// Both trees in one pass: L(u) is a property of the LBVH, found once.
+struct EmitTrees {
+    EmitTree t[kMaxTrees];
+    int ntrees;  // the trees with at least one split, smallest buckets first
+};
+
+// Every tree in one pass: L(u) is a property of the LBVH, found once.
 __global__ void __launch_bounds__(kBlock) k_bucket_emit(i64 n, i32 root, const i32* __restrict__ left,
                                                 const i32* __restrict__ parent, const i32* __restrict__ range,
                                                 const NodeRec<double>* __restrict__ nodes,
-                                                const LeafRec<double>* __restrict__ leaves, EmitTree t0, EmitTree t1,
-                                                int ntrees) {
+                                                const LeafRec<double>* __restrict__ leaves, EmitTrees et) {
     const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
     if (i > n - 1) return;
     const i64 lo = range[2 * (i - 1)], hi = range[2 * (i - 1) + 1];
-    if (hi - lo + 1 <= t0.bmax) return;  // a bucket (or inside one) in both trees
+    if (hi - lo + 1 <= et.t[0].bmax) return;  // a bucket (or inside one) in every tree
     i64 Lc = 0;
     for (i64 c = i, steps = n; c != root && steps > 0; --steps) {
         const i64 p = parent[c - 1];
@@ -107,8 +116,8 @@ __global__ void __launch_bounds__(kBlock) k_bucket_emit(i64 n, i32 root, const i
     }
     const i64 l = left[i - 1];
     const i64 s = l >= n ? lo : l;
-    for (int t = 0; t < ntrees; ++t) {
-        const EmitTree& T = t == 0 ? t0 : t1;
+    for (int t = 0; t < et.ntrees; ++t) {
+        const EmitTree& T = et.t[t];
         if (hi - lo + 1 <= T.bmax) continue;
         auto PB = [&](i64 k) { return (i64)(T.PB[k] - T.pb0); };
         auto PS = [&](i64 k) { return (i64)(T.PS[k] - T.ps0); };
@@ -291,7 +300,7 @@ void launch_bucket_ks(ph_lbvh* B, const CatChunk& c, int precision, const PointL
     // nearest-particle smoothing length of every sample (gather / symmetric)
     const double* ha = nullptr;
     if (STRAT != PH_SCATTER) {
-        const BucketView bv = bucket_view(B);
+        const BucketView bv = nearest_view(B);
         const unsigned g = sample_blocks(ga, S);
         const int xm = point_xcd(tl_launch);
         const double Kd = support_of<K>();
@@ -351,14 +360,15 @@ bool buckets_supported(const ph_lbvh* B) {
     return B->dims == 3 && B->precision == PH_PREC_FP64 && B->n <= kMaxBucketLeaves;
 }
 
-// Builds one or two bucket trees of B (bucket sizes bmax[0] < bmax[1]) into
-// recs[t], nrec[t]: one flag pass, one scan, one read-back of the record counts
-// (with the build's refit check when the build left it here), one emit pass.
+// Builds one to four bucket trees of B (bucket sizes bmax[0] < bmax[1] < ...)
+// into recs[t], nrec[t]: one flag pass, one scan, one read-back of the record
+// counts (with the build's refit check when the build left it here), one emit
+// pass.
 void build_bucket_trees(ph_lbvh* B, hipStream_t s, int ntrees, const int* bmax, DevBuf* const* recs, i64* nrec,
                         const u32* refit_bad) {
     const i64 n = B->n;
     if (n > kMaxBucketLeaves) throw Error{PH_ERR_CAPACITY, "the bucket traversal supports at most 2^30 particles"};
-    bool split[2] = {false, false};  // split[1] implies split[0]
+    bool split[kMaxTrees] = {false, false, false, false};  // split[t + 1] implies split[t]
     for (int t = 0; t < ntrees; ++t) {
         split[t] = n > bmax[t];
         nrec[t] = 1;
@@ -370,17 +380,20 @@ void build_bucket_trees(ph_lbvh* B, hipStream_t s, int ntrees, const int* bmax, 
     const i64 per_tree = 2 * n + 1, m = ntrees * per_tree + 1;
     DevBuf flags, tmp, info;
     if (split[0]) flags.alloc(sizeof(u32) * (size_t)m);
-    if (split[0] || refit_bad) info.alloc(sizeof(u32) * 8);
+    if (split[0] || refit_bad) info.alloc(sizeof(u32) * 12);
     u32* f = flags.as<u32>();
-    u32* big_s[2] = {f, f + per_tree};
-    u32* bs[2] = {f + n, f + per_tree + n};
-    u32 h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    FlagTrees ft{};
+    ft.ntrees = ntrees;
+    for (int t = 0; t < ntrees; ++t) {
+        ft.big_s[t] = f + t * per_tree;
+        ft.bs[t] = f + t * per_tree + n;
+        ft.bmax[t] = bmax[t];
+    }
+    u32 h[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (split[0] || refit_bad) {
         if (split[0]) {
             PH_HIP(hipMemsetAsync(f, 0, sizeof(u32) * (size_t)m, s));
-            k_bucket_flags<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->left.as<i32>(), B->range.as<i32>(), big_s[0],
-                                                             bs[0], big_s[1], bs[1], bmax[0],
-                                                             ntrees > 1 ? bmax[1] : 0);
+            k_bucket_flags<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->left.as<i32>(), B->range.as<i32>(), ft);
             PH_HIP(hipGetLastError());
             exclusive_sum_u32(f, m, tmp, s);
         }
@@ -388,9 +401,9 @@ void build_bucket_trees(ph_lbvh* B, hipStream_t s, int ntrees, const int* bmax, 
         PH_HIP(hipGetLastError());
         PH_HIP(hipMemcpyAsync(h, info.p, sizeof(h), hipMemcpyDeviceToHost, s));
         PH_HIP(hipStreamSynchronize(s));
-        if (h[5] != 0)
+        if (h[11] != 0)
             throw Error{PH_ERR_INTERNAL,
-                        "LinearBVH refit: " + std::to_string(h[5]) + " internal nodes were not reached twice"};
+                        "LinearBVH refit: " + std::to_string(h[11]) + " internal nodes were not reached twice"};
     }
     for (int t = 0; t < ntrees; ++t) {
         if (split[t]) nrec[t] = (i64)(h[2 * t + 1] - h[2 * t]) + (i64)(h[2 * t + 2] - h[2 * t + 1]);
@@ -406,14 +419,15 @@ void build_bucket_trees(ph_lbvh* B, hipStream_t s, int ntrees, const int* bmax, 
             PH_HIP(hipGetLastError());
         }
     if (split[0]) {
-        EmitTree et[2];
-        for (int t = 0; t < 2; ++t)
-            et[t] = t < ntrees ? EmitTree{big_s[t], bs[t], h[2 * t], h[2 * t + 1], recs[t]->as<BucketRec>(), bmax[t]}
-                               : et[0];
+        EmitTrees et{};
+        for (int t = 0; t < ntrees; ++t)
+            if (split[t]) {
+                et.t[et.ntrees++] = EmitTree{ft.big_s[t], ft.bs[t], h[2 * t], h[2 * t + 1], recs[t]->as<BucketRec>(),
+                                             bmax[t]};
+            }
         k_bucket_emit<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->root, B->left.as<i32>(), B->parent.as<i32>(),
                                                         B->range.as<i32>(), B->nodes.as<NodeRec<double>>(),
-                                                        B->leaves.as<LeafRec<double>>(), et[0], et[1],
-                                                        split[1] ? 2 : 1);
+                                                        B->leaves.as<LeafRec<double>>(), et);
         PH_HIP(hipGetLastError());
     }
     // flags / tmp / info are released on return: into the context's cache,
@@ -422,27 +436,24 @@ void build_bucket_trees(ph_lbvh* B, hipStream_t s, int ntrees, const int* bmax, 
 }
 
 void ensure_buckets(ph_lbvh* B, hipStream_t s, const u32* refit_bad) {
-    // Two bucket trees per LBVH, built together: buckets of at most kBucket
-    // leaves (the grid tile kernels) and of at most kBucketSmall (the per-lane
-    // kernels, the batch queue, adjoints and queries).
-    if (B->buckets_ready && B->buckets8_ready) return;
-    const int bmax[2] = {kBucketSmall, kBucket};
-    DevBuf* recs[2] = {&B->brecs, &B->brecs8};
-    i64 nrec[2] = {1, 1};
-    build_bucket_trees(B, s, 2, bmax, recs, nrec, refit_bad);
-    B->nrec = nrec[0];
-    B->nrec8 = nrec[1];
-    B->buckets_ready = B->buckets8_ready = true;
-}
-
-void ensure_buckets16(ph_lbvh* B, hipStream_t s) {
-    if (B->buckets16_ready) return;
-    const int bmax[1] = {kBucketLarge};
-    DevBuf* recs[1] = {&B->brecs16};
-    i64 nrec[1] = {1};
-    build_bucket_trees(B, s, 1, bmax, recs, nrec, nullptr);
-    B->nrec16 = nrec[0];
-    B->buckets16_ready = true;
+    // The bucket trees of an LBVH, built together: buckets of one leaf (the
+    // nearest-h searches: the LBVH itself in preorder with float32 boxes; left
+    // out when its 2n - 1 records would not fit int32 indices), of at most
+    // kBucketSmall leaves (the per-lane kernels, the batch queue, adjoints and
+    // queries), of at most kBucket and kBucketLarge (the grid tile walks).
+    if (B->buckets_ready) return;
+    const bool leaf_tree = 2 * B->n - 1 <= (i64)INT32_MAX - 16;
+    const int bmax[4] = {kBucketLeaf, kBucketSmall, kBucket, kBucketLarge};
+    DevBuf* recs[4] = {&B->brecs1, &B->brecs, &B->brecs8, &B->brecs16};
+    i64 nrec[4] = {1, 1, 1, 1};
+    const int skip = leaf_tree ? 0 : 1;
+    build_bucket_trees(B, s, 4 - skip, bmax + skip, recs + skip, nrec + skip, refit_bad);
+    B->nrec1 = leaf_tree ? nrec[0] : 0;
+    B->nrec = nrec[1];
+    B->nrec8 = nrec[2];
+    B->nrec16 = nrec[3];
+    B->buckets1_ready = leaf_tree;
+    B->buckets_ready = true;
 }
 
 void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, const PointList* pl, const GridAxes* ga,

@@ -734,7 +734,8 @@ __device__ __attribute__((always_inline)) void bucket_f64_sample(const BucketVie
     if (valid) acc.finish(c, out, S, s);
 }
 
-constexpr int kBucketLarge = 16;  // the third tree, for grid tile walks only
+constexpr int kBucketLarge = 16;  // the largest buckets, for grid tile walks only
+constexpr int kBucketLeaf = 1;    // buckets of one leaf: the nearest-h searches' tree
 
 // Whether the per-lane FP32C walk needs 64-bit addresses: the leaves or the
 // records (rw doubles each, plus the zero record) pass 4 GiB.
@@ -742,7 +743,16 @@ inline bool c32_wide(i64 n, int rw) {
     return (n + 1) * (i64)rw * 8 > (i64)0xffffffff || n * (i64)sizeof(LeafRec<double>) > (i64)0xffffffff;
 }
 
-BucketView bucket_view(const ph_lbvh* B, int leaves = kBucketSmall) {
+BucketView bucket_view(const ph_lbvh* B, int leaves = kBucketSmall);
+
+// The tree of the nearest-h searches: the one-leaf bucket tree when it was
+// built, else the kBucketSmall one (the same results: the search is exact over
+// any bucket tree of the LBVH).
+BucketView nearest_view(const ph_lbvh* B) { return bucket_view(B, B->buckets1_ready ? kBucketLeaf : kBucketSmall); }
+
+BucketView bucket_view(const ph_lbvh* B, int leaves) {
+    if (leaves == kBucketLeaf)
+        return BucketView{B->n, (i32)B->nrec1, B->brecs1.as<BucketRec>(), B->leaves.as<LeafRec<double>>()};
     if (leaves == kBucketLarge)
         return BucketView{B->n, (i32)B->nrec16, B->brecs16.as<BucketRec>(), B->leaves.as<LeafRec<double>>()};
     if (leaves == kBucket)

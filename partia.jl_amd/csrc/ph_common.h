@@ -247,15 +247,14 @@ struct ph_lbvh {
     ph::DevBuf ha;           // per-sample smoothing length of the nearest particle
     ph::DevBuf nn;           // per-sample Morton index of the nearest particle (adjoints)
     // Bucket trees (built with the tree, ensure_buckets): the LBVH in DFS
-    // preorder with every subtree of at most kBucketSmall (4) leaves, resp.
-    // kBucket (8) leaves for the grid tile kernels, collapsed into one bucket.
-    bool buckets_ready = false;
-    int64_t nrec = 0;
-    bool buckets8_ready = false;
-    int64_t nrec8 = 0;
+    // preorder with every subtree of at most 1 (the nearest-h searches),
+    // kBucketSmall (4; the per-lane kernels), kBucket (8) or kBucketLarge (16)
+    // leaves (the grid tile walks) collapsed into one bucket.
+    bool buckets_ready = false;    // all of them (the one-leaf tree if buckets1_ready)
+    bool buckets1_ready = false;
+    int64_t nrec = 0, nrec1 = 0, nrec8 = 0, nrec16 = 0;
+    ph::DevBuf brecs1;    // one-leaf bucket-tree records [nrec1]
     ph::DevBuf brecs8;    // 8-leaf bucket-tree records [nrec8]
-    bool buckets16_ready = false;  // built on first use (a 16-leaf tile walk)
-    int64_t nrec16 = 0;
     ph::DevBuf brecs16;   // 16-leaf bucket-tree records [nrec16]
     ph::DevBuf brecs;     // 4-leaf bucket-tree records [nrec]
     ph::DevBuf rec32;     // FP32-compensated records of the last call ({m/rho, 1/h}, columns)
@@ -264,14 +263,11 @@ struct ph_lbvh {
     // valid for ph_lbvh_destroy only).
     void release_device() {
         for (ph::DevBuf* b : {&cols, &codes, &order, &left, &right, &escape, &parent, &range, &nodes, &leaves,
-                              &leaf_escape, &rec, &ha, &nn, &brecs, &brecs8, &brecs16, &rec32})
+                              &leaf_escape, &rec, &ha, &nn, &brecs, &brecs8, &brecs16, &brecs1, &rec32})
             b->release();
         buckets_ready = false;
-        buckets8_ready = false;
-        buckets16_ready = false;
-        nrec = 0;
-        nrec8 = 0;
-        nrec16 = 0;
+        buckets1_ready = false;
+        nrec = nrec1 = nrec8 = nrec16 = 0;
     }
 };
 

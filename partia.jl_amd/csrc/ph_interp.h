@@ -89,8 +89,6 @@ void launch_bucket(ph_lbvh* B, const CatChunk& c, int strategy, int precision, c
 // refit_bad: the build's refit check count on the device, read back with the
 // record counts (0 expected), or null.
 void ensure_buckets(ph_lbvh* B, hipStream_t s, const u32* refit_bad = nullptr);
-// The 16-leaf bucket tree of the grid tile walks, built on first use.
-void ensure_buckets16(ph_lbvh* B, hipStream_t s);
 bool buckets_supported(const ph_lbvh* B);
 // Grid interpolation with one bucket-tree walk per wave (ph_tile.hip), FP32C
 // (float outputs) or float64 (precision PH_PREC_FP64, double outputs); the

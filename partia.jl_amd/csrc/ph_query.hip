@@ -78,7 +78,7 @@ void launch_query(ph_lbvh* B, const PointList& pl, i64 S, const double* radius, 
 
 void launch_find_nearest(ph_lbvh* B, const PointList& pl, i64 S, i64* idx, double* d2, hipStream_t s) {
     ensure_buckets(B, s);
-    k_find_nearest<<<(unsigned)((S + kBlock - 1) / kBlock), kBlock, 0, s>>>(bucket_view(B), pl, S, idx, d2);
+    k_find_nearest<<<(unsigned)((S + kBlock - 1) / kBlock), kBlock, 0, s>>>(nearest_view(B), pl, S, idx, d2);
     PH_HIP(hipGetLastError());
 }
 
