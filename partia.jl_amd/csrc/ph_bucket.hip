@@ -97,25 +97,80 @@ struct EmitTree {
 struct EmitTrees {
     EmitTree t[kMaxTrees];
     int ntrees;  // the trees with at least one split, smallest buckets first
+    // the one-leaf tree (the LBVH in preorder), or null: every internal node
+    // is big and every leaf a bucket, so PB(k) = PS(k) = k and its positions
+    // need no flags
+    BucketRec* leaf_recs;
 };
 
-// Every tree in one pass: L(u) is a property of the LBVH, found once.
-__global__ void __launch_bounds__(kBlock) k_bucket_emit(i64 n, i32 root, const i32* __restrict__ left,
-                                                const i32* __restrict__ parent, const i32* __restrict__ range,
-                                                const NodeRec<double>* __restrict__ nodes,
+// L(u) by pointer jumping: after k_lc_init and r rounds of k_lc_jump, up[u]
+// is the ancestor 2^(r+1) edges above internal node u (0 past the root) and
+// lc[u] counts the left-child edges on the way there, u's own edge included.
+__device__ inline i32 lc_edge(i32 c, i32 root, const i32* __restrict__ parent, const i32* __restrict__ left,
+                              i32& up) {
+    up = c == root ? 0 : parent[c - 1];
+    return (up != 0 && left[up - 1] == c) ? 1 : 0;
+}
+__global__ void __launch_bounds__(kBlock) k_lc_init(i64 n, i32 root, const i32* __restrict__ parent,
+                                            const i32* __restrict__ left, i32* __restrict__ up,
+                                            i32* __restrict__ lc) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
+    if (i > n - 1) return;
+    i32 u1 = 0, u2 = 0;
+    i32 e = lc_edge((i32)i, root, parent, left, u1);
+    if (u1 != 0) e += lc_edge(u1, root, parent, left, u2);
+    up[i - 1] = u2;
+    lc[i - 1] = e;
+}
+__global__ void __launch_bounds__(kBlock) k_lc_jump(i64 n, const i32* __restrict__ up_in,
+                                            const i32* __restrict__ lc_in, i32* __restrict__ up_out,
+                                            i32* __restrict__ lc_out) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
+    if (i > n - 1) return;
+    const i32 u = up_in[i - 1];
+    up_out[i - 1] = u == 0 ? 0 : up_in[u - 1];
+    lc_out[i - 1] = lc_in[i - 1] + (u == 0 ? 0 : lc_in[u - 1]);
+}
+constexpr int kLcJumps = 2;  // pointers of 8 edges
+
+// Every tree in one pass: L(u) and the boxes are properties of the LBVH,
+// found once per node.
+__global__ void __launch_bounds__(kBlock) k_bucket_emit(i64 n, const i32* __restrict__ left,
+                                                const i32* __restrict__ range, const i32* __restrict__ up,
+                                                const i32* __restrict__ lc, const NodeRec<double>* __restrict__ nodes,
                                                 const LeafRec<double>* __restrict__ leaves, EmitTrees et) {
     const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
     if (i > n - 1) return;
     const i64 lo = range[2 * (i - 1)], hi = range[2 * (i - 1) + 1];
-    if (hi - lo + 1 <= et.t[0].bmax) return;  // a bucket (or inside one) in every tree
-    i64 Lc = 0;
-    for (i64 c = i, steps = n; c != root && steps > 0; --steps) {
-        const i64 p = parent[c - 1];
-        if (left[p - 1] == c) ++Lc;
-        c = p;
-    }
+    // a bucket (or inside one) in every tree
+    if (!et.leaf_recs && (et.ntrees == 0 || hi - lo + 1 <= et.t[0].bmax)) return;
     const i64 l = left[i - 1];
     const i64 s = l >= n ? lo : l;
+    BucketRec Ri, Rl, Rr;  // the node, its left and its right child
+    node_box(i, n, nodes, leaves, Ri);
+    node_box(s == lo ? (lo + n - 1) : s, n, nodes, leaves, Rl);
+    node_box(s + 1 == hi ? (hi + n - 1) : s + 1, n, nodes, leaves, Rr);
+    i64 Lc = lc[i - 1];
+    for (i64 c = up[i - 1], steps = n; c != 0 && steps > 0; --steps) {
+        Lc += lc[c - 1];
+        c = up[c - 1];
+    }
+    if (et.leaf_recs) {
+        const i64 pre = Lc + 2 * (lo - 1);
+        BucketRec R = Ri;
+        R.link = (i32)(pre + 2 * (hi - lo) + 1);
+        et.leaf_recs[pre] = R;
+        if (s == lo) {  // left child a leaf, right after its parent
+            R = Rl;
+            encode_bucket(R, lo - 1, 1);
+            et.leaf_recs[pre + 1] = R;
+        }
+        if (s + 1 == hi) {
+            R = Rr;
+            encode_bucket(R, s, 1);
+            et.leaf_recs[Lc + 2 * s] = R;
+        }
+    }
     for (int t = 0; t < et.ntrees; ++t) {
         const EmitTree& T = et.t[t];
         if (hi - lo + 1 <= T.bmax) continue;
@@ -123,18 +178,17 @@ __global__ void __launch_bounds__(kBlock) k_bucket_emit(i64 n, i32 root, const i
         auto PS = [&](i64 k) { return (i64)(T.PS[k] - T.ps0); };
         const i64 pre = Lc + PB(lo - 1) + PS(lo - 1);
         const i64 cnt = (PB(hi - 1) - PB(lo - 1)) + (PS(hi) - PS(lo - 1));
-        BucketRec R;
-        node_box(i, n, nodes, leaves, R);
+        BucketRec R = Ri;
         R.link = (i32)(pre + cnt);
         T.recs[pre] = R;
         if (s - lo + 1 <= T.bmax) {  // left child is a bucket, right after its parent
-            node_box(s == lo ? (lo + n - 1) : s, n, nodes, leaves, R);
+            R = Rl;
             encode_bucket(R, lo - 1, s - lo + 1);
             T.recs[pre + 1] = R;
         }
         if (hi - s <= T.bmax) {
             const i64 rpre = Lc + PB(s) + PS(s);
-            node_box(s + 1 == hi ? (hi + n - 1) : s + 1, n, nodes, leaves, R);
+            R = Rr;
             encode_bucket(R, s, hi - s);
             T.recs[rpre] = R;
         }
@@ -361,9 +415,9 @@ bool buckets_supported(const ph_lbvh* B) {
 }
 
 // Builds one to four bucket trees of B (bucket sizes bmax[0] < bmax[1] < ...)
-// into recs[t], nrec[t]: one flag pass, one scan, one read-back of the record
-// counts (with the build's refit check when the build left it here), one emit
-// pass.
+// into recs[t], nrec[t]: one flag pass and one scan for the trees of more than
+// one leaf per bucket, one read-back of their record counts (with the build's
+// refit check when the build left it here), one emit pass for all of them.
 void build_bucket_trees(ph_lbvh* B, hipStream_t s, int ntrees, const int* bmax, DevBuf* const* recs, i64* nrec,
                         const u32* refit_bad) {
     const i64 n = B->n;
@@ -373,31 +427,52 @@ void build_bucket_trees(ph_lbvh* B, hipStream_t s, int ntrees, const int* bmax, 
         split[t] = n > bmax[t];
         nrec[t] = 1;
     }
+    // the one-leaf tree has 2n - 1 records in closed form
+    const int t0 = ntrees > 0 && bmax[0] == kBucketLeaf ? 1 : 0;
+    if (t0 && split[0]) nrec[0] = 2 * n - 1;
+    const int nf = ntrees - t0;  // trees with flags
     // buffers first.  Flags of the trees in one array, scanned as one: per
     // tree the big splits (n entries: n - 1 flags + the total) and the bucket
     // starts (n + 1 entries: n flags + the total), then one closing entry; a
     // segment's exclusive prefix is its scanned value minus that at its start.
-    const i64 per_tree = 2 * n + 1, m = ntrees * per_tree + 1;
-    DevBuf flags, tmp, info;
-    if (split[0]) flags.alloc(sizeof(u32) * (size_t)m);
-    if (split[0] || refit_bad) info.alloc(sizeof(u32) * 12);
+    const bool flagged = nf > 0 && split[t0];
+    const i64 per_tree = 2 * n + 1, m = nf * per_tree + 1;
+    DevBuf flags, tmp, info, lcb;
+    if (flagged) flags.alloc(sizeof(u32) * (size_t)m);
+    if (split[0]) lcb.alloc(sizeof(i32) * 4 * (size_t)(n - 1));
+    if (flagged || refit_bad) info.alloc(sizeof(u32) * 12);
     u32* f = flags.as<u32>();
     FlagTrees ft{};
-    ft.ntrees = ntrees;
-    for (int t = 0; t < ntrees; ++t) {
+    ft.ntrees = nf;
+    for (int t = 0; t < nf; ++t) {
         ft.big_s[t] = f + t * per_tree;
         ft.bs[t] = f + t * per_tree + n;
-        ft.bmax[t] = bmax[t];
+        ft.bmax[t] = bmax[t0 + t];
+    }
+    // L(u) pointers (before the read-back, so the host waits for them too)
+    i32* lcp = nullptr;  // up, then lc, of the last round
+    if (split[0]) {
+        i32* a = lcb.as<i32>();
+        i32* b = a + 2 * (n - 1);
+        k_lc_init<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->root, B->parent.as<i32>(), B->left.as<i32>(), a,
+                                                    a + (n - 1));
+        PH_HIP(hipGetLastError());
+        for (int r = 0; r < kLcJumps; ++r) {
+            k_lc_jump<<<grid_for(n - 1), kBlock, 0, s>>>(n, a, a + (n - 1), b, b + (n - 1));
+            PH_HIP(hipGetLastError());
+            std::swap(a, b);
+        }
+        lcp = a;
     }
     u32 h[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (split[0] || refit_bad) {
-        if (split[0]) {
+    if (flagged || refit_bad) {
+        if (flagged) {
             PH_HIP(hipMemsetAsync(f, 0, sizeof(u32) * (size_t)m, s));
             k_bucket_flags<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->left.as<i32>(), B->range.as<i32>(), ft);
             PH_HIP(hipGetLastError());
             exclusive_sum_u32(f, m, tmp, s);
         }
-        k_bucket_info<<<1, 1, 0, s>>>(split[0] ? f : nullptr, n, ntrees, refit_bad, info.as<u32>());
+        k_bucket_info<<<1, 1, 0, s>>>(flagged ? f : nullptr, n, nf, refit_bad, info.as<u32>());
         PH_HIP(hipGetLastError());
         PH_HIP(hipMemcpyAsync(h, info.p, sizeof(h), hipMemcpyDeviceToHost, s));
         PH_HIP(hipStreamSynchronize(s));
@@ -405,12 +480,10 @@ void build_bucket_trees(ph_lbvh* B, hipStream_t s, int ntrees, const int* bmax, 
             throw Error{PH_ERR_INTERNAL,
                         "LinearBVH refit: " + std::to_string(h[11]) + " internal nodes were not reached twice"};
     }
-    for (int t = 0; t < ntrees; ++t) {
-        if (split[t]) nrec[t] = (i64)(h[2 * t + 1] - h[2 * t]) + (i64)(h[2 * t + 2] - h[2 * t + 1]);
-        // record indices and escape links are int32
-        if (nrec[t] > (i64)INT32_MAX - 16)
-            throw Error{PH_ERR_CAPACITY, "bucket tree of more than 2^31 - 16 records"};
-    }
+    for (int t = 0; t < nf; ++t)
+        if (split[t0 + t]) nrec[t0 + t] = (i64)(h[2 * t + 1] - h[2 * t]) + (i64)(h[2 * t + 2] - h[2 * t + 1]);
+    for (int t = 0; t < ntrees; ++t)  // record indices and escape links are int32
+        if (nrec[t] > (i64)INT32_MAX - 16) throw Error{PH_ERR_CAPACITY, "bucket tree of more than 2^31 - 16 records"};
     for (int t = 0; t < ntrees; ++t) recs[t]->alloc(sizeof(BucketRec) * (size_t)nrec[t]);
     for (int t = 0; t < ntrees; ++t)
         if (!split[t]) {  // the whole tree is one bucket
@@ -420,13 +493,14 @@ void build_bucket_trees(ph_lbvh* B, hipStream_t s, int ntrees, const int* bmax, 
         }
     if (split[0]) {
         EmitTrees et{};
-        for (int t = 0; t < ntrees; ++t)
-            if (split[t]) {
-                et.t[et.ntrees++] = EmitTree{ft.big_s[t], ft.bs[t], h[2 * t], h[2 * t + 1], recs[t]->as<BucketRec>(),
-                                             bmax[t]};
+        et.leaf_recs = t0 ? recs[0]->as<BucketRec>() : nullptr;
+        for (int t = 0; t < nf; ++t)
+            if (split[t0 + t]) {
+                et.t[et.ntrees++] = EmitTree{ft.big_s[t], ft.bs[t], h[2 * t], h[2 * t + 1],
+                                             recs[t0 + t]->as<BucketRec>(), bmax[t0 + t]};
             }
-        k_bucket_emit<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->root, B->left.as<i32>(), B->parent.as<i32>(),
-                                                        B->range.as<i32>(), B->nodes.as<NodeRec<double>>(),
+        k_bucket_emit<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->left.as<i32>(), B->range.as<i32>(), lcp,
+                                                        lcp + (n - 1), B->nodes.as<NodeRec<double>>(),
                                                         B->leaves.as<LeafRec<double>>(), et);
         PH_HIP(hipGetLastError());
     }

@@ -313,86 +313,203 @@ __global__ void k_fill_u32(u32* p, i64 n, u32 v) {
     if (i < n) p[i] = v;
 }
 
-// Combines the two children of internal node p into nodes[p-1].
+// Box of unified id u: an internal node's refitted box, or a leaf's point
+// (its h as the scale).
 template <class T>
-__device__ inline void combine_children(const LeafRec<T>* __restrict__ leaves, const i32* __restrict__ left,
-                                        const i32* __restrict__ right, i64 n, NodeRec<T>* nodes, i32 p) {
-    const i32 l = left[p - 1], r = right[p - 1];
-    T lmin[3], lmax[3], rmin[3], rmax[3], hl, hr;
-    if (l >= (i32)n) {
-        const LeafRec<T> L = leaves[l - n];
-        lmin[0] = lmax[0] = L.x; lmin[1] = lmax[1] = L.y; lmin[2] = lmax[2] = L.z; hl = L.h;
+__device__ inline void node_or_leaf_box(const LeafRec<T>* __restrict__ leaves, const NodeRec<T>* nodes, i64 n, i32 u,
+                                        T mn[3], T mx[3], T& h) {
+    if (u >= (i32)n) {
+        const LeafRec<T> L = leaves[u - n];
+        mn[0] = mx[0] = L.x;
+        mn[1] = mx[1] = L.y;
+        mn[2] = mx[2] = L.z;
+        h = L.h;
     } else {
-        const NodeRec<T>* N = &nodes[l - 1];
-        for (int d = 0; d < 3; ++d) { lmin[d] = N->lo[d]; lmax[d] = N->hi[d]; }
-        hl = N->scale;
+        const NodeRec<T>* N = &nodes[u - 1];
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = N->lo[d];
+            mx[d] = N->hi[d];
+        }
+        h = N->scale;
     }
-    if (r >= (i32)n) {
-        const LeafRec<T> R = leaves[r - n];
-        rmin[0] = rmax[0] = R.x; rmin[1] = rmax[1] = R.y; rmin[2] = rmax[2] = R.z; hr = R.h;
-    } else {
-        const NodeRec<T>* N = &nodes[r - 1];
-        for (int d = 0; d < 3; ++d) { rmin[d] = N->lo[d]; rmax[d] = N->hi[d]; }
-        hr = N->scale;
-    }
-    NodeRec<T>* P = &nodes[p - 1];
-    for (int d = 0; d < 3; ++d) {
-        P->lo[d] = (lmin[d] < rmin[d]) ? lmin[d] : rmin[d];
-        P->hi[d] = (lmax[d] > rmax[d]) ? lmax[d] : rmax[d];
-    }
-    P->scale = (hl > hr) ? hl : hr;
 }
 
-constexpr int kRefitChunk = kBlock * 8;  // leaves per workgroup in the local refit
+constexpr int kRefitChunk = 512;              // leaves (and threads) per workgroup, level 1
+constexpr int kChunkStarts = 128;             // listed nodes per chunk at most
+constexpr int kSpanChunks = 64;               // chunks per workgroup, level 2
+constexpr i64 kRefitSpan = (i64)kRefitChunk * kSpanChunks;
+constexpr int kRefitSpanBlock = 1024;         // threads per workgroup, levels 2 and 3
 
-// Refit, phase 1: workgroup b owns leaves [c0, c1] (1-based) and refits every
-// internal node whose leaf range lies inside it, counting arrivals in LDS
-// (workgroup scope, no cross-XCD cache maintenance).  A node whose parent
-// spans another chunk is appended to `starts` for phase 2.
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_refit_local(const LeafRec<T>* __restrict__ leaves, const i32* __restrict__ parent,
-                                                const i32* __restrict__ left, const i32* __restrict__ right,
-                                                const i32* __restrict__ range, i64 n, NodeRec<T>* nodes, u32* visited,
-                                                i32* starts, u32* nstarts) {
+__global__ void __launch_bounds__(kRefitChunk) k_refit_chunk(const LeafRec<T>* __restrict__ leaves,
+                                                             const i32* __restrict__ parent,
+                                                             const i32* __restrict__ left,
+                                                             const i32* __restrict__ right,
+                                                             const i32* __restrict__ range, i64 n, NodeRec<T>* nodes,
+                                                             u32* visited, i32* starts, u32* nstarts, u32* bad) {
+    __shared__ T lf[kRefitChunk][4];  // leaf c0 + j: x, y, z, h
+    __shared__ T nb[kRefitChunk][7];  // internal node c0 + j: lo[3], hi[3], scale
+    __shared__ i32 tp[kRefitChunk][5];  // internal node c0 + j: parent, left, right, range lo, range hi
+    __shared__ i32 lp[kRefitChunk];  // parent of leaf c0 + j
     __shared__ u32 cnt[kRefitChunk];
+    __shared__ u32 nlisted;
     const i64 c0 = (i64)blockIdx.x * kRefitChunk + 1;
     const i64 c1 = c0 + kRefitChunk - 1 < n ? c0 + kRefitChunk - 1 : n;
-    for (int j = threadIdx.x; j < kRefitChunk; j += kBlock) cnt[j] = 0u;
+    const int j = threadIdx.x;
+    const i64 k = c0 + j;
+    if (k <= c1) {
+        const LeafRec<T> L = leaves[k - 1];
+        lf[j][0] = L.x;
+        lf[j][1] = L.y;
+        lf[j][2] = L.z;
+        lf[j][3] = L.h;
+        lp[j] = parent[n - 1 + k - 1];
+    }
+    if (k <= n - 1) {
+        tp[j][0] = parent[k - 1];
+        tp[j][1] = left[k - 1];
+        tp[j][2] = right[k - 1];
+        tp[j][3] = range[2 * (k - 1)];
+        tp[j][4] = range[2 * (k - 1) + 1];
+    }
+    cnt[j] = 0u;
+    if (j == 0) nlisted = 0u;
     __syncthreads();
-    for (i64 k = c0 + threadIdx.x; k <= c1; k += kBlock) {
+    // the box of unified id u, which lies inside the chunk (a leaf, or an
+    // internal node refitted here)
+    auto box = [&](i32 u, T mn[3], T mx[3], T& h) {
+        if (u >= (i32)n) {
+            const int q = (int)((i64)u - n + 1 - c0);
+            for (int d = 0; d < 3; ++d) mn[d] = mx[d] = lf[q][d];
+            h = lf[q][3];
+        } else {
+            const int q = (int)((i64)u - c0);
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = nb[q][d];
+                mx[d] = nb[q][3 + d];
+            }
+            h = nb[q][6];
+        }
+    };
+    if (k <= c1) {
         i32 child = (i32)(n - 1 + k);
-        i32 p = parent[child - 1];
+        i32 p = lp[j];
         for (i64 depth = n; p != 0 && depth > 0; --depth) {
-            const i64 lo = range[2 * (p - 1)], hi = range[2 * (p - 1) + 1];
-            if (lo < c0 || hi > c1) {  // p spans chunks: phase 2 climbs from `child`
-                const u32 slot = atomicAdd(nstarts, 1u);
-                starts[slot] = child;
+            const i64 q = (i64)p - c0;
+            const bool inside = q >= 0 && q < kRefitChunk && (i64)p <= n - 1 && tp[q][3] >= c0 && tp[q][4] <= c1;
+            if (!inside) {  // p spans chunks: level 2 climbs from `child`
+                const u32 slot = atomicAdd(&nlisted, 1u);
+                if (slot < (u32)kChunkStarts) starts[(i64)blockIdx.x * kChunkStarts + slot] = child;
+                else atomicAdd(bad, 1u);
                 break;
             }
-            const u32 old = __hip_atomic_fetch_add(&cnt[p - c0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            // the children's boxes are read from LDS: the arrival needs only
+            // LDS ordering (the global node stores are read by later kernels)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            const u32 old = __hip_atomic_fetch_add(&cnt[q], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
             if (old != 1u) break;  // first arrival
-            combine_children<T>(leaves, left, right, n, nodes, p);
+            T lmin[3], lmax[3], rmin[3], rmax[3], hl, hr;
+            box(tp[q][1], lmin, lmax, hl);
+            box(tp[q][2], rmin, rmax, hr);
+            NodeRec<T>* P = &nodes[p - 1];
+            for (int d = 0; d < 3; ++d) {
+                const T lo = (lmin[d] < rmin[d]) ? lmin[d] : rmin[d];
+                const T hi = (lmax[d] > rmax[d]) ? lmax[d] : rmax[d];
+                nb[q][d] = lo;
+                nb[q][3 + d] = hi;
+                P->lo[d] = lo;
+                P->hi[d] = hi;
+            }
+            const T sc = (hl > hr) ? hl : hr;
+            nb[q][6] = sc;
+            P->scale = sc;
             visited[p - 1] = 2u;
             child = p;
-            p = parent[p - 1];
+            p = tp[q][0];
         }
     }
+    __syncthreads();
+    if (j == 0) nstarts[blockIdx.x] = nlisted < (u32)kChunkStarts ? nlisted : (u32)kChunkStarts;
 }
 
-// Refit, phase 2: climbs from the recorded subtree roots through the nodes that
-// span chunks, with agent-scope acq_rel counters (visited[] zero for them).
+// Levels 2 and 3.  Level 2 (out != null): workgroup b owns the leaves
+// [b kRefitSpan + 1, (b + 1) kRefitSpan] and climbs from the nodes listed by
+// its chunks (starts / nstarts as k_refit_chunk writes them, nchunks chunks in
+// all) through the nodes whose range lies there.  Level 3 (out == null, one
+// workgroup): climbs from the nin[0] nodes of `in` to the root.
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_refit_top(const LeafRec<T>* __restrict__ leaves, const i32* __restrict__ parent,
-                                              const i32* __restrict__ left, const i32* __restrict__ right, i64 n,
-                                              NodeRec<T>* nodes, u32* visited, const i32* starts, const u32* nstarts) {
-    const u32 total = *nstarts;
-    for (u32 t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
-        i32 p = parent[starts[t] - 1];
+__global__ void __launch_bounds__(kRefitSpanBlock) k_refit_span(const LeafRec<T>* __restrict__ leaves,
+                                                                const i32* __restrict__ parent,
+                                                                const i32* __restrict__ left,
+                                                                const i32* __restrict__ right,
+                                                                const i32* __restrict__ range, i64 n, i64 nchunks,
+                                                                NodeRec<T>* nodes, u32* visited, const i32* in,
+                                                                const u32* nin, i32* out, u32* nout) {
+    const i64 C0 = (i64)blockIdx.x * kRefitSpan + 1, C1 = C0 + kRefitSpan - 1;
+    const i64 chunk0 = (i64)blockIdx.x * kSpanChunks;
+    // level 2: the span's listed nodes numbered consecutively (exclusive
+    // prefix of its chunks' list lengths), so that each thread climbs from at
+    // most a few of them
+    __shared__ u32 pre[kSpanChunks + 1];
+    if (out && threadIdx.x < kWave) {
+        const i64 ch = chunk0 + threadIdx.x;
+        u32 v = threadIdx.x < kSpanChunks && ch < nchunks ? nin[ch] : 0u;
+        for (int d = 1; d < kWave; d <<= 1) {  // inclusive scan over the wave
+            const u32 o = __shfl_up(v, d, kWave);
+            if ((int)threadIdx.x >= d) v += o;
+        }
+        if (threadIdx.x < kSpanChunks) pre[threadIdx.x + 1] = v;
+        if (threadIdx.x == 0) pre[0] = 0u;
+    }
+    __syncthreads();
+    const u32 total = out ? pre[kSpanChunks] : *nin;
+    for (u32 t = threadIdx.x; t < total; t += kRefitSpanBlock) {
+        i32 child;
+        if (out) {  // the chunk holding entry t: the last c with pre[c] <= t
+            int lo = 0, hi = kSpanChunks - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pre[mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            child = in[(chunk0 + lo) * kChunkStarts + (t - pre[lo])];
+        } else {
+            child = in[t];
+        }
+        // the climbing thread carries its node's box; per level it loads the
+        // parent's topology in one round, takes the arrival, then reads the
+        // sibling's box (three dependent rounds)
+        T mn[3], mx[3], h;
+        node_or_leaf_box<T>(leaves, nodes, n, child, mn, mx, h);
+        i32 p = parent[child - 1];
         for (i64 depth = n; p != 0 && depth > 0; --depth) {
-            const u32 old = __hip_atomic_fetch_add(&visited[p - 1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            const i32 plo = range[2 * (p - 1)], phi = range[2 * (p - 1) + 1];
+            const i32 pl = left[p - 1], pr = right[p - 1], pp = parent[p - 1];
+            if (out && (plo < C0 || phi > C1)) {
+                const u32 slot = atomicAdd(nout, 1u);
+                out[slot] = child;
+                break;
+            }
+            const u32 old = __hip_atomic_fetch_add(&visited[p - 1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (old != 1u) break;
-            combine_children<T>(leaves, left, right, n, nodes, p);
-            p = parent[p - 1];
+            T smn[3], smx[3], sh;
+            node_or_leaf_box<T>(leaves, nodes, n, pl == child ? pr : pl, smn, smx, sh);
+            NodeRec<T>* P = &nodes[p - 1];
+            for (int d = 0; d < 3; ++d) {
+                // the reference's order of the operands: left child first
+                const T lmin = pl == child ? mn[d] : smn[d], rmin = pl == child ? smn[d] : mn[d];
+                const T lmax = pl == child ? mx[d] : smx[d], rmax = pl == child ? smx[d] : mx[d];
+                mn[d] = (lmin < rmin) ? lmin : rmin;
+                mx[d] = (lmax > rmax) ? lmax : rmax;
+                P->lo[d] = mn[d];
+                P->hi[d] = mx[d];
+            }
+            const T hl = pl == child ? h : sh, hr = pl == child ? sh : h;
+            h = (hl > hr) ? hl : hr;
+            P->scale = h;
+            child = p;
+            p = pp;
         }
     }
 }
@@ -424,7 +541,11 @@ void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipS
     W.dk.alloc(sizeof(u64) * n);
     W.flag.alloc(sizeof(u32) * 64);
     W.adj.alloc(sizeof(i32) * n);
-    W.visited.alloc(sizeof(u32) * (n - 1) + sizeof(u32) * 64 + sizeof(i32) * n);
+    // refit: arrival counters and check count, the chunks' listed nodes, the
+    // level-3 list and the chunks' list lengths
+    W.visited.alloc(sizeof(u32) * (n - 1) + sizeof(u32) * 64 +
+                    sizeof(i32) * ((n + kRefitChunk - 1) / kRefitChunk * kChunkStarts + n) +
+                    sizeof(u32) * ((n + kRefitChunk - 1) / kRefitChunk));
     W.tmp.alloc(align(16 * (size_t)n) + (size_t(1) << 20));  // radix sort, scans, min/max
     const int rs = interleave_stride(ncols), rows = interleave_rows(rs);
     W.aos.alloc(sizeof(T) * (size_t)rs * n);  // the interleaved columns
@@ -488,22 +609,32 @@ void build_typed(ph_lbvh* B, const T* const* src_cols /* device, 6+ncol */, hipS
     k_leaf_escape<T><<<g, kBlock, 0, s>>>(B->parent.as<i32>(), B->left.as<i32>(), B->right.as<i32>(), n,
                                           B->escape.as<i32>(), B->leaf_escape.as<i32>(), B->nodes.as<NodeRec<T>>());
     PH_HIP(hipGetLastError());
-    // 9: refit (phase 1 per chunk in LDS, phase 2 across chunks)
+    // 9: refit (three levels: chunks in LDS, spans, the top)
     if (n >= 2) {
+        const i64 nchunks = (n + kRefitChunk - 1) / kRefitChunk;
         u32* visited = W.visited.as<u32>();
         u32* bad = visited + (n - 1);
-        u32* nstarts = bad + 1;
-        i32* starts = reinterpret_cast<i32*>(visited + (n - 1) + 64);
+        u32* nstarts2 = bad + 1;  // level-3 list length
+        i32* starts = reinterpret_cast<i32*>(visited + (n - 1) + 64);  // nchunks segments of kChunkStarts
+        i32* starts2 = starts + nchunks * kChunkStarts;                  // level-3 list
+        u32* nstarts = reinterpret_cast<u32*>(starts2 + n);             // per chunk
         k_fill_u32<<<grid_for(n - 1 + 2), kBlock, 0, s>>>(visited, n - 1 + 2, 0u);
         PH_HIP(hipGetLastError());
-        const int nchunks = (int)((n + kRefitChunk - 1) / kRefitChunk);
-        k_refit_local<T><<<nchunks, kBlock, 0, s>>>(B->leaves.as<LeafRec<T>>(), B->parent.as<i32>(), B->left.as<i32>(),
-                                                    B->right.as<i32>(), B->range.as<i32>(), n,
-                                                    B->nodes.as<NodeRec<T>>(), visited, starts, nstarts);
+        k_refit_chunk<T><<<(unsigned)nchunks, kRefitChunk, 0, s>>>(B->leaves.as<LeafRec<T>>(), B->parent.as<i32>(),
+                                                                  B->left.as<i32>(), B->right.as<i32>(),
+                                                                  B->range.as<i32>(), n, B->nodes.as<NodeRec<T>>(),
+                                                                  visited, starts, nstarts, bad);
         PH_HIP(hipGetLastError());
-        k_refit_top<T><<<grid_for(n, kBlock, 1024), kBlock, 0, s>>>(B->leaves.as<LeafRec<T>>(), B->parent.as<i32>(),
-                                                                    B->left.as<i32>(), B->right.as<i32>(), n,
-                                                                    B->nodes.as<NodeRec<T>>(), visited, starts, nstarts);
+        const unsigned nspans = (unsigned)((n + kRefitSpan - 1) / kRefitSpan);
+        k_refit_span<T><<<nspans, kRefitSpanBlock, 0, s>>>(B->leaves.as<LeafRec<T>>(), B->parent.as<i32>(),
+                                                          B->left.as<i32>(), B->right.as<i32>(), B->range.as<i32>(),
+                                                          n, nchunks, B->nodes.as<NodeRec<T>>(), visited, starts,
+                                                          nstarts, starts2, nstarts2);
+        PH_HIP(hipGetLastError());
+        k_refit_span<T><<<1, kRefitSpanBlock, 0, s>>>(B->leaves.as<LeafRec<T>>(), B->parent.as<i32>(),
+                                                     B->left.as<i32>(), B->right.as<i32>(), B->range.as<i32>(), n,
+                                                     nchunks, B->nodes.as<NodeRec<T>>(), visited, starts2, nstarts2,
+                                                     nullptr, nullptr);
         PH_HIP(hipGetLastError());
         k_check_visited<<<grid_for(n - 1), kBlock, 0, s>>>(visited, n - 1, bad);
         PH_HIP(hipGetLastError());

@@ -17,7 +17,10 @@ template <class T> void minmax3(const T* x, const T* y, const T* z, i64 n, T* ou
 void radix_sort_pairs(u64* keys, u32* vals, i64 n, int bits, DevBuf& tmp, hipStream_t s);
 void radix_sort_pairs(u32* keys, u32* vals, i64 n, int bits, DevBuf& tmp, hipStream_t s);
 
-inline int grid_for(i64 n, int block = kBlock, i64 cap = 1 << 20) {
+// Blocks of `block` threads covering n items, one item per thread: kernels
+// launched on it without a grid-stride loop need the default cap, which covers
+// 2^31 items at 256 threads (the LinearBVH's 2^30 leaves, 2^31 - 1 nodes).
+inline int grid_for(i64 n, int block = kBlock, i64 cap = i64(1) << 23) {
     i64 g = (n + block - 1) / block;
     if (g < 1) g = 1;
     if (g > cap) g = cap;

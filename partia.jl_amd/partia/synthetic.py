@@ -166,7 +166,7 @@ def gaussian_random_field(ng, seed, slope=-11.0 / 3.0, sigma_ln=1.5):
     k[0, 0, 0] = 1.0
     amp = k ** (slope / 2.0)
     amp[0, 0, 0] = 0.0
-    delta = np.fft.irfftn(fk * amp, s=(ng, ng, ng))
+    delta = np.fft.irfftn(fk * amp, s=(ng, ng, ng), axes=(0, 1, 2))
     delta *= sigma_ln / delta.std()
     return delta
 
