@@ -81,9 +81,10 @@ def parse():
     ap.add_argument("--traversal", choices=["auto", "lane", "bucket"], default="auto")
     ap.add_argument("--autotune", choices=["on", "off"], default="on",
                     help="tune the launch configuration on the first (untimed) call of each workload key")
-    ap.add_argument("--launch", default=None, metavar="OCC,XCD[,ORDER]",
-                    help="fixed launch configuration (occupancy, block order, tile order) of the point / grid kernels; "
-                         "turns the autotuner off")
+    ap.add_argument("--launch", default=None, metavar="OCC,XCD[,ORDER[,LEAVES[,BLOCK]]]",
+                    help="fixed launch configuration (occupancy, block order, tile order, leaves per bucket and "
+                         "threads per block of the grid tile walks) of the point / grid kernels; turns the autotuner "
+                         "off")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
     ap.add_argument("--workload", choices=["grid256", "column2048", "grid64", "batch512", "adjoint1024", "gridadjoint256"],
@@ -127,6 +128,7 @@ def apply_launch(ctx, args):
         v = args.launch.split(",")
         ctx.set_autotune(False)
         ctx.set_launch(float(v[0]), int(v[1]), int(v[2]) if len(v) > 2 else 0)
+        ctx.set_tile_shape(int(v[3]) if len(v) > 3 else 0, int(v[4]) if len(v) > 4 else 0)
     else:
         ctx.set_autotune(args.autotune == "on")
 
