@@ -21,6 +21,7 @@ from .errors import ArgumentError, DeviceError, DimensionMismatch, JuliaAssertio
 from .grids import (AbstractGrid, AbstractSamples, Cartesian, Cylindrical, Frame, GridBundle, LineSamples,
                     ParallelBeam, PointSamples, frame_forward, frame_position, frame_right, frame_up,
                     Polar, Spherical, StructuredGrid, coordinate_grid, flatten, restore_struct)
+from .gridsio import GridDataset, file_identifier, read_GridDataset, write_GridBundle, write_GridDataset
 from .input import (AABB, BinaryRadixTree, InterpolationInput, LinearBVH, LinearBVH_, UInt32, UInt64, build_input,
                     get_coord, get_xcoord, get_ycoord, get_zcoord, matches_lbvh_leaf_order)
 from .kernels import (ALL_KERNELS, AbstractExecutionBackend, AbstractInterpolationStrategy, AbstractSPHKernel,
