@@ -25,9 +25,13 @@ typedef __attribute__((address_space(3))) double LdsDouble;
 constexpr int kC2Rec = 6;
 
 // LDS of one wave, in doubles: 64 staged leaves and one zero slot of `slot`
-// doubles each, then 64 leaf indices.  FP32C: slot = 4 + rw (x, y, z,
-// (K h)^2, record); float64: slot = 5 + rw (x, y, z, (K h)^2, h, record).
-__host__ __device__ constexpr int tile_wave_doubles(int slot) { return (kWave + 1) * slot + kWave / 2; }
+// doubles each, then 64 leaf indices, rounded up to an even count so that
+// every wave's region is 16-byte aligned (it also holds the nearest-h search's
+// 64 LeafRec candidates).  FP32C: slot = 4 + rw (x, y, z, (K h)^2, record);
+// float64: slot = 5 + rw (x, y, z, (K h)^2, h, record).
+__host__ __device__ constexpr int tile_wave_doubles(int slot) {
+    return ((kWave + 1) * slot + kWave / 2 + 1) & ~1;
+}
 // Dynamic LDS of a tile kernel's workgroup of `block` threads.
 inline size_t tile_lds_bytes(int slot, int block = kBlock) {
     return (size_t)(block / kWave) * (size_t)tile_wave_doubles(slot) * sizeof(double);
