@@ -258,14 +258,19 @@ struct ph_lbvh {
     ph::DevBuf brecs16;   // 16-leaf bucket-tree records [nrec16]
     ph::DevBuf brecs;     // 4-leaf bucket-tree records [nrec]
     ph::DevBuf rec32;     // FP32-compensated records of the last call ({m/rho, 1/h}, columns)
+    // Node records of the per-wave line walks (LineNode<T> [n-1]: the box and
+    // the node-only terms of the line pruning test), built by the first line call.
+    bool lnodes_ready = false;
+    ph::DevBuf lnodes;
 
     // Frees every device buffer (the context is going away; the handle stays
     // valid for ph_lbvh_destroy only).
     void release_device() {
         for (ph::DevBuf* b : {&cols, &codes, &order, &left, &right, &escape, &parent, &range, &nodes, &leaves,
-                              &leaf_escape, &rec, &ha, &nn, &brecs, &brecs8, &brecs16, &brecs1, &rec32})
+                              &leaf_escape, &rec, &ha, &nn, &brecs, &brecs8, &brecs16, &brecs1, &rec32, &lnodes})
             b->release();
         buckets_ready = false;
+        lnodes_ready = false;
         buckets1_ready = false;
         nrec = nrec1 = nrec8 = nrec16 = 0;
     }

@@ -202,6 +202,19 @@ template <class T, int K> struct LineAcc {
         for (int j = 0; j < kMaxS; ++j)
             if (j < c.ns) acc[j] += take ? qv[j] * t : T(0);
     }
+    // add_sel for the first MS scalars only (c.ns <= MS), with invh = T(1) / h
+    // and rdq = T(1) / dq given (line_W_rdq).
+    template <int MS>
+    __device__ inline void add_sel_inv(bool take, T mb, T rb, const T (&qv)[MS], const CatChunk& c, T d2, T invh,
+                                       const T* __restrict__ inorm, T dq, T rdq) {
+        const T dr = sqrt(d2 < T(0) ? T(0) : d2);
+        const T W = line_W_rdq<T, K>(dr, invh, inorm, dq, rdq);
+        const T t = mb * W / rb;
+        S1 += take ? t : T(0);
+#pragma unroll
+        for (int j = 0; j < MS; ++j)
+            if (j < c.ns) acc[j] += take ? qv[j] * t : T(0);
+    }
     // NaN outputs when no particle contributes (:154-156), Shepard scalars * 1/S1.
     __device__ inline void store(const CatChunk& c, T* __restrict__ out, i64 S, i64 s) const {
         if (S1 == T(0)) {
@@ -250,15 +263,21 @@ __global__ void __launch_bounds__(kBlock) k_lines(TreeView<T> tv, const T* __res
 // right after u's subtree, so every lane tests exactly the nodes and leaves of
 // its own DFS (LinearBVHTraversalMacros.jl:539-644), in the same order, with the
 // same arithmetic as k_lines.  leaf(k, take, d2, L) is called by the whole wave
-// for every leaf of the union walk; take marks the lanes that accept it.
+// for every leaf of the union walk; take marks the lanes that accept it.  The
+// records come through the kernels' restrict-qualified parameters (lnodes,
+// leaves, lesc = tv's leaf escapes), so their wave-uniform loads are scalar
+// loads, and every lane tests every leaf (no branch around the load).
 template <class T, class Leaf>
-__device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, const Line<T>& l, bool valid,
+__device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, const LineNode<T>* __restrict__ lnodes,
+                                                         const LeafRec<T>* __restrict__ leaves,
+                                                         const i32* __restrict__ lesc, const Line<T>& l, bool valid,
                                                          T Kvalid, Leaf&& leaf) {
     const i64 n = tv.n;
     auto visit = [&](i64 k, bool active) {
-        const LeafRec<T> L = tv.leaves[k];
+        const LeafRec<T> L = leaves[k];
         const T d2 = d2_line_leaf(l, L);
-        leaf(k, active && d2 <= radius2<T, PH_SCATTER>(Kvalid, T(0), L.h), d2, L);
+        const bool close = d2 <= radius2<T, PH_SCATTER>(Kvalid, T(0), L.h);
+        leaf(k, active & close, d2, L);
     };
     if (tv.root == 0) {  // n == 1: the reference's brute-force branch
         for (i64 k = 0; k < n; ++k) visit(k, valid);
@@ -272,11 +291,12 @@ __device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, 
         active = active || node == resume;
         if (node >= (i32)n) {
             const i64 k = (i64)node - n;
+            const i32 esc = lesc[k];
             visit(k, active);
-            node = tv.leaf_escape[k];
+            node = esc;
         } else {
-            const NodeRec<T> N = tv.nodes[node - 1];
-            const bool prune = d2_line_aabb_lb(l, N) > radius2<T, PH_SCATTER>(Kvalid, T(0), N.scale);
+            const LineNode<T> N = lnodes[node - 1];
+            const bool prune = line_node_prune(l, N);
             resume = (active && prune) ? N.escape : resume;
             const bool descend = active && !prune;
             active = descend;
@@ -286,36 +306,86 @@ __device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, 
 }
 
 // Projection with the wave walk: results identical to k_lines, bit for bit.
-// The steps are branch-free within the wave (idle lanes compute and discard),
-// and the line-integrated table is staged in LDS.
-template <class T, int K, int SRC>
-__global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const T* __restrict__ cols, CatChunk c,
+// The walk only tests leaves (every lane, branch-free): the leaves some lane
+// accepts are listed (in walk order, i.e. increasing leaf index) with one bit
+// per lane and leaf, up to 64 at a time; then the wave stages their data in
+// LDS (one leaf per lane) and the lanes accumulate their own accepted leaves in
+// lockstep, lowest bit first, so each ray adds its neighbours in the per-ray
+// DFS's order with the same arithmetic (the distance is recomputed from the
+// staged leaf with the same operations).  The line-integrated table is staged
+// in LDS.  MS: scalar outputs staged per leaf (at least c.ns).
+template <class T, int K, int SRC, int MS>
+__global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const LineNode<T>* __restrict__ lnodes,
+                                                       const LeafRec<T>* __restrict__ leaves,
+                                                       const i32* __restrict__ lesc, const T* __restrict__ cols,
+                                                       CatChunk c,
                                                        LineSampler<T, K, PH_SCATTER, SRC> smp,
-                                                       const T* __restrict__ inorm, T dq, T* __restrict__ out,
-                                                       u64* counter) {
+                                                       const T* __restrict__ inorm, T dq, T rdq,
+                                                       T* __restrict__ out, u64* counter) {
+    constexpr int kSlot = 6 + MS;  // x, y, z, 1 / h, m, rho, the scalars' columns
     __shared__ T tab[PH_LUT_N_ENTRIES];
+    __shared__ T stage[kBlock / kWave][kWave * kSlot];
+    __shared__ u32 sidx[kBlock / kWave][kWave];
     for (int i = threadIdx.x; i < PH_LUT_N_ENTRIES; i += kBlock) tab[i] = inorm[i];
     __syncthreads();
     i64 s = 0;
     Line<T> l{};  // lanes past the end keep a zero line
     const bool valid = smp.get(s, l);
     if (__ballot(valid) == 0) return;  // wave-uniform; no barrier follows
+    const int lane = threadIdx.x & (kWave - 1);
+    T* __restrict__ st = stage[threadIdx.x / kWave];
+    u32* __restrict__ lidx = sidx[threadIdx.x / kWave];
     const i64 n = tv.n;
     const T* __restrict__ mcol = cols + 3 * n;
     const T* __restrict__ rcol = cols + 5 * n;
-    const T* qcol[kMaxS];
+    const T* qcol[MS];
 #pragma unroll
-    for (int j = 0; j < kMaxS; ++j) qcol[j] = cols + (i64)c.rsrc[c.spos[j < c.ns ? j : 0]] * n;
+    for (int j = 0; j < MS; ++j) qcol[j] = cols + (i64)c.rsrc[c.spos[j < c.ns ? j : 0]] * n;
     LineAcc<T, K> a;
-    u64 accepted = 0;
-    wave_walk(tv, l, valid, T(support_of<K>()), [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
-        const T mb = mcol[k], rb = rcol[k];
-        T qv[kMaxS];
+    u64 accepted = 0, mask = 0;
+    int nst = 0;  // listed leaves (wave-uniform)
+    auto flush = [&]() {
+        wave_sync();
+        if (lane < nst) {
+            const u32 k = lidx[lane];
+            const LeafRec<T> L = leaves[k];
+            T* d = st + lane * kSlot;
+            d[0] = L.x;
+            d[1] = L.y;
+            d[2] = L.z;
+            d[3] = T(1) / L.h;  // line_W_nb's reciprocal, once per staged leaf
+            d[4] = mcol[k];
+            d[5] = rcol[k];
 #pragma unroll
-        for (int j = 0; j < kMaxS; ++j) qv[j] = j < c.ns ? qcol[j][k] : T(0);
-        accepted += take ? 1 : 0;
-        a.add_sel(take, mb, rb, qv, c, d2, L.h, tab, dq);
+            for (int j = 0; j < MS; ++j) d[6 + j] = qcol[j][k];
+        }
+        wave_sync();
+        accepted += __popcll(mask);
+        const int nit = lockstep_count(mask);
+        for (int it = 0; it < nit; ++it) {
+            const bool live = mask != 0;
+            const int b = live ? __builtin_ctzll(mask) : 0;
+            mask &= mask - 1;
+            const T* d = st + b * kSlot;
+            const LeafRec<T> L{d[0], d[1], d[2], T(0)};
+            T qv[MS];
+#pragma unroll
+            for (int j = 0; j < MS; ++j) qv[j] = d[6 + j];
+            // lanes without a neighbour left read leaf 0 and add exact zeros
+            a.template add_sel_inv<MS>(live, d[4], d[5], qv, c, d2_line_leaf(l, L), d[3], tab, dq, rdq);
+        }
+        wave_sync();  // every lane is done with the staged leaves
+        nst = 0;
+    };
+    wave_walk(tv, lnodes, leaves, lesc, l, valid, T(support_of<K>()), [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
+        (void)d2;
+        (void)L;
+        if (__ballot(take) == 0) return;  // wave-uniform
+        if (lane == 0) lidx[nst] = (u32)k;
+        mask |= take ? (u64(1) << nst) : u64(0);
+        if (++nst == kWave) flush();
     });
+    if (nst > 0) flush();
     if (valid) a.store(c, out, smp.S, s);
     count_add(counter, valid ? accepted : 0);
 }
@@ -332,7 +402,10 @@ __global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const T* 
 // wave's rays in float64 and added with one float64 atomic per wave, particle
 // and parameter (so the float64 sums are order-dependent in the last bits).
 template <class T, int K, int SRC, bool SHEP>
-__global__ void __launch_bounds__(kBlock) k_lines_adjoint(TreeView<T> tv, const T* __restrict__ cols, i32 qsrc,
+__global__ void __launch_bounds__(kBlock) k_lines_adjoint(TreeView<T> tv, const LineNode<T>* __restrict__ lnodes,
+                                                          const LeafRec<T>* __restrict__ leaves,
+                                                          const i32* __restrict__ lesc, const T* __restrict__ cols,
+                                                          i32 qsrc,
                                                           LineSampler<T, K, PH_SCATTER, SRC> smp,
                                                           const T* __restrict__ inorm, T dq,
                                                           const T* __restrict__ cot, const u32* __restrict__ order,
@@ -353,7 +426,7 @@ __global__ void __launch_bounds__(kBlock) k_lines_adjoint(TreeView<T> tv, const 
     T S1 = T(0), out = T(0);
     if constexpr (SHEP) {  // the forward's sums, same arithmetic
         T N = T(0);
-        wave_walk(tv, l, valid, Kvalid, [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
+        wave_walk(tv, lnodes, leaves, lesc, l, valid, Kvalid, [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
             const T mb = mcol[k], rb = rcol[k], Ab = acol[k];
             const T dr = sqrt(d2 < T(0) ? T(0) : d2);
             const T t = mb * line_W_nb<T, K>(dr, L.h, tab, dq) / rb;
@@ -364,7 +437,7 @@ __global__ void __launch_bounds__(kBlock) k_lines_adjoint(TreeView<T> tv, const 
         out = S1 == T(0) ? T(0) : N * (T(1) / S1);
     }
     const T c2 = SHEP ? (S1 == T(0) ? T(0) : T(1) / S1) : T(1);
-    wave_walk(tv, l, valid, Kvalid, [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
+    wave_walk(tv, lnodes, leaves, lesc, l, valid, Kvalid, [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
         if (__ballot(take && g != T(0)) == 0) return;  // wave-uniform
         const T mb = mcol[k], rb = rcol[k], Ab = acol[k];
         const T dr = sqrt(d2 < T(0) ? T(0) : d2);
@@ -485,6 +558,42 @@ void launch_points_p(ph_lbvh* B, const CatChunk& c, int strategy, const PointLis
     }
 }
 
+// LineNode records of every internal node of a tree with kernel K.
+template <class T, int K>
+__global__ void __launch_bounds__(kBlock) k_line_nodes(const NodeRec<T>* __restrict__ nodes, i64 m,
+                                                       LineNode<T>* __restrict__ out) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    out[i] = line_node<T>(nodes[i], T(support_of<K>()));
+}
+
+// The tree's LineNode records (the first call builds them; the tree is immutable).
+template <class T, int K> const LineNode<T>* line_nodes(ph_lbvh* B, hipStream_t s) {
+    if (!B->lnodes_ready) {
+        const i64 m = B->n > 1 ? B->n - 1 : 1;
+        B->lnodes.alloc(sizeof(LineNode<T>) * (size_t)m);
+        if (B->n > 1) {
+            k_line_nodes<T, K><<<grid_for(B->n - 1), kBlock, 0, s>>>(B->nodes.as<NodeRec<T>>(), B->n - 1,
+                                                                    B->lnodes.as<LineNode<T>>());
+            PH_HIP(hipGetLastError());
+        }
+        B->lnodes_ready = true;
+    }
+    return B->lnodes.as<LineNode<T>>();
+}
+
+template <class T, int K, int SRC, class Smp>
+void launch_lines_wave(const TreeView<T>& tv, const LineNode<T>* ln, const T* cols, const CatChunk& c, const Smp& smp,
+                       unsigned g, const T* tab, T dq, T* out, u64* counters, hipStream_t s) {
+    const T rdq = T(1) / dq;  // correctly rounded (host IEEE division)
+    if (c.ns <= 1)
+        k_lines_wave<T, K, SRC, 1><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, tv.leaf_escape, cols, c, smp, tab, dq, rdq,
+                                                        out, counters);
+    else
+        k_lines_wave<T, K, SRC, kMaxS><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, tv.leaf_escape, cols, c, smp, tab, dq,
+                                                            rdq, out, counters);
+}
+
 template <class T, int K>
 void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const LineGrid* lg, i64 S, void* out,
                     u64* counters, const T* inorm, bool wave, hipStream_t s, hipEvent_t* ev) {
@@ -493,13 +602,14 @@ void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const Lin
     // Q_2 - Q_1 of the table in T (line_integrated_kernel.jl:14)
     const T dq = T((1.0 * support_of<K>()) / 4095.0) - T(0.0);
     const T* tab = inorm + (size_t)K * PH_LUT_N_ENTRIES;
+    const LineNode<T>* ln = wave ? line_nodes<T, K>(B, s) : nullptr;
     if (ev) PH_HIP(hipEventRecord(ev[0], s));
     if (lg) {
         const i64 tiles = ((lg->nx + 7) / 8) * ((lg->ny + 7) / 8);
         const unsigned g = (unsigned)((tiles + 3) / 4);
         LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S, line_xcd(tl_launch, g), line_order(tl_launch)};
         if (wave)
-            k_lines_wave<T, K, 1><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out), counters);
+            launch_lines_wave<T, K, 1>(tv, ln, cols, c, smp, g, tab, dq, static_cast<T*>(out), counters, s);
         else
             k_lines<T, K, PH_SCATTER, 1><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out),
                                                               counters);
@@ -507,7 +617,7 @@ void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const Lin
         const unsigned g = (unsigned)((S + kBlock - 1) / kBlock);
         LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S, line_xcd(tl_launch, g)};
         if (wave)
-            k_lines_wave<T, K, 0><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out), counters);
+            launch_lines_wave<T, K, 0>(tv, ln, cols, c, smp, g, tab, dq, static_cast<T*>(out), counters, s);
         else
             k_lines<T, K, PH_SCATTER, 0><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out),
                                                               counters);
@@ -537,6 +647,7 @@ void launch_adjoint_k(ph_lbvh* B, i32 qsrc, const LineList* ll, const LineGrid* 
     const T dq = T((1.0 * support_of<K>()) / 4095.0) - T(0.0);
     const T* tab = inorm + (size_t)K * PH_LUT_N_ENTRIES;
     const u32* order = B->order.as<u32>();
+    const LineNode<T>* ln = line_nodes<T, K>(B, s);
     k_zero_f64<<<grid_for(6 * B->n), kBlock, 0, s>>>(grad, 6 * B->n);
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[0], s));
@@ -544,11 +655,13 @@ void launch_adjoint_k(ph_lbvh* B, i32 qsrc, const LineList* ll, const LineGrid* 
         const i64 tiles = ((lg->nx + 7) / 8) * ((lg->ny + 7) / 8);
         const unsigned g = (unsigned)((tiles + 3) / 4);
         LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S, line_xcd(tl_launch, g), line_order(tl_launch)};
-        k_lines_adjoint<T, K, 1, SHEP><<<g, kBlock, 0, s>>>(tv, cols, qsrc, smp, tab, dq, cot, order, grad);
+        k_lines_adjoint<T, K, 1, SHEP><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, tv.leaf_escape, cols, qsrc, smp, tab, dq,
+                                                             cot, order, grad);
     } else {
         const unsigned g = (unsigned)((S + kBlock - 1) / kBlock);
         LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S, line_xcd(tl_launch, g)};
-        k_lines_adjoint<T, K, 0, SHEP><<<g, kBlock, 0, s>>>(tv, cols, qsrc, smp, tab, dq, cot, order, grad);
+        k_lines_adjoint<T, K, 0, SHEP><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, tv.leaf_escape, cols, qsrc, smp, tab, dq,
+                                                             cot, order, grad);
     }
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[1], s));

@@ -133,6 +133,85 @@ template <class T> __device__ inline T d2_line_aabb_lb(const Line<T>& l, const N
     return dl * dl;
 }
 
+// Node record of the per-wave line walks: the box, then the node-only terms of
+// d2_line_aabb_lb's bounding-sphere bound (centre c, squared half diagonal r2
+// and sr = sqrt(r2)) and the scatter radius rad2 = (K scale)^2, each computed
+// with the operations d2_line_aabb_lb and radius2 use, so the walk's pruning
+// decisions are the per-ray DFS's, bit for bit.  64 bytes for float, 112 for
+// double; built once per tree (k_line_nodes).
+template <class T> struct alignas(16) LineNode {
+    T lo[3], hi[3];
+    T c[3];
+    T r2, sr, rad2;
+    i32 left, escape;
+};
+template <class T> __device__ inline LineNode<T> line_node(const NodeRec<T>& N, T Kvalid) {
+    LineNode<T> R;
+    const T half = T(1) / T(2);
+    T r2 = T(0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        R.lo[d] = N.lo[d];
+        R.hi[d] = N.hi[d];
+        R.c[d] = (N.lo[d] + N.hi[d]) * half;
+        const T hh = (N.hi[d] - N.lo[d]) * half;
+        r2 += hh * hh;
+    }
+    R.r2 = r2;
+    R.sr = sqrt(r2);
+    const T r = Kvalid * N.scale;
+    R.rad2 = r * r;
+    R.left = N.left;
+    R.escape = N.escape;
+    return R;
+}
+// d2_line_aabb_lb(l, N) > (K N.scale)^2 from the node's LineNode: a slab hit or
+// a line within the bounding sphere gives the bound 0, which no radius (a
+// square, or NaN) is below.
+template <class T> __device__ inline bool line_node_prune(const Line<T>& l, const LineNode<T>& N) {
+    bool hit = true;
+    T tmin = -T(INFINITY), tmax = T(INFINITY);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const T a = N.lo[d], b = N.hi[d], o = l.o[d], m = l.m[d];
+        if (m == T(0)) {
+            if (o < a || o > b) hit = false;
+        } else {
+            T t0, t1;
+            if (m == T(1)) {
+                t0 = a - o;
+                t1 = b - o;
+            } else if (m == T(-1)) {
+                t0 = -(a - o);
+                t1 = -(b - o);
+            } else {
+                t0 = (a - o) / m;
+                t1 = (b - o) / m;
+            }
+            if (t0 > t1) {
+                const T tt = t0;
+                t0 = t1;
+                t1 = tt;
+            }
+            tmin = fmax(tmin, t0);
+            tmax = fmin(tmax, t1);
+            if (tmin > tmax) hit = false;
+        }
+    }
+    if (hit) return false;
+    T D2 = T(0), Dm = T(0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const T D = N.c[d] - l.o[d];
+        D2 += D * D;
+        Dm += D * l.m[d];
+    }
+    const T dc2 = D2 - Dm * Dm;
+    if (dc2 <= N.r2) return false;
+    const T dl = sqrt(dc2) - N.sr;
+    return dl * dl > N.rad2;
+}
+
 // Radius rule of the strategy (A13): gather (K ha)^2, scatter (K h)^2, symmetric max.
 template <class T, int STRAT> __device__ inline T radius2(T Kvalid, T r2a, T h) {
     if constexpr (STRAT == PH_GATHER) {
@@ -566,6 +645,13 @@ __device__ __attribute__((always_inline)) inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Lockstep iterations of a wave: the largest accepted count of its lanes.
+__device__ inline int lockstep_count(u64 mask) {
+    int nit = __popcll(mask);
+    for (int d = kWave / 2; d > 0; d >>= 1) nit = max(nit, __shfl_xor(nit, d, kWave));
+    return __builtin_amdgcn_readfirstlane(nit);
 }
 
 // Sum over the wave's 64 lanes (every lane gets the total).

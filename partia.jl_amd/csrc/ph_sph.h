@@ -13,6 +13,8 @@
 // powers are repeated multiplication (within 1 ulp of Julia's compensated `^`).
 #pragma once
 
+#include <type_traits>
+
 #include "ph_common.h"
 
 namespace ph {
@@ -183,11 +185,10 @@ template <class T, int K> __device__ inline T line_W(T r, T h, const T* __restri
     return invh * I;
 }
 
-// line_W without the branch: the table index is clamped for every q (a NaN or
-// huge q reads entry 1 or N-1), and the cutoff selects 0, so the value equals
-// line_W for every q.
-template <class T, int K> __device__ inline T line_W_nb(T r, T h, const T* __restrict__ inorm, T dq) {
-    T invh = T(1) / h;
+// line_W without the branch, from invh = T(1) / h: the table index is clamped
+// for every q (a NaN or huge q reads entry 1 or N-1), and the cutoff selects 0,
+// so the value equals line_W for every q.
+template <class T, int K> __device__ inline T line_W_inv(T r, T invh, const T* __restrict__ inorm, T dq) {
     T q = r * invh;
     T idxf = q / dq + T(1);
     const T lo = T(1), hi = T(PH_LUT_N_ENTRIES - 1);
@@ -198,6 +199,36 @@ template <class T, int K> __device__ inline T line_W_nb(T r, T h, const T* __res
     T I = inorm[i - 1] * (T(1) - t) + inorm[i] * t;
     I = (q >= T(support_of<K>())) ? T(0) : I;
     return invh * I;
+}
+template <class T, int K> __device__ inline T line_W_nb(T r, T h, const T* __restrict__ inorm, T dq) {
+    return line_W_inv<T, K>(r, T(1) / h, inorm, dq);
+}
+// line_W_inv with the float32 table argument q / dq + 1 formed without the
+// division: rdq = RN(1 / dq), q0 = q rdq, q0 + (q - q0 dq) rdq (two fmas;
+// Markstein's correction).  For the three float32 table steps (supports 2,
+// 2.5 and 3) fl(that + 1) equals fl(q / dq + 1) for every float32 q with
+// |q| <= 16 (checked exhaustively on the host), and an accepted neighbour has
+// q = r / h < 2 K <= 6 (r^2 <= (K h)^2, or (K h)^2 overflows and r^2 does
+// not); other lanes' values are discarded.  Float64 keeps the division.
+template <class T, int K>
+__device__ inline T line_W_rdq(T r, T invh, const T* __restrict__ inorm, T dq, T rdq) {
+    if constexpr (!std::is_same<T, float>::value) {
+        (void)rdq;
+        return line_W_inv<T, K>(r, invh, inorm, dq);
+    } else {
+        const float q = r * invh;
+        const float q0 = q * rdq;
+        const float e = __builtin_fmaf(-q0, dq, q);
+        const float idxf = __builtin_fmaf(e, rdq, q0) + 1.f;
+        const float lo = 1.f, hi = float(PH_LUT_N_ENTRIES - 1);
+        const float cf = idxf < lo ? lo : (idxf > hi ? hi : idxf);
+        i32 i = (i32)cf;
+        i = i < 1 ? 1 : (i > PH_LUT_N_ENTRIES - 1 ? PH_LUT_N_ENTRIES - 1 : i);
+        const float t = idxf - float(i);
+        float I = inorm[i - 1] * (1.f - t) + inorm[i] * t;
+        I = (q >= float(support_of<K>())) ? 0.f : I;
+        return invh * I;
+    }
 }
 
 }  // namespace ph

@@ -94,13 +94,6 @@ __device__ __attribute__((always_inline)) u64 test_staged(const LdsDouble* st, i
     return ((u64)mhi << 32) | mlo;
 }
 
-// Lockstep iterations of a wave: the largest accepted count of its lanes.
-__device__ inline int lockstep_count(u64 mask) {
-    int nit = __popcll(mask);
-    for (int d = kWave / 2; d > 0; d >>= 1) nit = max(nit, __shfl_xor(nit, d, kWave));
-    return __builtin_amdgcn_readfirstlane(nit);
-}
-
 // FP32C body: sample s at p (valid) with the nearest particle's h ha;
 // wave_lds is this wave's tile_wave_doubles(4 + rw) doubles of LDS.  recs is
 // bv.recs, restrict-qualified, so the walk's wave-uniform record loads are
