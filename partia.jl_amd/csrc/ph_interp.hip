@@ -140,6 +140,8 @@ template <class T, int K, int STRAT> struct LineSampler<T, K, STRAT, 1> {
     int xcd = 2;    // block order (xcd_block mode)
     int order = 0;  // tile order: 0 x-fastest rows, 1 bricks
     __device__ inline bool get(i64& s, Line<T>& l) const {
+        // every lane holds the direction (wave-uniform values)
+        for (int d = 0; d < 3; ++d) l.m[d] = T(lg.d[d]);
         const i64 tile = xcd_block(xcd) * (kBlock / kWave) + threadIdx.x / kWave;
         const int lane = threadIdx.x & (kWave - 1);
         const i64 tx = (lg.nx + 7) / 8, ty = (lg.ny + 7) / 8;
@@ -161,10 +163,7 @@ template <class T, int K, int STRAT> struct LineSampler<T, K, STRAT, 1> {
             a = lg.ax[i];
             b = lg.ay[j];
         }
-        for (int d = 0; d < 3; ++d) {
-            l.o[d] = T((lg.o[d] + lg.u[d] * a) + lg.v[d] * b);
-            l.m[d] = T(lg.d[d]);
-        }
+        for (int d = 0; d < 3; ++d) l.o[d] = T((lg.o[d] + lg.u[d] * a) + lg.v[d] * b);
         return true;
     }
 };
@@ -296,7 +295,7 @@ __device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, 
             node = esc;
         } else {
             const LineNode<T> N = lnodes[node - 1];
-            const bool prune = line_node_prune(l, N);
+            const bool prune = line_node_prune(l, N, active);
             resume = (active && prune) ? N.escape : resume;
             const bool descend = active && !prune;
             active = descend;

@@ -4,6 +4,8 @@
 // sample generators.  Included by the .hip sources only.
 #pragma once
 
+#include <type_traits>
+
 #include "ph_interp.h"
 #include "ph_sph.h"
 
@@ -137,14 +139,28 @@ template <class T> __device__ inline T d2_line_aabb_lb(const Line<T>& l, const N
 // d2_line_aabb_lb's bounding-sphere bound (centre c, squared half diagonal r2
 // and sr = sqrt(r2)) and the scatter radius rad2 = (K scale)^2, each computed
 // with the operations d2_line_aabb_lb and radius2 use, so the walk's pruning
-// decisions are the per-ray DFS's, bit for bit.  64 bytes for float, 112 for
-// double; built once per tree (k_line_nodes).
+// decisions are the per-ray DFS's, bit for bit; and two thresholds on the
+// squared centre distance dc2 that decide most lanes without the square root
+// (line_node_prune).  64 bytes for float, 128 for double; built once per tree
+// (k_line_nodes).
 template <class T> struct alignas(16) LineNode {
     T lo[3], hi[3];
     T c[3];
     T r2, sr, rad2;
+    T thr_lo, thr_hi;
     i32 left, escape;
 };
+// The per-ray test prunes iff the line misses the box (slab test), dc2 > r2
+// and p = RN(RN(RN(sqrt(dc2)) - sr)^2) > rad2.  With rr = sqrt(rad2) and the
+// margin e (2^-18 for float, 2^-40 for double; every rounding above is 2^-24
+// resp. 2^-53 relative):
+//   dc2 >= thr_hi = (sr + rr (1 + e))^2 (1 + e): sqrt(dc2) - sr >= rr (1 + e/2),
+//     so p >= rad2 (1 + e/2) > rad2 (the last term is pruned);
+//   r2 < dc2 <= thr_lo = (sr + rr (1 - e))^2 (1 - e): 0 <= RN(sqrt(dc2)) - sr
+//     <= rr (1 - e/2), so p < rad2 (not pruned).
+// Both are formed in float64 and rounded to T (a relative 2^-24 at most, inside
+// the margin).  They are NaN (no fast decision) unless r2, sr and both
+// thresholds are finite and rad2 is a positive normal number.
 template <class T> __device__ inline LineNode<T> line_node(const NodeRec<T>& N, T Kvalid) {
     LineNode<T> R;
     const T half = T(1) / T(2);
@@ -163,12 +179,36 @@ template <class T> __device__ inline LineNode<T> line_node(const NodeRec<T>& N, 
     R.rad2 = r * r;
     R.left = N.left;
     R.escape = N.escape;
+    const double e = std::is_same<T, float>::value ? 0x1p-18 : 0x1p-40;
+    const double rr = sqrt((double)R.rad2);
+    const double a = (double)R.sr + rr * (1.0 - e), b = (double)R.sr + rr * (1.0 + e);
+    const T lo = T(a * a * (1.0 - e)), hi = T(b * b * (1.0 + e));
+    const T tmin = std::is_same<T, float>::value ? T(__FLT_MIN__) : T(__DBL_MIN__);
+    const bool ok = isfinite(R.r2) && isfinite(R.sr) && R.rad2 >= tmin && isfinite(R.rad2) && isfinite(lo) &&
+                    isfinite(hi);
+    R.thr_lo = ok ? lo : T(NAN);
+    R.thr_hi = ok ? hi : T(NAN);
     return R;
 }
-// d2_line_aabb_lb(l, N) > (K N.scale)^2 from the node's LineNode: a slab hit or
-// a line within the bounding sphere gives the bound 0, which no radius (a
-// square, or NaN) is below.
-template <class T> __device__ inline bool line_node_prune(const Line<T>& l, const LineNode<T>& N) {
+// d2_line_aabb_lb(l, N) > (K N.scale)^2 from the node's LineNode, for the lanes
+// with active set (the others get false).  Called by the whole wave.  A slab hit
+// or a line within the bounding sphere gives the bound 0, which no radius (a
+// square, or NaN) is below.  The centre distance comes first: lanes with dc2 <=
+// r2 or dc2 <= thr_lo keep the node whatever the slab test says, and when every
+// active lane does, the slab test is skipped; lanes with dc2 >= thr_hi prune
+// unless the slab test hits; the rest (NaN, or within the margins) evaluate the
+// reference's square roots.
+template <class T> __device__ inline bool line_node_prune(const Line<T>& l, const LineNode<T>& N, bool active) {
+    T D2 = T(0), Dm = T(0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const T D = N.c[d] - l.o[d];
+        D2 += D * D;
+        Dm += D * l.m[d];
+    }
+    const T dc2 = D2 - Dm * Dm;
+    const bool keep = dc2 <= N.r2 || dc2 <= N.thr_lo;
+    if (__ballot(active && !keep) == 0) return false;  // wave-uniform
     bool hit = true;
     T tmin = -T(INFINITY), tmax = T(INFINITY);
 #pragma unroll
@@ -198,18 +238,12 @@ template <class T> __device__ inline bool line_node_prune(const Line<T>& l, cons
             if (tmin > tmax) hit = false;
         }
     }
-    if (hit) return false;
-    T D2 = T(0), Dm = T(0);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const T D = N.c[d] - l.o[d];
-        D2 += D * D;
-        Dm += D * l.m[d];
+    bool far = dc2 >= N.thr_hi;
+    if (!keep && !far) {  // the reference's arithmetic
+        const T dl = sqrt(dc2) - N.sr;
+        far = dl * dl > N.rad2;
     }
-    const T dc2 = D2 - Dm * Dm;
-    if (dc2 <= N.r2) return false;
-    const T dl = sqrt(dc2) - N.sr;
-    return dl * dl > N.rad2;
+    return active && !hit && !keep && far;
 }
 
 // Radius rule of the strategy (A13): gather (K ha)^2, scatter (K h)^2, symmetric max.
