@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(kBlock) k_batch(BucketView bv, const BucketRec
 // walk's staging, the per-lane bucket lists (kQ entries per lane) and the
 // nearest-h candidates (64 leaves of 4 doubles) share it.
 inline int batch_wave_doubles(const CatChunk& c, bool c32) {
-    const int slot = c32 ? 4 + (c2_shared(c) && c.rw == kC2Rec ? kC2Rec : c.rw) : 5 + c.rw;
+    const int slot = c32 ? kC32Head + (c2_shared(c) && c.rw == kC2Rec ? kC2Rec : c.rw) : 5 + c.rw;
     return std::max(std::max(tile_wave_doubles(slot), kQ * kWave / 2), kNnCand * 4);
 }
 

@@ -163,9 +163,45 @@ __global__ void __launch_bounds__(kBlock) k_pack_c32(const double* __restrict__ 
 
 // ---- FP32-compensated accumulator ----
 
-// Shape f(q) and f'(q) in float32 without branches: every piece is written with
-// clamped factors (x)_+ = max(x, 0), which reproduces the piecewise definitions
-// of KI/kernel_function/kernels/*.jl (zero beyond the support); q >= 0 here.
+// The Wendland kernels: f'(q) = q g(q) with a polynomial g (C2: -5 t^3; C4:
+// -14/3 t^5 (1 + 5/2 q); C6: -11/2 t^7 (1 + 7/2 q + 4 q^2); t = (1 - q/2)_+),
+// so their gradient sigma f'(q) / (h^4 r) delta = sigma g(q) / h^5 delta needs
+// no 1/r and is 0 at r = 0 like the reference's (kernel.jl:273-285).
+template <int K> constexpr bool wendland32() {
+    return K == PH_KERNEL_C2 || K == PH_KERNEL_C4 || K == PH_KERNEL_C6;
+}
+// Gradient norm of a kernel with 1/h = inv_h and value norm W_norm = sigma / h^3:
+// sigma / h^4 (times f'(q) / r), or sigma / h^5 for the Wendland kernels (times g(q)).
+template <int K> __device__ inline float grad_norm32(float W_norm, float inv_h) {
+    return wendland32<K>() ? W_norm * inv_h * inv_h : W_norm * inv_h;
+}
+
+// The symmetric strategy averages the two kernels: each norm carries the
+// factor 1/2 (exact, a power of two).
+template <int STRAT> constexpr float mix32() { return STRAT == PH_SYMMETRIC ? 0.5f : 1.f; }
+// Norms of a sample's own kernel (1/h_a = inv_ha): value and gradient.
+template <int K, int STRAT> __device__ inline void sample_norms32(float inv_ha, float& Wn, float& Gn) {
+    const float W = sigma3<float, K>() * inv_ha * inv_ha * inv_ha;
+    Wn = mix32<STRAT>() * W;
+    Gn = mix32<STRAT>() * grad_norm32<K>(W, inv_ha);
+}
+// Neighbour-only terms of the scatter and symmetric kernels: (1/h_b)^2 in
+// float64 (exact: the square of a float32) and the float32 norms sigma / h_b^3
+// and grad_norm32 (times mix32), from the record's float32 1/h_b.
+struct LeafNorms {
+    double ib2;
+    float Wn, Gn;
+};
+template <int K, int STRAT> __device__ inline LeafNorms leaf_norms32(float ib) {
+    const double ibd = ib;
+    const float W = sigma3<float, K>() * (ib * ib * ib);
+    return LeafNorms{ibd * ibd, mix32<STRAT>() * W, mix32<STRAT>() * grad_norm32<K>(W, ib)};
+}
+
+// Shape f(q) and df = f'(q) (the Wendland kernels: g(q) = f'(q) / q) in
+// float32 without branches: every piece is written with clamped factors
+// (x)_+ = max(x, 0), which reproduces the piecewise definitions of
+// KI/kernel_function/kernels/*.jl (zero beyond the support); q >= 0 here.
 template <int K> __device__ __attribute__((always_inline)) void shape32(float q, float& f, float& df) {
     if constexpr (K == PH_KERNEL_M4) {
         const float a = fmaxf(2.f - q, 0.f), b = fmaxf(1.f - q, 0.f);
@@ -188,15 +224,15 @@ template <int K> __device__ __attribute__((always_inline)) void shape32(float q,
         const float t2 = t * t, t3 = t2 * t, t4 = t2 * t2;
         if constexpr (K == PH_KERNEL_C2) {
             f = t4 * fmaf(2.f, q, 1.f);
-            df = fmaf(2.f, t4, -t3 * fmaf(4.f, q, 2.f));
+            df = -5.f * t3;
         } else if constexpr (K == PH_KERNEL_C4) {
             const float t5 = t4 * t, t6 = t4 * t2;
             f = t6 * fmaf(fmaf(35.f / 12.f, q, 3.f), q, 1.f);
-            df = fmaf(t6, fmaf(35.f / 6.f, q, 3.f), -t5 * fmaf(fmaf(8.75f, q, 9.f), q, 3.f));
+            df = (-14.f / 3.f) * t5 * fmaf(2.5f, q, 1.f);
         } else {
             const float t7 = t4 * t3, t8 = t4 * t4;
             f = t8 * fmaf(fmaf(fmaf(4.f, q, 6.25f), q, 4.f), q, 1.f);
-            df = fmaf(t8, fmaf(fmaf(12.f, q, 12.5f), q, 4.f), -t7 * fmaf(fmaf(fmaf(16.f, q, 25.f), q, 16.f), q, 4.f));
+            df = -5.5f * t7 * fmaf(fmaf(4.f, q, 3.5f), q, 1.f);
         }
     }
 }
@@ -267,35 +303,45 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT> struct Ac
     __device__ __attribute__((always_inline)) void add_r(bool live, const CatChunk& c, Rec&& r, double dx, double dy,
                                                          double dz, double d2, double iha2, float Wa_norm,
                                                          float Ga_norm) {
+        LeafNorms nb{0.0, 0.f, 0.f};
+        if constexpr (STRAT != PH_GATHER) nb = leaf_norms32<K, STRAT>(__builtin_bit_cast(float2, r(0)).y);
+        add_rn(live, c, r, nb, dx, dy, dz, d2, iha2, Wa_norm, Ga_norm);
+    }
+    // The same with the neighbour's norms nb given (leaf_norms32 of its 1/h).
+    // A lane without a pending neighbour (live false) reads a zero record
+    // (m / rho = 0, zero columns) at a finite position, so every term it adds
+    // is an exact zero; no select is needed.
+    template <class Rec>
+    __device__ __attribute__((always_inline)) void add_rn(bool live, const CatChunk& c, Rec&& r, const LeafNorms& nb,
+                                                          double dx, double dy, double dz, double d2, double iha2,
+                                                          float Wa_norm, float Ga_norm) {
         const float2 vi = __builtin_bit_cast(float2, r(0));
         const float d2f = (float)d2;
         float W, coef;
         if constexpr (STRAT == PH_GATHER) {
+            (void)nb;
             float f, df;
             shape32<K>(__builtin_amdgcn_sqrtf((float)(d2 * iha2)), f, df);
             W = Wa_norm * f;
             coef = Ga_norm * df;
         } else {
-            const float sig = sigma3<float, K>();
-            const float ib = vi.y;
-            const double ibd = ib;
-            const float ib3 = ib * ib * ib;
             float fb, dfb;
-            shape32<K>(__builtin_amdgcn_sqrtf((float)(d2 * (ibd * ibd))), fb, dfb);
-            const float Wb = (sig * ib3) * fb, Gb = (sig * ib3 * ib) * dfb;
+            shape32<K>(__builtin_amdgcn_sqrtf((float)(d2 * nb.ib2)), fb, dfb);
+            const float Wb = nb.Wn * fb, Gb = nb.Gn * dfb;
             if constexpr (STRAT == PH_SCATTER) {
                 W = Wb;
                 coef = Gb;
-            } else {
+            } else {  // the norms carry the mean's 1/2
                 float fa, dfa;
                 shape32<K>(__builtin_amdgcn_sqrtf((float)(d2 * iha2)), fa, dfa);
-                W = 0.5f * (Wa_norm * fa + Wb);
-                coef = 0.5f * (Ga_norm * dfa + Gb);
+                W = Wa_norm * fa + Wb;
+                coef = Ga_norm * dfa + Gb;
             }
         }
+        (void)live;
         // float64 accumulation with fused products (explicit fma: the build
         // disables contraction so that the float64 path stays reference-exact)
-        const double w = live ? (double)(vi.x * W) : 0.0;
+        const double w = (double)(vi.x * W);
         S1 += w;
 #pragma unroll
         for (int j = 0; j < MS; ++j) {
@@ -304,7 +350,9 @@ template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT> struct Ac
         }
         if constexpr (MG + MD + MC > 0) {
             // grad W = delta f'(q) sigma / (r h^4); zero at r = 0 (kernel.jl:273-285)
-            const double s = (live && d2 > 0.0) ? (double)(vi.x * coef * __builtin_amdgcn_rsqf(d2f)) : 0.0;
+            double s;
+            if constexpr (wendland32<K>()) s = (double)(vi.x * coef);
+            else s = d2 > 0.0 ? (double)(vi.x * coef * __builtin_amdgcn_rsqf(d2f)) : 0.0;
             const double g0 = dx * s, g1 = dy * s, g2 = dz * s;
             Q[0] += g0;
             Q[1] += g1;
@@ -640,8 +688,8 @@ __device__ __attribute__((always_inline)) void bucket_c32_sample(const BucketVie
     const double iha = STRAT == PH_SCATTER || !valid || none ? 0.0 : 1.0 / ha;
     const double iha2 = iha * iha;
     const float inv_ha = (float)iha;
-    const float sig = sigma3<float, K>();
-    const float Wa_norm = sig * inv_ha * inv_ha * inv_ha, Ga_norm = Wa_norm * inv_ha;
+    float Wa_norm, Ga_norm;
+    sample_norms32<K, STRAT>(inv_ha, Wa_norm, Ga_norm);
     const int rw = c.rw;
 
     // per-lane bucket list in LDS: read back with a per-lane index in the

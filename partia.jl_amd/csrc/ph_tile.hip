@@ -25,13 +25,12 @@ namespace {
 
 // Grid kernels: one wave per 4 x 4 x 4 tile of grid points (Sampler<double, 1>).
 template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT>
-__global__ void __launch_bounds__(kMaxTileBlock) k_tile_c32(BucketView bv, const BucketRec* __restrict__ recs,
-                                                     const double* __restrict__ rec, CatChunk c,
-                                                     Sampler<double, 1> smp, const double* __restrict__ ha_s,
-                                                     float* __restrict__ out, u64* counter) {
+__global__ void __launch_bounds__(kMaxTileBlock)
+k_tile_c32(BucketView bv, const BucketRec* __restrict__ recs, const double* __restrict__ rec, CatChunk c,
+           Sampler<double, 1> smp, const double* __restrict__ ha_s, float* __restrict__ out, u64* counter) {
     extern __shared__ double tile_smem[];
     const int wv = threadIdx.x / kWave;
-    const int slot = 4 + (EXACT == 2 ? kC2Rec : c.rw);
+    const int slot = kC32Head + (EXACT == 2 ? kC2Rec : c.rw);
     i64 s = 0;
     double p[3] = {0.0, 0.0, 0.0};
     const bool valid = smp.get(s, p[0], p[1], p[2]);
@@ -131,12 +130,14 @@ void launch_tile_ks(const BucketView& bv, const double* rec, const CatChunk& c, 
     }
     float* o = static_cast<float*>(out);
     if (small)
-        launch_tile_kernel(k_tile_c32<K, STRAT, 2, 0, 0, 0, 0>, 4 + c.rw, bv, rec, c, ga, S, ha, o, counters, s, ev);
+        launch_tile_kernel(k_tile_c32<K, STRAT, 2, 0, 0, 0, 0>, kC32Head + c.rw, bv, rec, c, ga, S, ha, o, counters, s,
+                           ev);
     else if (c2_shared(c) && c.rw == kC2Rec)
-        launch_tile_kernel(k_tile_c32<K, STRAT, 2, 1, 1, 1, 2>, 4 + kC2Rec, bv, rec, c, ga, S, ha, o, counters, s, ev);
+        launch_tile_kernel(k_tile_c32<K, STRAT, 2, 1, 1, 1, 2>, kC32Head + kC2Rec, bv, rec, c, ga, S, ha, o, counters,
+                           s, ev);
     else
-        launch_tile_kernel(k_tile_c32<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0>, 4 + c.rw, bv, rec, c, ga, S, ha, o,
-                           counters, s, ev);
+        launch_tile_kernel(k_tile_c32<K, STRAT, kMaxS, kMaxG, kMaxD, kMaxC, 0>, kC32Head + c.rw, bv, rec, c, ga, S, ha,
+                           o, counters, s, ev);
 }
 
 template <int K>

@@ -23,12 +23,15 @@ typedef __attribute__((address_space(3))) double LdsDouble;
 
 // Record width of the C2 layout {vi, s0, s1, d0, d1, d2}.
 constexpr int kC2Rec = 6;
+// Doubles of an FP32C staged leaf before its record: x, y, z, (K h)^2, the
+// float64 (1/h)^2 and the float32 norms of leaf_norms32.
+constexpr int kC32Head = 6;
 
 // LDS of one wave, in doubles: 64 staged leaves and one zero slot of `slot`
 // doubles each, then 64 leaf indices, rounded up to an even count so that
 // every wave's region is 16-byte aligned (it also holds the nearest-h search's
-// 64 LeafRec candidates).  FP32C: slot = 4 + rw (x, y, z, (K h)^2, record);
-// float64: slot = 5 + rw (x, y, z, (K h)^2, h, record).
+// 64 LeafRec candidates).  FP32C: slot = kC32Head + rw (x, y, z, (K h)^2,
+// (1/h)^2, norms, record); float64: slot = 5 + rw (x, y, z, (K h)^2, h, record).
 __host__ __device__ constexpr int tile_wave_doubles(int slot) {
     return ((kWave + 1) * slot + kWave / 2 + 1) & ~1;
 }
@@ -95,7 +98,7 @@ __device__ __attribute__((always_inline)) u64 test_staged(const LdsDouble* st, i
 }
 
 // FP32C body: sample s at p (valid) with the nearest particle's h ha;
-// wave_lds is this wave's tile_wave_doubles(4 + rw) doubles of LDS.  recs is
+// wave_lds is this wave's tile_wave_doubles(kC32Head + rw) doubles of LDS.  recs is
 // bv.recs, restrict-qualified, so the walk's wave-uniform record loads are
 // scalar loads.
 template <int K, int STRAT, int MS, int MG, int MD, int MC, int EXACT>
@@ -107,7 +110,7 @@ __device__ __attribute__((always_inline)) void tile_c32_sample(const BucketView&
                                                                double* wave_lds, u64& accepted) {
     const int lane = threadIdx.x & (kWave - 1);
     const int rw = EXACT == 2 ? kC2Rec : c.rw;  // the C2 layout: constant LDS offsets
-    const int slot = 4 + rw;  // doubles per staged leaf
+    const int slot = kC32Head + rw;  // doubles per staged leaf
     // an LDS-typed pointer: 32-bit offsets in the inner loops
     LdsDouble* st = (LdsDouble*)wave_lds;
     u32* lidx = reinterpret_cast<u32*>(wave_lds + (kWave + 1) * slot);
@@ -122,12 +125,12 @@ __device__ __attribute__((always_inline)) void tile_c32_sample(const BucketView&
     const double iha = STRAT == PH_SCATTER || !live_lane ? 0.0 : 1.0 / ha;
     const double iha2 = iha * iha;
     const float inv_ha = (float)iha;
-    const float sig = sigma3<float, K>();
-    const float Wa_norm = sig * inv_ha * inv_ha * inv_ha, Ga_norm = Wa_norm * inv_ha;
+    float Wa_norm, Ga_norm;
+    sample_norms32<K, STRAT>(inv_ha, Wa_norm, Ga_norm);
     // the zero slot read by lanes without a pending neighbour: position 0,
-    // record {m / rho = 0, 1 / h = 1} and zero columns, so every term is 0
+    // norms 0, record {m / rho = 0, 1 / h = 1} and zero columns, so every term is 0
     if (lane < slot)
-        st[kWave * slot + lane] = lane == 4 ? __builtin_bit_cast(double, make_float2(0.f, 1.f)) : 0.0;
+        st[kWave * slot + lane] = lane == kC32Head ? __builtin_bit_cast(double, make_float2(0.f, 1.f)) : 0.0;
 
     AccC<K, STRAT, MS, MG, MD, MC, EXACT> acc;
     acc.init();
@@ -150,7 +153,12 @@ __device__ __attribute__((always_inline)) void tile_c32_sample(const BucketView&
             const double rb = Kd * L.h;  // float64, as the reference
             d[3] = rb * rb;
             const double* r = rec + (size_t)k * rw;
-            for (int q = 0; q < rw; ++q) d[4 + q] = r[q];
+            for (int q = 0; q < rw; ++q) d[kC32Head + q] = r[q];
+            if constexpr (STRAT != PH_GATHER) {  // the neighbour's norms, once per staged leaf
+                const LeafNorms nb = leaf_norms32<K, STRAT>(__builtin_bit_cast(float2, r[0]).y);
+                d[4] = nb.ib2;
+                d[5] = __builtin_bit_cast(double, make_float2(nb.Wn, nb.Gn));
+            }
         }
         wave_sync();
         u64 mask = live_lane ? test_staged<STRAT>(st, slot, nl, p, r2a) : 0;
@@ -164,8 +172,9 @@ __device__ __attribute__((always_inline)) void tile_c32_sample(const BucketView&
             mask &= mask - 1;
             const LdsDouble* d = st + (live ? b : kWave) * slot;
             const double dx = p[0] - d[0], dy = p[1] - d[1], dz = p[2] - d[2];
-            acc.add_r(live, c, [&](int pos) { return d[4 + pos]; }, dx, dy, dz, (dx * dx + dy * dy) + dz * dz, iha2,
-                      Wa_norm, Ga_norm);
+            const float2 nw = __builtin_bit_cast(float2, (double)d[5]);
+            acc.add_rn(live, c, [&](int pos) { return d[kC32Head + pos]; }, LeafNorms{d[4], nw.x, nw.y}, dx, dy, dz,
+                       (dx * dx + dy * dy) + dz * dz, iha2, Wa_norm, Ga_norm);
         }
         wave_sync();  // every lane is done with the staged leaves
         if (!(i < bv.nrec && steps > 0)) break;
