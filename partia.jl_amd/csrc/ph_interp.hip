@@ -15,6 +15,8 @@
 // column-major flattening) and each wavefront owns a 4x4x4 block of grid points,
 // so the 64 lanes of a wave walk nearly the same nodes; workgroups are remapped
 // so that each of the 8 XCDs (separate L2 caches) owns a contiguous run of tiles.
+#include <type_traits>
+
 #include "ph_interp.h"
 #include "ph_interp_dev.h"
 #include "ph_primitives.h"
@@ -266,7 +268,7 @@ __global__ void __launch_bounds__(kBlock) k_lines(TreeView<T> tv, const T* __res
 // records come through the kernels' restrict-qualified parameters (lnodes,
 // leaves, lesc = tv's leaf escapes), so their wave-uniform loads are scalar
 // loads, and every lane tests every leaf (no branch around the load).
-template <class T, class Leaf>
+template <int AX = -1, class T, class Leaf>
 __device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, const LineNode<T>* __restrict__ lnodes,
                                                          const LeafRec<T>* __restrict__ leaves,
                                                          const i32* __restrict__ lesc, const Line<T>& l, bool valid,
@@ -274,7 +276,7 @@ __device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, 
     const i64 n = tv.n;
     auto visit = [&](i64 k, bool active) {
         const LeafRec<T> L = leaves[k];
-        const T d2 = d2_line_leaf(l, L);
+        const T d2 = d2_line_leaf_ax<AX>(l, L);
         const bool close = d2 <= radius2<T, PH_SCATTER>(Kvalid, T(0), L.h);
         leaf(k, active & close, d2, L);
     };
@@ -295,7 +297,7 @@ __device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, 
             node = esc;
         } else {
             const LineNode<T> N = lnodes[node - 1];
-            const bool prune = line_node_prune(l, N, active);
+            const bool prune = line_node_prune<AX>(l, N, active);
             resume = (active && prune) ? N.escape : resume;
             const bool descend = active && !prune;
             active = descend;
@@ -311,37 +313,22 @@ __device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, 
 // LDS (one leaf per lane) and the lanes accumulate their own accepted leaves in
 // lockstep, lowest bit first, so each ray adds its neighbours in the per-ray
 // DFS's order with the same arithmetic (the distance is recomputed from the
-// staged leaf with the same operations).  The line-integrated table is staged
-// in LDS.  MS: scalar outputs staged per leaf (at least c.ns).
-template <class T, int K, int SRC, int MS>
-__global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const LineNode<T>* __restrict__ lnodes,
-                                                       const LeafRec<T>* __restrict__ leaves,
-                                                       const i32* __restrict__ lesc, const T* __restrict__ cols,
-                                                       CatChunk c,
-                                                       LineSampler<T, K, PH_SCATTER, SRC> smp,
-                                                       const T* __restrict__ inorm, T dq, T rdq,
-                                                       T* __restrict__ out, u64* counter) {
+// staged leaf with the same operations).  MS: scalar outputs staged per leaf
+// (at least c.ns).  AX >= 0: the rays run along axis AX (d2_line_point).
+template <int AX, int MS, class T, int K>
+__device__ __attribute__((always_inline)) void lines_wave_body(
+    const TreeView<T>& tv, const LineNode<T>* __restrict__ lnodes, const LeafRec<T>* __restrict__ leaves,
+    const i32* __restrict__ lesc, const T* __restrict__ cols, const CatChunk& c, const Line<T>& l, bool valid,
+    const T* tab, T dq, T rdq, T* st, u32* lidx, LineAcc<T, K>& a, u64& accepted) {
     constexpr int kSlot = 6 + MS;  // x, y, z, 1 / h, m, rho, the scalars' columns
-    __shared__ T tab[PH_LUT_N_ENTRIES];
-    __shared__ T stage[kBlock / kWave][kWave * kSlot];
-    __shared__ u32 sidx[kBlock / kWave][kWave];
-    for (int i = threadIdx.x; i < PH_LUT_N_ENTRIES; i += kBlock) tab[i] = inorm[i];
-    __syncthreads();
-    i64 s = 0;
-    Line<T> l{};  // lanes past the end keep a zero line
-    const bool valid = smp.get(s, l);
-    if (__ballot(valid) == 0) return;  // wave-uniform; no barrier follows
     const int lane = threadIdx.x & (kWave - 1);
-    T* __restrict__ st = stage[threadIdx.x / kWave];
-    u32* __restrict__ lidx = sidx[threadIdx.x / kWave];
     const i64 n = tv.n;
     const T* __restrict__ mcol = cols + 3 * n;
     const T* __restrict__ rcol = cols + 5 * n;
     const T* qcol[MS];
 #pragma unroll
     for (int j = 0; j < MS; ++j) qcol[j] = cols + (i64)c.rsrc[c.spos[j < c.ns ? j : 0]] * n;
-    LineAcc<T, K> a;
-    u64 accepted = 0, mask = 0;
+    u64 mask = 0;
     int nst = 0;  // listed leaves (wave-uniform)
     auto flush = [&]() {
         wave_sync();
@@ -371,20 +358,63 @@ __global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const Lin
 #pragma unroll
             for (int j = 0; j < MS; ++j) qv[j] = d[6 + j];
             // lanes without a neighbour left read leaf 0 and add exact zeros
-            a.template add_sel_inv<MS>(live, d[4], d[5], qv, c, d2_line_leaf(l, L), d[3], tab, dq, rdq);
+            a.template add_sel_inv<MS>(live, d[4], d[5], qv, c, d2_line_leaf_ax<AX>(l, L), d[3], tab, dq, rdq);
         }
         wave_sync();  // every lane is done with the staged leaves
         nst = 0;
     };
-    wave_walk(tv, lnodes, leaves, lesc, l, valid, T(support_of<K>()), [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
-        (void)d2;
-        (void)L;
-        if (__ballot(take) == 0) return;  // wave-uniform
-        if (lane == 0) lidx[nst] = (u32)k;
-        mask |= take ? (u64(1) << nst) : u64(0);
-        if (++nst == kWave) flush();
-    });
+    wave_walk<AX>(tv, lnodes, leaves, lesc, l, valid, T(support_of<K>()),
+                  [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
+                      (void)d2;
+                      (void)L;
+                      if (__ballot(take) == 0) return;  // wave-uniform
+                      if (lane == 0) lidx[nst] = (u32)k;
+                      mask |= take ? (u64(1) << nst) : u64(0);
+                      if (++nst == kWave) flush();
+                  });
     if (nst > 0) flush();
+}
+
+// Coordinates below 2^(emax - 2) in magnitude: differences of two of them are finite.
+template <class T> __device__ inline bool coord_ok(T x) {
+    return fabs(x) <= (std::is_same<T, float>::value ? T(0x1p126f) : T(0x1p1022));
+}
+
+// The kernel.  AX >= 0 (image-plane rays along axis AX, chosen by the host)
+// takes the axis-aligned body when the tree's coordinates are bounded (flag
+// lflag, from k_line_nodes) and so are the wave's ray origins; otherwise, and
+// for AX < 0, the general body.  The line-integrated table is staged in LDS.
+template <class T, int K, int SRC, int MS, int AX>
+__global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const LineNode<T>* __restrict__ lnodes,
+                                                       const LeafRec<T>* __restrict__ leaves,
+                                                       const i32* __restrict__ lesc, const u32* __restrict__ lflag,
+                                                       const T* __restrict__ cols, CatChunk c,
+                                                       LineSampler<T, K, PH_SCATTER, SRC> smp,
+                                                       const T* __restrict__ inorm, T dq, T rdq,
+                                                       T* __restrict__ out, u64* counter) {
+    __shared__ T tab[PH_LUT_N_ENTRIES];
+    __shared__ T stage[kBlock / kWave][kWave * (6 + MS)];
+    __shared__ u32 sidx[kBlock / kWave][kWave];
+    for (int i = threadIdx.x; i < PH_LUT_N_ENTRIES; i += kBlock) tab[i] = inorm[i];
+    __syncthreads();
+    i64 s = 0;
+    Line<T> l{};  // lanes past the end keep a zero line
+    const bool valid = smp.get(s, l);
+    if (__ballot(valid) == 0) return;  // wave-uniform; no barrier follows
+    T* st = stage[threadIdx.x / kWave];
+    u32* lidx = sidx[threadIdx.x / kWave];
+    LineAcc<T, K> a;
+    u64 accepted = 0;
+    bool axis = false;
+    if constexpr (AX >= 0) {
+        const bool ok = coord_ok(l.o[0]) && coord_ok(l.o[1]) && coord_ok(l.o[2]);
+        axis = *lflag != 0 && __ballot(!ok) == 0;  // wave-uniform
+    }
+    if (axis)
+        lines_wave_body<(AX < 0 ? 0 : AX), MS>(tv, lnodes, leaves, lesc, cols, c, l, valid, tab, dq, rdq, st, lidx,
+                                                a, accepted);
+    else
+        lines_wave_body<-1, MS>(tv, lnodes, leaves, lesc, cols, c, l, valid, tab, dq, rdq, st, lidx, a, accepted);
     if (valid) a.store(c, out, smp.S, s);
     count_add(counter, valid ? accepted : 0);
 }
@@ -557,40 +587,78 @@ void launch_points_p(ph_lbvh* B, const CatChunk& c, int strategy, const PointLis
     }
 }
 
-// LineNode records of every internal node of a tree with kernel K.
+// LineNode records of the n - 1 internal nodes of a tree with kernel K, and
+// the flag of the axis-aligned walk: *flag (1 beforehand) is cleared when a
+// leaf coordinate is not finite or not below 2^(emax - 2) in magnitude (the
+// node boxes and centres lie within the leaves' range).
 template <class T, int K>
-__global__ void __launch_bounds__(kBlock) k_line_nodes(const NodeRec<T>* __restrict__ nodes, i64 m,
-                                                       LineNode<T>* __restrict__ out) {
+__global__ void __launch_bounds__(kBlock) k_line_nodes(const NodeRec<T>* __restrict__ nodes,
+                                                       const LeafRec<T>* __restrict__ leaves, i64 n,
+                                                       LineNode<T>* __restrict__ out, u32* flag) {
     const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= m) return;
-    out[i] = line_node<T>(nodes[i], T(support_of<K>()));
+    if (i >= n) return;
+    if (i < n - 1) out[i] = line_node<T>(nodes[i], T(support_of<K>()));
+    const LeafRec<T> L = leaves[i];
+    if (!(coord_ok(L.x) && coord_ok(L.y) && coord_ok(L.z))) atomicAnd(flag, 0u);
+}
+__global__ void k_set_u32(u32* p, u32 v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
 }
 
-// The tree's LineNode records (the first call builds them; the tree is immutable).
+// The tree's LineNode records and, after them, the axis flag (the first call
+// builds them; the tree is immutable).
 template <class T, int K> const LineNode<T>* line_nodes(ph_lbvh* B, hipStream_t s) {
     if (!B->lnodes_ready) {
-        const i64 m = B->n > 1 ? B->n - 1 : 1;
-        B->lnodes.alloc(sizeof(LineNode<T>) * (size_t)m);
-        if (B->n > 1) {
-            k_line_nodes<T, K><<<grid_for(B->n - 1), kBlock, 0, s>>>(B->nodes.as<NodeRec<T>>(), B->n - 1,
-                                                                    B->lnodes.as<LineNode<T>>());
-            PH_HIP(hipGetLastError());
-        }
+        const i64 m = B->n > 1 ? B->n - 1 : 0;
+        B->lnodes.alloc(sizeof(LineNode<T>) * (size_t)(m + 1));
+        LineNode<T>* ln = B->lnodes.as<LineNode<T>>();
+        u32* flag = reinterpret_cast<u32*>(ln + m);
+        k_set_u32<<<1, 1, 0, s>>>(flag, 1u);
+        PH_HIP(hipGetLastError());
+        k_line_nodes<T, K><<<grid_for(B->n), kBlock, 0, s>>>(B->nodes.as<NodeRec<T>>(), B->leaves.as<LeafRec<T>>(),
+                                                            B->n, ln, flag);
+        PH_HIP(hipGetLastError());
         B->lnodes_ready = true;
     }
     return B->lnodes.as<LineNode<T>>();
 }
+template <class T> const u32* line_flag(const ph_lbvh* B) {
+    return reinterpret_cast<const u32*>(B->lnodes.as<LineNode<T>>() + (B->n > 1 ? B->n - 1 : 0));
+}
 
-template <class T, int K, int SRC, class Smp>
-void launch_lines_wave(const TreeView<T>& tv, const LineNode<T>* ln, const T* cols, const CatChunk& c, const Smp& smp,
-                       unsigned g, const T* tab, T dq, T* out, u64* counters, hipStream_t s) {
+// The axis of image-plane rays with direction exactly +-e_axis, else -1.
+inline int ray_axis(const LineGrid& lg) {
+    for (int a = 0; a < 3; ++a) {
+        bool ok = lg.d[a] == 1.0 || lg.d[a] == -1.0;
+        for (int d = 0; d < 3; ++d)
+            if (d != a && lg.d[d] != 0.0) ok = false;
+        if (ok) return a;
+    }
+    return -1;
+}
+
+template <class T, int K, int SRC, int AX, class Smp>
+void launch_lines_wave_ax(const TreeView<T>& tv, const LineNode<T>* ln, const u32* flag, const T* cols,
+                          const CatChunk& c, const Smp& smp, unsigned g, const T* tab, T dq, T* out, u64* counters,
+                          hipStream_t s) {
     const T rdq = T(1) / dq;  // correctly rounded (host IEEE division)
     if (c.ns <= 1)
-        k_lines_wave<T, K, SRC, 1><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, tv.leaf_escape, cols, c, smp, tab, dq, rdq,
-                                                        out, counters);
+        k_lines_wave<T, K, SRC, 1, AX><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, tv.leaf_escape, flag, cols, c, smp, tab,
+                                                            dq, rdq, out, counters);
     else
-        k_lines_wave<T, K, SRC, kMaxS><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, tv.leaf_escape, cols, c, smp, tab, dq,
-                                                            rdq, out, counters);
+        k_lines_wave<T, K, SRC, kMaxS, AX><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, tv.leaf_escape, flag, cols, c, smp,
+                                                                tab, dq, rdq, out, counters);
+}
+template <class T, int K, int SRC, class Smp>
+void launch_lines_wave(const TreeView<T>& tv, const LineNode<T>* ln, const u32* flag, const T* cols,
+                       const CatChunk& c, const Smp& smp, int axis, unsigned g, const T* tab, T dq, T* out,
+                       u64* counters, hipStream_t s) {
+    switch (SRC == 1 ? axis : -1) {
+    case 0: launch_lines_wave_ax<T, K, SRC, 0>(tv, ln, flag, cols, c, smp, g, tab, dq, out, counters, s); break;
+    case 1: launch_lines_wave_ax<T, K, SRC, 1>(tv, ln, flag, cols, c, smp, g, tab, dq, out, counters, s); break;
+    case 2: launch_lines_wave_ax<T, K, SRC, 2>(tv, ln, flag, cols, c, smp, g, tab, dq, out, counters, s); break;
+    default: launch_lines_wave_ax<T, K, SRC, -1>(tv, ln, flag, cols, c, smp, g, tab, dq, out, counters, s); break;
+    }
 }
 
 template <class T, int K>
@@ -602,13 +670,15 @@ void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const Lin
     const T dq = T((1.0 * support_of<K>()) / 4095.0) - T(0.0);
     const T* tab = inorm + (size_t)K * PH_LUT_N_ENTRIES;
     const LineNode<T>* ln = wave ? line_nodes<T, K>(B, s) : nullptr;
+    const u32* flag = wave ? line_flag<T>(B) : nullptr;
     if (ev) PH_HIP(hipEventRecord(ev[0], s));
     if (lg) {
         const i64 tiles = ((lg->nx + 7) / 8) * ((lg->ny + 7) / 8);
         const unsigned g = (unsigned)((tiles + 3) / 4);
         LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S, line_xcd(tl_launch, g), line_order(tl_launch)};
         if (wave)
-            launch_lines_wave<T, K, 1>(tv, ln, cols, c, smp, g, tab, dq, static_cast<T*>(out), counters, s);
+            launch_lines_wave<T, K, 1>(tv, ln, flag, cols, c, smp, ray_axis(*lg), g, tab, dq, static_cast<T*>(out),
+                                       counters, s);
         else
             k_lines<T, K, PH_SCATTER, 1><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out),
                                                               counters);
@@ -616,7 +686,8 @@ void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const Lin
         const unsigned g = (unsigned)((S + kBlock - 1) / kBlock);
         LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S, line_xcd(tl_launch, g)};
         if (wave)
-            launch_lines_wave<T, K, 0>(tv, ln, cols, c, smp, g, tab, dq, static_cast<T*>(out), counters, s);
+            launch_lines_wave<T, K, 0>(tv, ln, flag, cols, c, smp, -1, g, tab, dq, static_cast<T*>(out), counters,
+                                       s);
         else
             k_lines<T, K, PH_SCATTER, 0><<<g, kBlock, 0, s>>>(tv, cols, c, smp, tab, dq, static_cast<T*>(out),
                                                               counters);

@@ -78,6 +78,32 @@ template <class T> __device__ inline T d2_line_leaf(const Line<T>& l, const Leaf
     }
     return D2 - Dm * Dm;
 }
+// The terms D2 = |D|^2 and Dm = D.m of the line distance d2 = D2 - Dm^2 with
+// D = c - o, in the reference's order.  AX = 0, 1, 2: m is +-e_AX and every
+// component of D is finite; then D_d m_d = +-0 for d != AX, those zeros leave
+// the sum unchanged (up to the sign of a zero, which Dm^2 drops), and
+// Dm = +-D_AX gives the same Dm^2 with two products and two sums fewer.
+template <int AX, class T> __device__ inline T d2_line_point(const Line<T>& l, const T (&c)[3]) {
+    T D[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) D[d] = c[d] - l.o[d];
+    T D2 = T(0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) D2 += D[d] * D[d];
+    T Dm;
+    if constexpr (AX >= 0) {
+        Dm = D[AX];
+    } else {
+        Dm = T(0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) Dm += D[d] * l.m[d];
+    }
+    return D2 - Dm * Dm;
+}
+template <int AX, class T> __device__ inline T d2_line_leaf_ax(const Line<T>& l, const LeafRec<T>& L) {
+    const T c[3] = {L.x, L.y, L.z};
+    return d2_line_point<AX>(l, c);
+}
 template <class T> __device__ inline T d2_line_aabb_lb(const Line<T>& l, const NodeRec<T>& N) {
     // _line_intersects_aabb (:878-916)
     T tmin = -T(INFINITY), tmax = T(INFINITY);
@@ -198,21 +224,22 @@ template <class T> __device__ inline LineNode<T> line_node(const NodeRec<T>& N, 
 // active lane does, the slab test is skipped; lanes with dc2 >= thr_hi prune
 // unless the slab test hits; the rest (NaN, or within the margins) evaluate the
 // reference's square roots.
-template <class T> __device__ inline bool line_node_prune(const Line<T>& l, const LineNode<T>& N, bool active) {
-    T D2 = T(0), Dm = T(0);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const T D = N.c[d] - l.o[d];
-        D2 += D * D;
-        Dm += D * l.m[d];
-    }
-    const T dc2 = D2 - Dm * Dm;
+// AX >= 0: m = +-e_AX and every coordinate finite (d2_line_point); the slab
+// test then reduces to the two axes with m_d = 0 (the AX axis's interval
+// [a - o, b - o] is never empty for a finite box).
+template <int AX = -1, class T>
+__device__ inline bool line_node_prune(const Line<T>& l, const LineNode<T>& N, bool active) {
+    const T dc2 = d2_line_point<AX>(l, N.c);
     const bool keep = dc2 <= N.r2 || dc2 <= N.thr_lo;
     if (__ballot(active && !keep) == 0) return false;  // wave-uniform
     bool hit = true;
     T tmin = -T(INFINITY), tmax = T(INFINITY);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
+        if constexpr (AX >= 0) {
+            if (d != AX && (l.o[d] < N.lo[d] || l.o[d] > N.hi[d])) hit = false;
+            continue;
+        }
         const T a = N.lo[d], b = N.hi[d], o = l.o[d], m = l.m[d];
         if (m == T(0)) {
             if (o < a || o > b) hit = false;
@@ -239,9 +266,16 @@ template <class T> __device__ inline bool line_node_prune(const Line<T>& l, cons
         }
     }
     bool far = dc2 >= N.thr_hi;
-    if (!keep && !far) {  // the reference's arithmetic
-        const T dl = sqrt(dc2) - N.sr;
-        far = dl * dl > N.rad2;
+    const bool band = active && !keep && !far;
+    if (__ballot(band) != 0) {  // wave-uniform: some lane within the margins
+        // an empty asm statement (no instruction) keeps the square root inside
+        // the rarely taken branch instead of being speculated above it
+        T x = dc2;
+        __asm__ volatile("" : "+v"(x));
+        if (band) {  // the reference's arithmetic
+            const T dl = sqrt(x) - N.sr;
+            far = dl * dl > N.rad2;
+        }
     }
     return active && !hit && !keep && far;
 }

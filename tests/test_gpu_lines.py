@@ -117,6 +117,63 @@ def test_line_grid_float32(backend):
             assert_same(line_grid(d, cat, Z_FRAME, ax, ay, precision=32), ref)
 
 
+AXIS_FRAMES = [
+    ([-0.5, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [1.0, 0.0, 0.0]),
+    ([0.0, 1.5, 0.0], [0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.0, -1.0, 0.0]),
+    ([0.0, 0.0, 1.5], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, -1.0]),
+    Z_FRAME,
+]
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("frame", range(len(AXIS_FRAMES)))
+def test_line_axis_frames_same_as_lane(backend, precision, frame):
+    """Image planes looking along +-x, +-y, +-z take the wave walk's axis-aligned
+    distances; the images equal the per-ray walk's bit for bit."""
+    c6, cols = cloud(7000, 61 + frame)
+    if precision == 32:
+        c6 = tuple(a.astype(np.float32).astype(np.float64) for a in c6)
+        cols = [a.astype(np.float32).astype(np.float64) for a in cols]
+    d = DeviceSet(backend.context, *c6, cols, kernel=frame % 6, precision=precision)
+    cat = dict(sslots=[1, 2], sflags=[0, 1])
+    ax, ay = synthetic.linrange(-0.1, 1.1, 37), synthetic.linrange(-0.1, 1.1, 23)
+    fr = AXIS_FRAMES[frame]
+    res = {}
+    for mode in (N.PH_TRAVERSAL_LANE, N.PH_TRAVERSAL_WAVE):
+        with traversal(d.ctx, mode):
+            res[mode] = line_grid(d, cat, fr, ax, ay, precision)
+    assert np.isfinite(res[N.PH_TRAVERSAL_LANE]).mean() > 0.5
+    assert_same(res[N.PH_TRAVERSAL_WAVE], res[N.PH_TRAVERSAL_LANE])
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_line_axis_frame_huge_coordinates(backend, precision):
+    """Coordinates beyond 2^(emax - 2) send the wave walk down the general
+    distances: ray origins there (those waves only; the others keep the axis
+    path) and a particle there (the whole tree).  The images still equal the
+    per-ray walk's."""
+    big = 1.0e38 if precision == 32 else 1.0e308
+    c6, cols = cloud(3000, 71)
+    ax = np.concatenate([synthetic.linrange(0.0, 1.0, 33), [big, -big, 0.25]])
+    ay = synthetic.linrange(0.0, 1.0, 9)
+    for outlier in (False, True):
+        p6, pc = c6, cols
+        if outlier:
+            p6 = tuple(np.append(v, e) for v, e in zip(c6, (big, 0.5, 0.5, 1.0, 0.05, 1.0)))
+            pc = [np.append(v, 1.0) for v in cols]
+        if precision == 32:
+            p6 = tuple(v.astype(np.float32).astype(np.float64) for v in p6)
+            pc = [v.astype(np.float32).astype(np.float64) for v in pc]
+        d = DeviceSet(backend.context, *p6, pc, kernel=0, precision=precision)
+        cat = dict(sslots=[1], sflags=[0])
+        res = {}
+        for mode in (N.PH_TRAVERSAL_LANE, N.PH_TRAVERSAL_WAVE):
+            with traversal(d.ctx, mode):
+                res[mode] = line_grid(d, cat, Z_FRAME, ax, ay, precision)
+        assert np.isfinite(res[N.PH_TRAVERSAL_LANE]).sum() > 100
+        assert_same(res[N.PH_TRAVERSAL_WAVE], res[N.PH_TRAVERSAL_LANE])
+
+
 @pytest.mark.parametrize("precision", [64, 32])
 def test_line_list_wave_same_as_lane(backend, precision):
     """An explicit list of random rays (some axis-aligned), walked per ray and per
