@@ -484,7 +484,8 @@ void build_bucket_trees(ph_lbvh* B, hipStream_t s, int ntrees, const int* bmax, 
         if (split[t0 + t]) nrec[t0 + t] = (i64)(h[2 * t + 1] - h[2 * t]) + (i64)(h[2 * t + 2] - h[2 * t + 1]);
     for (int t = 0; t < ntrees; ++t)  // record indices and escape links are int32
         if (nrec[t] > (i64)INT32_MAX - 16) throw Error{PH_ERR_CAPACITY, "bucket tree of more than 2^31 - 16 records"};
-    for (int t = 0; t < ntrees; ++t) recs[t]->alloc(sizeof(BucketRec) * (size_t)nrec[t]);
+    // one record of padding: the tile walks load a record and its successor together
+    for (int t = 0; t < ntrees; ++t) recs[t]->alloc(sizeof(BucketRec) * (size_t)(nrec[t] + 1));
     for (int t = 0; t < ntrees; ++t)
         if (!split[t]) {  // the whole tree is one bucket
             k_bucket_root<<<1, 1, 0, s>>>(n, B->root, B->nodes.as<NodeRec<double>>(),

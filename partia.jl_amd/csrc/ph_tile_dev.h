@@ -46,30 +46,46 @@ constexpr int kMaxTileBlock = 512;
 // (kept when any live lane's conservative test keeps them), in preorder = in
 // increasing leaf index, written to lidx until one more bucket would exceed 64
 // leaves.  Returns their count; 0 once the walk is done.  i and steps are
-// wave-uniform; the walk visits every record at most once.
+// wave-uniform; the walk visits every record at most once.  Each step loads a
+// record and the one after it together (the record arrays carry one record of
+// padding): the walk goes on to the next record in preorder whenever it keeps
+// an internal node or passes a bucket, and then that record's test needs no
+// load of its own.
 template <int STRAT>
 __device__ __attribute__((always_inline)) int union_batch(const BucketView& bv, const BucketRec* __restrict__ recs,
                                                           const Probe<STRAT>& pr, u64 live_mask, i32& i, i32& steps,
                                                           u32* lidx, int lane) {
     int nl = 0;
-    while (i < bv.nrec && steps > 0) {
-        --steps;
-        const BucketRec R = recs[i];
+    // one record: false when the batch is full (i unchanged), else i advanced
+    auto visit = [&](const BucketRec& R) -> bool {
         const bool keep = (pr.keep_mask(R) & live_mask) != 0;
         if (R.link >= 0) {
             i = __builtin_amdgcn_readfirstlane(keep ? i + 1 : R.link);
-            continue;
+            return true;
         }
         if (keep) {
             const int cnt = bucket_cnt(R);
-            if (nl + cnt > kWave) {  // starts the next batch
-                ++steps;
-                break;
-            }
+            if (nl + cnt > kWave) return false;  // starts the next batch
             if (lane < cnt) lidx[nl + lane] = (u32)(bucket_lo(R.link) + lane);
             nl += cnt;
         }
         i = __builtin_amdgcn_readfirstlane(i + 1);
+        return true;
+    };
+    while (i < bv.nrec && steps > 0) {
+        const i32 i0 = i;
+        const BucketRec R = recs[i0], R1 = recs[i0 + 1];
+        --steps;
+        if (!visit(R)) {
+            ++steps;
+            break;
+        }
+        if (i != i0 + 1 || !(i < bv.nrec && steps > 0)) continue;
+        --steps;
+        if (!visit(R1)) {
+            ++steps;
+            break;
+        }
     }
     return nl;
 }
