@@ -375,7 +375,7 @@ __device__ __attribute__((always_inline)) void lines_wave_body(
     if (nst > 0) flush();
 }
 
-// Coordinates below 2^(emax - 2) in magnitude: differences of two of them are finite.
+// Coordinates at most 2^(emax - 2) in magnitude: differences of two of them are finite.
 template <class T> __device__ inline bool coord_ok(T x) {
     return fabs(x) <= (std::is_same<T, float>::value ? T(0x1p126f) : T(0x1p1022));
 }
@@ -430,11 +430,11 @@ __global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const Lin
 // with D = x_b - o (0 at r = 0).  The terms are evaluated in T, summed over the
 // wave's rays in float64 and added with one float64 atomic per wave, particle
 // and parameter (so the float64 sums are order-dependent in the last bits).
-template <class T, int K, int SRC, bool SHEP>
+template <class T, int K, int SRC, bool SHEP, int AX>
 __global__ void __launch_bounds__(kBlock) k_lines_adjoint(TreeView<T> tv, const LineNode<T>* __restrict__ lnodes,
                                                           const LeafRec<T>* __restrict__ leaves,
-                                                          const i32* __restrict__ lesc, const T* __restrict__ cols,
-                                                          i32 qsrc,
+                                                          const i32* __restrict__ lesc, const u32* __restrict__ lflag,
+                                                          const T* __restrict__ cols, i32 qsrc,
                                                           LineSampler<T, K, PH_SCATTER, SRC> smp,
                                                           const T* __restrict__ inorm, T dq,
                                                           const T* __restrict__ cot, const u32* __restrict__ order,
@@ -451,52 +451,65 @@ __global__ void __launch_bounds__(kBlock) k_lines_adjoint(TreeView<T> tv, const 
     const T* __restrict__ mcol = cols + 3 * n;
     const T* __restrict__ rcol = cols + 5 * n;
     const T* __restrict__ acol = cols + (i64)qsrc * n;
-    T g = valid ? cot[s] : T(0);
-    T S1 = T(0), out = T(0);
-    if constexpr (SHEP) {  // the forward's sums, same arithmetic
-        T N = T(0);
-        wave_walk(tv, lnodes, leaves, lesc, l, valid, Kvalid, [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
+    // the walks with the axis-aligned distances (as k_lines_wave) or the general ones
+    bool axis = false;
+    if constexpr (AX >= 0) {
+        const bool ok = coord_ok(l.o[0]) && coord_ok(l.o[1]) && coord_ok(l.o[2]);
+        axis = *lflag != 0 && __ballot(!ok) == 0;  // wave-uniform
+    }
+    auto run = [&](auto axc) {
+        constexpr int AXV = decltype(axc)::value;
+        T g = valid ? cot[s] : T(0);
+        T S1 = T(0), out = T(0);
+        if constexpr (SHEP) {  // the forward's sums, same arithmetic
+            T N = T(0);
+            wave_walk<AXV>(tv, lnodes, leaves, lesc, l, valid, Kvalid,
+                           [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
+                const T mb = mcol[k], rb = rcol[k], Ab = acol[k];
+                const T dr = sqrt(d2 < T(0) ? T(0) : d2);
+                const T t = mb * line_W_nb<T, K>(dr, L.h, tab, dq) / rb;
+                S1 += take ? t : T(0);
+                N += take ? Ab * t : T(0);
+            });
+            if (S1 == T(0)) g = T(0);  // NaN output: no gradient
+            out = S1 == T(0) ? T(0) : N * (T(1) / S1);
+        }
+        const T c2 = SHEP ? (S1 == T(0) ? T(0) : T(1) / S1) : T(1);
+        wave_walk<AXV>(tv, lnodes, leaves, lesc, l, valid, Kvalid, [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
+            if (__ballot(take && g != T(0)) == 0) return;  // wave-uniform
             const T mb = mcol[k], rb = rcol[k], Ab = acol[k];
             const T dr = sqrt(d2 < T(0) ? T(0) : d2);
-            const T t = mb * line_W_nb<T, K>(dr, L.h, tab, dq) / rb;
-            S1 += take ? t : T(0);
-            N += take ? Ab * t : T(0);
+            const T invh = T(1) / L.h, q = dr * invh;
+            const T idxf = q / dq + T(1);
+            const T lo = T(1), hi = T(PH_LUT_N_ENTRIES - 1);
+            const T cf = idxf < lo ? lo : (idxf > hi ? hi : idxf);
+            i32 i = (i32)cf;
+            i = i < 1 ? 1 : (i > PH_LUT_N_ENTRIES - 1 ? PH_LUT_N_ENTRIES - 1 : i);
+            const T t = idxf - T(i);
+            const bool cut = q >= Kvalid;
+            const T I = cut ? T(0) : tab[i - 1] * (T(1) - t) + tab[i] * t;
+            const T dI = cut ? T(0) : (tab[i] - tab[i - 1]) / dq;
+            const T w = mb * (invh * I) / rb;
+            const T c1 = SHEP ? (Ab - out) * c2 : Ab;
+            const T v[3] = {g * c1 * ((invh * I) / rb), g * c2 * w, -(g * c1) * (mb / rb) * invh * invh * (I + q * dI)};
+            const T f = dr > T(0) ? g * c1 * (mb / rb) * invh * invh * dI / dr : T(0);
+            const T D[3] = {L.x - l.o[0], L.y - l.o[1], L.z - l.o[2]};
+            const T Dm = D[0] * l.m[0] + D[1] * l.m[1] + D[2] * l.m[2];
+            // lanes not taking the leaf contribute exact zeros (selected, not
+            // multiplied: their terms may be inf or NaN)
+            double part[6];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) part[j] = take ? (double)v[j] : 0.0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) part[3 + a] = take ? (double)(f * (D[a] - Dm * l.m[a])) : 0.0;
+            const double sum = wave_sum6(part);  // parameter lane >> 3
+            const int lane = threadIdx.x & (kWave - 1);
+            if ((lane & 7) == 0 && lane < 48 && sum != 0.0)
+                unsafeAtomicAdd(&grad[(i64)(lane >> 3) * n + order[k]], sum);
         });
-        if (S1 == T(0)) g = T(0);  // NaN output: no gradient
-        out = S1 == T(0) ? T(0) : N * (T(1) / S1);
-    }
-    const T c2 = SHEP ? (S1 == T(0) ? T(0) : T(1) / S1) : T(1);
-    wave_walk(tv, lnodes, leaves, lesc, l, valid, Kvalid, [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
-        if (__ballot(take && g != T(0)) == 0) return;  // wave-uniform
-        const T mb = mcol[k], rb = rcol[k], Ab = acol[k];
-        const T dr = sqrt(d2 < T(0) ? T(0) : d2);
-        const T invh = T(1) / L.h, q = dr * invh;
-        const T idxf = q / dq + T(1);
-        const T lo = T(1), hi = T(PH_LUT_N_ENTRIES - 1);
-        const T cf = idxf < lo ? lo : (idxf > hi ? hi : idxf);
-        i32 i = (i32)cf;
-        i = i < 1 ? 1 : (i > PH_LUT_N_ENTRIES - 1 ? PH_LUT_N_ENTRIES - 1 : i);
-        const T t = idxf - T(i);
-        const bool cut = q >= Kvalid;
-        const T I = cut ? T(0) : tab[i - 1] * (T(1) - t) + tab[i] * t;
-        const T dI = cut ? T(0) : (tab[i] - tab[i - 1]) / dq;
-        const T w = mb * (invh * I) / rb;
-        const T c1 = SHEP ? (Ab - out) * c2 : Ab;
-        const T v[3] = {g * c1 * ((invh * I) / rb), g * c2 * w, -(g * c1) * (mb / rb) * invh * invh * (I + q * dI)};
-        const T f = dr > T(0) ? g * c1 * (mb / rb) * invh * invh * dI / dr : T(0);
-        const T D[3] = {L.x - l.o[0], L.y - l.o[1], L.z - l.o[2]};
-        const T Dm = D[0] * l.m[0] + D[1] * l.m[1] + D[2] * l.m[2];
-        // lanes not taking the leaf contribute exact zeros (selected, not
-        // multiplied: their terms may be inf or NaN)
-        double part[6];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) part[j] = take ? (double)v[j] : 0.0;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) part[3 + a] = take ? (double)(f * (D[a] - Dm * l.m[a])) : 0.0;
-        const double sum = wave_sum6(part);  // parameter lane >> 3
-        const int lane = threadIdx.x & (kWave - 1);
-        if ((lane & 7) == 0 && lane < 48 && sum != 0.0) unsafeAtomicAdd(&grad[(i64)(lane >> 3) * n + order[k]], sum);
-    });
+    };
+    if (axis) run(std::integral_constant<int, (AX < 0 ? 0 : AX)>{});
+    else run(std::integral_constant<int, -1>{});
 }
 
 // grad[0 .. count) = 0 (a kernel, not a memset, so the call can be recorded).
@@ -718,6 +731,7 @@ void launch_adjoint_k(ph_lbvh* B, i32 qsrc, const LineList* ll, const LineGrid* 
     const T* tab = inorm + (size_t)K * PH_LUT_N_ENTRIES;
     const u32* order = B->order.as<u32>();
     const LineNode<T>* ln = line_nodes<T, K>(B, s);
+    const u32* flag = line_flag<T>(B);
     k_zero_f64<<<grid_for(6 * B->n), kBlock, 0, s>>>(grad, 6 * B->n);
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[0], s));
@@ -725,13 +739,21 @@ void launch_adjoint_k(ph_lbvh* B, i32 qsrc, const LineList* ll, const LineGrid* 
         const i64 tiles = ((lg->nx + 7) / 8) * ((lg->ny + 7) / 8);
         const unsigned g = (unsigned)((tiles + 3) / 4);
         LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S, line_xcd(tl_launch, g), line_order(tl_launch)};
-        k_lines_adjoint<T, K, 1, SHEP><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, tv.leaf_escape, cols, qsrc, smp, tab, dq,
-                                                             cot, order, grad);
+        auto launch = [&](auto axc) {
+            k_lines_adjoint<T, K, 1, SHEP, decltype(axc)::value><<<g, kBlock, 0, s>>>(
+                tv, ln, tv.leaves, tv.leaf_escape, flag, cols, qsrc, smp, tab, dq, cot, order, grad);
+        };
+        switch (ray_axis(*lg)) {
+        case 0: launch(std::integral_constant<int, 0>{}); break;
+        case 1: launch(std::integral_constant<int, 1>{}); break;
+        case 2: launch(std::integral_constant<int, 2>{}); break;
+        default: launch(std::integral_constant<int, -1>{}); break;
+        }
     } else {
         const unsigned g = (unsigned)((S + kBlock - 1) / kBlock);
         LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S, line_xcd(tl_launch, g)};
-        k_lines_adjoint<T, K, 0, SHEP><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, tv.leaf_escape, cols, qsrc, smp, tab, dq,
-                                                             cot, order, grad);
+        k_lines_adjoint<T, K, 0, SHEP, -1><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, tv.leaf_escape, flag, cols, qsrc,
+                                                                smp, tab, dq, cot, order, grad);
     }
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[1], s));
