@@ -103,35 +103,6 @@ struct EmitTrees {
     BucketRec* leaf_recs;
 };
 
-// L(u) by pointer jumping: after k_lc_init and r rounds of k_lc_jump, up[u]
-// is the ancestor 2^(r+1) edges above internal node u (0 past the root) and
-// lc[u] counts the left-child edges on the way there, u's own edge included.
-__device__ inline i32 lc_edge(i32 c, i32 root, const i32* __restrict__ parent, const i32* __restrict__ left,
-                              i32& up) {
-    up = c == root ? 0 : parent[c - 1];
-    return (up != 0 && left[up - 1] == c) ? 1 : 0;
-}
-__global__ void __launch_bounds__(kBlock) k_lc_init(i64 n, i32 root, const i32* __restrict__ parent,
-                                            const i32* __restrict__ left, i32* __restrict__ up,
-                                            i32* __restrict__ lc) {
-    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
-    if (i > n - 1) return;
-    i32 u1 = 0, u2 = 0;
-    i32 e = lc_edge((i32)i, root, parent, left, u1);
-    if (u1 != 0) e += lc_edge(u1, root, parent, left, u2);
-    up[i - 1] = u2;
-    lc[i - 1] = e;
-}
-__global__ void __launch_bounds__(kBlock) k_lc_jump(i64 n, const i32* __restrict__ up_in,
-                                            const i32* __restrict__ lc_in, i32* __restrict__ up_out,
-                                            i32* __restrict__ lc_out) {
-    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
-    if (i > n - 1) return;
-    const i32 u = up_in[i - 1];
-    up_out[i - 1] = u == 0 ? 0 : up_in[u - 1];
-    lc_out[i - 1] = lc_in[i - 1] + (u == 0 ? 0 : lc_in[u - 1]);
-}
-constexpr int kLcJumps = 2;  // pointers of 8 edges
 
 // Every tree in one pass: L(u) and the boxes are properties of the LBVH,
 // found once per node.
@@ -150,11 +121,7 @@ __global__ void __launch_bounds__(kBlock) k_bucket_emit(i64 n, const i32* __rest
     node_box(i, n, nodes, leaves, Ri);
     node_box(s == lo ? (lo + n - 1) : s, n, nodes, leaves, Rl);
     node_box(s + 1 == hi ? (hi + n - 1) : s + 1, n, nodes, leaves, Rr);
-    i64 Lc = lc[i - 1];
-    for (i64 c = up[i - 1], steps = n; c != 0 && steps > 0; --steps) {
-        Lc += lc[c - 1];
-        c = up[c - 1];
-    }
+    const i64 Lc = lc_total(i, n, up, lc);
     if (et.leaf_recs) {
         const i64 pre = Lc + 2 * (lo - 1);
         BucketRec R = Ri;

@@ -258,8 +258,9 @@ struct ph_lbvh {
     ph::DevBuf brecs16;   // 16-leaf bucket-tree records [nrec16]
     ph::DevBuf brecs;     // 4-leaf bucket-tree records [nrec]
     ph::DevBuf rec32;     // FP32-compensated records of the last call ({m/rho, 1/h}, columns)
-    // Node records of the per-wave line walks (LineNode<T> [n-1]: the box and
-    // the node-only terms of the line pruning test), built by the first line call.
+    // Records of the per-wave line walks (LineRec<T> [2n-1] in DFS preorder:
+    // the boxes and node-only terms of the line pruning test, the leaves; then
+    // one of padding and the axis flag), built by the first line call.
     bool lnodes_ready = false;
     ph::DevBuf lnodes;
 

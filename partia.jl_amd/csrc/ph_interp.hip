@@ -265,44 +265,49 @@ __global__ void __launch_bounds__(kBlock) k_lines(TreeView<T> tv, const T* __res
 // its own DFS (LinearBVHTraversalMacros.jl:539-644), in the same order, with the
 // same arithmetic as k_lines.  leaf(k, take, d2, L) is called by the whole wave
 // for every leaf of the union walk; take marks the lanes that accept it.  The
-// records come through the kernels' restrict-qualified parameters (lnodes,
-// leaves, lesc = tv's leaf escapes), so their wave-uniform loads are scalar
-// loads, and every lane tests every leaf (no branch around the load).
+// walk reads the tree's LineRec records in preorder (lrecs, 2n - 1 records and
+// one of padding): each step loads a record and the one after it together,
+// and when the walk goes on to the next record (a kept node's left child, or
+// the record after a leaf) it tests that one without a load of its own.  The
+// records come through the kernels' restrict-qualified parameters, so their
+// wave-uniform loads are scalar loads.
 template <int AX = -1, class T, class Leaf>
-__device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, const LineNode<T>* __restrict__ lnodes,
-                                                         const LeafRec<T>* __restrict__ leaves,
-                                                         const i32* __restrict__ lesc, const Line<T>& l, bool valid,
-                                                         T Kvalid, Leaf&& leaf) {
+__device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, const LineRec<T>* __restrict__ lrecs,
+                                                         const LeafRec<T>* __restrict__ leaves, const Line<T>& l,
+                                                         bool valid, T Kvalid, Leaf&& leaf) {
     const i64 n = tv.n;
-    auto visit = [&](i64 k, bool active) {
-        const LeafRec<T> L = leaves[k];
+    auto visit = [&](i64 k, const LeafRec<T>& L, bool active) {
         const T d2 = d2_line_leaf_ax<AX>(l, L);
         const bool close = d2 <= radius2<T, PH_SCATTER>(Kvalid, T(0), L.h);
         leaf(k, active & close, d2, L);
     };
     if (tv.root == 0) {  // n == 1: the reference's brute-force branch
-        for (i64 k = 0; k < n; ++k) visit(k, valid);
+        for (i64 k = 0; k < n; ++k) visit(k, leaves[k], valid);
         return;
     }
+    const i32 nrec = (i32)(2 * n - 1);
     bool active = valid;
-    i32 resume = -1;  // node at which an idle lane rejoins (0: never)
-    i32 node = tv.root;
-    for (i64 steps = 2 * n + 8; node != 0 && steps > 0; --steps) {
-        node = __builtin_amdgcn_readfirstlane(node);
-        active = active || node == resume;
-        if (node >= (i32)n) {
-            const i64 k = (i64)node - n;
-            const i32 esc = lesc[k];
-            visit(k, active);
-            node = esc;
+    i32 resume = -1;  // record at which an idle lane rejoins
+    i32 p = 0;        // the root
+    auto step = [&](const LineRec<T>& R) {
+        active = active || p == resume;
+        if (R.leaf >= 0) {
+            visit(R.leaf, LeafRec<T>{R.lo[0], R.lo[1], R.lo[2], R.hi[0]}, active);
+            p = p + 1;
         } else {
-            const LineNode<T> N = lnodes[node - 1];
-            const bool prune = line_node_prune<AX>(l, N, active);
-            resume = (active && prune) ? N.escape : resume;
+            const bool prune = line_node_prune<AX>(l, R, active);
+            resume = (active && prune) ? R.link : resume;
             const bool descend = active && !prune;
             active = descend;
-            node = __ballot(descend) ? N.left : N.escape;
+            p = __ballot(descend) ? p + 1 : R.link;
         }
+    };
+    for (i64 steps = 2 * n + 8; p < nrec && steps > 0; --steps) {
+        p = __builtin_amdgcn_readfirstlane(p);
+        const i32 p0 = p;
+        const LineRec<T> R = lrecs[p0], R1 = lrecs[p0 + 1];
+        step(R);
+        if (p == p0 + 1 && p < nrec && --steps > 0) step(R1);
     }
 }
 
@@ -317,8 +322,8 @@ __device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, 
 // (at least c.ns).  AX >= 0: the rays run along axis AX (d2_line_point).
 template <int AX, int MS, class T, int K>
 __device__ __attribute__((always_inline)) void lines_wave_body(
-    const TreeView<T>& tv, const LineNode<T>* __restrict__ lnodes, const LeafRec<T>* __restrict__ leaves,
-    const i32* __restrict__ lesc, const T* __restrict__ cols, const CatChunk& c, const Line<T>& l, bool valid,
+    const TreeView<T>& tv, const LineRec<T>* __restrict__ lrecs, const LeafRec<T>* __restrict__ leaves,
+    const T* __restrict__ cols, const CatChunk& c, const Line<T>& l, bool valid,
     const T* tab, T dq, T rdq, T* st, u32* lidx, LineAcc<T, K>& a, u64& accepted) {
     constexpr int kSlot = 6 + MS;  // x, y, z, 1 / h, m, rho, the scalars' columns
     const int lane = threadIdx.x & (kWave - 1);
@@ -363,7 +368,7 @@ __device__ __attribute__((always_inline)) void lines_wave_body(
         wave_sync();  // every lane is done with the staged leaves
         nst = 0;
     };
-    wave_walk<AX>(tv, lnodes, leaves, lesc, l, valid, T(support_of<K>()),
+    wave_walk<AX>(tv, lrecs, leaves, l, valid, T(support_of<K>()),
                   [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
                       (void)d2;
                       (void)L;
@@ -385,10 +390,10 @@ template <class T> __device__ inline bool coord_ok(T x) {
 // lflag, from k_line_nodes) and so are the wave's ray origins; otherwise, and
 // for AX < 0, the general body.  The line-integrated table is staged in LDS.
 template <class T, int K, int SRC, int MS, int AX>
-__global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const LineNode<T>* __restrict__ lnodes,
+__global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const LineRec<T>* __restrict__ lrecs,
                                                        const LeafRec<T>* __restrict__ leaves,
-                                                       const i32* __restrict__ lesc, const u32* __restrict__ lflag,
-                                                       const T* __restrict__ cols, CatChunk c,
+                                                       const u32* __restrict__ lflag, const T* __restrict__ cols,
+                                                       CatChunk c,
                                                        LineSampler<T, K, PH_SCATTER, SRC> smp,
                                                        const T* __restrict__ inorm, T dq, T rdq,
                                                        T* __restrict__ out, u64* counter) {
@@ -411,10 +416,10 @@ __global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const Lin
         axis = *lflag != 0 && __ballot(!ok) == 0;  // wave-uniform
     }
     if (axis)
-        lines_wave_body<(AX < 0 ? 0 : AX), MS>(tv, lnodes, leaves, lesc, cols, c, l, valid, tab, dq, rdq, st, lidx,
-                                                a, accepted);
+        lines_wave_body<(AX < 0 ? 0 : AX), MS>(tv, lrecs, leaves, cols, c, l, valid, tab, dq, rdq, st, lidx, a,
+                                                accepted);
     else
-        lines_wave_body<-1, MS>(tv, lnodes, leaves, lesc, cols, c, l, valid, tab, dq, rdq, st, lidx, a, accepted);
+        lines_wave_body<-1, MS>(tv, lrecs, leaves, cols, c, l, valid, tab, dq, rdq, st, lidx, a, accepted);
     if (valid) a.store(c, out, smp.S, s);
     count_add(counter, valid ? accepted : 0);
 }
@@ -431,10 +436,10 @@ __global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const Lin
 // wave's rays in float64 and added with one float64 atomic per wave, particle
 // and parameter (so the float64 sums are order-dependent in the last bits).
 template <class T, int K, int SRC, bool SHEP, int AX>
-__global__ void __launch_bounds__(kBlock) k_lines_adjoint(TreeView<T> tv, const LineNode<T>* __restrict__ lnodes,
+__global__ void __launch_bounds__(kBlock) k_lines_adjoint(TreeView<T> tv, const LineRec<T>* __restrict__ lrecs,
                                                           const LeafRec<T>* __restrict__ leaves,
-                                                          const i32* __restrict__ lesc, const u32* __restrict__ lflag,
-                                                          const T* __restrict__ cols, i32 qsrc,
+                                                          const u32* __restrict__ lflag, const T* __restrict__ cols,
+                                                          i32 qsrc,
                                                           LineSampler<T, K, PH_SCATTER, SRC> smp,
                                                           const T* __restrict__ inorm, T dq,
                                                           const T* __restrict__ cot, const u32* __restrict__ order,
@@ -463,7 +468,7 @@ __global__ void __launch_bounds__(kBlock) k_lines_adjoint(TreeView<T> tv, const 
         T S1 = T(0), out = T(0);
         if constexpr (SHEP) {  // the forward's sums, same arithmetic
             T N = T(0);
-            wave_walk<AXV>(tv, lnodes, leaves, lesc, l, valid, Kvalid,
+            wave_walk<AXV>(tv, lrecs, leaves, l, valid, Kvalid,
                            [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
                 const T mb = mcol[k], rb = rcol[k], Ab = acol[k];
                 const T dr = sqrt(d2 < T(0) ? T(0) : d2);
@@ -475,7 +480,7 @@ __global__ void __launch_bounds__(kBlock) k_lines_adjoint(TreeView<T> tv, const 
             out = S1 == T(0) ? T(0) : N * (T(1) / S1);
         }
         const T c2 = SHEP ? (S1 == T(0) ? T(0) : T(1) / S1) : T(1);
-        wave_walk<AXV>(tv, lnodes, leaves, lesc, l, valid, Kvalid, [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
+        wave_walk<AXV>(tv, lrecs, leaves, l, valid, Kvalid, [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
             if (__ballot(take && g != T(0)) == 0) return;  // wave-uniform
             const T mb = mcol[k], rb = rcol[k], Ab = acol[k];
             const T dr = sqrt(d2 < T(0) ? T(0) : d2);
@@ -600,17 +605,39 @@ void launch_points_p(ph_lbvh* B, const CatChunk& c, int strategy, const PointLis
     }
 }
 
-// LineNode records of the n - 1 internal nodes of a tree with kernel K, and
-// the flag of the axis-aligned walk: *flag (1 beforehand) is cleared when a
+// LineRec records of a tree with kernel K (n >= 2) in DFS preorder: internal
+// node u (1-based) with leaves [lo, hi] (1-based) sits at L(u) + 2 (lo - 1),
+// L(u) = its left-child edges from the root (lc_total), and its subtree ends
+// 2 (hi - lo) + 1 records later; a leaf child on the left follows its parent,
+// one on the right (leaf hi = s + 1 of split s) sits at L(u) + 2 s.  Each node
+// writes itself and its leaf children, so every record is written once.
+template <class T, int K>
+__global__ void __launch_bounds__(kBlock) k_line_recs(i64 n, const i32* __restrict__ left,
+                                                      const i32* __restrict__ range, const i32* __restrict__ up,
+                                                      const i32* __restrict__ lc, const NodeRec<T>* __restrict__ nodes,
+                                                      const LeafRec<T>* __restrict__ leaves,
+                                                      LineRec<T>* __restrict__ out) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
+    if (i > n - 1) return;
+    const i64 lo = range[2 * (i - 1)], hi = range[2 * (i - 1) + 1];
+    const i64 l = left[i - 1];
+    const i64 sp = l >= n ? lo : l;  // the split: left child covers [lo, sp]
+    const i64 Lc = lc_total(i, n, up, lc);
+    const i64 pre = Lc + 2 * (lo - 1);
+    out[pre] = line_rec_node<T>(nodes[i - 1], T(support_of<K>()), (i32)(pre + 2 * (hi - lo) + 1));
+    if (sp == lo) out[pre + 1] = line_rec_leaf<T>(leaves[lo - 1], (i32)(lo - 1), (i32)(pre + 2));
+    if (sp + 1 == hi) {
+        const i64 rp = Lc + 2 * sp;
+        out[rp] = line_rec_leaf<T>(leaves[hi - 1], (i32)(hi - 1), (i32)(rp + 1));
+    }
+}
+// The flag of the axis-aligned walk: *flag (1 beforehand) is cleared when a
 // leaf coordinate is not finite or not below 2^(emax - 2) in magnitude (the
 // node boxes and centres lie within the leaves' range).
-template <class T, int K>
-__global__ void __launch_bounds__(kBlock) k_line_nodes(const NodeRec<T>* __restrict__ nodes,
-                                                       const LeafRec<T>* __restrict__ leaves, i64 n,
-                                                       LineNode<T>* __restrict__ out, u32* flag) {
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_line_flag(const LeafRec<T>* __restrict__ leaves, i64 n, u32* flag) {
     const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    if (i < n - 1) out[i] = line_node<T>(nodes[i], T(support_of<K>()));
     const LeafRec<T> L = leaves[i];
     if (!(coord_ok(L.x) && coord_ok(L.y) && coord_ok(L.z))) atomicAnd(flag, 0u);
 }
@@ -618,25 +645,46 @@ __global__ void k_set_u32(u32* p, u32 v) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
 }
 
-// The tree's LineNode records and, after them, the axis flag (the first call
-// builds them; the tree is immutable).
-template <class T, int K> const LineNode<T>* line_nodes(ph_lbvh* B, hipStream_t s) {
+// The tree's LineRec records (2n - 1 in preorder, then one of padding read by
+// the walk's paired loads) and, after them, the axis flag; the first call
+// builds them (the tree is immutable).  A one-leaf tree has none (its walk is
+// the brute-force branch).
+template <class T> i64 line_rec_count(const ph_lbvh* B) { return B->n > 1 ? 2 * B->n - 1 : 0; }
+template <class T, int K> const LineRec<T>* line_recs(ph_lbvh* B, hipStream_t s) {
     if (!B->lnodes_ready) {
-        const i64 m = B->n > 1 ? B->n - 1 : 0;
-        B->lnodes.alloc(sizeof(LineNode<T>) * (size_t)(m + 1));
-        LineNode<T>* ln = B->lnodes.as<LineNode<T>>();
-        u32* flag = reinterpret_cast<u32*>(ln + m);
+        const i64 n = B->n, m = line_rec_count<T>(B);
+        B->lnodes.alloc(sizeof(LineRec<T>) * (size_t)(m + 2));
+        LineRec<T>* lr = B->lnodes.as<LineRec<T>>();
+        u32* flag = reinterpret_cast<u32*>(lr + m + 1);
         k_set_u32<<<1, 1, 0, s>>>(flag, 1u);
         PH_HIP(hipGetLastError());
-        k_line_nodes<T, K><<<grid_for(B->n), kBlock, 0, s>>>(B->nodes.as<NodeRec<T>>(), B->leaves.as<LeafRec<T>>(),
-                                                            B->n, ln, flag);
+        k_line_flag<T><<<grid_for(n), kBlock, 0, s>>>(B->leaves.as<LeafRec<T>>(), n, flag);
         PH_HIP(hipGetLastError());
+        if (n > 1) {
+            DevBuf lcb;  // up, lc pointers of two rounds (released into the context's cache)
+            lcb.alloc(sizeof(i32) * 4 * (size_t)(n - 1));
+            i32* a = lcb.as<i32>();
+            i32* b = a + 2 * (n - 1);
+            k_lc_init<<<grid_for(n - 1), kBlock, 0, s>>>(n, B->root, B->parent.as<i32>(), B->left.as<i32>(), a,
+                                                        a + (n - 1));
+            PH_HIP(hipGetLastError());
+            for (int r = 0; r < kLcJumps; ++r) {
+                k_lc_jump<<<grid_for(n - 1), kBlock, 0, s>>>(n, a, a + (n - 1), b, b + (n - 1));
+                PH_HIP(hipGetLastError());
+                std::swap(a, b);
+            }
+            k_line_recs<T, K><<<grid_for(n - 1), kBlock, 0, s>>>(n, B->left.as<i32>(), B->range.as<i32>(), a,
+                                                                a + (n - 1), B->nodes.as<NodeRec<T>>(),
+                                                                B->leaves.as<LeafRec<T>>(), lr);
+            PH_HIP(hipGetLastError());
+            if (!tl_cache) PH_HIP(hipStreamSynchronize(s));
+        }
         B->lnodes_ready = true;
     }
-    return B->lnodes.as<LineNode<T>>();
+    return B->lnodes.as<LineRec<T>>();
 }
 template <class T> const u32* line_flag(const ph_lbvh* B) {
-    return reinterpret_cast<const u32*>(B->lnodes.as<LineNode<T>>() + (B->n > 1 ? B->n - 1 : 0));
+    return reinterpret_cast<const u32*>(B->lnodes.as<LineRec<T>>() + line_rec_count<T>(B) + 1);
 }
 
 // The axis of image-plane rays with direction exactly +-e_axis, else -1.
@@ -651,19 +699,19 @@ inline int ray_axis(const LineGrid& lg) {
 }
 
 template <class T, int K, int SRC, int AX, class Smp>
-void launch_lines_wave_ax(const TreeView<T>& tv, const LineNode<T>* ln, const u32* flag, const T* cols,
+void launch_lines_wave_ax(const TreeView<T>& tv, const LineRec<T>* ln, const u32* flag, const T* cols,
                           const CatChunk& c, const Smp& smp, unsigned g, const T* tab, T dq, T* out, u64* counters,
                           hipStream_t s) {
     const T rdq = T(1) / dq;  // correctly rounded (host IEEE division)
     if (c.ns <= 1)
-        k_lines_wave<T, K, SRC, 1, AX><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, tv.leaf_escape, flag, cols, c, smp, tab,
-                                                            dq, rdq, out, counters);
+        k_lines_wave<T, K, SRC, 1, AX><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, flag, cols, c, smp, tab, dq, rdq, out,
+                                                            counters);
     else
-        k_lines_wave<T, K, SRC, kMaxS, AX><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, tv.leaf_escape, flag, cols, c, smp,
-                                                                tab, dq, rdq, out, counters);
+        k_lines_wave<T, K, SRC, kMaxS, AX><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, flag, cols, c, smp, tab, dq, rdq,
+                                                                out, counters);
 }
 template <class T, int K, int SRC, class Smp>
-void launch_lines_wave(const TreeView<T>& tv, const LineNode<T>* ln, const u32* flag, const T* cols,
+void launch_lines_wave(const TreeView<T>& tv, const LineRec<T>* ln, const u32* flag, const T* cols,
                        const CatChunk& c, const Smp& smp, int axis, unsigned g, const T* tab, T dq, T* out,
                        u64* counters, hipStream_t s) {
     switch (SRC == 1 ? axis : -1) {
@@ -682,7 +730,7 @@ void launch_lines_k(ph_lbvh* B, const CatChunk& c, const LineList* ll, const Lin
     // Q_2 - Q_1 of the table in T (line_integrated_kernel.jl:14)
     const T dq = T((1.0 * support_of<K>()) / 4095.0) - T(0.0);
     const T* tab = inorm + (size_t)K * PH_LUT_N_ENTRIES;
-    const LineNode<T>* ln = wave ? line_nodes<T, K>(B, s) : nullptr;
+    const LineRec<T>* ln = wave ? line_recs<T, K>(B, s) : nullptr;
     const u32* flag = wave ? line_flag<T>(B) : nullptr;
     if (ev) PH_HIP(hipEventRecord(ev[0], s));
     if (lg) {
@@ -730,7 +778,7 @@ void launch_adjoint_k(ph_lbvh* B, i32 qsrc, const LineList* ll, const LineGrid* 
     const T dq = T((1.0 * support_of<K>()) / 4095.0) - T(0.0);
     const T* tab = inorm + (size_t)K * PH_LUT_N_ENTRIES;
     const u32* order = B->order.as<u32>();
-    const LineNode<T>* ln = line_nodes<T, K>(B, s);
+    const LineRec<T>* ln = line_recs<T, K>(B, s);
     const u32* flag = line_flag<T>(B);
     k_zero_f64<<<grid_for(6 * B->n), kBlock, 0, s>>>(grad, 6 * B->n);
     PH_HIP(hipGetLastError());
@@ -741,7 +789,7 @@ void launch_adjoint_k(ph_lbvh* B, i32 qsrc, const LineList* ll, const LineGrid* 
         LineSampler<T, K, PH_SCATTER, 1> smp{*lg, S, line_xcd(tl_launch, g), line_order(tl_launch)};
         auto launch = [&](auto axc) {
             k_lines_adjoint<T, K, 1, SHEP, decltype(axc)::value><<<g, kBlock, 0, s>>>(
-                tv, ln, tv.leaves, tv.leaf_escape, flag, cols, qsrc, smp, tab, dq, cot, order, grad);
+                tv, ln, tv.leaves, flag, cols, qsrc, smp, tab, dq, cot, order, grad);
         };
         switch (ray_axis(*lg)) {
         case 0: launch(std::integral_constant<int, 0>{}); break;
@@ -752,8 +800,8 @@ void launch_adjoint_k(ph_lbvh* B, i32 qsrc, const LineList* ll, const LineGrid* 
     } else {
         const unsigned g = (unsigned)((S + kBlock - 1) / kBlock);
         LineSampler<T, K, PH_SCATTER, 0> smp{*ll, S, line_xcd(tl_launch, g)};
-        k_lines_adjoint<T, K, 0, SHEP, -1><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, tv.leaf_escape, flag, cols, qsrc,
-                                                                smp, tab, dq, cot, order, grad);
+        k_lines_adjoint<T, K, 0, SHEP, -1><<<g, kBlock, 0, s>>>(tv, ln, tv.leaves, flag, cols, qsrc, smp, tab, dq,
+                                                                cot, order, grad);
     }
     PH_HIP(hipGetLastError());
     if (ev) PH_HIP(hipEventRecord(ev[1], s));

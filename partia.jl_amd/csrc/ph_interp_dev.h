@@ -161,20 +161,24 @@ template <class T> __device__ inline T d2_line_aabb_lb(const Line<T>& l, const N
     return dl * dl;
 }
 
-// Node record of the per-wave line walks: the box, then the node-only terms of
-// d2_line_aabb_lb's bounding-sphere bound (centre c, squared half diagonal r2
-// and sr = sqrt(r2)) and the scatter radius rad2 = (K scale)^2, each computed
-// with the operations d2_line_aabb_lb and radius2 use, so the walk's pruning
-// decisions are the per-ray DFS's, bit for bit; and two thresholds on the
-// squared centre distance dc2 that decide most lanes without the square root
-// (line_node_prune).  64 bytes for float, 128 for double; built once per tree
-// (k_line_nodes).
-template <class T> struct alignas(16) LineNode {
+// Record of the per-wave line walks: the LBVH in DFS preorder, internal nodes
+// and leaves alike, so that a node's left child is the next record and a
+// pruned subtree is skipped with one link.  An internal node holds its box,
+// the node-only terms of d2_line_aabb_lb's bounding-sphere bound (centre c,
+// squared half diagonal r2 and sr = sqrt(r2)) and the scatter radius rad2 =
+// (K scale)^2, each computed with the operations d2_line_aabb_lb and radius2
+// use, so the walk's pruning decisions are the per-ray DFS's, bit for bit; and
+// two thresholds on the squared centre distance dc2 that decide most lanes
+// without the square root (line_node_prune).  A leaf holds its position in lo
+// and its h in hi[0].  link: the record after the subtree (a leaf: the next
+// one); leaf: the leaf's Morton index, or -1.  64 bytes for float, 128 for
+// double; built once per tree (k_line_recs).
+template <class T> struct alignas(16) LineRec {
     T lo[3], hi[3];
     T c[3];
     T r2, sr, rad2;
     T thr_lo, thr_hi;
-    i32 left, escape;
+    i32 link, leaf;
 };
 // The per-ray test prunes iff the line misses the box (slab test), dc2 > r2
 // and p = RN(RN(RN(sqrt(dc2)) - sr)^2) > rad2.  With rr = sqrt(rad2) and the
@@ -187,8 +191,8 @@ template <class T> struct alignas(16) LineNode {
 // Both are formed in float64 and rounded to T (a relative 2^-24 at most, inside
 // the margin).  They are NaN (no fast decision) unless r2, sr and both
 // thresholds are finite and rad2 is a positive normal number.
-template <class T> __device__ inline LineNode<T> line_node(const NodeRec<T>& N, T Kvalid) {
-    LineNode<T> R;
+template <class T> __device__ inline LineRec<T> line_rec_node(const NodeRec<T>& N, T Kvalid, i32 link) {
+    LineRec<T> R;
     const T half = T(1) / T(2);
     T r2 = T(0);
 #pragma unroll
@@ -203,8 +207,8 @@ template <class T> __device__ inline LineNode<T> line_node(const NodeRec<T>& N, 
     R.sr = sqrt(r2);
     const T r = Kvalid * N.scale;
     R.rad2 = r * r;
-    R.left = N.left;
-    R.escape = N.escape;
+    R.link = link;
+    R.leaf = -1;
     const double e = std::is_same<T, float>::value ? 0x1p-18 : 0x1p-40;
     const double rr = sqrt((double)R.rad2);
     const double a = (double)R.sr + rr * (1.0 - e), b = (double)R.sr + rr * (1.0 + e);
@@ -216,7 +220,17 @@ template <class T> __device__ inline LineNode<T> line_node(const NodeRec<T>& N, 
     R.thr_hi = ok ? hi : T(NAN);
     return R;
 }
-// d2_line_aabb_lb(l, N) > (K N.scale)^2 from the node's LineNode, for the lanes
+template <class T> __device__ inline LineRec<T> line_rec_leaf(const LeafRec<T>& L, i32 k, i32 link) {
+    LineRec<T> R{};
+    R.lo[0] = L.x;
+    R.lo[1] = L.y;
+    R.lo[2] = L.z;
+    R.hi[0] = L.h;
+    R.link = link;
+    R.leaf = k;
+    return R;
+}
+// d2_line_aabb_lb(l, N) > (K N.scale)^2 from the node's LineRec, for the lanes
 // with active set (the others get false).  Called by the whole wave.  A slab hit
 // or a line within the bounding sphere gives the bound 0, which no radius (a
 // square, or NaN) is below.  The centre distance comes first: lanes with dc2 <=
@@ -228,7 +242,7 @@ template <class T> __device__ inline LineNode<T> line_node(const NodeRec<T>& N, 
 // test then reduces to the two axes with m_d = 0 (the AX axis's interval
 // [a - o, b - o] is never empty for a finite box).
 template <int AX = -1, class T>
-__device__ inline bool line_node_prune(const Line<T>& l, const LineNode<T>& N, bool active) {
+__device__ inline bool line_node_prune(const Line<T>& l, const LineRec<T>& N, bool active) {
     const T dc2 = d2_line_point<AX>(l, N.c);
     const bool keep = dc2 <= N.r2 || dc2 <= N.thr_lo;
     if (__ballot(active && !keep) == 0) return false;  // wave-uniform
@@ -278,6 +292,46 @@ __device__ inline bool line_node_prune(const Line<T>& l, const LineNode<T>& N, b
         }
     }
     return active && !hit && !keep && far;
+}
+
+// L(u) by pointer jumping: after k_lc_init and r rounds of k_lc_jump, up[u]
+// is the ancestor 2^(r+1) edges above internal node u (0 past the root) and
+// lc[u] counts the left-child edges on the way there, u's own edge included.
+__device__ inline i32 lc_edge(i32 c, i32 root, const i32* __restrict__ parent, const i32* __restrict__ left,
+                              i32& up) {
+    up = c == root ? 0 : parent[c - 1];
+    return (up != 0 && left[up - 1] == c) ? 1 : 0;
+}
+__global__ void __launch_bounds__(kBlock) k_lc_init(i64 n, i32 root, const i32* __restrict__ parent,
+                                            const i32* __restrict__ left, i32* __restrict__ up,
+                                            i32* __restrict__ lc) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
+    if (i > n - 1) return;
+    i32 u1 = 0, u2 = 0;
+    i32 e = lc_edge((i32)i, root, parent, left, u1);
+    if (u1 != 0) e += lc_edge(u1, root, parent, left, u2);
+    up[i - 1] = u2;
+    lc[i - 1] = e;
+}
+__global__ void __launch_bounds__(kBlock) k_lc_jump(i64 n, const i32* __restrict__ up_in,
+                                            const i32* __restrict__ lc_in, i32* __restrict__ up_out,
+                                            i32* __restrict__ lc_out) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x + 1;
+    if (i > n - 1) return;
+    const i32 u = up_in[i - 1];
+    up_out[i - 1] = u == 0 ? 0 : up_in[u - 1];
+    lc_out[i - 1] = lc_in[i - 1] + (u == 0 ? 0 : lc_in[u - 1]);
+}
+constexpr int kLcJumps = 2;  // pointers of 8 edges
+// L(u): the left-child edges on the path from the root to internal node u
+// (1-based), from the pointers of the last k_lc_jump round.
+__device__ inline i64 lc_total(i64 u, i64 n, const i32* __restrict__ up, const i32* __restrict__ lc) {
+    i64 Lc = lc[u - 1];
+    for (i64 c = up[u - 1], steps = n; c != 0 && steps > 0; --steps) {
+        Lc += lc[c - 1];
+        c = up[c - 1];
+    }
+    return Lc;
 }
 
 // Radius rule of the strategy (A13): gather (K ha)^2, scatter (K h)^2, symmetric max.
