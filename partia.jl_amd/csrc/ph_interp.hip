@@ -271,6 +271,10 @@ __global__ void __launch_bounds__(kBlock) k_lines(TreeView<T> tv, const T* __res
 // the record after a leaf) it tests that one without a load of its own.  The
 // records come through the kernels' restrict-qualified parameters, so their
 // wave-uniform loads are scalar loads.
+#ifndef PH_LINE_AHEAD
+#define PH_LINE_AHEAD 2
+#endif
+constexpr int kLineAhead = PH_LINE_AHEAD;  // records loaded per step (the array carries kLineAhead - 1 of padding)
 template <int AX = -1, class T, class Leaf>
 __device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, const LineRec<T>* __restrict__ lrecs,
                                                          const LeafRec<T>* __restrict__ leaves, const Line<T>& l,
@@ -305,9 +309,13 @@ __device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, 
     for (i64 steps = 2 * n + 8; p < nrec && steps > 0; --steps) {
         p = __builtin_amdgcn_readfirstlane(p);
         const i32 p0 = p;
-        const LineRec<T> R = lrecs[p0], R1 = lrecs[p0 + 1];
-        step(R);
-        if (p == p0 + 1 && p < nrec && --steps > 0) step(R1);
+        LineRec<T> Q[kLineAhead];
+#pragma unroll
+        for (int j = 0; j < kLineAhead; ++j) Q[j] = lrecs[p0 + j];
+        step(Q[0]);
+#pragma unroll
+        for (int j = 1; j < kLineAhead; ++j)
+            if (p == p0 + j && p < nrec && --steps > 0) step(Q[j]);
     }
 }
 
@@ -653,9 +661,9 @@ template <class T> i64 line_rec_count(const ph_lbvh* B) { return B->n > 1 ? 2 * 
 template <class T, int K> const LineRec<T>* line_recs(ph_lbvh* B, hipStream_t s) {
     if (!B->lnodes_ready) {
         const i64 n = B->n, m = line_rec_count<T>(B);
-        B->lnodes.alloc(sizeof(LineRec<T>) * (size_t)(m + 2));
+        B->lnodes.alloc(sizeof(LineRec<T>) * (size_t)(m + kLineAhead));
         LineRec<T>* lr = B->lnodes.as<LineRec<T>>();
-        u32* flag = reinterpret_cast<u32*>(lr + m + 1);
+        u32* flag = reinterpret_cast<u32*>(lr + m + kLineAhead - 1);
         k_set_u32<<<1, 1, 0, s>>>(flag, 1u);
         PH_HIP(hipGetLastError());
         k_line_flag<T><<<grid_for(n), kBlock, 0, s>>>(B->leaves.as<LeafRec<T>>(), n, flag);
@@ -684,7 +692,7 @@ template <class T, int K> const LineRec<T>* line_recs(ph_lbvh* B, hipStream_t s)
     return B->lnodes.as<LineRec<T>>();
 }
 template <class T> const u32* line_flag(const ph_lbvh* B) {
-    return reinterpret_cast<const u32*>(B->lnodes.as<LineRec<T>>() + line_rec_count<T>(B) + 1);
+    return reinterpret_cast<const u32*>(B->lnodes.as<LineRec<T>>() + line_rec_count<T>(B) + kLineAhead - 1);
 }
 
 // The axis of image-plane rays with direction exactly +-e_axis, else -1.
