@@ -657,10 +657,10 @@ __global__ void k_set_u32(u32* p, u32 v) {
 // the walk's paired loads) and, after them, the axis flag; the first call
 // builds them (the tree is immutable).  A one-leaf tree has none (its walk is
 // the brute-force branch).
-template <class T> i64 line_rec_count(const ph_lbvh* B) { return B->n > 1 ? 2 * B->n - 1 : 0; }
+inline i64 line_rec_count(const ph_lbvh* B) { return B->n > 1 ? 2 * B->n - 1 : 0; }
 template <class T, int K> const LineRec<T>* line_recs(ph_lbvh* B, hipStream_t s) {
     if (!B->lnodes_ready) {
-        const i64 n = B->n, m = line_rec_count<T>(B);
+        const i64 n = B->n, m = line_rec_count(B);
         B->lnodes.alloc(sizeof(LineRec<T>) * (size_t)(m + kLineAhead));
         LineRec<T>* lr = B->lnodes.as<LineRec<T>>();
         u32* flag = reinterpret_cast<u32*>(lr + m + kLineAhead - 1);
@@ -692,7 +692,7 @@ template <class T, int K> const LineRec<T>* line_recs(ph_lbvh* B, hipStream_t s)
     return B->lnodes.as<LineRec<T>>();
 }
 template <class T> const u32* line_flag(const ph_lbvh* B) {
-    return reinterpret_cast<const u32*>(B->lnodes.as<LineRec<T>>() + line_rec_count<T>(B) + kLineAhead - 1);
+    return reinterpret_cast<const u32*>(B->lnodes.as<LineRec<T>>() + line_rec_count(B) + kLineAhead - 1);
 }
 
 // The axis of image-plane rays with direction exactly +-e_axis, else -1.
