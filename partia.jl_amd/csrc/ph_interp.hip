@@ -280,30 +280,28 @@ __device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, 
                                                          const LeafRec<T>* __restrict__ leaves, const Line<T>& l,
                                                          bool valid, T Kvalid, Leaf&& leaf) {
     const i64 n = tv.n;
-    auto visit = [&](i64 k, const LeafRec<T>& L, bool active) {
+    auto visit = [&](i64 k, const LeafRec<T>& L, u64 active) {
         const T d2 = d2_line_leaf_ax<AX>(l, L);
-        const bool close = d2 <= radius2<T, PH_SCATTER>(Kvalid, T(0), L.h);
-        leaf(k, active & close, d2, L);
+        leaf(k, active & lanes_cmp<kCmpOLE>(d2, radius2<T, PH_SCATTER>(Kvalid, T(0), L.h)), d2, L);
     };
     if (tv.root == 0) {  // n == 1: the reference's brute-force branch
-        for (i64 k = 0; k < n; ++k) visit(k, leaves[k], valid);
+        for (i64 k = 0; k < n; ++k) visit(k, leaves[k], wave_ballot(valid));
         return;
     }
     const i32 nrec = (i32)(2 * n - 1);
-    bool active = valid;
-    i32 resume = -1;  // record at which an idle lane rejoins
-    i32 p = 0;        // the root
+    u64 active = wave_ballot(valid);  // lanes walking (the others wait to resume)
+    i32 resume = -1;                  // record at which an idle lane rejoins
+    i32 p = 0;                        // the root
     auto step = [&](const LineRec<T>& R) {
-        active = active || p == resume;
+        active |= wave_ballot(resume == p);
         if (R.leaf >= 0) {
             visit(R.leaf, LeafRec<T>{R.lo[0], R.lo[1], R.lo[2], R.hi[0]}, active);
             p = p + 1;
         } else {
-            const bool prune = line_node_prune<AX>(l, R, active);
-            resume = (active && prune) ? R.link : resume;
-            const bool descend = active && !prune;
-            active = descend;
-            p = __ballot(descend) ? p + 1 : R.link;
+            const u64 prune = line_node_prune<AX>(l, R, active);
+            resume = lane_in(prune) ? R.link : resume;
+            active &= ~prune;
+            p = active != 0 ? p + 1 : R.link;
         }
     };
     for (i64 steps = 2 * n + 8; p < nrec && steps > 0; --steps) {
@@ -377,12 +375,12 @@ __device__ __attribute__((always_inline)) void lines_wave_body(
         nst = 0;
     };
     wave_walk<AX>(tv, lrecs, leaves, l, valid, T(support_of<K>()),
-                  [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
+                  [&](i64 k, u64 take, T d2, const LeafRec<T>& L) {
                       (void)d2;
                       (void)L;
-                      if (__ballot(take) == 0) return;  // wave-uniform
+                      if (take == 0) return;  // wave-uniform
                       if (lane == 0) lidx[nst] = (u32)k;
-                      mask |= take ? (u64(1) << nst) : u64(0);
+                      mask |= lane_in(take) ? (u64(1) << nst) : u64(0);
                       if (++nst == kWave) flush();
                   });
     if (nst > 0) flush();
@@ -477,7 +475,8 @@ __global__ void __launch_bounds__(kBlock) k_lines_adjoint(TreeView<T> tv, const 
         if constexpr (SHEP) {  // the forward's sums, same arithmetic
             T N = T(0);
             wave_walk<AXV>(tv, lrecs, leaves, l, valid, Kvalid,
-                           [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
+                           [&](i64 k, u64 take_m, T d2, const LeafRec<T>& L) {
+                const bool take = lane_in(take_m);
                 const T mb = mcol[k], rb = rcol[k], Ab = acol[k];
                 const T dr = sqrt(d2 < T(0) ? T(0) : d2);
                 const T t = mb * line_W_nb<T, K>(dr, L.h, tab, dq) / rb;
@@ -488,8 +487,9 @@ __global__ void __launch_bounds__(kBlock) k_lines_adjoint(TreeView<T> tv, const 
             out = S1 == T(0) ? T(0) : N * (T(1) / S1);
         }
         const T c2 = SHEP ? (S1 == T(0) ? T(0) : T(1) / S1) : T(1);
-        wave_walk<AXV>(tv, lrecs, leaves, l, valid, Kvalid, [&](i64 k, bool take, T d2, const LeafRec<T>& L) {
-            if (__ballot(take && g != T(0)) == 0) return;  // wave-uniform
+        wave_walk<AXV>(tv, lrecs, leaves, l, valid, Kvalid, [&](i64 k, u64 take_m, T d2, const LeafRec<T>& L) {
+            if ((take_m & lanes_cmp<kCmpUNE>(g, T(0))) == 0) return;  // wave-uniform
+            const bool take = lane_in(take_m);
             const T mb = mcol[k], rb = rcol[k], Ab = acol[k];
             const T dr = sqrt(d2 < T(0) ? T(0) : d2);
             const T invh = T(1) / L.h, q = dr * invh;

@@ -12,6 +12,25 @@
 namespace ph {
 namespace {
 
+// The lanes of the wave where b holds.  HIP's __ballot takes an int, and the
+// bool -> int -> bool round trip costs two vector instructions per vote.
+__device__ __attribute__((always_inline)) inline u64 wave_ballot(bool b) {
+    return __builtin_amdgcn_ballot_w64(b);
+}
+// Comparisons straight into lane masks (one v_cmp each, no per-lane bool in
+// between; LLVM's predicate codes) and a lane's bit of a mask as a bool.
+// Lanes outside exec compare false.
+enum : int { kCmpOGT = 2, kCmpOGE = 3, kCmpOLT = 4, kCmpOLE = 5, kCmpUGT = 10, kCmpUNE = 14 };
+template <int P> __device__ __attribute__((always_inline)) inline u64 lanes_cmp(float a, float b) {
+    return __builtin_amdgcn_fcmpf(a, b, P);
+}
+template <int P> __device__ __attribute__((always_inline)) inline u64 lanes_cmp(double a, double b) {
+    return __builtin_amdgcn_fcmp(a, b, P);
+}
+__device__ __attribute__((always_inline)) inline bool lane_in(u64 m) {
+    return __builtin_amdgcn_inverse_ballot_w64(m);
+}
+
 template <class T> struct TreeView {
     i64 n;
     i32 root;
@@ -164,19 +183,20 @@ template <class T> __device__ inline T d2_line_aabb_lb(const Line<T>& l, const N
 // Record of the per-wave line walks: the LBVH in DFS preorder, internal nodes
 // and leaves alike, so that a node's left child is the next record and a
 // pruned subtree is skipped with one link.  An internal node holds its box,
-// the node-only terms of d2_line_aabb_lb's bounding-sphere bound (centre c,
-// squared half diagonal r2 and sr = sqrt(r2)) and the scatter radius rad2 =
+// the node-only terms of d2_line_aabb_lb's bounding-sphere bound (centre c and
+// sr = sqrt(r2), r2 the squared half diagonal) and the scatter radius rad2 =
 // (K scale)^2, each computed with the operations d2_line_aabb_lb and radius2
 // use, so the walk's pruning decisions are the per-ray DFS's, bit for bit; and
-// two thresholds on the squared centre distance dc2 that decide most lanes
-// without the square root (line_node_prune).  A leaf holds its position in lo
+// thresholds on the squared centre distance dc2 that decide most lanes without
+// the square root (line_node_prune): keep2 = max(r2, thr_lo) (NaN operands
+// ignored) and thr_hi.  A leaf holds its position in lo
 // and its h in hi[0].  link: the record after the subtree (a leaf: the next
 // one); leaf: the leaf's Morton index, or -1.  64 bytes for float, 128 for
 // double; built once per tree (k_line_recs).
 template <class T> struct alignas(16) LineRec {
     T lo[3], hi[3];
     T c[3];
-    T r2, sr, rad2;
+    T keep2, sr, rad2;
     T thr_lo, thr_hi;
     i32 link, leaf;
 };
@@ -203,7 +223,6 @@ template <class T> __device__ inline LineRec<T> line_rec_node(const NodeRec<T>& 
         const T hh = (N.hi[d] - N.lo[d]) * half;
         r2 += hh * hh;
     }
-    R.r2 = r2;
     R.sr = sqrt(r2);
     const T r = Kvalid * N.scale;
     R.rad2 = r * r;
@@ -214,10 +233,13 @@ template <class T> __device__ inline LineRec<T> line_rec_node(const NodeRec<T>& 
     const double a = (double)R.sr + rr * (1.0 - e), b = (double)R.sr + rr * (1.0 + e);
     const T lo = T(a * a * (1.0 - e)), hi = T(b * b * (1.0 + e));
     const T tmin = std::is_same<T, float>::value ? T(__FLT_MIN__) : T(__DBL_MIN__);
-    const bool ok = isfinite(R.r2) && isfinite(R.sr) && R.rad2 >= tmin && isfinite(R.rad2) && isfinite(lo) &&
+    const bool ok = isfinite(r2) && isfinite(R.sr) && R.rad2 >= tmin && isfinite(R.rad2) && isfinite(lo) &&
                     isfinite(hi);
     R.thr_lo = ok ? lo : T(NAN);
     R.thr_hi = ok ? hi : T(NAN);
+    // dc2 <= r2 || dc2 <= thr_lo  <=>  dc2 <= fmax(r2, thr_lo), NaNs included
+    // (fmax returns the other operand of a NaN; two NaNs give NaN: never kept)
+    R.keep2 = fmax(r2, R.thr_lo);
     return R;
 }
 template <class T> __device__ inline LineRec<T> line_rec_leaf(const LeafRec<T>& L, i32 k, i32 link) {
@@ -230,68 +252,75 @@ template <class T> __device__ inline LineRec<T> line_rec_leaf(const LeafRec<T>& 
     R.leaf = k;
     return R;
 }
-// d2_line_aabb_lb(l, N) > (K N.scale)^2 from the node's LineRec, for the lanes
-// with active set (the others get false).  Called by the whole wave.  A slab hit
-// or a line within the bounding sphere gives the bound 0, which no radius (a
-// square, or NaN) is below.  The centre distance comes first: lanes with dc2 <=
-// r2 or dc2 <= thr_lo keep the node whatever the slab test says, and when every
-// active lane does, the slab test is skipped; lanes with dc2 >= thr_hi prune
-// unless the slab test hits; the rest (NaN, or within the margins) evaluate the
-// reference's square roots.
+// The lanes of active where d2_line_aabb_lb(l, N) > (K N.scale)^2, from the
+// node's LineRec.  Called by the whole wave.  A slab hit or a line within the
+// bounding sphere gives the bound 0, which no radius (a square, or NaN) is
+// below.  The centre distance comes first: lanes with dc2 <= keep2 keep the
+// node whatever the slab test says, and when every active lane does, the slab
+// test is skipped; lanes with dc2 >= thr_hi prune unless the slab test hits;
+// the rest (NaN, or within the margins) evaluate the reference's square roots.
+// The decisions are lane masks: comparisons write them directly and the
+// logic runs on the scalar unit.
 // AX >= 0: m = +-e_AX and every coordinate finite (d2_line_point); the slab
 // test then reduces to the two axes with m_d = 0 (the AX axis's interval
 // [a - o, b - o] is never empty for a finite box).
 template <int AX = -1, class T>
-__device__ inline bool line_node_prune(const Line<T>& l, const LineRec<T>& N, bool active) {
+__device__ inline u64 line_node_prune(const Line<T>& l, const LineRec<T>& N, u64 active) {
     const T dc2 = d2_line_point<AX>(l, N.c);
-    const bool keep = dc2 <= N.r2 || dc2 <= N.thr_lo;
-    if (__ballot(active && !keep) == 0) return false;  // wave-uniform
-    bool hit = true;
-    T tmin = -T(INFINITY), tmax = T(INFINITY);
+    const u64 open = active & lanes_cmp<kCmpUGT>(dc2, N.keep2);  // !(dc2 <= keep2)
+    if (open == 0) return 0;  // wave-uniform
+    u64 miss = 0;
+    if constexpr (AX >= 0) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        if constexpr (AX >= 0) {
-            if (d != AX && (l.o[d] < N.lo[d] || l.o[d] > N.hi[d])) hit = false;
-            continue;
-        }
-        const T a = N.lo[d], b = N.hi[d], o = l.o[d], m = l.m[d];
-        if (m == T(0)) {
-            if (o < a || o > b) hit = false;
-        } else {
-            T t0, t1;
-            if (m == T(1)) {
-                t0 = a - o;
-                t1 = b - o;
-            } else if (m == T(-1)) {
-                t0 = -(a - o);
-                t1 = -(b - o);
+        for (int d = 0; d < 3; ++d)
+            if (d != AX) miss |= lanes_cmp<kCmpOLT>(l.o[d], N.lo[d]) | lanes_cmp<kCmpOGT>(l.o[d], N.hi[d]);
+    } else {
+        bool hit = true;
+        T tmin = -T(INFINITY), tmax = T(INFINITY);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const T a = N.lo[d], b = N.hi[d], o = l.o[d], m = l.m[d];
+            if (m == T(0)) {
+                if (o < a || o > b) hit = false;
             } else {
-                t0 = (a - o) / m;
-                t1 = (b - o) / m;
+                T t0, t1;
+                if (m == T(1)) {
+                    t0 = a - o;
+                    t1 = b - o;
+                } else if (m == T(-1)) {
+                    t0 = -(a - o);
+                    t1 = -(b - o);
+                } else {
+                    t0 = (a - o) / m;
+                    t1 = (b - o) / m;
+                }
+                if (t0 > t1) {
+                    const T tt = t0;
+                    t0 = t1;
+                    t1 = tt;
+                }
+                tmin = fmax(tmin, t0);
+                tmax = fmin(tmax, t1);
+                if (tmin > tmax) hit = false;
             }
-            if (t0 > t1) {
-                const T tt = t0;
-                t0 = t1;
-                t1 = tt;
-            }
-            tmin = fmax(tmin, t0);
-            tmax = fmin(tmax, t1);
-            if (tmin > tmax) hit = false;
         }
+        miss = wave_ballot(!hit);
     }
-    bool far = dc2 >= N.thr_hi;
-    const bool band = active && !keep && !far;
-    if (__ballot(band) != 0) {  // wave-uniform: some lane within the margins
+    u64 far = lanes_cmp<kCmpOGE>(dc2, N.thr_hi);
+    const u64 band = open & ~far;
+    if (band != 0) {  // wave-uniform: some lane within the margins
         // an empty asm statement (no instruction) keeps the square root inside
         // the rarely taken branch instead of being speculated above it
         T x = dc2;
         __asm__ volatile("" : "+v"(x));
-        if (band) {  // the reference's arithmetic
+        bool exact_far = false;
+        if (lane_in(band)) {  // the reference's arithmetic
             const T dl = sqrt(x) - N.sr;
-            far = dl * dl > N.rad2;
+            exact_far = dl * dl > N.rad2;
         }
+        far |= wave_ballot(exact_far);
     }
-    return active && !hit && !keep && far;
+    return open & miss & far;
 }
 
 // L(u) by pointer jumping: after k_lc_init and r rounds of k_lc_jump, up[u]
