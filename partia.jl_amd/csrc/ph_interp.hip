@@ -280,12 +280,14 @@ __device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, 
                                                          const LeafRec<T>* __restrict__ leaves, const Line<T>& l,
                                                          bool valid, T Kvalid, Leaf&& leaf) {
     const i64 n = tv.n;
-    auto visit = [&](i64 k, const LeafRec<T>& L, u64 active) {
+    // rad2: radius2<T, PH_SCATTER>(Kvalid, 0, L.h)
+    auto visit = [&](i64 k, const LeafRec<T>& L, T rad2, u64 active) {
         const T d2 = d2_line_leaf_ax<AX>(l, L);
-        leaf(k, active & lanes_cmp<kCmpOLE>(d2, radius2<T, PH_SCATTER>(Kvalid, T(0), L.h)), d2, L);
+        leaf(k, active & lanes_cmp<kCmpOLE>(d2, rad2), d2, L);
     };
     if (tv.root == 0) {  // n == 1: the reference's brute-force branch
-        for (i64 k = 0; k < n; ++k) visit(k, leaves[k], wave_ballot(valid));
+        for (i64 k = 0; k < n; ++k)
+            visit(k, leaves[k], radius2<T, PH_SCATTER>(Kvalid, T(0), leaves[k].h), wave_ballot(valid));
         return;
     }
     const i32 nrec = (i32)(2 * n - 1);
@@ -295,7 +297,7 @@ __device__ __attribute__((always_inline)) void wave_walk(const TreeView<T>& tv, 
     auto step = [&](const LineRec<T>& R) {
         active |= wave_ballot(resume == p);
         if (R.leaf >= 0) {
-            visit(R.leaf, LeafRec<T>{R.lo[0], R.lo[1], R.lo[2], R.hi[0]}, active);
+            visit(R.leaf, LeafRec<T>{R.lo[0], R.lo[1], R.lo[2], R.hi[0]}, R.hi[1], active);
             p = p + 1;
         } else {
             const u64 prune = line_node_prune<AX>(l, R, active);
@@ -633,10 +635,14 @@ __global__ void __launch_bounds__(kBlock) k_line_recs(i64 n, const i32* __restri
     const i64 Lc = lc_total(i, n, up, lc);
     const i64 pre = Lc + 2 * (lo - 1);
     out[pre] = line_rec_node<T>(nodes[i - 1], T(support_of<K>()), (i32)(pre + 2 * (hi - lo) + 1));
-    if (sp == lo) out[pre + 1] = line_rec_leaf<T>(leaves[lo - 1], (i32)(lo - 1), (i32)(pre + 2));
+    auto leaf_rec = [&](i64 k, i32 link) {
+        const LeafRec<T> L = leaves[k];
+        return line_rec_leaf<T>(L, radius2<T, PH_SCATTER>(T(support_of<K>()), T(0), L.h), (i32)k, link);
+    };
+    if (sp == lo) out[pre + 1] = leaf_rec(lo - 1, (i32)(pre + 2));
     if (sp + 1 == hi) {
         const i64 rp = Lc + 2 * sp;
-        out[rp] = line_rec_leaf<T>(leaves[hi - 1], (i32)(hi - 1), (i32)(rp + 1));
+        out[rp] = leaf_rec(hi - 1, (i32)(rp + 1));
     }
 }
 // The flag of the axis-aligned walk: *flag (1 beforehand) is cleared when a

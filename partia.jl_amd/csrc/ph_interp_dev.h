@@ -189,8 +189,9 @@ template <class T> __device__ inline T d2_line_aabb_lb(const Line<T>& l, const N
 // use, so the walk's pruning decisions are the per-ray DFS's, bit for bit; and
 // thresholds on the squared centre distance dc2 that decide most lanes without
 // the square root (line_node_prune): keep2 = max(r2, thr_lo) (NaN operands
-// ignored) and thr_hi.  A leaf holds its position in lo
-// and its h in hi[0].  link: the record after the subtree (a leaf: the next
+// ignored) and thr_hi.  A leaf holds its position in lo, its h in hi[0] and
+// its scatter radius (K h)^2 (radius2) in hi[1].  link: the record after the
+// subtree (a leaf: the next
 // one); leaf: the leaf's Morton index, or -1.  64 bytes for float, 128 for
 // double; built once per tree (k_line_recs).
 template <class T> struct alignas(16) LineRec {
@@ -242,12 +243,14 @@ template <class T> __device__ inline LineRec<T> line_rec_node(const NodeRec<T>& 
     R.keep2 = fmax(r2, R.thr_lo);
     return R;
 }
-template <class T> __device__ inline LineRec<T> line_rec_leaf(const LeafRec<T>& L, i32 k, i32 link) {
+template <class T>
+__device__ inline LineRec<T> line_rec_leaf(const LeafRec<T>& L, T rad2, i32 k, i32 link) {
     LineRec<T> R{};
     R.lo[0] = L.x;
     R.lo[1] = L.y;
     R.lo[2] = L.z;
     R.hi[0] = L.h;
+    R.hi[1] = rad2;
     R.link = link;
     R.leaf = k;
     return R;

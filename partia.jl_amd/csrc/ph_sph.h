@@ -220,10 +220,11 @@ __device__ inline T line_W_rdq(T r, T invh, const T* __restrict__ inorm, T dq, T
         const float q0 = q * rdq;
         const float e = __builtin_fmaf(-q0, dq, q);
         const float idxf = __builtin_fmaf(e, rdq, q0) + 1.f;
-        const float lo = 1.f, hi = float(PH_LUT_N_ENTRIES - 1);
-        const float cf = idxf < lo ? lo : (idxf > hi ? hi : idxf);
-        i32 i = (i32)cf;
-        i = i < 1 ? 1 : (i > PH_LUT_N_ENTRIES - 1 ? PH_LUT_N_ENTRIES - 1 : i);
+        // line_W_inv's two clamps give the same i in one each: truncation is
+        // monotonic and both bounds are integers (NaN: i = 1 there, here the
+        // conversion's 0 raised to 1 or a bound; t and I are NaN either way)
+        const float cf = __builtin_amdgcn_fmed3f(idxf, 1.f, float(PH_LUT_N_ENTRIES - 1));
+        const i32 i = max((i32)cf, 1);
         const float t = idxf - float(i);
         float I = inorm[i - 1] * (1.f - t) + inorm[i] * t;
         I = (q >= float(support_of<K>())) ? 0.f : I;
