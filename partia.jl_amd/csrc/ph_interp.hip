@@ -332,7 +332,7 @@ template <int AX, int MS, class T, int K>
 __device__ __attribute__((always_inline)) void lines_wave_body(
     const TreeView<T>& tv, const LineRec<T>* __restrict__ lrecs, const LeafRec<T>* __restrict__ leaves,
     const T* __restrict__ cols, const CatChunk& c, const Line<T>& l, bool valid,
-    const T* tab, T dq, T rdq, T* st, u32* lidx, LineAcc<T, K>& a, u64& accepted) {
+    const T* tab, T dq, T rdq, T* st, LineAcc<T, K>& a, u64& accepted) {
     constexpr int kSlot = 6 + MS;  // x, y, z, 1 / h, m, rho, the scalars' columns
     const int lane = threadIdx.x & (kWave - 1);
     const i64 n = tv.n;
@@ -342,11 +342,12 @@ __device__ __attribute__((always_inline)) void lines_wave_body(
 #pragma unroll
     for (int j = 0; j < MS; ++j) qcol[j] = cols + (i64)c.rsrc[c.spos[j < c.ns ? j : 0]] * n;
     u64 mask = 0;
-    int nst = 0;  // listed leaves (wave-uniform)
+    int nst = 0;    // listed leaves (wave-uniform)
+    u32 kleaf = 0;  // lane j < nst: the Morton index of listed leaf j
     auto flush = [&]() {
         wave_sync();
         if (lane < nst) {
-            const u32 k = lidx[lane];
+            const u32 k = kleaf;
             const LeafRec<T> L = leaves[k];
             T* d = st + lane * kSlot;
             d[0] = L.x;
@@ -381,8 +382,9 @@ __device__ __attribute__((always_inline)) void lines_wave_body(
                       (void)d2;
                       (void)L;
                       if (take == 0) return;  // wave-uniform
-                      if (lane == 0) lidx[nst] = (u32)k;
-                      mask |= lane_in(take) ? (u64(1) << nst) : u64(0);
+                      kleaf = lane == nst ? (u32)k : kleaf;
+                      const u64 bit = u64(1) << nst;
+                      mask = lane_in(take) ? (mask | bit) : mask;
                       if (++nst == kWave) flush();
                   });
     if (nst > 0) flush();
@@ -407,7 +409,6 @@ __global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const Lin
                                                        T* __restrict__ out, u64* counter) {
     __shared__ T tab[PH_LUT_N_ENTRIES];
     __shared__ T stage[kBlock / kWave][kWave * (6 + MS)];
-    __shared__ u32 sidx[kBlock / kWave][kWave];
     for (int i = threadIdx.x; i < PH_LUT_N_ENTRIES; i += kBlock) tab[i] = inorm[i];
     __syncthreads();
     i64 s = 0;
@@ -415,7 +416,6 @@ __global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const Lin
     const bool valid = smp.get(s, l);
     if (__ballot(valid) == 0) return;  // wave-uniform; no barrier follows
     T* st = stage[threadIdx.x / kWave];
-    u32* lidx = sidx[threadIdx.x / kWave];
     LineAcc<T, K> a;
     u64 accepted = 0;
     bool axis = false;
@@ -424,10 +424,10 @@ __global__ void __launch_bounds__(kBlock) k_lines_wave(TreeView<T> tv, const Lin
         axis = *lflag != 0 && __ballot(!ok) == 0;  // wave-uniform
     }
     if (axis)
-        lines_wave_body<(AX < 0 ? 0 : AX), MS>(tv, lrecs, leaves, cols, c, l, valid, tab, dq, rdq, st, lidx, a,
+        lines_wave_body<(AX < 0 ? 0 : AX), MS>(tv, lrecs, leaves, cols, c, l, valid, tab, dq, rdq, st, a,
                                                 accepted);
     else
-        lines_wave_body<-1, MS>(tv, lrecs, leaves, cols, c, l, valid, tab, dq, rdq, st, lidx, a, accepted);
+        lines_wave_body<-1, MS>(tv, lrecs, leaves, cols, c, l, valid, tab, dq, rdq, st, a, accepted);
     if (valid) a.store(c, out, smp.S, s);
     count_add(counter, valid ? accepted : 0);
 }
